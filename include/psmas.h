@@ -1,0 +1,133 @@
+/*
+ * psmas — C ABI of the MI355X-native MAS-preconditioned PCG linear solver.
+ *
+ * Drop-in boundary for polysolve's "MAS" linear solver.  The reference exposes the solver as
+ * polysolve::linear::MASSolver (src/polysolve/linear/MASSolver.hpp:35-72), created by name through
+ * polysolve::linear::Solver::create("MAS", ...) (src/polysolve/linear/Solver.cpp:403-408) and used
+ * through the Solver virtuals (src/polysolve/linear/Solver.hpp:90-131).  Each entry point below
+ * names the reference interface it replaces.  Signatures use plain pointers and sizes only;
+ * matrices are compressed column-major with int32 indices, the layout of polysolve's
+ * StiffnessMatrix (src/polysolve/Types.hpp:11-15).
+ *
+ * Every call returns a status code.  PSMAS_OK is success; every other code corresponds to the
+ * exception the reference throws: PSMAS_ERROR_OUT_OF_MEMORY <-> CudaOutOfMemoryError
+ * (src/polysolve/linear/mas_utils/CudaUtils.cuh:33-80), all others <-> std::runtime_error.  The
+ * message is available from psmas_last_error.  A handle is not thread-safe; all calls are
+ * synchronous on return (MASSolver.cu:408).
+ */
+#ifndef PSMAS_H
+#define PSMAS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psmas_solver *psmas_handle;
+
+enum psmas_status
+{
+    PSMAS_OK = 0,
+    PSMAS_ERROR_RUNTIME = 1,          /* std::runtime_error */
+    PSMAS_ERROR_OUT_OF_MEMORY = 2,    /* CudaOutOfMemoryError */
+    PSMAS_ERROR_INVALID_ARGUMENT = 3, /* std::runtime_error ("[MAS] invalid ...") */
+    PSMAS_ERROR_NO_DEVICE = 4,        /* std::runtime_error from the constructor (MASSolver.cu:188-191) */
+    PSMAS_ERROR_BUFFER_TOO_SMALL = 5  /* output string truncated; *needed holds the size */
+};
+
+/* ---- Solver lifecycle ------------------------------------------------------------------- */
+
+/* Replaces Solver::create("MAS", precond) -> std::make_unique<MASSolver>()
+ * (Solver.cpp:403-408; constructor MASSolver.cu:186-196,598-601).  Fails with
+ * PSMAS_ERROR_NO_DEVICE when no GPU is visible. */
+int psmas_create(psmas_handle *out);
+
+/* Replaces ~MASSolver (MASSolver.cu:603). */
+int psmas_destroy(psmas_handle h);
+
+/* Replaces MASSolver::name() == "MAS" (MASSolver.cu:636-639).  Static string. */
+const char *psmas_name(void);
+
+/* Replaces MASSolver::set_parameters(const json&) (MASSolver.cu:198-212,605-614).  json is a
+ * JSON object text; its "MAS" member is read with the keys of linear-solver-spec.json:456-509
+ * (block_dim, max_iter, relative_tolerance, absolute_tolerance, lazy_partitioning,
+ * use_preconditioned_residual_norm) plus this implementation's extension keys (precision,
+ * aggregation, coarse_solver, reliable_update_delta, check_interval, true_residual_period,
+ * use_graphs, learned_correction).  Without a "MAS" member the text is read as the inner
+ * object when it holds any of those keys, and ignored otherwise. */
+int psmas_set_parameters(psmas_handle h, const char *json);
+
+/* Current parameters as JSON (no reference counterpart; for inspection). */
+int psmas_get_parameters(psmas_handle h, char *buf, size_t cap, size_t *needed);
+
+/* Replaces MASSolver::get_info(json&) (MASSolver.cu:214-219,616-619): writes a JSON object with
+ * solver_iter, solver_error, solver_status (strings of MASSolver.hpp:18-33) and extra keys. */
+int psmas_get_info(psmas_handle h, char *buf, size_t cap, size_t *needed);
+
+/* Replaces MASSolver::analyze_pattern(A, precond_num) — a no-op (MASSolver.cu:221,621-624). */
+int psmas_analyze_pattern(psmas_handle h, int n, const int *colptr, const int *rowidx, int precond_num);
+
+/* Replaces MASSolver::factorize(const StiffnessMatrix&) (MASSolver.cu:292-374,626-629).
+ * A is n x n, symmetric, compressed column-major: colptr[n+1], rowidx[nnz], vals[nnz].  The
+ * matrix is copied; the caller may free it on return. */
+int psmas_factorize_csc(psmas_handle h, int n, long long nnz, const int *colptr, const int *rowidx,
+                        const double *vals);
+
+/* Replaces MASSolver::solve(b, x) (MASSolver.cu:376-409,631-634).  Host arrays of length n.
+ * The initial x is ignored (the reference starts from zero, MASSolver.cu:395-398). */
+int psmas_solve(psmas_handle h, int n, const double *b, double *x);
+
+/* Error message of the last failing call on h (or of the last failing psmas_create when h is
+ * NULL).  Valid until the next call on h. */
+const char *psmas_last_error(psmas_handle h);
+
+/* ---- Extensions (no counterpart in the reference) --------------------------------------- */
+
+/* Same as psmas_solve with b and x in device memory (HIP pointers on the solver's device). */
+int psmas_solve_device(psmas_handle h, int n, const double *d_b, double *d_x);
+
+/* Node coordinates for Morton-ordered aggregation into 32-node domains: nb tuples of dim
+ * (1..3) doubles, one per block node (block_dim scalars).  Takes effect at the next factorize. */
+int psmas_set_coordinates(psmas_handle h, int nb, int dim, const double *xyz);
+
+/* Number of visible GPUs (0 on a machine without one). */
+int psmas_device_count(void);
+
+/* Parse parameters without a device (checks the JSON contract): writes the resulting parameter
+ * set as JSON. */
+int psmas_parse_parameters(const char *json, char *buf, size_t cap, size_t *needed);
+
+/* ---- Introspection used by the parity tests ---------------------------------------------- */
+
+/* info[0..11] = level_num, cco_nums[4], parts (even), padded nodes, bank matrices, nnz blocks,
+ * SELL slots, block nodes, n.  cap >= 12. */
+int psmas_get_level_info(psmas_handle h, int *info, int cap);
+
+/* Domain partition in the reference's part_offsets / permutation form (MASSolver.cu:86-114):
+ * offsets[parts+1] (parts even; a trailing empty part pads to an even count), perm[nb] maps an
+ * input block node to its position. */
+int psmas_get_partition(psmas_handle h, int *nparts, int *offsets, int offsets_cap, int *perm, int perm_cap);
+
+/* Coarse-space ids of each input block node for levels 0..3 (-1 past the last level): map[4*nb]. */
+int psmas_get_coarse_map(psmas_handle h, int *map, long long cap);
+
+/* Packed upper FP64 bank inverses of all levels, (bank matrices) x (32*bd)(32*bd+1)/2. */
+int psmas_get_bank_inverses(psmas_handle h, double *out, long long cap);
+
+/* z = M^-1 r in the input ordering (mixed != 0: the FP16/FP32 preconditioner of the mixed path). */
+int psmas_apply_preconditioner(psmas_handle h, int n, const double *r, double *z, int mixed);
+
+/* y = A x in the input ordering with the solver's stored matrix (fp32 != 0: FP32 copy). */
+int psmas_spmv(psmas_handle h, int n, const double *x, double *y, int fp32);
+
+/* Per-kernel HIP-event timing of the solve's dominant kernels (disables graph replay while on).
+ * psmas_get_kernel_timing writes {"name": {"launches": k, "total_ms": t, "bytes": b}, ...}. */
+int psmas_set_kernel_timing(psmas_handle h, int on);
+int psmas_get_kernel_timing(psmas_handle h, char *buf, size_t cap, size_t *needed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSMAS_H */

@@ -1,0 +1,313 @@
+// extern "C" layer: maps the C ABI of include/psmas.h onto psmas::Solver and turns exceptions
+// into status codes (reference exception classes: CudaUtils.cuh:33-80, MASSolver.cu).
+#include "../../include/psmas.h"
+
+#include "solver.hpp"
+
+#include <cstring>
+#include <new>
+#include <sstream>
+#include <string>
+
+struct psmas_solver
+{
+    psmas::Solver solver;
+    std::string last_error;
+};
+
+namespace
+{
+    thread_local std::string g_create_error;
+
+    template <typename F>
+    int guarded(std::string &err, F &&f)
+    {
+        try
+        {
+            f();
+            err.clear();
+            return PSMAS_OK;
+        }
+        catch (const psmas::OutOfMemoryError &e)
+        {
+            err = e.what();
+            return PSMAS_ERROR_OUT_OF_MEMORY;
+        }
+        catch (const psmas::NoDeviceError &e)
+        {
+            err = e.what();
+            return PSMAS_ERROR_NO_DEVICE;
+        }
+        catch (const psmas::InvalidArgumentError &e)
+        {
+            err = e.what();
+            return PSMAS_ERROR_INVALID_ARGUMENT;
+        }
+        catch (const std::bad_alloc &)
+        {
+            err = "[MAS] host allocation failed.";
+            return PSMAS_ERROR_OUT_OF_MEMORY;
+        }
+        catch (const std::exception &e)
+        {
+            err = e.what();
+            return PSMAS_ERROR_RUNTIME;
+        }
+        catch (...)
+        {
+            err = "[MAS] unknown error.";
+            return PSMAS_ERROR_RUNTIME;
+        }
+    }
+
+    int write_string(const std::string &s, char *buf, size_t cap, size_t *needed)
+    {
+        if (needed)
+            *needed = s.size() + 1;
+        if (buf == nullptr || cap == 0)
+            return PSMAS_ERROR_BUFFER_TOO_SMALL;
+        const size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+        return (s.size() + 1 > cap) ? PSMAS_ERROR_BUFFER_TOO_SMALL : PSMAS_OK;
+    }
+
+    int null_handle() { return PSMAS_ERROR_INVALID_ARGUMENT; }
+} // namespace
+
+extern "C"
+{
+    int psmas_create(psmas_handle *out)
+    {
+        if (out == nullptr)
+            return PSMAS_ERROR_INVALID_ARGUMENT;
+        *out = nullptr;
+        return guarded(g_create_error, [&] { *out = new psmas_solver(); });
+    }
+
+    int psmas_destroy(psmas_handle h)
+    {
+        delete h;
+        return PSMAS_OK;
+    }
+
+    const char *psmas_name(void) { return "MAS"; }
+
+    int psmas_set_parameters(psmas_handle h, const char *json)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            if (json == nullptr)
+                throw psmas::InvalidArgumentError("[MAS] null parameter string.");
+            h->solver.set_parameters_json(json);
+        });
+    }
+
+    int psmas_get_parameters(psmas_handle h, char *buf, size_t cap, size_t *needed)
+    {
+        if (!h)
+            return null_handle();
+        return write_string(h->solver.parameters_json(), buf, cap, needed);
+    }
+
+    int psmas_get_info(psmas_handle h, char *buf, size_t cap, size_t *needed)
+    {
+        if (!h)
+            return null_handle();
+        return write_string(h->solver.get_info_json(), buf, cap, needed);
+    }
+
+    int psmas_analyze_pattern(psmas_handle h, int, const int *, const int *, int)
+    {
+        if (!h)
+            return null_handle();
+        return PSMAS_OK;
+    }
+
+    int psmas_factorize_csc(psmas_handle h, int n, long long nnz, const int *colptr, const int *rowidx,
+                            const double *vals)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            psmas::CscView A;
+            A.n = n;
+            A.nnz = nnz;
+            A.colptr = colptr;
+            A.rowidx = rowidx;
+            A.vals = vals;
+            h->solver.factorize(A);
+        });
+    }
+
+    int psmas_solve(psmas_handle h, int n, const double *b, double *x)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.solve_host(n, b, x); });
+    }
+
+    int psmas_solve_device(psmas_handle h, int n, const double *d_b, double *d_x)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.solve_device(n, d_b, d_x); });
+    }
+
+    const char *psmas_last_error(psmas_handle h)
+    {
+        return h ? h->last_error.c_str() : g_create_error.c_str();
+    }
+
+    int psmas_set_coordinates(psmas_handle h, int nb, int dim, const double *xyz)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.set_coordinates(nb, dim, xyz); });
+    }
+
+    int psmas_device_count(void)
+    {
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return 0;
+        }
+        return c;
+    }
+
+    int psmas_parse_parameters(const char *json, char *buf, size_t cap, size_t *needed)
+    {
+        std::string err;
+        psmas::Params p;
+        const int rc = guarded(err, [&] {
+            if (json == nullptr)
+                throw psmas::InvalidArgumentError("[MAS] null parameter string.");
+            psmas::apply_json_parameters(p, json);
+        });
+        if (rc != PSMAS_OK)
+        {
+            write_string(err, buf, cap, needed);
+            return rc;
+        }
+        return write_string(psmas::params_to_json(p), buf, cap, needed);
+    }
+
+    int psmas_get_level_info(psmas_handle h, int *info, int cap)
+    {
+        if (!h || !info)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            const std::vector<int> v = h->solver.level_info();
+            if (cap < static_cast<int>(v.size()))
+                throw psmas::InvalidArgumentError("[MAS] level info buffer too small.");
+            std::memcpy(info, v.data(), sizeof(int) * v.size());
+        });
+    }
+
+    int psmas_get_partition(psmas_handle h, int *nparts, int *offsets, int offsets_cap, int *perm, int perm_cap)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            std::vector<int> off, pm;
+            h->solver.get_partition(off, pm);
+            if (nparts)
+                *nparts = static_cast<int>(off.size()) - 1;
+            if (offsets)
+            {
+                if (offsets_cap < static_cast<int>(off.size()))
+                    throw psmas::InvalidArgumentError("[MAS] partition offsets buffer too small.");
+                std::memcpy(offsets, off.data(), sizeof(int) * off.size());
+            }
+            if (perm)
+            {
+                if (perm_cap < static_cast<int>(pm.size()))
+                    throw psmas::InvalidArgumentError("[MAS] permutation buffer too small.");
+                std::memcpy(perm, pm.data(), sizeof(int) * pm.size());
+            }
+        });
+    }
+
+    int psmas_get_coarse_map(psmas_handle h, int *map, long long cap)
+    {
+        if (!h || !map)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            const std::vector<int> v = h->solver.coarse_map();
+            if (cap < static_cast<long long>(v.size()))
+                throw psmas::InvalidArgumentError("[MAS] coarse map buffer too small.");
+            std::memcpy(map, v.data(), sizeof(int) * v.size());
+        });
+    }
+
+    int psmas_get_bank_inverses(psmas_handle h, double *out, long long cap)
+    {
+        if (!h || !out)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            const std::vector<double> v = h->solver.bank_inverses();
+            if (cap < static_cast<long long>(v.size()))
+                throw psmas::InvalidArgumentError("[MAS] bank matrix buffer too small.");
+            std::memcpy(out, v.data(), sizeof(double) * v.size());
+        });
+    }
+
+    int psmas_apply_preconditioner(psmas_handle h, int n, const double *r, double *z, int mixed)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            if (n != h->solver.dim())
+                throw std::runtime_error("[MAS] Invalid vector size for MAS preconditioner.");
+            if (!r || !z)
+                throw psmas::InvalidArgumentError("[MAS] null vector.");
+            h->solver.apply_preconditioner(r, z, mixed != 0);
+        });
+    }
+
+    int psmas_spmv(psmas_handle h, int n, const double *x, double *y, int fp32)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            if (n != h->solver.dim())
+                throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
+            if (!x || !y)
+                throw psmas::InvalidArgumentError("[MAS] null vector.");
+            h->solver.spmv(x, y, fp32 != 0);
+        });
+    }
+
+    int psmas_set_kernel_timing(psmas_handle h, int on)
+    {
+        if (!h)
+            return null_handle();
+        h->solver.set_kernel_timing(on != 0);
+        if (on)
+            h->solver.reset_kernel_timings();
+        return PSMAS_OK;
+    }
+
+    int psmas_get_kernel_timing(psmas_handle h, char *buf, size_t cap, size_t *needed)
+    {
+        if (!h)
+            return null_handle();
+        std::ostringstream o;
+        o.precision(17);
+        o << "{";
+        bool first = true;
+        for (const auto &t : h->solver.kernel_timings())
+        {
+            if (!first)
+                o << ", ";
+            first = false;
+            o << "\"" << t.name << "\": {\"launches\": " << t.launches << ", \"total_ms\": " << t.total_ms
+              << ", \"bytes\": " << t.bytes_per_launch << "}";
+        }
+        o << "}";
+        return write_string(o.str(), buf, cap, needed);
+    }
+}

@@ -1,0 +1,454 @@
+#include "host_setup.hpp"
+#include "common.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <numeric>
+#include <atomic>
+#include <thread>
+
+namespace psmas
+{
+    void validate_csc(const CscView &A)
+    {
+        if (A.n <= 0 || A.colptr == nullptr || (A.nnz > 0 && (A.rowidx == nullptr || A.vals == nullptr)))
+            throw InvalidArgumentError("[MAS] invalid A: empty matrix or null arrays.");
+        if (A.nnz > INT_MAX)
+            throw InvalidArgumentError("[MAS] A is too large. Non-zero number exceeding int32 max.");
+        if (A.colptr[0] != 0 || A.colptr[A.n] != A.nnz)
+            throw InvalidArgumentError("[MAS] invalid A: column pointers do not span the non-zeros.");
+        for (int c = 0; c < A.n; ++c)
+            if (A.colptr[c + 1] < A.colptr[c])
+                throw InvalidArgumentError("[MAS] invalid A: column pointers are not monotone.");
+        for (long long k = 0; k < A.nnz; ++k)
+            if (A.rowidx[k] < 0 || A.rowidx[k] >= A.n)
+                throw InvalidArgumentError("[MAS] invalid A: row index out of range.");
+    }
+
+    BlockGraph build_block_graph(const CscView &A, int bd)
+    {
+        const int n = A.n;
+        const int nb = div_up(n, bd);
+        BlockGraph g;
+        g.nb = nb;
+        // Block column cb: unique block rows with accumulated |a|; edges stored in both
+        // directions, weights summed over (u,v) and (v,u).
+        std::vector<int> stamp(nb, -1);
+        std::vector<double> acc(nb, 0.0);
+        std::vector<int> touched;
+        std::vector<std::vector<std::pair<int, double>>> lists(nb);
+        for (int cb = 0; cb < nb; ++cb)
+        {
+            touched.clear();
+            for (int c = cb * bd; c < std::min(n, cb * bd + bd); ++c)
+                for (int k = A.colptr[c]; k < A.colptr[c + 1]; ++k)
+                {
+                    const int rb = A.rowidx[k] / bd;
+                    if (rb == cb)
+                        continue;
+                    if (stamp[rb] != cb)
+                    {
+                        stamp[rb] = cb;
+                        acc[rb] = 0.0;
+                        touched.push_back(rb);
+                    }
+                    acc[rb] += std::fabs(A.vals[k]);
+                }
+            for (int rb : touched)
+            {
+                lists[cb].emplace_back(rb, acc[rb]);
+                lists[rb].emplace_back(cb, acc[rb]);
+            }
+        }
+        g.ptr.assign(nb + 1, 0);
+        for (int u = 0; u < nb; ++u)
+        {
+            auto &L = lists[u];
+            std::sort(L.begin(), L.end());
+            int out = 0;
+            for (size_t i = 0; i < L.size(); ++i)
+            {
+                if (out > 0 && L[out - 1].first == L[i].first)
+                    L[out - 1].second += L[i].second;
+                else
+                    L[out++] = L[i];
+            }
+            L.resize(out);
+            g.ptr[u + 1] = g.ptr[u] + out;
+        }
+        g.adj.resize(g.ptr[nb]);
+        g.w.resize(g.ptr[nb]);
+        for (int u = 0; u < nb; ++u)
+        {
+            int o = g.ptr[u];
+            for (const auto &e : lists[u])
+            {
+                g.adj[o] = e.first;
+                g.w[o] = e.second;
+                ++o;
+            }
+        }
+        return g;
+    }
+
+    namespace
+    {
+        /// BFS order of every component, each started from a pseudo-peripheral node.
+        std::vector<int> bfs_order(const BlockGraph &g)
+        {
+            const int nb = g.nb;
+            std::vector<int> order;
+            order.reserve(nb);
+            std::vector<int> level(nb, -1);
+            std::vector<char> done(nb, 0);
+            std::vector<int> queue;
+            auto bfs = [&](int s, std::vector<int> &out, bool mark_done) {
+                out.clear();
+                out.push_back(s);
+                level[s] = 0;
+                for (size_t h = 0; h < out.size(); ++h)
+                {
+                    const int u = out[h];
+                    for (int k = g.ptr[u]; k < g.ptr[u + 1]; ++k)
+                    {
+                        const int v = g.adj[k];
+                        if (level[v] < 0)
+                        {
+                            level[v] = level[u] + 1;
+                            out.push_back(v);
+                        }
+                    }
+                }
+                for (int v : out)
+                {
+                    level[v] = -1;
+                    if (mark_done)
+                        done[v] = 1;
+                }
+            };
+            for (int s = 0; s < nb; ++s)
+            {
+                if (done[s])
+                    continue;
+                bfs(s, queue, false);
+                const int far = queue.back();
+                bfs(far, queue, true);
+                order.insert(order.end(), queue.begin(), queue.end());
+            }
+            return order;
+        }
+    } // namespace
+
+    Partition graph_aggregate(const BlockGraph &g, int max_size)
+    {
+        const int nb = g.nb;
+        Partition P;
+        if (nb <= max_size)
+        {
+            P.nparts = 1;
+            P.offsets = {0, nb};
+            P.perm.resize(nb);
+            std::iota(P.perm.begin(), P.perm.end(), 0);
+            return P;
+        }
+        const std::vector<int> order = bfs_order(g);
+        std::vector<int> rank(nb);
+        for (int i = 0; i < nb; ++i)
+            rank[order[i]] = i;
+
+        std::vector<int> agg(nb, -1);
+        std::vector<std::vector<int>> members;
+        std::vector<int> conn(nb, 0);
+        std::vector<double> wsum(nb, 0.0);
+        std::vector<int> cand_stamp(nb, -1);
+        std::vector<int> cand;
+        for (int s : order)
+        {
+            if (agg[s] != -1)
+                continue;
+            const int a = static_cast<int>(members.size());
+            members.emplace_back();
+            auto &mem = members.back();
+            cand.clear();
+            auto add = [&](int v) {
+                agg[v] = a;
+                mem.push_back(v);
+                for (int k = g.ptr[v]; k < g.ptr[v + 1]; ++k)
+                {
+                    const int u = g.adj[k];
+                    if (agg[u] != -1)
+                        continue;
+                    if (cand_stamp[u] != a)
+                    {
+                        cand_stamp[u] = a;
+                        conn[u] = 0;
+                        wsum[u] = 0.0;
+                        cand.push_back(u);
+                    }
+                    conn[u] += 1;
+                    wsum[u] += g.w[k];
+                }
+            };
+            add(s);
+            while (static_cast<int>(mem.size()) < max_size)
+            {
+                int best = -1;
+                for (int u : cand)
+                {
+                    if (agg[u] != -1)
+                        continue;
+                    if (best < 0 || conn[u] > conn[best] ||
+                        (conn[u] == conn[best] && (wsum[u] > wsum[best] ||
+                                                   (wsum[u] == wsum[best] && rank[u] < rank[best]))))
+                        best = u;
+                }
+                if (best < 0)
+                    break;
+                add(best);
+            }
+        }
+
+        // Merge small aggregates into their most strongly connected neighbour with room.
+        const int nagg = static_cast<int>(members.size());
+        std::vector<int> target(nagg);
+        std::iota(target.begin(), target.end(), 0);
+        std::vector<int> size(nagg);
+        for (int a = 0; a < nagg; ++a)
+            size[a] = static_cast<int>(members[a].size());
+        auto find = [&](int a) {
+            while (target[a] != a)
+                a = target[a];
+            return a;
+        };
+        std::vector<double> link(nagg, 0.0);
+        std::vector<int> link_stamp(nagg, -1);
+        for (int a = 0; a < nagg; ++a)
+        {
+            if (size[a] >= max_size / 4 || find(a) != a)
+                continue;
+            std::vector<int> nbrs;
+            for (int v : members[a])
+                for (int k = g.ptr[v]; k < g.ptr[v + 1]; ++k)
+                {
+                    const int b = find(agg[g.adj[k]]);
+                    if (b == a)
+                        continue;
+                    if (link_stamp[b] != a)
+                    {
+                        link_stamp[b] = a;
+                        link[b] = 0.0;
+                        nbrs.push_back(b);
+                    }
+                    link[b] += g.w[k];
+                }
+            int best = -1;
+            for (int b : nbrs)
+                if (size[b] + size[a] <= max_size && (best < 0 || link[b] > link[best]))
+                    best = b;
+            if (best >= 0)
+            {
+                target[a] = best;
+                size[best] += size[a];
+            }
+        }
+        std::vector<std::vector<int>> final_members;
+        std::vector<int> final_id(nagg, -1);
+        for (int a = 0; a < nagg; ++a)
+        {
+            const int r = find(a);
+            if (final_id[r] < 0)
+            {
+                final_id[r] = static_cast<int>(final_members.size());
+                final_members.emplace_back();
+            }
+        }
+        for (int a = 0; a < nagg; ++a)
+        {
+            auto &dst = final_members[final_id[find(a)]];
+            dst.insert(dst.end(), members[a].begin(), members[a].end());
+        }
+        P.nparts = static_cast<int>(final_members.size());
+        P.offsets.assign(P.nparts + 1, 0);
+        P.perm.assign(nb, -1);
+        int pos = 0;
+        for (int p = 0; p < P.nparts; ++p)
+        {
+            auto &m = final_members[p];
+            std::sort(m.begin(), m.end());
+            for (int v : m)
+                P.perm[v] = pos++;
+            P.offsets[p + 1] = pos;
+        }
+        return P;
+    }
+
+    Partition chunk_partition(const std::vector<int> &order, int max_size)
+    {
+        const int nb = static_cast<int>(order.size());
+        Partition P;
+        P.nparts = std::max(1, div_up(nb, max_size));
+        P.offsets.resize(P.nparts + 1);
+        for (int p = 0; p <= P.nparts; ++p)
+            P.offsets[p] = std::min(nb, p * max_size);
+        P.perm.resize(nb);
+        for (int i = 0; i < nb; ++i)
+            P.perm[order[i]] = i;
+        return P;
+    }
+
+    namespace
+    {
+        template <typename F>
+        void parallel_for_blocks(int count, F &&f)
+        {
+            unsigned hw = std::thread::hardware_concurrency();
+            int nt = static_cast<int>(std::min<unsigned>(hw == 0 ? 1 : hw, 16));
+            if (count < 4096 || nt <= 1)
+            {
+                f(0, count);
+                return;
+            }
+            std::vector<std::thread> th;
+            const int chunk = div_up(count, nt);
+            for (int t = 0; t < nt; ++t)
+            {
+                const int b = t * chunk, e = std::min(count, b + chunk);
+                if (b >= e)
+                    break;
+                th.emplace_back([&f, b, e] { f(b, e); });
+            }
+            for (auto &x : th)
+                x.join();
+        }
+    } // namespace
+
+    void build_padded_bsr(const CscView &A, int bd, const Partition &part, PaddedBsr &out)
+    {
+        const int n = A.n;
+        const int nb = div_up(n, bd);
+        const int bs = bd * bd;
+        const int padded_scalars = bd * nb - n;
+        out.bd = bd;
+        out.nb = nb;
+        out.nparts = part.nparts + (part.nparts & 1);
+        out.npad = out.nparts * BANK;
+        out.slot_of_block.assign(nb, -1);
+        out.block_of_slot.assign(out.npad, -1);
+        for (int p = 0; p < part.nparts; ++p)
+        {
+            const int size = part.offsets[p + 1] - part.offsets[p];
+            if (size < 0 || size > BANK)
+                throw InvalidArgumentError("[MAS] internal: domain larger than 32 nodes.");
+        }
+        // permuted id -> part
+        std::vector<int> part_of(nb);
+        for (int p = 0; p < part.nparts; ++p)
+            for (int i = part.offsets[p]; i < part.offsets[p + 1]; ++i)
+                part_of[i] = p;
+        for (int b = 0; b < nb; ++b)
+        {
+            const int pid = part.perm[b];
+            const int p = part_of[pid];
+            const int slot = BANK * p + (pid - part.offsets[p]);
+            out.slot_of_block[b] = slot;
+            out.block_of_slot[slot] = b;
+        }
+        const std::vector<int> &slot = out.slot_of_block;
+
+        // Pattern: per block column, unique block rows; transpose into rows.
+        std::vector<int> stamp(nb, -1);
+        std::vector<int> pr, pc;
+        pr.reserve(static_cast<size_t>(A.nnz / std::max(1, bs)) + nb);
+        pc.reserve(pr.capacity());
+        for (int cb = 0; cb < nb; ++cb)
+        {
+            const int C = slot[cb];
+            for (int c = cb * bd; c < std::min(n, cb * bd + bd); ++c)
+                for (int k = A.colptr[c]; k < A.colptr[c + 1]; ++k)
+                {
+                    const int rb = A.rowidx[k] / bd;
+                    if (stamp[rb] == cb)
+                        continue;
+                    stamp[rb] = cb;
+                    pr.push_back(slot[rb]);
+                    pc.push_back(C);
+                }
+            if (padded_scalars > 0 && cb == nb - 1 && stamp[cb] != cb)
+            {
+                stamp[cb] = cb;
+                pr.push_back(C);
+                pc.push_back(C);
+            }
+        }
+        out.rows.assign(out.npad + 1, 0);
+        for (int R : pr)
+            out.rows[R + 1]++;
+        for (int i = 0; i < out.npad; ++i)
+            out.rows[i + 1] += out.rows[i];
+        out.cols.resize(pr.size());
+        {
+            std::vector<int> cur(out.rows.begin(), out.rows.end() - 1);
+            for (size_t t = 0; t < pr.size(); ++t)
+                out.cols[cur[pr[t]]++] = pc[t];
+        }
+        std::vector<int>().swap(pr);
+        std::vector<int>().swap(pc);
+        parallel_for_blocks(out.npad, [&](int b, int e) {
+            for (int r = b; r < e; ++r)
+                std::sort(out.cols.begin() + out.rows[r], out.cols.begin() + out.rows[r + 1]);
+        });
+        out.vals.assign(out.cols.size() * bs, 0.0);
+        if (!refresh_padded_bsr_values(A, out))
+            throw InvalidArgumentError("[MAS] internal: BSR pattern mismatch.");
+    }
+
+    bool refresh_padded_bsr_values(const CscView &A, PaddedBsr &bsr)
+    {
+        const int n = A.n, bd = bsr.bd, nb = bsr.nb, bs = bd * bd;
+        if (div_up(n, bd) != nb)
+            return false;
+        std::fill(bsr.vals.begin(), bsr.vals.end(), 0.0);
+        std::atomic<bool> ok{true};
+        // Parallel over block columns: distinct block columns write distinct blocks.
+        parallel_for_blocks(nb, [&](int b0, int b1) {
+            for (int cb = b0; cb < b1; ++cb)
+            {
+                const int C = bsr.slot_of_block[cb];
+                for (int c = cb * bd; c < std::min(n, cb * bd + bd); ++c)
+                    for (int k = A.colptr[c]; k < A.colptr[c + 1]; ++k)
+                    {
+                        const int r = A.rowidx[k];
+                        const int R = bsr.slot_of_block[r / bd];
+                        const int *beg = bsr.cols.data() + bsr.rows[R];
+                        const int *end = bsr.cols.data() + bsr.rows[R + 1];
+                        const int *it = std::lower_bound(beg, end, C);
+                        if (it == end || *it != C)
+                        {
+                            ok = false;
+                            continue;
+                        }
+                        const size_t pos = static_cast<size_t>(it - bsr.cols.data());
+                        bsr.vals[pos * bs + (r % bd) * bd + (c % bd)] += A.vals[k];
+                    }
+            }
+        });
+        const int padded_scalars = bd * nb - n;
+        if (padded_scalars > 0)
+        {
+            const int T = bsr.slot_of_block[nb - 1];
+            const int *beg = bsr.cols.data() + bsr.rows[T];
+            const int *end = bsr.cols.data() + bsr.rows[T + 1];
+            const int *it = std::lower_bound(beg, end, T);
+            if (it == end || *it != T)
+                return false;
+            const size_t pos = static_cast<size_t>(it - bsr.cols.data());
+            for (int t = 0; t < padded_scalars; ++t)
+            {
+                const int local = (bd - padded_scalars) + t;
+                bsr.vals[pos * bs + local * bd + local] = 1.0;
+            }
+        }
+        return ok;
+    }
+} // namespace psmas

@@ -1,0 +1,79 @@
+// Host-side setup of the MAS solver: input validation, block graph, graph aggregation into
+// <=32-node domains, and the permuted, bank-padded BSR matrix.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace psmas
+{
+    constexpr int BANK = 32; ///< Nodes per MAS domain (bank); fixed by the reference (MASPreconditioner.cu:33).
+    constexpr int MAX_COARSE_LEVELS = 4; ///< Coarse levels (MASPreconditioner.hpp:70).
+
+    /// Compressed column-major input, the layout of polysolve's StiffnessMatrix
+    /// (src/polysolve/Types.hpp:11-15) with int32 indices.
+    struct CscView
+    {
+        int n = 0;
+        long long nnz = 0;
+        const int *colptr = nullptr;
+        const int *rowidx = nullptr;
+        const double *vals = nullptr;
+    };
+
+    /// Validate shape and index ranges; throws InvalidArgumentError ("[MAS] invalid A ...") the
+    /// way the reference rejects bad input (BSRAdjacency.cu:466-481, BSRMatrix.cu:438-442).
+    void validate_csc(const CscView &A);
+
+    /// Symmetric block graph without self loops; weight of {u,v} = sum of |a| over the scalars of
+    /// blocks (u,v) and (v,u) (the reference's adjacency weight, BSRAdjacency.cu:60-84, without
+    /// the integer quantisation that only its external partitioner needed).
+    struct BlockGraph
+    {
+        int nb = 0;
+        std::vector<int> ptr;
+        std::vector<int> adj;
+        std::vector<double> w;
+    };
+    BlockGraph build_block_graph(const CscView &A, int bd);
+
+    /// Domain decomposition: domain p holds permuted block ids [offsets[p], offsets[p+1]), at most
+    /// 32 nodes; perm maps an input block id to its permuted id.  Mirrors the reference's
+    /// part_offsets / permutation contract (MASSolver.cu:86-114,223-242).
+    struct Partition
+    {
+        int nparts = 0;
+        std::vector<int> offsets;
+        std::vector<int> perm;
+    };
+
+    /// Greedy aggregation on the block graph: seeds in breadth-first order from a
+    /// pseudo-peripheral node; each aggregate grows by the unassigned neighbour with the most
+    /// edges into it (ties: larger weight, then earlier BFS rank) until it holds max_size nodes.
+    /// Aggregates smaller than max_size/4 merge into their most strongly connected neighbour
+    /// aggregate when the merged size stays <= max_size.
+    Partition graph_aggregate(const BlockGraph &g, int max_size);
+
+    /// Partition from a node order (e.g. Morton): consecutive runs of 32.
+    Partition chunk_partition(const std::vector<int> &order, int max_size);
+
+    /// Bank-padded BSR in the solver's working space: node slot = 32*part + rank in part; the
+    /// part count is rounded up to an even number so that slices of 64 rows hold two whole banks.
+    /// Rows of virtual slots are empty.  Blocks are row-major bd x bd, columns sorted; the
+    /// trailing partial block gets 1.0 on its padded diagonal (BSRMatrix.cu:123-134,189-191).
+    struct PaddedBsr
+    {
+        int bd = 0;
+        int nb = 0;        ///< input block rows
+        int nparts = 0;    ///< even
+        int npad = 0;      ///< 32 * nparts
+        std::vector<int> slot_of_block; ///< input block -> slot
+        std::vector<int> block_of_slot; ///< slot -> input block, -1 virtual
+        std::vector<int> rows;          ///< npad + 1
+        std::vector<int> cols;
+        std::vector<double> vals;
+    };
+    void build_padded_bsr(const CscView &A, int bd, const Partition &part, PaddedBsr &out);
+    /// Refresh only the values of an existing pattern (same sparsity), for re-factorisation.
+    bool refresh_padded_bsr_values(const CscView &A, PaddedBsr &bsr);
+} // namespace psmas

@@ -1,0 +1,156 @@
+// Launch interface of the MI355X MAS kernels.  Every launcher enqueues on the given stream and
+// returns immediately.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace psmas
+{
+    /// SELL-64 block matrix: rows (bank slots) in slices of 64; slot k of lane l holds block k of
+    /// row 64*s + l.  cols[(slice_ptr[s] + k) * 64 + l]; vals[((slice_ptr[s] + k) * bd*bd + e) * 64 + l].
+    /// Short rows are padded with zero blocks whose column is the row itself.
+    template <typename TV>
+    struct SellView
+    {
+        int nslices = 0;
+        const int *slice_ptr = nullptr;
+        const int *cols = nullptr;
+        const TV *vals = nullptr;
+    };
+
+    /// Restriction of fine (bank-slot) values to coarse level 0: slot p that leads its bank-local
+    /// connected component carries the component's member mask (lanes of its bank) and the
+    /// component's global id.
+    struct FineRestrict
+    {
+        const uint32_t *lead = nullptr;
+        const int *gid = nullptr;
+    };
+
+    struct ReduceSite
+    {
+        double *partials = nullptr;
+        unsigned *counter = nullptr;
+    };
+
+    enum class SpmvMode
+    {
+        AP,  ///< y = A x; pAp = x . y; alpha = rz / pAp
+        RES, ///< y = b - A x; rr = y . y into a chosen slot; restrict y to coarse level 0
+    };
+
+    // ----- SpMV ---------------------------------------------------------------------------
+    /// y = A x (+ pAp and alpha).  TV matrix value type, TX vector type.
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, double *sc, ReduceSite site,
+                        hipStream_t s);
+
+    /// y = b - A x, rr = y.y -> sc[rr_slot]; restrict y -> yc0.  TX: x,b; TY: y.
+    template <int BD, typename TV, typename TX, typename TY>
+    void launch_spmv_res(const SellView<TV> &A, const TX *x, const TX *b, TY *y, FineRestrict fr, TY *yc0,
+                         double *sc, int rr_slot, ReduceSite site, hipStream_t s);
+
+    /// Plain y = A x without reductions (debug / parity path).
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_plain(const SellView<TV> &A, const TX *x, TX *y, hipStream_t s);
+
+    // ----- vectors ------------------------------------------------------------------------
+    /// x += alpha p; r -= alpha Ap; rr = r.r; restrict r -> rc0.  One thread per node.
+    template <int BD, typename T>
+    void launch_update_xr(int nnodes, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0, double *sc,
+                          ReduceSite site, hipStream_t s);
+
+    /// x += alpha p (scalars).
+    template <typename T>
+    void launch_axpy_alpha(long long n, const T *p, T *x, const double *sc, hipStream_t s);
+
+    /// p = z + beta p (scalars).
+    template <typename T>
+    void launch_update_p(long long n, const T *z, T *p, const double *sc, hipStream_t s);
+
+    /// dst = src (elementwise, with conversion).
+    template <typename TS, typename TD>
+    void launch_copy_convert(long long n, const TS *src, TD *dst, hipStream_t s);
+
+    /// x64 += x32; x32 = 0.
+    void launch_accumulate_correction(long long n, float *x32, double *x64, hipStream_t s);
+
+    /// Fill with a value (used instead of memset inside captured graphs).
+    template <typename T>
+    void launch_fill(long long n, T *dst, T value, hipStream_t s);
+
+    /// dst[slot*bd + c] = src[block*bd + c] for real slots (block = block_of_slot[slot], scalar < n),
+    /// zero elsewhere.
+    template <typename T>
+    void launch_gather_in(int npad, int bd, int n, const int *block_of_slot, const T *src, T *dst, hipStream_t s);
+    /// dst[block*bd + c] = src[slot_of_block[block]*bd + c] for scalar < n.
+    template <typename T>
+    void launch_scatter_out(int nb, int bd, int n, const int *slot_of_block, const T *src, T *dst, hipStream_t s);
+
+    /// sc[slot] = value
+    void launch_set_scalar(double *sc, int slot, double value, hipStream_t s);
+    /// Arbitrary dot product into a scalar slot (deterministic).
+    template <typename T>
+    void launch_dot(long long n, const T *a, const T *b, double *sc, int slot, ReduceSite site, hipStream_t s);
+
+    // ----- MAS apply ----------------------------------------------------------------------
+    struct CoarseRestrict
+    {
+        int level_num = 0;          ///< coarse levels
+        int counts[4] = {0, 0, 0, 0}; ///< coarse nodes per level
+        int offsets[5] = {0, 0, 0, 0, 0}; ///< node offset of multi-level segment (0 = fine)
+        const uint32_t *lead[4] = {nullptr, nullptr, nullptr, nullptr}; ///< index: level-(l-1) node, l >= 1
+        const int *gid[4] = {nullptr, nullptr, nullptr, nullptr};
+    };
+
+    /// Restrict coarse level 0 to levels 1..L-1 (one workgroup).
+    template <int BD, typename T>
+    void launch_restrict_coarse(const CoarseRestrict &cr, T *mr, hipStream_t s);
+
+    /// y_m = Inv_m x_m for matrices [mat_begin, mat_begin + count) of the multi-level vectors.
+    /// Storage: packed upper row-major.  Half storage is scaled: Inv = D S D with D per row.
+    template <int BD, typename TM, typename T>
+    void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
+                          hipStream_t s);
+
+    /// z_p = mz_p + sum_l mz_l[cmap_p.l]; rz = r.z with beta epilogue (or init epilogue).
+    template <int BD, typename T>
+    void launch_prolong_dot(int npad, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
+                            double *sc, bool init, ReduceSite site, hipStream_t s);
+
+    // ----- MAS build ----------------------------------------------------------------------
+    /// Bank-local connected components at coarse level 0 (one lane per slot, two banks per wave).
+    void launch_cc_level0(int npad, const SellView<double> &A, const int *row_len, const int *block_of_slot,
+                          uint32_t *lead, int *local_id, int *bank_count, hipStream_t s);
+    /// Neighbour masks among level-(lv-1) ids (atomicOr into nbr).
+    void launch_level_masks(int npad, int lv, const SellView<double> &A, const int *row_len, const int *block_of_slot,
+                            const int4 *cmap, uint32_t *nbr, hipStream_t s);
+    /// Components among `count` level-(lv-1) nodes grouped in banks of 32.
+    void launch_cc_coarse(int count, const uint32_t *nbr, uint32_t *lead, int *local_id, int *bank_count,
+                          hipStream_t s);
+    /// Exclusive scan of n ints (single workgroup); total -> out[n].
+    void launch_scan(int n, const int *in, int *out, hipStream_t s);
+    /// gid[q] = local[q] + prefix[q / 32] for q < count.
+    void launch_make_gid(int count, const int *local_id, const int *prefix, int *gid, hipStream_t s);
+    /// cmap[p].lv = (lv == 0) ? gid0[p] : gid_lv[cmap[p].(lv-1)] for real slots.
+    void launch_fill_cmap(int npad, int lv, const int *block_of_slot, const int *gid, int4 *cmap, hipStream_t s);
+    /// Galerkin bank matrices (packed upper FP64), all levels, from the fine matrix.
+    void launch_galerkin(int npad, int bd, int level_num, const int *mat_off, const SellView<double> &A,
+                         const int *row_len, const int *block_of_slot, const int4 *cmap, double *mats, hipStream_t s);
+    /// In-place batched inverse of packed SPD matrices (zero diagonal -> 1 first).  Sets *fail on
+    /// a zero / non-finite pivot or a non-positive inverse diagonal.
+    void launch_invert(int bd, int count, double *mats, int *fail, hipStream_t s);
+    /// Scaled FP16 form of packed inverses: Inv = D S D, S packed FP16, D FP32 per row.
+    void launch_make_half(int bd, int count, const double *mats, __half *out, float *dscale, hipStream_t s);
+
+    /// Morton keys of quantised coordinates (21 bits per axis).
+    void launch_morton_keys(int nb, int dim, const double *xyz, const double *lo, const double *inv_extent,
+                            uint64_t *keys, int *ids, hipStream_t s);
+    /// Radix sort (key, id) pairs ascending by key; temp storage managed by the caller.
+    size_t morton_sort_temp_bytes(int nb);
+    void morton_sort(int nb, void *temp, size_t temp_bytes, const uint64_t *keys_in, uint64_t *keys_out,
+                     const int *ids_in, int *ids_out, hipStream_t s);
+} // namespace psmas

@@ -1,0 +1,736 @@
+// MAS preconditioner kernels for gfx950: build (bank-local connected components over <= 4
+// coarse levels, Galerkin bank matrices, batched dense inverses, Morton keys) and apply
+// (coarse restriction, batched symmetric bank GEMV staged in LDS, prolongation fused with r.z).
+//
+// Semantics follow the reference preconditioner (src/polysolve/linear/mas_utils/
+// MASPreconditioner.cu): coarse space :117-481, Galerkin fill :484-591, zero-diagonal padding
+// :779-796, inverse :799-936, apply :593-777,1099-1164.  The kernels are this build's own:
+//  * the component search runs two 32-node banks per wave64 and reads each bank's half of a
+//    64-bit ballot;
+//  * restriction sums each component from its leader's member mask, in a fixed order, so no
+//    atomics and no zero-fill of the coarse vectors are needed per apply;
+//  * the inverse is an in-LDS Gauss-Jordan on the full matrix with 256 threads; a second kernel
+//    emits the FP16 form S = D^-1 Inv D^-1, D = sqrt(diag(Inv)), used by the mixed-precision path.
+#include "device.hpp"
+#include "kernels.hpp"
+#include "common.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+namespace psmas
+{
+    // Matrix-element conversion used by the apply kernel.
+    template <typename TA>
+    __device__ __forceinline__ TA to_float_or_double(double v) { return static_cast<TA>(v); }
+    template <typename TA>
+    __device__ __forceinline__ TA to_float_or_double(float v) { return static_cast<TA>(v); }
+    template <typename TA>
+    __device__ __forceinline__ TA to_float_or_double(__half v) { return static_cast<TA>(__half2float(v)); }
+
+    namespace
+    {
+        // ------------------------------------------------------------------------ restriction
+        template <int BD, typename T>
+        __global__ __launch_bounds__(1024) void k_restrict_coarse(CoarseRestrict cr, T *__restrict__ mr)
+        {
+            for (int l = 1; l < cr.level_num; ++l)
+            {
+                const T *src = mr + static_cast<size_t>(cr.offsets[l]) * BD;
+                T *dst = mr + static_cast<size_t>(cr.offsets[l + 1]) * BD;
+                const int count = cr.counts[l - 1];
+                for (int q = threadIdx.x; q < count; q += blockDim.x)
+                {
+                    uint32_t mask = cr.lead[l][q];
+                    if (mask == 0u)
+                        continue;
+                    const int base = q & ~31;
+                    double s[BD];
+#pragma unroll
+                    for (int i = 0; i < BD; ++i)
+                        s[i] = 0.0;
+                    while (mask)
+                    {
+                        const int b = __builtin_ctz(mask);
+                        mask &= mask - 1u;
+#pragma unroll
+                        for (int i = 0; i < BD; ++i)
+                            s[i] += static_cast<double>(src[static_cast<size_t>(base + b) * BD + i]);
+                    }
+                    const int g = cr.gid[l][q];
+#pragma unroll
+                    for (int i = 0; i < BD; ++i)
+                        dst[static_cast<size_t>(g) * BD + i] = static_cast<T>(s[i]);
+                }
+                __syncthreads();
+            }
+        }
+
+        // ------------------------------------------------------------------------------ apply
+        template <int BD>
+        constexpr int apply_block() { return BD == 1 ? 64 : (BD == 2 ? 128 : 192); }
+
+        /// y = Inv x for one bank matrix per workgroup.  The packed upper matrix is staged in LDS
+        /// with 16-byte loads; threads (row, half) each sum half of row `row`, reading element
+        /// (t, row) for t < row (packed row t, contiguous across threads) and (row, t) otherwise.
+        template <int BD, typename TM, typename T>
+        __global__ __launch_bounds__(apply_block<BD>()) void k_mas_apply(int mat_begin, const TM *__restrict__ mats,
+                                                                          const float *__restrict__ dscale,
+                                                                          const T *__restrict__ mr, T *__restrict__ mz)
+        {
+            constexpr int N = 32 * BD;
+            constexpr int L = N * (N + 1) / 2;
+            constexpr int BLOCK = apply_block<BD>();
+            constexpr bool SCALED = sizeof(TM) == 2;
+            using TA = typename std::conditional<sizeof(TM) == 8, double, float>::type;
+            static_assert((L * sizeof(TM)) % 16 == 0, "packed matrix must be a whole number of 16 B chunks");
+
+            __shared__ __attribute__((aligned(16))) TM sA[L];
+            __shared__ TA sx[N];
+            __shared__ TA part[N];
+
+            const int m = mat_begin + blockIdx.x;
+            const TM *A = mats + static_cast<size_t>(m) * L;
+            {
+                using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
+                const u32x4 *src = reinterpret_cast<const u32x4 *>(A);
+                u32x4 *dst = reinterpret_cast<u32x4 *>(sA);
+                constexpr int CH = static_cast<int>(L * sizeof(TM) / 16);
+                for (int i = threadIdx.x; i < CH; i += BLOCK)
+                    dst[i] = __builtin_nontemporal_load(src + i);
+            }
+            for (int i = threadIdx.x; i < N; i += BLOCK)
+            {
+                TA v = static_cast<TA>(mr[static_cast<size_t>(m) * N + i]);
+                if (SCALED)
+                    v *= static_cast<TA>(dscale[static_cast<size_t>(m) * N + i]);
+                sx[i] = v;
+            }
+            __syncthreads();
+
+            const int row = threadIdx.x % N;
+            const int half = threadIdx.x / N; // 0 or 1
+            const int t0 = half * (N / 2), t1 = t0 + N / 2;
+            const int rs_row = row * N - (row * (row + 1)) / 2;
+            int rs_t = t0 * N - (t0 * (t0 + 1)) / 2;
+            TA acc = TA(0);
+            if (threadIdx.x < 2 * N)
+            {
+#pragma unroll 8
+                for (int t = t0; t < t1; ++t)
+                {
+                    const int idx = (t < row) ? (rs_t + row) : (rs_row + t);
+                    acc = fma(to_float_or_double<TA>(sA[idx]), sx[t], acc);
+                    rs_t += N - t - 1;
+                }
+            }
+            if (half == 1 && threadIdx.x < 2 * N)
+                part[row] = acc;
+            __syncthreads();
+            if (half == 0)
+            {
+                TA y = acc + part[row];
+                if (SCALED)
+                    y *= static_cast<TA>(dscale[static_cast<size_t>(m) * N + row]);
+                mz[static_cast<size_t>(m) * N + row] = static_cast<T>(y);
+            }
+        }
+
+        // ----------------------------------------------------------------------- prolongation
+        template <int BD, typename T>
+        __global__ __launch_bounds__(256) void k_prolong_dot(int npad, int level_num, const int4 *__restrict__ cmap,
+                                                             const int *__restrict__ lev_off, const T *__restrict__ r,
+                                                             T *__restrict__ mz, double *__restrict__ sc, bool init,
+                                                             ReduceSite site)
+        {
+            const int p = blockIdx.x * 256 + threadIdx.x;
+            double dot = 0.0;
+            if (p < npad)
+            {
+                const int4 c = cmap[p];
+                const int cc[4] = {c.x, c.y, c.z, c.w};
+                double z[BD];
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                    z[i] = static_cast<double>(mz[static_cast<size_t>(p) * BD + i]);
+                if (cc[0] >= 0)
+                {
+                    for (int l = 0; l < level_num; ++l)
+                    {
+                        const size_t o = static_cast<size_t>(lev_off[l + 1] + cc[l]) * BD;
+#pragma unroll
+                        for (int i = 0; i < BD; ++i)
+                            z[i] += static_cast<double>(mz[o + i]);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                {
+                    const T zi = static_cast<T>(z[i]);
+                    mz[static_cast<size_t>(p) * BD + i] = zi;
+                    dot += static_cast<double>(r[static_cast<size_t>(p) * BD + i]) * static_cast<double>(zi);
+                }
+            }
+            grid_sum<256>(dot, site.partials, site.counter, [sc, init](double t) {
+                if (init)
+                {
+                    sc[S_RZ] = t;
+                    sc[S_RZ0] = t;
+                    sc[S_RZ_OLD] = t;
+                    sc[S_BETA] = 0.0;
+                }
+                else
+                {
+                    const double old = sc[S_RZ];
+                    sc[S_RZ_OLD] = old;
+                    sc[S_RZ] = t;
+                    sc[S_BETA] = safe_div(t, old, sc[S_GUARD]);
+                }
+            });
+        }
+
+        // ----------------------------------------------------------------------- build: levels
+        /// Closure of a node's neighbour mask inside its bank (bit-BFS).
+        __device__ __forceinline__ uint32_t closure(uint32_t conn, int lane32, const uint32_t *bank_masks)
+        {
+            uint32_t visited = 1u << lane32;
+            uint32_t to_visit = conn & ~visited;
+            while (to_visit)
+            {
+                const int v = __builtin_ctz(to_visit);
+                conn |= bank_masks[v];
+                visited |= 1u << v;
+                to_visit = conn & ~visited;
+            }
+            return conn;
+        }
+
+        __device__ __forceinline__ void leaders_and_ids(bool valid, uint32_t conn, int lane32, uint32_t &lead_mask,
+                                                        int &local_id, int &bank_count)
+        {
+            const bool is_lead = valid && (__builtin_ctz(conn) == lane32);
+            const unsigned long long ball = __ballot(is_lead);
+            const uint32_t half = static_cast<uint32_t>(ball >> (threadIdx.x & 32));
+            const int lead_lane = valid ? __builtin_ctz(conn) : 0;
+            const uint32_t below = static_cast<uint32_t>((1ull << lead_lane) - 1ull);
+            local_id = valid ? __builtin_popcount(half & below) : -1;
+            lead_mask = is_lead ? conn : 0u;
+            bank_count = __builtin_popcount(half);
+        }
+
+        __global__ __launch_bounds__(256) void k_cc_level0(int npad, SellView<double> A, const int *__restrict__ row_len,
+                                                           const int *__restrict__ block_of_slot, uint32_t *lead,
+                                                           int *local_id, int *bank_count)
+        {
+            __shared__ uint32_t masks[256];
+            const int p = blockIdx.x * 256 + threadIdx.x;
+            const int lane32 = threadIdx.x & 31;
+            const int bank = p / 32;
+            const bool valid = p < npad && block_of_slot[p] >= 0;
+            uint32_t m = valid ? (1u << lane32) : 0u;
+            if (valid)
+            {
+                const int slice = p / WAVE, lane = p % WAVE;
+                const int beg = A.slice_ptr[slice];
+                const int len = row_len[p];
+                for (int k = 0; k < len; ++k)
+                {
+                    const int q = A.cols[static_cast<size_t>(beg + k) * WAVE + lane];
+                    if (q / 32 == bank)
+                        m |= 1u << (q & 31);
+                }
+            }
+            masks[threadIdx.x] = m;
+            __syncthreads();
+            const uint32_t conn = valid ? closure(m, lane32, masks + (threadIdx.x & ~31)) : 0u;
+            uint32_t lm;
+            int lid, cnt;
+            leaders_and_ids(valid, conn, lane32, lm, lid, cnt);
+            if (p < npad)
+            {
+                lead[p] = lm;
+                local_id[p] = lid;
+                if (lane32 == 0)
+                    bank_count[bank] = cnt;
+            }
+        }
+
+        __device__ __forceinline__ int cmap_get(const int4 &c, int l)
+        {
+            return l == 0 ? c.x : (l == 1 ? c.y : (l == 2 ? c.z : c.w));
+        }
+        __device__ __forceinline__ void cmap_set(int4 &c, int l, int v)
+        {
+            if (l == 0)
+                c.x = v;
+            else if (l == 1)
+                c.y = v;
+            else if (l == 2)
+                c.z = v;
+            else
+                c.w = v;
+        }
+
+        __global__ void k_level_masks(int npad, int lv, SellView<double> A, const int *__restrict__ row_len,
+                                      const int *__restrict__ block_of_slot, const int4 *__restrict__ cmap,
+                                      uint32_t *nbr)
+        {
+            const int p = blockIdx.x * blockDim.x + threadIdx.x;
+            if (p >= npad || block_of_slot[p] < 0)
+                return;
+            const int c = cmap_get(cmap[p], lv - 1);
+            uint32_t m = 1u << (c & 31);
+            const int slice = p / WAVE, lane = p % WAVE;
+            const int beg = A.slice_ptr[slice];
+            const int len = row_len[p];
+            for (int k = 0; k < len; ++k)
+            {
+                const int q = A.cols[static_cast<size_t>(beg + k) * WAVE + lane];
+                const int cj = cmap_get(cmap[q], lv - 1);
+                if ((cj >> 5) == (c >> 5))
+                    m |= 1u << (cj & 31);
+            }
+            atomicOr(nbr + c, m);
+        }
+
+        __global__ __launch_bounds__(256) void k_cc_coarse(int count, const uint32_t *__restrict__ nbr, uint32_t *lead,
+                                                           int *local_id, int *bank_count)
+        {
+            const int q = blockIdx.x * 256 + threadIdx.x;
+            const int lane32 = threadIdx.x & 31;
+            const bool valid = q < count;
+            const uint32_t m = valid ? nbr[q] : 0u;
+            const uint32_t conn = valid ? closure(m, lane32, nbr + (q & ~31)) : 0u;
+            uint32_t lm;
+            int lid, cnt;
+            leaders_and_ids(valid, conn, lane32, lm, lid, cnt);
+            if (valid)
+            {
+                lead[q] = lm;
+                local_id[q] = lid;
+                if (lane32 == 0)
+                    bank_count[q / 32] = cnt;
+            }
+        }
+
+        __global__ __launch_bounds__(1024) void k_scan(int n, const int *__restrict__ in, int *__restrict__ out)
+        {
+            __shared__ int sums[1024];
+            const int per = div_up(n, 1024);
+            const int b = threadIdx.x * per, e = min(n, b + per);
+            int s = 0;
+            for (int i = b; i < e; ++i)
+                s += in[i];
+            sums[threadIdx.x] = s;
+            __syncthreads();
+            // Hillis-Steele inclusive scan of the 1024 chunk sums.
+            for (int off = 1; off < 1024; off <<= 1)
+            {
+                const int v = threadIdx.x >= off ? sums[threadIdx.x - off] : 0;
+                __syncthreads();
+                sums[threadIdx.x] += v;
+                __syncthreads();
+            }
+            int run = threadIdx.x > 0 ? sums[threadIdx.x - 1] : 0;
+            for (int i = b; i < e; ++i)
+            {
+                out[i] = run;
+                run += in[i];
+            }
+            if (threadIdx.x == 1023)
+                out[n] = sums[1023];
+        }
+
+        __global__ void k_make_gid(int count, const int *__restrict__ local_id, const int *__restrict__ prefix,
+                                   int *__restrict__ gid)
+        {
+            const int q = blockIdx.x * blockDim.x + threadIdx.x;
+            if (q < count)
+                gid[q] = local_id[q] + prefix[q / 32];
+        }
+
+        __global__ void k_fill_cmap(int npad, int lv, const int *__restrict__ block_of_slot, const int *__restrict__ gid,
+                                    int4 *cmap)
+        {
+            const int p = blockIdx.x * blockDim.x + threadIdx.x;
+            if (p >= npad)
+                return;
+            int4 c = cmap[p];
+            if (block_of_slot[p] < 0)
+                cmap_set(c, lv, -1);
+            else
+                cmap_set(c, lv, lv == 0 ? gid[p] : gid[cmap_get(c, lv - 1)]);
+            cmap[p] = c;
+        }
+
+        // --------------------------------------------------------------------- build: Galerkin
+        __device__ __forceinline__ int upper_index(int dim, int i, int j)
+        {
+            if (i > j)
+            {
+                const int t = i;
+                i = j;
+                j = t;
+            }
+            return i * dim - (i * (i + 1)) / 2 + j;
+        }
+
+        /// One thread per fine row i; blocks (i, j) with i <= j only (A symmetric), as the reference
+        /// fill (MASPreconditioner.cu:494-591).  Fine-level entries have exactly one writer; coarse
+        /// entries are summed with FP64 atomics.
+        __global__ void k_galerkin(int npad, int bd, int level_num, const int *__restrict__ mat_off,
+                                   SellView<double> A, const int *__restrict__ row_len,
+                                   const int *__restrict__ block_of_slot, const int4 *__restrict__ cmap,
+                                   double *__restrict__ mats)
+        {
+            const int i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= npad || block_of_slot[i] < 0)
+                return;
+            const int dim = 32 * bd;
+            const size_t st = static_cast<size_t>(dim) * (dim + 1) / 2;
+            const int bs = bd * bd;
+            const int slice = i / WAVE, lane = i % WAVE;
+            const int beg = A.slice_ptr[slice];
+            const int len = row_len[i];
+            const int4 ci4 = cmap[i];
+            for (int k = 0; k < len; ++k)
+            {
+                const size_t slot = static_cast<size_t>(beg + k);
+                const int j = A.cols[slot * WAVE + lane];
+                if (i > j)
+                    continue;
+                double v[9];
+                for (int e = 0; e < bs; ++e)
+                    v[e] = A.vals[(slot * bs + e) * WAVE + lane];
+                if ((i >> 5) == (j >> 5))
+                {
+                    double *mat = mats + static_cast<size_t>(i >> 5) * st;
+                    const int ri = (i & 31) * bd, rj = (j & 31) * bd;
+                    for (int bi = 0; bi < bd; ++bi)
+                        for (int bj = 0; bj < bd; ++bj)
+                        {
+                            const int row = ri + bi, col = rj + bj;
+                            if (row > col || (i == j && bi > bj))
+                                continue;
+                            mat[upper_index(dim, row, col)] = v[bi * bd + bj];
+                        }
+                }
+                const int4 cj4 = cmap[j];
+                for (int lv = 0; lv < level_num; ++lv)
+                {
+                    const int ci = cmap_get(ci4, lv), cj = cmap_get(cj4, lv);
+                    if ((ci >> 5) != (cj >> 5))
+                        continue;
+                    double *mat = mats + static_cast<size_t>(mat_off[lv + 1] + (ci >> 5)) * st;
+                    const int ri = (ci & 31) * bd, rj = (cj & 31) * bd;
+                    for (int bi = 0; bi < bd; ++bi)
+                        for (int bj = 0; bj < bd; ++bj)
+                        {
+                            const int row = ri + bi, col = rj + bj;
+                            if (row > col || (i == j && bi > bj))
+                                continue;
+                            double val = v[bi * bd + bj];
+                            if (i != j && ci == cj)
+                                val += v[bj * bd + bi];
+                            atomicAdd(mat + upper_index(dim, row, col), val);
+                        }
+                }
+            }
+        }
+
+        // ---------------------------------------------------------------------- build: inverse
+        /// Gauss-Jordan inverse (no pivoting; the matrices are SPD) of one packed matrix per
+        /// workgroup on the full matrix in LDS.  A zero diagonal entry is first replaced by 1
+        /// (virtual slots become identity rows, MASPreconditioner.cu:779-796).  Flags a zero or
+        /// non-finite pivot, a non-finite entry or a non-positive diagonal of the inverse.
+        template <int BD>
+        __global__ __launch_bounds__(256) void k_invert(double *__restrict__ mats, int *fail)
+        {
+            constexpr int N = 32 * BD;
+            constexpr int L = N * (N + 1) / 2;
+            constexpr int LD = N + 1; // padded leading dimension against bank conflicts
+            __shared__ double sA[N * LD];
+            __shared__ double rowk[N];
+            __shared__ double colk[N];
+            __shared__ double pinv;
+            __shared__ int bad;
+            double *g = mats + static_cast<size_t>(blockIdx.x) * L;
+            if (threadIdx.x == 0)
+                bad = 0;
+            for (int i = threadIdx.x; i < N; i += 256)
+            {
+                const int rs = i * N - (i * (i + 1)) / 2;
+                for (int j = i; j < N; ++j)
+                {
+                    double v = g[rs + j];
+                    if (i == j && v == 0.0)
+                        v = 1.0;
+                    sA[i * LD + j] = v;
+                    sA[j * LD + i] = v;
+                }
+            }
+            __syncthreads();
+            for (int k = 0; k < N; ++k)
+            {
+                if (threadIdx.x == 0)
+                {
+                    const double piv = sA[k * LD + k];
+                    if (!isfinite(piv) || piv == 0.0)
+                        bad = 1;
+                    pinv = 1.0 / piv;
+                }
+                __syncthreads();
+                const double pi = pinv;
+                for (int i = threadIdx.x; i < N; i += 256)
+                {
+                    colk[i] = sA[i * LD + k];
+                    rowk[i] = sA[k * LD + i] * pi;
+                }
+                __syncthreads();
+                for (int e = threadIdx.x; e < N * N; e += 256)
+                {
+                    const int i = e / N, j = e - (e / N) * N;
+                    double v;
+                    if (i == k)
+                        v = (j == k) ? pi : rowk[j];
+                    else if (j == k)
+                        v = -colk[i] * pi;
+                    else
+                        v = fma(-colk[i], rowk[j], sA[i * LD + j]);
+                    sA[i * LD + j] = v;
+                }
+                __syncthreads();
+            }
+            // Symmetrise, check and write back packed.
+            for (int i = threadIdx.x; i < N; i += 256)
+            {
+                const int rs = i * N - (i * (i + 1)) / 2;
+                if (!(sA[i * LD + i] > 0.0))
+                    bad = 1;
+                for (int j = i; j < N; ++j)
+                {
+                    const double v = 0.5 * (sA[i * LD + j] + sA[j * LD + i]);
+                    if (!isfinite(v))
+                        bad = 1;
+                    g[rs + j] = v;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0 && bad)
+                atomicExch(fail, 1);
+        }
+
+        /// FP16 form of packed inverses: Inv = D S D with D = diag(sqrt(Inv_ii)), so S has a unit
+        /// diagonal and |S_ij| <= 1; S is stored packed in FP16 and D in FP32.
+        template <int BD>
+        __global__ __launch_bounds__(256) void k_make_half(const double *__restrict__ mats, __half *__restrict__ out,
+                                                           float *__restrict__ dscale)
+        {
+            constexpr int N = 32 * BD;
+            constexpr int L = N * (N + 1) / 2;
+            __shared__ double d[N];
+            const double *g = mats + static_cast<size_t>(blockIdx.x) * L;
+            for (int i = threadIdx.x; i < N; i += 256)
+            {
+                const double v = sqrt(fmax(g[i * N - (i * (i + 1)) / 2 + i], 0.0));
+                d[i] = v;
+                dscale[static_cast<size_t>(blockIdx.x) * N + i] = static_cast<float>(v);
+            }
+            __syncthreads();
+            __half *o = out + static_cast<size_t>(blockIdx.x) * L;
+            for (int e = threadIdx.x; e < L; e += 256)
+            {
+                // Row of packed index e: largest i with i*N - i(i-1)/2 <= e (row starts decrease).
+                int i = 0;
+                int rs = 0;
+                while (i + 1 < N && rs + (N - i) <= e)
+                {
+                    rs += N - i;
+                    ++i;
+                }
+                const int j = i + (e - rs);
+                const double den = d[i] * d[j];
+                o[e] = __float2half_rn(static_cast<float>(den > 0.0 ? g[e] / den : 0.0));
+            }
+        }
+
+        // ---------------------------------------------------------------------- build: Morton
+        __device__ __forceinline__ uint64_t spread3(uint64_t v)
+        {
+            v &= 0x1fffffull;
+            v = (v | (v << 32)) & 0x1f00000000ffffull;
+            v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+            v = (v | (v << 8)) & 0x100f00f00f00f00full;
+            v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+            v = (v | (v << 2)) & 0x1249249249249249ull;
+            return v;
+        }
+
+        __global__ void k_morton_keys(int nb, int dim, const double *__restrict__ xyz, const double *__restrict__ lo,
+                                      const double *__restrict__ inv_extent, uint64_t *keys, int *ids)
+        {
+            const int i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= nb)
+                return;
+            uint64_t key = 0;
+            for (int d = 0; d < dim; ++d)
+            {
+                double u = (xyz[static_cast<size_t>(i) * dim + d] - lo[d]) * inv_extent[d];
+                u = u < 0.0 ? 0.0 : (u > 1.0 ? 1.0 : u);
+                const uint64_t q = static_cast<uint64_t>(u * 2097151.0);
+                key |= spread3(q) << d;
+            }
+            keys[i] = key;
+            ids[i] = i;
+        }
+    } // namespace
+
+    template <int BD, typename T>
+    void launch_restrict_coarse(const CoarseRestrict &cr, T *mr, hipStream_t s)
+    {
+        if (cr.level_num <= 1)
+            return;
+        hipLaunchKernelGGL((k_restrict_coarse<BD, T>), dim3(1), dim3(1024), 0, s, cr, mr);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD, typename TM, typename T>
+    void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
+                          hipStream_t s)
+    {
+        if (count <= 0)
+            return;
+        hipLaunchKernelGGL((k_mas_apply<BD, TM, T>), dim3(count), dim3(apply_block<BD>()), 0, s, mat_begin, mats,
+                           dscale, mr, mz);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD, typename T>
+    void launch_prolong_dot(int npad, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
+                            double *sc, bool init, ReduceSite site, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_prolong_dot<BD, T>), dim3(div_up(npad, 256)), dim3(256), 0, s, npad, level_num, cmap,
+                           lev_off, r, mz, sc, init, site);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_cc_level0(int npad, const SellView<double> &A, const int *row_len, const int *block_of_slot,
+                          uint32_t *lead, int *local_id, int *bank_count, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_cc_level0, dim3(div_up(npad, 256)), dim3(256), 0, s, npad, A, row_len, block_of_slot,
+                           lead, local_id, bank_count);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_level_masks(int npad, int lv, const SellView<double> &A, const int *row_len, const int *block_of_slot,
+                            const int4 *cmap, uint32_t *nbr, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_level_masks, dim3(div_up(npad, 256)), dim3(256), 0, s, npad, lv, A, row_len,
+                           block_of_slot, cmap, nbr);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_cc_coarse(int count, const uint32_t *nbr, uint32_t *lead, int *local_id, int *bank_count,
+                          hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_cc_coarse, dim3(div_up(count, 256)), dim3(256), 0, s, count, nbr, lead, local_id,
+                           bank_count);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_scan(int n, const int *in, int *out, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, n, in, out);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_make_gid(int count, const int *local_id, const int *prefix, int *gid, hipStream_t s)
+    {
+        if (count <= 0)
+            return;
+        hipLaunchKernelGGL(k_make_gid, dim3(div_up(count, 256)), dim3(256), 0, s, count, local_id, prefix, gid);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_fill_cmap(int npad, int lv, const int *block_of_slot, const int *gid, int4 *cmap, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_fill_cmap, dim3(div_up(npad, 256)), dim3(256), 0, s, npad, lv, block_of_slot, gid, cmap);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_galerkin(int npad, int bd, int level_num, const int *mat_off, const SellView<double> &A,
+                         const int *row_len, const int *block_of_slot, const int4 *cmap, double *mats, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_galerkin, dim3(div_up(npad, 128)), dim3(128), 0, s, npad, bd, level_num, mat_off, A,
+                           row_len, block_of_slot, cmap, mats);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_invert(int bd, int count, double *mats, int *fail, hipStream_t s)
+    {
+        if (count <= 0)
+            return;
+        if (bd == 1)
+            hipLaunchKernelGGL(k_invert<1>, dim3(count), dim3(256), 0, s, mats, fail);
+        else if (bd == 2)
+            hipLaunchKernelGGL(k_invert<2>, dim3(count), dim3(256), 0, s, mats, fail);
+        else
+            hipLaunchKernelGGL(k_invert<3>, dim3(count), dim3(256), 0, s, mats, fail);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_make_half(int bd, int count, const double *mats, __half *out, float *dscale, hipStream_t s)
+    {
+        if (count <= 0)
+            return;
+        if (bd == 1)
+            hipLaunchKernelGGL(k_make_half<1>, dim3(count), dim3(256), 0, s, mats, out, dscale);
+        else if (bd == 2)
+            hipLaunchKernelGGL(k_make_half<2>, dim3(count), dim3(256), 0, s, mats, out, dscale);
+        else
+            hipLaunchKernelGGL(k_make_half<3>, dim3(count), dim3(256), 0, s, mats, out, dscale);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_morton_keys(int nb, int dim, const double *xyz, const double *lo, const double *inv_extent,
+                            uint64_t *keys, int *ids, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_morton_keys, dim3(div_up(nb, 256)), dim3(256), 0, s, nb, dim, xyz, lo, inv_extent, keys,
+                           ids);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    size_t morton_sort_temp_bytes(int nb)
+    {
+        size_t bytes = 0;
+        PSMAS_HIP(rocprim::radix_sort_pairs(nullptr, bytes, static_cast<const uint64_t *>(nullptr),
+                                            static_cast<uint64_t *>(nullptr), static_cast<const int *>(nullptr),
+                                            static_cast<int *>(nullptr), static_cast<unsigned>(nb), 0u, 63u));
+        return bytes;
+    }
+
+    void morton_sort(int nb, void *temp, size_t temp_bytes, const uint64_t *keys_in, uint64_t *keys_out,
+                     const int *ids_in, int *ids_out, hipStream_t s)
+    {
+        PSMAS_HIP(rocprim::radix_sort_pairs(temp, temp_bytes, keys_in, keys_out, ids_in, ids_out,
+                                            static_cast<unsigned>(nb), 0u, 63u, s));
+    }
+
+#define PSMAS_INST_APPLY(BD)                                                                                     \
+    template void launch_restrict_coarse<BD, double>(const CoarseRestrict &, double *, hipStream_t);             \
+    template void launch_restrict_coarse<BD, float>(const CoarseRestrict &, float *, hipStream_t);               \
+    template void launch_mas_apply<BD, double, double>(int, int, const double *, const float *, const double *,   \
+                                                       double *, hipStream_t);                                   \
+    template void launch_mas_apply<BD, double, float>(int, int, const double *, const float *, const float *,     \
+                                                      float *, hipStream_t);                                     \
+    template void launch_mas_apply<BD, __half, float>(int, int, const __half *, const float *, const float *,     \
+                                                      float *, hipStream_t);                                     \
+    template void launch_prolong_dot<BD, double>(int, int, const int4 *, const int *, const double *, double *,   \
+                                                 double *, bool, ReduceSite, hipStream_t);                       \
+    template void launch_prolong_dot<BD, float>(int, int, const int4 *, const int *, const float *, float *,      \
+                                                double *, bool, ReduceSite, hipStream_t);
+
+    PSMAS_INST_APPLY(1)
+    PSMAS_INST_APPLY(2)
+    PSMAS_INST_APPLY(3)
+#undef PSMAS_INST_APPLY
+} // namespace psmas

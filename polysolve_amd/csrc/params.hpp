@@ -1,0 +1,75 @@
+// Solver parameters and their JSON reader.
+//
+// Keys, types and meaning follow the reference's MAS settings (linear-solver-spec.json:456-509,
+// read by MASSolverImpl::set_parameters at src/polysolve/linear/MASSolver.cu:198-212, which only
+// looks at params["MAS"] when called through MASSolver::set_parameters, MASSolver.cu:605-614).
+// Defaults are the reference's in-code defaults (MASSolver.cu:137-143), which are what a
+// MASSolver created by name and configured only through set_parameters uses.
+//
+// Keys this implementation adds (none collides with the reference spec):
+//   precision            "fp64" (reference algorithm) | "mixed" (FP32 inner PCG, FP16 bank
+//                        inverses, FP64 reliable updates)
+//   aggregation          "auto" | "morton" | "graph"; auto = morton when coordinates were given
+//   coarse_solver        "bank" (reference: every level is bank-block-diagonal) | "cholesky"
+//                        (coarsest level solved exactly by a dense FP64 Cholesky)
+//   reliable_update_delta   residual reduction that triggers an FP64 reliable update (mixed)
+//   check_interval       iterations between host convergence checks (reference: 10)
+//   true_residual_period reference: 4 (fp64 path only)
+//   use_graphs           capture the iteration sequence in HIP graphs
+//   learned_correction   use the trained coarse-correction network when one is loaded
+//   division_guard       alpha/beta are set to 0 when |denominator| < division_guard; 0 (default)
+//                        sets them to 0 only for a non-finite quotient; 1e-20 is the reference's
+//                        absolute rule (MASSolver.cu:43-56)
+#pragma once
+
+#include <string>
+
+namespace psmas
+{
+    enum class Precision
+    {
+        FP64,
+        Mixed,
+    };
+
+    enum class Aggregation
+    {
+        Auto,
+        Morton,
+        Graph,
+    };
+
+    enum class CoarseSolver
+    {
+        Bank,
+        Cholesky,
+    };
+
+    struct Params
+    {
+        int block_dim = 1;
+        int max_iter = 5000;
+        double rel_tol = 1e-5;
+        double abs_tol = 1e-8;
+        bool lazy_partitioning = false;
+        bool use_preconditioned_residual_norm = false;
+        int true_residual_period = 4;
+
+        Precision precision = Precision::FP64;
+        Aggregation aggregation = Aggregation::Auto;
+        CoarseSolver coarse_solver = CoarseSolver::Bank;
+        double reliable_update_delta = 0.1;
+        int check_interval = 10;
+        bool use_graphs = true;
+        bool learned_correction = false;
+        double division_guard = 0.0;
+    };
+
+    /// Apply a JSON object to params.  Accepts either {"MAS": {...}} (only the "MAS" member is
+    /// read, as MASSolver::set_parameters does) or the inner object itself.  Unknown keys are
+    /// ignored.  Throws InvalidArgumentError on malformed JSON or a bad value.
+    void apply_json_parameters(Params &p, const std::string &json);
+
+    /// Canonical JSON dump of all parameters.
+    std::string params_to_json(const Params &p);
+} // namespace psmas

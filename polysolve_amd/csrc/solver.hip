@@ -1,0 +1,1051 @@
+// Host orchestration of the MI355X MAS-preconditioned PCG solver.
+//
+// Follows the reference's MASSolverImpl (src/polysolve/linear/MASSolver.cu:134-596):
+//   factorize  partition (lazy reuse) -> permuted BSR -> MAS build -> PCG buffers  (:292-374)
+//   solve      b -> permuted space; x0 = 0; PCG; un-permute                         (:376-409)
+//   pcg_solve  x0 = 0, true residual every 4th iteration, convergence every 10th     (:469-595)
+// The working space is the bank-padded node space (domain p owns slots [32p, 32p+32)), so the
+// fine level of the preconditioner needs no gather at all, and every device scalar stays on the
+// device between the host convergence checks.  Iterations between checks are captured once into
+// HIP graphs and replayed.
+//
+// The mixed-precision path ("precision": "mixed") runs an FP32 inner PCG (FP32 SELL matrix,
+// FP16 fine-level bank inverses with FP32 scaling, FP64 coarse-level inverses) with reliable
+// updates: at a check point where the recurrence residual fell by reliable_update_delta (or
+// below the target), the FP32 correction is added to the FP64 solution and the FP64 residual
+// b - A x is recomputed with the FP64 matrix; convergence is decided on that FP64 residual only.
+#include "solver.hpp"
+#include "device.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <sstream>
+#include <thread>
+
+namespace psmas
+{
+    // ------------------------------------------------------------------------------ errors
+    void throw_hip_error(hipError_t err, const char *what, const char *file, int line)
+    {
+        std::ostringstream o;
+        o << "[MAS] HIP error " << hipGetErrorName(err) << " (" << hipGetErrorString(err) << ") in " << what
+          << " at " << file << ":" << line;
+        if (err == hipErrorOutOfMemory)
+            throw OutOfMemoryError(o.str());
+        throw std::runtime_error(o.str());
+    }
+
+    void *device_alloc(size_t bytes, const char *context)
+    {
+        void *p = nullptr;
+        const hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess)
+            return p;
+        (void)hipGetLastError();
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        constexpr size_t mb = 1024ull * 1024ull;
+        throw OutOfMemoryError(std::string("[MAS] Failed to allocate ") + std::to_string((bytes + mb - 1) / mb) +
+                               " MB for " + (context ? context : "device allocation") + "; only " +
+                               std::to_string(free_b / mb) + " MB free.");
+    }
+
+    void device_free(void *p) noexcept
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+
+    const char *status_string(Status s)
+    {
+        switch (s)
+        {
+        case Status::Running:
+            return "Running";
+        case Status::ReachRelativeTolerance:
+            return "Reach relative tolerance";
+        case Status::ReachAbsoluteTolerance:
+            return "Reach absolute tolerance";
+        case Status::ReachMaxIterations:
+            return "Reach max iterations";
+        }
+        return "Unknown";
+    }
+
+    // ------------------------------------------------------------------------------ graphs
+    struct Solver::Graphs
+    {
+        std::map<int, hipGraphExec_t> execs;
+        ~Graphs() { clear(); }
+        void clear()
+        {
+            for (auto &kv : execs)
+                (void)hipGraphExecDestroy(kv.second);
+            execs.clear();
+        }
+    };
+
+    // ------------------------------------------------------------------------ construction
+    Solver::Solver() : graphs_(new Graphs())
+    {
+        int count = 0;
+        const hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count == 0)
+        {
+            (void)hipGetLastError();
+            throw NoDeviceError("[MAS] No AMD GPU!!");
+        }
+        device_ = 0;
+        PSMAS_HIP(hipSetDevice(device_));
+        PSMAS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        d_sc_.allocate(S_COUNT, "MAS device scalars");
+        d_sc_.zero(stream_);
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+    }
+
+    Solver::~Solver()
+    {
+        (void)hipSetDevice(device_);
+        if (stream_)
+            (void)hipStreamSynchronize(stream_);
+        graphs_.reset();
+        for (hipEvent_t ev : event_pool_)
+            (void)hipEventDestroy(ev);
+        if (stream_)
+            (void)hipStreamDestroy(stream_);
+    }
+
+    void Solver::set_parameters_json(const std::string &json)
+    {
+        const Params before = params_;
+        apply_json_parameters(params_, json);
+        if (before.use_graphs != params_.use_graphs || before.check_interval != params_.check_interval ||
+            before.true_residual_period != params_.true_residual_period)
+            graphs_->clear();
+    }
+
+    std::string Solver::get_info_json() const
+    {
+        std::ostringstream o;
+        o.precision(17);
+        o << "{\"solver_iter\": " << iterations_ << ", \"solver_error\": " << residual_norm_
+          << ", \"solver_status\": \"" << status_string(status_) << "\""
+          << ", \"precision\": \"" << (params_.precision == Precision::FP64 ? "fp64" : "mixed") << "\""
+          << ", \"reliable_updates\": " << reliable_updates_
+          << ", \"true_relative_residual\": " << true_rel_residual_ << ", \"solve_ms\": " << solve_ms_
+          << ", \"levels\": " << level_num_ << ", \"parts\": " << nparts_ << ", \"aggregation\": \""
+          << aggregation_used_ << "\"}";
+        return o.str();
+    }
+
+    void Solver::set_coordinates(int nb, int dim, const double *xyz)
+    {
+        if (nb <= 0 || dim < 1 || dim > 3 || xyz == nullptr)
+            throw InvalidArgumentError("[MAS] invalid coordinates: need nb > 0 nodes with 1 to 3 components.");
+        coords_.assign(xyz, xyz + static_cast<size_t>(nb) * dim);
+        coord_dim_ = dim;
+        have_partition_ = false;
+    }
+
+    ReduceSite Solver::site(int s) const
+    {
+        ReduceSite r;
+        r.partials = d_partials_.data() + static_cast<size_t>(s) * partial_cap_;
+        r.counter = d_counters_.data() + s;
+        return r;
+    }
+
+    FineRestrict Solver::fine_restrict() const
+    {
+        FineRestrict f;
+        f.lead = d_lead0_.data();
+        f.gid = d_gid0_.data();
+        return f;
+    }
+
+    CoarseRestrict Solver::coarse_restrict() const
+    {
+        CoarseRestrict c;
+        c.level_num = level_num_;
+        for (int l = 0; l < MAX_COARSE_LEVELS; ++l)
+            c.counts[l] = cco_[l];
+        for (int l = 0; l <= MAX_COARSE_LEVELS; ++l)
+            c.offsets[l] = lev_off_[l];
+        for (int l = 1; l < MAX_COARSE_LEVELS; ++l)
+        {
+            c.lead[l] = d_leadc_[l].data();
+            c.gid[l] = d_gidc_[l].data();
+        }
+        return c;
+    }
+
+    template <typename TV>
+    SellView<TV> Solver::sell(const TV *vals) const
+    {
+        SellView<TV> v;
+        v.nslices = nslices_;
+        v.slice_ptr = d_slice_ptr_.data();
+        v.cols = d_cols_.data();
+        v.vals = vals;
+        return v;
+    }
+
+    double Solver::read_scalar(int slot)
+    {
+        double v = 0.0;
+        PSMAS_HIP(hipMemcpyAsync(&v, d_sc_.data() + slot, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        return v;
+    }
+
+    void Solver::read_scalars(double *out)
+    {
+        PSMAS_HIP(hipMemcpyAsync(out, d_sc_.data(), sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+    }
+
+    // --------------------------------------------------------------------------- factorize
+    void Solver::factorize(const CscView &A)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        validate_csc(A);
+        const int bd = params_.block_dim;
+        const int nb = div_up(A.n, bd);
+        factorized_ = false;
+        graphs_->clear();
+
+        const bool rebuild = !params_.lazy_partitioning || !have_partition_ ||
+                             static_cast<int>(part_.perm.size()) != nb || bd != bd_;
+        n_ = A.n;
+        bd_ = bd;
+        nb_ = nb;
+        if (rebuild)
+            build_partition(A);
+
+        PaddedBsr bsr;
+        build_padded_bsr(A, bd_, part_, bsr);
+        npad_ = bsr.npad;
+        nparts_ = bsr.nparts;
+        slot_of_block_ = bsr.slot_of_block;
+        block_of_slot_ = bsr.block_of_slot;
+        nnzb_ = static_cast<long long>(bsr.cols.size());
+        upload_matrix(bsr);
+        bsr = PaddedBsr();
+        d_vals32_.reset();
+        d_mats16_.reset();
+        d_dscale_.reset();
+        build_mas();
+        allocate_vectors();
+        if (params_.precision == Precision::Mixed)
+            ensure_mixed_data();
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        factorized_ = true;
+    }
+
+    void Solver::build_partition(const CscView &A)
+    {
+        Aggregation agg = params_.aggregation;
+        const bool have_coords = coord_dim_ > 0 && static_cast<int>(coords_.size()) == nb_ * coord_dim_;
+        if (agg == Aggregation::Auto)
+            agg = have_coords ? Aggregation::Morton : Aggregation::Graph;
+        if (agg == Aggregation::Morton && !have_coords)
+            throw InvalidArgumentError(
+                "[MAS] Morton aggregation needs one coordinate tuple per block node (psmas_set_coordinates).");
+        if (agg == Aggregation::Morton)
+        {
+            const int dim = coord_dim_;
+            double lo[3] = {0, 0, 0}, inv[3] = {0, 0, 0};
+            for (int d = 0; d < dim; ++d)
+            {
+                double mn = coords_[d], mx = coords_[d];
+                for (int i = 0; i < nb_; ++i)
+                {
+                    mn = std::min(mn, coords_[static_cast<size_t>(i) * dim + d]);
+                    mx = std::max(mx, coords_[static_cast<size_t>(i) * dim + d]);
+                }
+                lo[d] = mn;
+                inv[d] = (mx > mn) ? 1.0 / (mx - mn) : 0.0;
+            }
+            DeviceBuffer<double> d_xyz(coords_.size(), "MAS morton coordinates");
+            DeviceBuffer<double> d_lo(3, "MAS morton box"), d_inv(3, "MAS morton box");
+            DeviceBuffer<uint64_t> keys(nb_, "MAS morton keys"), keys_out(nb_, "MAS morton keys");
+            DeviceBuffer<int> ids(nb_, "MAS morton ids"), ids_out(nb_, "MAS morton ids");
+            PSMAS_HIP(hipMemcpyAsync(d_xyz.data(), coords_.data(), d_xyz.bytes(), hipMemcpyHostToDevice, stream_));
+            PSMAS_HIP(hipMemcpyAsync(d_lo.data(), lo, sizeof(lo), hipMemcpyHostToDevice, stream_));
+            PSMAS_HIP(hipMemcpyAsync(d_inv.data(), inv, sizeof(inv), hipMemcpyHostToDevice, stream_));
+            launch_morton_keys(nb_, dim, d_xyz.data(), d_lo.data(), d_inv.data(), keys.data(), ids.data(), stream_);
+            const size_t tb = morton_sort_temp_bytes(nb_);
+            DeviceBuffer<unsigned char> temp(std::max<size_t>(tb, 1), "MAS morton sort");
+            morton_sort(nb_, temp.data(), tb, keys.data(), keys_out.data(), ids.data(), ids_out.data(), stream_);
+            std::vector<int> order(nb_);
+            PSMAS_HIP(hipMemcpyAsync(order.data(), ids_out.data(), sizeof(int) * nb_, hipMemcpyDeviceToHost, stream_));
+            PSMAS_HIP(hipStreamSynchronize(stream_));
+            part_ = chunk_partition(order, BANK);
+            aggregation_used_ = "morton";
+        }
+        else
+        {
+            const BlockGraph g = build_block_graph(A, bd_);
+            part_ = graph_aggregate(g, BANK);
+            aggregation_used_ = "graph";
+        }
+        have_partition_ = true;
+    }
+
+    void Solver::upload_matrix(const PaddedBsr &bsr)
+    {
+        const int bs = bd_ * bd_;
+        nslices_ = npad_ / WAVE;
+        std::vector<int> slice_ptr(nslices_ + 1, 0);
+        std::vector<int> row_len(npad_, 0);
+        for (int r = 0; r < npad_; ++r)
+            row_len[r] = bsr.rows[r + 1] - bsr.rows[r];
+        for (int s = 0; s < nslices_; ++s)
+        {
+            int mx = 0;
+            for (int l = 0; l < WAVE; ++l)
+                mx = std::max(mx, row_len[s * WAVE + l]);
+            slice_ptr[s + 1] = slice_ptr[s] + mx;
+        }
+        nslots_ = slice_ptr[nslices_];
+        std::vector<int> cols(static_cast<size_t>(nslots_) * WAVE);
+        std::vector<double> vals(static_cast<size_t>(nslots_) * WAVE * bs);
+        auto fill = [&](int s0, int s1) {
+            for (int s = s0; s < s1; ++s)
+            {
+                const int len = slice_ptr[s + 1] - slice_ptr[s];
+                for (int l = 0; l < WAVE; ++l)
+                {
+                    const int row = s * WAVE + l;
+                    const int rl = row_len[row];
+                    for (int k = 0; k < len; ++k)
+                    {
+                        const size_t slot = static_cast<size_t>(slice_ptr[s] + k);
+                        if (k < rl)
+                        {
+                            const size_t src = static_cast<size_t>(bsr.rows[row] + k);
+                            cols[slot * WAVE + l] = bsr.cols[src];
+                            for (int e = 0; e < bs; ++e)
+                                vals[(slot * bs + e) * WAVE + l] = bsr.vals[src * bs + e];
+                        }
+                        else
+                        {
+                            cols[slot * WAVE + l] = row;
+                            for (int e = 0; e < bs; ++e)
+                                vals[(slot * bs + e) * WAVE + l] = 0.0;
+                        }
+                    }
+                }
+            }
+        };
+        {
+            const int nt = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+            std::vector<std::thread> th;
+            const int chunk = div_up(nslices_, nt);
+            for (int t = 0; t < nt; ++t)
+            {
+                const int b = t * chunk, e = std::min(nslices_, b + chunk);
+                if (b < e)
+                    th.emplace_back(fill, b, e);
+            }
+            for (auto &x : th)
+                x.join();
+        }
+        d_slice_ptr_.allocate(slice_ptr.size(), "MAS SELL slice pointers");
+        d_cols_.allocate(cols.size(), "MAS SELL columns");
+        d_vals64_.allocate(vals.size(), "MAS SELL values (FP64)");
+        d_row_len_.allocate(row_len.size(), "MAS SELL row lengths");
+        d_block_of_slot_.allocate(block_of_slot_.size(), "MAS slot map");
+        d_slot_of_block_.allocate(slot_of_block_.size(), "MAS block map");
+        PSMAS_HIP(hipMemcpy(d_slice_ptr_.data(), slice_ptr.data(), d_slice_ptr_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_cols_.data(), cols.data(), d_cols_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_vals64_.data(), vals.data(), d_vals64_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_row_len_.data(), row_len.data(), d_row_len_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_block_of_slot_.data(), block_of_slot_.data(), d_block_of_slot_.bytes(),
+                            hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_slot_of_block_.data(), slot_of_block_.data(), d_slot_of_block_.bytes(),
+                            hipMemcpyHostToDevice));
+    }
+
+    void Solver::build_mas()
+    {
+        const SellView<double> A = sell(d_vals64_.data());
+        const int banks = npad_ / BANK;
+        d_lead0_.allocate(npad_, "MAS level-0 leader masks");
+        d_gid0_.allocate(npad_, "MAS level-0 ids");
+        DeviceBuffer<int> local(npad_, "MAS local component ids");
+        DeviceBuffer<int> bank_count(banks, "MAS component counts");
+        DeviceBuffer<int> prefix(banks + 1, "MAS component prefix");
+        launch_cc_level0(npad_, A, d_row_len_.data(), d_block_of_slot_.data(), d_lead0_.data(), local.data(),
+                         bank_count.data(), stream_);
+        launch_scan(banks, bank_count.data(), prefix.data(), stream_);
+        int total = 0;
+        PSMAS_HIP(hipMemcpyAsync(&total, prefix.data() + banks, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        for (int l = 0; l < MAX_COARSE_LEVELS; ++l)
+        {
+            cco_[l] = 0;
+            d_leadc_[l].reset();
+            d_gidc_[l].reset();
+        }
+        cco_[0] = total;
+        launch_make_gid(npad_, local.data(), prefix.data(), d_gid0_.data(), stream_);
+        d_cmap_.allocate(npad_, "MAS coarse map");
+        launch_fill<int>(static_cast<long long>(npad_) * 4, reinterpret_cast<int *>(d_cmap_.data()), -1, stream_);
+        launch_fill_cmap(npad_, 0, d_block_of_slot_.data(), d_gid0_.data(), d_cmap_.data(), stream_);
+        level_num_ = 1;
+
+        for (int lv = 1; lv < MAX_COARSE_LEVELS; ++lv)
+        {
+            const int count = cco_[lv - 1];
+            const int nbank = div_up(count, BANK);
+            DeviceBuffer<uint32_t> nbr(count, "MAS coarse neighbour masks");
+            launch_fill<unsigned>(count, nbr.data(), 0u, stream_);
+            launch_level_masks(npad_, lv, A, d_row_len_.data(), d_block_of_slot_.data(), d_cmap_.data(), nbr.data(),
+                               stream_);
+            DeviceBuffer<uint32_t> lead(count, "MAS coarse leader masks");
+            DeviceBuffer<int> loc(count, "MAS coarse local ids");
+            DeviceBuffer<int> bc(nbank, "MAS coarse counts");
+            DeviceBuffer<int> pre(nbank + 1, "MAS coarse prefix");
+            launch_cc_coarse(count, nbr.data(), lead.data(), loc.data(), bc.data(), stream_);
+            launch_scan(nbank, bc.data(), pre.data(), stream_);
+            int next = 0;
+            PSMAS_HIP(hipMemcpyAsync(&next, pre.data() + nbank, sizeof(int), hipMemcpyDeviceToHost, stream_));
+            PSMAS_HIP(hipStreamSynchronize(stream_));
+            if (next == count)
+                break; // nothing merged (MASPreconditioner.cu:464-469)
+            DeviceBuffer<int> gid(count, "MAS coarse ids");
+            launch_make_gid(count, loc.data(), pre.data(), gid.data(), stream_);
+            launch_fill_cmap(npad_, lv, d_block_of_slot_.data(), gid.data(), d_cmap_.data(), stream_);
+            d_leadc_[lv] = std::move(lead);
+            d_gidc_[lv] = std::move(gid);
+            cco_[lv] = next;
+            level_num_ = lv + 1;
+            PSMAS_HIP(hipStreamSynchronize(stream_));
+        }
+
+        mat_cnt_[0] = banks;
+        mat_off_[0] = 0;
+        total_mats_ = banks;
+        lev_off_[0] = 0;
+        total_level_nodes_ = npad_;
+        for (int l = 0; l < MAX_COARSE_LEVELS; ++l)
+        {
+            mat_off_[l + 1] = total_mats_;
+            mat_cnt_[l + 1] = (l < level_num_) ? div_up(cco_[l], BANK) : 0;
+            total_mats_ += mat_cnt_[l + 1];
+            lev_off_[l + 1] = total_level_nodes_;
+            total_level_nodes_ += BANK * mat_cnt_[l + 1];
+        }
+        d_lev_off_.allocate(MAX_COARSE_LEVELS + 1, "MAS level offsets");
+        d_mat_off_.allocate(MAX_COARSE_LEVELS + 1, "MAS matrix offsets");
+        PSMAS_HIP(hipMemcpyAsync(d_lev_off_.data(), lev_off_, sizeof(lev_off_), hipMemcpyHostToDevice, stream_));
+        PSMAS_HIP(hipMemcpyAsync(d_mat_off_.data(), mat_off_, sizeof(mat_off_), hipMemcpyHostToDevice, stream_));
+
+        const int dim = BANK * bd_;
+        const size_t L = static_cast<size_t>(dim) * (dim + 1) / 2;
+        d_mats64_.allocate(static_cast<size_t>(total_mats_) * L, "MAS coarse_matrices");
+        d_mats64_.zero(stream_);
+        launch_galerkin(npad_, bd_, level_num_, d_mat_off_.data(), A, d_row_len_.data(), d_block_of_slot_.data(),
+                        d_cmap_.data(), d_mats64_.data(), stream_);
+        DeviceBuffer<int> fail(1, "MAS invert flag");
+        fail.zero(stream_);
+        launch_invert(bd_, total_mats_, d_mats64_.data(), fail.data(), stream_);
+        int h_fail = 0;
+        PSMAS_HIP(hipMemcpyAsync(&h_fail, fail.data(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        if (h_fail)
+            throw std::runtime_error("[MAS] MAS packed inverse failed.");
+    }
+
+    void Solver::allocate_vectors()
+    {
+        const size_t nvec = static_cast<size_t>(npad_) * bd_;
+        const size_t nml = static_cast<size_t>(total_level_nodes_) * bd_;
+        d_b_.allocate(nvec, "device_buffers b");
+        d_x_.allocate(nvec, "device_buffers x");
+        d_p_.allocate(nvec, "device_buffers p");
+        d_Ap_.allocate(nvec, "device_buffers Ap");
+        d_mr_.allocate(nml, "MAS coarse_vectors r");
+        d_mz_.allocate(nml, "MAS coarse_vectors z");
+        d_io_.allocate(static_cast<size_t>(n_), "device_buffers io");
+        d_mr_.zero(stream_);
+        d_mz_.zero(stream_);
+        d_x32_.reset();
+        d_mr32_.reset();
+        d_mz32_.reset();
+        d_p32_.reset();
+        d_Ap32_.reset();
+        partial_cap_ = std::max(npad_ / 256 + 8, 2048);
+        d_partials_.allocate(static_cast<size_t>(R_COUNT) * partial_cap_, "MAS reduction partials");
+        d_counters_.allocate(R_COUNT, "MAS reduction counters");
+        d_counters_.zero(stream_);
+    }
+
+    void Solver::ensure_mixed_data()
+    {
+        const int bs = bd_ * bd_;
+        if (!d_vals32_)
+        {
+            d_vals32_.allocate(static_cast<size_t>(nslots_) * WAVE * bs, "MAS SELL values (FP32)");
+            launch_copy_convert<double, float>(static_cast<long long>(d_vals32_.size()), d_vals64_.data(),
+                                               d_vals32_.data(), stream_);
+        }
+        const int dim = BANK * bd_;
+        const size_t L = static_cast<size_t>(dim) * (dim + 1) / 2;
+        if (!d_mats16_)
+        {
+            d_mats16_.allocate(static_cast<size_t>(mat_cnt_[0]) * L, "MAS fine bank inverses (FP16)");
+            d_dscale_.allocate(static_cast<size_t>(mat_cnt_[0]) * dim, "MAS fine bank scaling");
+            launch_make_half(bd_, mat_cnt_[0], d_mats64_.data(), d_mats16_.data(), d_dscale_.data(), stream_);
+        }
+        const size_t nvec = static_cast<size_t>(npad_) * bd_;
+        const size_t nml = static_cast<size_t>(total_level_nodes_) * bd_;
+        if (!d_x32_)
+        {
+            d_x32_.allocate(nvec, "device_buffers x32");
+            d_p32_.allocate(nvec, "device_buffers p32");
+            d_Ap32_.allocate(nvec, "device_buffers Ap32");
+            d_mr32_.allocate(nml, "MAS coarse_vectors r32");
+            d_mz32_.allocate(nml, "MAS coarse_vectors z32");
+            d_mr32_.zero(stream_);
+            d_mz32_.zero(stream_);
+        }
+    }
+
+    void Solver::check_ready(int n) const
+    {
+        if (!factorized_ || n <= 0 || n != n_)
+            throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
+    }
+
+    // ------------------------------------------------------------------------------- solve
+    void Solver::solve_host(int n, const double *b, double *x)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        status_ = Status::Running;
+        check_ready(n);
+        if (b == nullptr || x == nullptr)
+            throw InvalidArgumentError("[MAS] null b or x.");
+        PSMAS_HIP(hipMemcpyAsync(d_io_.data(), b, sizeof(double) * n, hipMemcpyHostToDevice, stream_));
+        launch_gather_in<double>(npad_, bd_, n_, d_block_of_slot_.data(), d_io_.data(), d_b_.data(), stream_);
+        solve_padded();
+        launch_scatter_out<double>(nb_, bd_, n_, d_slot_of_block_.data(), d_x_.data(), d_io_.data(), stream_);
+        PSMAS_HIP(hipMemcpyAsync(x, d_io_.data(), sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+    }
+
+    void Solver::solve_device(int n, const double *d_b, double *d_x)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        status_ = Status::Running;
+        check_ready(n);
+        if (d_b == nullptr || d_x == nullptr)
+            throw InvalidArgumentError("[MAS] null b or x.");
+        launch_gather_in<double>(npad_, bd_, n_, d_block_of_slot_.data(), d_b, d_b_.data(), stream_);
+        solve_padded();
+        launch_scatter_out<double>(nb_, bd_, n_, d_slot_of_block_.data(), d_x_.data(), d_x, stream_);
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+    }
+
+    void Solver::solve_padded()
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        iterations_ = 0;
+        residual_norm_ = 0.0;
+        reliable_updates_ = 0;
+        true_rel_residual_ = 0.0;
+        launch_set_scalar(d_sc_.data(), S_GUARD, params_.division_guard, stream_);
+        if (params_.precision == Precision::Mixed)
+        {
+            ensure_mixed_data();
+            solve_mixed();
+        }
+        else
+        {
+            solve_fp64();
+        }
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        collect_timings();
+        solve_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void Solver::run_block(int key, int count, const std::function<void()> &enqueue)
+    {
+        (void)count;
+        if (!params_.use_graphs || timing_on_)
+        {
+            enqueue();
+            return;
+        }
+        auto it = graphs_->execs.find(key);
+        if (it == graphs_->execs.end())
+        {
+            hipGraph_t g = nullptr;
+            PSMAS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            try
+            {
+                enqueue();
+            }
+            catch (...)
+            {
+                hipGraph_t dead = nullptr;
+                (void)hipStreamEndCapture(stream_, &dead);
+                if (dead)
+                    (void)hipGraphDestroy(dead);
+                throw;
+            }
+            PSMAS_HIP(hipStreamEndCapture(stream_, &g));
+            hipGraphExec_t ge = nullptr;
+            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess)
+                throw_hip_error(e, "hipGraphInstantiate", __FILE__, __LINE__);
+            it = graphs_->execs.emplace(key, ge).first;
+        }
+        PSMAS_HIP(hipGraphLaunch(it->second, stream_));
+    }
+
+    // ---- FP64 path: the reference schedule -------------------------------------------------
+    template <int BD>
+    void Solver::fp64_precondition(bool init)
+    {
+        const int fine_mats = total_mats_;
+        launch_restrict_coarse<BD, double>(coarse_restrict(), d_mr_.data(), stream_);
+        tic("mas_apply_fp64", static_cast<double>(fine_mats) * (BANK * BD) * (BANK * BD + 1) / 2 * 8.0 +
+                                  2.0 * fine_mats * BANK * BD * 8.0);
+        launch_mas_apply<BD, double, double>(0, total_mats_, d_mats64_.data(), nullptr, d_mr_.data(), d_mz_.data(),
+                                             stream_);
+        toc();
+        launch_prolong_dot<BD, double>(npad_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr_.data(),
+                                       d_mz_.data(), d_sc_.data(), init, site(R_RZ), stream_);
+    }
+
+    template <int BD>
+    void Solver::fp64_iteration(int k)
+    {
+        const SellView<double> A = sell(d_vals64_.data());
+        const long long nvec = static_cast<long long>(npad_) * BD;
+        const double spmv_bytes = static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_ + 1) * 4.0 + 2.0 * n_ * 8.0;
+        tic("spmv_fp64", spmv_bytes);
+        launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), d_sc_.data(), site(R_PAP), stream_);
+        toc();
+        if (k % params_.true_residual_period == 0)
+        {
+            launch_axpy_alpha<double>(nvec, d_p_.data(), d_x_.data(), d_sc_.data(), stream_);
+            tic("spmv_res_fp64", spmv_bytes + n_ * 8.0);
+            launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
+                                                        d_mr_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
+                                                        S_RR, site(R_RR), stream_);
+            toc();
+        }
+        else
+        {
+            launch_update_xr<BD, double>(npad_, d_p_.data(), d_Ap_.data(), d_x_.data(), d_mr_.data(), fine_restrict(),
+                                         d_mr_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(), site(R_RR),
+                                         stream_);
+        }
+        fp64_precondition<BD>(false);
+        launch_update_p<double>(nvec, d_mz_.data(), d_p_.data(), d_sc_.data(), stream_);
+    }
+
+    void Solver::solve_fp64()
+    {
+        const int bd = bd_;
+        auto body = [&](auto bdc) {
+            constexpr int BD = decltype(bdc)::value;
+            const SellView<double> A = sell(d_vals64_.data());
+            const long long nvec = static_cast<long long>(npad_) * BD;
+            launch_fill<double>(nvec, d_x_.data(), 0.0, stream_);
+            // r = b - A x (x = 0), rr, restriction.
+            launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
+                                                        d_mr_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
+                                                        S_RR, site(R_RR), stream_);
+            fp64_precondition<BD>(true);
+            launch_copy_convert<double, double>(nvec, d_mz_.data(), d_p_.data(), stream_);
+            double sc[S_COUNT];
+            read_scalars(sc);
+            const double rz0 = sc[S_RZ0];
+            if (std::isnan(rz0) || !std::isfinite(rz0))
+                throw std::runtime_error("[MAS] Invalid initial residual.");
+            const double rr0 = params_.use_preconditioned_residual_norm ? 0.0 : sc[S_RR];
+            const double rel2 = params_.rel_tol * params_.rel_tol;
+            const double abs2 = params_.abs_tol * params_.abs_tol;
+            const int ci = params_.check_interval;
+            const int trp = params_.true_residual_period;
+            int k = 1;
+            bool converged = false;
+            while (k <= params_.max_iter && !converged)
+            {
+                const int next_check = ((k - 1) / ci + 1) * ci;
+                const int last = std::min(next_check, params_.max_iter);
+                const int count = last - k + 1;
+                const int key = ((k % trp) << 20) | count;
+                const int k0 = k;
+                run_block(key, count, [&]() {
+                    for (int i = k0; i <= last; ++i)
+                        fp64_iteration<BD>(i);
+                });
+                iterations_ = last;
+                if (last % ci == 0)
+                {
+                    read_scalars(sc);
+                    if (params_.use_preconditioned_residual_norm)
+                    {
+                        const double rz = sc[S_RZ];
+                        residual_norm_ = std::sqrt(rz);
+                        if (rz <= rel2 * rz0 || rz <= abs2)
+                        {
+                            status_ = (rz <= abs2) ? Status::ReachAbsoluteTolerance : Status::ReachRelativeTolerance;
+                            converged = true;
+                        }
+                    }
+                    else
+                    {
+                        const double rr = sc[S_RR];
+                        residual_norm_ = std::sqrt(rr);
+                        if (rr <= rel2 * rr0 || rr <= abs2)
+                        {
+                            status_ = (rr <= abs2) ? Status::ReachAbsoluteTolerance : Status::ReachRelativeTolerance;
+                            converged = true;
+                        }
+                    }
+                }
+                k = last + 1;
+            }
+            if (iterations_ == params_.max_iter)
+                status_ = Status::ReachMaxIterations;
+            const double bb = rr0 > 0.0 ? rr0 : sc[S_RR];
+            true_rel_residual_ = bb > 0.0 ? residual_norm_ / std::sqrt(bb) : 0.0;
+        };
+        if (bd == 1)
+            body(std::integral_constant<int, 1>());
+        else if (bd == 2)
+            body(std::integral_constant<int, 2>());
+        else
+            body(std::integral_constant<int, 3>());
+    }
+
+    // ---- mixed path: FP32 inner PCG with FP64 reliable updates ------------------------------
+    template <int BD>
+    void Solver::mixed_part_a()
+    {
+        const SellView<float> A = sell(d_vals32_.data());
+        tic("spmv_fp32", static_cast<double>(nnzb_) * (BD * BD * 4 + 4) + (nb_ + 1) * 4.0 + 2.0 * n_ * 4.0);
+        launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), d_sc_.data(), site(R_PAP), stream_);
+        toc();
+        launch_update_xr<BD, float>(npad_, d_p32_.data(), d_Ap32_.data(), d_x32_.data(), d_mr32_.data(),
+                                    fine_restrict(), d_mr32_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
+                                    site(R_RR), stream_);
+    }
+
+    template <int BD>
+    void Solver::mixed_part_b(bool init)
+    {
+        const long long nvec = static_cast<long long>(npad_) * BD;
+        const int dim = BANK * BD;
+        const double L = static_cast<double>(dim) * (dim + 1) / 2;
+        launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), stream_);
+        tic("mas_apply_fp16", mat_cnt_[0] * (L * 2.0 + dim * 4.0 + 2.0 * dim * 4.0));
+        launch_mas_apply<BD, __half, float>(0, mat_cnt_[0], d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
+                                            d_mz32_.data(), stream_);
+        toc();
+        launch_mas_apply<BD, double, float>(mat_cnt_[0], total_mats_ - mat_cnt_[0], d_mats64_.data(), nullptr,
+                                            d_mr32_.data(), d_mz32_.data(), stream_);
+        launch_prolong_dot<BD, float>(npad_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
+                                      d_mz32_.data(), d_sc_.data(), init, site(R_RZ), stream_);
+        if (init)
+            launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_p32_.data(), stream_);
+        else
+            launch_update_p<float>(nvec, d_mz32_.data(), d_p32_.data(), d_sc_.data(), stream_);
+    }
+
+    template <int BD>
+    void Solver::mixed_reliable_residual()
+    {
+        const SellView<double> A = sell(d_vals64_.data());
+        tic("spmv_res_fp64", static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_ + 1) * 4.0 + 3.0 * n_ * 8.0);
+        launch_spmv_res<BD, double, double, float>(A, d_x_.data(), d_b_.data(), d_mr32_.data(), fine_restrict(),
+                                                   d_mr32_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
+                                                   S_RR_TRUE, site(R_RR_TRUE), stream_);
+        toc();
+    }
+
+    void Solver::solve_mixed()
+    {
+        const int bd = bd_;
+        auto body = [&](auto bdc) {
+            constexpr int BD = decltype(bdc)::value;
+            const long long nvec = static_cast<long long>(npad_) * BD;
+            launch_fill<double>(nvec, d_x_.data(), 0.0, stream_);
+            launch_fill<float>(nvec, d_x32_.data(), 0.0f, stream_);
+            mixed_reliable_residual<BD>();
+            mixed_part_b<BD>(true);
+            double sc[S_COUNT];
+            read_scalars(sc);
+            const double bb = sc[S_RR_TRUE];
+            const double bnorm = std::sqrt(bb);
+            if (std::isnan(sc[S_RZ0]) || !std::isfinite(sc[S_RZ0]))
+                throw std::runtime_error("[MAS] Invalid initial residual.");
+            const double target = std::max(params_.rel_tol * bnorm, params_.abs_tol);
+            residual_norm_ = bnorm;
+            true_rel_residual_ = bb > 0.0 ? 1.0 : 0.0;
+            if (bnorm <= params_.abs_tol)
+            {
+                status_ = Status::ReachAbsoluteTolerance;
+                return;
+            }
+            double ref = bnorm;
+            double best = bnorm;
+            int stalls = 0;
+            bool first = true;
+            bool converged = false;
+            int k = 0;
+            const int ci = params_.check_interval;
+            while (k < params_.max_iter && !converged)
+            {
+                const int count = std::min(ci, params_.max_iter - k);
+                const int key = ((first ? 1 : 2) << 20) | count;
+                const bool lead_b = !first;
+                run_block(key, count, [&]() {
+                    if (lead_b)
+                        mixed_part_b<BD>(false);
+                    for (int i = 0; i < count; ++i)
+                    {
+                        mixed_part_a<BD>();
+                        if (i + 1 < count)
+                            mixed_part_b<BD>(false);
+                    }
+                });
+                first = false;
+                k += count;
+                iterations_ = k;
+                read_scalars(sc);
+                const double rn = std::sqrt(std::max(sc[S_RR], 0.0));
+                if (!(rn <= params_.reliable_update_delta * ref || rn <= target) && k < params_.max_iter)
+                    continue;
+                // Reliable update: fold the FP32 correction into x and recompute r in FP64.
+                launch_accumulate_correction(nvec, d_x32_.data(), d_x_.data(), stream_);
+                mixed_reliable_residual<BD>();
+                ++reliable_updates_;
+                const double rt = std::sqrt(std::max(read_scalar(S_RR_TRUE), 0.0));
+                residual_norm_ = rt;
+                true_rel_residual_ = rt / bnorm;
+                if (rt <= target)
+                {
+                    status_ = (rt <= params_.abs_tol) ? Status::ReachAbsoluteTolerance
+                                                      : Status::ReachRelativeTolerance;
+                    converged = true;
+                    break;
+                }
+                if (!(rt < 0.9 * best))
+                {
+                    if (++stalls >= 8)
+                        break;
+                }
+                else
+                {
+                    stalls = 0;
+                }
+                best = std::min(best, rt);
+                ref = rt;
+            }
+            if (!converged)
+                status_ = Status::ReachMaxIterations;
+        };
+        if (bd == 1)
+            body(std::integral_constant<int, 1>());
+        else if (bd == 2)
+            body(std::integral_constant<int, 2>());
+        else
+            body(std::integral_constant<int, 3>());
+    }
+
+    // --------------------------------------------------------------------- introspection
+    void Solver::get_partition(std::vector<int> &offsets, std::vector<int> &perm) const
+    {
+        if (!factorized_)
+            throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
+        offsets.assign(nparts_ + 1, 0);
+        for (int p = 0; p <= part_.nparts; ++p)
+            offsets[p] = part_.offsets[p];
+        for (int p = part_.nparts + 1; p <= nparts_; ++p)
+            offsets[p] = part_.offsets[part_.nparts];
+        perm = part_.perm;
+    }
+
+    std::vector<int> Solver::level_info() const
+    {
+        return {level_num_, cco_[0], cco_[1], cco_[2], cco_[3], nparts_, npad_, total_mats_,
+                static_cast<int>(std::min<long long>(nnzb_, 2147483647LL)), static_cast<int>(nslots_), nb_, n_};
+    }
+
+    std::vector<int> Solver::coarse_map() const
+    {
+        if (!factorized_)
+            throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
+        std::vector<int4> cm(npad_);
+        PSMAS_HIP(hipMemcpy(cm.data(), d_cmap_.data(), sizeof(int4) * npad_, hipMemcpyDeviceToHost));
+        std::vector<int> out(static_cast<size_t>(nb_) * 4);
+        for (int b = 0; b < nb_; ++b)
+        {
+            const int4 c = cm[slot_of_block_[b]];
+            out[4 * b + 0] = c.x;
+            out[4 * b + 1] = c.y;
+            out[4 * b + 2] = c.z;
+            out[4 * b + 3] = c.w;
+        }
+        return out;
+    }
+
+    std::vector<double> Solver::bank_inverses() const
+    {
+        if (!factorized_)
+            throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
+        std::vector<double> out(d_mats64_.size());
+        PSMAS_HIP(hipMemcpy(out.data(), d_mats64_.data(), d_mats64_.bytes(), hipMemcpyDeviceToHost));
+        return out;
+    }
+
+    void Solver::apply_preconditioner(const double *r, double *z, bool mixed)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        check_ready(n_);
+        const long long nvec = static_cast<long long>(npad_) * bd_;
+        PSMAS_HIP(hipMemcpyAsync(d_io_.data(), r, sizeof(double) * n_, hipMemcpyHostToDevice, stream_));
+        launch_gather_in<double>(npad_, bd_, n_, d_block_of_slot_.data(), d_io_.data(), d_p_.data(), stream_);
+        launch_fill<double>(nvec, d_Ap_.data(), 0.0, stream_);
+        launch_set_scalar(d_sc_.data(), S_GUARD, params_.division_guard, stream_);
+        if (mixed)
+            ensure_mixed_data();
+        auto body = [&](auto bdc) {
+            constexpr int BD = decltype(bdc)::value;
+            const SellView<double> A = sell(d_vals64_.data());
+            if (!mixed)
+            {
+                // r - A*0 = r, restricted to coarse level 0 by the same fused kernel the solve uses.
+                launch_spmv_res<BD, double, double, double>(A, d_Ap_.data(), d_p_.data(), d_mr_.data(),
+                                                            fine_restrict(),
+                                                            d_mr_.data() + static_cast<size_t>(npad_) * BD,
+                                                            d_sc_.data(), S_RR_TRUE, site(R_RR_TRUE), stream_);
+                fp64_precondition<BD>(true);
+                launch_scatter_out<double>(nb_, BD, n_, d_slot_of_block_.data(), d_mz_.data(), d_io_.data(), stream_);
+            }
+            else
+            {
+                launch_spmv_res<BD, double, double, float>(A, d_Ap_.data(), d_p_.data(), d_mr32_.data(),
+                                                           fine_restrict(),
+                                                           d_mr32_.data() + static_cast<size_t>(npad_) * BD,
+                                                           d_sc_.data(), S_RR_TRUE, site(R_RR_TRUE), stream_);
+                mixed_part_b<BD>(true);
+                launch_copy_convert<float, double>(nvec, d_mz32_.data(), d_Ap_.data(), stream_);
+                launch_scatter_out<double>(nb_, BD, n_, d_slot_of_block_.data(), d_Ap_.data(), d_io_.data(), stream_);
+            }
+        };
+        if (bd_ == 1)
+            body(std::integral_constant<int, 1>());
+        else if (bd_ == 2)
+            body(std::integral_constant<int, 2>());
+        else
+            body(std::integral_constant<int, 3>());
+        PSMAS_HIP(hipMemcpyAsync(z, d_io_.data(), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        collect_timings();
+    }
+
+    void Solver::spmv(const double *x, double *y, bool fp32)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        check_ready(n_);
+        const long long nvec = static_cast<long long>(npad_) * bd_;
+        PSMAS_HIP(hipMemcpyAsync(d_io_.data(), x, sizeof(double) * n_, hipMemcpyHostToDevice, stream_));
+        launch_gather_in<double>(npad_, bd_, n_, d_block_of_slot_.data(), d_io_.data(), d_p_.data(), stream_);
+        if (fp32)
+            ensure_mixed_data();
+        auto body = [&](auto bdc) {
+            constexpr int BD = decltype(bdc)::value;
+            if (!fp32)
+            {
+                launch_spmv_plain<BD, double, double>(sell(d_vals64_.data()), d_p_.data(), d_Ap_.data(), stream_);
+            }
+            else
+            {
+                launch_copy_convert<double, float>(nvec, d_p_.data(), d_p32_.data(), stream_);
+                launch_spmv_plain<BD, float, float>(sell(d_vals32_.data()), d_p32_.data(), d_Ap32_.data(), stream_);
+                launch_copy_convert<float, double>(nvec, d_Ap32_.data(), d_Ap_.data(), stream_);
+            }
+        };
+        if (bd_ == 1)
+            body(std::integral_constant<int, 1>());
+        else if (bd_ == 2)
+            body(std::integral_constant<int, 2>());
+        else
+            body(std::integral_constant<int, 3>());
+        launch_scatter_out<double>(nb_, bd_, n_, d_slot_of_block_.data(), d_Ap_.data(), d_io_.data(), stream_);
+        PSMAS_HIP(hipMemcpyAsync(y, d_io_.data(), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+    }
+
+    // ------------------------------------------------------------------------------ timing
+    void Solver::tic(const char *name, double bytes)
+    {
+        if (!timing_on_)
+            return;
+        if (event_pool_used_ + 2 > event_pool_.size())
+        {
+            for (int i = 0; i < 64; ++i)
+            {
+                hipEvent_t ev;
+                PSMAS_HIP(hipEventCreate(&ev));
+                event_pool_.push_back(ev);
+            }
+        }
+        pending_start_ = event_pool_[event_pool_used_++];
+        pending_name_ = name;
+        pending_bytes_ = bytes;
+        PSMAS_HIP(hipEventRecord(pending_start_, stream_));
+    }
+
+    void Solver::toc()
+    {
+        if (!timing_on_ || pending_start_ == nullptr)
+            return;
+        hipEvent_t stop = event_pool_[event_pool_used_++];
+        PSMAS_HIP(hipEventRecord(stop, stream_));
+        pending_events_.push_back({pending_name_, {pending_start_, stop}});
+        auto &t = timings_[pending_name_];
+        t.name = pending_name_;
+        t.bytes_per_launch = pending_bytes_;
+        pending_start_ = nullptr;
+    }
+
+    void Solver::collect_timings()
+    {
+        if (pending_events_.empty())
+            return;
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        for (auto &pe : pending_events_)
+        {
+            float ms = 0.0f;
+            PSMAS_HIP(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+            auto &t = timings_[pe.first];
+            t.launches += 1;
+            t.total_ms += ms;
+        }
+        pending_events_.clear();
+        event_pool_used_ = 0;
+    }
+
+    std::vector<KernelTiming> Solver::kernel_timings() const
+    {
+        std::vector<KernelTiming> out;
+        for (const auto &kv : timings_)
+            out.push_back(kv.second);
+        return out;
+    }
+
+    void Solver::reset_kernel_timings() { timings_.clear(); }
+} // namespace psmas

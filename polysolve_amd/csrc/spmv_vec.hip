@@ -1,0 +1,451 @@
+// SELL-64 block SpMV and the fused vector kernels of the PCG loop (gfx950, wave64).
+//
+// Replaces the reference's cuSPARSE BSR SpMV (src/polysolve/linear/MASSolver.cu:245-290), its
+// inner_product (mas_utils/InnerProduct.cu:16-45), axpby / scalar_division
+// (MASSolver.cu:46-81) and the restriction half of gather_multi_level_r
+// (mas_utils/MASPreconditioner.cu:593-659).  Layout and fusion are this build's own:
+//  * one lane per block row; a wave owns a slice of 64 rows = two 32-node banks, so every
+//    matrix load of a wave is one contiguous 512 B (FP64) / 256 B (FP32) run;
+//  * reductions are deterministic (per-workgroup partials, summed in index order by the last
+//    workgroup) and their scalar epilogues (alpha, beta) run on the device;
+//  * the residual-producing kernels also restrict the residual to coarse level 0 through the
+//    bank-local component masks, so the preconditioner never re-reads r for that step.
+#include "device.hpp"
+#include "kernels.hpp"
+#include "common.hpp"
+
+namespace psmas
+{
+    namespace
+    {
+        constexpr int SPMV_BLOCK = 256; // 4 waves = 4 slices
+        constexpr int VEC_BLOCK = 256;
+
+        template <typename T>
+        struct Acc;
+        template <>
+        struct Acc<double>
+        {
+            using type = double;
+        };
+        template <>
+        struct Acc<float>
+        {
+            using type = float;
+        };
+
+        template <int BD, typename TV, typename TX, typename TA>
+        __device__ __forceinline__ void sell_row(const SellView<TV> &A, int slice, int lane, const TX *__restrict__ x,
+                                                 TA acc[BD])
+        {
+            const int beg = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice]);
+            const int end = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice + 1]);
+            const int *__restrict__ cols = A.cols + lane;
+            const TV *__restrict__ vals = A.vals + lane;
+#pragma unroll 2
+            for (int k = beg; k < end; ++k)
+            {
+                const int c = ldg_stream(cols + static_cast<size_t>(k) * WAVE);
+                TA xv[BD];
+#pragma unroll
+                for (int j = 0; j < BD; ++j)
+                    xv[j] = static_cast<TA>(x[static_cast<size_t>(c) * BD + j]);
+                const TV *__restrict__ v = vals + static_cast<size_t>(k) * (BD * BD * WAVE);
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+#pragma unroll
+                    for (int j = 0; j < BD; ++j)
+                        acc[i] = fma(static_cast<TA>(ldg_stream(v + (i * BD + j) * WAVE)), xv[j], acc[i]);
+            }
+        }
+
+        template <int BD, typename TV, typename TX>
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap(SellView<TV> A, const TX *__restrict__ x,
+                                                                TX *__restrict__ y, double *__restrict__ sc,
+                                                                ReduceSite site)
+        {
+            using TA = typename Acc<TV>::type;
+            const int lane = threadIdx.x % WAVE;
+            const int slice = blockIdx.x * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            double dot = 0.0;
+            if (slice < A.nslices)
+            {
+                TA acc[BD];
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                    acc[i] = TA(0);
+                sell_row<BD>(A, slice, lane, x, acc);
+                const size_t row = static_cast<size_t>(slice) * WAVE + lane;
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                {
+                    const TX yi = static_cast<TX>(acc[i]);
+                    y[row * BD + i] = yi;
+                    dot += static_cast<double>(x[row * BD + i]) * static_cast<double>(yi);
+                }
+            }
+            grid_sum<SPMV_BLOCK>(dot, site.partials, site.counter, [sc](double t) {
+                sc[S_PAP] = t;
+                sc[S_ALPHA] = safe_div(sc[S_RZ], t, sc[S_GUARD]);
+            });
+        }
+
+        /// Leader lanes of a bank sum their component's members from LDS into the coarse vector.
+        template <int BD, typename T>
+        __device__ __forceinline__ void restrict_bank(const T (*sv)[BD], int bank_base_in_block, uint32_t mask, int gid,
+                                                      T *__restrict__ yc0)
+        {
+            if (mask == 0u)
+                return;
+            double s[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                s[i] = 0.0;
+            while (mask)
+            {
+                const int b = __builtin_ctz(mask);
+                mask &= mask - 1u;
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                    s[i] += static_cast<double>(sv[bank_base_in_block + b][i]);
+            }
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                yc0[static_cast<size_t>(gid) * BD + i] = static_cast<T>(s[i]);
+        }
+
+        template <int BD, typename TV, typename TX, typename TY>
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_res(SellView<TV> A, const TX *__restrict__ x,
+                                                                 const TX *__restrict__ b, TY *__restrict__ y,
+                                                                 FineRestrict fr, TY *__restrict__ yc0,
+                                                                 double *__restrict__ sc, int rr_slot, ReduceSite site)
+        {
+            using TA = typename Acc<TV>::type;
+            __shared__ TY sv[SPMV_BLOCK][BD];
+            const int lane = threadIdx.x % WAVE;
+            const int slice = blockIdx.x * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            const size_t row = static_cast<size_t>(slice) * WAVE + lane;
+            double dot = 0.0;
+            uint32_t mask = 0u;
+            int gid = 0;
+            if (slice < A.nslices)
+            {
+                TA acc[BD];
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                    acc[i] = TA(0);
+                sell_row<BD>(A, slice, lane, x, acc);
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                {
+                    const double ri = static_cast<double>(b[row * BD + i]) - static_cast<double>(acc[i]);
+                    const TY yi = static_cast<TY>(ri);
+                    y[row * BD + i] = yi;
+                    sv[threadIdx.x][i] = yi;
+                    dot += ri * ri;
+                }
+                mask = fr.lead[row];
+                gid = fr.gid[row];
+            }
+            __syncthreads();
+            restrict_bank<BD, TY>(sv, (threadIdx.x / 32) * 32, mask, gid, yc0);
+            grid_sum<SPMV_BLOCK>(dot, site.partials, site.counter, [sc, rr_slot](double t) { sc[rr_slot] = t; });
+        }
+
+        template <int BD, typename TV, typename TX>
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_plain(SellView<TV> A, const TX *__restrict__ x,
+                                                                   TX *__restrict__ y)
+        {
+            using TA = typename Acc<TV>::type;
+            const int lane = threadIdx.x % WAVE;
+            const int slice = blockIdx.x * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            if (slice >= A.nslices)
+                return;
+            TA acc[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                acc[i] = TA(0);
+            sell_row<BD>(A, slice, lane, x, acc);
+            const size_t row = static_cast<size_t>(slice) * WAVE + lane;
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                y[row * BD + i] = static_cast<TX>(acc[i]);
+        }
+
+        template <int BD, typename T>
+        __global__ __launch_bounds__(VEC_BLOCK) void k_update_xr(int nnodes, const T *__restrict__ p,
+                                                                 const T *__restrict__ Ap, T *__restrict__ x,
+                                                                 T *__restrict__ r, FineRestrict fr, T *__restrict__ rc0,
+                                                                 double *__restrict__ sc, ReduceSite site)
+        {
+            __shared__ T sv[VEC_BLOCK][BD];
+            const int node = blockIdx.x * VEC_BLOCK + threadIdx.x;
+            const T alpha = static_cast<T>(sc[S_ALPHA]);
+            double dot = 0.0;
+            uint32_t mask = 0u;
+            int gid = 0;
+            if (node < nnodes)
+            {
+                const size_t o = static_cast<size_t>(node) * BD;
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                {
+                    const T pv = p[o + i];
+                    const T av = Ap[o + i];
+                    x[o + i] = fma(alpha, pv, x[o + i]);
+                    const T rv = fma(-alpha, av, r[o + i]);
+                    r[o + i] = rv;
+                    sv[threadIdx.x][i] = rv;
+                    dot += static_cast<double>(rv) * static_cast<double>(rv);
+                }
+                mask = fr.lead[node];
+                gid = fr.gid[node];
+            }
+            __syncthreads();
+            restrict_bank<BD, T>(sv, (threadIdx.x / 32) * 32, mask, gid, rc0);
+            grid_sum<VEC_BLOCK>(dot, site.partials, site.counter, [sc](double t) { sc[S_RR] = t; });
+        }
+
+        template <typename T>
+        __global__ void k_axpy_alpha(long long n, const T *__restrict__ p, T *__restrict__ x, const double *sc)
+        {
+            const T alpha = static_cast<T>(sc[S_ALPHA]);
+            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+                x[i] = fma(alpha, p[i], x[i]);
+        }
+
+        template <typename T>
+        __global__ void k_update_p(long long n, const T *__restrict__ z, T *__restrict__ p, const double *sc)
+        {
+            const T beta = static_cast<T>(sc[S_BETA]);
+            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+                p[i] = fma(beta, p[i], z[i]);
+        }
+
+        template <typename TS, typename TD>
+        __global__ void k_copy_convert(long long n, const TS *__restrict__ src, TD *__restrict__ dst)
+        {
+            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+                dst[i] = static_cast<TD>(src[i]);
+        }
+
+        __global__ void k_accumulate_correction(long long n, float *__restrict__ x32, double *__restrict__ x64)
+        {
+            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+            {
+                x64[i] += static_cast<double>(x32[i]);
+                x32[i] = 0.0f;
+            }
+        }
+
+        template <typename T>
+        __global__ void k_fill(long long n, T *__restrict__ dst, T value)
+        {
+            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+                dst[i] = value;
+        }
+
+        template <typename T>
+        __global__ void k_gather_in(int npad, int bd, int n, const int *__restrict__ block_of_slot,
+                                    const T *__restrict__ src, T *__restrict__ dst)
+        {
+            const long long total = static_cast<long long>(npad) * bd;
+            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < total;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+            {
+                const int slot = static_cast<int>(i / bd), c = static_cast<int>(i % bd);
+                const int blk = block_of_slot[slot];
+                const long long s = static_cast<long long>(blk) * bd + c;
+                dst[i] = (blk >= 0 && s < n) ? src[s] : T(0);
+            }
+        }
+
+        template <typename T>
+        __global__ void k_scatter_out(int nb, int bd, int n, const int *__restrict__ slot_of_block,
+                                      const T *__restrict__ src, T *__restrict__ dst)
+        {
+            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+            {
+                const int blk = static_cast<int>(i / bd), c = static_cast<int>(i % bd);
+                dst[i] = src[static_cast<long long>(slot_of_block[blk]) * bd + c];
+            }
+        }
+
+        __global__ void k_set_scalar(double *sc, int slot, double value) { sc[slot] = value; }
+
+        template <typename T>
+        __global__ __launch_bounds__(VEC_BLOCK) void k_dot(long long n, const T *__restrict__ a,
+                                                           const T *__restrict__ b, double *sc, int slot,
+                                                           ReduceSite site)
+        {
+            double s = 0.0;
+            for (long long i = blockIdx.x * static_cast<long long>(VEC_BLOCK) + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * VEC_BLOCK)
+                s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+            grid_sum<VEC_BLOCK>(s, site.partials, site.counter, [sc, slot](double t) { sc[slot] = t; });
+        }
+
+        int stream_grid(long long n)
+        {
+            long long g = div_up_ll(n, 256);
+            if (g > 8192)
+                g = 8192;
+            return static_cast<int>(g < 1 ? 1 : g);
+        }
+    } // namespace
+
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, double *sc, ReduceSite site, hipStream_t s)
+    {
+        const int grid = div_up(A.nslices, SPMV_BLOCK / WAVE);
+        hipLaunchKernelGGL((k_spmv_ap<BD, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, y, sc, site);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD, typename TV, typename TX, typename TY>
+    void launch_spmv_res(const SellView<TV> &A, const TX *x, const TX *b, TY *y, FineRestrict fr, TY *yc0,
+                         double *sc, int rr_slot, ReduceSite site, hipStream_t s)
+    {
+        const int grid = div_up(A.nslices, SPMV_BLOCK / WAVE);
+        hipLaunchKernelGGL((k_spmv_res<BD, TV, TX, TY>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, b, y, fr, yc0, sc,
+                           rr_slot, site);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_plain(const SellView<TV> &A, const TX *x, TX *y, hipStream_t s)
+    {
+        const int grid = div_up(A.nslices, SPMV_BLOCK / WAVE);
+        hipLaunchKernelGGL((k_spmv_plain<BD, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, y);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD, typename T>
+    void launch_update_xr(int nnodes, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0, double *sc,
+                          ReduceSite site, hipStream_t s)
+    {
+        const int grid = div_up(nnodes, VEC_BLOCK);
+        hipLaunchKernelGGL((k_update_xr<BD, T>), dim3(grid), dim3(VEC_BLOCK), 0, s, nnodes, p, Ap, x, r, fr, rc0, sc,
+                           site);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <typename T>
+    void launch_axpy_alpha(long long n, const T *p, T *x, const double *sc, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_axpy_alpha<T>), dim3(stream_grid(n)), dim3(256), 0, s, n, p, x, sc);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <typename T>
+    void launch_update_p(long long n, const T *z, T *p, const double *sc, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_update_p<T>), dim3(stream_grid(n)), dim3(256), 0, s, n, z, p, sc);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <typename TS, typename TD>
+    void launch_copy_convert(long long n, const TS *src, TD *dst, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_copy_convert<TS, TD>), dim3(stream_grid(n)), dim3(256), 0, s, n, src, dst);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_accumulate_correction(long long n, float *x32, double *x64, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_accumulate_correction, dim3(stream_grid(n)), dim3(256), 0, s, n, x32, x64);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <typename T>
+    void launch_fill(long long n, T *dst, T value, hipStream_t s)
+    {
+        if (n <= 0)
+            return;
+        hipLaunchKernelGGL((k_fill<T>), dim3(stream_grid(n)), dim3(256), 0, s, n, dst, value);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <typename T>
+    void launch_gather_in(int npad, int bd, int n, const int *block_of_slot, const T *src, T *dst, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_gather_in<T>), dim3(stream_grid(static_cast<long long>(npad) * bd)), dim3(256), 0, s,
+                           npad, bd, n, block_of_slot, src, dst);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <typename T>
+    void launch_scatter_out(int nb, int bd, int n, const int *slot_of_block, const T *src, T *dst, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_scatter_out<T>), dim3(stream_grid(n)), dim3(256), 0, s, nb, bd, n, slot_of_block, src,
+                           dst);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_set_scalar(double *sc, int slot, double value, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, slot, value);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <typename T>
+    void launch_dot(long long n, const T *a, const T *b, double *sc, int slot, ReduceSite site, hipStream_t s)
+    {
+        int grid = static_cast<int>(div_up_ll(n, VEC_BLOCK));
+        if (grid > 2048)
+            grid = 2048;
+        if (grid < 1)
+            grid = 1;
+        hipLaunchKernelGGL((k_dot<T>), dim3(grid), dim3(VEC_BLOCK), 0, s, n, a, b, sc, slot, site);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+#define PSMAS_INST_BD(BD)                                                                                         \
+    template void launch_spmv_ap<BD, double, double>(const SellView<double> &, const double *, double *, double *, \
+                                                     ReduceSite, hipStream_t);                                   \
+    template void launch_spmv_ap<BD, float, float>(const SellView<float> &, const float *, float *, double *,     \
+                                                   ReduceSite, hipStream_t);                                     \
+    template void launch_spmv_res<BD, double, double, double>(const SellView<double> &, const double *,          \
+                                                              const double *, double *, FineRestrict, double *,  \
+                                                              double *, int, ReduceSite, hipStream_t);           \
+    template void launch_spmv_res<BD, double, double, float>(const SellView<double> &, const double *,           \
+                                                             const double *, float *, FineRestrict, float *,     \
+                                                             double *, int, ReduceSite, hipStream_t);            \
+    template void launch_spmv_plain<BD, double, double>(const SellView<double> &, const double *, double *,      \
+                                                        hipStream_t);                                            \
+    template void launch_spmv_plain<BD, float, float>(const SellView<float> &, const float *, float *,           \
+                                                      hipStream_t);                                              \
+    template void launch_update_xr<BD, double>(int, const double *, const double *, double *, double *,          \
+                                               FineRestrict, double *, double *, ReduceSite, hipStream_t);       \
+    template void launch_update_xr<BD, float>(int, const float *, const float *, float *, float *, FineRestrict,  \
+                                              float *, double *, ReduceSite, hipStream_t);
+
+    PSMAS_INST_BD(1)
+    PSMAS_INST_BD(2)
+    PSMAS_INST_BD(3)
+#undef PSMAS_INST_BD
+
+    template void launch_axpy_alpha<double>(long long, const double *, double *, const double *, hipStream_t);
+    template void launch_axpy_alpha<float>(long long, const float *, float *, const double *, hipStream_t);
+    template void launch_update_p<double>(long long, const double *, double *, const double *, hipStream_t);
+    template void launch_update_p<float>(long long, const float *, float *, const double *, hipStream_t);
+    template void launch_copy_convert<double, float>(long long, const double *, float *, hipStream_t);
+    template void launch_copy_convert<float, double>(long long, const float *, double *, hipStream_t);
+    template void launch_copy_convert<double, double>(long long, const double *, double *, hipStream_t);
+    template void launch_copy_convert<float, float>(long long, const float *, float *, hipStream_t);
+    template void launch_fill<double>(long long, double *, double, hipStream_t);
+    template void launch_fill<float>(long long, float *, float, hipStream_t);
+    template void launch_fill<int>(long long, int *, int, hipStream_t);
+    template void launch_fill<unsigned>(long long, unsigned *, unsigned, hipStream_t);
+    template void launch_gather_in<double>(int, int, int, const int *, const double *, double *, hipStream_t);
+    template void launch_scatter_out<double>(int, int, int, const int *, const double *, double *, hipStream_t);
+    template void launch_dot<double>(long long, const double *, const double *, double *, int, ReduceSite,
+                                     hipStream_t);
+    template void launch_dot<float>(long long, const float *, const float *, double *, int, ReduceSite, hipStream_t);
+} // namespace psmas

@@ -1,0 +1,109 @@
+"""Shared helpers of the test suite (CPU-side constructions used as independent checks)."""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+
+def morton_order(coords: np.ndarray) -> np.ndarray:
+    """Node order by 3-D Morton code of coordinates quantised to 21 bits over the bounding box
+    (same quantisation contract as the solver's Morton aggregation)."""
+    lo = coords.min(axis=0)
+    ext = coords.max(axis=0) - lo
+    inv = np.where(ext > 0, 1.0 / np.where(ext > 0, ext, 1.0), 0.0)
+    u = np.clip((coords - lo) * inv, 0.0, 1.0)
+    q = (u * 2097151.0).astype(np.uint64)
+    key = np.zeros(coords.shape[0], dtype=np.uint64)
+    for d in range(coords.shape[1]):
+        v = q[:, d]
+        for bit in range(21):
+            key |= ((v >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + d)
+    return np.argsort(key, kind="stable")
+
+
+def chunk_partition(order: np.ndarray, even: bool = True):
+    nb = order.size
+    P = max(1, (nb + 31) // 32)
+    offsets = np.minimum(np.arange(P + 1) * 32, nb)
+    if even and P % 2:
+        offsets = np.append(offsets, nb)
+    perm = np.empty(nb, dtype=np.int32)
+    perm[order] = np.arange(nb, dtype=np.int32)
+    return offsets.astype(np.int32), perm
+
+
+def prefactor_matrix(pattern: sp.spmatrix, seed: int = 42) -> sp.csc_matrix:
+    """Random SPD matrix on a symmetric pattern, as polysolve's "pre_factor" test builds them
+    (tests/test_linear_solver.cpp:262-284): diagonal U(0.1,5)*100, symmetric off-diagonal
+    -U(0.1,5).  numpy's generator replaces std::default_random_engine."""
+    rng = np.random.default_rng(seed)
+    coo = sp.triu(pattern, k=1).tocoo()
+    n = pattern.shape[0]
+    off = -rng.uniform(0.1, 5.0, coo.nnz)
+    diag = rng.uniform(0.1, 5.0, n) * 100.0
+    U = sp.coo_matrix((off, (coo.row, coo.col)), shape=(n, n))
+    A = U + U.T + sp.diags(diag)
+    return A.tocsc()
+
+
+def grid_laplacian_pattern(nx: int, ny: int, nz: int = 1) -> sp.csc_matrix:
+    """7-point (or 5-point) grid pattern."""
+    def lap1(m):
+        return sp.diags([np.ones(m - 1), 2 * np.ones(m), np.ones(m - 1)], [-1, 0, 1])
+
+    Ix, Iy, Iz = sp.identity(nx), sp.identity(ny), sp.identity(nz)
+    L = sp.kron(Iz, sp.kron(Iy, lap1(nx))) + sp.kron(Iz, sp.kron(lap1(ny), Ix))
+    if nz > 1:
+        L = L + sp.kron(lap1(nz), sp.kron(Iy, Ix))
+    return (L != 0).astype(np.float64).tocsc()
+
+
+def dense_mas(A: np.ndarray, bd: int, offsets, perm, cmap_perm, level_num):
+    """Dense additive MAS operator sum_l R_l^T blockdiag(inv(R_l A R_l^T)|banks) R_l built from
+    scratch with numpy, given the partition and the coarse map (over permuted block ids)."""
+    n = A.shape[0]
+    nb = (n + bd - 1) // bd
+    Ap = np.zeros((nb * bd, nb * bd))
+    Ap[:n, :n] = A
+    for t in range(n, nb * bd):
+        Ap[t, t] = 1.0
+    # permuted block -> input block
+    iperm = np.empty(nb, dtype=np.int64)
+    iperm[np.asarray(perm)] = np.arange(nb)
+    P = np.zeros((nb * bd, nb * bd))
+    for new in range(nb):
+        for c in range(bd):
+            P[new * bd + c, iperm[new] * bd + c] = 1.0
+    Aq = P @ Ap @ P.T  # permuted
+    offsets = np.asarray(offsets)
+    M = np.zeros_like(Aq)
+    for p in range(offsets.size - 1):
+        ids = np.arange(offsets[p], offsets[p + 1])
+        if ids.size == 0:
+            continue
+        idx = (ids[:, None] * bd + np.arange(bd)[None, :]).ravel()
+        M[np.ix_(idx, idx)] += np.linalg.inv(Aq[np.ix_(idx, idx)])
+    for l in range(level_num):
+        cm = cmap_perm[:, l]
+        nc = int(cm.max()) + 1
+        R = np.zeros((nc * bd, nb * bd))
+        for blk in range(nb):
+            for c in range(bd):
+                R[cm[blk] * bd + c, blk * bd + c] = 1.0
+        Ac = R @ Aq @ R.T
+        Ci = np.zeros_like(Ac)
+        for bank in range((nc + 31) // 32):
+            ids = np.arange(bank * 32, min(nc, bank * 32 + 32))
+            idx = (ids[:, None] * bd + np.arange(bd)[None, :]).ravel()
+            Ci[np.ix_(idx, idx)] = np.linalg.inv(Ac[np.ix_(idx, idx)])
+        M += R.T @ Ci @ R
+    Mo = P.T @ M @ P
+    return Mo[:n, :n]
+
+
+def unpack_upper(v: np.ndarray, N: int) -> np.ndarray:
+    M = np.zeros((N, N))
+    iu = np.triu_indices(N)
+    M[iu] = v
+    M[(iu[1], iu[0])] = v
+    return M

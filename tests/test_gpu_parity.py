@@ -1,0 +1,277 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and closed forms."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from polysolve_amd import MASError, MASSolver, Solver, problems
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def _factorized(p, params=None, coords=True):
+    s = MASSolver()
+    prm = {"MAS": {"block_dim": 3}}
+    if params:
+        prm["MAS"].update(params)
+    s.set_parameters(prm)
+    if coords:
+        s.set_coordinates(p.coords)
+    s.factorize(p.A)
+    return s
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    return problems.config1()
+
+
+@pytest.fixture(scope="module")
+def het12():
+    return problems.hex_elasticity(12, E=problems.random_moduli(12, 3), name="het12")
+
+
+def test_spmv_fp64_matches_csc(gpu, cfg1):
+    s = _factorized(cfg1)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(cfg1.n)
+    y = s.spmv(x)
+    ref = cfg1.A @ x
+    assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_spmv_fp32_tolerance(gpu, cfg1):
+    s = _factorized(cfg1)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(cfg1.n)
+    y = s.spmv(x, fp32=True)
+    ref = cfg1.A @ x
+    # FP32 values, vectors and accumulation: relative error a few ulps(fp32) times sqrt(row length).
+    assert np.linalg.norm(y - ref) <= 2e-6 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("bd", [1, 2, 3])
+def test_partition_and_coarse_map_match_oracle(gpu, bd, het12):
+    p = het12
+    s = MASSolver()
+    s.set_parameters({"MAS": {"block_dim": bd, "aggregation": "graph"}})
+    s.factorize(p.A)
+    offsets, perm = s.partition()
+    nb = (p.n + bd - 1) // bd
+    assert perm.size == nb and np.array_equal(np.sort(perm), np.arange(nb))
+    sizes = np.diff(offsets)
+    assert sizes.max() <= 32 and sizes.min() >= 0 and offsets[-1] == nb and (offsets.size - 1) % 2 == 0
+    O = oracle.MasOracle(p.A, bd, offsets, perm)
+    assert O.status == 0
+    li = s.level_info()
+    assert li["level_num"] == O.info["level_num"]
+    assert [li[f"cco{i}"] for i in range(4)] == [O.info[f"cco{i}"] for i in range(4)]
+    om, _ = O.coarse_map()  # over permuted ids
+    gm = s.coarse_map()     # over input block ids
+    L = O.info["level_num"]
+    assert np.array_equal(gm[:, :L], om[perm, :L])
+
+
+def test_morton_partition_matches_numpy_order(gpu, cfg1):
+    s = _factorized(cfg1, {"aggregation": "morton"})
+    offsets, perm = s.partition()
+    order = helpers.morton_order(cfg1.coords)
+    ref_off, ref_perm = helpers.chunk_partition(order)
+    assert np.array_equal(perm, ref_perm)
+    assert np.array_equal(offsets, ref_off)
+
+
+@pytest.mark.parametrize("agg", ["morton", "graph"])
+def test_bank_inverses_match_oracle(gpu, cfg1, agg):
+    s = _factorized(cfg1, {"aggregation": agg})
+    offsets, perm = s.partition()
+    O = oracle.MasOracle(cfg1.A, 3, offsets, perm)
+    _, inv_ref = O.matrices()
+    inv = s.bank_inverses(3)
+    assert inv.shape == inv_ref.shape
+    scale = np.abs(inv_ref).max(axis=1, keepdims=True)
+    err = np.abs(inv - inv_ref) / scale
+    assert err.max() < 1e-9
+
+
+@pytest.mark.parametrize("agg", ["morton", "graph"])
+def test_apply_fp64_matches_oracle(gpu, cfg1, agg):
+    s = _factorized(cfg1, {"aggregation": agg})
+    offsets, perm = s.partition()
+    O = oracle.MasOracle(cfg1.A, 3, offsets, perm)
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        r = rng.standard_normal(cfg1.n)
+        z = s.apply_preconditioner(r)
+        zr = O.apply(r)
+        assert np.linalg.norm(z - zr) <= 1e-10 * np.linalg.norm(zr)
+
+
+def test_apply_mixed_close_to_fp64(gpu, het12):
+    s = _factorized(het12)
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(het12.n)
+    z64 = s.apply_preconditioner(r)
+    z16 = s.apply_preconditioner(r, mixed=True)
+    # FP16 fine-level inverses (11-bit significand, unit-diagonal scaling): ~1e-3 relative.
+    assert np.linalg.norm(z16 - z64) <= 5e-3 * np.linalg.norm(z64)
+    # Still a symmetric positive operator on this vector.
+    assert r @ z16 > 0
+
+
+def test_apply_matches_dense_construction(gpu):
+    p = problems.hex_elasticity(4, E=problems.random_moduli(4, 7))
+    s = _factorized(p, {"aggregation": "graph"})
+    offsets, perm = s.partition()
+    li = s.level_info()
+    cm = s.coarse_map()
+    cm_perm = np.empty_like(cm)
+    cm_perm[perm] = cm
+    M = helpers.dense_mas(p.A.toarray(), 3, offsets, perm, cm_perm, li["level_num"])
+    rng = np.random.default_rng(4)
+    r = rng.standard_normal(p.n)
+    z = s.apply_preconditioner(r)
+    assert np.linalg.norm(z - M @ r) <= 1e-10 * np.linalg.norm(M @ r)
+
+
+def test_pcg_fp64_matches_oracle_schedule(gpu, het12):
+    p = het12
+    s = _factorized(p, {"aggregation": "graph", "relative_tolerance": 1e-8, "absolute_tolerance": 0.0,
+                        "max_iter": 3000})
+    x = np.zeros(p.n)
+    s.solve(p.b, x)
+    info = s.get_info()
+    offsets, perm = s.partition()
+    O = oracle.MasOracle(p.A, 3, offsets, perm)
+    ref = O.pcg(p.b, max_iter=3000, rel_tol=1e-8, abs_tol=0.0)
+    assert info["solver_status"] == "Reach relative tolerance"
+    assert ref["status"] == 1
+    assert abs(info["solver_iter"] - ref["iters"]) <= 10
+    assert info["solver_iter"] % 10 == 0
+    assert np.linalg.norm(x - ref["x"]) <= 1e-6 * np.linalg.norm(ref["x"])
+    assert np.linalg.norm(p.b - p.A @ x) <= 2e-8 * np.linalg.norm(p.b)
+
+
+def test_fp64_solution_matches_direct(gpu, cfg1):
+    s = _factorized(cfg1, {"relative_tolerance": 1e-10, "absolute_tolerance": 0.0, "max_iter": 5000})
+    x = np.zeros(cfg1.n)
+    s.solve(cfg1.b, x)
+    xd = spla.spsolve(cfg1.A.tocsc(), cfg1.b)
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("cfg", ["cfg1", "het12"])
+def test_mixed_reaches_fp64_tolerance(gpu, cfg, request):
+    p = request.getfixturevalue(cfg)
+    s = _factorized(p, {"precision": "mixed", "relative_tolerance": 1e-8, "absolute_tolerance": 0.0,
+                        "max_iter": 5000})
+    x = np.zeros(p.n)
+    s.solve(p.b, x)
+    info = s.get_info()
+    assert info["solver_status"] == "Reach relative tolerance", info
+    rel = np.linalg.norm(p.b - p.A @ x) / np.linalg.norm(p.b)
+    assert rel <= 1e-8
+    xd = spla.spsolve(p.A.tocsc(), p.b)
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+
+
+def test_api_semantics(gpu, cfg1):
+    s = Solver.create("MAS", "")
+    assert s.name() == "MAS"
+    x = np.zeros(cfg1.n)
+    with pytest.raises(MASError, match="Size mismatch"):
+        s.solve(cfg1.b, x)
+    s.set_parameters({"MAS": {"block_dim": 3, "max_iter": 1}})
+    s.factorize(cfg1.A)
+    s.solve(cfg1.b, x)
+    info = s.get_info()
+    assert info["solver_status"] == "Reach max iterations" and info["solver_iter"] == 1
+    with pytest.raises(MASError, match="Size mismatch"):
+        s.solve(cfg1.b[:-3], x[:-3])
+    bad = cfg1.b.copy()
+    bad[5] = np.nan
+    with pytest.raises(MASError, match="Invalid initial residual"):
+        s.solve(bad, x)
+
+
+def test_parameters_ignored_without_mas_key(gpu):
+    s = MASSolver()
+    s.set_parameters({"Eigen::ConjugateGradient": {"max_iter": 3}})
+    assert s.get_parameters()["max_iter"] == 5000
+
+
+@pytest.mark.parametrize("bd", [1, 2, 3])
+def test_prefactor_sequence(gpu, bd):
+    """polysolve's "pre_factor" test shape: one solver, ten matrices on one pattern, each solved
+    to ||Ax - b|| < 1e-8 (tests/test_linear_solver.cpp:241-307)."""
+    pattern = helpers.grid_laplacian_pattern(17, 13, 5)
+    s = MASSolver()
+    s.set_parameters({"MAS": {"block_dim": bd, "relative_tolerance": 0.0, "absolute_tolerance": 1e-10,
+                              "max_iter": 5000}})
+    rng = np.random.default_rng(42)
+    for i in range(10):
+        A = helpers.prefactor_matrix(pattern, seed=42 + i)
+        b = rng.uniform(-1, 1, A.shape[0])
+        x = np.zeros_like(b)
+        s.factorize(A)
+        s.solve(b, x)
+        assert np.linalg.norm(A @ x - b) < 1e-8
+
+
+def test_odd_sizes_trailing_block_padding(gpu):
+    pattern = helpers.grid_laplacian_pattern(11, 7, 3)  # 231 unknowns
+    A = helpers.prefactor_matrix(pattern, seed=5)
+    b = np.random.default_rng(5).uniform(-1, 1, A.shape[0])
+    for bd in (2, 3):  # 231 = 3*77, not a multiple of 2; exercise both
+        s = MASSolver()
+        s.set_parameters({"MAS": {"block_dim": bd, "relative_tolerance": 0.0, "absolute_tolerance": 1e-10}})
+        s.factorize(A)
+        x = np.zeros_like(b)
+        s.solve(b, x)
+        assert np.linalg.norm(A @ x - b) < 1e-8
+    A2 = helpers.prefactor_matrix(helpers.grid_laplacian_pattern(10, 10, 1)[:-1, :-1], seed=6)  # 99 unknowns
+    b2 = np.ones(A2.shape[0])
+    s = MASSolver()
+    s.set_parameters({"MAS": {"block_dim": 2, "relative_tolerance": 0.0, "absolute_tolerance": 1e-10}})
+    s.factorize(A2)
+    x2 = np.zeros_like(b2)
+    s.solve(b2, x2)
+    assert np.linalg.norm(A2 @ x2 - b2) < 1e-8
+
+
+def test_lazy_partitioning_reuses_partition(gpu, het12):
+    s = MASSolver()
+    s.set_parameters({"MAS": {"block_dim": 3, "lazy_partitioning": True, "aggregation": "graph"}})
+    s.factorize(het12.A)
+    off1, perm1 = s.partition()
+    A2 = het12.A * 2.0
+    s.factorize(A2)
+    off2, perm2 = s.partition()
+    assert np.array_equal(off1, off2) and np.array_equal(perm1, perm2)
+
+
+def test_inverse_failure_raises(gpu):
+    pattern = helpers.grid_laplacian_pattern(8, 8, 1)
+    A = helpers.prefactor_matrix(pattern, seed=1).tolil()
+    A[5, 5] = np.nan
+    s = MASSolver()
+    s.set_parameters({"MAS": {"block_dim": 1}})
+    with pytest.raises(MASError, match="inverse failed"):
+        s.factorize(A.tocsc())
+
+
+def test_solve_device_pointers(gpu, cfg1):
+    torch = pytest.importorskip("torch")
+    s = _factorized(cfg1, {"relative_tolerance": 1e-8, "absolute_tolerance": 0.0})
+    b = torch.tensor(cfg1.b, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    s.solve_device(b.data_ptr(), x.data_ptr(), cfg1.n)
+    xh = x.cpu().numpy()
+    x_host = np.zeros(cfg1.n)
+    s.solve(cfg1.b, x_host)
+    assert np.array_equal(xh, x_host)

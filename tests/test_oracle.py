@@ -1,0 +1,271 @@
+"""Pin the CPU oracle (oracle/mas_oracle.c) with constructions that do not share its code.
+
+The reference ships no golden vectors for MAS (its tests only assert ||Ax - b|| bounds on data
+that is not available here), so each oracle function is checked against an independent
+construction: dense numpy linear algebra, scipy graph routines, closed forms, and invariants.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csg
+import scipy.sparse.linalg as spla
+
+import oracle
+from polysolve_amd import problems
+from tests import helpers
+
+
+def _morton_partition(p):
+    return helpers.chunk_partition(helpers.morton_order(p.coords))
+
+
+@pytest.fixture(scope="module")
+def small_het():
+    return problems.hex_elasticity(5, E=problems.random_moduli(5, 11))
+
+
+@pytest.mark.parametrize("bd", [1, 2, 3])
+def test_bsr_reconstructs_permuted_matrix(bd):
+    pattern = helpers.grid_laplacian_pattern(7, 5, 3)  # 105 unknowns: 105 % 2 != 0
+    A = helpers.prefactor_matrix(pattern, seed=3)
+    n = A.shape[0]
+    nb = -(-n // bd)
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(nb).astype(np.int32)
+    offsets = np.array(list(range(0, nb, 32)) + [nb], dtype=np.int32)
+    O = oracle.MasOracle(A, bd, offsets, perm)
+    rows, cols, vals = O.bsr()
+    D = np.zeros((nb * bd, nb * bd))
+    for r in range(nb):
+        assert np.all(np.diff(cols[rows[r]:rows[r + 1]]) > 0)  # sorted, unique
+        for k in range(rows[r], rows[r + 1]):
+            c = cols[k]
+            D[r * bd:(r + 1) * bd, c * bd:(c + 1) * bd] = vals[k]
+    # Expected: P A P^T with the trailing padded scalars set to identity (BSRMatrix.cu:123-134).
+    Ad = np.zeros((nb * bd, nb * bd))
+    Ad[:n, :n] = A.toarray()
+    for t in range(n, nb * bd):
+        Ad[t, t] = 1.0
+    idx = np.empty(nb * bd, dtype=np.int64)
+    for old in range(nb):
+        for c in range(bd):
+            idx[perm[old] * bd + c] = old * bd + c
+    assert np.array_equal(D, Ad[np.ix_(idx, idx)])
+
+
+def test_coarse_space_matches_connected_components(small_het):
+    p = small_het
+    offsets, perm = _morton_partition(p)
+    O = oracle.MasOracle(p.A, 3, offsets, perm)
+    m, r2p = O.coarse_map()
+    nb = O.info["nb"]
+    # Block graph in permuted ids.
+    B = (sp.csr_matrix(np.abs(p.A.toarray()).reshape(nb, 3, nb, 3).sum(axis=(1, 3))) != 0).astype(np.int8)
+    iperm = np.empty(nb, dtype=np.int64)
+    iperm[perm] = np.arange(nb)
+    Bp = B[iperm][:, iperm].tocsr()
+    prev = r2p // 32  # level-0 grouping key: bank of the padded slot
+    ids_prev = None
+    for lv in range(O.info["level_num"]):
+        if lv == 0:
+            group = prev
+        else:
+            group = ids_prev // 32
+        # Components of the graph restricted to pairs in the same group.
+        coo = Bp.tocoo()
+        keep = group[coo.row] == group[coo.col]
+        G = sp.coo_matrix((np.ones(keep.sum()), (coo.row[keep], coo.col[keep])), shape=(nb, nb))
+        ncomp, lab = csg.connected_components(G, directed=False)
+        ids = m[:, lv]
+        # Same partition of the nodes (bijection between labels and ids).
+        pairs = np.unique(np.stack([lab, ids], axis=1), axis=0)
+        assert pairs.shape[0] == ncomp == np.unique(ids).size
+        # Ids are ordered by group, then by the lowest member position within the group.
+        first = {}
+        for node in np.argsort(r2p if lv == 0 else ids_prev, kind="stable"):
+            first.setdefault(ids[node], (group[node], (r2p if lv == 0 else ids_prev)[node]))
+        keys = [first[i] for i in range(ncomp)]
+        assert keys == sorted(keys)
+        assert O.info[f"cco{lv}"] == ncomp
+        ids_prev = ids
+    # Termination: the next level would merge nothing, or the level limit is reached.
+    L = O.info["level_num"]
+    if L < 4:
+        group = ids_prev // 32
+        coo = Bp.tocoo()
+        keep = group[coo.row] == group[coo.col]
+        G = sp.coo_matrix((np.ones(keep.sum()), (ids_prev[coo.row[keep]], ids_prev[coo.col[keep]])),
+                          shape=(ids_prev.max() + 1,) * 2)
+        assert csg.connected_components(G, directed=False)[0] == ids_prev.max() + 1
+
+
+def test_galerkin_and_inverse_match_dense(small_het):
+    p = small_het
+    offsets, perm = _morton_partition(p)
+    O = oracle.MasOracle(p.A, 3, offsets, perm)
+    fwd, inv = O.matrices()
+    m, r2p = O.coarse_map()
+    nb = O.info["nb"]
+    A = p.A.toarray()
+    iperm = np.empty(nb, dtype=np.int64)
+    iperm[perm] = np.arange(nb)
+    idx = (iperm[:, None] * 3 + np.arange(3)[None, :]).ravel()
+    Aq = A[np.ix_(idx, idx)]
+    N = 96
+    # Fine banks.
+    for bank in range(O.info["mc0"]):
+        F = helpers.unpack_upper(fwd[bank], N)
+        ref = np.eye(N)
+        slots = np.flatnonzero(r2p // 32 == bank)
+        loc = (r2p[slots] % 32)
+        for a, sa in zip(slots, loc):
+            for b, sb in zip(slots, loc):
+                ref[sa * 3:sa * 3 + 3, sb * 3:sb * 3 + 3] = Aq[a * 3:a * 3 + 3, b * 3:b * 3 + 3]
+        assert np.allclose(F, ref, rtol=1e-14, atol=1e-12 * np.abs(ref).max())
+        I = helpers.unpack_upper(inv[bank], N)
+        assert np.allclose(I, np.linalg.inv(ref), rtol=1e-9, atol=1e-9 * np.abs(I).max())
+    # Coarse levels: R A R^T on same-bank pairs.
+    for lv in range(O.info["level_num"]):
+        nc = O.info[f"cco{lv}"]
+        R = np.zeros((nc * 3, nb * 3))
+        for blk in range(nb):
+            for c in range(3):
+                R[m[blk, lv] * 3 + c, blk * 3 + c] = 1.0
+        Ac = R @ Aq @ R.T
+        base = sum(O.info[f"mc{i}"] for i in range(lv + 1))
+        for bank in range(O.info[f"mc{lv + 1}"]):
+            ids = np.arange(bank * 32, min(nc, bank * 32 + 32))
+            ref = np.eye(N)
+            sl = (ids[:, None] * 3 + np.arange(3)[None, :]).ravel()
+            ref[:sl.size, :sl.size] = Ac[np.ix_(sl, sl)]
+            F = helpers.unpack_upper(fwd[base + bank], N)
+            assert np.allclose(F, ref, rtol=1e-13, atol=1e-12 * np.abs(ref).max())
+
+
+def test_apply_matches_dense_operator_and_is_spd(small_het):
+    p = small_het
+    offsets, perm = _morton_partition(p)
+    O = oracle.MasOracle(p.A, 3, offsets, perm)
+    m, _ = O.coarse_map()
+    M = helpers.dense_mas(p.A.toarray(), 3, offsets, perm, m, O.info["level_num"])
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        r = rng.standard_normal(p.n)
+        assert np.allclose(O.apply(r), M @ r, rtol=1e-10, atol=1e-12 * np.abs(M @ r).max())
+    x, y = rng.standard_normal(p.n), rng.standard_normal(p.n)
+    assert abs(x @ O.apply(y) - y @ O.apply(x)) <= 1e-10 * abs(x @ O.apply(y))
+    assert np.linalg.eigvalsh((M + M.T) / 2).min() > 0
+
+
+def test_single_bank_level0_is_exact_inverse():
+    """One domain holding every node: the fine level alone is A^-1 (closed form)."""
+    pattern = helpers.grid_laplacian_pattern(5, 6, 1)  # 30 unknowns, bd = 1 -> one bank
+    A = helpers.prefactor_matrix(pattern, seed=9)
+    O = oracle.MasOracle(A, 1, np.array([0, 30], dtype=np.int32), np.arange(30, dtype=np.int32))
+    fwd, inv = O.matrices()
+    I = helpers.unpack_upper(inv[0], 32)[:30, :30]
+    assert np.allclose(I, np.linalg.inv(A.toarray()), rtol=1e-10, atol=1e-14)
+
+
+def test_spmv_matches_scipy(small_het):
+    p = small_het
+    offsets, perm = _morton_partition(p)
+    O = oracle.MasOracle(p.A, 3, offsets, perm)
+    x = np.random.default_rng(2).standard_normal(p.n)
+    assert np.allclose(O.spmv(x), p.A @ x, rtol=1e-13, atol=1e-13 * np.abs(p.A @ x).max())
+
+
+def test_pcg_schedule_and_statuses(small_het):
+    p = small_het
+    offsets, perm = _morton_partition(p)
+    O = oracle.MasOracle(p.A, 3, offsets, perm)
+    r = O.pcg(p.b, max_iter=5000, rel_tol=1e-9, abs_tol=0.0)
+    assert r["rc"] == 0 and r["status"] == 1 and r["iters"] % 10 == 0
+    xd = spla.spsolve(p.A.tocsc(), p.b)
+    assert np.linalg.norm(r["x"] - xd) <= 1e-6 * np.linalg.norm(xd)
+    assert np.linalg.norm(p.b - p.A @ r["x"]) <= 2e-9 * np.linalg.norm(p.b)
+    # max_iter reached -> status 3 (reported even when the last check also converged).
+    r1 = O.pcg(p.b, max_iter=7, rel_tol=1e-9, abs_tol=0.0)
+    assert r1["status"] == 3 and r1["iters"] == 7
+    # Absolute tolerance hit first -> status 2.
+    r2 = O.pcg(p.b, max_iter=5000, rel_tol=0.0, abs_tol=1e3 * np.linalg.norm(p.b))
+    assert r2["status"] == 2 and r2["iters"] == 10
+    # NaN right-hand side -> invalid initial residual.
+    bad = p.b.copy()
+    bad[3] = np.nan
+    assert O.pcg(bad, max_iter=10)["rc"] == 4
+
+
+def test_pcg_iterates_match_textbook_pcg_with_dense_operator(small_het):
+    """Same iterates as a plain PCG (Shewchuk) with the dense MAS operator and true-residual
+    replacement every 4th step: the schedule, not just the end point, is pinned."""
+    p = small_het
+    offsets, perm = _morton_partition(p)
+    O = oracle.MasOracle(p.A, 3, offsets, perm)
+    m, _ = O.coarse_map()
+    M = helpers.dense_mas(p.A.toarray(), 3, offsets, perm, m, O.info["level_num"])
+    A = p.A.toarray()
+    b = p.b
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = M @ r
+    d = z.copy()
+    rz = r @ z
+    # CG iterates are sensitive to summation order (differences grow from 1e-15 at step 12 to
+    # ~1e-5 by step 30 on this system), so the schedule is pinned over the first 12 steps.
+    for k in range(1, 13):
+        q = A @ d
+        a = rz / (d @ q)
+        x = x + a * d
+        r = b - A @ x if k % 4 == 0 else r - a * q
+        z = M @ r
+        rz_new = r @ z
+        d = z + (rz_new / rz) * d
+        rz = rz_new
+    res = O.pcg(b, max_iter=12, rel_tol=0.0, abs_tol=0.0)
+    assert np.linalg.norm(res["x"] - x) <= 1e-12 * np.linalg.norm(x)
+
+
+def test_jacobi_cg_closed_forms():
+    # Diagonal A: the Jacobi-preconditioned direction is exact, Eigen's loop stops at i = 0.
+    A = sp.diags(np.linspace(1.0, 50.0, 40)).tocsc()
+    b = np.random.default_rng(3).standard_normal(40)
+    x, it, err = oracle.jacobi_cg(A, b, tol=1e-12, max_iter=100)
+    assert it == 0 and np.allclose(x, b / A.diagonal(), rtol=1e-14)
+    # D^-1/2 A D^-1/2 with 3 distinct eigenvalues: CG terminates within 3 steps.
+    rng = np.random.default_rng(4)
+    Q, _ = np.linalg.qr(rng.standard_normal((30, 30)))
+    lam = np.repeat([1.0, 4.0, 9.0], 10)
+    S = Q @ np.diag(lam) @ Q.T
+    dsc = rng.uniform(1.0, 3.0, 30)
+    Ad = np.diag(dsc) @ S @ np.diag(dsc)
+    # Jacobi scaling of Ad is diag(Ad); make the diagonal of S unit so D^-1/2 Ad D^-1/2 = S'.
+    A2 = sp.csc_matrix(Ad)
+    x2, it2, err2 = oracle.jacobi_cg(A2, rng.standard_normal(30), tol=1e-10, max_iter=200)
+    Dh = np.diag(1 / np.sqrt(np.diag(Ad)))
+    distinct = np.unique(np.round(np.linalg.eigvalsh(Dh @ Ad @ Dh), 8)).size
+    assert it2 <= distinct and err2 < 1e-10
+
+
+def test_jacobi_cg_solves_elasticity():
+    p = problems.hex_elasticity(4, E=1e5)
+    x, it, err = oracle.jacobi_cg(p.A, p.b, tol=1e-10, max_iter=2000)
+    assert err < 1e-10
+    assert np.linalg.norm(p.A @ x - p.b) <= 1.01e-10 * np.linalg.norm(p.b)
+
+
+def test_invalid_partition_is_rejected():
+    pattern = helpers.grid_laplacian_pattern(10, 10, 1)
+    A = helpers.prefactor_matrix(pattern, seed=2)
+    O = oracle.MasOracle(A, 1, np.array([0, 40, 100], dtype=np.int32), np.arange(100, dtype=np.int32))
+    assert O.status == 2  # a part larger than 32 nodes
+
+
+def test_zero_pivot_fails_inverse():
+    pattern = helpers.grid_laplacian_pattern(6, 6, 1)
+    A = helpers.prefactor_matrix(pattern, seed=2).tolil()
+    A[0, 0] = np.nan
+    O = oracle.MasOracle(A.tocsc(), 1, np.array([0, 32, 36], dtype=np.int32), np.arange(36, dtype=np.int32))
+    assert O.status == 1
