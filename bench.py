@@ -61,14 +61,14 @@ def make_problem(config: int):
     raise SystemExit(f"config {config} is not a single-system bench workload")
 
 
-def run_path(solver, b_dev, x_dev, n, steps, warmup, torch):
+def run_path(solver, b_dev, x_dev, n, steps, warmup, sync):
     for _ in range(warmup):
-        solver.solve_device(b_dev.data_ptr(), x_dev.data_ptr(), n)
-    torch.cuda.synchronize()
+        solver.solve_device(b_dev.ptr, x_dev.ptr, n)
+    sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        solver.solve_device(b_dev.data_ptr(), x_dev.data_ptr(), n)
-    torch.cuda.synchronize()
+        solver.solve_device(b_dev.ptr, x_dev.ptr, n)
+    sync()
     return (time.perf_counter() - t0) * 1e3 / max(steps, 1)
 
 
@@ -81,6 +81,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=30, help="oracle PCG iterations in the CPU sample")
     ap.add_argument("--aggregation", default="morton", choices=["morton", "graph"])
+    ap.add_argument("--coarse", default="cholesky", choices=["cholesky", "bank"],
+                    help="coarsest-level solver of the mixed path")
     ap.add_argument("--profile-only", action="store_true", help="time steps only (for rocprof runs)")
     args = ap.parse_args()
 
@@ -89,8 +91,8 @@ def main() -> int:
                                    "(see DESIGN.md), not a bench mode", "n_gpus": args.gpus}))
         return 2
 
-    import torch
     from polysolve_amd import MASSolver
+    from polysolve_amd.linear import DeviceArray, synchronize
 
     log(f"assembling config {args.config}")
     t = time.perf_counter()
@@ -109,32 +111,32 @@ def main() -> int:
     levels = solver.level_info()
     log(f"  factorize {setup_ms:.0f} ms, levels {levels}")
 
-    b_dev = torch.tensor(prob.b, dtype=torch.float64, device="cuda")
-    x_dev = torch.zeros_like(b_dev)
+    b_dev = DeviceArray(prob.n, prob.b)
+    x_dev = DeviceArray(prob.n, np.zeros(prob.n))
     n = prob.n
     bnorm = float(np.linalg.norm(prob.b))
 
     def accuracy():
-        x = x_dev.cpu().numpy()
+        x = x_dev.download()
         rel = float(np.linalg.norm(prob.b - prob.A @ x) / bnorm)
         return x, rel
 
     # FP64 reference path.
     log("fp64 path")
-    fp64_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, torch)
+    fp64_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
     info64 = solver.get_info()
     x64, rel64 = accuracy()
     log(f"  fp64 {fp64_ms:.2f} ms/solve, {info64['solver_iter']} it, true rel res {rel64:.2e}")
     if args.profile_only:
-        solver.set_parameters({"MAS": {"precision": "mixed"}})
-        mixed_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, torch)
+        solver.set_parameters({"MAS": {"precision": "mixed", "coarse_solver": args.coarse}})
+        mixed_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
         log(f"  mixed {mixed_ms:.2f} ms/solve")
         return 0
 
     # Mixed path (headline).
-    solver.set_parameters({"MAS": {"precision": "mixed"}})
+    solver.set_parameters({"MAS": {"precision": "mixed", "coarse_solver": args.coarse}})
     log("mixed path")
-    mixed_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, torch)
+    mixed_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
     infom = solver.get_info()
     xm, relm = accuracy()
     log(f"  mixed {mixed_ms:.2f} ms/solve, {infom['solver_iter']} it, {infom['reliable_updates']} updates, "
@@ -142,25 +144,25 @@ def main() -> int:
 
     # Accuracy reference: FP64 MAS-PCG driven to 1e-12 (no direct FP64 factorisation of an 824k
     # DOF 3-D system is available on this machine).
-    solver.set_parameters({"MAS": {"precision": "fp64", "relative_tolerance": 1e-12}})
-    solver.solve_device(b_dev.data_ptr(), x_dev.data_ptr(), n)
-    torch.cuda.synchronize()
+    solver.set_parameters({"MAS": {"precision": "fp64", "coarse_solver": "bank", "relative_tolerance": 1e-12}})
+    solver.solve_device(b_dev.ptr, x_dev.ptr, n)
+    synchronize()
     xref, relref = accuracy()
     err_m = float(np.linalg.norm(xm - xref) / np.linalg.norm(xref))
     err_64 = float(np.linalg.norm(x64 - xref) / np.linalg.norm(xref))
     solver.set_parameters({"MAS": {"relative_tolerance": tol}})
 
     # Roofline: dominant kernel of the mixed path, timed with HIP events on the solver stream.
-    solver.set_parameters({"MAS": {"precision": "mixed"}})
+    solver.set_parameters({"MAS": {"precision": "mixed", "coarse_solver": args.coarse}})
     solver.set_kernel_timing(True)
-    solver.solve_device(b_dev.data_ptr(), x_dev.data_ptr(), n)
-    torch.cuda.synchronize()
+    solver.solve_device(b_dev.ptr, x_dev.ptr, n)
+    synchronize()
     kt = solver.kernel_timing()
     solver.set_kernel_timing(False)
-    solver.set_parameters({"MAS": {"precision": "fp64"}})
+    solver.set_parameters({"MAS": {"precision": "fp64", "coarse_solver": "bank"}})
     solver.set_kernel_timing(True)
-    solver.solve_device(b_dev.data_ptr(), x_dev.data_ptr(), n)
-    torch.cuda.synchronize()
+    solver.solve_device(b_dev.ptr, x_dev.ptr, n)
+    synchronize()
     kt64 = solver.kernel_timing()
     solver.set_kernel_timing(False)
     kernels = {}

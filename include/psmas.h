@@ -91,8 +91,52 @@ int psmas_solve_device(psmas_handle h, int n, const double *d_b, double *d_x);
  * (1..3) doubles, one per block node (block_dim scalars).  Takes effect at the next factorize. */
 int psmas_set_coordinates(psmas_handle h, int nb, int dim, const double *xyz);
 
+/* Batches of independent systems (no counterpart in the reference): nsys systems are factorised
+ * together and solved in one launch sequence; each keeps its own CG scalars, convergence test and
+ * status.  Arrays of nsys pointers / sizes; each system as in psmas_factorize_csc.  After a
+ * batch factorisation psmas_solve / psmas_get_info refer to the batch only when nsys == 1. */
+int psmas_batch_factorize_csc(psmas_handle h, int nsys, const int *n, const long long *nnz,
+                              const int *const *colptr, const int *const *rowidx, const double *const *vals);
+/* b[s], x[s]: host arrays (device == 0) or device pointers (device != 0) of length n[s]. */
+int psmas_batch_solve(psmas_handle h, int nsys, const double *const *b, double *const *x, int device);
+/* get_info of system sys of the last batch solve. */
+int psmas_batch_get_info(psmas_handle h, int sys, char *buf, size_t cap, size_t *needed);
+/* Coordinates of system sys (before psmas_batch_factorize_csc). */
+int psmas_batch_set_coordinates(psmas_handle h, int sys, int nb, int dim, const double *xyz);
+
 /* Number of visible GPUs (0 on a machine without one). */
 int psmas_device_count(void);
+
+/* Device memory for callers without their own GPU allocator (benchmarks, tests): plain
+ * hipMalloc / hipFree / hipMemcpy on the solver's device.  kind: 1 host->device, 2 device->host,
+ * 3 device->device. */
+int psmas_device_alloc(size_t bytes, void **ptr);
+int psmas_device_free(void *ptr);
+int psmas_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int psmas_device_synchronize(void);
+
+/* ---- Learned coarse-correction network (no counterpart in the reference) ------------------ */
+
+/* MLP d_in -> hidden -> hidden -> d_out (ReLU), BF16-operand MFMA GEMMs with FP32 accumulation,
+ * FP32 master weights trained by Adam on the GPU.  Parameter vector (FP32):
+ * [W1 (hidden x d_in) | b1 | W2 (hidden x hidden) | b2 | W3 (d_out x hidden) | b3], row-major. */
+typedef struct psmas_mlp *psmas_mlp_handle;
+int psmas_mlp_create(int d_in, int d_out, int hidden, unsigned seed, psmas_mlp_handle *out);
+int psmas_mlp_destroy(psmas_mlp_handle h);
+const char *psmas_mlp_last_error(psmas_mlp_handle h);
+long long psmas_mlp_num_parameters(psmas_mlp_handle h);
+int psmas_mlp_set_parameters(psmas_mlp_handle h, const float *theta);
+int psmas_mlp_get_parameters(psmas_mlp_handle h, float *theta);
+/* Y[batch x d_out] = f(X[batch x d_in]) (host arrays). */
+int psmas_mlp_forward(psmas_mlp_handle h, int batch, const float *X, float *Y);
+/* Mean squared error against T and its gradient w.r.t. the parameter vector (host arrays). */
+int psmas_mlp_loss_grad(psmas_mlp_handle h, int batch, const float *X, const float *T, double *loss, float *grad);
+/* Adam training on n samples (host arrays); *final_loss = mean loss of the last epoch. */
+int psmas_mlp_train(psmas_mlp_handle h, int n, const float *X, const float *T, int epochs, int batch, float lr,
+                    unsigned seed, double *final_loss);
+/* C[M x N] = op(A) B with BF16-rounded operands and FP32 accumulation (the MLP's GEMM);
+ * trans_a: A stored K x M. */
+int psmas_gemm_bf16(int M, int N, int K, int trans_a, const float *A, const float *B, float *C);
 
 /* Parse parameters without a device (checks the JSON contract): writes the resulting parameter
  * set as JSON. */

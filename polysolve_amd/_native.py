@@ -44,7 +44,28 @@ SIGNATURES = {
     "psmas_last_error": (_cp, [_vp]),
     "psmas_solve_device": (_i, [_vp, _i, _vp, _vp]),
     "psmas_set_coordinates": (_i, [_vp, _i, _i, _pd]),
+    "psmas_batch_factorize_csc": (_i, [_vp, _i, _pi, C.POINTER(_ll), C.POINTER(_pi), C.POINTER(_pi),
+                                       C.POINTER(_pd)]),
+    "psmas_batch_solve": (_i, [_vp, _i, C.POINTER(_pd), C.POINTER(_pd), _i]),
+    "psmas_batch_get_info": (_i, [_vp, _i, C.c_char_p, _sz, C.POINTER(_sz)]),
+    "psmas_batch_set_coordinates": (_i, [_vp, _i, _i, _i, _pd]),
     "psmas_device_count": (_i, []),
+    "psmas_device_alloc": (_i, [_sz, C.POINTER(_vp)]),
+    "psmas_device_free": (_i, [_vp]),
+    "psmas_memcpy": (_i, [_vp, _vp, _sz, _i]),
+    "psmas_device_synchronize": (_i, []),
+    "psmas_mlp_create": (_i, [_i, _i, _i, C.c_uint, C.POINTER(_vp)]),
+    "psmas_mlp_destroy": (_i, [_vp]),
+    "psmas_mlp_last_error": (_cp, [_vp]),
+    "psmas_mlp_num_parameters": (_ll, [_vp]),
+    "psmas_mlp_set_parameters": (_i, [_vp, C.POINTER(C.c_float)]),
+    "psmas_mlp_get_parameters": (_i, [_vp, C.POINTER(C.c_float)]),
+    "psmas_mlp_forward": (_i, [_vp, _i, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "psmas_mlp_loss_grad": (_i, [_vp, _i, C.POINTER(C.c_float), C.POINTER(C.c_float), _pd,
+                                 C.POINTER(C.c_float)]),
+    "psmas_mlp_train": (_i, [_vp, _i, C.POINTER(C.c_float), C.POINTER(C.c_float), _i, _i, C.c_float, C.c_uint,
+                             _pd]),
+    "psmas_gemm_bf16": (_i, [_i, _i, _i, _i, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "psmas_parse_parameters": (_i, [_cp, C.c_char_p, _sz, C.POINTER(_sz)]),
     "psmas_get_level_info": (_i, [_vp, _pi, _i]),
     "psmas_get_partition": (_i, [_vp, _pi, _pi, _i, _pi, _i]),

@@ -22,8 +22,8 @@ INCLUDE = HERE.parent / "include"
 BUILD_DIR = HERE.parent / "_build" / "psmas"
 ARCH = os.environ.get("PSMAS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["spmv_vec.hip", "mas_kernels.hip", "solver.hip"]
-CXX_SOURCES = ["host_setup.cpp", "params.cpp", "capi.cpp"]
+HIP_SOURCES = ["spmv_vec.hip", "mas_kernels.hip", "coarse_solve.hip", "mlp.hip", "solver.hip"]
+CXX_SOURCES = ["host_setup.cpp", "params.cpp", "capi.cpp", "mlp_capi.cpp"]
 
 # Packed-FP32 VALU forms are disabled for every HIP translation unit (hipcc warns that the host
 # pass does not know the feature; that warning is harmless).

@@ -164,7 +164,59 @@ extern "C"
     {
         if (!h)
             return null_handle();
-        return guarded(h->last_error, [&] { h->solver.set_coordinates(nb, dim, xyz); });
+        return guarded(h->last_error, [&] { h->solver.set_coordinates(0, nb, dim, xyz); });
+    }
+
+    int psmas_batch_set_coordinates(psmas_handle h, int sys, int nb, int dim, const double *xyz)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.set_coordinates(sys, nb, dim, xyz); });
+    }
+
+    int psmas_batch_factorize_csc(psmas_handle h, int nsys, const int *n, const long long *nnz,
+                                  const int *const *colptr, const int *const *rowidx, const double *const *vals)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            if (nsys <= 0 || !n || !nnz || !colptr || !rowidx || !vals)
+                throw psmas::InvalidArgumentError("[MAS] invalid batch arguments.");
+            std::vector<psmas::CscView> As(nsys);
+            for (int s = 0; s < nsys; ++s)
+            {
+                As[s].n = n[s];
+                As[s].nnz = nnz[s];
+                As[s].colptr = colptr[s];
+                As[s].rowidx = rowidx[s];
+                As[s].vals = vals[s];
+            }
+            h->solver.factorize_batch(As);
+        });
+    }
+
+    int psmas_batch_solve(psmas_handle h, int nsys, const double *const *b, double *const *x, int device)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            if (nsys != h->solver.num_systems() || !b || !x)
+                throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
+            std::vector<const double *> bv(b, b + nsys);
+            std::vector<double *> xv(x, x + nsys);
+            h->solver.solve_batch(bv, xv, device != 0);
+        });
+    }
+
+    int psmas_batch_get_info(psmas_handle h, int sys, char *buf, size_t cap, size_t *needed)
+    {
+        if (!h)
+            return null_handle();
+        std::string s;
+        const int rc = guarded(h->last_error, [&] { s = h->solver.system_info_json(sys); });
+        if (rc != PSMAS_OK)
+            return rc;
+        return write_string(s, buf, cap, needed);
     }
 
     int psmas_device_count(void)
@@ -176,6 +228,34 @@ extern "C"
             return 0;
         }
         return c;
+    }
+
+    int psmas_device_alloc(size_t bytes, void **ptr)
+    {
+        if (!ptr)
+            return PSMAS_ERROR_INVALID_ARGUMENT;
+        *ptr = nullptr;
+        return guarded(g_create_error, [&] { *ptr = psmas::device_alloc(bytes, "caller buffer"); });
+    }
+
+    int psmas_device_free(void *ptr)
+    {
+        psmas::device_free(ptr);
+        return PSMAS_OK;
+    }
+
+    int psmas_memcpy(void *dst, const void *src, size_t bytes, int kind)
+    {
+        return guarded(g_create_error, [&] {
+            const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                                              : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+            PSMAS_HIP(hipMemcpy(dst, src, bytes, k));
+        });
+    }
+
+    int psmas_device_synchronize(void)
+    {
+        return guarded(g_create_error, [&] { PSMAS_HIP(hipDeviceSynchronize()); });
     }
 
     int psmas_parse_parameters(const char *json, char *buf, size_t cap, size_t *needed)

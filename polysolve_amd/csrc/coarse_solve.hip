@@ -1,0 +1,167 @@
+// Exact FP64 solve on the coarsest MAS level.
+//
+// The reference's coarsest level is bank-block-diagonal like every other level
+// (src/polysolve/linear/mas_utils/MASPreconditioner.cu:939-979): coarse nodes in different
+// 32-node banks never couple.  Here the coarsest level's *active* nodes — those containing at
+// least one fine node with an off-diagonal block (Dirichlet rows and other isolated nodes form
+// singleton components that only ever see their own diagonal) — get their full Galerkin matrix
+// R A R^T, factorised by a dense FP64 Cholesky, and its inverse is applied as one dense GEMV.
+// Isolated coarsest nodes keep their bank treatment.  The operator stays symmetric positive
+// definite, and it adds the global coarse solve that the additive bank hierarchy lacks.
+#include "coarse_solve.hpp"
+#include "common.hpp"
+#include "device.hpp"
+
+#include <cmath>
+#include <stdexcept>
+
+namespace psmas
+{
+    namespace
+    {
+        __device__ __forceinline__ int cmap_level(const int4 &c, int l)
+        {
+            return l == 0 ? c.x : (l == 1 ? c.y : (l == 2 ? c.z : c.w));
+        }
+
+        /// dense[(3a+bi)*nd + 3b+bj] += A_ij[bi][bj] for every stored block (i, j) whose coarsest
+        /// ids map to active indices a, b.  Both triangles are accumulated (no symmetry shortcut).
+        template <int BD>
+        __global__ void k_dense_galerkin(int npad, int level, SellView<double> A, const int *__restrict__ row_len,
+                                         const int *__restrict__ block_of_slot, const int4 *__restrict__ cmap,
+                                         const int *__restrict__ active_index, int nd, double *__restrict__ dense)
+        {
+            const int i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i >= npad || block_of_slot[i] < 0)
+                return;
+            const int a = active_index[cmap_level(cmap[i], level)];
+            if (a < 0)
+                return;
+            const int slice = i / WAVE, lane = i % WAVE;
+            const int beg = A.slice_ptr[slice];
+            const int len = row_len[i];
+            for (int k = 0; k < len; ++k)
+            {
+                const size_t slot = static_cast<size_t>(beg + k);
+                const int j = A.cols[slot * WAVE + lane];
+                const int b = active_index[cmap_level(cmap[j], level)];
+                if (b < 0)
+                    continue;
+                for (int bi = 0; bi < BD; ++bi)
+                    for (int bj = 0; bj < BD; ++bj)
+                        atomicAdd(dense + static_cast<size_t>(BD * a + bi) * nd + BD * b + bj,
+                                  A.vals[(slot * BD * BD + bi * BD + bj) * WAVE + lane]);
+            }
+        }
+
+        /// y[active] = Inv x[active] on the coarsest segment: one wave per dense row, the
+        /// restricted vector staged through the cache.  Rows of inactive nodes are not touched.
+        template <int BD, typename T>
+        __global__ __launch_bounds__(256) void k_dense_apply(int nd, const double *__restrict__ inv,
+                                                             const int *__restrict__ active_ids,
+                                                             const T *__restrict__ xseg, T *__restrict__ yseg)
+        {
+            const int row = blockIdx.x * 4 + threadIdx.x / WAVE;
+            const int lane = threadIdx.x % WAVE;
+            if (row >= nd)
+                return;
+            const double *a = inv + static_cast<size_t>(row) * nd;
+            double s = 0.0;
+            for (int c = lane; c < nd; c += WAVE)
+            {
+                const int node = active_ids[c / BD];
+                s += a[c] * static_cast<double>(xseg[static_cast<size_t>(node) * BD + c % BD]);
+            }
+            s = wave_sum(s);
+            if (lane == 0)
+            {
+                const int node = active_ids[row / BD];
+                yseg[static_cast<size_t>(node) * BD + row % BD] = static_cast<T>(s);
+            }
+        }
+    } // namespace
+
+    void launch_dense_galerkin(int bd, int npad, int level, const SellView<double> &A, const int *row_len,
+                               const int *block_of_slot, const int4 *cmap, const int *active_index, int nd,
+                               double *dense, hipStream_t s)
+    {
+        const dim3 grid(div_up(npad, 128)), block(128);
+        if (bd == 1)
+            hipLaunchKernelGGL(k_dense_galerkin<1>, grid, block, 0, s, npad, level, A, row_len, block_of_slot, cmap,
+                               active_index, nd, dense);
+        else if (bd == 2)
+            hipLaunchKernelGGL(k_dense_galerkin<2>, grid, block, 0, s, npad, level, A, row_len, block_of_slot, cmap,
+                               active_index, nd, dense);
+        else
+            hipLaunchKernelGGL(k_dense_galerkin<3>, grid, block, 0, s, npad, level, A, row_len, block_of_slot, cmap,
+                               active_index, nd, dense);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD, typename T>
+    void launch_dense_apply(int nd, const double *inv, const int *active_ids, const T *xseg, T *yseg, hipStream_t s)
+    {
+        if (nd <= 0)
+            return;
+        hipLaunchKernelGGL((k_dense_apply<BD, T>), dim3(div_up(nd, 4)), dim3(256), 0, s, nd, inv, active_ids, xseg,
+                           yseg);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    bool cholesky_inverse(int n, std::vector<double> &A)
+    {
+        // A = L L^T (lower, in place), then Inv = L^-T L^-1, returned full and symmetric.
+        std::vector<double> L(A);
+        for (int j = 0; j < n; ++j)
+        {
+            double d = L[static_cast<size_t>(j) * n + j];
+            for (int k = 0; k < j; ++k)
+                d -= L[static_cast<size_t>(j) * n + k] * L[static_cast<size_t>(j) * n + k];
+            if (!(d > 0.0) || !std::isfinite(d))
+                return false;
+            d = std::sqrt(d);
+            L[static_cast<size_t>(j) * n + j] = d;
+            for (int i = j + 1; i < n; ++i)
+            {
+                double v = L[static_cast<size_t>(i) * n + j];
+                for (int k = 0; k < j; ++k)
+                    v -= L[static_cast<size_t>(i) * n + k] * L[static_cast<size_t>(j) * n + k];
+                L[static_cast<size_t>(i) * n + j] = v / d;
+            }
+        }
+        // W = L^-1 (lower triangular), column by column.
+        std::vector<double> W(static_cast<size_t>(n) * n, 0.0);
+        for (int c = 0; c < n; ++c)
+        {
+            W[static_cast<size_t>(c) * n + c] = 1.0 / L[static_cast<size_t>(c) * n + c];
+            for (int i = c + 1; i < n; ++i)
+            {
+                double v = 0.0;
+                for (int k = c; k < i; ++k)
+                    v -= L[static_cast<size_t>(i) * n + k] * W[static_cast<size_t>(k) * n + c];
+                W[static_cast<size_t>(i) * n + c] = v / L[static_cast<size_t>(i) * n + i];
+            }
+        }
+        // Inv = W^T W.
+        for (int i = 0; i < n; ++i)
+            for (int j = i; j < n; ++j)
+            {
+                double v = 0.0;
+                for (int k = j; k < n; ++k)
+                    v += W[static_cast<size_t>(k) * n + i] * W[static_cast<size_t>(k) * n + j];
+                A[static_cast<size_t>(i) * n + j] = v;
+                A[static_cast<size_t>(j) * n + i] = v;
+            }
+        return true;
+    }
+
+#define PSMAS_INST_DENSE(BD)                                                                                       \
+    template void launch_dense_apply<BD, double>(int, const double *, const int *, const double *, double *,      \
+                                                 hipStream_t);                                                    \
+    template void launch_dense_apply<BD, float>(int, const double *, const int *, const float *, float *,         \
+                                                hipStream_t);
+    PSMAS_INST_DENSE(1)
+    PSMAS_INST_DENSE(2)
+    PSMAS_INST_DENSE(3)
+#undef PSMAS_INST_DENSE
+} // namespace psmas

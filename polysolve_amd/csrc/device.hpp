@@ -8,6 +8,8 @@
 
 #include <cstdint>
 
+#include "kernels.hpp"
+
 namespace psmas
 {
     constexpr int WAVE = 64;
@@ -103,6 +105,37 @@ namespace psmas
         {
             epilogue(t);
             __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+
+    /// Per-system deterministic sum over the node blocks of system `sys` (the workgroup's own).
+    /// The system's last workgroup to arrive sums the partials of its blocks in index order and
+    /// calls epilogue(total) from thread 0.
+    template <int BLOCK, typename Epilogue>
+    __device__ __forceinline__ void system_sum(double v, int sys, const SysMap &sm, double *partials,
+                                               unsigned *counters, Epilogue epilogue)
+    {
+        __shared__ bool am_last;
+        const double s = block_sum<BLOCK>(v);
+        if (threadIdx.x == 0)
+        {
+            __hip_atomic_store(partials + blockIdx.x, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned prev =
+                __hip_atomic_fetch_add(counters + sys, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            am_last = (prev == static_cast<unsigned>(sm.blk_count[sys]) - 1u);
+        }
+        __syncthreads();
+        if (!am_last)
+            return;
+        const int b0 = sm.blk_begin[sys], nb = sm.blk_count[sys];
+        double t = 0.0;
+        for (int i = threadIdx.x; i < nb; i += BLOCK)
+            t += __hip_atomic_load(partials + b0 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t = block_sum<BLOCK>(t);
+        if (threadIdx.x == 0)
+        {
+            epilogue(t);
+            __hip_atomic_store(counters + sys, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 

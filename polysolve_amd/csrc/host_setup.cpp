@@ -331,7 +331,7 @@ namespace psmas
         const int padded_scalars = bd * nb - n;
         out.bd = bd;
         out.nb = nb;
-        out.nparts = part.nparts + (part.nparts & 1);
+        out.nparts = div_up(std::max(part.nparts, 1), 8) * 8; // whole 256-node blocks
         out.npad = out.nparts * BANK;
         out.slot_of_block.assign(nb, -1);
         out.block_of_slot.assign(out.npad, -1);

@@ -58,14 +58,15 @@ namespace psmas
     Partition chunk_partition(const std::vector<int> &order, int max_size);
 
     /// Bank-padded BSR in the solver's working space: node slot = 32*part + rank in part; the
-    /// part count is rounded up to an even number so that slices of 64 rows hold two whole banks.
+    /// part count is rounded up to a multiple of 8 so the system covers whole 256-node blocks
+    /// (four SELL slices of 64 rows, each holding two whole banks).
     /// Rows of virtual slots are empty.  Blocks are row-major bd x bd, columns sorted; the
     /// trailing partial block gets 1.0 on its padded diagonal (BSRMatrix.cu:123-134,189-191).
     struct PaddedBsr
     {
         int bd = 0;
         int nb = 0;        ///< input block rows
-        int nparts = 0;    ///< even
+        int nparts = 0;    ///< multiple of 8
         int npad = 0;      ///< 32 * nparts
         std::vector<int> slot_of_block; ///< input block -> slot
         std::vector<int> block_of_slot; ///< slot -> input block, -1 virtual

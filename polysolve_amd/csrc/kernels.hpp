@@ -9,6 +9,10 @@
 
 namespace psmas
 {
+    /// Nodes per workgroup of the node-blocked kernels.  Every system of a batch occupies a whole
+    /// number of these blocks, so a workgroup never straddles two systems.
+    constexpr int NODE_BLOCK = 256;
+
     /// SELL-64 block matrix: rows (bank slots) in slices of 64; slot k of lane l holds block k of
     /// row 64*s + l.  cols[(slice_ptr[s] + k) * 64 + l]; vals[((slice_ptr[s] + k) * bd*bd + e) * 64 + l].
     /// Short rows are padded with zero blocks whose column is the row itself.
@@ -30,53 +34,62 @@ namespace psmas
         const int *gid = nullptr;
     };
 
+    /// Partials (one per workgroup) and arrival counters (one per system) of a reduction.
     struct ReduceSite
     {
         double *partials = nullptr;
         unsigned *counter = nullptr;
     };
 
-    enum class SpmvMode
+    /// Systems of a batch (a single solve is a batch of one): node block -> system, each
+    /// system's block range, and per-system flags written by the host between graph launches.
+    struct SysMap
     {
-        AP,  ///< y = A x; pAp = x . y; alpha = rz / pAp
-        RES, ///< y = b - A x; rr = y . y into a chosen slot; restrict y to coarse level 0
+        const int *block_sys = nullptr; ///< node block -> system
+        const int *blk_begin = nullptr; ///< system -> first node block
+        const int *blk_count = nullptr; ///< system -> node blocks
+        const int *active = nullptr;    ///< system -> 1 while it iterates
+        const int *select = nullptr;    ///< system -> 1 when picked for a reliable update
+        const int *bank_sys = nullptr;  ///< fine bank -> system
     };
 
     // ----- SpMV ---------------------------------------------------------------------------
-    /// y = A x (+ pAp and alpha).  TV matrix value type, TX vector type.
+    /// y = A x (+ per-system pAp and alpha).  TV matrix value type, TX vector type.
     template <int BD, typename TV, typename TX>
-    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, double *sc, ReduceSite site,
+    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, double *sc, const SysMap &sm, ReduceSite site,
                         hipStream_t s);
 
-    /// y = b - A x, rr = y.y -> sc[rr_slot]; restrict y -> yc0.  TX: x,b; TY: y.
+    /// y = b - A x, rr = y.y -> sc[sys][rr_slot]; restrict y -> yc0.  TX: x,b; TY: y.
+    /// selected_only: only systems with select == 1.
     template <int BD, typename TV, typename TX, typename TY>
     void launch_spmv_res(const SellView<TV> &A, const TX *x, const TX *b, TY *y, FineRestrict fr, TY *yc0,
-                         double *sc, int rr_slot, ReduceSite site, hipStream_t s);
+                         double *sc, int rr_slot, const SysMap &sm, ReduceSite site, bool selected_only, hipStream_t s);
 
-    /// Plain y = A x without reductions (debug / parity path).
+    /// Plain y = A x without reductions (introspection path).
     template <int BD, typename TV, typename TX>
     void launch_spmv_plain(const SellView<TV> &A, const TX *x, TX *y, hipStream_t s);
 
-    // ----- vectors ------------------------------------------------------------------------
-    /// x += alpha p; r -= alpha Ap; rr = r.r; restrict r -> rc0.  One thread per node.
+    // ----- vectors (node-blocked: nblocks workgroups of NODE_BLOCK nodes) ------------------
+    /// x += alpha p; r -= alpha Ap; rr = r.r; restrict r -> rc0.
     template <int BD, typename T>
-    void launch_update_xr(int nnodes, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0, double *sc,
-                          ReduceSite site, hipStream_t s);
+    void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0, double *sc,
+                          const SysMap &sm, ReduceSite site, hipStream_t s);
 
-    /// x += alpha p (scalars).
-    template <typename T>
-    void launch_axpy_alpha(long long n, const T *p, T *x, const double *sc, hipStream_t s);
+    /// x += alpha p.
+    template <int BD, typename T>
+    void launch_axpy_alpha(int nblocks, const T *p, T *x, const double *sc, const SysMap &sm, hipStream_t s);
 
-    /// p = z + beta p (scalars).
-    template <typename T>
-    void launch_update_p(long long n, const T *z, T *p, const double *sc, hipStream_t s);
+    /// p = z + beta p.
+    template <int BD, typename T>
+    void launch_update_p(int nblocks, const T *z, T *p, const double *sc, const SysMap &sm, hipStream_t s);
 
-    /// dst = src (elementwise, with conversion).
+    /// x64 += x32; x32 = 0 for systems selected for a reliable update.
+    template <int BD>
+    void launch_accumulate_correction(int nblocks, float *x32, double *x64, const SysMap &sm, hipStream_t s);
+
+    // ----- plain element-wise helpers ------------------------------------------------------
     template <typename TS, typename TD>
     void launch_copy_convert(long long n, const TS *src, TD *dst, hipStream_t s);
-
-    /// x64 += x32; x32 = 0.
-    void launch_accumulate_correction(long long n, float *x32, double *x64, hipStream_t s);
 
     /// Fill with a value (used instead of memset inside captured graphs).
     template <typename T>
@@ -90,18 +103,18 @@ namespace psmas
     template <typename T>
     void launch_scatter_out(int nb, int bd, int n, const int *slot_of_block, const T *src, T *dst, hipStream_t s);
 
-    /// sc[slot] = value
-    void launch_set_scalar(double *sc, int slot, double value, hipStream_t s);
-    /// Arbitrary dot product into a scalar slot (deterministic).
+    /// sc[s][slot] = value for systems s < count.
+    void launch_set_scalar(double *sc, int count, int slot, double value, hipStream_t s);
+    /// Dot product into a scalar slot (deterministic, single system).
     template <typename T>
     void launch_dot(long long n, const T *a, const T *b, double *sc, int slot, ReduceSite site, hipStream_t s);
 
     // ----- MAS apply ----------------------------------------------------------------------
     struct CoarseRestrict
     {
-        int level_num = 0;          ///< coarse levels
-        int counts[4] = {0, 0, 0, 0}; ///< coarse nodes per level
-        int offsets[5] = {0, 0, 0, 0, 0}; ///< node offset of multi-level segment (0 = fine)
+        int level_num = 0;                  ///< coarse levels
+        int counts[4] = {0, 0, 0, 0};       ///< coarse nodes per level
+        int offsets[5] = {0, 0, 0, 0, 0};   ///< node offset of multi-level segment (0 = fine)
         const uint32_t *lead[4] = {nullptr, nullptr, nullptr, nullptr}; ///< index: level-(l-1) node, l >= 1
         const int *gid[4] = {nullptr, nullptr, nullptr, nullptr};
     };
@@ -112,14 +125,15 @@ namespace psmas
 
     /// y_m = Inv_m x_m for matrices [mat_begin, mat_begin + count) of the multi-level vectors.
     /// Storage: packed upper row-major.  Half storage is scaled: Inv = D S D with D per row.
+    /// bank_sys/active (may be null): skip fine banks of systems that stopped iterating.
     template <int BD, typename TM, typename T>
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
-                          hipStream_t s);
+                          const int *bank_sys, const int *active, hipStream_t s);
 
-    /// z_p = mz_p + sum_l mz_l[cmap_p.l]; rz = r.z with beta epilogue (or init epilogue).
+    /// z_p = mz_p + sum_l mz_l[cmap_p.l]; per-system rz = r.z with beta epilogue (or init).
     template <int BD, typename T>
-    void launch_prolong_dot(int npad, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
-                            double *sc, bool init, ReduceSite site, hipStream_t s);
+    void launch_prolong_dot(int nblocks, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
+                            double *sc, bool init, const SysMap &sm, ReduceSite site, hipStream_t s);
 
     // ----- MAS build ----------------------------------------------------------------------
     /// Bank-local connected components at coarse level 0 (one lane per slot, two banks per wave).

@@ -75,7 +75,9 @@ namespace psmas
         template <int BD, typename TM, typename T>
         __global__ __launch_bounds__(apply_block<BD>()) void k_mas_apply(int mat_begin, const TM *__restrict__ mats,
                                                                           const float *__restrict__ dscale,
-                                                                          const T *__restrict__ mr, T *__restrict__ mz)
+                                                                          const T *__restrict__ mr, T *__restrict__ mz,
+                                                                          const int *__restrict__ bank_sys,
+                                                                          const int *__restrict__ active)
         {
             constexpr int N = 32 * BD;
             constexpr int L = N * (N + 1) / 2;
@@ -89,6 +91,8 @@ namespace psmas
             __shared__ TA part[N];
 
             const int m = mat_begin + blockIdx.x;
+            if (bank_sys != nullptr && active[bank_sys[m]] == 0)
+                return;
             const TM *A = mats + static_cast<size_t>(m) * L;
             {
                 using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
@@ -137,53 +141,55 @@ namespace psmas
 
         // ----------------------------------------------------------------------- prolongation
         template <int BD, typename T>
-        __global__ __launch_bounds__(256) void k_prolong_dot(int npad, int level_num, const int4 *__restrict__ cmap,
-                                                             const int *__restrict__ lev_off, const T *__restrict__ r,
-                                                             T *__restrict__ mz, double *__restrict__ sc, bool init,
-                                                             ReduceSite site)
+        __global__ __launch_bounds__(NODE_BLOCK) void k_prolong_dot(int level_num, const int4 *__restrict__ cmap,
+                                                                    const int *__restrict__ lev_off,
+                                                                    const T *__restrict__ r, T *__restrict__ mz,
+                                                                    double *__restrict__ sc, bool init, SysMap sm,
+                                                                    ReduceSite site)
         {
-            const int p = blockIdx.x * 256 + threadIdx.x;
-            double dot = 0.0;
-            if (p < npad)
+            const int sys = sm.block_sys[blockIdx.x];
+            if (sm.active[sys] == 0)
+                return;
+            const int p = blockIdx.x * NODE_BLOCK + threadIdx.x;
+            const int4 c = cmap[p];
+            const int cc[4] = {c.x, c.y, c.z, c.w};
+            double z[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                z[i] = static_cast<double>(mz[static_cast<size_t>(p) * BD + i]);
+            if (cc[0] >= 0)
             {
-                const int4 c = cmap[p];
-                const int cc[4] = {c.x, c.y, c.z, c.w};
-                double z[BD];
-#pragma unroll
-                for (int i = 0; i < BD; ++i)
-                    z[i] = static_cast<double>(mz[static_cast<size_t>(p) * BD + i]);
-                if (cc[0] >= 0)
+                for (int l = 0; l < level_num; ++l)
                 {
-                    for (int l = 0; l < level_num; ++l)
-                    {
-                        const size_t o = static_cast<size_t>(lev_off[l + 1] + cc[l]) * BD;
+                    const size_t o = static_cast<size_t>(lev_off[l + 1] + cc[l]) * BD;
 #pragma unroll
-                        for (int i = 0; i < BD; ++i)
-                            z[i] += static_cast<double>(mz[o + i]);
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < BD; ++i)
-                {
-                    const T zi = static_cast<T>(z[i]);
-                    mz[static_cast<size_t>(p) * BD + i] = zi;
-                    dot += static_cast<double>(r[static_cast<size_t>(p) * BD + i]) * static_cast<double>(zi);
+                    for (int i = 0; i < BD; ++i)
+                        z[i] += static_cast<double>(mz[o + i]);
                 }
             }
-            grid_sum<256>(dot, site.partials, site.counter, [sc, init](double t) {
+            double dot = 0.0;
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+            {
+                const T zi = static_cast<T>(z[i]);
+                mz[static_cast<size_t>(p) * BD + i] = zi;
+                dot += static_cast<double>(r[static_cast<size_t>(p) * BD + i]) * static_cast<double>(zi);
+            }
+            double *scs = sc + static_cast<size_t>(sys) * S_COUNT;
+            system_sum<NODE_BLOCK>(dot, sys, sm, site.partials, site.counter, [scs, init](double t) {
                 if (init)
                 {
-                    sc[S_RZ] = t;
-                    sc[S_RZ0] = t;
-                    sc[S_RZ_OLD] = t;
-                    sc[S_BETA] = 0.0;
+                    scs[S_RZ] = t;
+                    scs[S_RZ0] = t;
+                    scs[S_RZ_OLD] = t;
+                    scs[S_BETA] = 0.0;
                 }
                 else
                 {
-                    const double old = sc[S_RZ];
-                    sc[S_RZ_OLD] = old;
-                    sc[S_RZ] = t;
-                    sc[S_BETA] = safe_div(t, old, sc[S_GUARD]);
+                    const double old = scs[S_RZ];
+                    scs[S_RZ_OLD] = old;
+                    scs[S_RZ] = t;
+                    scs[S_BETA] = safe_div(t, old, scs[S_GUARD]);
                 }
             });
         }
@@ -595,21 +601,21 @@ namespace psmas
 
     template <int BD, typename TM, typename T>
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
-                          hipStream_t s)
+                          const int *bank_sys, const int *active, hipStream_t s)
     {
         if (count <= 0)
             return;
         hipLaunchKernelGGL((k_mas_apply<BD, TM, T>), dim3(count), dim3(apply_block<BD>()), 0, s, mat_begin, mats,
-                           dscale, mr, mz);
+                           dscale, mr, mz, bank_sys, active);
         PSMAS_LAUNCH_CHECK();
     }
 
     template <int BD, typename T>
-    void launch_prolong_dot(int npad, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
-                            double *sc, bool init, ReduceSite site, hipStream_t s)
+    void launch_prolong_dot(int nblocks, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
+                            double *sc, bool init, const SysMap &sm, ReduceSite site, hipStream_t s)
     {
-        hipLaunchKernelGGL((k_prolong_dot<BD, T>), dim3(div_up(npad, 256)), dim3(256), 0, s, npad, level_num, cmap,
-                           lev_off, r, mz, sc, init, site);
+        hipLaunchKernelGGL((k_prolong_dot<BD, T>), dim3(nblocks), dim3(NODE_BLOCK), 0, s, level_num, cmap, lev_off, r,
+                           mz, sc, init, sm, site);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -719,15 +725,15 @@ namespace psmas
     template void launch_restrict_coarse<BD, double>(const CoarseRestrict &, double *, hipStream_t);             \
     template void launch_restrict_coarse<BD, float>(const CoarseRestrict &, float *, hipStream_t);               \
     template void launch_mas_apply<BD, double, double>(int, int, const double *, const float *, const double *,   \
-                                                       double *, hipStream_t);                                   \
+                                                       double *, const int *, const int *, hipStream_t);         \
     template void launch_mas_apply<BD, double, float>(int, int, const double *, const float *, const float *,     \
-                                                      float *, hipStream_t);                                     \
+                                                      float *, const int *, const int *, hipStream_t);           \
     template void launch_mas_apply<BD, __half, float>(int, int, const __half *, const float *, const float *,     \
-                                                      float *, hipStream_t);                                     \
+                                                      float *, const int *, const int *, hipStream_t);           \
     template void launch_prolong_dot<BD, double>(int, int, const int4 *, const int *, const double *, double *,   \
-                                                 double *, bool, ReduceSite, hipStream_t);                       \
+                                                 double *, bool, const SysMap &, ReduceSite, hipStream_t);       \
     template void launch_prolong_dot<BD, float>(int, int, const int4 *, const int *, const float *, float *,      \
-                                                double *, bool, ReduceSite, hipStream_t);
+                                                double *, bool, const SysMap &, ReduceSite, hipStream_t);
 
     PSMAS_INST_APPLY(1)
     PSMAS_INST_APPLY(2)

@@ -1,0 +1,64 @@
+// GPU MLP used by the learned coarse correction: d_in -> hidden -> hidden -> d_out with ReLU,
+// BF16-operand MFMA GEMMs with FP32 accumulation, FP32 master weights trained by Adam.
+//
+// Parameter vector layout (FP32): [W1 (hidden x d_in) | b1 | W2 (hidden x hidden) | b2 |
+// W3 (d_out x hidden) | b3], row-major; y = W3 relu(W2 relu(W1 x + b1) + b2) + b3.
+#pragma once
+
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace psmas
+{
+    class Mlp
+    {
+    public:
+        Mlp(int d_in, int d_out, int hidden = 128, unsigned seed = 1234);
+        ~Mlp();
+        Mlp(const Mlp &) = delete;
+        Mlp &operator=(const Mlp &) = delete;
+
+        int d_in() const { return d_in_; }
+        int d_out() const { return d_out_; }
+        int hidden() const { return h_; }
+        size_t num_parameters() const { return nparams_; }
+
+        void set_parameters(const float *host);
+        void get_parameters(float *host) const;
+
+        /// Y[B x d_out] = f(X[B x d_in]); device pointers; enqueued on s.
+        void forward_device(int B, const float *X, float *Y, hipStream_t s);
+        /// Mean squared error over the batch and its gradient (kept on the device; see
+        /// get_gradients).  Synchronous.
+        double loss_and_grad_device(int B, const float *X, const float *T);
+        void get_gradients(float *host);
+        void adam_step(float lr, float weight_decay);
+        /// Adam training over n device samples; returns the last epoch's mean loss.
+        double train(int n, const float *X, const float *T, int epochs, int batch, float lr, unsigned seed);
+        hipStream_t stream() const { return stream_; }
+        const unsigned short *bf16_weights() const { return wb_.data(); }
+        const float *fp32_weights() const { return w_.data(); }
+        size_t offset(int i) const { return offs_[i]; }
+
+    private:
+        int d_in_, d_out_, h_;
+        size_t sizes_[6] = {}, offs_[6] = {};
+        size_t nparams_ = 0;
+        int step_ = 0;
+        int cap_ = 0;
+        hipStream_t stream_ = nullptr;
+        DeviceBuffer<float> w_, g_, m_, v_;
+        DeviceBuffer<unsigned short> wb_;
+        DeviceBuffer<float> h1_, h2_, y_, dy_, dh1_, dh2_;
+        DeviceBuffer<double> loss_;
+        void ensure_batch(int B);
+    };
+
+    /// C = A B (row-major FP32 operands rounded to BF16, FP32 accumulation) — GEMM check entry.
+    void mlp_gemm_nn(int M, int N, int K, const float *A, const float *B, float *C, hipStream_t s);
+    /// C = A^T B with A stored K x M.
+    void mlp_gemm_tn(int M, int N, int K, const float *A, const float *B, float *C, hipStream_t s);
+} // namespace psmas

@@ -9,12 +9,20 @@
 // device between the host convergence checks.  Iterations between checks are captured once into
 // HIP graphs and replayed.
 //
+// A factorisation holds K >= 1 independent systems laid out one after another, each covering
+// whole 256-node blocks.  They share every kernel launch; each has its own CG scalars, its own
+// deterministic reductions and its own convergence test, and stops (its workgroups exit at
+// once) when it converges.  The preconditioner's coarse levels are built over the whole batch,
+// which is the reference construction applied to the block-diagonal batch matrix.
+//
 // The mixed-precision path ("precision": "mixed") runs an FP32 inner PCG (FP32 SELL matrix,
 // FP16 fine-level bank inverses with FP32 scaling, FP64 coarse-level inverses) with reliable
-// updates: at a check point where the recurrence residual fell by reliable_update_delta (or
-// below the target), the FP32 correction is added to the FP64 solution and the FP64 residual
-// b - A x is recomputed with the FP64 matrix; convergence is decided on that FP64 residual only.
+// updates: at a check point where a system's recurrence residual fell by reliable_update_delta
+// (or below its target), its FP32 correction is added to the FP64 solution and its FP64
+// residual b - A x is recomputed with the FP64 matrix; convergence is decided on that FP64
+// residual only.
 #include "solver.hpp"
+#include "coarse_solve.hpp"
 #include "device.hpp"
 
 #include <algorithm>
@@ -75,6 +83,12 @@ namespace psmas
         return "Unknown";
     }
 
+    namespace
+    {
+        /// Graph cache keys: precision path, iteration pattern, iterations per graph.
+        int graph_key(int path, int pattern, int count) { return (path << 24) | (pattern << 16) | count; }
+    } // namespace
+
     // ------------------------------------------------------------------------------ graphs
     struct Solver::Graphs
     {
@@ -101,9 +115,7 @@ namespace psmas
         device_ = 0;
         PSMAS_HIP(hipSetDevice(device_));
         PSMAS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-        d_sc_.allocate(S_COUNT, "MAS device scalars");
-        d_sc_.zero(stream_);
-        PSMAS_HIP(hipStreamSynchronize(stream_));
+        sys_.resize(1);
     }
 
     Solver::~Solver()
@@ -123,39 +135,60 @@ namespace psmas
         const Params before = params_;
         apply_json_parameters(params_, json);
         if (before.use_graphs != params_.use_graphs || before.check_interval != params_.check_interval ||
-            before.true_residual_period != params_.true_residual_period)
+            before.true_residual_period != params_.true_residual_period || before.precision != params_.precision ||
+            before.coarse_solver != params_.coarse_solver)
             graphs_->clear();
     }
 
-    std::string Solver::get_info_json() const
+    std::string Solver::system_info_json(int s) const
     {
+        if (s < 0 || s >= static_cast<int>(sys_.size()))
+            throw InvalidArgumentError("[MAS] system index out of range.");
+        const System &S = sys_[s];
         std::ostringstream o;
         o.precision(17);
-        o << "{\"solver_iter\": " << iterations_ << ", \"solver_error\": " << residual_norm_
-          << ", \"solver_status\": \"" << status_string(status_) << "\""
+        o << "{\"solver_iter\": " << S.iterations << ", \"solver_error\": " << S.residual_norm
+          << ", \"solver_status\": \"" << status_string(S.status) << "\""
           << ", \"precision\": \"" << (params_.precision == Precision::FP64 ? "fp64" : "mixed") << "\""
-          << ", \"reliable_updates\": " << reliable_updates_
-          << ", \"true_relative_residual\": " << true_rel_residual_ << ", \"solve_ms\": " << solve_ms_
-          << ", \"levels\": " << level_num_ << ", \"parts\": " << nparts_ << ", \"aggregation\": \""
-          << aggregation_used_ << "\"}";
+          << ", \"reliable_updates\": " << S.reliable_updates << ", \"relative_residual\": " << S.rel_residual
+          << ", \"solve_ms\": " << solve_ms_ << ", \"levels\": " << level_num_ << ", \"parts\": " << S.nparts
+          << ", \"systems\": " << sys_.size() << ", \"aggregation\": \"" << S.aggregation << "\""
+          << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\"}";
         return o.str();
     }
 
-    void Solver::set_coordinates(int nb, int dim, const double *xyz)
+    void Solver::set_coordinates(int sys, int nb, int dim, const double *xyz)
     {
         if (nb <= 0 || dim < 1 || dim > 3 || xyz == nullptr)
             throw InvalidArgumentError("[MAS] invalid coordinates: need nb > 0 nodes with 1 to 3 components.");
-        coords_.assign(xyz, xyz + static_cast<size_t>(nb) * dim);
-        coord_dim_ = dim;
-        have_partition_ = false;
+        if (sys < 0)
+            throw InvalidArgumentError("[MAS] system index out of range.");
+        if (sys >= static_cast<int>(sys_.size()))
+            sys_.resize(sys + 1);
+        System &S = sys_[sys];
+        S.coords.assign(xyz, xyz + static_cast<size_t>(nb) * dim);
+        S.coord_dim = dim;
+        S.have_partition = false;
     }
 
     ReduceSite Solver::site(int s) const
     {
         ReduceSite r;
         r.partials = d_partials_.data() + static_cast<size_t>(s) * partial_cap_;
-        r.counter = d_counters_.data() + s;
+        r.counter = d_counters_.data() + static_cast<size_t>(s) * sys_.size();
         return r;
+    }
+
+    SysMap Solver::sysmap() const
+    {
+        SysMap m;
+        m.block_sys = d_block_sys_.data();
+        m.blk_begin = d_blk_begin_.data();
+        m.blk_count = d_blk_count_.data();
+        m.active = d_active_.data();
+        m.select = d_select_.data();
+        m.bank_sys = d_bank_sys_.data();
+        return m;
     }
 
     FineRestrict Solver::fine_restrict() const
@@ -193,47 +226,136 @@ namespace psmas
         return v;
     }
 
-    double Solver::read_scalar(int slot)
+    void Solver::read_scalars()
     {
-        double v = 0.0;
-        PSMAS_HIP(hipMemcpyAsync(&v, d_sc_.data() + slot, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        h_sc_.resize(static_cast<size_t>(S_COUNT) * sys_.size());
+        PSMAS_HIP(hipMemcpyAsync(h_sc_.data(), d_sc_.data(), sizeof(double) * h_sc_.size(), hipMemcpyDeviceToHost,
+                                 stream_));
         PSMAS_HIP(hipStreamSynchronize(stream_));
-        return v;
     }
 
-    void Solver::read_scalars(double *out)
+    void Solver::upload_flags()
     {
-        PSMAS_HIP(hipMemcpyAsync(out, d_sc_.data(), sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipMemcpyAsync(d_active_.data(), h_active_.data(), sizeof(int) * h_active_.size(),
+                                 hipMemcpyHostToDevice, stream_));
+        PSMAS_HIP(hipMemcpyAsync(d_select_.data(), h_select_.data(), sizeof(int) * h_select_.size(),
+                                 hipMemcpyHostToDevice, stream_));
         PSMAS_HIP(hipStreamSynchronize(stream_));
     }
 
     // --------------------------------------------------------------------------- factorize
-    void Solver::factorize(const CscView &A)
+    void Solver::factorize_batch(const std::vector<CscView> &As)
     {
         PSMAS_HIP(hipSetDevice(device_));
-        validate_csc(A);
+        if (As.empty())
+            throw InvalidArgumentError("[MAS] empty batch.");
+        for (const auto &A : As)
+            validate_csc(A);
         const int bd = params_.block_dim;
-        const int nb = div_up(A.n, bd);
         factorized_ = false;
         graphs_->clear();
-
-        const bool rebuild = !params_.lazy_partitioning || !have_partition_ ||
-                             static_cast<int>(part_.perm.size()) != nb || bd != bd_;
-        n_ = A.n;
+        const int K = static_cast<int>(As.size());
+        if (static_cast<int>(sys_.size()) != K)
+            sys_.resize(K);
+        const bool bd_changed = (bd != bd_);
         bd_ = bd;
-        nb_ = nb;
-        if (rebuild)
-            build_partition(A);
 
-        PaddedBsr bsr;
-        build_padded_bsr(A, bd_, part_, bsr);
-        npad_ = bsr.npad;
-        nparts_ = bsr.nparts;
-        slot_of_block_ = bsr.slot_of_block;
-        block_of_slot_ = bsr.block_of_slot;
-        nnzb_ = static_cast<long long>(bsr.cols.size());
-        upload_matrix(bsr);
-        bsr = PaddedBsr();
+        // Per-system partition and bank-padded BSR, concatenated.
+        PaddedBsr all;
+        all.bd = bd;
+        all.rows.assign(1, 0);
+        block_of_slot_.clear();
+        int node_off = 0;
+        nnzb_ = 0;
+        nnz_scalar_ = 0;
+        n_total_ = 0;
+        nb_total_ = 0;
+        int sob_off = 0;
+        std::vector<int> h_slot_of_block;
+        for (int s = 0; s < K; ++s)
+        {
+            System &S = sys_[s];
+            const CscView &A = As[s];
+            const int nb = div_up(A.n, bd);
+            const bool rebuild = !params_.lazy_partitioning || !S.have_partition ||
+                                 static_cast<int>(S.part.perm.size()) != nb || bd_changed;
+            S.n = A.n;
+            S.nb = nb;
+            if (rebuild)
+                build_partition(S, A);
+            PaddedBsr bsr;
+            build_padded_bsr(A, bd, S.part, bsr);
+            S.nparts = bsr.nparts;
+            S.npad = bsr.npad;
+            S.node_off = node_off;
+            S.blk_begin = node_off / NODE_BLOCK;
+            S.nnzb = static_cast<long long>(bsr.cols.size());
+            S.slot_of_block = bsr.slot_of_block;
+            S.sob_off = sob_off;
+            sob_off += nb;
+            h_slot_of_block.insert(h_slot_of_block.end(), bsr.slot_of_block.begin(), bsr.slot_of_block.end());
+            block_of_slot_.insert(block_of_slot_.end(), bsr.block_of_slot.begin(), bsr.block_of_slot.end());
+            const int row_base = all.rows.back();
+            for (int r = 0; r < bsr.npad; ++r)
+                all.rows.push_back(row_base + bsr.rows[r + 1]);
+            for (int c : bsr.cols)
+                all.cols.push_back(c + node_off);
+            all.vals.insert(all.vals.end(), bsr.vals.begin(), bsr.vals.end());
+            node_off += bsr.npad;
+            nnzb_ += S.nnzb;
+            nnz_scalar_ += A.nnz;
+            n_total_ += A.n;
+            nb_total_ += nb;
+        }
+        npad_ = node_off;
+        nparts_ = npad_ / BANK;
+        nblocks_ = npad_ / NODE_BLOCK;
+        all.npad = npad_;
+        all.nparts = nparts_;
+        nonisolated_.assign(npad_, 0);
+        for (int r = 0; r < npad_; ++r)
+            for (int k = all.rows[r]; k < all.rows[r + 1]; ++k)
+                if (all.cols[k] != r)
+                {
+                    nonisolated_[r] = 1;
+                    break;
+                }
+        dense_built_ = false;
+        dense_nd_ = 0;
+        dense_note_.clear();
+        d_dense_inv_.reset();
+        d_dense_ids_.reset();
+
+        upload_matrix(all);
+        all = PaddedBsr();
+
+        // System maps.
+        std::vector<int> block_sys(nblocks_), blk_begin(K), blk_count(K), bank_sys(nparts_);
+        for (int s = 0; s < K; ++s)
+        {
+            blk_begin[s] = sys_[s].blk_begin;
+            blk_count[s] = sys_[s].npad / NODE_BLOCK;
+            for (int b = 0; b < blk_count[s]; ++b)
+                block_sys[blk_begin[s] + b] = s;
+            for (int p = 0; p < sys_[s].nparts; ++p)
+                bank_sys[sys_[s].node_off / BANK + p] = s;
+        }
+        d_block_sys_.allocate(block_sys.size(), "MAS block systems");
+        d_blk_begin_.allocate(K, "MAS system blocks");
+        d_blk_count_.allocate(K, "MAS system blocks");
+        d_bank_sys_.allocate(bank_sys.size(), "MAS bank systems");
+        d_active_.allocate(K, "MAS system flags");
+        d_select_.allocate(K, "MAS system flags");
+        d_slot_of_block_.allocate(h_slot_of_block.size(), "MAS block map");
+        PSMAS_HIP(hipMemcpy(d_block_sys_.data(), block_sys.data(), d_block_sys_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_blk_begin_.data(), blk_begin.data(), d_blk_begin_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_blk_count_.data(), blk_count.data(), d_blk_count_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_bank_sys_.data(), bank_sys.data(), d_bank_sys_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_slot_of_block_.data(), h_slot_of_block.data(), d_slot_of_block_.bytes(),
+                            hipMemcpyHostToDevice));
+        h_active_.assign(K, 1);
+        h_select_.assign(K, 1);
+
         d_vals32_.reset();
         d_mats16_.reset();
         d_dscale_.reset();
@@ -245,10 +367,11 @@ namespace psmas
         factorized_ = true;
     }
 
-    void Solver::build_partition(const CscView &A)
+    void Solver::build_partition(System &S, const CscView &A)
     {
+        const int nb = S.nb;
         Aggregation agg = params_.aggregation;
-        const bool have_coords = coord_dim_ > 0 && static_cast<int>(coords_.size()) == nb_ * coord_dim_;
+        const bool have_coords = S.coord_dim > 0 && static_cast<int>(S.coords.size()) == nb * S.coord_dim;
         if (agg == Aggregation::Auto)
             agg = have_coords ? Aggregation::Morton : Aggregation::Graph;
         if (agg == Aggregation::Morton && !have_coords)
@@ -256,43 +379,43 @@ namespace psmas
                 "[MAS] Morton aggregation needs one coordinate tuple per block node (psmas_set_coordinates).");
         if (agg == Aggregation::Morton)
         {
-            const int dim = coord_dim_;
+            const int dim = S.coord_dim;
             double lo[3] = {0, 0, 0}, inv[3] = {0, 0, 0};
             for (int d = 0; d < dim; ++d)
             {
-                double mn = coords_[d], mx = coords_[d];
-                for (int i = 0; i < nb_; ++i)
+                double mn = S.coords[d], mx = S.coords[d];
+                for (int i = 0; i < nb; ++i)
                 {
-                    mn = std::min(mn, coords_[static_cast<size_t>(i) * dim + d]);
-                    mx = std::max(mx, coords_[static_cast<size_t>(i) * dim + d]);
+                    mn = std::min(mn, S.coords[static_cast<size_t>(i) * dim + d]);
+                    mx = std::max(mx, S.coords[static_cast<size_t>(i) * dim + d]);
                 }
                 lo[d] = mn;
                 inv[d] = (mx > mn) ? 1.0 / (mx - mn) : 0.0;
             }
-            DeviceBuffer<double> d_xyz(coords_.size(), "MAS morton coordinates");
+            DeviceBuffer<double> d_xyz(S.coords.size(), "MAS morton coordinates");
             DeviceBuffer<double> d_lo(3, "MAS morton box"), d_inv(3, "MAS morton box");
-            DeviceBuffer<uint64_t> keys(nb_, "MAS morton keys"), keys_out(nb_, "MAS morton keys");
-            DeviceBuffer<int> ids(nb_, "MAS morton ids"), ids_out(nb_, "MAS morton ids");
-            PSMAS_HIP(hipMemcpyAsync(d_xyz.data(), coords_.data(), d_xyz.bytes(), hipMemcpyHostToDevice, stream_));
+            DeviceBuffer<uint64_t> keys(nb, "MAS morton keys"), keys_out(nb, "MAS morton keys");
+            DeviceBuffer<int> ids(nb, "MAS morton ids"), ids_out(nb, "MAS morton ids");
+            PSMAS_HIP(hipMemcpyAsync(d_xyz.data(), S.coords.data(), d_xyz.bytes(), hipMemcpyHostToDevice, stream_));
             PSMAS_HIP(hipMemcpyAsync(d_lo.data(), lo, sizeof(lo), hipMemcpyHostToDevice, stream_));
             PSMAS_HIP(hipMemcpyAsync(d_inv.data(), inv, sizeof(inv), hipMemcpyHostToDevice, stream_));
-            launch_morton_keys(nb_, dim, d_xyz.data(), d_lo.data(), d_inv.data(), keys.data(), ids.data(), stream_);
-            const size_t tb = morton_sort_temp_bytes(nb_);
+            launch_morton_keys(nb, dim, d_xyz.data(), d_lo.data(), d_inv.data(), keys.data(), ids.data(), stream_);
+            const size_t tb = morton_sort_temp_bytes(nb);
             DeviceBuffer<unsigned char> temp(std::max<size_t>(tb, 1), "MAS morton sort");
-            morton_sort(nb_, temp.data(), tb, keys.data(), keys_out.data(), ids.data(), ids_out.data(), stream_);
-            std::vector<int> order(nb_);
-            PSMAS_HIP(hipMemcpyAsync(order.data(), ids_out.data(), sizeof(int) * nb_, hipMemcpyDeviceToHost, stream_));
+            morton_sort(nb, temp.data(), tb, keys.data(), keys_out.data(), ids.data(), ids_out.data(), stream_);
+            std::vector<int> order(nb);
+            PSMAS_HIP(hipMemcpyAsync(order.data(), ids_out.data(), sizeof(int) * nb, hipMemcpyDeviceToHost, stream_));
             PSMAS_HIP(hipStreamSynchronize(stream_));
-            part_ = chunk_partition(order, BANK);
-            aggregation_used_ = "morton";
+            S.part = chunk_partition(order, BANK);
+            S.aggregation = "morton";
         }
         else
         {
             const BlockGraph g = build_block_graph(A, bd_);
-            part_ = graph_aggregate(g, BANK);
-            aggregation_used_ = "graph";
+            S.part = graph_aggregate(g, BANK);
+            S.aggregation = "graph";
         }
-        have_partition_ = true;
+        S.have_partition = true;
     }
 
     void Solver::upload_matrix(const PaddedBsr &bsr)
@@ -359,14 +482,11 @@ namespace psmas
         d_vals64_.allocate(vals.size(), "MAS SELL values (FP64)");
         d_row_len_.allocate(row_len.size(), "MAS SELL row lengths");
         d_block_of_slot_.allocate(block_of_slot_.size(), "MAS slot map");
-        d_slot_of_block_.allocate(slot_of_block_.size(), "MAS block map");
         PSMAS_HIP(hipMemcpy(d_slice_ptr_.data(), slice_ptr.data(), d_slice_ptr_.bytes(), hipMemcpyHostToDevice));
         PSMAS_HIP(hipMemcpy(d_cols_.data(), cols.data(), d_cols_.bytes(), hipMemcpyHostToDevice));
         PSMAS_HIP(hipMemcpy(d_vals64_.data(), vals.data(), d_vals64_.bytes(), hipMemcpyHostToDevice));
         PSMAS_HIP(hipMemcpy(d_row_len_.data(), row_len.data(), d_row_len_.bytes(), hipMemcpyHostToDevice));
         PSMAS_HIP(hipMemcpy(d_block_of_slot_.data(), block_of_slot_.data(), d_block_of_slot_.bytes(),
-                            hipMemcpyHostToDevice));
-        PSMAS_HIP(hipMemcpy(d_slot_of_block_.data(), slot_of_block_.data(), d_slot_of_block_.bytes(),
                             hipMemcpyHostToDevice));
     }
 
@@ -465,13 +585,16 @@ namespace psmas
     {
         const size_t nvec = static_cast<size_t>(npad_) * bd_;
         const size_t nml = static_cast<size_t>(total_level_nodes_) * bd_;
+        int nmax = 0;
+        for (const auto &S : sys_)
+            nmax = std::max(nmax, S.n);
         d_b_.allocate(nvec, "device_buffers b");
         d_x_.allocate(nvec, "device_buffers x");
         d_p_.allocate(nvec, "device_buffers p");
         d_Ap_.allocate(nvec, "device_buffers Ap");
         d_mr_.allocate(nml, "MAS coarse_vectors r");
         d_mz_.allocate(nml, "MAS coarse_vectors z");
-        d_io_.allocate(static_cast<size_t>(n_), "device_buffers io");
+        d_io_.allocate(static_cast<size_t>(nmax), "device_buffers io");
         d_mr_.zero(stream_);
         d_mz_.zero(stream_);
         d_x32_.reset();
@@ -479,9 +602,12 @@ namespace psmas
         d_mz32_.reset();
         d_p32_.reset();
         d_Ap32_.reset();
-        partial_cap_ = std::max(npad_ / 256 + 8, 2048);
+        const int K = static_cast<int>(sys_.size());
+        d_sc_.allocate(static_cast<size_t>(S_COUNT) * K, "MAS device scalars");
+        d_sc_.zero(stream_);
+        partial_cap_ = std::max(nblocks_, 2048);
         d_partials_.allocate(static_cast<size_t>(R_COUNT) * partial_cap_, "MAS reduction partials");
-        d_counters_.allocate(R_COUNT, "MAS reduction counters");
+        d_counters_.allocate(static_cast<size_t>(R_COUNT) * K, "MAS reduction counters");
         d_counters_.zero(stream_);
     }
 
@@ -516,49 +642,145 @@ namespace psmas
         }
     }
 
-    void Solver::check_ready(int n) const
+    void Solver::ensure_coarse_solver()
     {
-        if (!factorized_ || n <= 0 || n != n_)
+        if (params_.coarse_solver != CoarseSolver::Cholesky || dense_built_)
+            return;
+        dense_built_ = true;
+        dense_nd_ = 0;
+        const int level = level_num_ - 1; // coarsest coarse level (segment level_num_)
+        const int count = cco_[level];
+        std::vector<int4> cm(npad_);
+        PSMAS_HIP(hipMemcpy(cm.data(), d_cmap_.data(), sizeof(int4) * npad_, hipMemcpyDeviceToHost));
+        std::vector<int> active_index(count, -1);
+        for (int p = 0; p < npad_; ++p)
+        {
+            if (block_of_slot_[p] < 0 || !nonisolated_[p])
+                continue;
+            const int4 c = cm[p];
+            const int id = level == 0 ? c.x : (level == 1 ? c.y : (level == 2 ? c.z : c.w));
+            active_index[id] = 1;
+        }
+        std::vector<int> ids;
+        for (int c = 0; c < count; ++c)
+            if (active_index[c] > 0)
+            {
+                active_index[c] = static_cast<int>(ids.size());
+                ids.push_back(c);
+            }
+        dense_nodes_ = static_cast<int>(ids.size());
+        if (ids.empty() || dense_nodes_ > DENSE_COARSE_MAX_NODES)
+        {
+            dense_note_ = ids.empty() ? "no active coarsest nodes" : "too many active coarsest nodes";
+            return;
+        }
+        const int nd = bd_ * dense_nodes_;
+        DeviceBuffer<int> d_index(count, "MAS coarsest active index");
+        PSMAS_HIP(hipMemcpy(d_index.data(), active_index.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+        DeviceBuffer<double> dense(static_cast<size_t>(nd) * nd, "MAS coarsest Galerkin matrix");
+        dense.zero(stream_);
+        launch_dense_galerkin(bd_, npad_, level, sell(d_vals64_.data()), d_row_len_.data(), d_block_of_slot_.data(),
+                              d_cmap_.data(), d_index.data(), nd, dense.data(), stream_);
+        std::vector<double> h(static_cast<size_t>(nd) * nd);
+        PSMAS_HIP(hipMemcpyAsync(h.data(), dense.data(), dense.bytes(), hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        for (int i = 0; i < nd; ++i) // atomics leave round-off asymmetry: use the symmetric part
+            for (int j = i + 1; j < nd; ++j)
+            {
+                const double v = 0.5 * (h[static_cast<size_t>(i) * nd + j] + h[static_cast<size_t>(j) * nd + i]);
+                h[static_cast<size_t>(i) * nd + j] = v;
+                h[static_cast<size_t>(j) * nd + i] = v;
+            }
+        if (!cholesky_inverse(nd, h))
+            throw std::runtime_error("[MAS] coarsest-level Cholesky failed (coarse matrix not positive definite).");
+        d_dense_inv_.allocate(h.size(), "MAS coarsest inverse");
+        d_dense_ids_.allocate(ids.size(), "MAS coarsest active ids");
+        PSMAS_HIP(hipMemcpy(d_dense_inv_.data(), h.data(), d_dense_inv_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_dense_ids_.data(), ids.data(), d_dense_ids_.bytes(), hipMemcpyHostToDevice));
+        dense_nd_ = nd;
+        dense_note_ = "dense FP64 Cholesky on " + std::to_string(dense_nodes_) + " coarsest nodes";
+    }
+
+    template <int BD, typename T>
+    void Solver::apply_dense_coarse(const T *mr, T *mz)
+    {
+        if (params_.coarse_solver != CoarseSolver::Cholesky || dense_nd_ == 0)
+            return;
+        const size_t off = static_cast<size_t>(lev_off_[level_num_]) * BD;
+        launch_dense_apply<BD, T>(dense_nd_, d_dense_inv_.data(), d_dense_ids_.data(), mr + off, mz + off, stream_);
+    }
+
+    void Solver::check_ready_single(int n) const
+    {
+        if (!factorized_ || sys_.size() != 1 || n <= 0 || n != sys_[0].n)
             throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
     }
 
     // ------------------------------------------------------------------------------- solve
     void Solver::solve_host(int n, const double *b, double *x)
     {
-        PSMAS_HIP(hipSetDevice(device_));
-        status_ = Status::Running;
-        check_ready(n);
-        if (b == nullptr || x == nullptr)
-            throw InvalidArgumentError("[MAS] null b or x.");
-        PSMAS_HIP(hipMemcpyAsync(d_io_.data(), b, sizeof(double) * n, hipMemcpyHostToDevice, stream_));
-        launch_gather_in<double>(npad_, bd_, n_, d_block_of_slot_.data(), d_io_.data(), d_b_.data(), stream_);
-        solve_padded();
-        launch_scatter_out<double>(nb_, bd_, n_, d_slot_of_block_.data(), d_x_.data(), d_io_.data(), stream_);
-        PSMAS_HIP(hipMemcpyAsync(x, d_io_.data(), sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
-        PSMAS_HIP(hipStreamSynchronize(stream_));
+        sys_[0].status = Status::Running;
+        check_ready_single(n);
+        solve_batch({b}, {x}, false);
     }
 
     void Solver::solve_device(int n, const double *d_b, double *d_x)
     {
+        sys_[0].status = Status::Running;
+        check_ready_single(n);
+        solve_batch({d_b}, {d_x}, true);
+    }
+
+    void Solver::solve_batch(const std::vector<const double *> &b, const std::vector<double *> &x, bool device)
+    {
         PSMAS_HIP(hipSetDevice(device_));
-        status_ = Status::Running;
-        check_ready(n);
-        if (d_b == nullptr || d_x == nullptr)
-            throw InvalidArgumentError("[MAS] null b or x.");
-        launch_gather_in<double>(npad_, bd_, n_, d_block_of_slot_.data(), d_b, d_b_.data(), stream_);
+        for (auto &S : sys_)
+            S.status = Status::Running;
+        if (!factorized_ || b.size() != sys_.size() || x.size() != sys_.size())
+            throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
+        for (size_t s = 0; s < sys_.size(); ++s)
+        {
+            const System &S = sys_[s];
+            if (b[s] == nullptr || x[s] == nullptr)
+                throw InvalidArgumentError("[MAS] null b or x.");
+            const double *src = b[s];
+            if (!device)
+            {
+                PSMAS_HIP(hipMemcpyAsync(d_io_.data(), b[s], sizeof(double) * S.n, hipMemcpyHostToDevice, stream_));
+                src = d_io_.data();
+            }
+            launch_gather_in<double>(S.npad, bd_, S.n, d_block_of_slot_.data() + S.node_off, src,
+                                     d_b_.data() + static_cast<size_t>(S.node_off) * bd_, stream_);
+        }
         solve_padded();
-        launch_scatter_out<double>(nb_, bd_, n_, d_slot_of_block_.data(), d_x_.data(), d_x, stream_);
+        for (size_t s = 0; s < sys_.size(); ++s)
+        {
+            const System &S = sys_[s];
+            double *dst = device ? x[s] : d_io_.data();
+            launch_scatter_out<double>(S.nb, bd_, S.n, d_slot_of_block_.data() + S.sob_off,
+                                       d_x_.data() + static_cast<size_t>(S.node_off) * bd_, dst, stream_);
+            if (!device)
+                PSMAS_HIP(hipMemcpyAsync(x[s], d_io_.data(), sizeof(double) * S.n, hipMemcpyDeviceToHost, stream_));
+        }
         PSMAS_HIP(hipStreamSynchronize(stream_));
     }
 
     void Solver::solve_padded()
     {
         const auto t0 = std::chrono::steady_clock::now();
-        iterations_ = 0;
-        residual_norm_ = 0.0;
-        reliable_updates_ = 0;
-        true_rel_residual_ = 0.0;
-        launch_set_scalar(d_sc_.data(), S_GUARD, params_.division_guard, stream_);
+        const int K = static_cast<int>(sys_.size());
+        for (auto &S : sys_)
+        {
+            S.iterations = 0;
+            S.residual_norm = 0.0;
+            S.reliable_updates = 0;
+            S.rel_residual = 0.0;
+        }
+        h_active_.assign(K, 1);
+        h_select_.assign(K, 1);
+        upload_flags();
+        launch_set_scalar(d_sc_.data(), K, S_GUARD, params_.division_guard, stream_);
+        ensure_coarse_solver();
         if (params_.precision == Precision::Mixed)
         {
             ensure_mixed_data();
@@ -573,9 +795,8 @@ namespace psmas
         solve_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 
-    void Solver::run_block(int key, int count, const std::function<void()> &enqueue)
+    void Solver::run_block(int key, const std::function<void()> &enqueue)
     {
-        (void)count;
         if (!params_.use_graphs || timing_on_)
         {
             enqueue();
@@ -613,117 +834,123 @@ namespace psmas
     template <int BD>
     void Solver::fp64_precondition(bool init)
     {
-        const int fine_mats = total_mats_;
         launch_restrict_coarse<BD, double>(coarse_restrict(), d_mr_.data(), stream_);
-        tic("mas_apply_fp64", static_cast<double>(fine_mats) * (BANK * BD) * (BANK * BD + 1) / 2 * 8.0 +
-                                  2.0 * fine_mats * BANK * BD * 8.0);
+        tic("mas_apply_fp64", static_cast<double>(total_mats_) * ((BANK * BD) * (BANK * BD + 1) / 2 * 8.0 +
+                                                                  2.0 * BANK * BD * 8.0));
         launch_mas_apply<BD, double, double>(0, total_mats_, d_mats64_.data(), nullptr, d_mr_.data(), d_mz_.data(),
-                                             stream_);
+                                             nullptr, nullptr, stream_);
         toc();
-        launch_prolong_dot<BD, double>(npad_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr_.data(),
-                                       d_mz_.data(), d_sc_.data(), init, site(R_RZ), stream_);
+        apply_dense_coarse<BD, double>(d_mr_.data(), d_mz_.data());
+        launch_prolong_dot<BD, double>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr_.data(),
+                                       d_mz_.data(), d_sc_.data(), init, sysmap(), site(R_RZ), stream_);
     }
 
     template <int BD>
     void Solver::fp64_iteration(int k)
     {
         const SellView<double> A = sell(d_vals64_.data());
-        const long long nvec = static_cast<long long>(npad_) * BD;
-        const double spmv_bytes = static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_ + 1) * 4.0 + 2.0 * n_ * 8.0;
+        const double spmv_bytes =
+            static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_total_ + 1) * 4.0 + 2.0 * n_total_ * 8.0;
         tic("spmv_fp64", spmv_bytes);
-        launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), d_sc_.data(), site(R_PAP), stream_);
+        launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), d_sc_.data(), sysmap(), site(R_PAP), stream_);
         toc();
         if (k % params_.true_residual_period == 0)
         {
-            launch_axpy_alpha<double>(nvec, d_p_.data(), d_x_.data(), d_sc_.data(), stream_);
-            tic("spmv_res_fp64", spmv_bytes + n_ * 8.0);
+            launch_axpy_alpha<BD, double>(nblocks_, d_p_.data(), d_x_.data(), d_sc_.data(), sysmap(), stream_);
+            tic("spmv_res_fp64", spmv_bytes + n_total_ * 8.0);
             launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
                                                         d_mr_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
-                                                        S_RR, site(R_RR), stream_);
+                                                        S_RR, sysmap(), site(R_RR), false, stream_);
             toc();
         }
         else
         {
-            launch_update_xr<BD, double>(npad_, d_p_.data(), d_Ap_.data(), d_x_.data(), d_mr_.data(), fine_restrict(),
-                                         d_mr_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(), site(R_RR),
-                                         stream_);
+            launch_update_xr<BD, double>(nblocks_, d_p_.data(), d_Ap_.data(), d_x_.data(), d_mr_.data(),
+                                         fine_restrict(), d_mr_.data() + static_cast<size_t>(npad_) * BD,
+                                         d_sc_.data(), sysmap(), site(R_RR), stream_);
         }
         fp64_precondition<BD>(false);
-        launch_update_p<double>(nvec, d_mz_.data(), d_p_.data(), d_sc_.data(), stream_);
+        launch_update_p<BD, double>(nblocks_, d_mz_.data(), d_p_.data(), d_sc_.data(), sysmap(), stream_);
     }
 
     void Solver::solve_fp64()
     {
-        const int bd = bd_;
         auto body = [&](auto bdc) {
             constexpr int BD = decltype(bdc)::value;
+            const int K = static_cast<int>(sys_.size());
             const SellView<double> A = sell(d_vals64_.data());
             const long long nvec = static_cast<long long>(npad_) * BD;
             launch_fill<double>(nvec, d_x_.data(), 0.0, stream_);
             // r = b - A x (x = 0), rr, restriction.
             launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
                                                         d_mr_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
-                                                        S_RR, site(R_RR), stream_);
+                                                        S_RR, sysmap(), site(R_RR), false, stream_);
             fp64_precondition<BD>(true);
             launch_copy_convert<double, double>(nvec, d_mz_.data(), d_p_.data(), stream_);
-            double sc[S_COUNT];
-            read_scalars(sc);
-            const double rz0 = sc[S_RZ0];
-            if (std::isnan(rz0) || !std::isfinite(rz0))
-                throw std::runtime_error("[MAS] Invalid initial residual.");
-            const double rr0 = params_.use_preconditioned_residual_norm ? 0.0 : sc[S_RR];
+            read_scalars();
+            std::vector<double> rz0(K), rr0(K), bb(K);
+            for (int s = 0; s < K; ++s)
+            {
+                rz0[s] = *scal(s, S_RZ0);
+                if (std::isnan(rz0[s]) || !std::isfinite(rz0[s]))
+                    throw std::runtime_error(K == 1 ? std::string("[MAS] Invalid initial residual.")
+                                                    : "[MAS] Invalid initial residual (system " +
+                                                          std::to_string(s) + ").");
+                bb[s] = *scal(s, S_RR);
+                rr0[s] = params_.use_preconditioned_residual_norm ? 0.0 : bb[s];
+            }
             const double rel2 = params_.rel_tol * params_.rel_tol;
             const double abs2 = params_.abs_tol * params_.abs_tol;
             const int ci = params_.check_interval;
             const int trp = params_.true_residual_period;
             int k = 1;
-            bool converged = false;
-            while (k <= params_.max_iter && !converged)
+            int running = K;
+            while (k <= params_.max_iter && running > 0)
             {
                 const int next_check = ((k - 1) / ci + 1) * ci;
                 const int last = std::min(next_check, params_.max_iter);
                 const int count = last - k + 1;
-                const int key = ((k % trp) << 20) | count;
                 const int k0 = k;
-                run_block(key, count, [&]() {
+                run_block(graph_key(0, k % trp, count), [&]() {
                     for (int i = k0; i <= last; ++i)
                         fp64_iteration<BD>(i);
                 });
-                iterations_ = last;
+                for (int s = 0; s < K; ++s)
+                    if (h_active_[s])
+                        sys_[s].iterations = last;
                 if (last % ci == 0)
                 {
-                    read_scalars(sc);
-                    if (params_.use_preconditioned_residual_norm)
+                    read_scalars();
+                    bool changed = false;
+                    for (int s = 0; s < K; ++s)
                     {
-                        const double rz = sc[S_RZ];
-                        residual_norm_ = std::sqrt(rz);
-                        if (rz <= rel2 * rz0 || rz <= abs2)
+                        if (!h_active_[s])
+                            continue;
+                        System &S = sys_[s];
+                        const double v = params_.use_preconditioned_residual_norm ? *scal(s, S_RZ) : *scal(s, S_RR);
+                        const double v0 = params_.use_preconditioned_residual_norm ? rz0[s] : rr0[s];
+                        S.residual_norm = std::sqrt(v);
+                        S.rel_residual = bb[s] > 0.0 ? std::sqrt(*scal(s, S_RR) / bb[s]) : 0.0;
+                        if (v <= rel2 * v0 || v <= abs2)
                         {
-                            status_ = (rz <= abs2) ? Status::ReachAbsoluteTolerance : Status::ReachRelativeTolerance;
-                            converged = true;
+                            S.status = (v <= abs2) ? Status::ReachAbsoluteTolerance : Status::ReachRelativeTolerance;
+                            h_active_[s] = 0;
+                            --running;
+                            changed = true;
                         }
                     }
-                    else
-                    {
-                        const double rr = sc[S_RR];
-                        residual_norm_ = std::sqrt(rr);
-                        if (rr <= rel2 * rr0 || rr <= abs2)
-                        {
-                            status_ = (rr <= abs2) ? Status::ReachAbsoluteTolerance : Status::ReachRelativeTolerance;
-                            converged = true;
-                        }
-                    }
+                    if (changed && running > 0)
+                        upload_flags();
                 }
                 k = last + 1;
             }
-            if (iterations_ == params_.max_iter)
-                status_ = Status::ReachMaxIterations;
-            const double bb = rr0 > 0.0 ? rr0 : sc[S_RR];
-            true_rel_residual_ = bb > 0.0 ? residual_norm_ / std::sqrt(bb) : 0.0;
+            for (int s = 0; s < K; ++s)
+                if (sys_[s].iterations == params_.max_iter)
+                    sys_[s].status = Status::ReachMaxIterations;
         };
-        if (bd == 1)
+        if (bd_ == 1)
             body(std::integral_constant<int, 1>());
-        else if (bd == 2)
+        else if (bd_ == 2)
             body(std::integral_constant<int, 2>());
         else
             body(std::integral_constant<int, 3>());
@@ -734,12 +961,13 @@ namespace psmas
     void Solver::mixed_part_a()
     {
         const SellView<float> A = sell(d_vals32_.data());
-        tic("spmv_fp32", static_cast<double>(nnzb_) * (BD * BD * 4 + 4) + (nb_ + 1) * 4.0 + 2.0 * n_ * 4.0);
-        launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), d_sc_.data(), site(R_PAP), stream_);
+        tic("spmv_fp32", static_cast<double>(nnzb_) * (BD * BD * 4 + 4) + (nb_total_ + 1) * 4.0 + 2.0 * n_total_ * 4.0);
+        launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), d_sc_.data(), sysmap(), site(R_PAP),
+                                         stream_);
         toc();
-        launch_update_xr<BD, float>(npad_, d_p32_.data(), d_Ap32_.data(), d_x32_.data(), d_mr32_.data(),
+        launch_update_xr<BD, float>(nblocks_, d_p32_.data(), d_Ap32_.data(), d_x32_.data(), d_mr32_.data(),
                                     fine_restrict(), d_mr32_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
-                                    site(R_RR), stream_);
+                                    sysmap(), site(R_RR), stream_);
     }
 
     template <int BD>
@@ -751,66 +979,75 @@ namespace psmas
         launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), stream_);
         tic("mas_apply_fp16", mat_cnt_[0] * (L * 2.0 + dim * 4.0 + 2.0 * dim * 4.0));
         launch_mas_apply<BD, __half, float>(0, mat_cnt_[0], d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
-                                            d_mz32_.data(), stream_);
+                                            d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_);
         toc();
         launch_mas_apply<BD, double, float>(mat_cnt_[0], total_mats_ - mat_cnt_[0], d_mats64_.data(), nullptr,
-                                            d_mr32_.data(), d_mz32_.data(), stream_);
-        launch_prolong_dot<BD, float>(npad_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
-                                      d_mz32_.data(), d_sc_.data(), init, site(R_RZ), stream_);
+                                            d_mr32_.data(), d_mz32_.data(), nullptr, nullptr, stream_);
+        apply_dense_coarse<BD, float>(d_mr32_.data(), d_mz32_.data());
+        launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
+                                      d_mz32_.data(), d_sc_.data(), init, sysmap(), site(R_RZ), stream_);
         if (init)
             launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_p32_.data(), stream_);
         else
-            launch_update_p<float>(nvec, d_mz32_.data(), d_p32_.data(), d_sc_.data(), stream_);
+            launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), d_sc_.data(), sysmap(), stream_);
     }
 
     template <int BD>
     void Solver::mixed_reliable_residual()
     {
         const SellView<double> A = sell(d_vals64_.data());
-        tic("spmv_res_fp64", static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_ + 1) * 4.0 + 3.0 * n_ * 8.0);
+        tic("spmv_res_fp64",
+            static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_total_ + 1) * 4.0 + 3.0 * n_total_ * 8.0);
         launch_spmv_res<BD, double, double, float>(A, d_x_.data(), d_b_.data(), d_mr32_.data(), fine_restrict(),
                                                    d_mr32_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
-                                                   S_RR_TRUE, site(R_RR_TRUE), stream_);
+                                                   S_RR_TRUE, sysmap(), site(R_RR_TRUE), true, stream_);
         toc();
     }
 
     void Solver::solve_mixed()
     {
-        const int bd = bd_;
         auto body = [&](auto bdc) {
             constexpr int BD = decltype(bdc)::value;
+            const int K = static_cast<int>(sys_.size());
             const long long nvec = static_cast<long long>(npad_) * BD;
             launch_fill<double>(nvec, d_x_.data(), 0.0, stream_);
             launch_fill<float>(nvec, d_x32_.data(), 0.0f, stream_);
-            mixed_reliable_residual<BD>();
+            mixed_reliable_residual<BD>(); // every system selected: r = b
             mixed_part_b<BD>(true);
-            double sc[S_COUNT];
-            read_scalars(sc);
-            const double bb = sc[S_RR_TRUE];
-            const double bnorm = std::sqrt(bb);
-            if (std::isnan(sc[S_RZ0]) || !std::isfinite(sc[S_RZ0]))
-                throw std::runtime_error("[MAS] Invalid initial residual.");
-            const double target = std::max(params_.rel_tol * bnorm, params_.abs_tol);
-            residual_norm_ = bnorm;
-            true_rel_residual_ = bb > 0.0 ? 1.0 : 0.0;
-            if (bnorm <= params_.abs_tol)
+            read_scalars();
+            std::vector<double> bnorm(K), target(K), ref(K), best(K);
+            std::vector<int> stalls(K, 0);
+            int running = K;
+            for (int s = 0; s < K; ++s)
             {
-                status_ = Status::ReachAbsoluteTolerance;
-                return;
+                System &S = sys_[s];
+                const double rz0 = *scal(s, S_RZ0);
+                if (std::isnan(rz0) || !std::isfinite(rz0))
+                    throw std::runtime_error(K == 1 ? std::string("[MAS] Invalid initial residual.")
+                                                    : "[MAS] Invalid initial residual (system " +
+                                                          std::to_string(s) + ").");
+                bnorm[s] = std::sqrt(*scal(s, S_RR_TRUE));
+                target[s] = std::max(params_.rel_tol * bnorm[s], params_.abs_tol);
+                ref[s] = best[s] = bnorm[s];
+                S.residual_norm = bnorm[s];
+                S.rel_residual = bnorm[s] > 0.0 ? 1.0 : 0.0;
+                if (bnorm[s] <= params_.abs_tol)
+                {
+                    S.status = Status::ReachAbsoluteTolerance;
+                    h_active_[s] = 0;
+                    --running;
+                }
             }
-            double ref = bnorm;
-            double best = bnorm;
-            int stalls = 0;
+            std::fill(h_select_.begin(), h_select_.end(), 0);
+            upload_flags();
             bool first = true;
-            bool converged = false;
             int k = 0;
             const int ci = params_.check_interval;
-            while (k < params_.max_iter && !converged)
+            while (k < params_.max_iter && running > 0)
             {
                 const int count = std::min(ci, params_.max_iter - k);
-                const int key = ((first ? 1 : 2) << 20) | count;
                 const bool lead_b = !first;
-                run_block(key, count, [&]() {
+                run_block(graph_key(1, first ? 1 : 2, count), [&]() {
                     if (lead_b)
                         mixed_part_b<BD>(false);
                     for (int i = 0; i < count; ++i)
@@ -822,43 +1059,72 @@ namespace psmas
                 });
                 first = false;
                 k += count;
-                iterations_ = k;
-                read_scalars(sc);
-                const double rn = std::sqrt(std::max(sc[S_RR], 0.0));
-                if (!(rn <= params_.reliable_update_delta * ref || rn <= target) && k < params_.max_iter)
+                read_scalars();
+                int nsel = 0;
+                for (int s = 0; s < K; ++s)
+                {
+                    h_select_[s] = 0;
+                    if (!h_active_[s])
+                        continue;
+                    sys_[s].iterations = k;
+                    const double rn = std::sqrt(std::max(*scal(s, S_RR), 0.0));
+                    if (rn <= params_.reliable_update_delta * ref[s] || rn <= target[s] || k >= params_.max_iter ||
+                        !std::isfinite(rn))
+                    {
+                        h_select_[s] = 1;
+                        ++nsel;
+                    }
+                }
+                if (nsel == 0)
                     continue;
                 // Reliable update: fold the FP32 correction into x and recompute r in FP64.
-                launch_accumulate_correction(nvec, d_x32_.data(), d_x_.data(), stream_);
+                upload_flags();
+                launch_accumulate_correction<BD>(nblocks_, d_x32_.data(), d_x_.data(), sysmap(), stream_);
                 mixed_reliable_residual<BD>();
-                ++reliable_updates_;
-                const double rt = std::sqrt(std::max(read_scalar(S_RR_TRUE), 0.0));
-                residual_norm_ = rt;
-                true_rel_residual_ = rt / bnorm;
-                if (rt <= target)
+                read_scalars();
+                for (int s = 0; s < K; ++s)
                 {
-                    status_ = (rt <= params_.abs_tol) ? Status::ReachAbsoluteTolerance
-                                                      : Status::ReachRelativeTolerance;
-                    converged = true;
-                    break;
+                    if (!h_select_[s])
+                        continue;
+                    System &S = sys_[s];
+                    ++S.reliable_updates;
+                    const double rt = std::sqrt(std::max(*scal(s, S_RR_TRUE), 0.0));
+                    S.residual_norm = rt;
+                    S.rel_residual = bnorm[s] > 0.0 ? rt / bnorm[s] : 0.0;
+                    bool stop = false;
+                    if (rt <= target[s])
+                    {
+                        S.status = (rt <= params_.abs_tol) ? Status::ReachAbsoluteTolerance
+                                                           : Status::ReachRelativeTolerance;
+                        stop = true;
+                    }
+                    else
+                    {
+                        stalls[s] = (rt < 0.9 * best[s]) ? 0 : stalls[s] + 1;
+                        if (stalls[s] >= 8 || !std::isfinite(rt))
+                        {
+                            S.status = Status::ReachMaxIterations;
+                            stop = true;
+                        }
+                        best[s] = std::min(best[s], rt);
+                        ref[s] = rt;
+                    }
+                    if (stop)
+                    {
+                        h_active_[s] = 0;
+                        --running;
+                    }
+                    h_select_[s] = 0;
                 }
-                if (!(rt < 0.9 * best))
-                {
-                    if (++stalls >= 8)
-                        break;
-                }
-                else
-                {
-                    stalls = 0;
-                }
-                best = std::min(best, rt);
-                ref = rt;
+                upload_flags();
             }
-            if (!converged)
-                status_ = Status::ReachMaxIterations;
+            for (int s = 0; s < K; ++s)
+                if (sys_[s].status == Status::Running)
+                    sys_[s].status = Status::ReachMaxIterations;
         };
-        if (bd == 1)
+        if (bd_ == 1)
             body(std::integral_constant<int, 1>());
-        else if (bd == 2)
+        else if (bd_ == 2)
             body(std::integral_constant<int, 2>());
         else
             body(std::integral_constant<int, 3>());
@@ -869,18 +1135,20 @@ namespace psmas
     {
         if (!factorized_)
             throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
-        offsets.assign(nparts_ + 1, 0);
-        for (int p = 0; p <= part_.nparts; ++p)
-            offsets[p] = part_.offsets[p];
-        for (int p = part_.nparts + 1; p <= nparts_; ++p)
-            offsets[p] = part_.offsets[part_.nparts];
-        perm = part_.perm;
+        const System &S = sys_[0];
+        offsets.assign(S.nparts + 1, 0);
+        for (int p = 0; p <= S.part.nparts; ++p)
+            offsets[p] = S.part.offsets[p];
+        for (int p = S.part.nparts + 1; p <= S.nparts; ++p)
+            offsets[p] = S.part.offsets[S.part.nparts];
+        perm = S.part.perm;
     }
 
     std::vector<int> Solver::level_info() const
     {
-        return {level_num_, cco_[0], cco_[1], cco_[2], cco_[3], nparts_, npad_, total_mats_,
-                static_cast<int>(std::min<long long>(nnzb_, 2147483647LL)), static_cast<int>(nslots_), nb_, n_};
+        const System &S = sys_[0];
+        return {level_num_, cco_[0], cco_[1], cco_[2], cco_[3], S.nparts, S.npad, total_mats_,
+                static_cast<int>(std::min<long long>(S.nnzb, 2147483647LL)), static_cast<int>(nslots_), S.nb, S.n};
     }
 
     std::vector<int> Solver::coarse_map() const
@@ -889,10 +1157,11 @@ namespace psmas
             throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
         std::vector<int4> cm(npad_);
         PSMAS_HIP(hipMemcpy(cm.data(), d_cmap_.data(), sizeof(int4) * npad_, hipMemcpyDeviceToHost));
-        std::vector<int> out(static_cast<size_t>(nb_) * 4);
-        for (int b = 0; b < nb_; ++b)
+        const System &S = sys_[0];
+        std::vector<int> out(static_cast<size_t>(S.nb) * 4);
+        for (int b = 0; b < S.nb; ++b)
         {
-            const int4 c = cm[slot_of_block_[b]];
+            const int4 c = cm[S.node_off + S.slot_of_block[b]];
             out[4 * b + 0] = c.x;
             out[4 * b + 1] = c.y;
             out[4 * b + 2] = c.z;
@@ -913,12 +1182,17 @@ namespace psmas
     void Solver::apply_preconditioner(const double *r, double *z, bool mixed)
     {
         PSMAS_HIP(hipSetDevice(device_));
-        check_ready(n_);
+        check_ready_single(dim());
+        const System &S = sys_[0];
         const long long nvec = static_cast<long long>(npad_) * bd_;
-        PSMAS_HIP(hipMemcpyAsync(d_io_.data(), r, sizeof(double) * n_, hipMemcpyHostToDevice, stream_));
-        launch_gather_in<double>(npad_, bd_, n_, d_block_of_slot_.data(), d_io_.data(), d_p_.data(), stream_);
+        h_active_.assign(1, 1);
+        h_select_.assign(1, 1);
+        upload_flags();
+        PSMAS_HIP(hipMemcpyAsync(d_io_.data(), r, sizeof(double) * S.n, hipMemcpyHostToDevice, stream_));
+        launch_gather_in<double>(npad_, bd_, S.n, d_block_of_slot_.data(), d_io_.data(), d_p_.data(), stream_);
         launch_fill<double>(nvec, d_Ap_.data(), 0.0, stream_);
-        launch_set_scalar(d_sc_.data(), S_GUARD, params_.division_guard, stream_);
+        launch_set_scalar(d_sc_.data(), 1, S_GUARD, params_.division_guard, stream_);
+        ensure_coarse_solver();
         if (mixed)
             ensure_mixed_data();
         auto body = [&](auto bdc) {
@@ -930,19 +1204,23 @@ namespace psmas
                 launch_spmv_res<BD, double, double, double>(A, d_Ap_.data(), d_p_.data(), d_mr_.data(),
                                                             fine_restrict(),
                                                             d_mr_.data() + static_cast<size_t>(npad_) * BD,
-                                                            d_sc_.data(), S_RR_TRUE, site(R_RR_TRUE), stream_);
+                                                            d_sc_.data(), S_RR_TRUE, sysmap(), site(R_RR_TRUE),
+                                                            false, stream_);
                 fp64_precondition<BD>(true);
-                launch_scatter_out<double>(nb_, BD, n_, d_slot_of_block_.data(), d_mz_.data(), d_io_.data(), stream_);
+                launch_scatter_out<double>(S.nb, BD, S.n, d_slot_of_block_.data(), d_mz_.data(), d_io_.data(),
+                                           stream_);
             }
             else
             {
                 launch_spmv_res<BD, double, double, float>(A, d_Ap_.data(), d_p_.data(), d_mr32_.data(),
                                                            fine_restrict(),
                                                            d_mr32_.data() + static_cast<size_t>(npad_) * BD,
-                                                           d_sc_.data(), S_RR_TRUE, site(R_RR_TRUE), stream_);
+                                                           d_sc_.data(), S_RR_TRUE, sysmap(), site(R_RR_TRUE),
+                                                           false, stream_);
                 mixed_part_b<BD>(true);
                 launch_copy_convert<float, double>(nvec, d_mz32_.data(), d_Ap_.data(), stream_);
-                launch_scatter_out<double>(nb_, BD, n_, d_slot_of_block_.data(), d_Ap_.data(), d_io_.data(), stream_);
+                launch_scatter_out<double>(S.nb, BD, S.n, d_slot_of_block_.data(), d_Ap_.data(), d_io_.data(),
+                                           stream_);
             }
         };
         if (bd_ == 1)
@@ -951,7 +1229,7 @@ namespace psmas
             body(std::integral_constant<int, 2>());
         else
             body(std::integral_constant<int, 3>());
-        PSMAS_HIP(hipMemcpyAsync(z, d_io_.data(), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipMemcpyAsync(z, d_io_.data(), sizeof(double) * S.n, hipMemcpyDeviceToHost, stream_));
         PSMAS_HIP(hipStreamSynchronize(stream_));
         collect_timings();
     }
@@ -959,10 +1237,11 @@ namespace psmas
     void Solver::spmv(const double *x, double *y, bool fp32)
     {
         PSMAS_HIP(hipSetDevice(device_));
-        check_ready(n_);
+        check_ready_single(dim());
+        const System &S = sys_[0];
         const long long nvec = static_cast<long long>(npad_) * bd_;
-        PSMAS_HIP(hipMemcpyAsync(d_io_.data(), x, sizeof(double) * n_, hipMemcpyHostToDevice, stream_));
-        launch_gather_in<double>(npad_, bd_, n_, d_block_of_slot_.data(), d_io_.data(), d_p_.data(), stream_);
+        PSMAS_HIP(hipMemcpyAsync(d_io_.data(), x, sizeof(double) * S.n, hipMemcpyHostToDevice, stream_));
+        launch_gather_in<double>(npad_, bd_, S.n, d_block_of_slot_.data(), d_io_.data(), d_p_.data(), stream_);
         if (fp32)
             ensure_mixed_data();
         auto body = [&](auto bdc) {
@@ -984,8 +1263,8 @@ namespace psmas
             body(std::integral_constant<int, 2>());
         else
             body(std::integral_constant<int, 3>());
-        launch_scatter_out<double>(nb_, bd_, n_, d_slot_of_block_.data(), d_Ap_.data(), d_io_.data(), stream_);
-        PSMAS_HIP(hipMemcpyAsync(y, d_io_.data(), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+        launch_scatter_out<double>(S.nb, bd_, S.n, d_slot_of_block_.data(), d_Ap_.data(), d_io_.data(), stream_);
+        PSMAS_HIP(hipMemcpyAsync(y, d_io_.data(), sizeof(double) * S.n, hipMemcpyDeviceToHost, stream_));
         PSMAS_HIP(hipStreamSynchronize(stream_));
     }
 

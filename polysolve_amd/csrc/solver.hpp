@@ -3,7 +3,11 @@
 // Public surface mirrors polysolve::linear::MASSolver (src/polysolve/linear/MASSolver.hpp:35-72):
 // set_parameters / get_info / analyze_pattern / factorize / solve / name, with the same
 // semantics (MASSolver.cu:134-639), plus device-pointer solves, optional node coordinates for
-// Morton aggregation, and introspection used by the parity tests.
+// Morton aggregation, batches of independent systems solved in one launch sequence, and
+// introspection used by the parity tests.
+//
+// Internally every factorisation is a batch of K >= 1 systems laid out one after another in the
+// bank-padded node space; a single solve is K = 1.
 #pragma once
 
 #include "common.hpp"
@@ -47,19 +51,23 @@ namespace psmas
         Solver &operator=(const Solver &) = delete;
 
         void set_parameters_json(const std::string &json);
-        std::string get_info_json() const;
+        std::string get_info_json() const { return system_info_json(0); }
+        std::string system_info_json(int s) const;
         std::string parameters_json() const { return params_to_json(params_); }
         void analyze_pattern(const CscView &) {}
-        void factorize(const CscView &A);
+        void factorize(const CscView &A) { factorize_batch(std::vector<CscView>{A}); }
+        void factorize_batch(const std::vector<CscView> &As);
         void solve_host(int n, const double *b, double *x);
         void solve_device(int n, const double *d_b, double *d_x);
-        void set_coordinates(int nb, int dim, const double *xyz);
+        void solve_batch(const std::vector<const double *> &b, const std::vector<double *> &x, bool device);
+        void set_coordinates(int sys, int nb, int dim, const double *xyz);
+        int num_systems() const { return static_cast<int>(sys_.size()); }
 
-        // Introspection for tests (host arrays, original ordering unless stated).
+        // Introspection for tests (host arrays, original ordering unless stated; system 0).
         void get_partition(std::vector<int> &offsets, std::vector<int> &perm) const;
         std::vector<int> level_info() const;
-        std::vector<int> coarse_map() const;                 ///< nb x 4, per input block
-        std::vector<double> bank_inverses() const;           ///< packed FP64, all levels
+        std::vector<int> coarse_map() const;       ///< nb x 4, per input block
+        std::vector<double> bank_inverses() const; ///< packed FP64, all levels
         void apply_preconditioner(const double *r, double *z, bool mixed);
         void spmv(const double *x, double *y, bool fp32);
 
@@ -67,31 +75,44 @@ namespace psmas
         std::vector<KernelTiming> kernel_timings() const;
         void reset_kernel_timings();
         int block_dim() const { return bd_; }
-        int dim() const { return n_; }
+        int dim() const { return sys_.empty() ? 0 : sys_[0].n; }
 
     private:
         struct Graphs;
 
+        /// One system of the batch.
+        struct System
+        {
+            int n = 0, nb = 0, nparts = 0, npad = 0;
+            int node_off = 0;  ///< first padded node
+            int blk_begin = 0; ///< first node block
+            long long nnzb = 0;
+            Partition part;
+            bool have_partition = false;
+            std::vector<int> slot_of_block; ///< local slots
+            std::vector<double> coords;
+            int coord_dim = 0;
+            std::string aggregation;
+            int sob_off = 0; ///< offset of this system's slot_of_block in the device array
+            // last solve
+            int iterations = 0;
+            double residual_norm = 0.0;
+            Status status = Status::Running;
+            int reliable_updates = 0;
+            double rel_residual = 0.0;
+        };
+
         Params params_;
-        int iterations_ = 0;
-        double residual_norm_ = 0.0;
-        Status status_ = Status::Running;
-        int reliable_updates_ = 0;
-        double true_rel_residual_ = 0.0;
+        std::vector<System> sys_;
         double solve_ms_ = 0.0;
 
         hipStream_t stream_ = nullptr;
         int device_ = 0;
 
-        int n_ = 0, bd_ = 0, nb_ = 0, npad_ = 0, nparts_ = 0;
-        long long nnzb_ = 0;
+        int bd_ = 0, npad_ = 0, nparts_ = 0, nblocks_ = 0;
+        long long nnzb_ = 0, nnz_scalar_ = 0, n_total_ = 0, nb_total_ = 0;
         bool factorized_ = false;
-        std::vector<double> coords_;
-        int coord_dim_ = 0;
-        Partition part_;
-        bool have_partition_ = false;
-        std::vector<int> slot_of_block_, block_of_slot_;
-        std::string aggregation_used_;
+        std::vector<int> block_of_slot_; ///< global slot -> local block (-1 virtual)
 
         // SELL matrix
         int nslices_ = 0;
@@ -99,6 +120,10 @@ namespace psmas
         DeviceBuffer<int> d_slice_ptr_, d_cols_, d_row_len_, d_block_of_slot_, d_slot_of_block_;
         DeviceBuffer<double> d_vals64_;
         DeviceBuffer<float> d_vals32_;
+
+        // systems
+        DeviceBuffer<int> d_block_sys_, d_blk_begin_, d_blk_count_, d_active_, d_select_, d_bank_sys_;
+        std::vector<int> h_active_, h_select_;
 
         // MAS
         int level_num_ = 0;
@@ -117,6 +142,15 @@ namespace psmas
         DeviceBuffer<__half> d_mats16_;
         DeviceBuffer<float> d_dscale_;
 
+        // exact coarsest-level solve (coarse_solver = "cholesky")
+        std::vector<char> nonisolated_; ///< slot has an off-diagonal block
+        bool dense_built_ = false;      ///< tried for this factorisation
+        int dense_nd_ = 0;              ///< rows of the dense coarse matrix (0 = not used)
+        int dense_nodes_ = 0;
+        std::string dense_note_;
+        DeviceBuffer<double> d_dense_inv_;
+        DeviceBuffer<int> d_dense_ids_;
+
         // vectors
         DeviceBuffer<double> d_b_, d_x_, d_mr_, d_mz_, d_p_, d_Ap_, d_io_;
         DeviceBuffer<float> d_x32_, d_mr32_, d_mz32_, d_p32_, d_Ap32_;
@@ -124,6 +158,7 @@ namespace psmas
         DeviceBuffer<double> d_partials_;
         DeviceBuffer<unsigned> d_counters_;
         int partial_cap_ = 0;
+        std::vector<double> h_sc_;
 
         std::unique_ptr<Graphs> graphs_;
 
@@ -134,21 +169,27 @@ namespace psmas
         size_t event_pool_used_ = 0;
 
         ReduceSite site(int s) const;
+        SysMap sysmap() const;
         FineRestrict fine_restrict() const;
         CoarseRestrict coarse_restrict() const;
         template <typename TV>
         SellView<TV> sell(const TV *vals) const;
 
-        void build_partition(const CscView &A);
+        void build_partition(System &S, const CscView &A);
         void upload_matrix(const PaddedBsr &bsr);
         void build_mas();
         void ensure_mixed_data();
+        void ensure_coarse_solver();
+        template <int BD, typename T>
+        void apply_dense_coarse(const T *mr, T *mz);
         void allocate_vectors();
-        void check_ready(int n) const;
+        void check_ready_single(int n) const;
 
         void solve_padded();
         void solve_fp64();
         void solve_mixed();
+        void upload_flags();
+        double *scal(int s, int slot) { return &h_sc_[static_cast<size_t>(s) * 16 + slot]; }
 
         // per-iteration pieces
         template <int BD>
@@ -162,7 +203,7 @@ namespace psmas
         template <int BD>
         void mixed_reliable_residual();
 
-        void run_block(int key, int count, const std::function<void()> &enqueue);
+        void run_block(int key, const std::function<void()> &enqueue);
 
         // timing helpers
         void tic(const char *name, double bytes);
@@ -172,7 +213,6 @@ namespace psmas
         double pending_bytes_ = 0.0;
         hipEvent_t pending_start_ = nullptr;
 
-        double read_scalar(int slot);
-        void read_scalars(double *out);
+        void read_scalars();
     };
 } // namespace psmas

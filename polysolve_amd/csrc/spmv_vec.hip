@@ -6,10 +6,13 @@
 // (mas_utils/MASPreconditioner.cu:593-659).  Layout and fusion are this build's own:
 //  * one lane per block row; a wave owns a slice of 64 rows = two 32-node banks, so every
 //    matrix load of a wave is one contiguous 512 B (FP64) / 256 B (FP32) run;
-//  * reductions are deterministic (per-workgroup partials, summed in index order by the last
-//    workgroup) and their scalar epilogues (alpha, beta) run on the device;
+//  * a workgroup owns 256 consecutive rows (a node block); a batch of systems is laid out so
+//    that each system covers whole node blocks, and every reduction is a deterministic
+//    per-system sum (per-workgroup partials, summed in index order by the system's last
+//    workgroup), whose scalar epilogue (alpha, beta) runs on the device;
 //  * the residual-producing kernels also restrict the residual to coarse level 0 through the
-//    bank-local component masks, so the preconditioner never re-reads r for that step.
+//    bank-local component masks, so the preconditioner never re-reads r for that step;
+//  * systems that have converged (active == 0) skip their workgroups.
 #include "device.hpp"
 #include "kernels.hpp"
 #include "common.hpp"
@@ -18,20 +21,13 @@ namespace psmas
 {
     namespace
     {
-        constexpr int SPMV_BLOCK = 256; // 4 waves = 4 slices
-        constexpr int VEC_BLOCK = 256;
+        constexpr int SPMV_BLOCK = NODE_BLOCK; // 4 waves = 4 slices of 64 rows
+        constexpr int VEC_BLOCK = NODE_BLOCK;
 
         template <typename T>
-        struct Acc;
-        template <>
-        struct Acc<double>
+        struct Acc
         {
-            using type = double;
-        };
-        template <>
-        struct Acc<float>
-        {
-            using type = float;
+            using type = T;
         };
 
         template <int BD, typename TV, typename TX, typename TA>
@@ -59,40 +55,43 @@ namespace psmas
             }
         }
 
+        __device__ __forceinline__ int block_system(const SysMap &sm) { return sm.block_sys[blockIdx.x]; }
+
         template <int BD, typename TV, typename TX>
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap(SellView<TV> A, const TX *__restrict__ x,
-                                                                TX *__restrict__ y, double *__restrict__ sc,
+                                                                TX *__restrict__ y, double *__restrict__ sc, SysMap sm,
                                                                 ReduceSite site)
         {
             using TA = typename Acc<TV>::type;
+            const int sys = block_system(sm);
+            if (sm.active[sys] == 0)
+                return;
             const int lane = threadIdx.x % WAVE;
             const int slice = blockIdx.x * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            TA acc[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                acc[i] = TA(0);
+            sell_row<BD>(A, slice, lane, x, acc);
+            const size_t row = static_cast<size_t>(slice) * WAVE + lane;
             double dot = 0.0;
-            if (slice < A.nslices)
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
             {
-                TA acc[BD];
-#pragma unroll
-                for (int i = 0; i < BD; ++i)
-                    acc[i] = TA(0);
-                sell_row<BD>(A, slice, lane, x, acc);
-                const size_t row = static_cast<size_t>(slice) * WAVE + lane;
-#pragma unroll
-                for (int i = 0; i < BD; ++i)
-                {
-                    const TX yi = static_cast<TX>(acc[i]);
-                    y[row * BD + i] = yi;
-                    dot += static_cast<double>(x[row * BD + i]) * static_cast<double>(yi);
-                }
+                const TX yi = static_cast<TX>(acc[i]);
+                y[row * BD + i] = yi;
+                dot += static_cast<double>(x[row * BD + i]) * static_cast<double>(yi);
             }
-            grid_sum<SPMV_BLOCK>(dot, site.partials, site.counter, [sc](double t) {
-                sc[S_PAP] = t;
-                sc[S_ALPHA] = safe_div(sc[S_RZ], t, sc[S_GUARD]);
+            double *scs = sc + static_cast<size_t>(sys) * S_COUNT;
+            system_sum<SPMV_BLOCK>(dot, sys, sm, site.partials, site.counter, [scs](double t) {
+                scs[S_PAP] = t;
+                scs[S_ALPHA] = safe_div(scs[S_RZ], t, scs[S_GUARD]);
             });
         }
 
         /// Leader lanes of a bank sum their component's members from LDS into the coarse vector.
         template <int BD, typename T>
-        __device__ __forceinline__ void restrict_bank(const T (*sv)[BD], int bank_base_in_block, uint32_t mask, int gid,
+        __device__ __forceinline__ void restrict_bank(const T *sv, int bank_base_in_block, uint32_t mask, int gid,
                                                       T *__restrict__ yc0)
         {
             if (mask == 0u)
@@ -107,7 +106,7 @@ namespace psmas
                 mask &= mask - 1u;
 #pragma unroll
                 for (int i = 0; i < BD; ++i)
-                    s[i] += static_cast<double>(sv[bank_base_in_block + b][i]);
+                    s[i] += static_cast<double>(sv[(bank_base_in_block + b) * BD + i]);
             }
 #pragma unroll
             for (int i = 0; i < BD; ++i)
@@ -118,38 +117,39 @@ namespace psmas
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_res(SellView<TV> A, const TX *__restrict__ x,
                                                                  const TX *__restrict__ b, TY *__restrict__ y,
                                                                  FineRestrict fr, TY *__restrict__ yc0,
-                                                                 double *__restrict__ sc, int rr_slot, ReduceSite site)
+                                                                 double *__restrict__ sc, int rr_slot, SysMap sm,
+                                                                 ReduceSite site, int selected_only)
         {
             using TA = typename Acc<TV>::type;
-            __shared__ TY sv[SPMV_BLOCK][BD];
+            __shared__ TY sv[SPMV_BLOCK * BD];
+            const int sys = block_system(sm);
+            if (sm.active[sys] == 0 || (selected_only && sm.select[sys] == 0))
+                return;
             const int lane = threadIdx.x % WAVE;
             const int slice = blockIdx.x * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
+            TA acc[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                acc[i] = TA(0);
+            sell_row<BD>(A, slice, lane, x, acc);
             double dot = 0.0;
-            uint32_t mask = 0u;
-            int gid = 0;
-            if (slice < A.nslices)
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
             {
-                TA acc[BD];
-#pragma unroll
-                for (int i = 0; i < BD; ++i)
-                    acc[i] = TA(0);
-                sell_row<BD>(A, slice, lane, x, acc);
-#pragma unroll
-                for (int i = 0; i < BD; ++i)
-                {
-                    const double ri = static_cast<double>(b[row * BD + i]) - static_cast<double>(acc[i]);
-                    const TY yi = static_cast<TY>(ri);
-                    y[row * BD + i] = yi;
-                    sv[threadIdx.x][i] = yi;
-                    dot += ri * ri;
-                }
-                mask = fr.lead[row];
-                gid = fr.gid[row];
+                const double ri = static_cast<double>(b[row * BD + i]) - static_cast<double>(acc[i]);
+                const TY yi = static_cast<TY>(ri);
+                y[row * BD + i] = yi;
+                sv[threadIdx.x * BD + i] = yi;
+                dot += ri * ri;
             }
+            const uint32_t mask = fr.lead[row];
+            const int gid = fr.gid[row];
             __syncthreads();
             restrict_bank<BD, TY>(sv, (threadIdx.x / 32) * 32, mask, gid, yc0);
-            grid_sum<SPMV_BLOCK>(dot, site.partials, site.counter, [sc, rr_slot](double t) { sc[rr_slot] = t; });
+            double *scs = sc + static_cast<size_t>(sys) * S_COUNT;
+            system_sum<SPMV_BLOCK>(dot, sys, sm, site.partials, site.counter,
+                                   [scs, rr_slot](double t) { scs[rr_slot] = t; });
         }
 
         template <int BD, typename TV, typename TX>
@@ -172,56 +172,90 @@ namespace psmas
                 y[row * BD + i] = static_cast<TX>(acc[i]);
         }
 
+        /// x += alpha p; r -= alpha Ap on the block's 256*BD contiguous scalars (coalesced), then
+        /// r . r and the restriction of r to coarse level 0 from LDS.
         template <int BD, typename T>
-        __global__ __launch_bounds__(VEC_BLOCK) void k_update_xr(int nnodes, const T *__restrict__ p,
-                                                                 const T *__restrict__ Ap, T *__restrict__ x,
-                                                                 T *__restrict__ r, FineRestrict fr, T *__restrict__ rc0,
-                                                                 double *__restrict__ sc, ReduceSite site)
+        __global__ __launch_bounds__(VEC_BLOCK) void k_update_xr(const T *__restrict__ p, const T *__restrict__ Ap,
+                                                                 T *__restrict__ x, T *__restrict__ r, FineRestrict fr,
+                                                                 T *__restrict__ rc0, double *__restrict__ sc,
+                                                                 SysMap sm, ReduceSite site)
         {
-            __shared__ T sv[VEC_BLOCK][BD];
-            const int node = blockIdx.x * VEC_BLOCK + threadIdx.x;
-            const T alpha = static_cast<T>(sc[S_ALPHA]);
+            __shared__ T sv[VEC_BLOCK * BD];
+            const int sys = block_system(sm);
+            if (sm.active[sys] == 0)
+                return;
+            double *scs = sc + static_cast<size_t>(sys) * S_COUNT;
+            const T alpha = static_cast<T>(scs[S_ALPHA]);
+            const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
             double dot = 0.0;
-            uint32_t mask = 0u;
-            int gid = 0;
-            if (node < nnodes)
-            {
-                const size_t o = static_cast<size_t>(node) * BD;
 #pragma unroll
-                for (int i = 0; i < BD; ++i)
-                {
-                    const T pv = p[o + i];
-                    const T av = Ap[o + i];
-                    x[o + i] = fma(alpha, pv, x[o + i]);
-                    const T rv = fma(-alpha, av, r[o + i]);
-                    r[o + i] = rv;
-                    sv[threadIdx.x][i] = rv;
-                    dot += static_cast<double>(rv) * static_cast<double>(rv);
-                }
-                mask = fr.lead[node];
-                gid = fr.gid[node];
+            for (int k = 0; k < BD; ++k)
+            {
+                const int j = threadIdx.x + k * VEC_BLOCK;
+                const size_t o = base + j;
+                x[o] = fma(alpha, p[o], x[o]);
+                const T rv = fma(-alpha, Ap[o], r[o]);
+                r[o] = rv;
+                sv[j] = rv;
+                dot += static_cast<double>(rv) * static_cast<double>(rv);
             }
+            const size_t node = static_cast<size_t>(blockIdx.x) * VEC_BLOCK + threadIdx.x;
+            const uint32_t mask = fr.lead[node];
+            const int gid = fr.gid[node];
             __syncthreads();
             restrict_bank<BD, T>(sv, (threadIdx.x / 32) * 32, mask, gid, rc0);
-            grid_sum<VEC_BLOCK>(dot, site.partials, site.counter, [sc](double t) { sc[S_RR] = t; });
+            system_sum<VEC_BLOCK>(dot, sys, sm, site.partials, site.counter, [scs](double t) { scs[S_RR] = t; });
         }
 
-        template <typename T>
-        __global__ void k_axpy_alpha(long long n, const T *__restrict__ p, T *__restrict__ x, const double *sc)
+        template <int BD, typename T>
+        __global__ __launch_bounds__(VEC_BLOCK) void k_axpy_alpha(const T *__restrict__ p, T *__restrict__ x,
+                                                                  const double *__restrict__ sc, SysMap sm)
         {
-            const T alpha = static_cast<T>(sc[S_ALPHA]);
-            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
-                 i += static_cast<long long>(gridDim.x) * blockDim.x)
-                x[i] = fma(alpha, p[i], x[i]);
+            const int sys = block_system(sm);
+            if (sm.active[sys] == 0)
+                return;
+            const T alpha = static_cast<T>(sc[static_cast<size_t>(sys) * S_COUNT + S_ALPHA]);
+            const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
+#pragma unroll
+            for (int k = 0; k < BD; ++k)
+            {
+                const size_t o = base + threadIdx.x + k * VEC_BLOCK;
+                x[o] = fma(alpha, p[o], x[o]);
+            }
         }
 
-        template <typename T>
-        __global__ void k_update_p(long long n, const T *__restrict__ z, T *__restrict__ p, const double *sc)
+        template <int BD, typename T>
+        __global__ __launch_bounds__(VEC_BLOCK) void k_update_p(const T *__restrict__ z, T *__restrict__ p,
+                                                                const double *__restrict__ sc, SysMap sm)
         {
-            const T beta = static_cast<T>(sc[S_BETA]);
-            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
-                 i += static_cast<long long>(gridDim.x) * blockDim.x)
-                p[i] = fma(beta, p[i], z[i]);
+            const int sys = block_system(sm);
+            if (sm.active[sys] == 0)
+                return;
+            const T beta = static_cast<T>(sc[static_cast<size_t>(sys) * S_COUNT + S_BETA]);
+            const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
+#pragma unroll
+            for (int k = 0; k < BD; ++k)
+            {
+                const size_t o = base + threadIdx.x + k * VEC_BLOCK;
+                p[o] = fma(beta, p[o], z[o]);
+            }
+        }
+
+        template <int BD>
+        __global__ __launch_bounds__(VEC_BLOCK) void k_accumulate_correction(float *__restrict__ x32,
+                                                                             double *__restrict__ x64, SysMap sm)
+        {
+            const int sys = block_system(sm);
+            if (sm.active[sys] == 0 || sm.select[sys] == 0)
+                return;
+            const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
+#pragma unroll
+            for (int k = 0; k < BD; ++k)
+            {
+                const size_t o = base + threadIdx.x + k * VEC_BLOCK;
+                x64[o] += static_cast<double>(x32[o]);
+                x32[o] = 0.0f;
+            }
         }
 
         template <typename TS, typename TD>
@@ -230,16 +264,6 @@ namespace psmas
             for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
                  i += static_cast<long long>(gridDim.x) * blockDim.x)
                 dst[i] = static_cast<TD>(src[i]);
-        }
-
-        __global__ void k_accumulate_correction(long long n, float *__restrict__ x32, double *__restrict__ x64)
-        {
-            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
-                 i += static_cast<long long>(gridDim.x) * blockDim.x)
-            {
-                x64[i] += static_cast<double>(x32[i]);
-                x32[i] = 0.0f;
-            }
         }
 
         template <typename T>
@@ -277,7 +301,12 @@ namespace psmas
             }
         }
 
-        __global__ void k_set_scalar(double *sc, int slot, double value) { sc[slot] = value; }
+        __global__ void k_set_scalar(double *sc, int stride, int count, int slot, double value)
+        {
+            const int s = blockIdx.x * blockDim.x + threadIdx.x;
+            if (s < count)
+                sc[static_cast<size_t>(s) * stride + slot] = value;
+        }
 
         template <typename T>
         __global__ __launch_bounds__(VEC_BLOCK) void k_dot(long long n, const T *__restrict__ a,
@@ -301,20 +330,21 @@ namespace psmas
     } // namespace
 
     template <int BD, typename TV, typename TX>
-    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, double *sc, ReduceSite site, hipStream_t s)
+    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, double *sc, const SysMap &sm, ReduceSite site,
+                        hipStream_t s)
     {
-        const int grid = div_up(A.nslices, SPMV_BLOCK / WAVE);
-        hipLaunchKernelGGL((k_spmv_ap<BD, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, y, sc, site);
+        const int grid = A.nslices / (SPMV_BLOCK / WAVE);
+        hipLaunchKernelGGL((k_spmv_ap<BD, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, y, sc, sm, site);
         PSMAS_LAUNCH_CHECK();
     }
 
     template <int BD, typename TV, typename TX, typename TY>
     void launch_spmv_res(const SellView<TV> &A, const TX *x, const TX *b, TY *y, FineRestrict fr, TY *yc0,
-                         double *sc, int rr_slot, ReduceSite site, hipStream_t s)
+                         double *sc, int rr_slot, const SysMap &sm, ReduceSite site, bool selected_only, hipStream_t s)
     {
-        const int grid = div_up(A.nslices, SPMV_BLOCK / WAVE);
+        const int grid = A.nslices / (SPMV_BLOCK / WAVE);
         hipLaunchKernelGGL((k_spmv_res<BD, TV, TX, TY>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, b, y, fr, yc0, sc,
-                           rr_slot, site);
+                           rr_slot, sm, site, selected_only ? 1 : 0);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -327,39 +357,41 @@ namespace psmas
     }
 
     template <int BD, typename T>
-    void launch_update_xr(int nnodes, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0, double *sc,
-                          ReduceSite site, hipStream_t s)
+    void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0, double *sc,
+                          const SysMap &sm, ReduceSite site, hipStream_t s)
     {
-        const int grid = div_up(nnodes, VEC_BLOCK);
-        hipLaunchKernelGGL((k_update_xr<BD, T>), dim3(grid), dim3(VEC_BLOCK), 0, s, nnodes, p, Ap, x, r, fr, rc0, sc,
+        hipLaunchKernelGGL((k_update_xr<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, Ap, x, r, fr, rc0, sc, sm,
                            site);
         PSMAS_LAUNCH_CHECK();
     }
 
-    template <typename T>
-    void launch_axpy_alpha(long long n, const T *p, T *x, const double *sc, hipStream_t s)
+    template <int BD, typename T>
+    void launch_axpy_alpha(int nblocks, const T *p, T *x, const double *sc, const SysMap &sm, hipStream_t s)
     {
-        hipLaunchKernelGGL((k_axpy_alpha<T>), dim3(stream_grid(n)), dim3(256), 0, s, n, p, x, sc);
+        hipLaunchKernelGGL((k_axpy_alpha<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, x, sc, sm);
         PSMAS_LAUNCH_CHECK();
     }
 
-    template <typename T>
-    void launch_update_p(long long n, const T *z, T *p, const double *sc, hipStream_t s)
+    template <int BD, typename T>
+    void launch_update_p(int nblocks, const T *z, T *p, const double *sc, const SysMap &sm, hipStream_t s)
     {
-        hipLaunchKernelGGL((k_update_p<T>), dim3(stream_grid(n)), dim3(256), 0, s, n, z, p, sc);
+        hipLaunchKernelGGL((k_update_p<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, z, p, sc, sm);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD>
+    void launch_accumulate_correction(int nblocks, float *x32, double *x64, const SysMap &sm, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_accumulate_correction<BD>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, x32, x64, sm);
         PSMAS_LAUNCH_CHECK();
     }
 
     template <typename TS, typename TD>
     void launch_copy_convert(long long n, const TS *src, TD *dst, hipStream_t s)
     {
+        if (n <= 0)
+            return;
         hipLaunchKernelGGL((k_copy_convert<TS, TD>), dim3(stream_grid(n)), dim3(256), 0, s, n, src, dst);
-        PSMAS_LAUNCH_CHECK();
-    }
-
-    void launch_accumulate_correction(long long n, float *x32, double *x64, hipStream_t s)
-    {
-        hipLaunchKernelGGL(k_accumulate_correction, dim3(stream_grid(n)), dim3(256), 0, s, n, x32, x64);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -388,9 +420,10 @@ namespace psmas
         PSMAS_LAUNCH_CHECK();
     }
 
-    void launch_set_scalar(double *sc, int slot, double value, hipStream_t s)
+    void launch_set_scalar(double *sc, int count, int slot, double value, hipStream_t s)
     {
-        hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, slot, value);
+        hipLaunchKernelGGL(k_set_scalar, dim3(div_up(count, 256)), dim3(256), 0, s, sc, static_cast<int>(S_COUNT),
+                           count, slot, value);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -406,35 +439,41 @@ namespace psmas
         PSMAS_LAUNCH_CHECK();
     }
 
-#define PSMAS_INST_BD(BD)                                                                                         \
-    template void launch_spmv_ap<BD, double, double>(const SellView<double> &, const double *, double *, double *, \
-                                                     ReduceSite, hipStream_t);                                   \
-    template void launch_spmv_ap<BD, float, float>(const SellView<float> &, const float *, float *, double *,     \
-                                                   ReduceSite, hipStream_t);                                     \
-    template void launch_spmv_res<BD, double, double, double>(const SellView<double> &, const double *,          \
-                                                              const double *, double *, FineRestrict, double *,  \
-                                                              double *, int, ReduceSite, hipStream_t);           \
-    template void launch_spmv_res<BD, double, double, float>(const SellView<double> &, const double *,           \
-                                                             const double *, float *, FineRestrict, float *,     \
-                                                             double *, int, ReduceSite, hipStream_t);            \
-    template void launch_spmv_plain<BD, double, double>(const SellView<double> &, const double *, double *,      \
-                                                        hipStream_t);                                            \
-    template void launch_spmv_plain<BD, float, float>(const SellView<float> &, const float *, float *,           \
-                                                      hipStream_t);                                              \
-    template void launch_update_xr<BD, double>(int, const double *, const double *, double *, double *,          \
-                                               FineRestrict, double *, double *, ReduceSite, hipStream_t);       \
+#define PSMAS_INST_BD(BD)                                                                                          \
+    template void launch_spmv_ap<BD, double, double>(const SellView<double> &, const double *, double *, double *,  \
+                                                     const SysMap &, ReduceSite, hipStream_t);                    \
+    template void launch_spmv_ap<BD, float, float>(const SellView<float> &, const float *, float *, double *,      \
+                                                   const SysMap &, ReduceSite, hipStream_t);                      \
+    template void launch_spmv_res<BD, double, double, double>(const SellView<double> &, const double *,           \
+                                                              const double *, double *, FineRestrict, double *,   \
+                                                              double *, int, const SysMap &, ReduceSite, bool,    \
+                                                              hipStream_t);                                       \
+    template void launch_spmv_res<BD, double, double, float>(const SellView<double> &, const double *,            \
+                                                             const double *, float *, FineRestrict, float *,      \
+                                                             double *, int, const SysMap &, ReduceSite, bool,     \
+                                                             hipStream_t);                                        \
+    template void launch_spmv_plain<BD, double, double>(const SellView<double> &, const double *, double *,       \
+                                                        hipStream_t);                                             \
+    template void launch_spmv_plain<BD, float, float>(const SellView<float> &, const float *, float *,            \
+                                                      hipStream_t);                                               \
+    template void launch_update_xr<BD, double>(int, const double *, const double *, double *, double *,           \
+                                               FineRestrict, double *, double *, const SysMap &, ReduceSite,      \
+                                               hipStream_t);                                                      \
     template void launch_update_xr<BD, float>(int, const float *, const float *, float *, float *, FineRestrict,  \
-                                              float *, double *, ReduceSite, hipStream_t);
+                                              float *, double *, const SysMap &, ReduceSite, hipStream_t);        \
+    template void launch_axpy_alpha<BD, double>(int, const double *, double *, const double *, const SysMap &,    \
+                                                hipStream_t);                                                     \
+    template void launch_update_p<BD, double>(int, const double *, double *, const double *, const SysMap &,      \
+                                              hipStream_t);                                                       \
+    template void launch_update_p<BD, float>(int, const float *, float *, const double *, const SysMap &,         \
+                                             hipStream_t);                                                        \
+    template void launch_accumulate_correction<BD>(int, float *, double *, const SysMap &, hipStream_t);
 
     PSMAS_INST_BD(1)
     PSMAS_INST_BD(2)
     PSMAS_INST_BD(3)
 #undef PSMAS_INST_BD
 
-    template void launch_axpy_alpha<double>(long long, const double *, double *, const double *, hipStream_t);
-    template void launch_axpy_alpha<float>(long long, const float *, float *, const double *, hipStream_t);
-    template void launch_update_p<double>(long long, const double *, double *, const double *, hipStream_t);
-    template void launch_update_p<float>(long long, const float *, float *, const double *, hipStream_t);
     template void launch_copy_convert<double, float>(long long, const double *, float *, hipStream_t);
     template void launch_copy_convert<float, double>(long long, const float *, double *, hipStream_t);
     template void launch_copy_convert<double, double>(long long, const double *, double *, hipStream_t);

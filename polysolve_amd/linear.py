@@ -172,6 +172,48 @@ class MASSolver(Solver):
         _check(N.lib().psmas_set_coordinates(self._h, int(xyz.shape[0]), int(xyz.shape[1]),
                                              _ptr(xyz, C.c_double)), self._h)
 
+    # -- batches of independent systems -------------------------------------------------------
+    def factorize_batch(self, As, coords=None) -> None:
+        """Factorise several systems together (one launch sequence per PCG phase for all)."""
+        L = N.lib()
+        if coords is not None:
+            for s, xyz in enumerate(coords):
+                xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+                _check(L.psmas_batch_set_coordinates(self._h, s, int(xyz.shape[0]), int(xyz.shape[1]),
+                                                     _ptr(xyz, C.c_double)), self._h)
+        parts = [_as_csc(A) for A in As]
+        k = len(parts)
+        ns = np.array([p[0] for p in parts], dtype=np.int32)
+        nnz = (C.c_longlong * k)(*[int(p[2].size) for p in parts])
+        cp = (C.POINTER(C.c_int) * k)(*[_ptr(p[1], C.c_int) for p in parts])
+        ri = (C.POINTER(C.c_int) * k)(*[_ptr(p[2], C.c_int) for p in parts])
+        va = (C.POINTER(C.c_double) * k)(*[_ptr(p[3], C.c_double) for p in parts])
+        self._batch_keep = parts
+        _check(L.psmas_batch_factorize_csc(self._h, k, _ptr(ns, C.c_int), nnz, cp, ri, va), self._h)
+        self._batch_n = [int(v) for v in ns]
+
+    def solve_batch(self, bs, xs) -> None:
+        """Solve every system of the batch; xs (contiguous float64 arrays) are written in place."""
+        k = len(bs)
+        bs = [np.ascontiguousarray(b, dtype=np.float64) for b in bs]
+        for x in xs:
+            if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+                raise MASError("[MAS] x must be a contiguous float64 numpy array")
+        bp = (C.POINTER(C.c_double) * k)(*[_ptr(b, C.c_double) for b in bs])
+        xp = (C.POINTER(C.c_double) * k)(*[_ptr(x, C.c_double) for x in xs])
+        _check(N.lib().psmas_batch_solve(self._h, k, bp, xp, 0), self._h)
+
+    def solve_batch_device(self, b_ptrs, x_ptrs) -> None:
+        k = len(b_ptrs)
+        bp = (C.POINTER(C.c_double) * k)(*[C.cast(C.c_void_p(p), C.POINTER(C.c_double)) for p in b_ptrs])
+        xp = (C.POINTER(C.c_double) * k)(*[C.cast(C.c_void_p(p), C.POINTER(C.c_double)) for p in x_ptrs])
+        _check(N.lib().psmas_batch_solve(self._h, k, bp, xp, 1), self._h)
+
+    def batch_info(self, s: int) -> dict:
+        rc, txt = N.read_string(N.lib().psmas_batch_get_info, self._h, int(s))
+        _check(rc, self._h)
+        return _json.loads(txt)
+
     def level_info(self) -> dict:
         info = np.zeros(16, dtype=np.int32)
         _check(N.lib().psmas_get_level_info(self._h, _ptr(info, C.c_int), 16), self._h)
@@ -236,3 +278,131 @@ def parse_parameters(params) -> dict:
 
 def device_count() -> int:
     return int(N.lib().psmas_device_count())
+
+
+def synchronize() -> None:
+    _check(N.lib().psmas_device_synchronize(), None)
+
+
+class DeviceArray:
+    """A float64 vector in device memory owned through the C ABI (psmas_device_alloc).
+
+    For callers without their own GPU allocator: ``solve_device`` takes ``.ptr``.
+    """
+
+    def __init__(self, n: int, host: np.ndarray | None = None) -> None:
+        self.n = int(n)
+        p = C.c_void_p()
+        _check(N.lib().psmas_device_alloc(max(1, self.n) * 8, C.byref(p)), None)
+        self.ptr = int(p.value)
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host) -> None:
+        h = np.ascontiguousarray(host, dtype=np.float64)
+        if h.size != self.n:
+            raise MASError("[MAS] invalid host array size")
+        _check(N.lib().psmas_memcpy(C.c_void_p(self.ptr), h.ctypes.data_as(C.c_void_p), self.n * 8, 1), None)
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float64)
+        _check(N.lib().psmas_memcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.n * 8, 2), None)
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            N.lib().psmas_device_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """GPU MLP d_in -> hidden -> hidden -> d_out (ReLU) of the learned coarse correction.
+
+    BF16-operand MFMA GEMMs with FP32 accumulation, FP32 master weights, Adam on the GPU.
+    """
+
+    def __init__(self, d_in: int, d_out: int, hidden: int = 128, seed: int = 1234) -> None:
+        h = C.c_void_p()
+        rc = N.lib().psmas_mlp_create(int(d_in), int(d_out), int(hidden), int(seed), C.byref(h))
+        if rc != N.PSMAS_OK:
+            raise MASError(N.lib().psmas_mlp_last_error(None).decode())
+        self._h = h
+        self.d_in, self.d_out, self.hidden = int(d_in), int(d_out), int(hidden)
+
+    def _check(self, rc: int) -> None:
+        if rc != N.PSMAS_OK:
+            msg = N.lib().psmas_mlp_last_error(self._h).decode()
+            raise (OutOfMemoryError if rc == N.PSMAS_ERROR_OUT_OF_MEMORY else MASError)(msg)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                N.lib().psmas_mlp_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def num_parameters(self) -> int:
+        return int(N.lib().psmas_mlp_num_parameters(self._h))
+
+    def set_parameters(self, theta) -> None:
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        if t.size != self.num_parameters:
+            raise MASError("[MAS] invalid MLP parameter vector size")
+        self._check(N.lib().psmas_mlp_set_parameters(self._h, _fptr(t)))
+
+    def get_parameters(self) -> np.ndarray:
+        t = np.empty(self.num_parameters, dtype=np.float32)
+        self._check(N.lib().psmas_mlp_get_parameters(self._h, _fptr(t)))
+        return t
+
+    def forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, self.d_in)
+        Y = np.empty((X.shape[0], self.d_out), dtype=np.float32)
+        self._check(N.lib().psmas_mlp_forward(self._h, X.shape[0], _fptr(X), _fptr(Y)))
+        return Y
+
+    def loss_and_grad(self, X, T) -> tuple[float, np.ndarray]:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, self.d_in)
+        T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1, self.d_out)
+        if X.shape[0] != T.shape[0]:
+            raise MASError("[MAS] invalid MLP batch")
+        loss = C.c_double()
+        g = np.empty(self.num_parameters, dtype=np.float32)
+        self._check(N.lib().psmas_mlp_loss_grad(self._h, X.shape[0], _fptr(X), _fptr(T), C.byref(loss), _fptr(g)))
+        return float(loss.value), g
+
+    def train(self, X, T, epochs: int = 10, batch: int = 256, lr: float = 1e-3, seed: int = 0) -> float:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, self.d_in)
+        T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1, self.d_out)
+        loss = C.c_double()
+        self._check(N.lib().psmas_mlp_train(self._h, X.shape[0], _fptr(X), _fptr(T), int(epochs), int(batch),
+                                            float(lr), int(seed), C.byref(loss)))
+        return float(loss.value)
+
+
+def gemm_bf16(A, B, trans_a: bool = False) -> np.ndarray:
+    """C = op(A) B on the GPU with BF16-rounded operands and FP32 accumulation (the MLP's GEMM)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    K, Nn = B.shape
+    M = A.shape[1] if trans_a else A.shape[0]
+    if (A.shape[0] if trans_a else A.shape[1]) != K:
+        raise MASError("[MAS] invalid GEMM shapes")
+    Cm = np.empty((M, Nn), dtype=np.float32)
+    rc = N.lib().psmas_gemm_bf16(M, Nn, K, int(trans_a), _fptr(A), _fptr(B), _fptr(Cm))
+    if rc != N.PSMAS_OK:
+        raise MASError(f"psmas_gemm_bf16 failed ({rc})")
+    return Cm

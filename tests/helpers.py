@@ -22,11 +22,13 @@ def morton_order(coords: np.ndarray) -> np.ndarray:
 
 
 def chunk_partition(order: np.ndarray, even: bool = True):
+    """Consecutive runs of 32 nodes; with ``even`` the part count is padded with empty parts to a
+    multiple of 8 (the solver's layout: whole 256-node blocks)."""
     nb = order.size
     P = max(1, (nb + 31) // 32)
     offsets = np.minimum(np.arange(P + 1) * 32, nb)
-    if even and P % 2:
-        offsets = np.append(offsets, nb)
+    if even and P % 8:
+        offsets = np.append(offsets, np.full(8 - P % 8, nb))
     perm = np.empty(nb, dtype=np.int32)
     perm[order] = np.arange(nb, dtype=np.int32)
     return offsets.astype(np.int32), perm
@@ -58,9 +60,11 @@ def grid_laplacian_pattern(nx: int, ny: int, nz: int = 1) -> sp.csc_matrix:
     return (L != 0).astype(np.float64).tocsc()
 
 
-def dense_mas(A: np.ndarray, bd: int, offsets, perm, cmap_perm, level_num):
+def dense_mas(A: np.ndarray, bd: int, offsets, perm, cmap_perm, level_num, coarsest_dense: bool = False):
     """Dense additive MAS operator sum_l R_l^T blockdiag(inv(R_l A R_l^T)|banks) R_l built from
-    scratch with numpy, given the partition and the coarse map (over permuted block ids)."""
+    scratch with numpy, given the partition and the coarse map (over permuted block ids).
+    coarsest_dense: on the last level, the coarse nodes that contain a node with an off-diagonal
+    block get the inverse of their full Galerkin matrix instead of the bank blocks."""
     n = A.shape[0]
     nb = (n + bd - 1) // bd
     Ap = np.zeros((nb * bd, nb * bd))
@@ -95,6 +99,14 @@ def dense_mas(A: np.ndarray, bd: int, offsets, perm, cmap_perm, level_num):
         for bank in range((nc + 31) // 32):
             ids = np.arange(bank * 32, min(nc, bank * 32 + 32))
             idx = (ids[:, None] * bd + np.arange(bd)[None, :]).ravel()
+            Ci[np.ix_(idx, idx)] = np.linalg.inv(Ac[np.ix_(idx, idx)])
+        if coarsest_dense and l == level_num - 1:
+            blocks = np.abs(Aq).reshape(nb, bd, nb, bd).sum(axis=(1, 3))
+            coupled = (blocks != 0).sum(axis=1) > 1  # has an off-diagonal block
+            active = np.unique(cm[coupled])
+            idx = (active[:, None] * bd + np.arange(bd)[None, :]).ravel()
+            Ci[idx, :] = 0.0
+            Ci[:, idx] = 0.0
             Ci[np.ix_(idx, idx)] = np.linalg.inv(Ac[np.ix_(idx, idx)])
         M += R.T @ Ci @ R
     Mo = P.T @ M @ P

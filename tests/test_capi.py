@@ -14,7 +14,7 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def declared_functions():
     text = (ROOT / "include" / "psmas.h").read_text()
-    return sorted(set(re.findall(r"\b(psmas_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(psmas_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -8,6 +8,7 @@ import scipy.sparse.linalg as spla
 
 import oracle
 from polysolve_amd import MASError, MASSolver, Solver, problems
+from polysolve_amd.linear import DeviceArray
 from tests import helpers
 
 pytestmark = pytest.mark.gpu
@@ -265,13 +266,57 @@ def test_inverse_failure_raises(gpu):
 
 
 def test_solve_device_pointers(gpu, cfg1):
-    torch = pytest.importorskip("torch")
     s = _factorized(cfg1, {"relative_tolerance": 1e-8, "absolute_tolerance": 0.0})
-    b = torch.tensor(cfg1.b, dtype=torch.float64, device="cuda")
-    x = torch.zeros_like(b)
-    torch.cuda.synchronize()
-    s.solve_device(b.data_ptr(), x.data_ptr(), cfg1.n)
-    xh = x.cpu().numpy()
+    b = DeviceArray(cfg1.n, cfg1.b)
+    x = DeviceArray(cfg1.n, np.zeros(cfg1.n))
+    s.solve_device(b.ptr, x.ptr, cfg1.n)
+    xh = x.download()
     x_host = np.zeros(cfg1.n)
     s.solve(cfg1.b, x_host)
     assert np.array_equal(xh, x_host)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_matches_individual_solves(gpu, precision):
+    """Uneven systems solved in one batch: each meets its tolerance and agrees with solving it
+    alone (same kernels; the batch's coarse levels span all systems, so agreement is to tolerance)."""
+    sizes = [3, 6, 4, 8]
+    probs = [problems.hex_elasticity(e, E=problems.random_moduli(e, 100 + i)) for i, e in enumerate(sizes)]
+    prm = {"MAS": {"block_dim": 3, "precision": precision, "relative_tolerance": 1e-9, "absolute_tolerance": 0.0,
+                   "max_iter": 5000}}
+    s = MASSolver()
+    s.set_parameters(prm)
+    s.factorize_batch([p.A for p in probs], coords=[p.coords for p in probs])
+    xs = [np.zeros(p.n) for p in probs]
+    s.solve_batch([p.b for p in probs], xs)
+    for i, p in enumerate(probs):
+        info = s.batch_info(i)
+        assert info["solver_status"] == "Reach relative tolerance", (i, info)
+        assert np.linalg.norm(p.b - p.A @ xs[i]) <= 1.5e-9 * np.linalg.norm(p.b)
+        solo = _factorized(p, prm["MAS"])
+        x1 = np.zeros(p.n)
+        solo.solve(p.b, x1)
+        assert np.linalg.norm(xs[i] - x1) <= 1e-6 * np.linalg.norm(x1)
+
+
+def test_reference_division_guard_reproduces_stall(gpu):
+    """division_guard = 1e-20 is the reference rule (MASSolver.cu:43-56): on a small-magnitude
+    system p^T A p falls below 1e-20 and the iteration freezes, in the oracle and on the GPU."""
+    p = problems.hex_elasticity(5, E=problems.random_moduli(5, 11))
+    s = _factorized(p, {"aggregation": "graph", "relative_tolerance": 1e-12, "absolute_tolerance": 0.0,
+                        "max_iter": 300, "division_guard": 1e-20})
+    x = np.zeros(p.n)
+    s.solve(p.b, x)
+    info = s.get_info()
+    offsets, perm = s.partition()
+    ref = oracle.MasOracle(p.A, 3, offsets, perm).pcg(p.b, max_iter=300, rel_tol=1e-12, abs_tol=0.0,
+                                                       division_guard=1e-20)
+    assert info["solver_status"] == "Reach max iterations" and ref["status"] == 3
+    rel_gpu = np.linalg.norm(p.b - p.A @ x) / np.linalg.norm(p.b)
+    rel_ref = np.linalg.norm(p.b - p.A @ ref["x"]) / np.linalg.norm(p.b)
+    assert 1e-9 < rel_gpu < 1e-6 and 1e-9 < rel_ref < 1e-6
+    assert abs(np.log10(rel_gpu) - np.log10(rel_ref)) < 0.5
+    # The default rule converges on the same system.
+    s.set_parameters({"MAS": {"division_guard": 0.0}})
+    s.solve(p.b, x)
+    assert s.get_info()["solver_status"] == "Reach relative tolerance"
