@@ -285,6 +285,9 @@ namespace psmas
 
     double Mlp::loss_and_grad_device(int B, const float *X, const float *T)
     {
+        // Allocate the batch buffers before taking y_'s address: forward_device's own
+        // ensure_batch runs after its arguments are evaluated.
+        ensure_batch(B);
         forward_device(B, X, y_.data(), stream_);
         const long long cnt = static_cast<long long>(B) * d_out_;
         // loss = sum (Y - T)^2 / (B * d_out); dL/dY = 2 (Y - T) / (B * d_out)
