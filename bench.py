@@ -142,15 +142,20 @@ def main() -> int:
     log(f"  mixed {mixed_ms:.2f} ms/solve, {infom['solver_iter']} it, {infom['reliable_updates']} updates, "
         f"true rel res {relm:.2e}")
 
-    # Accuracy reference: FP64 MAS-PCG driven to 1e-12 (no direct FP64 factorisation of an 824k
-    # DOF 3-D system is available on this machine).
-    solver.set_parameters({"MAS": {"precision": "fp64", "coarse_solver": "bank", "relative_tolerance": 1e-12}})
+    # Accuracy reference: the FP64 path driven to relative residual 1e-11 (a direct FP64
+    # factorisation of this 3-D system does not fit the host in the time of a bench run; on the
+    # small workloads the tests compare against a direct solve).  The reference's own residual
+    # is reported: the solution-error figures are only meaningful while it is well below 1e-8.
+    solver.set_parameters({"MAS": {"precision": "fp64", "coarse_solver": args.coarse, "relative_tolerance": 1e-11,
+                                   "max_iter": 6000}})
     solver.solve_device(b_dev.ptr, x_dev.ptr, n)
     synchronize()
+    inforef = solver.get_info()
     xref, relref = accuracy()
+    log(f"  reference: {inforef['solver_iter']} it, true rel res {relref:.2e}")
     err_m = float(np.linalg.norm(xm - xref) / np.linalg.norm(xref))
     err_64 = float(np.linalg.norm(x64 - xref) / np.linalg.norm(xref))
-    solver.set_parameters({"MAS": {"relative_tolerance": tol}})
+    solver.set_parameters({"MAS": {"relative_tolerance": tol, "max_iter": 20000}})
 
     # Roofline: dominant kernel of the mixed path, timed with HIP events on the solver stream.
     solver.set_parameters({"MAS": {"precision": "mixed", "coarse_solver": args.coarse}})
@@ -217,9 +222,9 @@ def main() -> int:
         "iters": infom["solver_iter"],
         "reliable_updates": infom["reliable_updates"],
         "true_rel_residual": relm,
-        "rel_err_vs_fp64_1e-12": err_m,
+        "rel_err_vs_fp64_ref": err_m, "fp64_ref_rel_residual": relref,
         "fp64": {"ms": round(fp64_ms, 3), "iters": info64["solver_iter"], "true_rel_residual": rel64,
-                 "rel_err_vs_fp64_1e-12": err_64},
+                 "rel_err_vs_fp64_ref": err_64},
         "speedup_vs_fp64": round(fp64_ms / mixed_ms, 3),
         "setup_ms": round(setup_ms, 1),
         "roofline": roofline,

@@ -1,6 +1,5 @@
-// Device-side building blocks shared by the MAS kernels: wave64 and workgroup reductions and a
-// deterministic grid reduction (per-workgroup partials summed in a fixed order by the last
-// workgroup to arrive), plus the solver's device-scalar slots.
+// Device-side building blocks shared by the MAS kernels: wave64 and workgroup reductions, the
+// per-system partial-sum protocol of the PCG reductions, and the solver's device-scalar slots.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -13,37 +12,6 @@
 namespace psmas
 {
     constexpr int WAVE = 64;
-
-    /// Device scalar slots (double) used by the PCG loops.  All reductions of one iteration
-    /// land here so that alpha / beta never travel to the host.
-    enum Scalar : int
-    {
-        S_RZ = 0,      ///< r . z (current)
-        S_RZ_OLD = 1,  ///< r . z of the previous iteration
-        S_PAP = 2,     ///< p . A p
-        S_ALPHA = 3,
-        S_BETA = 4,
-        S_RR = 5,      ///< r . r of the recurrence residual
-        S_RR_TRUE = 6, ///< r . r of an explicitly computed residual b - A x (FP64)
-        S_RZ0 = 7,
-        S_RR0 = 8,
-        S_RZ_PREV_Z = 9, ///< r_new . z_old (flexible PCG)
-        S_DELTA_R = 10,  ///< learned correction . restricted r
-        S_DELTA_R_OLD = 11,
-        S_GUARD = 12,    ///< division guard (see safe_div)
-        S_COUNT = 16,
-    };
-
-    /// Reduction sites: each owns a partials array and an arrival counter.
-    enum Site : int
-    {
-        R_PAP = 0,
-        R_RR = 1,
-        R_RZ = 2,
-        R_RR_TRUE = 3,
-        R_RZ_PREV = 4,
-        R_COUNT = 5,
-    };
 
     __device__ __forceinline__ double wave_sum(double v)
     {
@@ -79,64 +47,33 @@ namespace psmas
         return s;
     }
 
-    /// Grid-wide deterministic sum.  Every workgroup calls this once with its threads' values.
-    /// The last workgroup to arrive adds all partials in index order and calls epilogue(total)
-    /// from thread 0, then rearms the counter.  Returns true in the workgroup that finished.
-    template <int BLOCK, typename Epilogue>
-    __device__ __forceinline__ void grid_sum(double v, double *partials, unsigned *counter, Epilogue epilogue)
+    /// The workgroup's sum of v -> partials[blockIdx.x].
+    template <int BLOCK>
+    __device__ __forceinline__ void store_partial(double v, double *partials)
     {
-        __shared__ bool am_last;
         const double s = block_sum<BLOCK>(v);
         if (threadIdx.x == 0)
-        {
-            __hip_atomic_store(partials + blockIdx.x, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned prev =
-                __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            am_last = (prev == gridDim.x - 1);
-        }
-        __syncthreads();
-        if (!am_last)
-            return;
-        double t = 0.0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += BLOCK)
-            t += __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        t = block_sum<BLOCK>(t);
-        if (threadIdx.x == 0)
-        {
-            epilogue(t);
-            __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+            partials[blockIdx.x] = s;
     }
 
-    /// Per-system deterministic sum over the node blocks of system `sys` (the workgroup's own).
-    /// The system's last workgroup to arrive sums the partials of its blocks in index order and
-    /// calls epilogue(total) from thread 0.
-    template <int BLOCK, typename Epilogue>
-    __device__ __forceinline__ void system_sum(double v, int sys, const SysMap &sm, double *partials,
-                                               unsigned *counters, Epilogue epilogue)
+    /// Sum of the partials of system sys's node blocks, in block order, returned to every thread.
+    /// The partials come from an earlier kernel; every workgroup computes the same value bit
+    /// for bit.
+    template <int BLOCK>
+    __device__ __forceinline__ double system_total(const double *__restrict__ partials, int sys, const SysMap &sm)
     {
-        __shared__ bool am_last;
-        const double s = block_sum<BLOCK>(v);
-        if (threadIdx.x == 0)
-        {
-            __hip_atomic_store(partials + blockIdx.x, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned prev =
-                __hip_atomic_fetch_add(counters + sys, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            am_last = (prev == static_cast<unsigned>(sm.blk_count[sys]) - 1u);
-        }
-        __syncthreads();
-        if (!am_last)
-            return;
+        __shared__ double total;
         const int b0 = sm.blk_begin[sys], nb = sm.blk_count[sys];
         double t = 0.0;
         for (int i = threadIdx.x; i < nb; i += BLOCK)
-            t += __hip_atomic_load(partials + b0 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            t += partials[b0 + i];
         t = block_sum<BLOCK>(t);
         if (threadIdx.x == 0)
-        {
-            epilogue(t);
-            __hip_atomic_store(counters + sys, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+            total = t;
+        __syncthreads();
+        const double r = total;
+        __syncthreads();
+        return r;
     }
 
     /// alpha / beta division.  The reference returns 0 when |den| < 1e-20 (MASSolver.cu:43-56)

@@ -34,11 +34,43 @@ namespace psmas
         const int *gid = nullptr;
     };
 
-    /// Partials (one per workgroup) and arrival counters (one per system) of a reduction.
-    struct ReduceSite
+    /// Per-system device scalars (double): the division guard set by the host, and the totals
+    /// of the PCG reductions published by launch_finalize for the host's convergence checks.
+    enum Scalar : int
     {
-        double *partials = nullptr;
-        unsigned *counter = nullptr;
+        S_RZ = 0,      ///< r . z (current)
+        S_RZ_OLD = 1,  ///< r . z of the previous iteration
+        S_PAP = 2,     ///< p . A p
+        S_RR = 3,      ///< r . r of the recurrence residual
+        S_RR_TRUE = 4, ///< r . r of an explicitly computed residual b - A x (FP64)
+        S_GUARD = 5,   ///< division guard (see safe_div)
+        S_COUNT = 8,
+    };
+
+    /// Reduction sites (one partials array of node-block length each).
+    enum Site : int
+    {
+        R_PAP = 0,
+        R_RR = 1,
+        R_RZ = 2,
+        R_RR_TRUE = 3,
+        R_RZ_OLD = 4,
+        R_COUNT = 5,
+    };
+
+    /// The PCG reductions.  Producers write one partial sum per node block (workgroup); every
+    /// consumer re-sums the partials of its own system in one fixed order, so all workgroups see
+    /// bit-identical alpha / beta without atomics or cross-workgroup fences.  sc holds S_COUNT
+    /// doubles per system: the division guard (input) and the totals that launch_finalize
+    /// publishes for the host.
+    struct Reductions
+    {
+        double *pap = nullptr;     ///< p . A p
+        double *rr = nullptr;      ///< r . r of the recurrence residual
+        double *rz = nullptr;      ///< r . z (current)
+        double *rz_old = nullptr;  ///< r . z of the previous iteration
+        double *rr_true = nullptr; ///< r . r of an explicitly computed residual
+        double *sc = nullptr;
     };
 
     /// Systems of a batch (a single solve is a batch of one): node block -> system, each
@@ -54,34 +86,37 @@ namespace psmas
     };
 
     // ----- SpMV ---------------------------------------------------------------------------
-    /// y = A x (+ per-system pAp and alpha).  TV matrix value type, TX vector type.
+    /// y = A x and the partials of x . y -> red.pap.  TV matrix value type, TX vector type.
     template <int BD, typename TV, typename TX>
-    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, double *sc, const SysMap &sm, ReduceSite site,
+    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, const Reductions &red, const SysMap &sm,
                         hipStream_t s);
 
-    /// y = b - A x, rr = y.y -> sc[sys][rr_slot]; restrict y -> yc0.  TX: x,b; TY: y.
+    /// y = b - A x, partials of y . y -> rr_partials; restrict y -> yc0.  TX: x,b; TY: y.
     /// selected_only: only systems with select == 1.
     template <int BD, typename TV, typename TX, typename TY>
     void launch_spmv_res(const SellView<TV> &A, const TX *x, const TX *b, TY *y, FineRestrict fr, TY *yc0,
-                         double *sc, int rr_slot, const SysMap &sm, ReduceSite site, bool selected_only, hipStream_t s);
+                         double *rr_partials, const SysMap &sm, bool selected_only, hipStream_t s);
 
     /// Plain y = A x without reductions (introspection path).
     template <int BD, typename TV, typename TX>
     void launch_spmv_plain(const SellView<TV> &A, const TX *x, TX *y, hipStream_t s);
 
     // ----- vectors (node-blocked: nblocks workgroups of NODE_BLOCK nodes) ------------------
-    /// x += alpha p; r -= alpha Ap; rr = r.r; restrict r -> rc0.
+    /// alpha = rz / pap; x += alpha p; r -= alpha Ap; partials of r . r -> red.rr; restrict r -> rc0.
     template <int BD, typename T>
-    void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0, double *sc,
-                          const SysMap &sm, ReduceSite site, hipStream_t s);
+    void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0,
+                          const Reductions &red, const SysMap &sm, hipStream_t s);
 
-    /// x += alpha p.
+    /// x += alpha p (alpha = rz / pap).
     template <int BD, typename T>
-    void launch_axpy_alpha(int nblocks, const T *p, T *x, const double *sc, const SysMap &sm, hipStream_t s);
+    void launch_axpy_alpha(int nblocks, const T *p, T *x, const Reductions &red, const SysMap &sm, hipStream_t s);
 
-    /// p = z + beta p.
+    /// p = z + beta p (beta = rz / rz_old).
     template <int BD, typename T>
-    void launch_update_p(int nblocks, const T *z, T *p, const double *sc, const SysMap &sm, hipStream_t s);
+    void launch_update_p(int nblocks, const T *z, T *p, const Reductions &red, const SysMap &sm, hipStream_t s);
+
+    /// Per-system totals of every reduction -> sc (S_PAP, S_RR, S_RZ, S_RZ_OLD, S_RR_TRUE) for the host.
+    void launch_finalize(int nsys, const Reductions &red, const SysMap &sm, hipStream_t s);
 
     /// x64 += x32; x32 = 0 for systems selected for a reliable update.
     template <int BD>
@@ -105,9 +140,6 @@ namespace psmas
 
     /// sc[s][slot] = value for systems s < count.
     void launch_set_scalar(double *sc, int count, int slot, double value, hipStream_t s);
-    /// Dot product into a scalar slot (deterministic, single system).
-    template <typename T>
-    void launch_dot(long long n, const T *a, const T *b, double *sc, int slot, ReduceSite site, hipStream_t s);
 
     // ----- MAS apply ----------------------------------------------------------------------
     struct CoarseRestrict
@@ -119,7 +151,7 @@ namespace psmas
         const int *gid[4] = {nullptr, nullptr, nullptr, nullptr};
     };
 
-    /// Restrict coarse level 0 to levels 1..L-1 (one workgroup).
+    /// Restrict coarse level 0 to levels 1..L-1 (one launch per level, one thread per node).
     template <int BD, typename T>
     void launch_restrict_coarse(const CoarseRestrict &cr, T *mr, hipStream_t s);
 
@@ -130,10 +162,11 @@ namespace psmas
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
                           const int *bank_sys, const int *active, hipStream_t s);
 
-    /// z_p = mz_p + sum_l mz_l[cmap_p.l]; per-system rz = r.z with beta epilogue (or init).
+    /// z_p = mz_p + sum_l mz_l[cmap_p.l]; partials of r . z -> red.rz (the previous ones move to
+    /// red.rz_old unless init).
     template <int BD, typename T>
     void launch_prolong_dot(int nblocks, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
-                            double *sc, bool init, const SysMap &sm, ReduceSite site, hipStream_t s);
+                            const Reductions &red, bool init, const SysMap &sm, hipStream_t s);
 
     // ----- MAS build ----------------------------------------------------------------------
     /// Bank-local connected components at coarse level 0 (one lane per slot, two banks per wave).

@@ -30,39 +30,36 @@ namespace psmas
     namespace
     {
         // ------------------------------------------------------------------------ restriction
+        /// Level l from level l-1: one thread per level-(l-1) node; a leader sums the members of its
+        /// component (bit b of its mask = node base+b of the same 32-id group) in bit order.
         template <int BD, typename T>
-        __global__ __launch_bounds__(1024) void k_restrict_coarse(CoarseRestrict cr, T *__restrict__ mr)
+        __global__ __launch_bounds__(256) void k_restrict_level(int count, const uint32_t *__restrict__ lead,
+                                                                const int *__restrict__ gid,
+                                                                const T *__restrict__ src, T *__restrict__ dst)
         {
-            for (int l = 1; l < cr.level_num; ++l)
+            const int q = blockIdx.x * 256 + threadIdx.x;
+            if (q >= count)
+                return;
+            uint32_t mask = lead[q];
+            if (mask == 0u)
+                return;
+            const int g = gid[q];
+            const int base = q & ~31;
+            double s[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                s[i] = 0.0;
+            while (mask)
             {
-                const T *src = mr + static_cast<size_t>(cr.offsets[l]) * BD;
-                T *dst = mr + static_cast<size_t>(cr.offsets[l + 1]) * BD;
-                const int count = cr.counts[l - 1];
-                for (int q = threadIdx.x; q < count; q += blockDim.x)
-                {
-                    uint32_t mask = cr.lead[l][q];
-                    if (mask == 0u)
-                        continue;
-                    const int base = q & ~31;
-                    double s[BD];
+                const int b = __builtin_ctz(mask);
+                mask &= mask - 1u;
 #pragma unroll
-                    for (int i = 0; i < BD; ++i)
-                        s[i] = 0.0;
-                    while (mask)
-                    {
-                        const int b = __builtin_ctz(mask);
-                        mask &= mask - 1u;
-#pragma unroll
-                        for (int i = 0; i < BD; ++i)
-                            s[i] += static_cast<double>(src[static_cast<size_t>(base + b) * BD + i]);
-                    }
-                    const int g = cr.gid[l][q];
-#pragma unroll
-                    for (int i = 0; i < BD; ++i)
-                        dst[static_cast<size_t>(g) * BD + i] = static_cast<T>(s[i]);
-                }
-                __syncthreads();
+                for (int i = 0; i < BD; ++i)
+                    s[i] += static_cast<double>(src[static_cast<size_t>(base + b) * BD + i]);
             }
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                dst[static_cast<size_t>(g) * BD + i] = static_cast<T>(s[i]);
         }
 
         // ------------------------------------------------------------------------------ apply
@@ -144,12 +141,13 @@ namespace psmas
         __global__ __launch_bounds__(NODE_BLOCK) void k_prolong_dot(int level_num, const int4 *__restrict__ cmap,
                                                                     const int *__restrict__ lev_off,
                                                                     const T *__restrict__ r, T *__restrict__ mz,
-                                                                    double *__restrict__ sc, bool init, SysMap sm,
-                                                                    ReduceSite site)
+                                                                    Reductions red, bool init, SysMap sm)
         {
             const int sys = sm.block_sys[blockIdx.x];
             if (sm.active[sys] == 0)
                 return;
+            if (!init && threadIdx.x == 0)
+                red.rz_old[blockIdx.x] = red.rz[blockIdx.x];
             const int p = blockIdx.x * NODE_BLOCK + threadIdx.x;
             const int4 c = cmap[p];
             const int cc[4] = {c.x, c.y, c.z, c.w};
@@ -175,23 +173,7 @@ namespace psmas
                 mz[static_cast<size_t>(p) * BD + i] = zi;
                 dot += static_cast<double>(r[static_cast<size_t>(p) * BD + i]) * static_cast<double>(zi);
             }
-            double *scs = sc + static_cast<size_t>(sys) * S_COUNT;
-            system_sum<NODE_BLOCK>(dot, sys, sm, site.partials, site.counter, [scs, init](double t) {
-                if (init)
-                {
-                    scs[S_RZ] = t;
-                    scs[S_RZ0] = t;
-                    scs[S_RZ_OLD] = t;
-                    scs[S_BETA] = 0.0;
-                }
-                else
-                {
-                    const double old = scs[S_RZ];
-                    scs[S_RZ_OLD] = old;
-                    scs[S_RZ] = t;
-                    scs[S_BETA] = safe_div(t, old, scs[S_GUARD]);
-                }
-            });
+            store_partial<NODE_BLOCK>(dot, red.rz);
         }
 
         // ----------------------------------------------------------------------- build: levels
@@ -595,8 +577,16 @@ namespace psmas
     {
         if (cr.level_num <= 1)
             return;
-        hipLaunchKernelGGL((k_restrict_coarse<BD, T>), dim3(1), dim3(1024), 0, s, cr, mr);
-        PSMAS_LAUNCH_CHECK();
+        for (int l = 1; l < cr.level_num; ++l)
+        {
+            const int count = cr.counts[l - 1];
+            if (count <= 0)
+                continue;
+            hipLaunchKernelGGL((k_restrict_level<BD, T>), dim3(div_up(count, 256)), dim3(256), 0, s, count, cr.lead[l],
+                               cr.gid[l], mr + static_cast<size_t>(cr.offsets[l]) * BD,
+                               mr + static_cast<size_t>(cr.offsets[l + 1]) * BD);
+            PSMAS_LAUNCH_CHECK();
+        }
     }
 
     template <int BD, typename TM, typename T>
@@ -612,10 +602,10 @@ namespace psmas
 
     template <int BD, typename T>
     void launch_prolong_dot(int nblocks, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
-                            double *sc, bool init, const SysMap &sm, ReduceSite site, hipStream_t s)
+                            const Reductions &red, bool init, const SysMap &sm, hipStream_t s)
     {
         hipLaunchKernelGGL((k_prolong_dot<BD, T>), dim3(nblocks), dim3(NODE_BLOCK), 0, s, level_num, cmap, lev_off, r,
-                           mz, sc, init, sm, site);
+                           mz, red, init, sm);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -731,9 +721,9 @@ namespace psmas
     template void launch_mas_apply<BD, __half, float>(int, int, const __half *, const float *, const float *,     \
                                                       float *, const int *, const int *, hipStream_t);           \
     template void launch_prolong_dot<BD, double>(int, int, const int4 *, const int *, const double *, double *,   \
-                                                 double *, bool, const SysMap &, ReduceSite, hipStream_t);       \
+                                                 const Reductions &, bool, const SysMap &, hipStream_t);         \
     template void launch_prolong_dot<BD, float>(int, int, const int4 *, const int *, const float *, float *,      \
-                                                double *, bool, const SysMap &, ReduceSite, hipStream_t);
+                                                const Reductions &, bool, const SysMap &, hipStream_t);
 
     PSMAS_INST_APPLY(1)
     PSMAS_INST_APPLY(2)

@@ -171,12 +171,22 @@ namespace psmas
         S.have_partition = false;
     }
 
-    ReduceSite Solver::site(int s) const
+    Reductions Solver::reductions() const
     {
-        ReduceSite r;
-        r.partials = d_partials_.data() + static_cast<size_t>(s) * partial_cap_;
-        r.counter = d_counters_.data() + static_cast<size_t>(s) * sys_.size();
+        Reductions r;
+        auto site = [&](int i) { return d_partials_.data() + static_cast<size_t>(i) * partial_cap_; };
+        r.pap = site(R_PAP);
+        r.rr = site(R_RR);
+        r.rz = site(R_RZ);
+        r.rz_old = site(R_RZ_OLD);
+        r.rr_true = site(R_RR_TRUE);
+        r.sc = d_sc_.data();
         return r;
+    }
+
+    void Solver::finalize_scalars()
+    {
+        launch_finalize(static_cast<int>(sys_.size()), reductions(), sysmap(), stream_);
     }
 
     SysMap Solver::sysmap() const
@@ -605,10 +615,9 @@ namespace psmas
         const int K = static_cast<int>(sys_.size());
         d_sc_.allocate(static_cast<size_t>(S_COUNT) * K, "MAS device scalars");
         d_sc_.zero(stream_);
-        partial_cap_ = std::max(nblocks_, 2048);
+        partial_cap_ = nblocks_;
         d_partials_.allocate(static_cast<size_t>(R_COUNT) * partial_cap_, "MAS reduction partials");
-        d_counters_.allocate(static_cast<size_t>(R_COUNT) * K, "MAS reduction counters");
-        d_counters_.zero(stream_);
+        d_partials_.zero(stream_);
     }
 
     void Solver::ensure_mixed_data()
@@ -842,7 +851,7 @@ namespace psmas
         toc();
         apply_dense_coarse<BD, double>(d_mr_.data(), d_mz_.data());
         launch_prolong_dot<BD, double>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr_.data(),
-                                       d_mz_.data(), d_sc_.data(), init, sysmap(), site(R_RZ), stream_);
+                                       d_mz_.data(), reductions(), init, sysmap(), stream_);
     }
 
     template <int BD>
@@ -852,25 +861,25 @@ namespace psmas
         const double spmv_bytes =
             static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_total_ + 1) * 4.0 + 2.0 * n_total_ * 8.0;
         tic("spmv_fp64", spmv_bytes);
-        launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), d_sc_.data(), sysmap(), site(R_PAP), stream_);
+        launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), reductions(), sysmap(), stream_);
         toc();
         if (k % params_.true_residual_period == 0)
         {
-            launch_axpy_alpha<BD, double>(nblocks_, d_p_.data(), d_x_.data(), d_sc_.data(), sysmap(), stream_);
+            launch_axpy_alpha<BD, double>(nblocks_, d_p_.data(), d_x_.data(), reductions(), sysmap(), stream_);
             tic("spmv_res_fp64", spmv_bytes + n_total_ * 8.0);
             launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
-                                                        d_mr_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
-                                                        S_RR, sysmap(), site(R_RR), false, stream_);
+                                                        d_mr_.data() + static_cast<size_t>(npad_) * BD,
+                                                        reductions().rr, sysmap(), false, stream_);
             toc();
         }
         else
         {
             launch_update_xr<BD, double>(nblocks_, d_p_.data(), d_Ap_.data(), d_x_.data(), d_mr_.data(),
                                          fine_restrict(), d_mr_.data() + static_cast<size_t>(npad_) * BD,
-                                         d_sc_.data(), sysmap(), site(R_RR), stream_);
+                                         reductions(), sysmap(), stream_);
         }
         fp64_precondition<BD>(false);
-        launch_update_p<BD, double>(nblocks_, d_mz_.data(), d_p_.data(), d_sc_.data(), sysmap(), stream_);
+        launch_update_p<BD, double>(nblocks_, d_mz_.data(), d_p_.data(), reductions(), sysmap(), stream_);
     }
 
     void Solver::solve_fp64()
@@ -883,15 +892,16 @@ namespace psmas
             launch_fill<double>(nvec, d_x_.data(), 0.0, stream_);
             // r = b - A x (x = 0), rr, restriction.
             launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
-                                                        d_mr_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
-                                                        S_RR, sysmap(), site(R_RR), false, stream_);
+                                                        d_mr_.data() + static_cast<size_t>(npad_) * BD,
+                                                        reductions().rr, sysmap(), false, stream_);
             fp64_precondition<BD>(true);
             launch_copy_convert<double, double>(nvec, d_mz_.data(), d_p_.data(), stream_);
+            finalize_scalars();
             read_scalars();
             std::vector<double> rz0(K), rr0(K), bb(K);
             for (int s = 0; s < K; ++s)
             {
-                rz0[s] = *scal(s, S_RZ0);
+                rz0[s] = *scal(s, S_RZ);
                 if (std::isnan(rz0[s]) || !std::isfinite(rz0[s]))
                     throw std::runtime_error(K == 1 ? std::string("[MAS] Invalid initial residual.")
                                                     : "[MAS] Invalid initial residual (system " +
@@ -914,6 +924,7 @@ namespace psmas
                 run_block(graph_key(0, k % trp, count), [&]() {
                     for (int i = k0; i <= last; ++i)
                         fp64_iteration<BD>(i);
+                    finalize_scalars();
                 });
                 for (int s = 0; s < K; ++s)
                     if (h_active_[s])
@@ -962,12 +973,11 @@ namespace psmas
     {
         const SellView<float> A = sell(d_vals32_.data());
         tic("spmv_fp32", static_cast<double>(nnzb_) * (BD * BD * 4 + 4) + (nb_total_ + 1) * 4.0 + 2.0 * n_total_ * 4.0);
-        launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), d_sc_.data(), sysmap(), site(R_PAP),
-                                         stream_);
+        launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), reductions(), sysmap(), stream_);
         toc();
         launch_update_xr<BD, float>(nblocks_, d_p32_.data(), d_Ap32_.data(), d_x32_.data(), d_mr32_.data(),
-                                    fine_restrict(), d_mr32_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
-                                    sysmap(), site(R_RR), stream_);
+                                    fine_restrict(), d_mr32_.data() + static_cast<size_t>(npad_) * BD, reductions(),
+                                    sysmap(), stream_);
     }
 
     template <int BD>
@@ -985,11 +995,11 @@ namespace psmas
                                             d_mr32_.data(), d_mz32_.data(), nullptr, nullptr, stream_);
         apply_dense_coarse<BD, float>(d_mr32_.data(), d_mz32_.data());
         launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
-                                      d_mz32_.data(), d_sc_.data(), init, sysmap(), site(R_RZ), stream_);
+                                      d_mz32_.data(), reductions(), init, sysmap(), stream_);
         if (init)
             launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_p32_.data(), stream_);
         else
-            launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), d_sc_.data(), sysmap(), stream_);
+            launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), reductions(), sysmap(), stream_);
     }
 
     template <int BD>
@@ -999,8 +1009,8 @@ namespace psmas
         tic("spmv_res_fp64",
             static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_total_ + 1) * 4.0 + 3.0 * n_total_ * 8.0);
         launch_spmv_res<BD, double, double, float>(A, d_x_.data(), d_b_.data(), d_mr32_.data(), fine_restrict(),
-                                                   d_mr32_.data() + static_cast<size_t>(npad_) * BD, d_sc_.data(),
-                                                   S_RR_TRUE, sysmap(), site(R_RR_TRUE), true, stream_);
+                                                   d_mr32_.data() + static_cast<size_t>(npad_) * BD,
+                                                   reductions().rr_true, sysmap(), true, stream_);
         toc();
     }
 
@@ -1014,6 +1024,7 @@ namespace psmas
             launch_fill<float>(nvec, d_x32_.data(), 0.0f, stream_);
             mixed_reliable_residual<BD>(); // every system selected: r = b
             mixed_part_b<BD>(true);
+            finalize_scalars();
             read_scalars();
             std::vector<double> bnorm(K), target(K), ref(K), best(K);
             std::vector<int> stalls(K, 0);
@@ -1021,7 +1032,7 @@ namespace psmas
             for (int s = 0; s < K; ++s)
             {
                 System &S = sys_[s];
-                const double rz0 = *scal(s, S_RZ0);
+                const double rz0 = *scal(s, S_RZ);
                 if (std::isnan(rz0) || !std::isfinite(rz0))
                     throw std::runtime_error(K == 1 ? std::string("[MAS] Invalid initial residual.")
                                                     : "[MAS] Invalid initial residual (system " +
@@ -1056,6 +1067,7 @@ namespace psmas
                         if (i + 1 < count)
                             mixed_part_b<BD>(false);
                     }
+                    finalize_scalars();
                 });
                 first = false;
                 k += count;
@@ -1081,6 +1093,7 @@ namespace psmas
                 upload_flags();
                 launch_accumulate_correction<BD>(nblocks_, d_x32_.data(), d_x_.data(), sysmap(), stream_);
                 mixed_reliable_residual<BD>();
+                finalize_scalars();
                 read_scalars();
                 for (int s = 0; s < K; ++s)
                 {
@@ -1204,8 +1217,7 @@ namespace psmas
                 launch_spmv_res<BD, double, double, double>(A, d_Ap_.data(), d_p_.data(), d_mr_.data(),
                                                             fine_restrict(),
                                                             d_mr_.data() + static_cast<size_t>(npad_) * BD,
-                                                            d_sc_.data(), S_RR_TRUE, sysmap(), site(R_RR_TRUE),
-                                                            false, stream_);
+                                                            reductions().rr_true, sysmap(), false, stream_);
                 fp64_precondition<BD>(true);
                 launch_scatter_out<double>(S.nb, BD, S.n, d_slot_of_block_.data(), d_mz_.data(), d_io_.data(),
                                            stream_);
@@ -1215,8 +1227,7 @@ namespace psmas
                 launch_spmv_res<BD, double, double, float>(A, d_Ap_.data(), d_p_.data(), d_mr32_.data(),
                                                            fine_restrict(),
                                                            d_mr32_.data() + static_cast<size_t>(npad_) * BD,
-                                                           d_sc_.data(), S_RR_TRUE, sysmap(), site(R_RR_TRUE),
-                                                           false, stream_);
+                                                           reductions().rr_true, sysmap(), false, stream_);
                 mixed_part_b<BD>(true);
                 launch_copy_convert<float, double>(nvec, d_mz32_.data(), d_Ap_.data(), stream_);
                 launch_scatter_out<double>(S.nb, BD, S.n, d_slot_of_block_.data(), d_Ap_.data(), d_io_.data(),

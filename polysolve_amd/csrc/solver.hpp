@@ -156,7 +156,6 @@ namespace psmas
         DeviceBuffer<float> d_x32_, d_mr32_, d_mz32_, d_p32_, d_Ap32_;
         DeviceBuffer<double> d_sc_;
         DeviceBuffer<double> d_partials_;
-        DeviceBuffer<unsigned> d_counters_;
         int partial_cap_ = 0;
         std::vector<double> h_sc_;
 
@@ -168,7 +167,8 @@ namespace psmas
         std::vector<hipEvent_t> event_pool_;
         size_t event_pool_used_ = 0;
 
-        ReduceSite site(int s) const;
+        Reductions reductions() const;
+        void finalize_scalars();
         SysMap sysmap() const;
         FineRestrict fine_restrict() const;
         CoarseRestrict coarse_restrict() const;
@@ -189,7 +189,7 @@ namespace psmas
         void solve_fp64();
         void solve_mixed();
         void upload_flags();
-        double *scal(int s, int slot) { return &h_sc_[static_cast<size_t>(s) * 16 + slot]; }
+        double *scal(int s, int slot) { return &h_sc_[static_cast<size_t>(s) * S_COUNT + slot]; }
 
         // per-iteration pieces
         template <int BD>

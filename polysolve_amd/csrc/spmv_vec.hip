@@ -7,9 +7,9 @@
 //  * one lane per block row; a wave owns a slice of 64 rows = two 32-node banks, so every
 //    matrix load of a wave is one contiguous 512 B (FP64) / 256 B (FP32) run;
 //  * a workgroup owns 256 consecutive rows (a node block); a batch of systems is laid out so
-//    that each system covers whole node blocks, and every reduction is a deterministic
-//    per-system sum (per-workgroup partials, summed in index order by the system's last
-//    workgroup), whose scalar epilogue (alpha, beta) runs on the device;
+//    that each system covers whole node blocks.  Every reduction writes one partial per
+//    workgroup; the kernels that need alpha or beta re-sum their system's partials in block
+//    order at their start, so the scalars never leave the device and need no atomics;
 //  * the residual-producing kernels also restrict the residual to coarse level 0 through the
 //    bank-local component masks, so the preconditioner never re-reads r for that step;
 //  * systems that have converged (active == 0) skip their workgroups.
@@ -59,8 +59,7 @@ namespace psmas
 
         template <int BD, typename TV, typename TX>
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap(SellView<TV> A, const TX *__restrict__ x,
-                                                                TX *__restrict__ y, double *__restrict__ sc, SysMap sm,
-                                                                ReduceSite site)
+                                                                TX *__restrict__ y, Reductions red, SysMap sm)
         {
             using TA = typename Acc<TV>::type;
             const int sys = block_system(sm);
@@ -82,11 +81,7 @@ namespace psmas
                 y[row * BD + i] = yi;
                 dot += static_cast<double>(x[row * BD + i]) * static_cast<double>(yi);
             }
-            double *scs = sc + static_cast<size_t>(sys) * S_COUNT;
-            system_sum<SPMV_BLOCK>(dot, sys, sm, site.partials, site.counter, [scs](double t) {
-                scs[S_PAP] = t;
-                scs[S_ALPHA] = safe_div(scs[S_RZ], t, scs[S_GUARD]);
-            });
+            store_partial<SPMV_BLOCK>(dot, red.pap);
         }
 
         /// Leader lanes of a bank sum their component's members from LDS into the coarse vector.
@@ -117,8 +112,8 @@ namespace psmas
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_res(SellView<TV> A, const TX *__restrict__ x,
                                                                  const TX *__restrict__ b, TY *__restrict__ y,
                                                                  FineRestrict fr, TY *__restrict__ yc0,
-                                                                 double *__restrict__ sc, int rr_slot, SysMap sm,
-                                                                 ReduceSite site, int selected_only)
+                                                                 double *__restrict__ rr_partials, SysMap sm,
+                                                                 int selected_only)
         {
             using TA = typename Acc<TV>::type;
             __shared__ TY sv[SPMV_BLOCK * BD];
@@ -147,9 +142,7 @@ namespace psmas
             const int gid = fr.gid[row];
             __syncthreads();
             restrict_bank<BD, TY>(sv, (threadIdx.x / 32) * 32, mask, gid, yc0);
-            double *scs = sc + static_cast<size_t>(sys) * S_COUNT;
-            system_sum<SPMV_BLOCK>(dot, sys, sm, site.partials, site.counter,
-                                   [scs, rr_slot](double t) { scs[rr_slot] = t; });
+            store_partial<SPMV_BLOCK>(dot, rr_partials);
         }
 
         template <int BD, typename TV, typename TX>
@@ -172,20 +165,26 @@ namespace psmas
                 y[row * BD + i] = static_cast<TX>(acc[i]);
         }
 
+        /// alpha = rz / pap of the block's system (reference scalar_division, MASSolver.cu:46-56).
+        __device__ __forceinline__ double system_alpha(const Reductions &red, int sys, const SysMap &sm)
+        {
+            const double rz = system_total<VEC_BLOCK>(red.rz, sys, sm);
+            const double pap = system_total<VEC_BLOCK>(red.pap, sys, sm);
+            return safe_div(rz, pap, red.sc[static_cast<size_t>(sys) * S_COUNT + S_GUARD]);
+        }
+
         /// x += alpha p; r -= alpha Ap on the block's 256*BD contiguous scalars (coalesced), then
         /// r . r and the restriction of r to coarse level 0 from LDS.
         template <int BD, typename T>
         __global__ __launch_bounds__(VEC_BLOCK) void k_update_xr(const T *__restrict__ p, const T *__restrict__ Ap,
                                                                  T *__restrict__ x, T *__restrict__ r, FineRestrict fr,
-                                                                 T *__restrict__ rc0, double *__restrict__ sc,
-                                                                 SysMap sm, ReduceSite site)
+                                                                 T *__restrict__ rc0, Reductions red, SysMap sm)
         {
             __shared__ T sv[VEC_BLOCK * BD];
             const int sys = block_system(sm);
             if (sm.active[sys] == 0)
                 return;
-            double *scs = sc + static_cast<size_t>(sys) * S_COUNT;
-            const T alpha = static_cast<T>(scs[S_ALPHA]);
+            const T alpha = static_cast<T>(system_alpha(red, sys, sm));
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
             double dot = 0.0;
 #pragma unroll
@@ -204,17 +203,17 @@ namespace psmas
             const int gid = fr.gid[node];
             __syncthreads();
             restrict_bank<BD, T>(sv, (threadIdx.x / 32) * 32, mask, gid, rc0);
-            system_sum<VEC_BLOCK>(dot, sys, sm, site.partials, site.counter, [scs](double t) { scs[S_RR] = t; });
+            store_partial<VEC_BLOCK>(dot, red.rr);
         }
 
         template <int BD, typename T>
         __global__ __launch_bounds__(VEC_BLOCK) void k_axpy_alpha(const T *__restrict__ p, T *__restrict__ x,
-                                                                  const double *__restrict__ sc, SysMap sm)
+                                                                  Reductions red, SysMap sm)
         {
             const int sys = block_system(sm);
             if (sm.active[sys] == 0)
                 return;
-            const T alpha = static_cast<T>(sc[static_cast<size_t>(sys) * S_COUNT + S_ALPHA]);
+            const T alpha = static_cast<T>(system_alpha(red, sys, sm));
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
 #pragma unroll
             for (int k = 0; k < BD; ++k)
@@ -226,12 +225,14 @@ namespace psmas
 
         template <int BD, typename T>
         __global__ __launch_bounds__(VEC_BLOCK) void k_update_p(const T *__restrict__ z, T *__restrict__ p,
-                                                                const double *__restrict__ sc, SysMap sm)
+                                                                Reductions red, SysMap sm)
         {
             const int sys = block_system(sm);
             if (sm.active[sys] == 0)
                 return;
-            const T beta = static_cast<T>(sc[static_cast<size_t>(sys) * S_COUNT + S_BETA]);
+            const double rz = system_total<VEC_BLOCK>(red.rz, sys, sm);
+            const double rz_old = system_total<VEC_BLOCK>(red.rz_old, sys, sm);
+            const T beta = static_cast<T>(safe_div(rz, rz_old, red.sc[static_cast<size_t>(sys) * S_COUNT + S_GUARD]));
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
 #pragma unroll
             for (int k = 0; k < BD; ++k)
@@ -308,16 +309,24 @@ namespace psmas
                 sc[static_cast<size_t>(s) * stride + slot] = value;
         }
 
-        template <typename T>
-        __global__ __launch_bounds__(VEC_BLOCK) void k_dot(long long n, const T *__restrict__ a,
-                                                           const T *__restrict__ b, double *sc, int slot,
-                                                           ReduceSite site)
+        /// One workgroup per system: totals of every reduction site -> the system's scalars.
+        __global__ __launch_bounds__(VEC_BLOCK) void k_finalize(Reductions red, SysMap sm)
         {
-            double s = 0.0;
-            for (long long i = blockIdx.x * static_cast<long long>(VEC_BLOCK) + threadIdx.x; i < n;
-                 i += static_cast<long long>(gridDim.x) * VEC_BLOCK)
-                s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
-            grid_sum<VEC_BLOCK>(s, site.partials, site.counter, [sc, slot](double t) { sc[slot] = t; });
+            const int sys = blockIdx.x;
+            const double pap = system_total<VEC_BLOCK>(red.pap, sys, sm);
+            const double rr = system_total<VEC_BLOCK>(red.rr, sys, sm);
+            const double rz = system_total<VEC_BLOCK>(red.rz, sys, sm);
+            const double rz_old = system_total<VEC_BLOCK>(red.rz_old, sys, sm);
+            const double rr_true = system_total<VEC_BLOCK>(red.rr_true, sys, sm);
+            if (threadIdx.x == 0)
+            {
+                double *scs = red.sc + static_cast<size_t>(sys) * S_COUNT;
+                scs[S_PAP] = pap;
+                scs[S_RR] = rr;
+                scs[S_RZ] = rz;
+                scs[S_RZ_OLD] = rz_old;
+                scs[S_RR_TRUE] = rr_true;
+            }
         }
 
         int stream_grid(long long n)
@@ -330,21 +339,21 @@ namespace psmas
     } // namespace
 
     template <int BD, typename TV, typename TX>
-    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, double *sc, const SysMap &sm, ReduceSite site,
+    void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, const Reductions &red, const SysMap &sm,
                         hipStream_t s)
     {
         const int grid = A.nslices / (SPMV_BLOCK / WAVE);
-        hipLaunchKernelGGL((k_spmv_ap<BD, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, y, sc, sm, site);
+        hipLaunchKernelGGL((k_spmv_ap<BD, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, y, red, sm);
         PSMAS_LAUNCH_CHECK();
     }
 
     template <int BD, typename TV, typename TX, typename TY>
     void launch_spmv_res(const SellView<TV> &A, const TX *x, const TX *b, TY *y, FineRestrict fr, TY *yc0,
-                         double *sc, int rr_slot, const SysMap &sm, ReduceSite site, bool selected_only, hipStream_t s)
+                         double *rr_partials, const SysMap &sm, bool selected_only, hipStream_t s)
     {
         const int grid = A.nslices / (SPMV_BLOCK / WAVE);
-        hipLaunchKernelGGL((k_spmv_res<BD, TV, TX, TY>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, b, y, fr, yc0, sc,
-                           rr_slot, sm, site, selected_only ? 1 : 0);
+        hipLaunchKernelGGL((k_spmv_res<BD, TV, TX, TY>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, b, y, fr, yc0,
+                           rr_partials, sm, selected_only ? 1 : 0);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -357,25 +366,24 @@ namespace psmas
     }
 
     template <int BD, typename T>
-    void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0, double *sc,
-                          const SysMap &sm, ReduceSite site, hipStream_t s)
+    void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0,
+                          const Reductions &red, const SysMap &sm, hipStream_t s)
     {
-        hipLaunchKernelGGL((k_update_xr<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, Ap, x, r, fr, rc0, sc, sm,
-                           site);
+        hipLaunchKernelGGL((k_update_xr<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, Ap, x, r, fr, rc0, red, sm);
         PSMAS_LAUNCH_CHECK();
     }
 
     template <int BD, typename T>
-    void launch_axpy_alpha(int nblocks, const T *p, T *x, const double *sc, const SysMap &sm, hipStream_t s)
+    void launch_axpy_alpha(int nblocks, const T *p, T *x, const Reductions &red, const SysMap &sm, hipStream_t s)
     {
-        hipLaunchKernelGGL((k_axpy_alpha<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, x, sc, sm);
+        hipLaunchKernelGGL((k_axpy_alpha<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, x, red, sm);
         PSMAS_LAUNCH_CHECK();
     }
 
     template <int BD, typename T>
-    void launch_update_p(int nblocks, const T *z, T *p, const double *sc, const SysMap &sm, hipStream_t s)
+    void launch_update_p(int nblocks, const T *z, T *p, const Reductions &red, const SysMap &sm, hipStream_t s)
     {
-        hipLaunchKernelGGL((k_update_p<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, z, p, sc, sm);
+        hipLaunchKernelGGL((k_update_p<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, z, p, red, sm);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -427,45 +435,37 @@ namespace psmas
         PSMAS_LAUNCH_CHECK();
     }
 
-    template <typename T>
-    void launch_dot(long long n, const T *a, const T *b, double *sc, int slot, ReduceSite site, hipStream_t s)
+    void launch_finalize(int nsys, const Reductions &red, const SysMap &sm, hipStream_t s)
     {
-        int grid = static_cast<int>(div_up_ll(n, VEC_BLOCK));
-        if (grid > 2048)
-            grid = 2048;
-        if (grid < 1)
-            grid = 1;
-        hipLaunchKernelGGL((k_dot<T>), dim3(grid), dim3(VEC_BLOCK), 0, s, n, a, b, sc, slot, site);
+        hipLaunchKernelGGL(k_finalize, dim3(nsys), dim3(VEC_BLOCK), 0, s, red, sm);
         PSMAS_LAUNCH_CHECK();
     }
 
 #define PSMAS_INST_BD(BD)                                                                                          \
-    template void launch_spmv_ap<BD, double, double>(const SellView<double> &, const double *, double *, double *,  \
-                                                     const SysMap &, ReduceSite, hipStream_t);                    \
-    template void launch_spmv_ap<BD, float, float>(const SellView<float> &, const float *, float *, double *,      \
-                                                   const SysMap &, ReduceSite, hipStream_t);                      \
+    template void launch_spmv_ap<BD, double, double>(const SellView<double> &, const double *, double *,          \
+                                                     const Reductions &, const SysMap &, hipStream_t);            \
+    template void launch_spmv_ap<BD, float, float>(const SellView<float> &, const float *, float *,               \
+                                                   const Reductions &, const SysMap &, hipStream_t);              \
     template void launch_spmv_res<BD, double, double, double>(const SellView<double> &, const double *,           \
                                                               const double *, double *, FineRestrict, double *,   \
-                                                              double *, int, const SysMap &, ReduceSite, bool,    \
-                                                              hipStream_t);                                       \
+                                                              double *, const SysMap &, bool, hipStream_t);       \
     template void launch_spmv_res<BD, double, double, float>(const SellView<double> &, const double *,            \
                                                              const double *, float *, FineRestrict, float *,      \
-                                                             double *, int, const SysMap &, ReduceSite, bool,     \
-                                                             hipStream_t);                                        \
+                                                             double *, const SysMap &, bool, hipStream_t);        \
     template void launch_spmv_plain<BD, double, double>(const SellView<double> &, const double *, double *,       \
                                                         hipStream_t);                                             \
     template void launch_spmv_plain<BD, float, float>(const SellView<float> &, const float *, float *,            \
                                                       hipStream_t);                                               \
     template void launch_update_xr<BD, double>(int, const double *, const double *, double *, double *,           \
-                                               FineRestrict, double *, double *, const SysMap &, ReduceSite,      \
+                                               FineRestrict, double *, const Reductions &, const SysMap &,        \
                                                hipStream_t);                                                      \
     template void launch_update_xr<BD, float>(int, const float *, const float *, float *, float *, FineRestrict,  \
-                                              float *, double *, const SysMap &, ReduceSite, hipStream_t);        \
-    template void launch_axpy_alpha<BD, double>(int, const double *, double *, const double *, const SysMap &,    \
+                                              float *, const Reductions &, const SysMap &, hipStream_t);          \
+    template void launch_axpy_alpha<BD, double>(int, const double *, double *, const Reductions &, const SysMap &, \
                                                 hipStream_t);                                                     \
-    template void launch_update_p<BD, double>(int, const double *, double *, const double *, const SysMap &,      \
+    template void launch_update_p<BD, double>(int, const double *, double *, const Reductions &, const SysMap &,   \
                                               hipStream_t);                                                       \
-    template void launch_update_p<BD, float>(int, const float *, float *, const double *, const SysMap &,         \
+    template void launch_update_p<BD, float>(int, const float *, float *, const Reductions &, const SysMap &,      \
                                              hipStream_t);                                                        \
     template void launch_accumulate_correction<BD>(int, float *, double *, const SysMap &, hipStream_t);
 
@@ -484,7 +484,4 @@ namespace psmas
     template void launch_fill<unsigned>(long long, unsigned *, unsigned, hipStream_t);
     template void launch_gather_in<double>(int, int, int, const int *, const double *, double *, hipStream_t);
     template void launch_scatter_out<double>(int, int, int, const int *, const double *, double *, hipStream_t);
-    template void launch_dot<double>(long long, const double *, const double *, double *, int, ReduceSite,
-                                     hipStream_t);
-    template void launch_dot<float>(long long, const float *, const float *, double *, int, ReduceSite, hipStream_t);
 } // namespace psmas
