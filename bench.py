@@ -84,6 +84,10 @@ def main() -> int:
     ap.add_argument("--coarse", default="cholesky", choices=["cholesky", "bank"],
                     help="coarsest-level solver of the mixed path")
     ap.add_argument("--profile-only", action="store_true", help="time steps only (for rocprof runs)")
+    ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (no HIP graph replay)")
+    ap.add_argument("--max-iter", type=int, default=20000, help="iteration cap (short counter-collection runs)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="profile mode: first run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -100,8 +104,8 @@ def main() -> int:
     log(f"  n={prob.n} nnz={prob.A.nnz} ({time.perf_counter() - t:.1f}s)")
 
     tol = 1e-8
-    base = {"block_dim": 3, "relative_tolerance": tol, "absolute_tolerance": 0.0, "max_iter": 20000,
-            "aggregation": args.aggregation}
+    base = {"block_dim": 3, "relative_tolerance": tol, "absolute_tolerance": 0.0, "max_iter": args.max_iter,
+            "aggregation": args.aggregation, "use_graphs": not args.no_graphs}
     solver = MASSolver()
     solver.set_parameters({"MAS": dict(base, precision="fp64")})
     solver.set_coordinates(prob.coords)
@@ -120,6 +124,19 @@ def main() -> int:
         x = x_dev.download()
         rel = float(np.linalg.norm(prob.b - prob.A @ x) / bnorm)
         return x, rel
+
+    if args.profile_only and args.copy_calibration_mib > 0:
+        # A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counter readings give
+        # the factor that turns those counters into bytes for the solver's kernels.
+        import ctypes
+        from polysolve_amd import _native
+        cnt = args.copy_calibration_mib * (1 << 20) // 8
+        src, dst = DeviceArray(cnt), DeviceArray(cnt)
+        _native.lib().psmas_memcpy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), cnt * 8, 3)
+        synchronize()
+        log(f"  calibration copy: {args.copy_calibration_mib} MiB device to device")
+        src.free()
+        dst.free()
 
     # FP64 reference path.
     log("fp64 path")

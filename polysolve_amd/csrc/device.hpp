@@ -47,13 +47,13 @@ namespace psmas
         return s;
     }
 
-    /// The workgroup's sum of v -> partials[blockIdx.x].
+    /// The workgroup's sum of v -> partials[block] (block = the node block it processed).
     template <int BLOCK>
-    __device__ __forceinline__ void store_partial(double v, double *partials)
+    __device__ __forceinline__ void store_partial(double v, double *partials, int block)
     {
         const double s = block_sum<BLOCK>(v);
         if (threadIdx.x == 0)
-            partials[blockIdx.x] = s;
+            partials[block] = s;
     }
 
     /// Sum of the partials of system sys's node blocks, in block order, returned to every thread.

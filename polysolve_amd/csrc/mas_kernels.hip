@@ -173,7 +173,7 @@ namespace psmas
                 mz[static_cast<size_t>(p) * BD + i] = zi;
                 dot += static_cast<double>(r[static_cast<size_t>(p) * BD + i]) * static_cast<double>(zi);
             }
-            store_partial<NODE_BLOCK>(dot, red.rz);
+            store_partial<NODE_BLOCK>(dot, red.rz, blockIdx.x);
         }
 
         // ----------------------------------------------------------------------- build: levels

@@ -30,29 +30,72 @@ namespace psmas
             using type = T;
         };
 
+        /// Row `lane` of SELL slice `slice`: acc += A_row x.  Slots are processed U at a time: the
+        /// batch's x gathers use column indices loaded one batch earlier, so per batch the wave
+        /// waits for one memory round trip instead of two (column, then x).  Every load is issued
+        /// unconditionally (slot indices past the slice end are clamped to its last slot, whose
+        /// lines are in cache, and their values are zeroed), which lets the compiler count the
+        /// loads in flight and keep the column prefetch outstanding across the FMAs.
         template <int BD, typename TV, typename TX, typename TA>
         __device__ __forceinline__ void sell_row(const SellView<TV> &A, int slice, int lane, const TX *__restrict__ x,
                                                  TA acc[BD])
         {
+            constexpr int U = 4;
             const int beg = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice]);
             const int end = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice + 1]);
+            if (end <= beg)
+                return;
+            const int last = end - 1;
             const int *__restrict__ cols = A.cols + lane;
             const TV *__restrict__ vals = A.vals + lane;
-#pragma unroll 2
-            for (int k = beg; k < end; ++k)
+            int c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                c[u] = ldg_stream(cols + static_cast<size_t>(min(beg + u, last)) * WAVE);
+            for (int k = beg; k < end; k += U)
             {
-                const int c = ldg_stream(cols + static_cast<size_t>(k) * WAVE);
-                TA xv[BD];
+                TA xv[U][BD];
+                TV v[U][BD * BD];
 #pragma unroll
-                for (int j = 0; j < BD; ++j)
-                    xv[j] = static_cast<TA>(x[static_cast<size_t>(c) * BD + j]);
-                const TV *__restrict__ v = vals + static_cast<size_t>(k) * (BD * BD * WAVE);
-#pragma unroll
-                for (int i = 0; i < BD; ++i)
+                for (int u = 0; u < U; ++u)
+                {
+                    const size_t slot = static_cast<size_t>(min(k + u, last));
 #pragma unroll
                     for (int j = 0; j < BD; ++j)
-                        acc[i] = fma(static_cast<TA>(ldg_stream(v + (i * BD + j) * WAVE)), xv[j], acc[i]);
+                        xv[u][j] = static_cast<TA>(x[static_cast<size_t>(c[u]) * BD + j]);
+#pragma unroll
+                    for (int e = 0; e < BD * BD; ++e)
+                        v[u][e] = ldg_stream(vals + (slot * (BD * BD) + e) * WAVE);
+                }
+                // Keep the next batch's column loads behind this batch's loads and ahead of the
+                // FMAs, so that they stay in flight while the FMAs wait for this batch.
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    c[u] = ldg_stream(cols + static_cast<size_t>(min(k + U + u, last)) * WAVE);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    const bool ok = k + u < end; // wave-uniform
+#pragma unroll
+                    for (int i = 0; i < BD; ++i)
+#pragma unroll
+                        for (int j = 0; j < BD; ++j)
+                            acc[i] = fma(ok ? static_cast<TA>(v[u][i * BD + j]) : TA(0), xv[u][j], acc[i]);
+                }
             }
+        }
+
+        /// Node block of this workgroup for the SpMV kernels.  Workgroups are dealt to the 8 XCDs
+        /// round-robin; this gives each XCD one contiguous range of node blocks, so the x gathers
+        /// of neighbouring rows hit the same L2.  A bijection on [0, gridDim.x).
+        __device__ __forceinline__ int xcd_block()
+        {
+            constexpr int NXCD = 8;
+            const int G = gridDim.x, b = blockIdx.x;
+            const int q = G / NXCD, r = G % NXCD, xcd = b % NXCD;
+            return xcd * q + (xcd < r ? xcd : r) + b / NXCD;
         }
 
         __device__ __forceinline__ int block_system(const SysMap &sm) { return sm.block_sys[blockIdx.x]; }
@@ -62,11 +105,12 @@ namespace psmas
                                                                 TX *__restrict__ y, Reductions red, SysMap sm)
         {
             using TA = typename Acc<TV>::type;
-            const int sys = block_system(sm);
+            const int blk = xcd_block();
+            const int sys = sm.block_sys[blk];
             if (sm.active[sys] == 0)
                 return;
             const int lane = threadIdx.x % WAVE;
-            const int slice = blockIdx.x * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
             TA acc[BD];
 #pragma unroll
             for (int i = 0; i < BD; ++i)
@@ -81,7 +125,7 @@ namespace psmas
                 y[row * BD + i] = yi;
                 dot += static_cast<double>(x[row * BD + i]) * static_cast<double>(yi);
             }
-            store_partial<SPMV_BLOCK>(dot, red.pap);
+            store_partial<SPMV_BLOCK>(dot, red.pap, blk);
         }
 
         /// Leader lanes of a bank sum their component's members from LDS into the coarse vector.
@@ -117,11 +161,12 @@ namespace psmas
         {
             using TA = typename Acc<TV>::type;
             __shared__ TY sv[SPMV_BLOCK * BD];
-            const int sys = block_system(sm);
+            const int blk = xcd_block();
+            const int sys = sm.block_sys[blk];
             if (sm.active[sys] == 0 || (selected_only && sm.select[sys] == 0))
                 return;
             const int lane = threadIdx.x % WAVE;
-            const int slice = blockIdx.x * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
             TA acc[BD];
 #pragma unroll
@@ -142,7 +187,7 @@ namespace psmas
             const int gid = fr.gid[row];
             __syncthreads();
             restrict_bank<BD, TY>(sv, (threadIdx.x / 32) * 32, mask, gid, yc0);
-            store_partial<SPMV_BLOCK>(dot, rr_partials);
+            store_partial<SPMV_BLOCK>(dot, rr_partials, blk);
         }
 
         template <int BD, typename TV, typename TX>
@@ -203,7 +248,7 @@ namespace psmas
             const int gid = fr.gid[node];
             __syncthreads();
             restrict_bank<BD, T>(sv, (threadIdx.x / 32) * 32, mask, gid, rc0);
-            store_partial<VEC_BLOCK>(dot, red.rr);
+            store_partial<VEC_BLOCK>(dot, red.rr, blockIdx.x);
         }
 
         template <int BD, typename T>
