@@ -72,6 +72,15 @@ def run_path(solver, b_dev, x_dev, n, steps, warmup, sync):
     return (time.perf_counter() - t0) * 1e3 / max(steps, 1)
 
 
+# HBM bytes per launch measured with the FETCH_SIZE and WRITE_SIZE counters (rocprofv3 --pmc, one
+# counter per run) on the 64^3 workload, each scaled by the reading for a 256 MiB device copy
+# (FETCH_SIZE reads 0.500x the bytes such a copy reads, WRITE_SIZE 1.000x).  Keyed by
+# (config, kernel); re-measure when a kernel's memory access changes.
+MEASURED_TRAFFIC = {
+    (2, "spmv_fp32"): 302.5e6 + 3.3e6,
+}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,7 +205,7 @@ def main() -> int:
     dom_name = max(kt.items(), key=lambda kv: kv[1]["total_ms"])[0]
     dom = kernels[dom_name]
     roofline = {"bound": "hbm", "achieved": round(dom["GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name,
+                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get((args.config, dom_name)), "kernel": dom_name,
                 "avg_us": round(dom["avg_us"], 2), "bytes_per_launch": dom["bytes"]}
 
     # CPU baseline: the oracle's FP64 MAS-PCG (reference schedule), single thread, on a bounded
