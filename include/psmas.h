@@ -104,6 +104,17 @@ int psmas_batch_get_info(psmas_handle h, int sys, char *buf, size_t cap, size_t 
 /* Coordinates of system sys (before psmas_batch_factorize_csc). */
 int psmas_batch_set_coordinates(psmas_handle h, int sys, int nb, int dim, const double *xyz);
 
+/* Learned coarse correction of the mixed path (no counterpart in the reference).  With the
+ * parameter learned_collect, every converged single-system mixed solve turns snapshot iterates
+ * into training pairs (u = P^T A P w, w) on a lattice coarse space P built from the coordinates;
+ * psmas_learned_train fits the correction network to all pairs collected so far (Adam on the GPU,
+ * cosine learning-rate decay) and *loss receives the final epoch's mean loss; with the parameter
+ * learned_correction the mixed solve then adds the network's coarse correction inside a flexible
+ * PCG and falls back to plain MAS when the residual stops decreasing.  psmas_learned_reset drops
+ * the network (and the pairs when drop_pairs != 0). */
+int psmas_learned_train(psmas_handle h, int epochs, int batch, float lr, unsigned seed, double *loss);
+int psmas_learned_reset(psmas_handle h, int drop_pairs);
+
 /* Number of visible GPUs (0 on a machine without one). */
 int psmas_device_count(void);
 
@@ -117,11 +128,12 @@ int psmas_device_synchronize(void);
 
 /* ---- Learned coarse-correction network (no counterpart in the reference) ------------------ */
 
-/* MLP d_in -> hidden -> hidden -> d_out (ReLU), BF16-operand MFMA GEMMs with FP32 accumulation,
- * FP32 master weights trained by Adam on the GPU.  Parameter vector (FP32):
- * [W1 (hidden x d_in) | b1 | W2 (hidden x hidden) | b2 | W3 (d_out x hidden) | b3], row-major. */
+/* MLP d_in -> hidden -> hidden -> d_out (ReLU) with an optional linear skip layer S (skip != 0),
+ * BF16-operand MFMA GEMMs with FP32 accumulation, FP32 master weights trained by Adam on the GPU.
+ * Parameter vector (FP32): [W1 (hidden x d_in) | b1 | W2 (hidden x hidden) | b2 | W3 (d_out x hidden)
+ * | b3 | S (d_out x d_in, skip only)], row-major; y = W3 relu(W2 relu(W1 x + b1) + b2) + b3 (+ S x). */
 typedef struct psmas_mlp *psmas_mlp_handle;
-int psmas_mlp_create(int d_in, int d_out, int hidden, unsigned seed, psmas_mlp_handle *out);
+int psmas_mlp_create(int d_in, int d_out, int hidden, int skip, unsigned seed, psmas_mlp_handle *out);
 int psmas_mlp_destroy(psmas_mlp_handle h);
 const char *psmas_mlp_last_error(psmas_mlp_handle h);
 long long psmas_mlp_num_parameters(psmas_mlp_handle h);
@@ -164,6 +176,13 @@ int psmas_apply_preconditioner(psmas_handle h, int n, const double *r, double *z
 
 /* y = A x in the input ordering with the solver's stored matrix (fp32 != 0: FP32 copy). */
 int psmas_spmv(psmas_handle h, int n, const double *x, double *y, int fp32);
+
+/* Lattice coarse space of the learned correction for the factorised single system (built from
+ * its coordinates with the current lattice_points): *m = coarse unknowns; u[m] = P^T v and
+ * e[n] = P w with v, e in the input ordering. */
+int psmas_lattice_size(psmas_handle h, int *m);
+int psmas_lattice_restrict(psmas_handle h, int n, const double *v, double *u);
+int psmas_lattice_prolong(psmas_handle h, int n, const double *w, double *e);
 
 /* Per-kernel HIP-event timing of the solve's dominant kernels (disables graph replay while on).
  * psmas_get_kernel_timing writes {"name": {"launches": k, "total_ms": t, "bytes": b}, ...}. */

@@ -49,12 +49,17 @@ SIGNATURES = {
     "psmas_batch_solve": (_i, [_vp, _i, C.POINTER(_pd), C.POINTER(_pd), _i]),
     "psmas_batch_get_info": (_i, [_vp, _i, C.c_char_p, _sz, C.POINTER(_sz)]),
     "psmas_batch_set_coordinates": (_i, [_vp, _i, _i, _i, _pd]),
+    "psmas_lattice_size": (_i, [_vp, _pi]),
+    "psmas_lattice_restrict": (_i, [_vp, _i, _pd, _pd]),
+    "psmas_lattice_prolong": (_i, [_vp, _i, _pd, _pd]),
+    "psmas_learned_train": (_i, [_vp, _i, _i, C.c_float, C.c_uint, _pd]),
+    "psmas_learned_reset": (_i, [_vp, _i]),
     "psmas_device_count": (_i, []),
     "psmas_device_alloc": (_i, [_sz, C.POINTER(_vp)]),
     "psmas_device_free": (_i, [_vp]),
     "psmas_memcpy": (_i, [_vp, _vp, _sz, _i]),
     "psmas_device_synchronize": (_i, []),
-    "psmas_mlp_create": (_i, [_i, _i, _i, C.c_uint, C.POINTER(_vp)]),
+    "psmas_mlp_create": (_i, [_i, _i, _i, _i, C.c_uint, C.POINTER(_vp)]),
     "psmas_mlp_destroy": (_i, [_vp]),
     "psmas_mlp_last_error": (_cp, [_vp]),
     "psmas_mlp_num_parameters": (_ll, [_vp]),

@@ -361,6 +361,53 @@ extern "C"
         });
     }
 
+    int psmas_lattice_size(psmas_handle h, int *m)
+    {
+        if (!h || !m)
+            return h ? PSMAS_ERROR_INVALID_ARGUMENT : null_handle();
+        return guarded(h->last_error, [&] { *m = h->solver.lattice_size(); });
+    }
+
+    int psmas_lattice_restrict(psmas_handle h, int n, const double *v, double *u)
+    {
+        if (!h || !v || !u)
+            return h ? PSMAS_ERROR_INVALID_ARGUMENT : null_handle();
+        return guarded(h->last_error, [&] {
+            if (n != h->solver.dim())
+                throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
+            h->solver.lattice_restrict(v, u);
+        });
+    }
+
+    int psmas_lattice_prolong(psmas_handle h, int n, const double *w, double *e)
+    {
+        if (!h || !w || !e)
+            return h ? PSMAS_ERROR_INVALID_ARGUMENT : null_handle();
+        return guarded(h->last_error, [&] {
+            if (n != h->solver.dim())
+                throw std::runtime_error("[MAS] Size mismatch. Did you forget to call factorize?");
+            h->solver.lattice_prolong(w, e);
+        });
+    }
+
+    int psmas_learned_train(psmas_handle h, int epochs, int batch, float lr, unsigned seed, double *loss)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            const double l = h->solver.learned_train(epochs, batch, lr, seed);
+            if (loss)
+                *loss = l;
+        });
+    }
+
+    int psmas_learned_reset(psmas_handle h, int drop_pairs)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.learned_reset(drop_pairs != 0); });
+    }
+
     int psmas_set_kernel_timing(psmas_handle h, int on)
     {
         if (!h)

@@ -58,6 +58,22 @@ namespace psmas
         R_COUNT = 5,
     };
 
+    /// Per-slot lattice data and the coarse coefficients, as the prolongation kernel reads them.
+    struct LccApply
+    {
+        const int4 *cell = nullptr; ///< {corner-0 lattice point (global) or -1, fx, fy, fz as float bits}
+        const float *coef = nullptr; ///< [lattice point][bd] coarse correction
+        int k = 0;                   ///< lattice points per axis
+    };
+
+    /// Flexible-PCG extras for the prolongation kernel: r . z_prev partials.
+    template <typename T>
+    struct FlexDot
+    {
+        const T *zprev = nullptr;
+        double *rzo = nullptr; ///< per-node-block partials of r . z_prev
+    };
+
     /// The PCG reductions.  Producers write one partial sum per node block (workgroup); every
     /// consumer re-sums the partials of its own system in one fixed order, so all workgroups see
     /// bit-identical alpha / beta without atomics or cross-workgroup fences.  sc holds S_COUNT
@@ -111,9 +127,16 @@ namespace psmas
     template <int BD, typename T>
     void launch_axpy_alpha(int nblocks, const T *p, T *x, const Reductions &red, const SysMap &sm, hipStream_t s);
 
-    /// p = z + beta p (beta = rz / rz_old).
+    /// p = z + beta p (beta = rz / rz_old).  Flexible form when rzo is set: beta =
+    /// (rz - rzo) / rz_old, and z is copied to zprev.
     template <int BD, typename T>
-    void launch_update_p(int nblocks, const T *z, T *p, const Reductions &red, const SysMap &sm, hipStream_t s);
+    void launch_update_p(int nblocks, const T *z, T *p, const Reductions &red, const SysMap &sm, hipStream_t s,
+                         const double *rzo = nullptr, T *zprev = nullptr);
+
+    /// out = x64 + x32 (snapshot of the mixed path's iterate).
+    void launch_snapshot(long long n, const double *x64, const float *x32, double *out, hipStream_t s);
+    /// out = a - b.
+    void launch_difference(long long n, const double *a, const double *b, double *out, hipStream_t s);
 
     /// Per-system totals of every reduction -> sc (S_PAP, S_RR, S_RZ, S_RZ_OLD, S_RR_TRUE) for the host.
     void launch_finalize(int nsys, const Reductions &red, const SysMap &sm, hipStream_t s);
@@ -162,11 +185,13 @@ namespace psmas
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
                           const int *bank_sys, const int *active, hipStream_t s);
 
-    /// z_p = mz_p + sum_l mz_l[cmap_p.l]; partials of r . z -> red.rz (the previous ones move to
-    /// red.rz_old unless init).
+    /// z_p = mz_p + sum_l mz_l[cmap_p.l] (+ the lattice correction P c when lcc.cell is set);
+    /// partials of r . z -> red.rz (the previous ones move to red.rz_old unless init) and, when
+    /// flex.zprev is set, of r . z_prev -> flex.rzo.
     template <int BD, typename T>
     void launch_prolong_dot(int nblocks, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
-                            const Reductions &red, bool init, const SysMap &sm, hipStream_t s);
+                            const Reductions &red, bool init, const SysMap &sm, const LccApply &lcc,
+                            const FlexDot<T> &flex, hipStream_t s);
 
     // ----- MAS build ----------------------------------------------------------------------
     /// Bank-local connected components at coarse level 0 (one lane per slot, two banks per wave).

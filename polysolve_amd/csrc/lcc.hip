@@ -1,0 +1,540 @@
+// Learned coarse correction: lattice coarse space, restriction / prolongation kernels and the
+// BF16 GEMV inference of the correction network (see lcc.hpp).  No counterpart in the reference.
+#include "lcc.hpp"
+#include "coarse_solve.hpp"
+#include "device.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace psmas
+{
+    namespace
+    {
+        constexpr int LCC_CHUNK = 1024; ///< restriction entries per workgroup
+        constexpr int GEMV_SYS = 8;     ///< systems per wave in the inference GEMV
+
+        __device__ __forceinline__ float wave_sum_f(float v)
+        {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+                v += __shfl_xor(v, off, WAVE);
+            return v;
+        }
+
+        __device__ __forceinline__ float trilinear_weight(int corner, float fx, float fy, float fz)
+        {
+            const float wx = (corner & 1) ? fx : 1.0f - fx;
+            const float wy = (corner & 2) ? fy : 1.0f - fy;
+            const float wz = (corner & 4) ? fz : 1.0f - fz;
+            return wx * wy * wz;
+        }
+
+        /// partial[c][d] = sum over chunk c's entries (slot, w) of w * v[slot][d], in entry order.
+        template <int BD, typename T>
+        __global__ __launch_bounds__(256) void k_lcc_restrict_partial(const int *__restrict__ chunk_begin,
+                                                                      const int2 *__restrict__ entries,
+                                                                      const T *__restrict__ v,
+                                                                      double *__restrict__ partial)
+        {
+            const int c = blockIdx.x;
+            const int e0 = chunk_begin[c], e1 = chunk_begin[c + 1];
+            double acc[BD];
+#pragma unroll
+            for (int d = 0; d < BD; ++d)
+                acc[d] = 0.0;
+            for (int e = e0 + threadIdx.x; e < e1; e += 256)
+            {
+                const int2 en = entries[e];
+                const double w = static_cast<double>(__int_as_float(en.y));
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                    acc[d] += w * static_cast<double>(v[static_cast<size_t>(en.x) * BD + d]);
+            }
+#pragma unroll
+            for (int d = 0; d < BD; ++d)
+            {
+                const double s = block_sum<256>(acc[d]);
+                if (threadIdx.x == 0)
+                    partial[static_cast<size_t>(c) * BD + d] = s;
+            }
+        }
+
+        /// One workgroup per system: u = sum of the chunk partials per lattice point (in order),
+        /// x = u sqrt(m) / |u| (zero-padded to md), scale = |u| a / sqrt(m).
+        template <int BD>
+        __global__ __launch_bounds__(256) void k_lcc_finish(int np, int md, const int *__restrict__ point_chunk,
+                                                            const double *__restrict__ partial, double *__restrict__ u,
+                                                            float *__restrict__ x, float *__restrict__ scale,
+                                                            float ascale, const int *__restrict__ active)
+        {
+            const int s = blockIdx.x;
+            if (active != nullptr && active[s] == 0)
+                return;
+            const int m = np * BD;
+            double *us = u + static_cast<size_t>(s) * m;
+            double sq = 0.0;
+            for (int j = threadIdx.x; j < np; j += 256)
+            {
+                const int J = s * np + j;
+                double a[BD];
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                    a[d] = 0.0;
+                for (int c = point_chunk[J]; c < point_chunk[J + 1]; ++c)
+#pragma unroll
+                    for (int d = 0; d < BD; ++d)
+                        a[d] += partial[static_cast<size_t>(c) * BD + d];
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                {
+                    us[j * BD + d] = a[d];
+                    sq += a[d] * a[d];
+                }
+            }
+            __shared__ double norm;
+            sq = block_sum<256>(sq);
+            if (threadIdx.x == 0)
+                norm = sqrt(sq);
+            __syncthreads();
+            const double nu = norm;
+            const double f = nu > 0.0 ? sqrt(static_cast<double>(m)) / nu : 0.0;
+            float *xs = x + static_cast<size_t>(s) * md;
+            for (int i = threadIdx.x; i < md; i += 256)
+                xs[i] = i < m ? static_cast<float>(us[i] * f) : 0.0f;
+            if (threadIdx.x == 0)
+                scale[s] = static_cast<float>(nu * ascale / sqrt(static_cast<double>(m)));
+        }
+
+        struct GemvSeg
+        {
+            const unsigned short *W = nullptr; ///< BF16 rows of length K (K % 8 == 0)
+            const float *bias = nullptr;
+            int rows = 0;
+            int relu = 0;
+            float *Y = nullptr;
+            int ldy = 0;
+        };
+
+        __device__ __forceinline__ float bf16_lo(unsigned v) { return __uint_as_float(v << 16); }
+        __device__ __forceinline__ float bf16_hi(unsigned v) { return __uint_as_float(v & 0xffff0000u); }
+
+        /// Y[s][row] = act(W[row] . X[s] + bias[row]) (+ add[s][row]) (* scale[s]) for the rows of
+        /// two weight segments; one wave per (row, group of 8 systems), FP32 accumulation.
+        __global__ __launch_bounds__(256) void k_lcc_gemv(GemvSeg a, GemvSeg b, int K, const float *__restrict__ X,
+                                                          int ldx, int nsys, const float *__restrict__ add,
+                                                          int ld_add, const float *__restrict__ scale,
+                                                          const int *__restrict__ active)
+        {
+            const int wave = blockIdx.x * 4 + threadIdx.x / WAVE;
+            const int lane = threadIdx.x % WAVE;
+            const int ngroups = (nsys + GEMV_SYS - 1) / GEMV_SYS;
+            const int row_all = wave / ngroups, g = wave % ngroups;
+            if (row_all >= a.rows + b.rows)
+                return;
+            const bool first = row_all < a.rows;
+            const int row = first ? row_all : row_all - a.rows;
+            const unsigned short *W = first ? a.W : b.W;
+            const int s0 = g * GEMV_SYS;
+            const int ns = min(GEMV_SYS, nsys - s0);
+            float acc[GEMV_SYS];
+#pragma unroll
+            for (int j = 0; j < GEMV_SYS; ++j)
+                acc[j] = 0.0f;
+            const uint4 *wrow = reinterpret_cast<const uint4 *>(W + static_cast<size_t>(row) * K);
+            for (int c = lane; c < K / 8; c += WAVE)
+            {
+                const uint4 wv = wrow[c];
+                const float w[8] = {bf16_lo(wv.x), bf16_hi(wv.x), bf16_lo(wv.y), bf16_hi(wv.y),
+                                    bf16_lo(wv.z), bf16_hi(wv.z), bf16_lo(wv.w), bf16_hi(wv.w)};
+#pragma unroll
+                for (int j = 0; j < GEMV_SYS; ++j)
+                {
+                    if (j < ns)
+                    {
+                        const float4 *xp = reinterpret_cast<const float4 *>(X + static_cast<size_t>(s0 + j) * ldx) + 2 * c;
+                        const float4 x0 = xp[0], x1 = xp[1];
+                        acc[j] += w[0] * x0.x + w[1] * x0.y + w[2] * x0.z + w[3] * x0.w + w[4] * x1.x + w[5] * x1.y +
+                                  w[6] * x1.z + w[7] * x1.w;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < GEMV_SYS; ++j)
+                acc[j] = wave_sum_f(acc[j]);
+            if (lane != 0)
+                return;
+            const GemvSeg &S = first ? a : b;
+#pragma unroll
+            for (int j = 0; j < GEMV_SYS; ++j)
+            {
+                if (j >= ns)
+                    break;
+                const int s = s0 + j;
+                if (active != nullptr && active[s] == 0)
+                    continue;
+                float y = acc[j];
+                if (S.bias)
+                    y += S.bias[row];
+                if (S.relu)
+                    y = y > 0.0f ? y : 0.0f;
+                if (add)
+                    y += add[static_cast<size_t>(s) * ld_add + row];
+                if (scale)
+                    y *= scale[s];
+                S.Y[static_cast<size_t>(s) * S.ldy + row] = y;
+            }
+        }
+
+        /// e[p][d] = sum_corners w * coef[corner][d] for the slots of system 0; zero elsewhere.
+        template <int BD>
+        __global__ void k_lcc_prolong_fine(int npad, int np, const int4 *__restrict__ cell, int k,
+                                           const double *__restrict__ coef, double *__restrict__ e)
+        {
+            const int p = blockIdx.x * blockDim.x + threadIdx.x;
+            if (p >= npad)
+                return;
+            const int4 c = cell[p];
+            double z[BD];
+#pragma unroll
+            for (int d = 0; d < BD; ++d)
+                z[d] = 0.0;
+            if (c.x >= 0 && c.x < np)
+            {
+                const float fx = __int_as_float(c.y), fy = __int_as_float(c.z), fz = __int_as_float(c.w);
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                {
+                    const int id = c.x + (q & 1) + ((q >> 1) & 1) * k + ((q >> 2) & 1) * k * k;
+                    const double w = static_cast<double>(trilinear_weight(q, fx, fy, fz));
+#pragma unroll
+                    for (int d = 0; d < BD; ++d)
+                        z[d] += w * coef[static_cast<size_t>(id) * BD + d];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < BD; ++d)
+                e[static_cast<size_t>(p) * BD + d] = z[d];
+        }
+    } // namespace
+
+    // --------------------------------------------------------------------------- lattice
+    void LearnedCoarse::build_lattice(int k, int bd, const std::vector<int> &node_off, const std::vector<int> &npad,
+                                      const std::vector<const std::vector<double> *> &coords,
+                                      const std::vector<int> &block_of_slot, const std::vector<char> &coupled,
+                                      hipStream_t s)
+    {
+        if (k < 2 || k > 16)
+            throw InvalidArgumentError("[MAS] learned correction: lattice_points must be in [2, 16].");
+        const int nsys = static_cast<int>(node_off.size());
+        const int np = k * k * k;
+        const int total_pad = node_off.back() + npad.back();
+        std::vector<int4> cell(total_pad, int4{-1, 0, 0, 0});
+        for (int sy = 0; sy < nsys; ++sy)
+        {
+            const std::vector<double> &X = *coords[sy];
+            const int nb = static_cast<int>(X.size() / 3);
+            if (nb == 0)
+                throw InvalidArgumentError("[MAS] learned correction needs 3-D node coordinates (psmas_set_coordinates).");
+            double lo[3], hi[3];
+            for (int d = 0; d < 3; ++d)
+            {
+                lo[d] = hi[d] = X[d];
+                for (int i = 0; i < nb; ++i)
+                {
+                    lo[d] = std::min(lo[d], X[static_cast<size_t>(i) * 3 + d]);
+                    hi[d] = std::max(hi[d], X[static_cast<size_t>(i) * 3 + d]);
+                }
+            }
+            for (int p = node_off[sy]; p < node_off[sy] + npad[sy]; ++p)
+            {
+                const int b = block_of_slot[p];
+                if (b < 0 || b >= nb || !coupled[p])
+                    continue;
+                int ci[3];
+                float f[3];
+                for (int d = 0; d < 3; ++d)
+                {
+                    const double ext = hi[d] - lo[d];
+                    const double t = ext > 0.0 ? (X[static_cast<size_t>(b) * 3 + d] - lo[d]) / ext * (k - 1) : 0.0;
+                    int c = static_cast<int>(std::floor(t));
+                    c = std::min(std::max(c, 0), k - 2);
+                    ci[d] = c;
+                    f[d] = static_cast<float>(std::min(std::max(t - c, 0.0), 1.0));
+                }
+                int4 v;
+                v.x = sy * np + ci[0] + k * (ci[1] + k * ci[2]);
+                std::memcpy(&v.y, &f[0], 4);
+                std::memcpy(&v.z, &f[1], 4);
+                std::memcpy(&v.w, &f[2], 4);
+                cell[p] = v;
+            }
+        }
+        // Restriction entries grouped by lattice point, in slot order; chunks of LCC_CHUNK.
+        const int npts = nsys * np;
+        std::vector<int> count(npts + 1, 0);
+        auto corners = [&](const int4 &c, int q, int &id, float &w) {
+            float fx, fy, fz;
+            std::memcpy(&fx, &c.y, 4);
+            std::memcpy(&fy, &c.z, 4);
+            std::memcpy(&fz, &c.w, 4);
+            id = c.x + (q & 1) + ((q >> 1) & 1) * k + ((q >> 2) & 1) * k * k;
+            const float wx = (q & 1) ? fx : 1.0f - fx;
+            const float wy = (q & 2) ? fy : 1.0f - fy;
+            const float wz = (q & 4) ? fz : 1.0f - fz;
+            w = wx * wy * wz;
+        };
+        for (int p = 0; p < total_pad; ++p)
+            if (cell[p].x >= 0)
+                for (int q = 0; q < 8; ++q)
+                {
+                    int id;
+                    float w;
+                    corners(cell[p], q, id, w);
+                    ++count[id + 1];
+                }
+        for (int j = 0; j < npts; ++j)
+            count[j + 1] += count[j];
+        std::vector<int2> entries(count[npts]);
+        std::vector<int> fillp(count.begin(), count.end() - 1);
+        for (int p = 0; p < total_pad; ++p)
+            if (cell[p].x >= 0)
+                for (int q = 0; q < 8; ++q)
+                {
+                    int id;
+                    float w;
+                    corners(cell[p], q, id, w);
+                    int wb;
+                    std::memcpy(&wb, &w, 4);
+                    entries[fillp[id]++] = int2{p, wb};
+                }
+        std::vector<int> chunk_begin{0}, point_chunk(npts + 1, 0);
+        for (int j = 0; j < npts; ++j)
+        {
+            for (int e = count[j]; e < count[j + 1]; e += LCC_CHUNK)
+                chunk_begin.push_back(std::min(e + LCC_CHUNK, count[j + 1]));
+            point_chunk[j + 1] = static_cast<int>(chunk_begin.size()) - 1;
+        }
+        nchunks_ = static_cast<int>(chunk_begin.size()) - 1;
+
+        // (P^T P)^-1 of system 0 (same for every component): the least-squares projection.
+        std::vector<double> ptp(static_cast<size_t>(np) * np, 0.0);
+        for (int p = node_off[0]; p < node_off[0] + npad[0]; ++p)
+        {
+            if (cell[p].x < 0)
+                continue;
+            int id[8];
+            float w[8];
+            for (int q = 0; q < 8; ++q)
+                corners(cell[p], q, id[q], w[q]);
+            for (int a = 0; a < 8; ++a)
+                for (int c = 0; c < 8; ++c)
+                    ptp[static_cast<size_t>(id[a]) * np + id[c]] += static_cast<double>(w[a]) * w[c];
+        }
+        for (int j = 0; j < np; ++j)
+            if (!(ptp[static_cast<size_t>(j) * np + j] > 0.0))
+                ptp[static_cast<size_t>(j) * np + j] = 1.0; // lattice point without slots
+        if (!cholesky_inverse(np, ptp))
+            throw std::runtime_error("[MAS] learned correction: lattice mass matrix is singular.");
+        ptp_inv_ = std::move(ptp);
+        h_cell0_.assign(cell.begin() + node_off[0], cell.begin() + node_off[0] + npad[0]);
+
+        k_ = k;
+        bd_ = bd;
+        np_ = np;
+        nsys_ = nsys;
+        md_ = (np * bd + 7) / 8 * 8;
+        d_cell_.allocate(cell.size(), "MAS lattice cells");
+        d_entries_.allocate(std::max<size_t>(entries.size(), 1), "MAS lattice restriction");
+        d_chunk_begin_.allocate(chunk_begin.size(), "MAS lattice chunks");
+        d_point_chunk_.allocate(point_chunk.size(), "MAS lattice chunks");
+        PSMAS_HIP(hipMemcpy(d_cell_.data(), cell.data(), d_cell_.bytes(), hipMemcpyHostToDevice));
+        if (!entries.empty())
+            PSMAS_HIP(hipMemcpy(d_entries_.data(), entries.data(), sizeof(int2) * entries.size(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_chunk_begin_.data(), chunk_begin.data(), d_chunk_begin_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_point_chunk_.data(), point_chunk.data(), d_point_chunk_.bytes(), hipMemcpyHostToDevice));
+        d_partial_.allocate(static_cast<size_t>(std::max(nchunks_, 1)) * bd, "MAS lattice partials");
+        d_u_.allocate(static_cast<size_t>(nsys) * m(), "MAS lattice residual");
+        d_x_.allocate(static_cast<size_t>(nsys) * md_, "MAS network input");
+        d_scale_.allocate(nsys, "MAS network scale");
+        d_coef_.allocate(static_cast<size_t>(nsys) * m(), "MAS lattice correction");
+        d_x_.zero(s);
+        d_coef_.zero(s);
+        d_h1_.reset();
+        d_h2_.reset();
+        d_skip_.reset();
+    }
+
+    template <int BD, typename T>
+    void LearnedCoarse::restrict_to_lattice(const T *v, const int *active, hipStream_t s, float output_factor)
+    {
+        if (nchunks_ > 0)
+        {
+            hipLaunchKernelGGL((k_lcc_restrict_partial<BD, T>), dim3(nchunks_), dim3(256), 0, s, d_chunk_begin_.data(),
+                               d_entries_.data(), v, d_partial_.data());
+            PSMAS_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL((k_lcc_finish<BD>), dim3(nsys_), dim3(256), 0, s, np_, md_, d_point_chunk_.data(),
+                           d_partial_.data(), d_u_.data(), d_x_.data(), d_scale_.data(), ascale_ * output_factor,
+                           active);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void LearnedCoarse::infer(const int *active, hipStream_t s)
+    {
+        if (!net_)
+            throw std::runtime_error("[MAS] learned correction: the network is not trained.");
+        if (net_->d_in() != md_ || net_->d_out() != md_)
+            throw std::runtime_error("[MAS] learned correction: network width does not match the lattice.");
+        const int h = net_->hidden();
+        if (!d_h1_)
+        {
+            d_h1_.allocate(static_cast<size_t>(nsys_) * h, "MAS network activations");
+            d_h2_.allocate(static_cast<size_t>(nsys_) * h, "MAS network activations");
+            d_skip_.allocate(static_cast<size_t>(nsys_) * md_, "MAS network skip output");
+        }
+        const unsigned short *wb = net_->bf16_weights();
+        const float *wf = net_->fp32_weights();
+        const int ngroups = (nsys_ + GEMV_SYS - 1) / GEMV_SYS;
+        auto grid = [&](int rows) { return dim3(div_up(rows * ngroups, 4)); };
+        // Layer 1 and the skip layer read the same input.
+        GemvSeg l1{wb + net_->offset(0), wf + net_->offset(1), h, 1, d_h1_.data(), h};
+        GemvSeg sk{wb + net_->offset(6), nullptr, md_, 0, d_skip_.data(), md_};
+        hipLaunchKernelGGL(k_lcc_gemv, grid(h + md_), dim3(256), 0, s, l1, sk, md_, d_x_.data(), md_, nsys_,
+                           static_cast<const float *>(nullptr), 0, static_cast<const float *>(nullptr), active);
+        PSMAS_LAUNCH_CHECK();
+        GemvSeg l2{wb + net_->offset(2), wf + net_->offset(3), h, 1, d_h2_.data(), h};
+        hipLaunchKernelGGL(k_lcc_gemv, grid(h), dim3(256), 0, s, l2, GemvSeg{}, h, d_h1_.data(), h, nsys_,
+                           static_cast<const float *>(nullptr), 0, static_cast<const float *>(nullptr), active);
+        PSMAS_LAUNCH_CHECK();
+        // Output layer: only the m real outputs, plus the skip output, times the scale.
+        GemvSeg l3{wb + net_->offset(4), wf + net_->offset(5), m(), 0, d_coef_.data(), m()};
+        hipLaunchKernelGGL(k_lcc_gemv, grid(m()), dim3(256), 0, s, l3, GemvSeg{}, h, d_h2_.data(), h, nsys_,
+                           static_cast<const float *>(d_skip_.data()), md_, static_cast<const float *>(d_scale_.data()),
+                           active);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    LccApply LearnedCoarse::apply_view() const
+    {
+        LccApply v;
+        v.cell = d_cell_.data();
+        v.coef = d_coef_.data();
+        v.k = k_;
+        return v;
+    }
+
+    template <int BD>
+    void LearnedCoarse::prolong_host_coef(const std::vector<double> &w, double *e, long long nvec, hipStream_t s)
+    {
+        const int npad0 = static_cast<int>(h_cell0_.size());
+        if (static_cast<long long>(npad0) * BD > nvec || static_cast<int>(w.size()) != m())
+            throw std::runtime_error("[MAS] learned correction: size mismatch in prolongation.");
+        DeviceBuffer<double> dw(w.size(), "MAS lattice coefficients");
+        PSMAS_HIP(hipMemcpyAsync(dw.data(), w.data(), dw.bytes(), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((k_lcc_prolong_fine<BD>), dim3(div_up(npad0, 256)), dim3(256), 0, s, npad0, np_,
+                           d_cell_.data(), k_, dw.data(), e);
+        PSMAS_LAUNCH_CHECK();
+        PSMAS_HIP(hipStreamSynchronize(s));
+    }
+
+    std::vector<double> LearnedCoarse::download_u(hipStream_t s) const
+    {
+        std::vector<double> u(m());
+        PSMAS_HIP(hipMemcpyAsync(u.data(), d_u_.data(), sizeof(double) * u.size(), hipMemcpyDeviceToHost, s));
+        PSMAS_HIP(hipStreamSynchronize(s));
+        return u;
+    }
+
+    std::vector<double> LearnedCoarse::ls_project(const std::vector<double> &v) const
+    {
+        std::vector<double> w(v.size(), 0.0);
+        for (int i = 0; i < np_; ++i)
+            for (int j = 0; j < np_; ++j)
+            {
+                const double g = ptp_inv_[static_cast<size_t>(i) * np_ + j];
+                for (int d = 0; d < bd_; ++d)
+                    w[static_cast<size_t>(i) * bd_ + d] += g * v[static_cast<size_t>(j) * bd_ + d];
+            }
+        return w;
+    }
+
+    // --------------------------------------------------------------------------- network
+    void LearnedCoarse::add_pair(const std::vector<double> &u, const std::vector<double> &w)
+    {
+        if (m_pairs_ != 0 && m_pairs_ != m())
+            clear_pairs();
+        m_pairs_ = m();
+        for (double v : u)
+            pairs_u_.push_back(static_cast<float>(v));
+        for (double v : w)
+            pairs_w_.push_back(static_cast<float>(v));
+    }
+
+    void LearnedCoarse::clear_pairs()
+    {
+        pairs_u_.clear();
+        pairs_w_.clear();
+        m_pairs_ = 0;
+    }
+
+    void LearnedCoarse::reset_network()
+    {
+        net_.reset();
+        trained_ = false;
+    }
+
+    double LearnedCoarse::train(int hidden, int epochs, int batch, float lr, unsigned seed)
+    {
+        const int mm = m_pairs_;
+        if (mm == 0 || pairs_u_.empty())
+            throw std::runtime_error("[MAS] learned correction: no training pairs collected.");
+        const size_t n = pairs_u_.size() / mm;
+        const int md = (mm + 7) / 8 * 8;
+        // Normalise: x = u sqrt(m) / |u|, t = w sqrt(m) / |u| / a with a the RMS of w sqrt(m) / |u|.
+        std::vector<float> X(n * md, 0.0f), T(n * md, 0.0f);
+        double sumsq = 0.0;
+        size_t used = 0;
+        for (size_t i = 0; i < n; ++i)
+        {
+            double nu = 0.0;
+            for (int j = 0; j < mm; ++j)
+                nu += static_cast<double>(pairs_u_[i * mm + j]) * pairs_u_[i * mm + j];
+            nu = std::sqrt(nu);
+            if (!(nu > 0.0) || !std::isfinite(nu))
+                continue;
+            const double f = std::sqrt(static_cast<double>(mm)) / nu;
+            for (int j = 0; j < mm; ++j)
+            {
+                X[used * md + j] = static_cast<float>(pairs_u_[i * mm + j] * f);
+                const double t = pairs_w_[i * mm + j] * f;
+                T[used * md + j] = static_cast<float>(t);
+                sumsq += t * t;
+            }
+            ++used;
+        }
+        if (used == 0)
+            throw std::runtime_error("[MAS] learned correction: all training pairs are zero.");
+        const double a = std::sqrt(sumsq / (static_cast<double>(used) * mm));
+        for (size_t i = 0; i < used * md; ++i)
+            T[i] = static_cast<float>(T[i] / a);
+        net_ = std::make_unique<Mlp>(md, md, hidden, seed, true);
+        DeviceBuffer<float> dX(used * md, "MAS training inputs"), dT(used * md, "MAS training targets");
+        PSMAS_HIP(hipMemcpy(dX.data(), X.data(), dX.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(dT.data(), T.data(), dT.bytes(), hipMemcpyHostToDevice));
+        last_loss_ = net_->train(static_cast<int>(used), dX.data(), dT.data(), epochs, batch, lr, seed, true);
+        ascale_ = static_cast<float>(a);
+        trained_ = true;
+        return last_loss_;
+    }
+
+#define PSMAS_INST_LCC(BD)                                                                                          \
+    template void LearnedCoarse::restrict_to_lattice<BD, double>(const double *, const int *, hipStream_t, float); \
+    template void LearnedCoarse::restrict_to_lattice<BD, float>(const float *, const int *, hipStream_t, float);   \
+    template void LearnedCoarse::prolong_host_coef<BD>(const std::vector<double> &, double *, long long, hipStream_t);
+    PSMAS_INST_LCC(1)
+    PSMAS_INST_LCC(2)
+    PSMAS_INST_LCC(3)
+#undef PSMAS_INST_LCC
+} // namespace psmas

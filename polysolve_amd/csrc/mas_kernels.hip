@@ -141,7 +141,8 @@ namespace psmas
         __global__ __launch_bounds__(NODE_BLOCK) void k_prolong_dot(int level_num, const int4 *__restrict__ cmap,
                                                                     const int *__restrict__ lev_off,
                                                                     const T *__restrict__ r, T *__restrict__ mz,
-                                                                    Reductions red, bool init, SysMap sm)
+                                                                    Reductions red, bool init, SysMap sm,
+                                                                    LccApply lcc, FlexDot<T> flex)
         {
             const int sys = sm.block_sys[blockIdx.x];
             if (sm.active[sys] == 0)
@@ -165,15 +166,39 @@ namespace psmas
                         z[i] += static_cast<double>(mz[o + i]);
                 }
             }
-            double dot = 0.0;
+            if (lcc.cell != nullptr)
+            {
+                // Learned lattice correction: trilinear interpolation of the coarse coefficients.
+                const int4 lc = lcc.cell[p];
+                if (lc.x >= 0)
+                {
+                    const float fx = __int_as_float(lc.y), fy = __int_as_float(lc.z), fz = __int_as_float(lc.w);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                    {
+                        const float w = ((q & 1) ? fx : 1.0f - fx) * ((q & 2) ? fy : 1.0f - fy) *
+                                        ((q & 4) ? fz : 1.0f - fz);
+                        const int id = lc.x + (q & 1) + ((q >> 1) & 1) * lcc.k + ((q >> 2) & 1) * lcc.k * lcc.k;
+#pragma unroll
+                        for (int i = 0; i < BD; ++i)
+                            z[i] += static_cast<double>(w) * static_cast<double>(lcc.coef[static_cast<size_t>(id) * BD + i]);
+                    }
+                }
+            }
+            double dot = 0.0, dot_old = 0.0;
 #pragma unroll
             for (int i = 0; i < BD; ++i)
             {
                 const T zi = static_cast<T>(z[i]);
                 mz[static_cast<size_t>(p) * BD + i] = zi;
-                dot += static_cast<double>(r[static_cast<size_t>(p) * BD + i]) * static_cast<double>(zi);
+                const double ri = static_cast<double>(r[static_cast<size_t>(p) * BD + i]);
+                dot += ri * static_cast<double>(zi);
+                if (flex.zprev != nullptr)
+                    dot_old += ri * static_cast<double>(flex.zprev[static_cast<size_t>(p) * BD + i]);
             }
             store_partial<NODE_BLOCK>(dot, red.rz, blockIdx.x);
+            if (flex.zprev != nullptr)
+                store_partial<NODE_BLOCK>(dot_old, flex.rzo, blockIdx.x);
         }
 
         // ----------------------------------------------------------------------- build: levels
@@ -602,10 +627,11 @@ namespace psmas
 
     template <int BD, typename T>
     void launch_prolong_dot(int nblocks, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
-                            const Reductions &red, bool init, const SysMap &sm, hipStream_t s)
+                            const Reductions &red, bool init, const SysMap &sm, const LccApply &lcc,
+                            const FlexDot<T> &flex, hipStream_t s)
     {
         hipLaunchKernelGGL((k_prolong_dot<BD, T>), dim3(nblocks), dim3(NODE_BLOCK), 0, s, level_num, cmap, lev_off, r,
-                           mz, red, init, sm);
+                           mz, red, init, sm, lcc, flex);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -721,9 +747,11 @@ namespace psmas
     template void launch_mas_apply<BD, __half, float>(int, int, const __half *, const float *, const float *,     \
                                                       float *, const int *, const int *, hipStream_t);           \
     template void launch_prolong_dot<BD, double>(int, int, const int4 *, const int *, const double *, double *,   \
-                                                 const Reductions &, bool, const SysMap &, hipStream_t);         \
+                                                 const Reductions &, bool, const SysMap &, const LccApply &,     \
+                                                 const FlexDot<double> &, hipStream_t);                          \
     template void launch_prolong_dot<BD, float>(int, int, const int4 *, const int *, const float *, float *,      \
-                                                const Reductions &, bool, const SysMap &, hipStream_t);
+                                                const Reductions &, bool, const SysMap &, const LccApply &,      \
+                                                const FlexDot<float> &, hipStream_t);
 
     PSMAS_INST_APPLY(1)
     PSMAS_INST_APPLY(2)

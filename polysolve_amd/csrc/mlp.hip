@@ -1,7 +1,7 @@
-// Small MLP (d_in -> 128 -> 128 -> d_out, ReLU) trained and evaluated on the GPU with hand-written
-// kernels: BF16-operand MFMA GEMMs with FP32 accumulation (v_mfma_f32_32x32x8_bf16), FP32 master
-// weights updated by Adam, BF16 working copies for the GEMMs.  Used as the learned coarse
-// correction of the MAS solver; no counterpart in the reference.
+// Small MLP (d_in -> 128 -> 128 -> d_out, ReLU, optional linear skip layer) trained and evaluated
+// on the GPU with hand-written kernels: BF16-operand MFMA GEMMs with FP32 accumulation
+// (v_mfma_f32_32x32x8_bf16), FP32 master weights updated by Adam, BF16 working copies for the
+// GEMMs.  Used as the learned coarse correction of the MAS solver; no counterpart in the reference.
 #include "mlp.hpp"
 #include "common.hpp"
 #include "device.hpp"
@@ -38,13 +38,13 @@ namespace psmas
 
         constexpr int TM = 64, TN = 64, TK = 32;
 
-        /// C[M x N] = op(A)[M x K] * op(B)[K x N] (+ bias[n]) (ReLU).  Row-major storage:
+        /// C[M x N] = op(A)[M x K] * op(B)[K x N] (+ bias[n]) (ReLU) (+ C if accum).  Row-major storage:
         /// op(A)[m][k] = TA ? A[k*lda + m] : A[m*lda + k]; op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n].
         /// Operands become BF16 in LDS; each wave computes a 32x32 sub-tile with MFMA, FP32 accumulate.
         template <typename AT, bool TA, typename BT, bool TB>
         __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const AT *__restrict__ A, int lda,
                                                       const BT *__restrict__ B, int ldb, float *__restrict__ C,
-                                                      int ldc, const float *__restrict__ bias, int relu)
+                                                      int ldc, const float *__restrict__ bias, int relu, int accum)
         {
             __shared__ unsigned short As[TM][TK + 4];
             __shared__ unsigned short Bs[TK][TN + 4];
@@ -103,6 +103,8 @@ namespace psmas
                         v += bias[n];
                     if (relu)
                         v = v > 0.0f ? v : 0.0f;
+                    if (accum)
+                        v += C[static_cast<size_t>(m) * ldc + n];
                     C[static_cast<size_t>(m) * ldc + n] = v;
                 }
             }
@@ -183,11 +185,11 @@ namespace psmas
 
         template <typename AT, bool TA, typename BT, bool TB>
         void gemm(int M, int N, int K, const AT *A, int lda, const BT *B, int ldb, float *C, int ldc,
-                  const float *bias, bool relu, hipStream_t s)
+                  const float *bias, bool relu, hipStream_t s, bool accum = false)
         {
             const dim3 grid(div_up(N, TN), div_up(M, TM));
             hipLaunchKernelGGL((k_gemm<AT, TA, BT, TB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias,
-                               relu ? 1 : 0);
+                               relu ? 1 : 0, accum ? 1 : 0);
             PSMAS_LAUNCH_CHECK();
         }
 
@@ -195,7 +197,8 @@ namespace psmas
     } // namespace
 
     // ------------------------------------------------------------------------------ Mlp
-    Mlp::Mlp(int d_in, int d_out, int hidden, unsigned seed) : d_in_(d_in), d_out_(d_out), h_(hidden)
+    Mlp::Mlp(int d_in, int d_out, int hidden, unsigned seed, bool skip)
+        : d_in_(d_in), d_out_(d_out), h_(hidden), skip_(skip)
     {
         if (d_in <= 0 || d_out <= 0 || hidden <= 0)
             throw InvalidArgumentError("[MAS] MLP sizes must be positive.");
@@ -205,8 +208,9 @@ namespace psmas
         sizes_[3] = h_;
         sizes_[4] = static_cast<size_t>(d_out_) * h_;
         sizes_[5] = d_out_;
+        sizes_[6] = skip_ ? static_cast<size_t>(d_out_) * d_in_ : 0;
         size_t tot = 0;
-        for (int i = 0; i < 6; ++i)
+        for (int i = 0; i < 7; ++i)
         {
             offs_[i] = tot;
             tot += sizes_[i];
@@ -230,6 +234,7 @@ namespace psmas
         fill(0, d_in_, h_, true);
         fill(2, h_, h_, true);
         fill(4, h_, d_out_, false);
+        // The skip layer starts at zero: the network starts as the plain MLP.
         set_parameters(h.data());
     }
 
@@ -281,6 +286,9 @@ namespace psmas
         gemm<float, false, unsigned short, true>(B, h_, d_in_, X, d_in_, W1, d_in_, h1_.data(), h_, b1, true, s);
         gemm<float, false, unsigned short, true>(B, h_, h_, h1_.data(), h_, W2, h_, h2_.data(), h_, b2, true, s);
         gemm<float, false, unsigned short, true>(B, d_out_, h_, h2_.data(), h_, W3, h_, Y, d_out_, b3, false, s);
+        if (skip_)
+            gemm<float, false, unsigned short, true>(B, d_out_, d_in_, X, d_in_, wb_.data() + offs_[6], d_in_, Y, d_out_,
+                                                     nullptr, false, s, true);
     }
 
     double Mlp::loss_and_grad_device(int B, const float *X, const float *T)
@@ -316,6 +324,9 @@ namespace psmas
         // Layer 1.
         gemm<float, true, float, false>(h_, d_in_, B, dh1_.data(), h_, X, d_in_, g + offs_[0], d_in_, nullptr, false,
                                         stream_);
+        if (skip_) // dS = dY^T X
+            gemm<float, true, float, false>(d_out_, d_in_, B, dy_.data(), d_out_, X, d_in_, g + offs_[6], d_in_, nullptr,
+                                            false, stream_);
         hipLaunchKernelGGL(k_colsum, dim3(div_up(h_, 256)), dim3(256), 0, stream_, B, h_, dh1_.data(), g + offs_[1]);
         PSMAS_LAUNCH_CHECK();
         double loss = 0.0;
@@ -342,7 +353,7 @@ namespace psmas
         PSMAS_HIP(hipMemcpy(host, g_.data(), sizeof(float) * nparams_, hipMemcpyDeviceToHost));
     }
 
-    double Mlp::train(int n, const float *X, const float *T, int epochs, int batch, float lr, unsigned seed)
+    double Mlp::train(int n, const float *X, const float *T, int epochs, int batch, float lr, unsigned seed, bool cosine)
     {
         // X, T device arrays (n samples).  Mini-batches are contiguous windows of a host-shuffled
         // sample order, gathered on the device.
@@ -361,6 +372,8 @@ namespace psmas
         for (int ep = 0; ep < epochs; ++ep)
         {
             std::shuffle(order.begin(), order.end(), rng);
+            const float lr_ep =
+                cosine ? static_cast<float>(lr * 0.5 * (1.0 + std::cos(3.14159265358979323846 * ep / epochs))) : lr;
             double sum = 0.0;
             int nb = 0;
             for (int b0 = 0; b0 + batch <= n; b0 += batch)
@@ -373,7 +386,7 @@ namespace psmas
                                    tb.data());
                 PSMAS_LAUNCH_CHECK();
                 sum += loss_and_grad_device(batch, xb.data(), tb.data());
-                adam_step(lr, 0.0f);
+                adam_step(lr_ep, 0.0f);
                 ++nb;
             }
             last = nb ? sum / nb : 0.0;

@@ -1,8 +1,10 @@
-// GPU MLP used by the learned coarse correction: d_in -> hidden -> hidden -> d_out with ReLU,
-// BF16-operand MFMA GEMMs with FP32 accumulation, FP32 master weights trained by Adam.
+// GPU MLP used by the learned coarse correction: d_in -> hidden -> hidden -> d_out with ReLU and an
+// optional linear skip layer S, BF16-operand MFMA GEMMs with FP32 accumulation, FP32 master weights
+// trained by Adam.
 //
 // Parameter vector layout (FP32): [W1 (hidden x d_in) | b1 | W2 (hidden x hidden) | b2 |
-// W3 (d_out x hidden) | b3], row-major; y = W3 relu(W2 relu(W1 x + b1) + b2) + b3.
+// W3 (d_out x hidden) | b3 | S (d_out x d_in, skip only)], row-major;
+// y = W3 relu(W2 relu(W1 x + b1) + b2) + b3 (+ S x).
 #pragma once
 
 #include "common.hpp"
@@ -16,7 +18,7 @@ namespace psmas
     class Mlp
     {
     public:
-        Mlp(int d_in, int d_out, int hidden = 128, unsigned seed = 1234);
+        Mlp(int d_in, int d_out, int hidden = 128, unsigned seed = 1234, bool skip = false);
         ~Mlp();
         Mlp(const Mlp &) = delete;
         Mlp &operator=(const Mlp &) = delete;
@@ -24,6 +26,7 @@ namespace psmas
         int d_in() const { return d_in_; }
         int d_out() const { return d_out_; }
         int hidden() const { return h_; }
+        bool skip() const { return skip_; }
         size_t num_parameters() const { return nparams_; }
 
         void set_parameters(const float *host);
@@ -36,8 +39,10 @@ namespace psmas
         double loss_and_grad_device(int B, const float *X, const float *T);
         void get_gradients(float *host);
         void adam_step(float lr, float weight_decay);
-        /// Adam training over n device samples; returns the last epoch's mean loss.
-        double train(int n, const float *X, const float *T, int epochs, int batch, float lr, unsigned seed);
+        /// Adam training over n device samples; returns the last epoch's mean loss.  cosine: the
+        /// learning rate decays as lr (1 + cos(pi epoch / epochs)) / 2.
+        double train(int n, const float *X, const float *T, int epochs, int batch, float lr, unsigned seed,
+                     bool cosine = false);
         hipStream_t stream() const { return stream_; }
         const unsigned short *bf16_weights() const { return wb_.data(); }
         const float *fp32_weights() const { return w_.data(); }
@@ -45,7 +50,8 @@ namespace psmas
 
     private:
         int d_in_, d_out_, h_;
-        size_t sizes_[6] = {}, offs_[6] = {};
+        bool skip_ = false;
+        size_t sizes_[7] = {}, offs_[7] = {};
         size_t nparams_ = 0;
         int step_ = 0;
         int cap_ = 0;

@@ -13,7 +13,7 @@ struct psmas_mlp
 {
     psmas::Mlp net;
     std::string last_error;
-    psmas_mlp(int d_in, int d_out, int hidden, unsigned seed) : net(d_in, d_out, hidden, seed) {}
+    psmas_mlp(int d_in, int d_out, int hidden, bool skip, unsigned seed) : net(d_in, d_out, hidden, seed, skip) {}
 };
 
 namespace
@@ -57,12 +57,12 @@ namespace
 
 extern "C"
 {
-    int psmas_mlp_create(int d_in, int d_out, int hidden, unsigned seed, psmas_mlp_handle *out)
+    int psmas_mlp_create(int d_in, int d_out, int hidden, int skip, unsigned seed, psmas_mlp_handle *out)
     {
         if (!out)
             return PSMAS_ERROR_INVALID_ARGUMENT;
         *out = nullptr;
-        return guarded_mlp(g_mlp_error, [&] { *out = new psmas_mlp(d_in, d_out, hidden, seed); });
+        return guarded_mlp(g_mlp_error, [&] { *out = new psmas_mlp(d_in, d_out, hidden, skip != 0, seed); });
     }
 
     int psmas_mlp_destroy(psmas_mlp_handle h)

@@ -270,7 +270,8 @@ namespace psmas
             "block_dim", "max_iter", "relative_tolerance", "absolute_tolerance", "lazy_partitioning",
             "use_preconditioned_residual_norm", "precision", "aggregation", "coarse_solver",
             "reliable_update_delta", "check_interval", "true_residual_period", "use_graphs",
-            "learned_correction", "division_guard"};
+            "learned_correction", "division_guard", "learned_collect", "lattice_points", "learned_hidden",
+            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale"};
     } // namespace
 
     void apply_json_parameters(Params &p, const std::string &json)
@@ -372,6 +373,44 @@ namespace psmas
             if (!(q.division_guard >= 0.0))
                 bad("division_guard", "must be >= 0");
         }
+        if (const JValue *v = obj->get("learned_collect"))
+            q.learned_collect = as_bool(*v, "learned_collect");
+        if (const JValue *v = obj->get("lattice_points"))
+        {
+            q.lattice_points = as_int(*v, "lattice_points");
+            if (q.lattice_points < 2 || q.lattice_points > 16)
+                bad("lattice_points", "must be in [2, 16]");
+        }
+        if (const JValue *v = obj->get("learned_hidden"))
+        {
+            q.learned_hidden = as_int(*v, "learned_hidden");
+            if (q.learned_hidden < 8 || q.learned_hidden > 1024 || q.learned_hidden % 8 != 0)
+                bad("learned_hidden", "must be a multiple of 8 in [8, 1024]");
+        }
+        if (const JValue *v = obj->get("learned_random_pairs"))
+        {
+            q.learned_random_pairs = as_int(*v, "learned_random_pairs");
+            if (q.learned_random_pairs < 0 || q.learned_random_pairs > 100000)
+                bad("learned_random_pairs", "must be in [0, 100000]");
+        }
+        if (const JValue *v = obj->get("learned_snapshots"))
+        {
+            q.learned_snapshots = as_int(*v, "learned_snapshots");
+            if (q.learned_snapshots < 0 || q.learned_snapshots > 1000)
+                bad("learned_snapshots", "must be in [0, 1000]");
+        }
+        if (const JValue *v = obj->get("learned_output_scale"))
+        {
+            q.learned_output_scale = as_double(*v, "learned_output_scale");
+            if (!std::isfinite(q.learned_output_scale))
+                bad("learned_output_scale", "must be finite");
+        }
+        if (const JValue *v = obj->get("learned_stall_segments"))
+        {
+            q.learned_stall_segments = as_int(*v, "learned_stall_segments");
+            if (q.learned_stall_segments < 1 || q.learned_stall_segments > 1000)
+                bad("learned_stall_segments", "must be in [1, 1000]");
+        }
         p = q;
     }
 
@@ -392,7 +431,13 @@ namespace psmas
           << ", \"check_interval\": " << p.check_interval
           << ", \"use_graphs\": " << (p.use_graphs ? "true" : "false")
           << ", \"learned_correction\": " << (p.learned_correction ? "true" : "false")
-          << ", \"division_guard\": " << p.division_guard << "}";
+          << ", \"division_guard\": " << p.division_guard
+          << ", \"learned_collect\": " << (p.learned_collect ? "true" : "false")
+          << ", \"lattice_points\": " << p.lattice_points << ", \"learned_hidden\": " << p.learned_hidden
+          << ", \"learned_random_pairs\": " << p.learned_random_pairs
+          << ", \"learned_snapshots\": " << p.learned_snapshots
+          << ", \"learned_stall_segments\": " << p.learned_stall_segments
+          << ", \"learned_output_scale\": " << p.learned_output_scale << "}";
         return o.str();
     }
 } // namespace psmas

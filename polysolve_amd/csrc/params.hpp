@@ -16,7 +16,17 @@
 //   check_interval       iterations between host convergence checks (reference: 10)
 //   true_residual_period reference: 4 (fp64 path only)
 //   use_graphs           capture the iteration sequence in HIP graphs
-//   learned_correction   use the trained coarse-correction network when one is loaded
+//   learned_correction   mixed path: add the trained lattice coarse correction (flexible PCG;
+//                        plain MAS takes over when the residual stops decreasing)
+//   learned_collect      mixed path: after each converged solve, turn snapshot iterates into
+//                        training pairs for the coarse-correction network
+//   lattice_points       points per axis of the correction's coarse lattice (k; k^3 points)
+//   learned_hidden       hidden width of the correction network (multiple of 8)
+//   learned_random_pairs Gaussian training pairs added per collected solve
+//   learned_snapshots    at most this many snapshot iterates per collected solve
+//   learned_stall_segments  check intervals without a new smallest residual before the learned
+//                        correction is switched off for the rest of the solve
+//   learned_output_scale factor on the learned correction (1 = as trained)
 //   division_guard       alpha/beta are set to 0 when |denominator| < division_guard; 0 (default)
 //                        sets them to 0 only for a non-finite quotient; 1e-20 is the reference's
 //                        absolute rule (MASSolver.cu:43-56)
@@ -63,6 +73,13 @@ namespace psmas
         bool use_graphs = true;
         bool learned_correction = false;
         double division_guard = 0.0;
+        bool learned_collect = false;
+        int lattice_points = 7;
+        int learned_hidden = 128;
+        int learned_random_pairs = 8;
+        int learned_snapshots = 64;
+        int learned_stall_segments = 3;
+        double learned_output_scale = 1.0;
     };
 
     /// Apply a JSON object to params.  Accepts either {"MAS": {...}} (only the "MAS" member is

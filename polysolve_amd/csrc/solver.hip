@@ -153,7 +153,15 @@ namespace psmas
           << ", \"reliable_updates\": " << S.reliable_updates << ", \"relative_residual\": " << S.rel_residual
           << ", \"solve_ms\": " << solve_ms_ << ", \"levels\": " << level_num_ << ", \"parts\": " << S.nparts
           << ", \"systems\": " << sys_.size() << ", \"aggregation\": \"" << S.aggregation << "\""
-          << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\"}";
+          << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\""
+          << ", \"learned\": {\"used\": " << (learned_used_ ? "true" : "false")
+          << ", \"fallback\": " << (learned_fallback_ ? "true" : "false")
+          << ", \"fallback_iteration\": " << learned_fallback_iter_
+          << ", \"trained\": " << (lcc_ && lcc_->trained() ? "true" : "false")
+          << ", \"pairs\": " << (lcc_ ? lcc_->num_pairs() : 0)
+          << ", \"train_loss\": " << (lcc_ ? lcc_->last_loss() : 0.0)
+          << ", \"lattice_points\": " << (lcc_ && lcc_->lattice_ready() ? lcc_->k() : 0)
+          << ", \"note\": \"" << learned_note_ << "\"}}";
         return o.str();
     }
 
@@ -369,6 +377,7 @@ namespace psmas
         d_vals32_.reset();
         d_mats16_.reset();
         d_dscale_.reset();
+        lattice_stale_ = true;
         build_mas();
         allocate_vectors();
         if (params_.precision == Precision::Mixed)
@@ -851,7 +860,8 @@ namespace psmas
         toc();
         apply_dense_coarse<BD, double>(d_mr_.data(), d_mz_.data());
         launch_prolong_dot<BD, double>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr_.data(),
-                                       d_mz_.data(), reductions(), init, sysmap(), stream_);
+                                       d_mz_.data(), reductions(), init, sysmap(), LccApply{}, FlexDot<double>{},
+                                       stream_);
     }
 
     template <int BD>
@@ -981,12 +991,15 @@ namespace psmas
     }
 
     template <int BD>
-    void Solver::mixed_part_b(bool init)
+    void Solver::mixed_part_b(bool init, bool learned)
     {
         const long long nvec = static_cast<long long>(npad_) * BD;
         const int dim = BANK * BD;
         const double L = static_cast<double>(dim) * (dim + 1) / 2;
         launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), stream_);
+        if (learned)
+            lcc_->restrict_to_lattice<BD, float>(d_mr32_.data(), d_active_.data(), stream_,
+                                                 static_cast<float>(params_.learned_output_scale));
         tic("mas_apply_fp16", mat_cnt_[0] * (L * 2.0 + dim * 4.0 + 2.0 * dim * 4.0));
         launch_mas_apply<BD, __half, float>(0, mat_cnt_[0], d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
                                             d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_);
@@ -994,12 +1007,35 @@ namespace psmas
         launch_mas_apply<BD, double, float>(mat_cnt_[0], total_mats_ - mat_cnt_[0], d_mats64_.data(), nullptr,
                                             d_mr32_.data(), d_mz32_.data(), nullptr, nullptr, stream_);
         apply_dense_coarse<BD, float>(d_mr32_.data(), d_mz32_.data());
+        LccApply lv;
+        FlexDot<float> flex;
+        if (learned)
+        {
+            lcc_->infer(d_active_.data(), stream_);
+            lv = lcc_->apply_view();
+            if (!init)
+            {
+                flex.zprev = d_zprev32_.data();
+                flex.rzo = d_rzo_.data();
+            }
+        }
         launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
-                                      d_mz32_.data(), reductions(), init, sysmap(), stream_);
+                                      d_mz32_.data(), reductions(), init, sysmap(), lv, flex, stream_);
         if (init)
+        {
             launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_p32_.data(), stream_);
+            if (learned)
+                launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_zprev32_.data(), stream_);
+        }
+        else if (learned)
+        {
+            launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), reductions(), sysmap(), stream_,
+                                       d_rzo_.data(), d_zprev32_.data());
+        }
         else
+        {
             launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), reductions(), sysmap(), stream_);
+        }
     }
 
     template <int BD>
@@ -1020,14 +1056,53 @@ namespace psmas
             constexpr int BD = decltype(bdc)::value;
             const int K = static_cast<int>(sys_.size());
             const long long nvec = static_cast<long long>(npad_) * BD;
+            // Learned coarse correction: used when requested and a trained network matches the
+            // lattice; snapshots for training pairs when collecting (single systems only).
+            learned_used_ = false;
+            learned_fallback_ = false;
+            learned_fallback_iter_ = -1;
+            learned_note_.clear();
+            bool learned = false;
+            if (params_.learned_correction)
+            {
+                if (!lcc_ || !lcc_->trained())
+                    learned_note_ = "requested but no trained network";
+                else
+                {
+                    ensure_lattice();
+                    if (lcc_->trained())
+                        learned = true;
+                    else
+                        learned_note_ = "network dropped: lattice size changed";
+                }
+            }
+            const bool collect = params_.learned_collect && K == 1 && params_.learned_snapshots > 0;
+            if (params_.learned_collect && K != 1)
+                learned_note_ = "pair collection needs a single system";
+            nsnap_ = 0;
+            if (collect)
+            {
+                ensure_lattice();
+                const size_t need = static_cast<size_t>(params_.learned_snapshots) * nvec;
+                if (d_snap_.size() < need)
+                    d_snap_.allocate(need, "MAS learned snapshots");
+            }
+            if (learned)
+            {
+                if (d_zprev32_.size() < static_cast<size_t>(nvec))
+                    d_zprev32_.allocate(nvec, "MAS flexible z_prev");
+                if (d_rzo_.size() < static_cast<size_t>(nblocks_))
+                    d_rzo_.allocate(nblocks_, "MAS flexible partials");
+                learned_used_ = true;
+            }
             launch_fill<double>(nvec, d_x_.data(), 0.0, stream_);
             launch_fill<float>(nvec, d_x32_.data(), 0.0f, stream_);
             mixed_reliable_residual<BD>(); // every system selected: r = b
-            mixed_part_b<BD>(true);
+            mixed_part_b<BD>(true, learned);
             finalize_scalars();
             read_scalars();
-            std::vector<double> bnorm(K), target(K), ref(K), best(K);
-            std::vector<int> stalls(K, 0);
+            std::vector<double> bnorm(K), target(K), ref(K), best(K), seg_best(K);
+            std::vector<int> stalls(K, 0), seg_stall(K, 0);
             int running = K;
             for (int s = 0; s < K; ++s)
             {
@@ -1039,7 +1114,7 @@ namespace psmas
                                                           std::to_string(s) + ").");
                 bnorm[s] = std::sqrt(*scal(s, S_RR_TRUE));
                 target[s] = std::max(params_.rel_tol * bnorm[s], params_.abs_tol);
-                ref[s] = best[s] = bnorm[s];
+                ref[s] = best[s] = seg_best[s] = bnorm[s];
                 S.residual_norm = bnorm[s];
                 S.rel_residual = bnorm[s] > 0.0 ? 1.0 : 0.0;
                 if (bnorm[s] <= params_.abs_tol)
@@ -1051,27 +1126,65 @@ namespace psmas
             }
             std::fill(h_select_.begin(), h_select_.end(), 0);
             upload_flags();
-            bool first = true;
+            bool first = true, restart_plain = false;
             int k = 0;
             const int ci = params_.check_interval;
             while (k < params_.max_iter && running > 0)
             {
+                if (restart_plain)
+                {
+                    // Restart from the current residual with plain MAS: z = M r, p = z.
+                    mixed_part_b<BD>(true, false);
+                    first = true;
+                    restart_plain = false;
+                }
                 const int count = std::min(ci, params_.max_iter - k);
                 const bool lead_b = !first;
-                run_block(graph_key(1, first ? 1 : 2, count), [&]() {
+                run_block(graph_key(learned ? 2 : 1, first ? 1 : 2, count), [&]() {
                     if (lead_b)
-                        mixed_part_b<BD>(false);
+                        mixed_part_b<BD>(false, learned);
                     for (int i = 0; i < count; ++i)
                     {
                         mixed_part_a<BD>();
                         if (i + 1 < count)
-                            mixed_part_b<BD>(false);
+                            mixed_part_b<BD>(false, learned);
                     }
                     finalize_scalars();
                 });
                 first = false;
                 k += count;
                 read_scalars();
+                if (collect && nsnap_ < params_.learned_snapshots)
+                {
+                    launch_snapshot(nvec, d_x_.data(), d_x32_.data(), d_snap_.data() + static_cast<size_t>(nsnap_) * nvec,
+                                    stream_);
+                    ++nsnap_;
+                }
+                // Plain MAS judges the learned correction: when some system's recurrence residual
+                // has not reached a new minimum for learned_stall_segments check intervals, the
+                // correction is switched off and PCG restarts from the current residual.
+                bool stalled = false;
+                for (int s = 0; s < K; ++s)
+                {
+                    if (!h_active_[s])
+                        continue;
+                    const double rn = std::sqrt(std::max(*scal(s, S_RR), 0.0));
+                    if (rn < seg_best[s])
+                    {
+                        seg_best[s] = rn;
+                        seg_stall[s] = 0;
+                    }
+                    else if (++seg_stall[s] >= params_.learned_stall_segments)
+                        stalled = true;
+                }
+                if (learned && stalled)
+                {
+                    learned = false;
+                    learned_fallback_ = true;
+                    learned_fallback_iter_ = k;
+                    std::fill(seg_stall.begin(), seg_stall.end(), 0);
+                    restart_plain = true; // after any reliable update below has replaced r
+                }
                 int nsel = 0;
                 for (int s = 0; s < K; ++s)
                 {
@@ -1134,6 +1247,9 @@ namespace psmas
             for (int s = 0; s < K; ++s)
                 if (sys_[s].status == Status::Running)
                     sys_[s].status = Status::ReachMaxIterations;
+            if (collect && (sys_[0].status == Status::ReachRelativeTolerance ||
+                            sys_[0].status == Status::ReachAbsoluteTolerance))
+                learned_collect_pairs<BD>();
         };
         if (bd_ == 1)
             body(std::integral_constant<int, 1>());
@@ -1141,6 +1257,97 @@ namespace psmas
             body(std::integral_constant<int, 2>());
         else
             body(std::integral_constant<int, 3>());
+    }
+
+    // ------------------------------------------------------------ learned coarse correction
+    void Solver::ensure_lattice()
+    {
+        if (!lcc_)
+            lcc_ = std::make_unique<LearnedCoarse>();
+        if (!lattice_stale_ && lcc_->lattice_ready() && lcc_->k() == params_.lattice_points)
+            return;
+        std::vector<int> offs, pads;
+        std::vector<const std::vector<double> *> coords;
+        for (const auto &S : sys_)
+        {
+            if (S.coord_dim != 3 || static_cast<int>(S.coords.size()) != 3 * S.nb)
+                throw InvalidArgumentError(
+                    "[MAS] learned correction needs 3-D coordinates for every block node (psmas_set_coordinates).");
+            offs.push_back(S.node_off);
+            pads.push_back(S.npad);
+            coords.push_back(&S.coords);
+        }
+        const int old_width = lcc_->lattice_ready() ? lcc_->width() : -1;
+        lcc_->build_lattice(params_.lattice_points, bd_, offs, pads, coords, block_of_slot_, nonisolated_, stream_);
+        if (old_width >= 0 && old_width != lcc_->width() && lcc_->trained())
+            lcc_->reset_network();
+        lattice_stale_ = false;
+        graphs_->clear(); // captured launches hold the lattice buffers
+    }
+
+    template <int BD>
+    void Solver::learned_pair_from_coef(const std::vector<double> &w)
+    {
+        // e = P w, r = A e (FP64), u = P^T r: an exact pair (u, w) of the lattice Galerkin map.
+        const long long nvec = static_cast<long long>(npad_) * BD;
+        lcc_->prolong_host_coef<BD>(w, d_p_.data(), nvec, stream_);
+        launch_spmv_plain<BD, double, double>(sell(d_vals64_.data()), d_p_.data(), d_Ap_.data(), stream_);
+        lcc_->restrict_to_lattice<BD, double>(d_Ap_.data(), nullptr, stream_);
+        const std::vector<double> u = lcc_->download_u(stream_);
+        double nu = 0.0;
+        for (double v : u)
+            nu += v * v;
+        if (nu > 0.0 && std::isfinite(nu))
+            lcc_->add_pair(u, w);
+    }
+
+    template <int BD>
+    void Solver::learned_collect_pairs()
+    {
+        // Snapshot errors e_i = x* - x_i, projected onto the lattice in the least-squares sense.
+        const long long nvec = static_cast<long long>(npad_) * BD;
+        for (int i = 0; i < nsnap_; ++i)
+        {
+            launch_difference(nvec, d_x_.data(), d_snap_.data() + static_cast<size_t>(i) * nvec, d_Ap_.data(), stream_);
+            lcc_->restrict_to_lattice<BD, double>(d_Ap_.data(), nullptr, stream_);
+            const std::vector<double> v = lcc_->download_u(stream_);
+            double nv = 0.0;
+            for (double x : v)
+                nv += x * x;
+            if (!(nv > 0.0) || !std::isfinite(nv))
+                continue;
+            learned_pair_from_coef<BD>(lcc_->ls_project(v));
+        }
+        std::normal_distribution<double> N01(0.0, 1.0);
+        for (int j = 0; j < params_.learned_random_pairs; ++j)
+        {
+            std::vector<double> w(lcc_->m());
+            for (double &x : w)
+                x = N01(lcc_->rng());
+            learned_pair_from_coef<BD>(w);
+        }
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+    }
+
+    double Solver::learned_train(int epochs, int batch, float lr, unsigned seed)
+    {
+        if (!lcc_)
+            throw std::runtime_error("[MAS] learned correction: no training pairs collected.");
+        if (epochs < 1 || batch < 1 || !(lr > 0.0f))
+            throw InvalidArgumentError("[MAS] learned correction: epochs, batch and lr must be positive.");
+        const double loss = lcc_->train(params_.learned_hidden, epochs, batch, lr, seed);
+        graphs_->clear(); // captured launches hold the network's buffers
+        return loss;
+    }
+
+    void Solver::learned_reset(bool drop_pairs)
+    {
+        if (!lcc_)
+            return;
+        lcc_->reset_network();
+        if (drop_pairs)
+            lcc_->clear_pairs();
+        graphs_->clear();
     }
 
     // --------------------------------------------------------------------- introspection
@@ -1276,6 +1483,54 @@ namespace psmas
             body(std::integral_constant<int, 3>());
         launch_scatter_out<double>(S.nb, bd_, S.n, d_slot_of_block_.data(), d_Ap_.data(), d_io_.data(), stream_);
         PSMAS_HIP(hipMemcpyAsync(y, d_io_.data(), sizeof(double) * S.n, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+    }
+
+    int Solver::lattice_size()
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        check_ready_single(dim());
+        ensure_lattice();
+        return lcc_->m();
+    }
+
+    void Solver::lattice_restrict(const double *v, double *u)
+    {
+        const int m = lattice_size();
+        const System &S = sys_[0];
+        PSMAS_HIP(hipMemcpyAsync(d_io_.data(), v, sizeof(double) * S.n, hipMemcpyHostToDevice, stream_));
+        launch_gather_in<double>(npad_, bd_, S.n, d_block_of_slot_.data(), d_io_.data(), d_p_.data(), stream_);
+        auto body = [&](auto bdc) {
+            constexpr int BD = decltype(bdc)::value;
+            lcc_->restrict_to_lattice<BD, double>(d_p_.data(), nullptr, stream_);
+        };
+        if (bd_ == 1)
+            body(std::integral_constant<int, 1>());
+        else if (bd_ == 2)
+            body(std::integral_constant<int, 2>());
+        else
+            body(std::integral_constant<int, 3>());
+        const std::vector<double> h = lcc_->download_u(stream_);
+        std::copy(h.begin(), h.begin() + m, u);
+    }
+
+    void Solver::lattice_prolong(const double *w, double *e)
+    {
+        const int m = lattice_size();
+        const System &S = sys_[0];
+        const std::vector<double> hw(w, w + m);
+        auto body = [&](auto bdc) {
+            constexpr int BD = decltype(bdc)::value;
+            lcc_->prolong_host_coef<BD>(hw, d_p_.data(), static_cast<long long>(npad_) * BD, stream_);
+        };
+        if (bd_ == 1)
+            body(std::integral_constant<int, 1>());
+        else if (bd_ == 2)
+            body(std::integral_constant<int, 2>());
+        else
+            body(std::integral_constant<int, 3>());
+        launch_scatter_out<double>(S.nb, bd_, S.n, d_slot_of_block_.data(), d_p_.data(), d_io_.data(), stream_);
+        PSMAS_HIP(hipMemcpyAsync(e, d_io_.data(), sizeof(double) * S.n, hipMemcpyDeviceToHost, stream_));
         PSMAS_HIP(hipStreamSynchronize(stream_));
     }
 

@@ -13,6 +13,7 @@
 #include "common.hpp"
 #include "host_setup.hpp"
 #include "kernels.hpp"
+#include "lcc.hpp"
 #include "params.hpp"
 
 #include <functional>
@@ -70,6 +71,17 @@ namespace psmas
         std::vector<double> bank_inverses() const; ///< packed FP64, all levels
         void apply_preconditioner(const double *r, double *z, bool mixed);
         void spmv(const double *x, double *y, bool fp32);
+        /// Learned-correction lattice of system 0: number of coarse unknowns m, u = P^T v and
+        /// e = P w in the input ordering.
+        int lattice_size();
+        void lattice_restrict(const double *v, double *u);
+        void lattice_prolong(const double *w, double *e);
+
+        // Learned coarse correction (mixed path).
+        /// Train the correction network on the pairs collected so far; returns the final loss.
+        double learned_train(int epochs, int batch, float lr, unsigned seed);
+        /// Drop the trained network (and the collected pairs when drop_pairs).
+        void learned_reset(bool drop_pairs);
 
         void set_kernel_timing(bool on) { timing_on_ = on; }
         std::vector<KernelTiming> kernel_timings() const;
@@ -161,6 +173,17 @@ namespace psmas
 
         std::unique_ptr<Graphs> graphs_;
 
+        // learned coarse correction
+        std::unique_ptr<LearnedCoarse> lcc_;
+        bool lattice_stale_ = true;
+        DeviceBuffer<float> d_zprev32_;
+        DeviceBuffer<double> d_rzo_;
+        DeviceBuffer<double> d_snap_;
+        int nsnap_ = 0;
+        bool learned_used_ = false, learned_fallback_ = false;
+        int learned_fallback_iter_ = -1;
+        std::string learned_note_;
+
         bool timing_on_ = false;
         std::map<std::string, KernelTiming> timings_;
         std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending_events_;
@@ -188,6 +211,11 @@ namespace psmas
         void solve_padded();
         void solve_fp64();
         void solve_mixed();
+        void ensure_lattice();
+        template <int BD>
+        void learned_collect_pairs();
+        template <int BD>
+        void learned_pair_from_coef(const std::vector<double> &w);
         void upload_flags();
         double *scal(int s, int slot) { return &h_sc_[static_cast<size_t>(s) * S_COUNT + slot]; }
 
@@ -199,7 +227,7 @@ namespace psmas
         template <int BD>
         void mixed_part_a();
         template <int BD>
-        void mixed_part_b(bool init);
+        void mixed_part_b(bool init, bool learned = false);
         template <int BD>
         void mixed_reliable_residual();
 

@@ -270,14 +270,30 @@ namespace psmas
 
         template <int BD, typename T>
         __global__ __launch_bounds__(VEC_BLOCK) void k_update_p(const T *__restrict__ z, T *__restrict__ p,
-                                                                Reductions red, SysMap sm)
+                                                                Reductions red, SysMap sm,
+                                                                const double *__restrict__ rzo, T *__restrict__ zprev)
         {
             const int sys = block_system(sm);
             if (sm.active[sys] == 0)
                 return;
-            const double rz = system_total<VEC_BLOCK>(red.rz, sys, sm);
+            double rz = system_total<VEC_BLOCK>(red.rz, sys, sm);
             const double rz_old = system_total<VEC_BLOCK>(red.rz_old, sys, sm);
+            if (rzo != nullptr) // flexible (Polak-Ribiere): r_new . (z_new - z_old)
+                rz -= system_total<VEC_BLOCK>(rzo, sys, sm);
             const T beta = static_cast<T>(safe_div(rz, rz_old, red.sc[static_cast<size_t>(sys) * S_COUNT + S_GUARD]));
+            if (zprev != nullptr)
+            {
+                const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
+#pragma unroll
+                for (int k = 0; k < BD; ++k)
+                {
+                    const size_t o = base + threadIdx.x + k * VEC_BLOCK;
+                    const T zo = z[o];
+                    p[o] = fma(beta, p[o], zo);
+                    zprev[o] = zo;
+                }
+                return;
+            }
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
 #pragma unroll
             for (int k = 0; k < BD; ++k)
@@ -354,6 +370,22 @@ namespace psmas
                 sc[static_cast<size_t>(s) * stride + slot] = value;
         }
 
+        __global__ void k_snapshot(long long n, const double *__restrict__ x64, const float *__restrict__ x32,
+                                   double *__restrict__ out)
+        {
+            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+                out[i] = x64[i] + static_cast<double>(x32[i]);
+        }
+
+        __global__ void k_difference(long long n, const double *__restrict__ a, const double *__restrict__ b,
+                                     double *__restrict__ out)
+        {
+            for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+                 i += static_cast<long long>(gridDim.x) * blockDim.x)
+                out[i] = a[i] - b[i];
+        }
+
         /// One workgroup per system: totals of every reduction site -> the system's scalars.
         __global__ __launch_bounds__(VEC_BLOCK) void k_finalize(Reductions red, SysMap sm)
         {
@@ -426,9 +458,10 @@ namespace psmas
     }
 
     template <int BD, typename T>
-    void launch_update_p(int nblocks, const T *z, T *p, const Reductions &red, const SysMap &sm, hipStream_t s)
+    void launch_update_p(int nblocks, const T *z, T *p, const Reductions &red, const SysMap &sm, hipStream_t s,
+                         const double *rzo, T *zprev)
     {
-        hipLaunchKernelGGL((k_update_p<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, z, p, red, sm);
+        hipLaunchKernelGGL((k_update_p<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, z, p, red, sm, rzo, zprev);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -480,6 +513,18 @@ namespace psmas
         PSMAS_LAUNCH_CHECK();
     }
 
+    void launch_snapshot(long long n, const double *x64, const float *x32, double *out, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_snapshot, dim3(stream_grid(n)), dim3(256), 0, s, n, x64, x32, out);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_difference(long long n, const double *a, const double *b, double *out, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_difference, dim3(stream_grid(n)), dim3(256), 0, s, n, a, b, out);
+        PSMAS_LAUNCH_CHECK();
+    }
+
     void launch_finalize(int nsys, const Reductions &red, const SysMap &sm, hipStream_t s)
     {
         hipLaunchKernelGGL(k_finalize, dim3(nsys), dim3(VEC_BLOCK), 0, s, red, sm);
@@ -509,9 +554,9 @@ namespace psmas
     template void launch_axpy_alpha<BD, double>(int, const double *, double *, const Reductions &, const SysMap &, \
                                                 hipStream_t);                                                     \
     template void launch_update_p<BD, double>(int, const double *, double *, const Reductions &, const SysMap &,   \
-                                              hipStream_t);                                                       \
+                                              hipStream_t, const double *, double *);                             \
     template void launch_update_p<BD, float>(int, const float *, float *, const Reductions &, const SysMap &,      \
-                                             hipStream_t);                                                        \
+                                             hipStream_t, const double *, float *);                               \
     template void launch_accumulate_correction<BD>(int, float *, double *, const SysMap &, hipStream_t);
 
     PSMAS_INST_BD(1)

@@ -258,6 +258,36 @@ class MASSolver(Solver):
                self._h)
         return y
 
+    # -- learned coarse correction (mixed path) ------------------------------------------------
+    def learned_train(self, epochs: int = 200, batch: int = 64, lr: float = 3e-3, seed: int = 0) -> float:
+        """Fit the coarse-correction network to the pairs collected by solves run with
+        ``learned_collect``; returns the final mean training loss."""
+        loss = C.c_double()
+        _check(N.lib().psmas_learned_train(self._h, int(epochs), int(batch), float(lr), int(seed), C.byref(loss)),
+               self._h)
+        return float(loss.value)
+
+    def lattice_restrict(self, v) -> np.ndarray:
+        """u = P^T v on the learned correction's lattice (GPU kernels)."""
+        m = C.c_int()
+        _check(N.lib().psmas_lattice_size(self._h, C.byref(m)), self._h)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        u = np.empty(m.value)
+        _check(N.lib().psmas_lattice_restrict(self._h, int(v.size), _ptr(v, C.c_double), _ptr(u, C.c_double)),
+               self._h)
+        return u
+
+    def lattice_prolong(self, w) -> np.ndarray:
+        """e = P w on the learned correction's lattice (GPU kernels)."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        e = np.empty(self._n)
+        _check(N.lib().psmas_lattice_prolong(self._h, int(self._n), _ptr(w, C.c_double), _ptr(e, C.c_double)),
+               self._h)
+        return e
+
+    def learned_reset(self, drop_pairs: bool = False) -> None:
+        _check(N.lib().psmas_learned_reset(self._h, int(bool(drop_pairs))), self._h)
+
     def set_kernel_timing(self, on: bool) -> None:
         _check(N.lib().psmas_set_kernel_timing(self._h, int(on)), self._h)
 
@@ -331,13 +361,13 @@ class Mlp:
     BF16-operand MFMA GEMMs with FP32 accumulation, FP32 master weights, Adam on the GPU.
     """
 
-    def __init__(self, d_in: int, d_out: int, hidden: int = 128, seed: int = 1234) -> None:
+    def __init__(self, d_in: int, d_out: int, hidden: int = 128, seed: int = 1234, skip: bool = False) -> None:
         h = C.c_void_p()
-        rc = N.lib().psmas_mlp_create(int(d_in), int(d_out), int(hidden), int(seed), C.byref(h))
+        rc = N.lib().psmas_mlp_create(int(d_in), int(d_out), int(hidden), int(bool(skip)), int(seed), C.byref(h))
         if rc != N.PSMAS_OK:
             raise MASError(N.lib().psmas_mlp_last_error(None).decode())
         self._h = h
-        self.d_in, self.d_out, self.hidden = int(d_in), int(d_out), int(hidden)
+        self.d_in, self.d_out, self.hidden, self.skip = int(d_in), int(d_out), int(hidden), bool(skip)
 
     def _check(self, rc: int) -> None:
         if rc != N.PSMAS_OK:

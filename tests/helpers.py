@@ -119,3 +119,38 @@ def unpack_upper(v: np.ndarray, N: int) -> np.ndarray:
     M[iu] = v
     M[(iu[1], iu[0])] = v
     return M
+
+
+def lattice_prolongation(A, coords, k, bd=3):
+    """Trilinear lattice prolongation P (n x bd*k^3) of the learned coarse correction, built the
+    way the solver builds it: bounding box of the coordinates, t = (x - lo) / (hi - lo) (k - 1),
+    cell c = clamp(floor(t), 0, k - 2), fraction clamp(t - c, 0, 1) in float32, weights
+    wx*wy*wz in float32; block rows without off-diagonal blocks (Dirichlet rows) are left out."""
+    import scipy.sparse as sp
+
+    X = np.asarray(coords, dtype=np.float64).reshape(-1, 3)
+    nb = X.shape[0]
+    n = A.shape[0]
+    lo, hi = X.min(0), X.max(0)
+    ext = hi - lo
+    t = np.where(ext > 0, (X - lo) / np.where(ext > 0, ext, 1.0) * (k - 1), 0.0)
+    c = np.clip(np.floor(t).astype(np.int64), 0, k - 2)
+    f = np.clip(t - c, 0.0, 1.0).astype(np.float32)
+    Ab = sp.csr_matrix(A)
+    rows = np.repeat(np.arange(n), np.diff(Ab.indptr)) // bd
+    cols = Ab.indices // bd
+    coupled = np.zeros(nb, dtype=bool)
+    coupled[rows[rows != cols]] = True
+    R, Cc, V = [], [], []
+    one = np.float32(1.0)
+    for q in range(8):
+        dx, dy, dz = q & 1, (q >> 1) & 1, (q >> 2) & 1
+        w = ((f[:, 0] if dx else one - f[:, 0]) * (f[:, 1] if dy else one - f[:, 1])).astype(np.float32)
+        w = (w * (f[:, 2] if dz else one - f[:, 2])).astype(np.float32)
+        j = (c[:, 0] + dx) + k * ((c[:, 1] + dy) + k * (c[:, 2] + dz))
+        for d in range(bd):
+            idx = np.nonzero(coupled & (np.arange(nb) * bd + d < n))[0]
+            R.append(idx * bd + d)
+            Cc.append(j[idx] * bd + d)
+            V.append(w[idx].astype(np.float64))
+    return sp.csr_matrix((np.concatenate(V), (np.concatenate(R), np.concatenate(Cc))), shape=(n, bd * k ** 3))

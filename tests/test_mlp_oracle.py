@@ -55,3 +55,22 @@ def test_bf16_rounding():
     z_ref = Xb @ W1.T + b1
     z_bf = mo._mm(Xb, W1.T, True) + b1
     assert np.allclose(z_ref, z_bf, rtol=0, atol=1e-12)
+
+
+def test_skip_layer_gradient_matches_finite_differences():
+    rng = np.random.default_rng(5)
+    d, h = 4, 6
+    n = sum(int(np.prod(q)) for q in mo.shapes(d, d, h, skip=True))
+    theta = rng.standard_normal(n) * 0.5
+    X = rng.standard_normal((5, d))
+    T = rng.standard_normal((5, d))
+    _, g = mo.loss_and_grad(theta, X, T, d, d, h, skip=True)
+    fd = mo.finite_difference(theta, X, T, d, d, h, idx=np.arange(n), eps=1e-6, skip=True)
+    assert np.max(np.abs(fd - g)) <= 1e-7 * max(1.0, np.abs(g).max())
+    # With W3 = 0 and b3 = 0 the network is the skip layer alone.
+    W = mo.unpack(theta.copy(), d, d, h, skip=True)
+    W[4][:] = 0.0
+    W[5][:] = 0.0
+    th2 = np.concatenate([w.ravel() for w in W])
+    y, _ = mo.forward(th2, X, d, d, h, skip=True)
+    assert np.allclose(y, X @ W[6].T)
