@@ -211,7 +211,19 @@ def main() -> int:
         xl, rell = accuracy()
         log(f"  learned {learned_ms:.2f} ms/solve, {infol['solver_iter']} it, true rel res {rell:.2e}, "
             f"learned {infol['learned']}")
+        trainer.set_kernel_timing(True)
+        trainer.solve_device(b_dev.ptr, x_dev.ptr, n)
+        synchronize()
+        ktl = trainer.kernel_timing()
+        trainer.set_kernel_timing(False)
+        learned_kernels = {}
+        for name, rec in ktl.items():
+            if rec["launches"]:
+                avg = rec["total_ms"] / rec["launches"]
+                learned_kernels[name] = {"launches": rec["launches"], "avg_us": round(avg * 1e3, 2),
+                                         "GBps": round(rec["bytes"] / (avg * 1e-3) / 1e9, 1)}
         learned = {"ms": round(learned_ms, 3), "iters": infol["solver_iter"], "true_rel_residual": rell,
+                   "kernels": learned_kernels,
                    "reliable_updates": infol["reliable_updates"], "fallback": infol["learned"]["fallback"],
                    "train_solves": args.train_solves, "pairs": pairs, "collect_s": round(collect_s, 1),
                    "train_s": round(train_s, 1), "train_loss": loss, "lattice_points": args.lattice}

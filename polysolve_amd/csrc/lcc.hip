@@ -317,6 +317,7 @@ namespace psmas
             point_chunk[j + 1] = static_cast<int>(chunk_begin.size()) - 1;
         }
         nchunks_ = static_cast<int>(chunk_begin.size()) - 1;
+        entries_count_ = static_cast<long long>(entries.size());
 
         // (P^T P)^-1 of system 0 (same for every component): the least-squares projection.
         std::vector<double> ptp(static_cast<size_t>(np) * np, 0.0);

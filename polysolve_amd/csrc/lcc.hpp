@@ -38,6 +38,12 @@ namespace psmas
         int m() const { return np_ * bd_; }      ///< coarse unknowns per system
         int width() const { return md_; }        ///< network width (m rounded up to 8)
         int systems() const { return nsys_; }
+        long long restrict_entries() const { return entries_count_; }
+        /// BF16 weight bytes one inference reads (all layers).
+        double inference_weight_bytes() const
+        {
+            return net_ ? 2.0 * static_cast<double>(net_->num_parameters()) : 0.0;
+        }
 
         /// u = P^T v (FP64 sums) for all systems -> d_u_ [nsys][m]; and the normalised network
         /// input and scale for every active system.
@@ -75,6 +81,7 @@ namespace psmas
         DeviceBuffer<int2> d_entries_;    ///< {slot, weight bits}, grouped by lattice point, slot order
         DeviceBuffer<int> d_point_chunk_; ///< chunk offsets per global lattice point (nsys*np + 1)
         int nchunks_ = 0;
+        long long entries_count_ = 0;
         DeviceBuffer<double> d_partial_;  ///< [chunk][bd]
         DeviceBuffer<double> d_u_;        ///< [sys][m]
         DeviceBuffer<float> d_x_;         ///< [sys][md] normalised input

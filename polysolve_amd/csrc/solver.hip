@@ -998,8 +998,13 @@ namespace psmas
         const double L = static_cast<double>(dim) * (dim + 1) / 2;
         launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), stream_);
         if (learned)
+        {
+            // Each coupled slot contributes to 8 lattice points: {slot, weight} entry + r gather.
+            tic("lcc_restrict", static_cast<double>(lcc_->restrict_entries()) * (8.0 + BD * 4.0));
             lcc_->restrict_to_lattice<BD, float>(d_mr32_.data(), d_active_.data(), stream_,
                                                  static_cast<float>(params_.learned_output_scale));
+            toc();
+        }
         tic("mas_apply_fp16", mat_cnt_[0] * (L * 2.0 + dim * 4.0 + 2.0 * dim * 4.0));
         launch_mas_apply<BD, __half, float>(0, mat_cnt_[0], d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
                                             d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_);
@@ -1011,7 +1016,9 @@ namespace psmas
         FlexDot<float> flex;
         if (learned)
         {
+            tic("lcc_infer", static_cast<double>(lcc_->inference_weight_bytes()));
             lcc_->infer(d_active_.data(), stream_);
+            toc();
             lv = lcc_->apply_view();
             if (!init)
             {
@@ -1019,8 +1026,10 @@ namespace psmas
                 flex.rzo = d_rzo_.data();
             }
         }
+        tic(learned ? "prolong_dot_learned" : "prolong_dot", static_cast<double>(n_total_) * 4.0 * 3 + nblocks_ * 256.0 * 16);
         launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
                                       d_mz32_.data(), reductions(), init, sysmap(), lv, flex, stream_);
+        toc();
         if (init)
         {
             launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_p32_.data(), stream_);
