@@ -115,6 +115,14 @@ int psmas_batch_set_coordinates(psmas_handle h, int sys, int nb, int dim, const 
 int psmas_learned_train(psmas_handle h, int epochs, int batch, float lr, unsigned seed, double *loss);
 int psmas_learned_reset(psmas_handle h, int drop_pairs);
 
+/* Autotuner (no counterpart in the reference): on the current factorisation (single system or
+ * batch) and the right-hand sides of the last solve, times the SpMV row-loop batch depth (2, 4, 8;
+ * reps SpMVs each) and, by full solves, the check interval (5, 10, 20) with each coarsest-level
+ * solver; keeps the fastest converged choice in the parameters (spmv_unroll, check_interval,
+ * coarse_solver) and writes a JSON report of every timing.  Re-runs the solves, so call it once
+ * per workload shape with a buffer large enough for the report (4 KB). */
+int psmas_autotune(psmas_handle h, int reps, char *buf, size_t cap, size_t *needed);
+
 /* Number of visible GPUs (0 on a machine without one). */
 int psmas_device_count(void);
 

@@ -49,6 +49,7 @@ SIGNATURES = {
     "psmas_batch_solve": (_i, [_vp, _i, C.POINTER(_pd), C.POINTER(_pd), _i]),
     "psmas_batch_get_info": (_i, [_vp, _i, C.c_char_p, _sz, C.POINTER(_sz)]),
     "psmas_batch_set_coordinates": (_i, [_vp, _i, _i, _i, _pd]),
+    "psmas_autotune": (_i, [_vp, _i, C.c_char_p, _sz, C.POINTER(_sz)]),
     "psmas_lattice_size": (_i, [_vp, _pi]),
     "psmas_lattice_restrict": (_i, [_vp, _i, _pd, _pd]),
     "psmas_lattice_prolong": (_i, [_vp, _i, _pd, _pd]),

@@ -361,6 +361,17 @@ extern "C"
         });
     }
 
+    int psmas_autotune(psmas_handle h, int reps, char *buf, size_t cap, size_t *needed)
+    {
+        if (!h)
+            return null_handle();
+        std::string report;
+        const int rc = guarded(h->last_error, [&] { report = h->solver.autotune(reps); });
+        if (rc != PSMAS_OK)
+            return rc;
+        return write_string(report, buf, cap, needed);
+    }
+
     int psmas_lattice_size(psmas_handle h, int *m)
     {
         if (!h || !m)

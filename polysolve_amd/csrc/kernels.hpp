@@ -23,6 +23,7 @@ namespace psmas
         const int *slice_ptr = nullptr;
         const int *cols = nullptr;
         const TV *vals = nullptr;
+        int unroll = 4; ///< slots per batch in the SpMV row loop: 2, 4 or 8 (autotuned)
     };
 
     /// Restriction of fine (bank-slot) values to coarse level 0: slot p that leads its bank-local

@@ -14,6 +14,7 @@ namespace psmas
     {
         constexpr int LCC_CHUNK = 1024; ///< restriction entries per workgroup
         constexpr int GEMV_SYS = 8;     ///< systems per wave in the inference GEMV
+        constexpr int LCC_TMAX = 64;    ///< distinct lattice points one node block may touch
 
         __device__ __forceinline__ float wave_sum_f(float v)
         {
@@ -61,10 +62,93 @@ namespace psmas
             }
         }
 
-        /// One workgroup per system: u = sum of the chunk partials per lattice point (in order),
+        template <typename T>
+        __device__ __forceinline__ T wave_sum_t(T v)
+        {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+                v += __shfl_xor(v, off, WAVE);
+            return v;
+        }
+
+        /// One workgroup per 256-node block: partial[b][t][d] = sum over the block's coupled slots
+        /// of w_q * v[slot][d] for the corners q whose block-local lattice point is t (t < nt[b]).
+        /// Each wave reduces with shuffles; the four waves' sums are added in wave order.
+        template <int BD, typename T>
+        __global__ __launch_bounds__(256) void k_lcc_restrict_blocks(const int4 *__restrict__ cell,
+                                                                     const uint2 *__restrict__ local,
+                                                                     const int *__restrict__ block_nt,
+                                                                     const T *__restrict__ v,
+                                                                     double *__restrict__ partial)
+        {
+            __shared__ T wpart[4][LCC_TMAX][BD];
+            const int b = blockIdx.x;
+            const int nt = block_nt[b];
+            if (nt == 0)
+                return;
+            const int p = b * 256 + threadIdx.x;
+            const int4 c = cell[p];
+            float w[8];
+            int loc[8];
+            T r[BD];
+            if (c.x >= 0)
+            {
+                const float fx = __int_as_float(c.y), fy = __int_as_float(c.z), fz = __int_as_float(c.w);
+                const uint2 L = local[p];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                {
+                    w[q] = trilinear_weight(q, fx, fy, fz);
+                    loc[q] = static_cast<int>(((q < 4 ? L.x : L.y) >> (8 * (q & 3))) & 0xffu);
+                }
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                    r[d] = v[static_cast<size_t>(p) * BD + d];
+            }
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                {
+                    w[q] = 0.0f;
+                    loc[q] = -1;
+                }
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                    r[d] = T(0);
+            }
+            const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+            for (int t = 0; t < nt; ++t)
+            {
+                float a = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    a += (loc[q] == t) ? w[q] : 0.0f;
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                {
+                    const T s = wave_sum_t<T>(static_cast<T>(a) * r[d]);
+                    if (lane == 0)
+                        wpart[wave][t][d] = s;
+                }
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < nt * BD; i += 256)
+            {
+                const int t = i / BD, d = i % BD;
+                double s = 0.0;
+#pragma unroll
+                for (int wv = 0; wv < 4; ++wv)
+                    s += static_cast<double>(wpart[wv][t][d]);
+                partial[(static_cast<size_t>(b) * LCC_TMAX + t) * BD + d] = s;
+            }
+        }
+
+        /// One workgroup per system: u = sum of each lattice point's partials (in list order),
         /// x = u sqrt(m) / |u| (zero-padded to md), scale = |u| a / sqrt(m).
         template <int BD>
-        __global__ __launch_bounds__(256) void k_lcc_finish(int np, int md, const int *__restrict__ point_chunk,
+        __global__ __launch_bounds__(256) void k_lcc_finish(int np, int md, const int *__restrict__ point_ptr,
+                                                            const int *__restrict__ point_src,
                                                             const double *__restrict__ partial, double *__restrict__ u,
                                                             float *__restrict__ x, float *__restrict__ scale,
                                                             float ascale, const int *__restrict__ active)
@@ -82,10 +166,13 @@ namespace psmas
 #pragma unroll
                 for (int d = 0; d < BD; ++d)
                     a[d] = 0.0;
-                for (int c = point_chunk[J]; c < point_chunk[J + 1]; ++c)
+                for (int i = point_ptr[J]; i < point_ptr[J + 1]; ++i)
+                {
+                    const size_t c = static_cast<size_t>(point_src[i]);
 #pragma unroll
                     for (int d = 0; d < BD; ++d)
-                        a[d] += partial[static_cast<size_t>(c) * BD + d];
+                        a[d] += partial[c * BD + d];
+                }
 #pragma unroll
                 for (int d = 0; d < BD; ++d)
                 {
@@ -184,6 +271,88 @@ namespace psmas
                 if (scale)
                     y *= scale[s];
                 S.Y[static_cast<size_t>(s) * S.ldy + row] = y;
+            }
+        }
+
+        constexpr int LCC_HMAX = 1024; ///< largest hidden width the fused output kernel holds in LDS
+
+        /// Hidden layer 2 and the output layer in one launch.  Workgroup (row chunk, system group):
+        /// the 256 threads first compute h2 = relu(W2 h1 + b2) for the group's systems into LDS (every
+        /// workgroup of the group computes the same values), then each wave produces one output
+        /// row: coef[s][row] = (W3[row] . h2[s] + b3[row] + skip[s][row]) * scale[s].
+        __global__ __launch_bounds__(256) void k_lcc_out(const unsigned short *__restrict__ W2,
+                                                         const float *__restrict__ b2, int h,
+                                                         const float *__restrict__ H1,
+                                                         const unsigned short *__restrict__ W3,
+                                                         const float *__restrict__ b3, int rows,
+                                                         const float *__restrict__ skip, int ld_skip,
+                                                         const float *__restrict__ scale, float *__restrict__ coef,
+                                                         int ldc, int nsys, const int *__restrict__ active)
+        {
+            __shared__ float h2s[GEMV_SYS][LCC_HMAX];
+            const int ngroups = (nsys + GEMV_SYS - 1) / GEMV_SYS;
+            const int g = blockIdx.x % ngroups, chunk = blockIdx.x / ngroups;
+            const int s0 = g * GEMV_SYS;
+            const int ns = min(GEMV_SYS, nsys - s0);
+            for (int idx = threadIdx.x; idx < h * ns; idx += 256)
+            {
+                const int r = idx % h, j = idx / h;
+                const uint4 *wrow = reinterpret_cast<const uint4 *>(W2 + static_cast<size_t>(r) * h);
+                const float *x = H1 + static_cast<size_t>(s0 + j) * h;
+                float a = 0.0f;
+                for (int c = 0; c < h / 8; ++c)
+                {
+                    const uint4 wv = wrow[c];
+                    const float4 x0 = reinterpret_cast<const float4 *>(x)[2 * c];
+                    const float4 x1 = reinterpret_cast<const float4 *>(x)[2 * c + 1];
+                    a += bf16_lo(wv.x) * x0.x + bf16_hi(wv.x) * x0.y + bf16_lo(wv.y) * x0.z + bf16_hi(wv.y) * x0.w +
+                         bf16_lo(wv.z) * x1.x + bf16_hi(wv.z) * x1.y + bf16_lo(wv.w) * x1.z + bf16_hi(wv.w) * x1.w;
+                }
+                a += b2[r];
+                h2s[j][r] = a > 0.0f ? a : 0.0f;
+            }
+            __syncthreads();
+            const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+            const int row = chunk * 4 + wave;
+            if (row >= rows)
+                return;
+            float acc[GEMV_SYS];
+#pragma unroll
+            for (int j = 0; j < GEMV_SYS; ++j)
+                acc[j] = 0.0f;
+            const uint4 *wrow = reinterpret_cast<const uint4 *>(W3 + static_cast<size_t>(row) * h);
+            for (int c = lane; c < h / 8; c += WAVE)
+            {
+                const uint4 wv = wrow[c];
+                const float w[8] = {bf16_lo(wv.x), bf16_hi(wv.x), bf16_lo(wv.y), bf16_hi(wv.y),
+                                    bf16_lo(wv.z), bf16_hi(wv.z), bf16_lo(wv.w), bf16_hi(wv.w)};
+#pragma unroll
+                for (int j = 0; j < GEMV_SYS; ++j)
+                {
+                    if (j < ns)
+                    {
+                        const float *x = &h2s[j][c * 8];
+#pragma unroll
+                        for (int e = 0; e < 8; ++e)
+                            acc[j] += w[e] * x[e];
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < GEMV_SYS; ++j)
+                acc[j] = wave_sum_f(acc[j]);
+            if (lane != 0)
+                return;
+#pragma unroll
+            for (int j = 0; j < GEMV_SYS; ++j)
+            {
+                if (j >= ns)
+                    break;
+                const int s = s0 + j;
+                if (active != nullptr && active[s] == 0)
+                    continue;
+                const float y = acc[j] + b3[row] + skip[static_cast<size_t>(s) * ld_skip + row];
+                coef[static_cast<size_t>(s) * ldc + row] = y * scale[s];
             }
         }
 
@@ -319,6 +488,73 @@ namespace psmas
         nchunks_ = static_cast<int>(chunk_begin.size()) - 1;
         entries_count_ = static_cast<long long>(entries.size());
 
+        // Block path: each 256-node block's distinct lattice points (sorted) get local ids; a slot
+        // stores its 8 corners' local ids as bytes.  Used when no block touches more than
+        // LCC_TMAX points; otherwise the chunked entry list above is used.
+        const int nblocks = total_pad / 256;
+        std::vector<uint2> local(total_pad, uint2{0u, 0u});
+        std::vector<int> block_nt(nblocks, 0);
+        std::vector<std::vector<int>> point_lists(npts);
+        bool blocks_ok = true;
+        for (int b = 0; b < nblocks && blocks_ok; ++b)
+        {
+            std::vector<int> pts;
+            for (int p = b * 256; p < (b + 1) * 256; ++p)
+                if (cell[p].x >= 0)
+                    for (int q = 0; q < 8; ++q)
+                    {
+                        int id;
+                        float w;
+                        corners(cell[p], q, id, w);
+                        pts.push_back(id);
+                    }
+            std::sort(pts.begin(), pts.end());
+            pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+            if (static_cast<int>(pts.size()) > LCC_TMAX)
+            {
+                blocks_ok = false;
+                break;
+            }
+            block_nt[b] = static_cast<int>(pts.size());
+            for (int t = 0; t < block_nt[b]; ++t)
+                point_lists[pts[t]].push_back(b * LCC_TMAX + t);
+            for (int p = b * 256; p < (b + 1) * 256; ++p)
+                if (cell[p].x >= 0)
+                {
+                    unsigned lo = 0, hi = 0;
+                    for (int q = 0; q < 8; ++q)
+                    {
+                        int id;
+                        float w;
+                        corners(cell[p], q, id, w);
+                        const unsigned t = static_cast<unsigned>(std::lower_bound(pts.begin(), pts.end(), id) - pts.begin());
+                        if (q < 4)
+                            lo |= t << (8 * q);
+                        else
+                            hi |= t << (8 * (q - 4));
+                    }
+                    local[p] = uint2{lo, hi};
+                }
+        }
+        std::vector<int> point_ptr(npts + 1, 0), point_src;
+        if (blocks_ok)
+        {
+            for (int j = 0; j < npts; ++j)
+            {
+                point_src.insert(point_src.end(), point_lists[j].begin(), point_lists[j].end());
+                point_ptr[j + 1] = static_cast<int>(point_src.size());
+            }
+        }
+        else
+        {
+            point_ptr = point_chunk;
+            point_src.resize(nchunks_);
+            for (int c = 0; c < nchunks_; ++c)
+                point_src[c] = c;
+        }
+        use_blocks_ = blocks_ok;
+        nblocks_ = nblocks;
+
         // (P^T P)^-1 of system 0 (same for every component): the least-squares projection.
         std::vector<double> ptp(static_cast<size_t>(np) * np, 0.0);
         for (int p = node_off[0]; p < node_off[0] + npad[0]; ++p)
@@ -349,13 +585,30 @@ namespace psmas
         d_cell_.allocate(cell.size(), "MAS lattice cells");
         d_entries_.allocate(std::max<size_t>(entries.size(), 1), "MAS lattice restriction");
         d_chunk_begin_.allocate(chunk_begin.size(), "MAS lattice chunks");
-        d_point_chunk_.allocate(point_chunk.size(), "MAS lattice chunks");
+        d_point_ptr_.allocate(point_ptr.size(), "MAS lattice point lists");
+        d_point_src_.allocate(std::max<size_t>(point_src.size(), 1), "MAS lattice point lists");
         PSMAS_HIP(hipMemcpy(d_cell_.data(), cell.data(), d_cell_.bytes(), hipMemcpyHostToDevice));
         if (!entries.empty())
             PSMAS_HIP(hipMemcpy(d_entries_.data(), entries.data(), sizeof(int2) * entries.size(), hipMemcpyHostToDevice));
         PSMAS_HIP(hipMemcpy(d_chunk_begin_.data(), chunk_begin.data(), d_chunk_begin_.bytes(), hipMemcpyHostToDevice));
-        PSMAS_HIP(hipMemcpy(d_point_chunk_.data(), point_chunk.data(), d_point_chunk_.bytes(), hipMemcpyHostToDevice));
-        d_partial_.allocate(static_cast<size_t>(std::max(nchunks_, 1)) * bd, "MAS lattice partials");
+        PSMAS_HIP(hipMemcpy(d_point_ptr_.data(), point_ptr.data(), d_point_ptr_.bytes(), hipMemcpyHostToDevice));
+        if (!point_src.empty())
+            PSMAS_HIP(hipMemcpy(d_point_src_.data(), point_src.data(), sizeof(int) * point_src.size(),
+                                hipMemcpyHostToDevice));
+        if (use_blocks_)
+        {
+            d_local_.allocate(local.size(), "MAS lattice local ids");
+            d_block_nt_.allocate(block_nt.size(), "MAS lattice block targets");
+            PSMAS_HIP(hipMemcpy(d_local_.data(), local.data(), d_local_.bytes(), hipMemcpyHostToDevice));
+            PSMAS_HIP(hipMemcpy(d_block_nt_.data(), block_nt.data(), d_block_nt_.bytes(), hipMemcpyHostToDevice));
+            d_partial_.allocate(static_cast<size_t>(nblocks) * LCC_TMAX * bd, "MAS lattice partials");
+        }
+        else
+        {
+            d_local_.reset();
+            d_block_nt_.reset();
+            d_partial_.allocate(static_cast<size_t>(std::max(nchunks_, 1)) * bd, "MAS lattice partials");
+        }
         d_u_.allocate(static_cast<size_t>(nsys) * m(), "MAS lattice residual");
         d_x_.allocate(static_cast<size_t>(nsys) * md_, "MAS network input");
         d_scale_.allocate(nsys, "MAS network scale");
@@ -370,15 +623,21 @@ namespace psmas
     template <int BD, typename T>
     void LearnedCoarse::restrict_to_lattice(const T *v, const int *active, hipStream_t s, float output_factor)
     {
-        if (nchunks_ > 0)
+        if (use_blocks_)
+        {
+            hipLaunchKernelGGL((k_lcc_restrict_blocks<BD, T>), dim3(nblocks_), dim3(256), 0, s, d_cell_.data(),
+                               d_local_.data(), d_block_nt_.data(), v, d_partial_.data());
+            PSMAS_LAUNCH_CHECK();
+        }
+        else if (nchunks_ > 0)
         {
             hipLaunchKernelGGL((k_lcc_restrict_partial<BD, T>), dim3(nchunks_), dim3(256), 0, s, d_chunk_begin_.data(),
                                d_entries_.data(), v, d_partial_.data());
             PSMAS_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL((k_lcc_finish<BD>), dim3(nsys_), dim3(256), 0, s, np_, md_, d_point_chunk_.data(),
-                           d_partial_.data(), d_u_.data(), d_x_.data(), d_scale_.data(), ascale_ * output_factor,
-                           active);
+        hipLaunchKernelGGL((k_lcc_finish<BD>), dim3(nsys_), dim3(256), 0, s, np_, md_, d_point_ptr_.data(),
+                           d_point_src_.data(), d_partial_.data(), d_u_.data(), d_x_.data(), d_scale_.data(),
+                           ascale_ * output_factor, active);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -389,6 +648,8 @@ namespace psmas
         if (net_->d_in() != md_ || net_->d_out() != md_)
             throw std::runtime_error("[MAS] learned correction: network width does not match the lattice.");
         const int h = net_->hidden();
+        if (h > LCC_HMAX || h % 8 != 0)
+            throw std::runtime_error("[MAS] learned correction: hidden width must be a multiple of 8 up to 1024.");
         if (!d_h1_)
         {
             d_h1_.allocate(static_cast<size_t>(nsys_) * h, "MAS network activations");
@@ -405,15 +666,12 @@ namespace psmas
         hipLaunchKernelGGL(k_lcc_gemv, grid(h + md_), dim3(256), 0, s, l1, sk, md_, d_x_.data(), md_, nsys_,
                            static_cast<const float *>(nullptr), 0, static_cast<const float *>(nullptr), active);
         PSMAS_LAUNCH_CHECK();
-        GemvSeg l2{wb + net_->offset(2), wf + net_->offset(3), h, 1, d_h2_.data(), h};
-        hipLaunchKernelGGL(k_lcc_gemv, grid(h), dim3(256), 0, s, l2, GemvSeg{}, h, d_h1_.data(), h, nsys_,
-                           static_cast<const float *>(nullptr), 0, static_cast<const float *>(nullptr), active);
-        PSMAS_LAUNCH_CHECK();
-        // Output layer: only the m real outputs, plus the skip output, times the scale.
-        GemvSeg l3{wb + net_->offset(4), wf + net_->offset(5), m(), 0, d_coef_.data(), m()};
-        hipLaunchKernelGGL(k_lcc_gemv, grid(m()), dim3(256), 0, s, l3, GemvSeg{}, h, d_h2_.data(), h, nsys_,
-                           static_cast<const float *>(d_skip_.data()), md_, static_cast<const float *>(d_scale_.data()),
-                           active);
+        // Layer 2 fused into the output layer (only the m real outputs), plus the skip output,
+        // times the scale.
+        hipLaunchKernelGGL(k_lcc_out, dim3(div_up(m(), 4) * ngroups), dim3(256), 0, s, wb + net_->offset(2),
+                           wf + net_->offset(3), h, static_cast<const float *>(d_h1_.data()), wb + net_->offset(4),
+                           wf + net_->offset(5), m(), static_cast<const float *>(d_skip_.data()), md_,
+                           static_cast<const float *>(d_scale_.data()), d_coef_.data(), m(), nsys_, active);
         PSMAS_LAUNCH_CHECK();
     }
 

@@ -39,6 +39,7 @@ namespace psmas
         int width() const { return md_; }        ///< network width (m rounded up to 8)
         int systems() const { return nsys_; }
         long long restrict_entries() const { return entries_count_; }
+        bool block_restriction() const { return use_blocks_; }
         /// BF16 weight bytes one inference reads (all layers).
         double inference_weight_bytes() const
         {
@@ -79,7 +80,12 @@ namespace psmas
         DeviceBuffer<int4> d_cell_;
         DeviceBuffer<int> d_chunk_begin_; ///< entry offsets of restriction chunks (nchunks + 1)
         DeviceBuffer<int2> d_entries_;    ///< {slot, weight bits}, grouped by lattice point, slot order
-        DeviceBuffer<int> d_point_chunk_; ///< chunk offsets per global lattice point (nsys*np + 1)
+        DeviceBuffer<int> d_point_ptr_;   ///< per global lattice point: range of point_src (nsys*np + 1)
+        DeviceBuffer<int> d_point_src_;   ///< partial indices summed into each lattice point, in order
+        DeviceBuffer<uint2> d_local_;     ///< block path: 8 corner local ids (bytes) per slot
+        DeviceBuffer<int> d_block_nt_;    ///< block path: distinct lattice points per node block
+        bool use_blocks_ = false;
+        int nblocks_ = 0;
         int nchunks_ = 0;
         long long entries_count_ = 0;
         DeviceBuffer<double> d_partial_;  ///< [chunk][bd]

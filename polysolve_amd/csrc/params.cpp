@@ -271,7 +271,7 @@ namespace psmas
             "use_preconditioned_residual_norm", "precision", "aggregation", "coarse_solver",
             "reliable_update_delta", "check_interval", "true_residual_period", "use_graphs",
             "learned_correction", "division_guard", "learned_collect", "lattice_points", "learned_hidden",
-            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale"};
+            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll"};
     } // namespace
 
     void apply_json_parameters(Params &p, const std::string &json)
@@ -399,6 +399,12 @@ namespace psmas
             if (q.learned_snapshots < 0 || q.learned_snapshots > 1000)
                 bad("learned_snapshots", "must be in [0, 1000]");
         }
+        if (const JValue *v = obj->get("spmv_unroll"))
+        {
+            q.spmv_unroll = as_int(*v, "spmv_unroll");
+            if (q.spmv_unroll != 2 && q.spmv_unroll != 4 && q.spmv_unroll != 8)
+                bad("spmv_unroll", "must be 2, 4 or 8");
+        }
         if (const JValue *v = obj->get("learned_output_scale"))
         {
             q.learned_output_scale = as_double(*v, "learned_output_scale");
@@ -437,7 +443,8 @@ namespace psmas
           << ", \"learned_random_pairs\": " << p.learned_random_pairs
           << ", \"learned_snapshots\": " << p.learned_snapshots
           << ", \"learned_stall_segments\": " << p.learned_stall_segments
-          << ", \"learned_output_scale\": " << p.learned_output_scale << "}";
+          << ", \"learned_output_scale\": " << p.learned_output_scale << ", \"spmv_unroll\": " << p.spmv_unroll
+          << "}";
         return o.str();
     }
 } // namespace psmas

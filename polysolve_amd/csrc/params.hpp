@@ -27,6 +27,7 @@
 //   learned_stall_segments  check intervals without a new smallest residual before the learned
 //                        correction is switched off for the rest of the solve
 //   learned_output_scale factor on the learned correction (1 = as trained)
+//   spmv_unroll          SpMV row-loop batch depth 2 | 4 | 8 (chosen by psmas_autotune)
 //   division_guard       alpha/beta are set to 0 when |denominator| < division_guard; 0 (default)
 //                        sets them to 0 only for a non-finite quotient; 1e-20 is the reference's
 //                        absolute rule (MASSolver.cu:43-56)
@@ -80,6 +81,7 @@ namespace psmas
         int learned_snapshots = 64;
         int learned_stall_segments = 3;
         double learned_output_scale = 1.0;
+        int spmv_unroll = 4;
     };
 
     /// Apply a JSON object to params.  Accepts either {"MAS": {...}} (only the "MAS" member is

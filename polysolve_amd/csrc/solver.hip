@@ -136,8 +136,105 @@ namespace psmas
         apply_json_parameters(params_, json);
         if (before.use_graphs != params_.use_graphs || before.check_interval != params_.check_interval ||
             before.true_residual_period != params_.true_residual_period || before.precision != params_.precision ||
-            before.coarse_solver != params_.coarse_solver)
+            before.coarse_solver != params_.coarse_solver || before.spmv_unroll != params_.spmv_unroll)
             graphs_->clear();
+    }
+
+    std::string Solver::autotune(int reps)
+    {
+        // Tile and schedule choices, timed on this factorisation and the last right-hand side:
+        // (1) the SpMV row-loop batch depth, by timing `reps` SpMVs per candidate with HIP events;
+        // (2) the check interval (graph segment length) and the coarsest-level solver, by timing
+        // full solves.  The fastest converged candidate is kept in the parameters.
+        PSMAS_HIP(hipSetDevice(device_));
+        if (!factorized_ || !rhs_ready_)
+            throw std::runtime_error("[MAS] autotune needs a factorisation and one solve first.");
+        reps = std::max(reps, 1);
+        std::ostringstream o;
+        o.precision(6);
+        const bool mixed = params_.precision == Precision::Mixed;
+        if (mixed)
+            ensure_mixed_data();
+        hipEvent_t e0, e1;
+        PSMAS_HIP(hipEventCreate(&e0));
+        PSMAS_HIP(hipEventCreate(&e1));
+        const long long nvec = static_cast<long long>(npad_) * bd_;
+        int best_u = params_.spmv_unroll;
+        float best_t = 1e30f;
+        o << "{\"spmv_us\": {";
+        const int cands[3] = {2, 4, 8};
+        for (int ci = 0; ci < 3; ++ci)
+        {
+            params_.spmv_unroll = cands[ci];
+            auto body = [&](auto bdc) {
+                constexpr int BD = decltype(bdc)::value;
+                if (mixed)
+                    launch_spmv_plain<BD, float, float>(sell(d_vals32_.data()), d_p32_.data(), d_Ap32_.data(), stream_);
+                else
+                    launch_spmv_plain<BD, double, double>(sell(d_vals64_.data()), d_p_.data(), d_Ap_.data(), stream_);
+            };
+            auto run = [&]() {
+                if (bd_ == 1)
+                    body(std::integral_constant<int, 1>());
+                else if (bd_ == 2)
+                    body(std::integral_constant<int, 2>());
+                else
+                    body(std::integral_constant<int, 3>());
+            };
+            run(); // warm
+            PSMAS_HIP(hipEventRecord(e0, stream_));
+            for (int r = 0; r < reps; ++r)
+                run();
+            PSMAS_HIP(hipEventRecord(e1, stream_));
+            PSMAS_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            PSMAS_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const float us = ms * 1e3f / reps;
+            o << (ci ? ", " : "") << "\"" << cands[ci] << "\": " << us;
+            if (us < best_t)
+            {
+                best_t = us;
+                best_u = cands[ci];
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)nvec;
+        params_.spmv_unroll = best_u;
+        o << "}, \"spmv_unroll\": " << best_u << ", \"solve_ms\": {";
+        const Params base = params_;
+        const int intervals[3] = {5, 10, 20};
+        const CoarseSolver coarse[2] = {CoarseSolver::Bank, CoarseSolver::Cholesky};
+        double best_ms = 1e300;
+        Params best = base;
+        bool firstc = true;
+        for (CoarseSolver cs : coarse)
+            for (int iv : intervals)
+            {
+                params_ = base;
+                params_.check_interval = iv;
+                params_.coarse_solver = cs;
+                graphs_->clear();
+                solve_padded(); // captures this variant's graphs
+                solve_padded();
+                bool ok = true;
+                for (const auto &S : sys_)
+                    ok = ok && (S.status == Status::ReachRelativeTolerance || S.status == Status::ReachAbsoluteTolerance);
+                o << (firstc ? "" : ", ") << "\"" << (cs == CoarseSolver::Bank ? "bank" : "cholesky") << "/" << iv
+                  << "\": " << (ok ? solve_ms_ : -1.0);
+                firstc = false;
+                if (ok && solve_ms_ < best_ms)
+                {
+                    best_ms = solve_ms_;
+                    best = params_;
+                }
+            }
+        params_ = best;
+        graphs_->clear();
+        o << "}, \"check_interval\": " << params_.check_interval << ", \"coarse_solver\": \""
+          << (params_.coarse_solver == CoarseSolver::Bank ? "bank" : "cholesky") << "\", \"best_solve_ms\": "
+          << best_ms << "}";
+        return o.str();
     }
 
     std::string Solver::system_info_json(int s) const
@@ -161,6 +258,7 @@ namespace psmas
           << ", \"pairs\": " << (lcc_ ? lcc_->num_pairs() : 0)
           << ", \"train_loss\": " << (lcc_ ? lcc_->last_loss() : 0.0)
           << ", \"lattice_points\": " << (lcc_ && lcc_->lattice_ready() ? lcc_->k() : 0)
+          << ", \"block_restriction\": " << (lcc_ && lcc_->block_restriction() ? "true" : "false")
           << ", \"note\": \"" << learned_note_ << "\"}}";
         return o.str();
     }
@@ -241,6 +339,7 @@ namespace psmas
         v.slice_ptr = d_slice_ptr_.data();
         v.cols = d_cols_.data();
         v.vals = vals;
+        v.unroll = params_.spmv_unroll;
         return v;
     }
 
@@ -378,6 +477,7 @@ namespace psmas
         d_mats16_.reset();
         d_dscale_.reset();
         lattice_stale_ = true;
+        rhs_ready_ = false;
         build_mas();
         allocate_vectors();
         if (params_.precision == Precision::Mixed)
@@ -770,6 +870,7 @@ namespace psmas
             launch_gather_in<double>(S.npad, bd_, S.n, d_block_of_slot_.data() + S.node_off, src,
                                      d_b_.data() + static_cast<size_t>(S.node_off) * bd_, stream_);
         }
+        rhs_ready_ = true;
         solve_padded();
         for (size_t s = 0; s < sys_.size(); ++s)
         {
@@ -999,8 +1100,11 @@ namespace psmas
         launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), stream_);
         if (learned)
         {
-            // Each coupled slot contributes to 8 lattice points: {slot, weight} entry + r gather.
-            tic("lcc_restrict", static_cast<double>(lcc_->restrict_entries()) * (8.0 + BD * 4.0));
+            // Block path: per slot the lattice cell (16 B), 8 local ids (8 B) and r; entry path: per
+            // coupled slot 8 {slot, weight} entries (8 B each) and 8 gathers of r.
+            tic("lcc_restrict", lcc_->block_restriction()
+                                    ? static_cast<double>(npad_) * (24.0 + BD * 4.0)
+                                    : static_cast<double>(lcc_->restrict_entries()) * (8.0 + BD * 4.0));
             lcc_->restrict_to_lattice<BD, float>(d_mr32_.data(), d_active_.data(), stream_,
                                                  static_cast<float>(params_.learned_output_scale));
             toc();

@@ -83,6 +83,10 @@ namespace psmas
         /// Drop the trained network (and the collected pairs when drop_pairs).
         void learned_reset(bool drop_pairs);
 
+        /// Time the tile / schedule candidates on this factorisation and the last right-hand side,
+        /// keep the fastest in the parameters, return a JSON report.
+        std::string autotune(int reps);
+
         void set_kernel_timing(bool on) { timing_on_ = on; }
         std::vector<KernelTiming> kernel_timings() const;
         void reset_kernel_timings();
@@ -124,6 +128,7 @@ namespace psmas
         int bd_ = 0, npad_ = 0, nparts_ = 0, nblocks_ = 0;
         long long nnzb_ = 0, nnz_scalar_ = 0, n_total_ = 0, nb_total_ = 0;
         bool factorized_ = false;
+        bool rhs_ready_ = false; ///< d_b_ holds the last solve's right-hand sides
         std::vector<int> block_of_slot_; ///< global slot -> local block (-1 virtual)
 
         // SELL matrix

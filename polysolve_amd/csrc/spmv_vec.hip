@@ -36,11 +36,10 @@ namespace psmas
         /// unconditionally (slot indices past the slice end are clamped to its last slot, whose
         /// lines are in cache, and their values are zeroed), which lets the compiler count the
         /// loads in flight and keep the column prefetch outstanding across the FMAs.
-        template <int BD, typename TV, typename TX, typename TA>
+        template <int BD, int U, typename TV, typename TX, typename TA>
         __device__ __forceinline__ void sell_row(const SellView<TV> &A, int slice, int lane, const TX *__restrict__ x,
                                                  TA acc[BD])
         {
-            constexpr int U = 4;
             const int beg = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice]);
             const int end = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice + 1]);
             if (end <= beg)
@@ -100,7 +99,7 @@ namespace psmas
 
         __device__ __forceinline__ int block_system(const SysMap &sm) { return sm.block_sys[blockIdx.x]; }
 
-        template <int BD, typename TV, typename TX>
+        template <int BD, int U, typename TV, typename TX>
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap(SellView<TV> A, const TX *__restrict__ x,
                                                                 TX *__restrict__ y, Reductions red, SysMap sm)
         {
@@ -115,7 +114,7 @@ namespace psmas
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD>(A, slice, lane, x, acc);
+            sell_row<BD, U>(A, slice, lane, x, acc);
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
             double dot = 0.0;
 #pragma unroll
@@ -152,7 +151,7 @@ namespace psmas
                 yc0[static_cast<size_t>(gid) * BD + i] = static_cast<T>(s[i]);
         }
 
-        template <int BD, typename TV, typename TX, typename TY>
+        template <int BD, int U, typename TV, typename TX, typename TY>
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_res(SellView<TV> A, const TX *__restrict__ x,
                                                                  const TX *__restrict__ b, TY *__restrict__ y,
                                                                  FineRestrict fr, TY *__restrict__ yc0,
@@ -172,7 +171,7 @@ namespace psmas
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD>(A, slice, lane, x, acc);
+            sell_row<BD, U>(A, slice, lane, x, acc);
             double dot = 0.0;
 #pragma unroll
             for (int i = 0; i < BD; ++i)
@@ -190,7 +189,7 @@ namespace psmas
             store_partial<SPMV_BLOCK>(dot, rr_partials, blk);
         }
 
-        template <int BD, typename TV, typename TX>
+        template <int BD, int U, typename TV, typename TX>
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_plain(SellView<TV> A, const TX *__restrict__ x,
                                                                    TX *__restrict__ y)
         {
@@ -203,7 +202,7 @@ namespace psmas
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD>(A, slice, lane, x, acc);
+            sell_row<BD, U>(A, slice, lane, x, acc);
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
 #pragma unroll
             for (int i = 0; i < BD; ++i)
@@ -415,12 +414,38 @@ namespace psmas
         }
     } // namespace
 
+    // The row loop's batch depth is a template parameter; the launchers pick the instance from
+    // SellView::unroll (2, 4 or 8).
+#define PSMAS_UNROLL_DISPATCH(U_, CALL)                                                                             \
+    switch (U_)                                                                                                    \
+    {                                                                                                              \
+    case 2:                                                                                                        \
+    {                                                                                                              \
+        constexpr int U = 2;                                                                                       \
+        CALL;                                                                                                      \
+        break;                                                                                                     \
+    }                                                                                                              \
+    case 8:                                                                                                        \
+    {                                                                                                              \
+        constexpr int U = 8;                                                                                       \
+        CALL;                                                                                                      \
+        break;                                                                                                     \
+    }                                                                                                              \
+    default:                                                                                                       \
+    {                                                                                                              \
+        constexpr int U = 4;                                                                                       \
+        CALL;                                                                                                      \
+        break;                                                                                                     \
+    }                                                                                                              \
+    }
+
     template <int BD, typename TV, typename TX>
     void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, const Reductions &red, const SysMap &sm,
                         hipStream_t s)
     {
         const int grid = A.nslices / (SPMV_BLOCK / WAVE);
-        hipLaunchKernelGGL((k_spmv_ap<BD, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, y, red, sm);
+        PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_ap<BD, U, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0,
+                                                           s, A, x, y, red, sm));
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -429,8 +454,9 @@ namespace psmas
                          double *rr_partials, const SysMap &sm, bool selected_only, hipStream_t s)
     {
         const int grid = A.nslices / (SPMV_BLOCK / WAVE);
-        hipLaunchKernelGGL((k_spmv_res<BD, TV, TX, TY>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, b, y, fr, yc0,
-                           rr_partials, sm, selected_only ? 1 : 0);
+        PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_res<BD, U, TV, TX, TY>), dim3(grid),
+                                                           dim3(SPMV_BLOCK), 0, s, A, x, b, y, fr, yc0, rr_partials,
+                                                           sm, selected_only ? 1 : 0));
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -438,9 +464,11 @@ namespace psmas
     void launch_spmv_plain(const SellView<TV> &A, const TX *x, TX *y, hipStream_t s)
     {
         const int grid = div_up(A.nslices, SPMV_BLOCK / WAVE);
-        hipLaunchKernelGGL((k_spmv_plain<BD, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0, s, A, x, y);
+        PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_plain<BD, U, TV, TX>), dim3(grid), dim3(SPMV_BLOCK),
+                                                           0, s, A, x, y));
         PSMAS_LAUNCH_CHECK();
     }
+#undef PSMAS_UNROLL_DISPATCH
 
     template <int BD, typename T>
     void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0,

@@ -267,6 +267,15 @@ class MASSolver(Solver):
                self._h)
         return float(loss.value)
 
+    def autotune(self, reps: int = 20) -> dict:
+        """Time the SpMV batch depth, check interval and coarsest solver on this factorisation and
+        the last right-hand side(s); keeps the fastest; returns the timing report."""
+        cap = 65536
+        buf = C.create_string_buffer(cap)
+        needed = C.c_size_t(0)
+        _check(N.lib().psmas_autotune(self._h, int(reps), buf, cap, C.byref(needed)), self._h)
+        return _json.loads(buf.value.decode())
+
     def lattice_restrict(self, v) -> np.ndarray:
         """u = P^T v on the learned correction's lattice (GPU kernels)."""
         m = C.c_int()

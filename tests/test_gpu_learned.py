@@ -23,12 +23,15 @@ def _solver(params):
     return s
 
 
-def test_lattice_restriction_and_prolongation_match_numpy(gpu):
+@pytest.mark.parametrize("k,block_path", [(4, True), (10, False)])
+def test_lattice_restriction_and_prolongation_match_numpy(gpu, k, block_path):
+    """k = 4: every node block touches few lattice points (per-block reduction); k = 10 on an 8^3
+    mesh: blocks touch more than 64 points and the per-point entry lists are used."""
     p = problems.hex_elasticity(8, E=problems.random_moduli(8, 11))
-    s = _solver({"lattice_points": 4})
+    s = _solver({"lattice_points": k})
     s.set_coordinates(p.coords)
     s.factorize(p.A)
-    P = helpers.lattice_prolongation(p.A, p.coords, 4)
+    P = helpers.lattice_prolongation(p.A, p.coords, k)
     rng = np.random.default_rng(0)
     v = rng.standard_normal(p.n)
     u = s.lattice_restrict(v)
@@ -40,6 +43,7 @@ def test_lattice_restriction_and_prolongation_match_numpy(gpu):
     assert np.linalg.norm(e - P @ w) <= 1e-6 * np.linalg.norm(P @ w)
     # Dirichlet rows receive no correction.
     assert np.all(e[P.getnnz(axis=1) == 0] == 0.0)
+    assert s.get_info()["learned"]["block_restriction"] == block_path
 
 
 def test_mlp_skip_layer_gradients_match_oracle(gpu):
