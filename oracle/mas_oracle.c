@@ -251,6 +251,8 @@ static int build_coarse_space(orc_mas *M)
     const int nb = M->nb, npad = M->npad;
     const int banks = npad / ORC_BANK;
     M->map = (int *)malloc(sizeof(int) * (size_t)nb * ORC_MAX_COARSE);
+    for (size_t i = 0; i < (size_t)nb * ORC_MAX_COARSE; ++i)
+        M->map[i] = -1; /* levels past the last one */
     int *local = (int *)malloc(sizeof(int) * (size_t)npad);
     int *count = (int *)calloc((size_t)banks + 1, sizeof(int));
     uint32_t *nbr = (uint32_t *)calloc((size_t)npad, sizeof(uint32_t));
