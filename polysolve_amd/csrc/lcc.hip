@@ -555,27 +555,8 @@ namespace psmas
         use_blocks_ = blocks_ok;
         nblocks_ = nblocks;
 
-        // (P^T P)^-1 of system 0 (same for every component): the least-squares projection.
-        std::vector<double> ptp(static_cast<size_t>(np) * np, 0.0);
-        for (int p = node_off[0]; p < node_off[0] + npad[0]; ++p)
-        {
-            if (cell[p].x < 0)
-                continue;
-            int id[8];
-            float w[8];
-            for (int q = 0; q < 8; ++q)
-                corners(cell[p], q, id[q], w[q]);
-            for (int a = 0; a < 8; ++a)
-                for (int c = 0; c < 8; ++c)
-                    ptp[static_cast<size_t>(id[a]) * np + id[c]] += static_cast<double>(w[a]) * w[c];
-        }
-        for (int j = 0; j < np; ++j)
-            if (!(ptp[static_cast<size_t>(j) * np + j] > 0.0))
-                ptp[static_cast<size_t>(j) * np + j] = 1.0; // lattice point without slots
-        if (!cholesky_inverse(np, ptp))
-            throw std::runtime_error("[MAS] learned correction: lattice mass matrix is singular.");
-        ptp_inv_ = std::move(ptp);
         h_cell0_.assign(cell.begin() + node_off[0], cell.begin() + node_off[0] + npad[0]);
+        ptp_inv_.clear(); // built on first use (pair collection only)
 
         k_ = k;
         bd_ = bd;
@@ -706,8 +687,49 @@ namespace psmas
         return u;
     }
 
-    std::vector<double> LearnedCoarse::ls_project(const std::vector<double> &v) const
+    void LearnedCoarse::ensure_ptp()
     {
+        if (!ptp_inv_.empty())
+            return;
+        // (P^T P)^-1 of system 0 (the same for every component), with a Tikhonov shift of 1e-8
+        // times the mean diagonal so that lattice points with little or no support (a lattice
+        // finer than the mesh) keep the projection well posed.
+        const int np = np_, k = k_;
+        std::vector<double> ptp(static_cast<size_t>(np) * np, 0.0);
+        for (const int4 &c : h_cell0_)
+        {
+            if (c.x < 0)
+                continue;
+            float f[3];
+            std::memcpy(&f[0], &c.y, 4);
+            std::memcpy(&f[1], &c.z, 4);
+            std::memcpy(&f[2], &c.w, 4);
+            int id[8];
+            double w[8];
+            for (int q = 0; q < 8; ++q)
+            {
+                id[q] = c.x + (q & 1) + ((q >> 1) & 1) * k + ((q >> 2) & 1) * k * k;
+                w[q] = static_cast<double>(((q & 1) ? f[0] : 1.0f - f[0]) * ((q & 2) ? f[1] : 1.0f - f[1]) *
+                                           ((q & 4) ? f[2] : 1.0f - f[2]));
+            }
+            for (int a = 0; a < 8; ++a)
+                for (int b = 0; b < 8; ++b)
+                    ptp[static_cast<size_t>(id[a]) * np + id[b]] += w[a] * w[b];
+        }
+        double tr = 0.0;
+        for (int j = 0; j < np; ++j)
+            tr += ptp[static_cast<size_t>(j) * np + j];
+        const double shift = 1e-8 * (tr > 0.0 ? tr / np : 1.0);
+        for (int j = 0; j < np; ++j)
+            ptp[static_cast<size_t>(j) * np + j] += shift;
+        if (!cholesky_inverse(np, ptp))
+            throw std::runtime_error("[MAS] learned correction: lattice mass matrix is singular.");
+        ptp_inv_ = std::move(ptp);
+    }
+
+    std::vector<double> LearnedCoarse::ls_project(const std::vector<double> &v)
+    {
+        ensure_ptp();
         std::vector<double> w(v.size(), 0.0);
         for (int i = 0; i < np_; ++i)
             for (int j = 0; j < np_; ++j)

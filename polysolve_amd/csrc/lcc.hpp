@@ -61,7 +61,7 @@ namespace psmas
         /// Host copy of u of system 0 (after restrict_to_lattice).
         std::vector<double> download_u(hipStream_t s) const;
         /// w = (P^T P)^-1 v for system 0 (least-squares lattice projection).
-        std::vector<double> ls_project(const std::vector<double> &v) const;
+        std::vector<double> ls_project(const std::vector<double> &v);
 
         // network
         bool trained() const { return trained_; }
@@ -93,7 +93,8 @@ namespace psmas
         DeviceBuffer<float> d_x_;         ///< [sys][md] normalised input
         DeviceBuffer<float> d_scale_;     ///< [sys]
         DeviceBuffer<float> d_h1_, d_h2_, d_skip_, d_coef_;
-        std::vector<double> ptp_inv_;     ///< (P^T P)^-1 of system 0, np x np
+        std::vector<double> ptp_inv_;     ///< (P^T P)^-1 of system 0, np x np (on first use)
+        void ensure_ptp();
         std::vector<int4> h_cell0_;       ///< host copy, system 0
 
         // network and training data
