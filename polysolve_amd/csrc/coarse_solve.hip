@@ -40,6 +40,25 @@ namespace psmas
             const int slice = i / WAVE, lane = i % WAVE;
             const int beg = A.slice_ptr[slice];
             const int len = row_len[i];
+            // Consecutive blocks on the same coarsest column are merged before the atomics.
+            int cur = -1;
+            double acc[BD * BD];
+#pragma unroll
+            for (int e = 0; e < BD * BD; ++e)
+                acc[e] = 0.0;
+            auto flush = [&]() {
+                if (cur < 0)
+                    return;
+#pragma unroll
+                for (int bi = 0; bi < BD; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < BD; ++bj)
+                    {
+                        if (acc[bi * BD + bj] != 0.0)
+                            atomicAdd(dense + static_cast<size_t>(BD * a + bi) * nd + BD * cur + bj, acc[bi * BD + bj]);
+                        acc[bi * BD + bj] = 0.0;
+                    }
+            };
             for (int k = 0; k < len; ++k)
             {
                 const size_t slot = static_cast<size_t>(beg + k);
@@ -47,11 +66,16 @@ namespace psmas
                 const int b = active_index[cmap_level(cmap[j], level)];
                 if (b < 0)
                     continue;
-                for (int bi = 0; bi < BD; ++bi)
-                    for (int bj = 0; bj < BD; ++bj)
-                        atomicAdd(dense + static_cast<size_t>(BD * a + bi) * nd + BD * b + bj,
-                                  A.vals[(slot * BD * BD + bi * BD + bj) * WAVE + lane]);
+                if (b != cur)
+                {
+                    flush();
+                    cur = b;
+                }
+#pragma unroll
+                for (int e = 0; e < BD * BD; ++e)
+                    acc[e] += A.vals[(slot * BD * BD + e) * WAVE + lane];
             }
+            flush();
         }
 
         /// y[active] = Inv x[active] on the coarsest segment: one wave per dense row, the

@@ -62,84 +62,50 @@ namespace psmas
             }
         }
 
-        template <typename T>
-        __device__ __forceinline__ T wave_sum_t(T v)
-        {
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1)
-                v += __shfl_xor(v, off, WAVE);
-            return v;
-        }
-
         /// One workgroup per 256-node block: partial[b][t][d] = sum over the block's coupled slots
         /// of w_q * v[slot][d] for the corners q whose block-local lattice point is t (t < nt[b]).
-        /// Each wave reduces with shuffles; the four waves' sums are added in wave order.
+        /// Every thread stages its 8 weighted contributions in LDS; thread (t, d) then adds the
+        /// entries of target t's list (thread << 3 | corner, built on the host) in list order.
         template <int BD, typename T>
         __global__ __launch_bounds__(256) void k_lcc_restrict_blocks(const int4 *__restrict__ cell,
-                                                                     const uint2 *__restrict__ local,
                                                                      const int *__restrict__ block_nt,
+                                                                     const int *__restrict__ tbase,
+                                                                     const int *__restrict__ tptr,
+                                                                     const unsigned short *__restrict__ tentry,
                                                                      const T *__restrict__ v,
                                                                      double *__restrict__ partial)
         {
-            __shared__ T wpart[4][LCC_TMAX][BD];
+            __shared__ T vals[256 * 8 * BD];
             const int b = blockIdx.x;
             const int nt = block_nt[b];
             if (nt == 0)
                 return;
             const int p = b * 256 + threadIdx.x;
             const int4 c = cell[p];
-            float w[8];
-            int loc[8];
-            T r[BD];
             if (c.x >= 0)
             {
                 const float fx = __int_as_float(c.y), fy = __int_as_float(c.z), fz = __int_as_float(c.w);
-                const uint2 L = local[p];
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                {
-                    w[q] = trilinear_weight(q, fx, fy, fz);
-                    loc[q] = static_cast<int>(((q < 4 ? L.x : L.y) >> (8 * (q & 3))) & 0xffu);
-                }
+                T r[BD];
 #pragma unroll
                 for (int d = 0; d < BD; ++d)
                     r[d] = v[static_cast<size_t>(p) * BD + d];
-            }
-            else
-            {
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
                 {
-                    w[q] = 0.0f;
-                    loc[q] = -1;
-                }
+                    const T w = static_cast<T>(trilinear_weight(q, fx, fy, fz));
 #pragma unroll
-                for (int d = 0; d < BD; ++d)
-                    r[d] = T(0);
-            }
-            const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-            for (int t = 0; t < nt; ++t)
-            {
-                float a = 0.0f;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    a += (loc[q] == t) ? w[q] : 0.0f;
-#pragma unroll
-                for (int d = 0; d < BD; ++d)
-                {
-                    const T s = wave_sum_t<T>(static_cast<T>(a) * r[d]);
-                    if (lane == 0)
-                        wpart[wave][t][d] = s;
+                    for (int d = 0; d < BD; ++d)
+                        vals[(threadIdx.x * 8 + q) * BD + d] = w * r[d];
                 }
             }
             __syncthreads();
+            const int base = tbase[b];
             for (int i = threadIdx.x; i < nt * BD; i += 256)
             {
                 const int t = i / BD, d = i % BD;
                 double s = 0.0;
-#pragma unroll
-                for (int wv = 0; wv < 4; ++wv)
-                    s += static_cast<double>(wpart[wv][t][d]);
+                for (int e = tptr[base + t]; e < tptr[base + t + 1]; ++e)
+                    s += static_cast<double>(vals[static_cast<int>(tentry[e]) * BD + d]);
                 partial[(static_cast<size_t>(b) * LCC_TMAX + t) * BD + d] = s;
             }
         }
@@ -492,8 +458,8 @@ namespace psmas
         // stores its 8 corners' local ids as bytes.  Used when no block touches more than
         // LCC_TMAX points; otherwise the chunked entry list above is used.
         const int nblocks = total_pad / 256;
-        std::vector<uint2> local(total_pad, uint2{0u, 0u});
-        std::vector<int> block_nt(nblocks, 0);
+        std::vector<int> block_nt(nblocks, 0), tbase(nblocks + 1, 0), tptr{0};
+        std::vector<unsigned short> tentry;
         std::vector<std::vector<int>> point_lists(npts);
         bool blocks_ok = true;
         for (int b = 0; b < nblocks && blocks_ok; ++b)
@@ -518,23 +484,24 @@ namespace psmas
             block_nt[b] = static_cast<int>(pts.size());
             for (int t = 0; t < block_nt[b]; ++t)
                 point_lists[pts[t]].push_back(b * LCC_TMAX + t);
+            // Per target: the (thread, corner) pairs that feed it, in thread-then-corner order.
+            std::vector<std::vector<unsigned short>> lists(block_nt[b]);
             for (int p = b * 256; p < (b + 1) * 256; ++p)
                 if (cell[p].x >= 0)
-                {
-                    unsigned lo = 0, hi = 0;
                     for (int q = 0; q < 8; ++q)
                     {
                         int id;
                         float w;
                         corners(cell[p], q, id, w);
-                        const unsigned t = static_cast<unsigned>(std::lower_bound(pts.begin(), pts.end(), id) - pts.begin());
-                        if (q < 4)
-                            lo |= t << (8 * q);
-                        else
-                            hi |= t << (8 * (q - 4));
+                        const int t = static_cast<int>(std::lower_bound(pts.begin(), pts.end(), id) - pts.begin());
+                        lists[t].push_back(static_cast<unsigned short>(((p - b * 256) << 3) | q));
                     }
-                    local[p] = uint2{lo, hi};
-                }
+            tbase[b] = static_cast<int>(tptr.size()) - 1;
+            for (const auto &l : lists)
+            {
+                tentry.insert(tentry.end(), l.begin(), l.end());
+                tptr.push_back(static_cast<int>(tentry.size()));
+            }
         }
         std::vector<int> point_ptr(npts + 1, 0), point_src;
         if (blocks_ok)
@@ -578,16 +545,24 @@ namespace psmas
                                 hipMemcpyHostToDevice));
         if (use_blocks_)
         {
-            d_local_.allocate(local.size(), "MAS lattice local ids");
             d_block_nt_.allocate(block_nt.size(), "MAS lattice block targets");
-            PSMAS_HIP(hipMemcpy(d_local_.data(), local.data(), d_local_.bytes(), hipMemcpyHostToDevice));
+            d_tbase_.allocate(tbase.size(), "MAS lattice block lists");
+            d_tptr_.allocate(tptr.size(), "MAS lattice block lists");
+            d_tentry_.allocate(std::max<size_t>(tentry.size(), 1), "MAS lattice block lists");
             PSMAS_HIP(hipMemcpy(d_block_nt_.data(), block_nt.data(), d_block_nt_.bytes(), hipMemcpyHostToDevice));
+            PSMAS_HIP(hipMemcpy(d_tbase_.data(), tbase.data(), d_tbase_.bytes(), hipMemcpyHostToDevice));
+            PSMAS_HIP(hipMemcpy(d_tptr_.data(), tptr.data(), d_tptr_.bytes(), hipMemcpyHostToDevice));
+            if (!tentry.empty())
+                PSMAS_HIP(hipMemcpy(d_tentry_.data(), tentry.data(), sizeof(unsigned short) * tentry.size(),
+                                    hipMemcpyHostToDevice));
             d_partial_.allocate(static_cast<size_t>(nblocks) * LCC_TMAX * bd, "MAS lattice partials");
         }
         else
         {
-            d_local_.reset();
             d_block_nt_.reset();
+            d_tbase_.reset();
+            d_tptr_.reset();
+            d_tentry_.reset();
             d_partial_.allocate(static_cast<size_t>(std::max(nchunks_, 1)) * bd, "MAS lattice partials");
         }
         d_u_.allocate(static_cast<size_t>(nsys) * m(), "MAS lattice residual");
@@ -607,7 +582,8 @@ namespace psmas
         if (use_blocks_)
         {
             hipLaunchKernelGGL((k_lcc_restrict_blocks<BD, T>), dim3(nblocks_), dim3(256), 0, s, d_cell_.data(),
-                               d_local_.data(), d_block_nt_.data(), v, d_partial_.data());
+                               d_block_nt_.data(), d_tbase_.data(), d_tptr_.data(), d_tentry_.data(), v,
+                               d_partial_.data());
             PSMAS_LAUNCH_CHECK();
         }
         else if (nchunks_ > 0)

@@ -82,8 +82,10 @@ namespace psmas
         DeviceBuffer<int2> d_entries_;    ///< {slot, weight bits}, grouped by lattice point, slot order
         DeviceBuffer<int> d_point_ptr_;   ///< per global lattice point: range of point_src (nsys*np + 1)
         DeviceBuffer<int> d_point_src_;   ///< partial indices summed into each lattice point, in order
-        DeviceBuffer<uint2> d_local_;     ///< block path: 8 corner local ids (bytes) per slot
         DeviceBuffer<int> d_block_nt_;    ///< block path: distinct lattice points per node block
+        DeviceBuffer<int> d_tbase_;       ///< block path: first (block, target) list of each block
+        DeviceBuffer<int> d_tptr_;        ///< block path: entry offsets of every (block, target) list
+        DeviceBuffer<unsigned short> d_tentry_; ///< block path: thread << 3 | corner
         bool use_blocks_ = false;
         int nblocks_ = 0;
         int nchunks_ = 0;

@@ -388,66 +388,112 @@ namespace psmas
         }
 
         /// One thread per fine row i; blocks (i, j) with i <= j only (A symmetric), as the reference
-        /// fill (MASPreconditioner.cu:494-591).  Fine-level entries have exactly one writer; coarse
-        /// entries are summed with FP64 atomics.
-        __global__ void k_galerkin(int npad, int bd, int level_num, const int *__restrict__ mat_off,
-                                   SellView<double> A, const int *__restrict__ row_len,
-                                   const int *__restrict__ block_of_slot, const int4 *__restrict__ cmap,
-                                   double *__restrict__ mats)
+        /// fill (MASPreconditioner.cu:494-591).  Fine-level entries have exactly one writer.  Coarse
+        /// entries are summed with FP64 atomics, but each row first merges consecutive blocks that
+        /// land on the same coarse column of a level (Morton-ordered rows give long runs), so an
+        /// atomic is issued per run and entry instead of per block and entry.
+        constexpr int GAL_LEVELS = 4; ///< coarse levels of the MAS hierarchy (MASPreconditioner.hpp:70)
+
+        template <int BD>
+        __global__ void k_galerkin(int npad, int level_num, const int *__restrict__ mat_off, SellView<double> A,
+                                   const int *__restrict__ row_len, const int *__restrict__ block_of_slot,
+                                   const int4 *__restrict__ cmap, double *__restrict__ mats)
         {
+            constexpr int BS = BD * BD;
             const int i = blockIdx.x * blockDim.x + threadIdx.x;
             if (i >= npad || block_of_slot[i] < 0)
                 return;
-            const int dim = 32 * bd;
-            const size_t st = static_cast<size_t>(dim) * (dim + 1) / 2;
-            const int bs = bd * bd;
+            constexpr int dim = 32 * BD;
+            constexpr size_t st = static_cast<size_t>(dim) * (dim + 1) / 2;
             const int slice = i / WAVE, lane = i % WAVE;
             const int beg = A.slice_ptr[slice];
             const int len = row_len[i];
             const int4 ci4 = cmap[i];
+            int cur[GAL_LEVELS];
+            double acc[GAL_LEVELS][BS];
+#pragma unroll
+            for (int lv = 0; lv < GAL_LEVELS; ++lv)
+            {
+                cur[lv] = -1;
+#pragma unroll
+                for (int e = 0; e < BS; ++e)
+                    acc[lv][e] = 0.0;
+            }
+            auto flush = [&](int lv) {
+                const int ci = cmap_get(ci4, lv), cj = cur[lv];
+                if (cj < 0)
+                    return;
+                double *mat = mats + static_cast<size_t>(mat_off[lv + 1] + (ci >> 5)) * st;
+                const int ri = (ci & 31) * BD, rj = (cj & 31) * BD;
+#pragma unroll
+                for (int bi = 0; bi < BD; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < BD; ++bj)
+                    {
+                        const int row = ri + bi, col = rj + bj;
+                        const double a = acc[lv][bi * BD + bj];
+                        if (row <= col && a != 0.0)
+                            atomicAdd(mat + upper_index(dim, row, col), a);
+                        acc[lv][bi * BD + bj] = 0.0;
+                    }
+            };
             for (int k = 0; k < len; ++k)
             {
                 const size_t slot = static_cast<size_t>(beg + k);
                 const int j = A.cols[slot * WAVE + lane];
                 if (i > j)
                     continue;
-                double v[9];
-                for (int e = 0; e < bs; ++e)
-                    v[e] = A.vals[(slot * bs + e) * WAVE + lane];
+                double v[BS];
+#pragma unroll
+                for (int e = 0; e < BS; ++e)
+                    v[e] = A.vals[(slot * BS + e) * WAVE + lane];
                 if ((i >> 5) == (j >> 5))
                 {
                     double *mat = mats + static_cast<size_t>(i >> 5) * st;
-                    const int ri = (i & 31) * bd, rj = (j & 31) * bd;
-                    for (int bi = 0; bi < bd; ++bi)
-                        for (int bj = 0; bj < bd; ++bj)
+                    const int ri = (i & 31) * BD, rj = (j & 31) * BD;
+#pragma unroll
+                    for (int bi = 0; bi < BD; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < BD; ++bj)
                         {
                             const int row = ri + bi, col = rj + bj;
                             if (row > col || (i == j && bi > bj))
                                 continue;
-                            mat[upper_index(dim, row, col)] = v[bi * bd + bj];
+                            mat[upper_index(dim, row, col)] = v[bi * BD + bj];
                         }
                 }
                 const int4 cj4 = cmap[j];
-                for (int lv = 0; lv < level_num; ++lv)
+#pragma unroll
+                for (int lv = 0; lv < GAL_LEVELS; ++lv)
                 {
+                    if (lv >= level_num)
+                        break;
                     const int ci = cmap_get(ci4, lv), cj = cmap_get(cj4, lv);
                     if ((ci >> 5) != (cj >> 5))
                         continue;
-                    double *mat = mats + static_cast<size_t>(mat_off[lv + 1] + (ci >> 5)) * st;
-                    const int ri = (ci & 31) * bd, rj = (cj & 31) * bd;
-                    for (int bi = 0; bi < bd; ++bi)
-                        for (int bj = 0; bj < bd; ++bj)
+                    if (cj != cur[lv])
+                    {
+                        flush(lv);
+                        cur[lv] = cj;
+                    }
+#pragma unroll
+                    for (int bi = 0; bi < BD; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < BD; ++bj)
                         {
-                            const int row = ri + bi, col = rj + bj;
-                            if (row > col || (i == j && bi > bj))
-                                continue;
-                            double val = v[bi * bd + bj];
+                            double c = v[bi * BD + bj];
                             if (i != j && ci == cj)
-                                val += v[bj * bd + bi];
-                            atomicAdd(mat + upper_index(dim, row, col), val);
+                                c += v[bj * BD + bi];
+                            if (i == j && bi > bj)
+                                c = 0.0;
+                            acc[lv][bi * BD + bj] += c;
                         }
                 }
             }
+#pragma unroll
+            for (int lv = 0; lv < GAL_LEVELS; ++lv)
+                if (lv < level_num)
+                    flush(lv);
         }
 
         // ---------------------------------------------------------------------- build: inverse
@@ -682,8 +728,16 @@ namespace psmas
     void launch_galerkin(int npad, int bd, int level_num, const int *mat_off, const SellView<double> &A,
                          const int *row_len, const int *block_of_slot, const int4 *cmap, double *mats, hipStream_t s)
     {
-        hipLaunchKernelGGL(k_galerkin, dim3(div_up(npad, 128)), dim3(128), 0, s, npad, bd, level_num, mat_off, A,
-                           row_len, block_of_slot, cmap, mats);
+        const dim3 grid(div_up(npad, 128)), block(128);
+        if (bd == 1)
+            hipLaunchKernelGGL(k_galerkin<1>, grid, block, 0, s, npad, level_num, mat_off, A, row_len, block_of_slot,
+                               cmap, mats);
+        else if (bd == 2)
+            hipLaunchKernelGGL(k_galerkin<2>, grid, block, 0, s, npad, level_num, mat_off, A, row_len, block_of_slot,
+                               cmap, mats);
+        else
+            hipLaunchKernelGGL(k_galerkin<3>, grid, block, 0, s, npad, level_num, mat_off, A, row_len, block_of_slot,
+                               cmap, mats);
         PSMAS_LAUNCH_CHECK();
     }
 

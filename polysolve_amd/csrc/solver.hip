@@ -1100,10 +1100,10 @@ namespace psmas
         launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), stream_);
         if (learned)
         {
-            // Block path: per slot the lattice cell (16 B), 8 local ids (8 B) and r; entry path: per
+            // Block path: per slot the lattice cell (16 B) and r, 2 B per list entry; entry path: per
             // coupled slot 8 {slot, weight} entries (8 B each) and 8 gathers of r.
             tic("lcc_restrict", lcc_->block_restriction()
-                                    ? static_cast<double>(npad_) * (24.0 + BD * 4.0)
+                                    ? static_cast<double>(npad_) * (16.0 + BD * 4.0) + lcc_->restrict_entries() * 2.0
                                     : static_cast<double>(lcc_->restrict_entries()) * (8.0 + BD * 4.0));
             lcc_->restrict_to_lattice<BD, float>(d_mr32_.data(), d_active_.data(), stream_,
                                                  static_cast<float>(params_.learned_output_scale));
