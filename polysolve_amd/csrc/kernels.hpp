@@ -181,7 +181,8 @@ namespace psmas
 
     /// y_m = Inv_m x_m for matrices [mat_begin, mat_begin + count) of the multi-level vectors.
     /// Storage: packed upper row-major.  Half storage is scaled: Inv = D S D with D per row.
-    /// bank_sys/active (may be null): skip fine banks of systems that stopped iterating.
+    /// bank_sys/active (may be null): skip matrices of systems that stopped iterating
+    /// (bank_sys[m] = system of matrix m, -1 for coarse matrices, which are never skipped).
     template <int BD, typename TM, typename T>
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
                           const int *bank_sys, const int *active, hipStream_t s);

@@ -76,10 +76,16 @@ namespace psmas
                                                                      double *__restrict__ partial)
         {
             __shared__ T vals[256 * 8 * BD];
+            __shared__ unsigned short ent[256 * 8];
             const int b = blockIdx.x;
             const int nt = block_nt[b];
             if (nt == 0)
                 return;
+            // The block's lists are contiguous (at most 8 entries per slot): stage them in LDS.
+            const int base = tbase[b];
+            const int e_begin = tptr[base], e_count = tptr[base + nt] - e_begin;
+            for (int e = threadIdx.x; e < e_count; e += 256)
+                ent[e] = tentry[e_begin + e];
             const int p = b * 256 + threadIdx.x;
             const int4 c = cell[p];
             if (c.x >= 0)
@@ -99,13 +105,13 @@ namespace psmas
                 }
             }
             __syncthreads();
-            const int base = tbase[b];
             for (int i = threadIdx.x; i < nt * BD; i += 256)
             {
                 const int t = i / BD, d = i % BD;
+                const int e0 = tptr[base + t] - e_begin, e1 = tptr[base + t + 1] - e_begin;
                 double s = 0.0;
-                for (int e = tptr[base + t]; e < tptr[base + t + 1]; ++e)
-                    s += static_cast<double>(vals[static_cast<int>(tentry[e]) * BD + d]);
+                for (int e = e0; e < e1; ++e)
+                    s += static_cast<double>(vals[static_cast<int>(ent[e]) * BD + d]);
                 partial[(static_cast<size_t>(b) * LCC_TMAX + t) * BD + d] = s;
             }
         }

@@ -88,8 +88,12 @@ namespace psmas
             __shared__ TA part[N];
 
             const int m = mat_begin + blockIdx.x;
-            if (bank_sys != nullptr && active[bank_sys[m]] == 0)
-                return;
+            if (bank_sys != nullptr)
+            {
+                const int sy = bank_sys[m]; // -1: a coarse matrix (its nodes may span systems)
+                if (sy >= 0 && active[sy] == 0)
+                    return;
+            }
             const TM *A = mats + static_cast<size_t>(m) * L;
             {
                 using u32x4 = unsigned int __attribute__((ext_vector_type(4)));

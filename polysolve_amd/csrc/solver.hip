@@ -447,7 +447,7 @@ namespace psmas
         all = PaddedBsr();
 
         // System maps.
-        std::vector<int> block_sys(nblocks_), blk_begin(K), blk_count(K), bank_sys(nparts_);
+        std::vector<int> block_sys(nblocks_), blk_begin(K), blk_count(K), bank_sys(nparts_, -1);
         for (int s = 0; s < K; ++s)
         {
             blk_begin[s] = sys_[s].blk_begin;
@@ -479,6 +479,13 @@ namespace psmas
         lattice_stale_ = true;
         rhs_ready_ = false;
         build_mas();
+        {
+            // Matrix -> system for the apply's skip test: fine banks as above, coarse matrices -1.
+            std::vector<int> mat_sys(total_mats_, -1);
+            std::copy(bank_sys.begin(), bank_sys.end(), mat_sys.begin());
+            d_bank_sys_.allocate(mat_sys.size(), "MAS matrix systems");
+            PSMAS_HIP(hipMemcpy(d_bank_sys_.data(), mat_sys.data(), d_bank_sys_.bytes(), hipMemcpyHostToDevice));
+        }
         allocate_vectors();
         if (params_.precision == Precision::Mixed)
             ensure_mixed_data();
@@ -742,9 +749,11 @@ namespace psmas
         const size_t L = static_cast<size_t>(dim) * (dim + 1) / 2;
         if (!d_mats16_)
         {
-            d_mats16_.allocate(static_cast<size_t>(mat_cnt_[0]) * L, "MAS fine bank inverses (FP16)");
-            d_dscale_.allocate(static_cast<size_t>(mat_cnt_[0]) * dim, "MAS fine bank scaling");
-            launch_make_half(bd_, mat_cnt_[0], d_mats64_.data(), d_mats16_.data(), d_dscale_.data(), stream_);
+            // Every level's inverses in scaled FP16 (S = D^-1 Inv D^-1, unit diagonal): one apply
+            // launch covers all matrices.
+            d_mats16_.allocate(static_cast<size_t>(total_mats_) * L, "MAS bank inverses (FP16)");
+            d_dscale_.allocate(static_cast<size_t>(total_mats_) * dim, "MAS bank scaling");
+            launch_make_half(bd_, total_mats_, d_mats64_.data(), d_mats16_.data(), d_dscale_.data(), stream_);
         }
         const size_t nvec = static_cast<size_t>(npad_) * bd_;
         const size_t nml = static_cast<size_t>(total_level_nodes_) * bd_;
@@ -1109,12 +1118,10 @@ namespace psmas
                                                  static_cast<float>(params_.learned_output_scale));
             toc();
         }
-        tic("mas_apply_fp16", mat_cnt_[0] * (L * 2.0 + dim * 4.0 + 2.0 * dim * 4.0));
-        launch_mas_apply<BD, __half, float>(0, mat_cnt_[0], d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
+        tic("mas_apply_fp16", total_mats_ * (L * 2.0 + dim * 4.0 + 2.0 * dim * 4.0));
+        launch_mas_apply<BD, __half, float>(0, total_mats_, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
                                             d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_);
         toc();
-        launch_mas_apply<BD, double, float>(mat_cnt_[0], total_mats_ - mat_cnt_[0], d_mats64_.data(), nullptr,
-                                            d_mr32_.data(), d_mz32_.data(), nullptr, nullptr, stream_);
         apply_dense_coarse<BD, float>(d_mr32_.data(), d_mz32_.data());
         LccApply lv;
         FlexDot<float> flex;
