@@ -24,6 +24,9 @@ namespace psmas
             return v;
         }
 
+        __device__ __forceinline__ float bf16_lo(unsigned v) { return __uint_as_float(v << 16); }
+        __device__ __forceinline__ float bf16_hi(unsigned v) { return __uint_as_float(v & 0xffff0000u); }
+
         __device__ __forceinline__ float trilinear_weight(int corner, float fx, float fy, float fz)
         {
             const float wx = (corner & 1) ? fx : 1.0f - fx;
@@ -116,93 +119,95 @@ namespace psmas
             }
         }
 
-        /// One workgroup per system: u = sum of each lattice point's partials (in list order),
-        /// x = u sqrt(m) / |u| (zero-padded to md), scale = |u| a / sqrt(m).
+        /// One wave per (global) lattice point J: u[J][d] = sum of its partials, lanes striding over
+        /// the point's source list and a fixed-order wave reduction.
         template <int BD>
-        __global__ __launch_bounds__(256) void k_lcc_finish(int np, int md, const int *__restrict__ point_ptr,
+        __global__ __launch_bounds__(256) void k_lcc_gather(int npts, const int *__restrict__ point_ptr,
                                                             const int *__restrict__ point_src,
-                                                            const double *__restrict__ partial, double *__restrict__ u,
-                                                            float *__restrict__ x, float *__restrict__ scale,
-                                                            float ascale, const int *__restrict__ active)
+                                                            const double *__restrict__ partial, double *__restrict__ u)
         {
-            const int s = blockIdx.x;
-            if (active != nullptr && active[s] == 0)
+            const int J = blockIdx.x * 4 + threadIdx.x / WAVE;
+            const int lane = threadIdx.x % WAVE;
+            if (J >= npts)
                 return;
-            const int m = np * BD;
-            double *us = u + static_cast<size_t>(s) * m;
-            double sq = 0.0;
-            for (int j = threadIdx.x; j < np; j += 256)
+            double a[BD];
+#pragma unroll
+            for (int d = 0; d < BD; ++d)
+                a[d] = 0.0;
+            for (int i = point_ptr[J] + lane; i < point_ptr[J + 1]; i += WAVE)
             {
-                const int J = s * np + j;
-                double a[BD];
+                const size_t c = static_cast<size_t>(point_src[i]);
 #pragma unroll
                 for (int d = 0; d < BD; ++d)
-                    a[d] = 0.0;
-                for (int i = point_ptr[J]; i < point_ptr[J + 1]; ++i)
-                {
-                    const size_t c = static_cast<size_t>(point_src[i]);
-#pragma unroll
-                    for (int d = 0; d < BD; ++d)
-                        a[d] += partial[c * BD + d];
-                }
-#pragma unroll
-                for (int d = 0; d < BD; ++d)
-                {
-                    us[j * BD + d] = a[d];
-                    sq += a[d] * a[d];
-                }
+                    a[d] += partial[c * BD + d];
             }
-            __shared__ double norm;
-            sq = block_sum<256>(sq);
-            if (threadIdx.x == 0)
-                norm = sqrt(sq);
-            __syncthreads();
-            const double nu = norm;
-            const double f = nu > 0.0 ? sqrt(static_cast<double>(m)) / nu : 0.0;
-            float *xs = x + static_cast<size_t>(s) * md;
-            for (int i = threadIdx.x; i < md; i += 256)
-                xs[i] = i < m ? static_cast<float>(us[i] * f) : 0.0f;
-            if (threadIdx.x == 0)
-                scale[s] = static_cast<float>(nu * ascale / sqrt(static_cast<double>(m)));
+#pragma unroll
+            for (int d = 0; d < BD; ++d)
+                a[d] = wave_sum(a[d]);
+            if (lane == 0)
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                    u[static_cast<size_t>(J) * BD + d] = a[d];
         }
 
-        struct GemvSeg
+        /// Layer 1 (ReLU) and the skip layer on the normalised lattice residual.  Workgroup (row
+        /// chunk, system group): the threads compute |u_s| for the group's systems, stage
+        /// x_s = u_s sqrt(m) / |u_s| (zero-padded to md) in LDS and, in the first chunk, write
+        /// scale[s] = |u_s| a / sqrt(m); then each wave produces one row (rows < h: W1 -> h1,
+        /// else the skip layer S -> skip).
+        __global__ __launch_bounds__(256) void k_lcc_in(const double *__restrict__ u, int m, int md,
+                                                        const unsigned short *__restrict__ W1,
+                                                        const float *__restrict__ b1, int h,
+                                                        const unsigned short *__restrict__ S, float *__restrict__ h1,
+                                                        float *__restrict__ skip, float *__restrict__ scale,
+                                                        float ascale, int nsys, int gsize,
+                                                        const int *__restrict__ active)
         {
-            const unsigned short *W = nullptr; ///< BF16 rows of length K (K % 8 == 0)
-            const float *bias = nullptr;
-            int rows = 0;
-            int relu = 0;
-            float *Y = nullptr;
-            int ldy = 0;
-        };
-
-        __device__ __forceinline__ float bf16_lo(unsigned v) { return __uint_as_float(v << 16); }
-        __device__ __forceinline__ float bf16_hi(unsigned v) { return __uint_as_float(v & 0xffff0000u); }
-
-        /// Y[s][row] = act(W[row] . X[s] + bias[row]) (+ add[s][row]) (* scale[s]) for the rows of
-        /// two weight segments; one wave per (row, group of 8 systems), FP32 accumulation.
-        __global__ __launch_bounds__(256) void k_lcc_gemv(GemvSeg a, GemvSeg b, int K, const float *__restrict__ X,
-                                                          int ldx, int nsys, const float *__restrict__ add,
-                                                          int ld_add, const float *__restrict__ scale,
-                                                          const int *__restrict__ active)
-        {
-            const int wave = blockIdx.x * 4 + threadIdx.x / WAVE;
-            const int lane = threadIdx.x % WAVE;
-            const int ngroups = (nsys + GEMV_SYS - 1) / GEMV_SYS;
-            const int row_all = wave / ngroups, g = wave % ngroups;
-            if (row_all >= a.rows + b.rows)
+            extern __shared__ float xs[]; // [gsize][md]
+            __shared__ double red[GEMV_SYS][4];
+            __shared__ double fac[GEMV_SYS];
+            const int ngroups = (nsys + gsize - 1) / gsize;
+            const int g = blockIdx.x % ngroups, chunk = blockIdx.x / ngroups;
+            const int s0 = g * gsize;
+            const int ns = min(gsize, nsys - s0);
+            const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+            for (int j = 0; j < ns; ++j)
+            {
+                const double *us = u + static_cast<size_t>(s0 + j) * m;
+                double q = 0.0;
+                for (int i = threadIdx.x; i < m; i += 256)
+                    q += us[i] * us[i];
+                q = wave_sum(q);
+                if (lane == 0)
+                    red[j][wave] = q;
+            }
+            __syncthreads();
+            if (threadIdx.x < ns)
+            {
+                const int j = threadIdx.x;
+                const double nu = sqrt(red[j][0] + red[j][1] + red[j][2] + red[j][3]);
+                fac[j] = nu > 0.0 ? sqrt(static_cast<double>(m)) / nu : 0.0;
+                if (chunk == 0)
+                    scale[s0 + j] = static_cast<float>(nu * ascale / sqrt(static_cast<double>(m)));
+            }
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < ns * md; idx += 256)
+            {
+                const int j = idx / md, i = idx % md;
+                xs[idx] = i < m ? static_cast<float>(u[static_cast<size_t>(s0 + j) * m + i] * fac[j]) : 0.0f;
+            }
+            __syncthreads();
+            const int row_all = chunk * 4 + wave;
+            if (row_all >= h + md)
                 return;
-            const bool first = row_all < a.rows;
-            const int row = first ? row_all : row_all - a.rows;
-            const unsigned short *W = first ? a.W : b.W;
-            const int s0 = g * GEMV_SYS;
-            const int ns = min(GEMV_SYS, nsys - s0);
+            const bool first = row_all < h;
+            const int row = first ? row_all : row_all - h;
+            const uint4 *wrow = reinterpret_cast<const uint4 *>((first ? W1 : S) + static_cast<size_t>(row) * md);
             float acc[GEMV_SYS];
 #pragma unroll
             for (int j = 0; j < GEMV_SYS; ++j)
                 acc[j] = 0.0f;
-            const uint4 *wrow = reinterpret_cast<const uint4 *>(W + static_cast<size_t>(row) * K);
-            for (int c = lane; c < K / 8; c += WAVE)
+            for (int c = lane; c < md / 8; c += WAVE)
             {
                 const uint4 wv = wrow[c];
                 const float w[8] = {bf16_lo(wv.x), bf16_hi(wv.x), bf16_lo(wv.y), bf16_hi(wv.y),
@@ -212,10 +217,10 @@ namespace psmas
                 {
                     if (j < ns)
                     {
-                        const float4 *xp = reinterpret_cast<const float4 *>(X + static_cast<size_t>(s0 + j) * ldx) + 2 * c;
-                        const float4 x0 = xp[0], x1 = xp[1];
-                        acc[j] += w[0] * x0.x + w[1] * x0.y + w[2] * x0.z + w[3] * x0.w + w[4] * x1.x + w[5] * x1.y +
-                                  w[6] * x1.z + w[7] * x1.w;
+                        const float *x = xs + j * md + c * 8;
+#pragma unroll
+                        for (int e = 0; e < 8; ++e)
+                            acc[j] += w[e] * x[e];
                     }
                 }
             }
@@ -224,7 +229,6 @@ namespace psmas
                 acc[j] = wave_sum_f(acc[j]);
             if (lane != 0)
                 return;
-            const GemvSeg &S = first ? a : b;
 #pragma unroll
             for (int j = 0; j < GEMV_SYS; ++j)
             {
@@ -233,16 +237,13 @@ namespace psmas
                 const int s = s0 + j;
                 if (active != nullptr && active[s] == 0)
                     continue;
-                float y = acc[j];
-                if (S.bias)
-                    y += S.bias[row];
-                if (S.relu)
-                    y = y > 0.0f ? y : 0.0f;
-                if (add)
-                    y += add[static_cast<size_t>(s) * ld_add + row];
-                if (scale)
-                    y *= scale[s];
-                S.Y[static_cast<size_t>(s) * S.ldy + row] = y;
+                if (first)
+                {
+                    const float y = acc[j] + b1[row];
+                    h1[static_cast<size_t>(s) * h + row] = y > 0.0f ? y : 0.0f;
+                }
+                else
+                    skip[static_cast<size_t>(s) * md + row] = acc[j];
             }
         }
 
@@ -572,10 +573,8 @@ namespace psmas
             d_partial_.allocate(static_cast<size_t>(std::max(nchunks_, 1)) * bd, "MAS lattice partials");
         }
         d_u_.allocate(static_cast<size_t>(nsys) * m(), "MAS lattice residual");
-        d_x_.allocate(static_cast<size_t>(nsys) * md_, "MAS network input");
         d_scale_.allocate(nsys, "MAS network scale");
         d_coef_.allocate(static_cast<size_t>(nsys) * m(), "MAS lattice correction");
-        d_x_.zero(s);
         d_coef_.zero(s);
         d_h1_.reset();
         d_h2_.reset();
@@ -598,10 +597,12 @@ namespace psmas
                                d_entries_.data(), v, d_partial_.data());
             PSMAS_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL((k_lcc_finish<BD>), dim3(nsys_), dim3(256), 0, s, np_, md_, d_point_ptr_.data(),
-                           d_point_src_.data(), d_partial_.data(), d_u_.data(), d_x_.data(), d_scale_.data(),
-                           ascale_ * output_factor, active);
+        const int npts = nsys_ * np_;
+        hipLaunchKernelGGL((k_lcc_gather<BD>), dim3(div_up(npts, 4)), dim3(256), 0, s, npts, d_point_ptr_.data(),
+                           d_point_src_.data(), d_partial_.data(), d_u_.data());
         PSMAS_LAUNCH_CHECK();
+        out_factor_ = output_factor;
+        (void)active;
     }
 
     void LearnedCoarse::infer(const int *active, hipStream_t s)
@@ -622,12 +623,14 @@ namespace psmas
         const unsigned short *wb = net_->bf16_weights();
         const float *wf = net_->fp32_weights();
         const int ngroups = (nsys_ + GEMV_SYS - 1) / GEMV_SYS;
-        auto grid = [&](int rows) { return dim3(div_up(rows * ngroups, 4)); };
-        // Layer 1 and the skip layer read the same input.
-        GemvSeg l1{wb + net_->offset(0), wf + net_->offset(1), h, 1, d_h1_.data(), h};
-        GemvSeg sk{wb + net_->offset(6), nullptr, md_, 0, d_skip_.data(), md_};
-        hipLaunchKernelGGL(k_lcc_gemv, grid(h + md_), dim3(256), 0, s, l1, sk, md_, d_x_.data(), md_, nsys_,
-                           static_cast<const float *>(nullptr), 0, static_cast<const float *>(nullptr), active);
+        // Layer 1 and the skip layer read the same normalised input, built in the kernel.
+        const int gsize = std::max(1, std::min(GEMV_SYS, static_cast<int>(60000 / (sizeof(float) * md_))));
+        const int ngroups_in = div_up(nsys_, gsize);
+        const size_t lds = sizeof(float) * static_cast<size_t>(std::min(gsize, nsys_)) * md_;
+        hipLaunchKernelGGL(k_lcc_in, dim3(div_up(h + md_, 4) * ngroups_in), dim3(256), lds, s,
+                           static_cast<const double *>(d_u_.data()), m(), md_, wb + net_->offset(0), wf + net_->offset(1),
+                           h, wb + net_->offset(6), d_h1_.data(), d_skip_.data(), d_scale_.data(), ascale_ * out_factor_,
+                           nsys_, gsize, active);
         PSMAS_LAUNCH_CHECK();
         // Layer 2 fused into the output layer (only the m real outputs), plus the skip output,
         // times the scale.

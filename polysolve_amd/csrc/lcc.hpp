@@ -46,8 +46,7 @@ namespace psmas
             return net_ ? 2.0 * static_cast<double>(net_->num_parameters()) : 0.0;
         }
 
-        /// u = P^T v (FP64 sums) for all systems -> d_u_ [nsys][m]; and the normalised network
-        /// input and scale for every active system.
+        /// u = P^T v (FP64 sums) for all systems -> d_u_ [nsys][m] (the network normalises it).
         template <int BD, typename T>
         void restrict_to_lattice(const T *v, const int *active, hipStream_t s, float output_factor = 1.0f);
         /// Network inference on the normalised inputs -> coef [nsys][m] (scaled back).
@@ -92,7 +91,6 @@ namespace psmas
         long long entries_count_ = 0;
         DeviceBuffer<double> d_partial_;  ///< [chunk][bd]
         DeviceBuffer<double> d_u_;        ///< [sys][m]
-        DeviceBuffer<float> d_x_;         ///< [sys][md] normalised input
         DeviceBuffer<float> d_scale_;     ///< [sys]
         DeviceBuffer<float> d_h1_, d_h2_, d_skip_, d_coef_;
         std::vector<double> ptp_inv_;     ///< (P^T P)^-1 of system 0, np x np (on first use)
@@ -103,6 +101,7 @@ namespace psmas
         std::unique_ptr<Mlp> net_;
         bool trained_ = false;
         float ascale_ = 1.0f;
+        float out_factor_ = 1.0f; ///< learned_output_scale of the current restriction
         double last_loss_ = 0.0;
         int m_pairs_ = 0;
         std::vector<float> pairs_u_, pairs_w_;
