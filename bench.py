@@ -176,6 +176,8 @@ def main() -> int:
     ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (no HIP graph replay)")
     ap.add_argument("--max-iter", type=int, default=20000, help="iteration cap (short counter-collection runs)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction path")
+    ap.add_argument("--profile-learned", action="store_true",
+                    help="profile mode: also train briefly (2 solves, 20 epochs) and time learned solves")
     ap.add_argument("--train-solves", type=int, default=8,
                     help="learned path: solves of the drifting sequence (steps 1..T) that provide training pairs")
     ap.add_argument("--lattice", type=int, default=7, help="learned path: lattice points per axis")
@@ -246,6 +248,24 @@ def main() -> int:
         solver.set_parameters({"MAS": {"precision": "mixed", "coarse_solver": args.coarse}})
         mixed_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
         log(f"  mixed {mixed_ms:.2f} ms/solve")
+        if args.profile_learned and args.config == 2:
+            # Kernel durations do not depend on how well the network is trained.
+            from polysolve_amd import problems as P
+
+            solver.set_parameters({"MAS": {"learned_collect": True, "lattice_points": args.lattice,
+                                           "learned_random_pairs": 32}})
+            seq = P.config5_moduli(prob.ne, steps=3, seed=0)
+            next(seq)
+            for E in seq:
+                pt = P.hex_elasticity(prob.ne, E=E)
+                solver.factorize(pt.A)
+                xt = np.zeros(pt.n)
+                solver.solve(pt.b, xt)
+            solver.learned_train(epochs=20, batch=64, lr=3e-3, seed=0)
+            solver.set_parameters({"MAS": {"learned_collect": False, "learned_correction": True}})
+            solver.factorize(prob.A)
+            learned_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
+            log(f"  learned {learned_ms:.2f} ms/solve, {solver.get_info()['solver_iter']} it")
         return 0
 
     # Mixed path (headline).

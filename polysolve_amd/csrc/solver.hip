@@ -1008,7 +1008,9 @@ namespace psmas
     template <int BD>
     void Solver::fp64_precondition(bool init)
     {
+        tic("restrict_levels_fp64", static_cast<double>(flat_nodes_) * 4.0 * 2.0);
         launch_restrict_coarse<BD, double>(coarse_restrict(), d_mr_.data(), stream_);
+        toc();
         tic("mas_apply_fp64", static_cast<double>(total_mats_) * ((BANK * BD) * (BANK * BD + 1) / 2 * 8.0 +
                                                                   2.0 * BANK * BD * 8.0));
         launch_mas_apply<BD, double, double>(0, total_mats_, d_mats64_.data(), nullptr, d_mr_.data(), d_mz_.data(),
@@ -1152,7 +1154,9 @@ namespace psmas
         const long long nvec = static_cast<long long>(npad_) * BD;
         const int dim = BANK * BD;
         const double L = static_cast<double>(dim) * (dim + 1) / 2;
+        tic("restrict_levels", static_cast<double>(flat_nodes_) * 4.0 * 2.0);
         launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), stream_);
+        toc();
         if (learned)
         {
             // Block path: per slot the lattice cell (16 B) and r, 2 B per list entry; entry path: per
