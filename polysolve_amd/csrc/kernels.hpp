@@ -173,17 +173,10 @@ namespace psmas
         int offsets[5] = {0, 0, 0, 0, 0};   ///< node offset of multi-level segment (0 = fine)
         const uint32_t *lead[4] = {nullptr, nullptr, nullptr, nullptr}; ///< index: level-(l-1) node, l >= 1
         const int *gid[4] = {nullptr, nullptr, nullptr, nullptr};
-        // Flat form (one launch): every node of levels 1..L-1 with the ascending list of the level-0
-        // nodes of its subtree, and its position in the multi-level vector.
-        int flat_nodes = 0;
-        const int *flat_ptr = nullptr;    ///< flat_nodes + 1
-        const int *flat_member = nullptr; ///< level-0 ids
-        const int *flat_out = nullptr;    ///< node position in the multi-level vector
     };
 
-    /// Restrict coarse level 0 to levels 1..L-1: one launch, one wave per coarse node summing the
-    /// level-0 nodes of its subtree in ascending order (when the flat lists are set; otherwise one
-    /// launch per level through the leader masks).
+    /// Restrict coarse level 0 to levels 1..L-1: one launch per level, one thread per node; a
+    /// level-l node sums at most 32 level-(l-1) members (its component inside one bank).
     template <int BD, typename T>
     void launch_restrict_coarse(const CoarseRestrict &cr, T *mr, hipStream_t s);
 

@@ -62,40 +62,6 @@ namespace psmas
                 dst[static_cast<size_t>(g) * BD + i] = static_cast<T>(s[i]);
         }
 
-        /// One wave per node of levels 1..L-1: the sum of the level-0 values of its subtree.
-        template <int BD, typename T>
-        __global__ __launch_bounds__(256) void k_restrict_flat(int nodes, const int *__restrict__ ptr,
-                                                               const int *__restrict__ member,
-                                                               const int *__restrict__ out, const T *__restrict__ src,
-                                                               T *__restrict__ mr)
-        {
-            const int node = blockIdx.x * 4 + threadIdx.x / WAVE;
-            const int lane = threadIdx.x % WAVE;
-            if (node >= nodes)
-                return;
-            double a[BD];
-#pragma unroll
-            for (int d = 0; d < BD; ++d)
-                a[d] = 0.0;
-            for (int e = ptr[node] + lane; e < ptr[node + 1]; e += WAVE)
-            {
-                const size_t m = static_cast<size_t>(member[e]) * BD;
-#pragma unroll
-                for (int d = 0; d < BD; ++d)
-                    a[d] += static_cast<double>(src[m + d]);
-            }
-#pragma unroll
-            for (int d = 0; d < BD; ++d)
-                a[d] = wave_sum(a[d]);
-            if (lane == 0)
-            {
-                const size_t o = static_cast<size_t>(out[node]) * BD;
-#pragma unroll
-                for (int d = 0; d < BD; ++d)
-                    mr[o + d] = static_cast<T>(a[d]);
-            }
-        }
-
         // ------------------------------------------------------------------------------ apply
         template <int BD>
         constexpr int apply_block() { return BD == 1 ? 64 : (BD == 2 ? 128 : 192); }
@@ -686,14 +652,6 @@ namespace psmas
     {
         if (cr.level_num <= 1)
             return;
-        if (cr.flat_nodes > 0)
-        {
-            hipLaunchKernelGGL((k_restrict_flat<BD, T>), dim3(div_up(cr.flat_nodes, 4)), dim3(256), 0, s, cr.flat_nodes,
-                               cr.flat_ptr, cr.flat_member, cr.flat_out, mr + static_cast<size_t>(cr.offsets[1]) * BD,
-                               mr);
-            PSMAS_LAUNCH_CHECK();
-            return;
-        }
         for (int l = 1; l < cr.level_num; ++l)
         {
             const int count = cr.counts[l - 1];

@@ -26,7 +26,6 @@
 #include "device.hpp"
 
 #include <algorithm>
-#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -329,10 +328,6 @@ namespace psmas
             c.lead[l] = d_leadc_[l].data();
             c.gid[l] = d_gidc_[l].data();
         }
-        c.flat_nodes = flat_nodes_;
-        c.flat_ptr = d_flat_ptr_.data();
-        c.flat_member = d_flat_member_.data();
-        c.flat_out = d_flat_out_.data();
         return c;
     }
 
@@ -691,47 +686,6 @@ namespace psmas
             lev_off_[l + 1] = total_level_nodes_;
             total_level_nodes_ += BANK * mat_cnt_[l + 1];
         }
-        // Flat restriction lists: every node of levels 1..L-1 with the level-0 nodes of its subtree.
-        flat_nodes_ = 0;
-        if (level_num_ > 1)
-        {
-            std::vector<int4> cm(npad_);
-            PSMAS_HIP(hipMemcpy(cm.data(), d_cmap_.data(), sizeof(int4) * npad_, hipMemcpyDeviceToHost));
-            std::vector<std::array<int, MAX_COARSE_LEVELS>> up(cco_[0]);
-            for (int p = 0; p < npad_; ++p)
-            {
-                if (block_of_slot_[p] < 0 || cm[p].x < 0)
-                    continue;
-                up[cm[p].x] = {cm[p].x, cm[p].y, cm[p].z, cm[p].w};
-            }
-            std::vector<int> ptr{0}, member, out;
-            for (int l = 1; l < level_num_; ++l)
-            {
-                std::vector<std::vector<int>> lists(cco_[l]);
-                for (int c0 = 0; c0 < cco_[0]; ++c0)
-                {
-                    const int g = up[c0][l];
-                    if (g >= 0)
-                        lists[g].push_back(c0);
-                }
-                for (int g = 0; g < cco_[l]; ++g)
-                {
-                    member.insert(member.end(), lists[g].begin(), lists[g].end());
-                    ptr.push_back(static_cast<int>(member.size()));
-                    out.push_back(lev_off_[l + 1] + g);
-                }
-            }
-            flat_nodes_ = static_cast<int>(out.size());
-            d_flat_ptr_.allocate(ptr.size(), "MAS flat restriction");
-            d_flat_member_.allocate(std::max<size_t>(member.size(), 1), "MAS flat restriction");
-            d_flat_out_.allocate(std::max<size_t>(out.size(), 1), "MAS flat restriction");
-            PSMAS_HIP(hipMemcpy(d_flat_ptr_.data(), ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
-            if (!member.empty())
-                PSMAS_HIP(hipMemcpy(d_flat_member_.data(), member.data(), sizeof(int) * member.size(),
-                                    hipMemcpyHostToDevice));
-            if (!out.empty())
-                PSMAS_HIP(hipMemcpy(d_flat_out_.data(), out.data(), sizeof(int) * out.size(), hipMemcpyHostToDevice));
-        }
         d_lev_off_.allocate(MAX_COARSE_LEVELS + 1, "MAS level offsets");
         d_mat_off_.allocate(MAX_COARSE_LEVELS + 1, "MAS matrix offsets");
         PSMAS_HIP(hipMemcpyAsync(d_lev_off_.data(), lev_off_, sizeof(lev_off_), hipMemcpyHostToDevice, stream_));
@@ -1008,7 +962,7 @@ namespace psmas
     template <int BD>
     void Solver::fp64_precondition(bool init)
     {
-        tic("restrict_levels_fp64", static_cast<double>(flat_nodes_) * 4.0 * 2.0);
+        tic("restrict_levels_fp64", static_cast<double>(cco_[0] + cco_[1] + cco_[2]) * BD * 8.0 * 2.0);
         launch_restrict_coarse<BD, double>(coarse_restrict(), d_mr_.data(), stream_);
         toc();
         tic("mas_apply_fp64", static_cast<double>(total_mats_) * ((BANK * BD) * (BANK * BD + 1) / 2 * 8.0 +
@@ -1154,7 +1108,7 @@ namespace psmas
         const long long nvec = static_cast<long long>(npad_) * BD;
         const int dim = BANK * BD;
         const double L = static_cast<double>(dim) * (dim + 1) / 2;
-        tic("restrict_levels", static_cast<double>(flat_nodes_) * 4.0 * 2.0);
+        tic("restrict_levels", static_cast<double>(cco_[0] + cco_[1] + cco_[2]) * BD * 4.0 * 2.0);
         launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), stream_);
         toc();
         if (learned)

@@ -154,8 +154,6 @@ namespace psmas
         DeviceBuffer<uint32_t> d_leadc_[MAX_COARSE_LEVELS];
         DeviceBuffer<int> d_gidc_[MAX_COARSE_LEVELS];
         DeviceBuffer<int4> d_cmap_;
-        int flat_nodes_ = 0;
-        DeviceBuffer<int> d_flat_ptr_, d_flat_member_, d_flat_out_;
         DeviceBuffer<int> d_lev_off_, d_mat_off_;
         DeviceBuffer<double> d_mats64_;
         DeviceBuffer<__half> d_mats16_;
