@@ -67,33 +67,31 @@ namespace psmas
 
         /// One workgroup per 256-node block: partial[b][t][d] = sum over the block's coupled slots
         /// of w_q * v[slot][d] for the corners q whose block-local lattice point is t (t < nt[b]).
-        /// Every thread stages its 8 weighted contributions in LDS; thread (t, d) then adds the
-        /// entries of target t's list (thread << 3 | corner, built on the host) in list order.
+        /// The host orders the block's (slot, corner) pairs target by target and gives every pair
+        /// its position (pos, 8 per slot); each thread writes its 8 weighted contributions to those
+        /// positions in LDS, so target t's terms are a contiguous range that thread (t, d) adds in
+        /// order.
         template <int BD, typename T>
         __global__ __launch_bounds__(256) void k_lcc_restrict_blocks(const int4 *__restrict__ cell,
+                                                                     const uint4 *__restrict__ pos,
                                                                      const int *__restrict__ block_nt,
                                                                      const int *__restrict__ tbase,
                                                                      const int *__restrict__ tptr,
-                                                                     const unsigned short *__restrict__ tentry,
                                                                      const T *__restrict__ v,
                                                                      double *__restrict__ partial)
         {
             __shared__ T vals[256 * 8 * BD];
-            __shared__ unsigned short ent[256 * 8];
             const int b = blockIdx.x;
             const int nt = block_nt[b];
             if (nt == 0)
                 return;
-            // The block's lists are contiguous (at most 8 entries per slot): stage them in LDS.
-            const int base = tbase[b];
-            const int e_begin = tptr[base], e_count = tptr[base + nt] - e_begin;
-            for (int e = threadIdx.x; e < e_count; e += 256)
-                ent[e] = tentry[e_begin + e];
             const int p = b * 256 + threadIdx.x;
             const int4 c = cell[p];
             if (c.x >= 0)
             {
                 const float fx = __int_as_float(c.y), fy = __int_as_float(c.z), fz = __int_as_float(c.w);
+                const uint4 P = pos[p];
+                const unsigned pk[4] = {P.x, P.y, P.z, P.w};
                 T r[BD];
 #pragma unroll
                 for (int d = 0; d < BD; ++d)
@@ -101,20 +99,23 @@ namespace psmas
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
                 {
+                    const int at = static_cast<int>((pk[q >> 1] >> (16 * (q & 1))) & 0xffffu);
                     const T w = static_cast<T>(trilinear_weight(q, fx, fy, fz));
 #pragma unroll
                     for (int d = 0; d < BD; ++d)
-                        vals[(threadIdx.x * 8 + q) * BD + d] = w * r[d];
+                        vals[at * BD + d] = w * r[d];
                 }
             }
             __syncthreads();
+            const int base = tbase[b];
+            const int e_begin = tptr[base];
             for (int i = threadIdx.x; i < nt * BD; i += 256)
             {
                 const int t = i / BD, d = i % BD;
                 const int e0 = tptr[base + t] - e_begin, e1 = tptr[base + t + 1] - e_begin;
                 double s = 0.0;
                 for (int e = e0; e < e1; ++e)
-                    s += static_cast<double>(vals[static_cast<int>(ent[e]) * BD + d]);
+                    s += static_cast<double>(vals[e * BD + d]);
                 partial[(static_cast<size_t>(b) * LCC_TMAX + t) * BD + d] = s;
             }
         }
@@ -466,7 +467,7 @@ namespace psmas
         // LCC_TMAX points; otherwise the chunked entry list above is used.
         const int nblocks = total_pad / 256;
         std::vector<int> block_nt(nblocks, 0), tbase(nblocks + 1, 0), tptr{0};
-        std::vector<unsigned short> tentry;
+        std::vector<uint4> pos(total_pad, uint4{0u, 0u, 0u, 0u});
         std::vector<std::vector<int>> point_lists(npts);
         bool blocks_ok = true;
         for (int b = 0; b < nblocks && blocks_ok; ++b)
@@ -491,8 +492,9 @@ namespace psmas
             block_nt[b] = static_cast<int>(pts.size());
             for (int t = 0; t < block_nt[b]; ++t)
                 point_lists[pts[t]].push_back(b * LCC_TMAX + t);
-            // Per target: the (thread, corner) pairs that feed it, in thread-then-corner order.
-            std::vector<std::vector<unsigned short>> lists(block_nt[b]);
+            // Per target: the (slot, corner) pairs that feed it, in slot-then-corner order; every
+            // pair's position in the block's target-by-target order goes to pos.
+            std::vector<std::vector<int>> lists(block_nt[b]);
             for (int p = b * 256; p < (b + 1) * 256; ++p)
                 if (cell[p].x >= 0)
                     for (int q = 0; q < 8; ++q)
@@ -501,13 +503,21 @@ namespace psmas
                         float w;
                         corners(cell[p], q, id, w);
                         const int t = static_cast<int>(std::lower_bound(pts.begin(), pts.end(), id) - pts.begin());
-                        lists[t].push_back(static_cast<unsigned short>(((p - b * 256) << 3) | q));
+                        lists[t].push_back(((p - b * 256) << 3) | q);
                     }
             tbase[b] = static_cast<int>(tptr.size()) - 1;
+            const int e_begin = tptr.back();
             for (const auto &l : lists)
             {
-                tentry.insert(tentry.end(), l.begin(), l.end());
-                tptr.push_back(static_cast<int>(tentry.size()));
+                for (size_t k = 0; k < l.size(); ++k)
+                {
+                    const int pq = l[k];
+                    const int at = tptr.back() - e_begin + static_cast<int>(k);
+                    const int p = b * 256 + (pq >> 3), q = pq & 7;
+                    unsigned *word = (q >> 1) == 0 ? &pos[p].x : (q >> 1) == 1 ? &pos[p].y : (q >> 1) == 2 ? &pos[p].z : &pos[p].w;
+                    *word |= static_cast<unsigned>(at) << (16 * (q & 1));
+                }
+                tptr.push_back(tptr.back() + static_cast<int>(l.size()));
             }
         }
         std::vector<int> point_ptr(npts + 1, 0), point_src;
@@ -555,13 +565,11 @@ namespace psmas
             d_block_nt_.allocate(block_nt.size(), "MAS lattice block targets");
             d_tbase_.allocate(tbase.size(), "MAS lattice block lists");
             d_tptr_.allocate(tptr.size(), "MAS lattice block lists");
-            d_tentry_.allocate(std::max<size_t>(tentry.size(), 1), "MAS lattice block lists");
+            d_pos_.allocate(pos.size(), "MAS lattice block positions");
             PSMAS_HIP(hipMemcpy(d_block_nt_.data(), block_nt.data(), d_block_nt_.bytes(), hipMemcpyHostToDevice));
             PSMAS_HIP(hipMemcpy(d_tbase_.data(), tbase.data(), d_tbase_.bytes(), hipMemcpyHostToDevice));
             PSMAS_HIP(hipMemcpy(d_tptr_.data(), tptr.data(), d_tptr_.bytes(), hipMemcpyHostToDevice));
-            if (!tentry.empty())
-                PSMAS_HIP(hipMemcpy(d_tentry_.data(), tentry.data(), sizeof(unsigned short) * tentry.size(),
-                                    hipMemcpyHostToDevice));
+            PSMAS_HIP(hipMemcpy(d_pos_.data(), pos.data(), d_pos_.bytes(), hipMemcpyHostToDevice));
             d_partial_.allocate(static_cast<size_t>(nblocks) * LCC_TMAX * bd, "MAS lattice partials");
         }
         else
@@ -569,7 +577,7 @@ namespace psmas
             d_block_nt_.reset();
             d_tbase_.reset();
             d_tptr_.reset();
-            d_tentry_.reset();
+            d_pos_.reset();
             d_partial_.allocate(static_cast<size_t>(std::max(nchunks_, 1)) * bd, "MAS lattice partials");
         }
         d_u_.allocate(static_cast<size_t>(nsys) * m(), "MAS lattice residual");
@@ -587,7 +595,7 @@ namespace psmas
         if (use_blocks_)
         {
             hipLaunchKernelGGL((k_lcc_restrict_blocks<BD, T>), dim3(nblocks_), dim3(256), 0, s, d_cell_.data(),
-                               d_block_nt_.data(), d_tbase_.data(), d_tptr_.data(), d_tentry_.data(), v,
+                               d_pos_.data(), d_block_nt_.data(), d_tbase_.data(), d_tptr_.data(), v,
                                d_partial_.data());
             PSMAS_LAUNCH_CHECK();
         }

@@ -84,7 +84,7 @@ namespace psmas
         DeviceBuffer<int> d_block_nt_;    ///< block path: distinct lattice points per node block
         DeviceBuffer<int> d_tbase_;       ///< block path: first (block, target) list of each block
         DeviceBuffer<int> d_tptr_;        ///< block path: entry offsets of every (block, target) list
-        DeviceBuffer<unsigned short> d_tentry_; ///< block path: thread << 3 | corner
+        DeviceBuffer<uint4> d_pos_;       ///< block path: position of each (slot, corner) term, 8 x 16 bit
         bool use_blocks_ = false;
         int nblocks_ = 0;
         int nchunks_ = 0;
