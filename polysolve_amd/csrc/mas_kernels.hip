@@ -17,6 +17,8 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include <type_traits>
+
 namespace psmas
 {
     // Matrix-element conversion used by the apply kernel.
@@ -66,11 +68,268 @@ namespace psmas
         template <int BD>
         constexpr int apply_block() { return BD == 1 ? 64 : (BD == 2 ? 128 : 192); }
 
-        /// y = Inv x for one bank matrix per workgroup.  The packed upper matrix is staged in LDS
-        /// with 16-byte loads; threads (row, half) each sum half of row `row`, reading element
-        /// (t, row) for t < row (packed row t, contiguous across threads) and (row, t) otherwise.
+        // FP16 tiled storage of a scaled bank matrix S (N = 32 BD, T = N / 8 tiles of 8 x 8 per
+        // side, unit diagonal implicit), N*N/2 values in blocks of 64:
+        //   block q < Q = T(T-1)/2: off-diagonal tile (I, J), I < J, row-major; tiles ordered by
+        //   diagonal J - I = 1, 2, ... and by I within a diagonal (neighbouring blocks then start
+        //   in different LDS banks when unpacked);
+        //   block Q + p, p < T/2: diagonal tiles 2p and 2p+1 folded into one 8 x 8 block X with
+        //   X[r][c] = S(16p + r, 16p + c) for c > r and X[r][c] = S(16p + 8 + c, 16p + 8 + r) for
+        //   c < r (X[r][r] unused).
+        template <int BD>
+        constexpr int half_matrix_len() { return (32 * BD) * (32 * BD) / 2; }
+
+        /// (I, J) of off-diagonal tile block q, I < J, for T tiles per side.
+        template <int T>
+        __device__ __forceinline__ void off_tile_of_block(int q, int &I, int &J)
+        {
+            int d = 1;
+            while (q >= T - d) // diagonal d holds T - d tiles
+            {
+                q -= T - d;
+                ++d;
+            }
+            I = q;
+            J = q + d;
+        }
+
+        /// Matrix element (i, j), i <= j, held at position e of the tiled FP16 layout (i == j: unused).
+        template <int BD>
+        __device__ __forceinline__ void half_tile_element(int e, int &i, int &j)
+        {
+            constexpr int T = 32 * BD / 8;
+            constexpr int Q = T * (T - 1) / 2;
+            const int blk = e >> 6, r = (e >> 3) & 7, c = e & 7;
+            if (blk < Q)
+            {
+                int I, J;
+                off_tile_of_block<T>(blk, I, J);
+                i = 8 * I + r;
+                j = 8 * J + c;
+            }
+            else
+            {
+                const int base = 16 * (blk - Q);
+                if (c >= r)
+                {
+                    i = base + r;
+                    j = base + c;
+                }
+                else
+                {
+                    i = base + 8 + c;
+                    j = base + 8 + r;
+                }
+            }
+        }
+
+        /// y_m = D_m S_m D_m x_m for FP16 tiled matrices [mat_begin, mat_begin + count), grid-stride.
+        /// Each matrix is unpacked into a full padded row-major copy in LDS.  An off-diagonal tile
+        /// row is one 16-byte store; its transpose is written by lane pairs (rows r, r^1 of a tile):
+        /// after one lane exchange each lane packs the two rows' values of every other column into
+        /// one 4-byte store.  Folded diagonal blocks are unpacked element by element.  The LDS
+        /// addresses of every thread's chunks are decoded once per launch.  Thread (half, row) =
+        /// (tid / N, tid % N) then sums half of row `row` with 16-byte LDS reads.  Two matrices are
+        /// in flight per workgroup: the loads of the one after next are issued before the current
+        /// matrix is computed.
+        template <int BD>
+        __global__ __launch_bounds__(apply_block<BD>()) void k_mas_apply_h16(int mat_begin, int count,
+                                                                              const __half *__restrict__ mats,
+                                                                              const float *__restrict__ dscale,
+                                                                              const float *__restrict__ mr,
+                                                                              float *__restrict__ mz,
+                                                                              const int *__restrict__ bank_sys,
+                                                                              const int *__restrict__ active)
+        {
+            constexpr int N = 32 * BD;
+            constexpr int T = N / 8;
+            constexpr int Q = T * (T - 1) / 2;
+            constexpr int LEN = half_matrix_len<BD>();
+            constexpr int BLOCK = apply_block<BD>();
+            constexpr int CH = LEN / 8; // 16-byte chunks
+            constexpr int PER = CH / BLOCK;
+            constexpr int LDW = N + 8; // padded row, 16-byte multiple
+            constexpr int HALF = N / 2;
+            static_assert(BLOCK == 2 * N && CH % BLOCK == 0 && BLOCK % 8 == 0, "thread layout");
+            using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
+
+            __shared__ __attribute__((aligned(16))) __half sF[N * LDW];
+            __shared__ __attribute__((aligned(16))) float sx[N];
+            __shared__ float part[N];
+
+            auto skip = [&](int m) {
+                if (bank_sys == nullptr)
+                    return false;
+                const int sy = bank_sys[m]; // -1: a coarse matrix (its nodes may span systems)
+                return sy >= 0 && active[sy] == 0;
+            };
+            const int end = mat_begin + count;
+            auto step = [&](int m) {
+                m += gridDim.x;
+                while (m < end && skip(m))
+                    m += gridDim.x;
+                return m;
+            };
+            int cur = mat_begin + static_cast<int>(blockIdx.x);
+            while (cur < end && skip(cur))
+                cur += gridDim.x;
+            if (cur >= end)
+                return;
+
+            const int tid = threadIdx.x;
+            // LDS targets of this thread's chunks (in halves).  Off-diagonal tile row r: ca = start
+            // of the row copy, cb = first 4-byte transposed target (row 8J + (r & 1), columns
+            // 8I + (r & ~1) .. +1), sel = byte selector pairing this lane's and its partner's
+            // values.  Folded block: ca = base + r LDW, cb = base + r (base = top-left corner).
+            int ca[PER], cb[PER], cr[PER];
+            unsigned sel[PER];
+            bool folded[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k)
+            {
+                const int c = tid + k * BLOCK;
+                const int blk = c >> 3, r = c & 7;
+                cr[k] = r;
+                // even row: own low half, partner low half; odd row: partner high, own high
+                sel[k] = (r & 1) ? 0x03020706u : 0x05040100u;
+                if (blk < Q)
+                {
+                    int I, J;
+                    off_tile_of_block<T>(blk, I, J);
+                    ca[k] = (8 * I + r) * LDW + 8 * J;
+                    cb[k] = (8 * J + (r & 1)) * LDW + 8 * I + (r & ~1);
+                    folded[k] = false;
+                }
+                else
+                {
+                    const int base = 16 * (blk - Q) * (LDW + 1);
+                    ca[k] = base + r * LDW;
+                    cb[k] = base + r;
+                    folded[k] = true;
+                }
+            }
+            for (int i = tid; i < N; i += BLOCK)
+                sF[i * LDW + i] = __float2half_rn(1.0f); // never overwritten below
+
+            auto load = [&](u32x4(&reg)[PER], float &xr, int mm) {
+                const u32x4 *src = reinterpret_cast<const u32x4 *>(mats + static_cast<size_t>(mm) * LEN);
+#pragma unroll
+                for (int k = 0; k < PER; ++k)
+                    reg[k] = __builtin_nontemporal_load(src + tid + k * BLOCK);
+                if (tid < N)
+                    xr = mr[static_cast<size_t>(mm) * N + tid] * dscale[static_cast<size_t>(mm) * N + tid];
+            };
+            auto unpack = [&](const u32x4(&reg)[PER], float xr) {
+#pragma unroll
+                for (int k = 0; k < PER; ++k)
+                {
+                    const u32x4 v = reg[k];
+                    // Folded blocks hold the last chunks: slots with (k + 1) BLOCK <= 8 Q have none.
+                    const bool fold = (k + 1) * BLOCK > 8 * Q && folded[k];
+                    if (!fold)
+                    {
+                        *reinterpret_cast<u32x4 *>(sF + ca[k]) = v;
+#pragma unroll
+                        for (int w = 0; w < 4; ++w)
+                        {
+                            // quad_perm [1, 0, 3, 2]: the partner lane (row r ^ 1 of the same tile)
+                            const unsigned pw = static_cast<unsigned>(__builtin_amdgcn_update_dpp(
+                                0, static_cast<int>(v[w]), 0xB1, 0xF, 0xF, false));
+                            *reinterpret_cast<unsigned *>(sF + cb[k] + 2 * w * LDW) =
+                                __builtin_amdgcn_perm(pw, v[w], sel[k]);
+                        }
+                    }
+                    else
+                    {
+                        __half h[8];
+                        __builtin_memcpy(h, &v, 16);
+                        const int r = cr[k];
+#pragma unroll
+                        for (int c = 0; c < 8; ++c)
+                        {
+                            const bool up = c > r; // element of the first tile, else of the second
+                            const int p1 = up ? ca[k] + c : cb[k] + (8 + c) * LDW + 8;
+                            const int p2 = up ? cb[k] + c * LDW : ca[k] + 8 * LDW + 8 + c;
+                            if (c != r)
+                            {
+                                sF[p1] = h[c];
+                                sF[p2] = h[c];
+                            }
+                        }
+                    }
+                }
+                if (tid < N)
+                    sx[tid] = xr;
+                __syncthreads();
+            };
+            const int row = tid % N, hf = tid / N;
+            auto compute = [&](int mm) {
+                const __half *arow = sF + row * LDW + hf * HALF;
+                const float *xs = sx + hf * HALF;
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < HALF; t += 8)
+                {
+                    const u32x4 av = *reinterpret_cast<const u32x4 *>(arow + t);
+                    __half h[8];
+                    __builtin_memcpy(h, &av, 16);
+                    const float4 x0 = *reinterpret_cast<const float4 *>(xs + t);
+                    const float4 x1 = *reinterpret_cast<const float4 *>(xs + t + 4);
+                    acc = fmaf(__half2float(h[0]), x0.x, acc);
+                    acc = fmaf(__half2float(h[1]), x0.y, acc);
+                    acc = fmaf(__half2float(h[2]), x0.z, acc);
+                    acc = fmaf(__half2float(h[3]), x0.w, acc);
+                    acc = fmaf(__half2float(h[4]), x1.x, acc);
+                    acc = fmaf(__half2float(h[5]), x1.y, acc);
+                    acc = fmaf(__half2float(h[6]), x1.z, acc);
+                    acc = fmaf(__half2float(h[7]), x1.w, acc);
+                }
+                if (hf == 1)
+                    part[row] = acc;
+                __syncthreads(); // also: sF and sx may be rewritten after this point
+                if (hf == 0)
+                    mz[static_cast<size_t>(mm) * N + row] = (acc + part[row]) * dscale[static_cast<size_t>(mm) * N + row];
+                // part[] is next written after the following unpack's barrier
+            };
+
+            // Invariant: regA holds matrix cur, regB matrix nxt (when nxt < end).
+            u32x4 regA[PER], regB[PER];
+            float xa = 0.0f, xb = 0.0f;
+            int nxt = step(cur);
+            load(regA, xa, cur);
+            if (nxt < end)
+                load(regB, xb, nxt);
+            while (true)
+            {
+                int nn = nxt < end ? step(nxt) : end;
+                unpack(regA, xa);
+                if (nn < end)
+                    load(regA, xa, nn);
+                compute(cur);
+                cur = nxt;
+                nxt = nn;
+                if (cur >= end)
+                    break;
+                nn = nxt < end ? step(nxt) : end;
+                unpack(regB, xb);
+                if (nn < end)
+                    load(regB, xb, nn);
+                compute(cur);
+                cur = nxt;
+                nxt = nn;
+                if (cur >= end)
+                    break;
+            }
+        }
+        /// y_m = Inv_m x_m for matrices [mat_begin, mat_begin + count), grid-stride: each workgroup
+        /// takes every gridDim.x-th matrix.  The packed upper matrix moves global -> registers ->
+        /// LDS; the next matrix's loads are issued before the current one is computed, so every
+        /// workgroup keeps a matrix in flight.  Threads (row, half) each sum half of row `row`,
+        /// reading element (t, row) for t < row (packed row t, contiguous across threads) and
+        /// (row, t) otherwise.
         template <int BD, typename TM, typename T>
-        __global__ __launch_bounds__(apply_block<BD>()) void k_mas_apply(int mat_begin, const TM *__restrict__ mats,
+        __global__ __launch_bounds__(apply_block<BD>()) void k_mas_apply(int mat_begin, int count,
+                                                                          const TM *__restrict__ mats,
                                                                           const float *__restrict__ dscale,
                                                                           const T *__restrict__ mr, T *__restrict__ mz,
                                                                           const int *__restrict__ bank_sys,
@@ -81,62 +340,92 @@ namespace psmas
             constexpr int BLOCK = apply_block<BD>();
             constexpr bool SCALED = sizeof(TM) == 2;
             using TA = typename std::conditional<sizeof(TM) == 8, double, float>::type;
+            using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
             static_assert((L * sizeof(TM)) % 16 == 0, "packed matrix must be a whole number of 16 B chunks");
+            constexpr int CH = static_cast<int>(L * sizeof(TM) / 16);
+            constexpr int PER = (CH + BLOCK - 1) / BLOCK; // 16-byte chunks per thread
 
             __shared__ __attribute__((aligned(16))) TM sA[L];
             __shared__ TA sx[N];
             __shared__ TA part[N];
 
-            const int m = mat_begin + blockIdx.x;
-            if (bank_sys != nullptr)
-            {
+            auto skip = [&](int m) {
+                if (bank_sys == nullptr)
+                    return false;
                 const int sy = bank_sys[m]; // -1: a coarse matrix (its nodes may span systems)
-                if (sy >= 0 && active[sy] == 0)
-                    return;
-            }
-            const TM *A = mats + static_cast<size_t>(m) * L;
-            {
-                using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
-                const u32x4 *src = reinterpret_cast<const u32x4 *>(A);
-                u32x4 *dst = reinterpret_cast<u32x4 *>(sA);
-                constexpr int CH = static_cast<int>(L * sizeof(TM) / 16);
-                for (int i = threadIdx.x; i < CH; i += BLOCK)
-                    dst[i] = __builtin_nontemporal_load(src + i);
-            }
-            for (int i = threadIdx.x; i < N; i += BLOCK)
-            {
-                TA v = static_cast<TA>(mr[static_cast<size_t>(m) * N + i]);
-                if (SCALED)
-                    v *= static_cast<TA>(dscale[static_cast<size_t>(m) * N + i]);
-                sx[i] = v;
-            }
-            __syncthreads();
-
+                return sy >= 0 && active[sy] == 0;
+            };
+            const int end = mat_begin + count;
+            int m = mat_begin + static_cast<int>(blockIdx.x);
+            while (m < end && skip(m))
+                m += gridDim.x;
+            if (m >= end)
+                return;
+            u32x4 reg[PER];
+            TA xr = TA(0);
+            auto load = [&](int mm) {
+                const u32x4 *src = reinterpret_cast<const u32x4 *>(mats + static_cast<size_t>(mm) * L);
+#pragma unroll
+                for (int k = 0; k < PER; ++k)
+                {
+                    const int i = threadIdx.x + k * BLOCK;
+                    if (i < CH)
+                        reg[k] = __builtin_nontemporal_load(src + i);
+                }
+                if (threadIdx.x < N)
+                {
+                    xr = static_cast<TA>(mr[static_cast<size_t>(mm) * N + threadIdx.x]);
+                    if (SCALED)
+                        xr *= static_cast<TA>(dscale[static_cast<size_t>(mm) * N + threadIdx.x]);
+                }
+            };
+            load(m);
             const int row = threadIdx.x % N;
             const int half = threadIdx.x / N; // 0 or 1
             const int t0 = half * (N / 2), t1 = t0 + N / 2;
             const int rs_row = row * N - (row * (row + 1)) / 2;
-            int rs_t = t0 * N - (t0 * (t0 + 1)) / 2;
-            TA acc = TA(0);
-            if (threadIdx.x < 2 * N)
+            while (m < end)
             {
-#pragma unroll 8
-                for (int t = t0; t < t1; ++t)
+                u32x4 *dst = reinterpret_cast<u32x4 *>(sA);
+#pragma unroll
+                for (int k = 0; k < PER; ++k)
                 {
-                    const int idx = (t < row) ? (rs_t + row) : (rs_row + t);
-                    acc = fma(to_float_or_double<TA>(sA[idx]), sx[t], acc);
-                    rs_t += N - t - 1;
+                    const int i = threadIdx.x + k * BLOCK;
+                    if (i < CH)
+                        dst[i] = reg[k];
                 }
-            }
-            if (half == 1 && threadIdx.x < 2 * N)
-                part[row] = acc;
-            __syncthreads();
-            if (half == 0)
-            {
-                TA y = acc + part[row];
-                if (SCALED)
-                    y *= static_cast<TA>(dscale[static_cast<size_t>(m) * N + row]);
-                mz[static_cast<size_t>(m) * N + row] = static_cast<T>(y);
+                if (threadIdx.x < N)
+                    sx[threadIdx.x] = xr;
+                __syncthreads();
+                int next = m + gridDim.x;
+                while (next < end && skip(next))
+                    next += gridDim.x;
+                if (next < end)
+                    load(next); // in flight while this matrix is computed
+                int rs_t = t0 * N - (t0 * (t0 + 1)) / 2;
+                TA acc = TA(0);
+                if (threadIdx.x < 2 * N)
+                {
+#pragma unroll 8
+                    for (int t = t0; t < t1; ++t)
+                    {
+                        const int idx = (t < row) ? (rs_t + row) : (rs_row + t);
+                        acc = fma(to_float_or_double<TA>(sA[idx]), sx[t], acc);
+                        rs_t += N - t - 1;
+                    }
+                }
+                if (half == 1 && threadIdx.x < 2 * N)
+                    part[row] = acc;
+                __syncthreads();
+                if (half == 0)
+                {
+                    TA y = acc + part[row];
+                    if (SCALED)
+                        y *= static_cast<TA>(dscale[static_cast<size_t>(m) * N + row]);
+                    mz[static_cast<size_t>(m) * N + row] = static_cast<T>(y);
+                }
+                __syncthreads(); // sA, sx and part are rewritten for the next matrix
+                m = next;
             }
         }
 
@@ -583,13 +872,15 @@ namespace psmas
         }
 
         /// FP16 form of packed inverses: Inv = D S D with D = diag(sqrt(Inv_ii)), so S has a unit
-        /// diagonal and |S_ij| <= 1; S is stored packed in FP16 and D in FP32.
+        /// diagonal and |S_ij| <= 1; D is stored in FP32 and S in FP16, tiled for k_mas_apply_h16
+        /// (half_tile_index gives the layout; N*N/2 values, the unit diagonal is implicit).
         template <int BD>
         __global__ __launch_bounds__(256) void k_make_half(const double *__restrict__ mats, __half *__restrict__ out,
                                                            float *__restrict__ dscale)
         {
             constexpr int N = 32 * BD;
             constexpr int L = N * (N + 1) / 2;
+            constexpr int LEN = half_matrix_len<BD>();
             __shared__ double d[N];
             const double *g = mats + static_cast<size_t>(blockIdx.x) * L;
             for (int i = threadIdx.x; i < N; i += 256)
@@ -599,20 +890,19 @@ namespace psmas
                 dscale[static_cast<size_t>(blockIdx.x) * N + i] = static_cast<float>(v);
             }
             __syncthreads();
-            __half *o = out + static_cast<size_t>(blockIdx.x) * L;
-            for (int e = threadIdx.x; e < L; e += 256)
+            __half *o = out + static_cast<size_t>(blockIdx.x) * LEN;
+            for (int e = threadIdx.x; e < LEN; e += 256)
             {
-                // Row of packed index e: largest i with i*N - i(i-1)/2 <= e (row starts decrease).
-                int i = 0;
-                int rs = 0;
-                while (i + 1 < N && rs + (N - i) <= e)
+                int i, j;
+                half_tile_element<BD>(e, i, j);
+                if (i == j)
                 {
-                    rs += N - i;
-                    ++i;
+                    o[e] = __float2half_rn(1.0f); // unused slot
+                    continue;
                 }
-                const int j = i + (e - rs);
                 const double den = d[i] * d[j];
-                o[e] = __float2half_rn(static_cast<float>(den > 0.0 ? g[e] / den : 0.0));
+                const double v = g[i * N - (i * (i + 1)) / 2 + j];
+                o[e] = __float2half_rn(static_cast<float>(den > 0.0 ? v / den : 0.0));
             }
         }
 
@@ -670,8 +960,32 @@ namespace psmas
     {
         if (count <= 0)
             return;
-        hipLaunchKernelGGL((k_mas_apply<BD, TM, T>), dim3(count), dim3(apply_block<BD>()), 0, s, mat_begin, mats,
-                           dscale, mr, mz, bank_sys, active);
+        // As many workgroups as fit on 256 CUs at once (LDS-limited, at most eight per CU); each
+        // walks its matrices grid-stride.
+        constexpr int N = 32 * BD;
+        if constexpr (std::is_same<TM, __half>::value)
+        {
+            // Exactly the workgroups that are resident at once (registers and LDS decide).
+            static const int resident = [] {
+                int per_cu = 0, cus = 0, dev = 0;
+                PSMAS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, reinterpret_cast<const void *>(&k_mas_apply_h16<BD>), apply_block<BD>(), 0));
+                PSMAS_HIP(hipGetDevice(&dev));
+                PSMAS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+                return (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+            }();
+            const int grid = count < resident ? count : resident;
+            hipLaunchKernelGGL((k_mas_apply_h16<BD>), dim3(grid), dim3(apply_block<BD>()), 0, s, mat_begin, count,
+                               mats, dscale, mr, mz, bank_sys, active);
+        }
+        else
+        {
+            constexpr int lds = N * (N + 1) / 2 * static_cast<int>(sizeof(TM)) + 2 * N * 8;
+            constexpr int per_cu = (160 * 1024 / lds) < 8 ? (160 * 1024 / lds) : 8;
+            const int grid = count < 256 * per_cu ? count : 256 * per_cu;
+            hipLaunchKernelGGL((k_mas_apply<BD, TM, T>), dim3(grid), dim3(apply_block<BD>()), 0, s, mat_begin, count,
+                               mats, dscale, mr, mz, bank_sys, active);
+        }
         PSMAS_LAUNCH_CHECK();
     }
 

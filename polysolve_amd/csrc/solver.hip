@@ -115,7 +115,29 @@ namespace psmas
         device_ = 0;
         PSMAS_HIP(hipSetDevice(device_));
         PSMAS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        for (int i = 0; i < 2; ++i)
+        {
+            PSMAS_HIP(hipStreamCreateWithFlags(&side_[i], hipStreamNonBlocking));
+            PSMAS_HIP(hipEventCreateWithFlags(&join_ev_[i], hipEventDisableTiming));
+        }
+        PSMAS_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
         sys_.resize(1);
+    }
+
+    void Solver::fork(int nside)
+    {
+        PSMAS_HIP(hipEventRecord(fork_ev_, stream_));
+        for (int i = 0; i < nside; ++i)
+            PSMAS_HIP(hipStreamWaitEvent(side_[i], fork_ev_, 0));
+    }
+
+    void Solver::join(int nside)
+    {
+        for (int i = 0; i < nside; ++i)
+        {
+            PSMAS_HIP(hipEventRecord(join_ev_[i], side_[i]));
+            PSMAS_HIP(hipStreamWaitEvent(stream_, join_ev_[i], 0));
+        }
     }
 
     Solver::~Solver()
@@ -123,9 +145,20 @@ namespace psmas
         (void)hipSetDevice(device_);
         if (stream_)
             (void)hipStreamSynchronize(stream_);
+        for (hipStream_t st : side_)
+            if (st)
+                (void)hipStreamSynchronize(st);
         graphs_.reset();
         for (hipEvent_t ev : event_pool_)
             (void)hipEventDestroy(ev);
+        for (hipEvent_t ev : join_ev_)
+            if (ev)
+                (void)hipEventDestroy(ev);
+        if (fork_ev_)
+            (void)hipEventDestroy(fork_ev_);
+        for (hipStream_t st : side_)
+            if (st)
+                (void)hipStreamDestroy(st);
         if (stream_)
             (void)hipStreamDestroy(stream_);
     }
@@ -746,12 +779,11 @@ namespace psmas
                                                d_vals32_.data(), stream_);
         }
         const int dim = BANK * bd_;
-        const size_t L = static_cast<size_t>(dim) * (dim + 1) / 2;
         if (!d_mats16_)
         {
             // Every level's inverses in scaled FP16 (S = D^-1 Inv D^-1, unit diagonal): one apply
             // launch covers all matrices.
-            d_mats16_.allocate(static_cast<size_t>(total_mats_) * L, "MAS bank inverses (FP16)");
+            d_mats16_.allocate(static_cast<size_t>(total_mats_) * (dim * dim / 2), "MAS bank inverses (FP16)");
             d_dscale_.allocate(static_cast<size_t>(total_mats_) * dim, "MAS bank scaling");
             launch_make_half(bd_, total_mats_, d_mats64_.data(), d_mats16_.data(), d_dscale_.data(), stream_);
         }
@@ -829,12 +861,12 @@ namespace psmas
     }
 
     template <int BD, typename T>
-    void Solver::apply_dense_coarse(const T *mr, T *mz)
+    void Solver::apply_dense_coarse(const T *mr, T *mz, hipStream_t s)
     {
         if (params_.coarse_solver != CoarseSolver::Cholesky || dense_nd_ == 0)
             return;
         const size_t off = static_cast<size_t>(lev_off_[level_num_]) * BD;
-        launch_dense_apply<BD, T>(dense_nd_, d_dense_inv_.data(), d_dense_ids_.data(), mr + off, mz + off, stream_);
+        launch_dense_apply<BD, T>(dense_nd_, d_dense_inv_.data(), d_dense_ids_.data(), mr + off, mz + off, s);
     }
 
     void Solver::check_ready_single(int n) const
@@ -962,15 +994,28 @@ namespace psmas
     template <int BD>
     void Solver::fp64_precondition(bool init)
     {
+        // Branch: coarse levels 2.. (restriction, their banks, the dense coarsest solve) beside the
+        // fine banks, which need only the fine residual.
+        const bool par = branches();
+        hipStream_t sc = par ? side_[0] : stream_;
+        const int nfine = mat_cnt_[0];
+        const double per_mat = (BANK * BD) * (BANK * BD + 1) / 2 * 8.0 + 2.0 * BANK * BD * 8.0;
+        if (par)
+            fork(1);
         tic("restrict_levels_fp64", static_cast<double>(cco_[0] + cco_[1] + cco_[2]) * BD * 8.0 * 2.0);
-        launch_restrict_coarse<BD, double>(coarse_restrict(), d_mr_.data(), stream_);
+        launch_restrict_coarse<BD, double>(coarse_restrict(), d_mr_.data(), sc);
         toc();
-        tic("mas_apply_fp64", static_cast<double>(total_mats_) * ((BANK * BD) * (BANK * BD + 1) / 2 * 8.0 +
-                                                                  2.0 * BANK * BD * 8.0));
-        launch_mas_apply<BD, double, double>(0, total_mats_, d_mats64_.data(), nullptr, d_mr_.data(), d_mz_.data(),
+        tic("mas_apply_fp64_coarse", static_cast<double>(total_mats_ - nfine) * per_mat);
+        launch_mas_apply<BD, double, double>(nfine, total_mats_ - nfine, d_mats64_.data(), nullptr, d_mr_.data(),
+                                             d_mz_.data(), nullptr, nullptr, sc);
+        toc();
+        apply_dense_coarse<BD, double>(d_mr_.data(), d_mz_.data(), sc);
+        tic("mas_apply_fp64", static_cast<double>(nfine) * per_mat);
+        launch_mas_apply<BD, double, double>(0, nfine, d_mats64_.data(), nullptr, d_mr_.data(), d_mz_.data(),
                                              nullptr, nullptr, stream_);
         toc();
-        apply_dense_coarse<BD, double>(d_mr_.data(), d_mz_.data());
+        if (par)
+            join(1);
         launch_prolong_dot<BD, double>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr_.data(),
                                        d_mz_.data(), reductions(), init, sysmap(), LccApply{}, FlexDot<double>{},
                                        stream_);
@@ -1107,10 +1152,27 @@ namespace psmas
     {
         const long long nvec = static_cast<long long>(npad_) * BD;
         const int dim = BANK * BD;
-        const double L = static_cast<double>(dim) * (dim + 1) / 2;
+        // Branches beside the fine banks (which need only the fine residual): coarse levels 2..
+        // (restriction, their banks, the dense coarsest solve) and the learned correction
+        // (lattice restriction and network), joined before the prolongation.
+        const bool par = branches();
+        const int nside = learned ? 2 : 1;
+        hipStream_t sc = par ? side_[0] : stream_;
+        hipStream_t sl = par ? side_[1] : stream_;
+        const int nfine = mat_cnt_[0];
+        const double per_mat = dim * dim * 1.0 + dim * 4.0 + 2.0 * dim * 4.0;
+        if (par)
+            fork(nside);
         tic("restrict_levels", static_cast<double>(cco_[0] + cco_[1] + cco_[2]) * BD * 4.0 * 2.0);
-        launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), stream_);
+        launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), sc);
         toc();
+        tic("mas_apply_fp16_coarse", (total_mats_ - nfine) * per_mat);
+        launch_mas_apply<BD, __half, float>(nfine, total_mats_ - nfine, d_mats16_.data(), d_dscale_.data(),
+                                            d_mr32_.data(), d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), sc);
+        toc();
+        apply_dense_coarse<BD, float>(d_mr32_.data(), d_mz32_.data(), sc);
+        LccApply lv;
+        FlexDot<float> flex;
         if (learned)
         {
             // Block path: per slot the lattice cell (16 B) and r, 2 B per list entry; entry path: per
@@ -1118,21 +1180,11 @@ namespace psmas
             tic("lcc_restrict", lcc_->block_restriction()
                                     ? static_cast<double>(npad_) * (16.0 + BD * 4.0) + lcc_->restrict_entries() * 2.0
                                     : static_cast<double>(lcc_->restrict_entries()) * (8.0 + BD * 4.0));
-            lcc_->restrict_to_lattice<BD, float>(d_mr32_.data(), d_active_.data(), stream_,
+            lcc_->restrict_to_lattice<BD, float>(d_mr32_.data(), d_active_.data(), sl,
                                                  static_cast<float>(params_.learned_output_scale));
             toc();
-        }
-        tic("mas_apply_fp16", total_mats_ * (L * 2.0 + dim * 4.0 + 2.0 * dim * 4.0));
-        launch_mas_apply<BD, __half, float>(0, total_mats_, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
-                                            d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_);
-        toc();
-        apply_dense_coarse<BD, float>(d_mr32_.data(), d_mz32_.data());
-        LccApply lv;
-        FlexDot<float> flex;
-        if (learned)
-        {
             tic("lcc_infer", static_cast<double>(lcc_->inference_weight_bytes()));
-            lcc_->infer(d_active_.data(), stream_);
+            lcc_->infer(d_active_.data(), sl);
             toc();
             lv = lcc_->apply_view();
             if (!init)
@@ -1141,6 +1193,12 @@ namespace psmas
                 flex.rzo = d_rzo_.data();
             }
         }
+        tic("mas_apply_fp16", nfine * per_mat);
+        launch_mas_apply<BD, __half, float>(0, nfine, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
+                                            d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_);
+        toc();
+        if (par)
+            join(nside);
         tic(learned ? "prolong_dot_learned" : "prolong_dot", static_cast<double>(n_total_) * 4.0 * 3 + nblocks_ * 256.0 * 16);
         launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
                                       d_mz32_.data(), reductions(), init, sysmap(), lv, flex, stream_);

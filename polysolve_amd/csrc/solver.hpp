@@ -123,6 +123,12 @@ namespace psmas
         double solve_ms_ = 0.0;
 
         hipStream_t stream_ = nullptr;
+        /// Side streams for the branches of the preconditioner (coarse levels, learned correction):
+        /// forked from and joined back into stream_ by events, so graph capture records them as
+        /// parallel branches.  Default priority, like stream_.
+        hipStream_t side_[2] = {nullptr, nullptr};
+        hipEvent_t fork_ev_ = nullptr;
+        hipEvent_t join_ev_[2] = {nullptr, nullptr};
         int device_ = 0;
 
         int bd_ = 0, npad_ = 0, nparts_ = 0, nblocks_ = 0;
@@ -209,7 +215,12 @@ namespace psmas
         void ensure_mixed_data();
         void ensure_coarse_solver();
         template <int BD, typename T>
-        void apply_dense_coarse(const T *mr, T *mz);
+        void apply_dense_coarse(const T *mr, T *mz, hipStream_t s);
+        /// Branch streams of the preconditioner: side streams unless kernels are being timed
+        /// (then everything runs in order on stream_ so per-kernel events stay meaningful).
+        bool branches() const { return !timing_on_; }
+        void fork(int nside);
+        void join(int nside);
         void allocate_vectors();
         void check_ready_single(int n) const;
 

@@ -122,6 +122,13 @@ def test_apply_mixed_close_to_fp64(gpu, het12):
     assert np.linalg.norm(z16 - z64) <= 5e-3 * np.linalg.norm(z64)
     # Still a symmetric positive operator on this vector.
     assert r @ z16 > 0
+    # Symmetric to FP32 rounding: every off-diagonal FP16 entry lands at (i, j) and (j, i).
+    v = rng.standard_normal(het12.n)
+    zv = s.apply_preconditioner(v, mixed=True)
+    assert abs(v @ z16 - r @ zv) <= 1e-5 * np.linalg.norm(v) * np.linalg.norm(z16)
+    # Linear in the input (no state kept between calls).
+    zs = s.apply_preconditioner(r + 2.0 * v, mixed=True)
+    assert np.linalg.norm(zs - (z16 + 2.0 * zv)) <= 1e-5 * np.linalg.norm(zs)
 
 
 def test_apply_matches_dense_construction(gpu):
