@@ -184,9 +184,47 @@ namespace psmas
     /// Storage: packed upper row-major.  Half storage is scaled: Inv = D S D with D per row.
     /// bank_sys/active (may be null): skip matrices of systems that stopped iterating
     /// (bank_sys[m] = system of matrix m, -1 for coarse matrices, which are never skipped).
+    /// Longest descendant list of a restriction task that takes one thread; longer lists are cut
+    /// into pieces of at most RESTRICT_PIECE entries that take a wave each.
+    constexpr int RESTRICT_SMALL_MAX = 8;
+    constexpr int RESTRICT_PIECE = 512;
+
+    /// Work done by the leading `wgs` workgroups of an apply launch beside the bank matrices (the
+    /// other workgroups apply the matrices).  Either part may be empty.
+    struct ApplySide
+    {
+        int wgs = 0;
+        // Restriction of coarse segments 2.. from segment 1: task i sums the segment-1 values of
+        // rlist[rtask[i].y .. rtask[i].z) (ascending node ids) in FP64.  Tasks [0, nsmall) take a
+        // thread each, [nsmall, nsmall + nbig) a wave each (lanes stride the list, fixed-order
+        // reduction).  rtask[i].w < 0: the sum is multi-level node rtask[i].x's value (rounded to
+        // the vector type, written to out); otherwise it is partial rtask[i].w of a node cut into
+        // pieces (written to rpart, combined by launch_restrict_combine).
+        int nsmall = 0, nbig = 0;
+        const int4 *rtask = nullptr;
+        const int *rlist = nullptr;
+        const void *seg1 = nullptr; ///< segment-1 values, of the launch's vector type
+        void *out = nullptr;        ///< multi-level vector the sums go to
+        double *rpart = nullptr;    ///< [pieces][bd]
+        // Dense coarsest solve y = Inv x, one wave per row; its nodes are coarsest-segment nodes
+        // dense_seg + dense_ids[i].  Bank outputs of nodes with dense_index[n - dense_seg] >= 0
+        // (n - dense_seg < dense_count) are left to it.
+        int dense_nd = 0;
+        const double *dense_inv = nullptr;
+        const int *dense_ids = nullptr;
+        int dense_seg = 0;
+        const int *dense_index = nullptr;
+        int dense_count = 0;
+    };
+
     template <int BD, typename TM, typename T>
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
-                          const int *bank_sys, const int *active, hipStream_t s);
+                          const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side = ApplySide{});
+
+    /// out[comb[i].x] = sum of rpart[comb[i].y .. comb[i].y + comb[i].z) for i < n (nodes whose
+    /// restriction was cut into pieces), in piece order.
+    template <int BD, typename T>
+    void launch_restrict_combine(int n, const int4 *comb, const double *rpart, T *out, hipStream_t s);
 
     /// z_p = mz_p + sum_l mz_l[cmap_p.l] (+ the lattice correction P c when lcc.cell is set);
     /// partials of r . z -> red.rz (the previous ones move to red.rz_old unless init) and, when

@@ -68,6 +68,113 @@ namespace psmas
         template <int BD>
         constexpr int apply_block() { return BD == 1 ? 64 : (BD == 2 ? 128 : 192); }
 
+        /// True when the dense coarsest solve owns node `node` (its bank output is not stored).
+        __device__ __forceinline__ bool dense_owned(const ApplySide &sd, int node)
+        {
+            if (sd.dense_index == nullptr)
+                return false;
+            const int rel = node - sd.dense_seg;
+            return rel >= 0 && rel < sd.dense_count && sd.dense_index[rel] >= 0;
+        }
+
+        /// Side work of workgroup `wg` < sd.wgs: restriction tasks (thread per small node, wave per
+        /// large node), then dense coarsest rows (wave per row, lanes over columns).
+        template <int BD, typename T>
+        __device__ void apply_side_work(const ApplySide &sd, const T *__restrict__ mr, T *__restrict__ mz, int wg)
+        {
+            const int nt = sd.wgs * static_cast<int>(blockDim.x);
+            const int t = wg * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+            const int lane = threadIdx.x & 63;
+            if (sd.nsmall + sd.nbig > 0)
+            {
+                const T *seg1 = static_cast<const T *>(sd.seg1);
+                T *out = static_cast<T *>(sd.out);
+                // A small node's list (at most RESTRICT_SMALL_MAX entries) is loaded in one batch,
+                // then its values in one batch: two memory round trips per node.
+                for (int i = t; i < sd.nsmall; i += nt)
+                {
+                    const int4 task = sd.rtask[i];
+                    int q[RESTRICT_SMALL_MAX];
+#pragma unroll
+                    for (int k = 0; k < RESTRICT_SMALL_MAX; ++k)
+                        q[k] = task.y + k < task.z ? sd.rlist[task.y + k] : -1;
+                    T v[RESTRICT_SMALL_MAX][BD];
+#pragma unroll
+                    for (int k = 0; k < RESTRICT_SMALL_MAX; ++k)
+#pragma unroll
+                        for (int d = 0; d < BD; ++d)
+                            v[k][d] = q[k] >= 0 ? seg1[static_cast<size_t>(q[k]) * BD + d] : T(0);
+#pragma unroll
+                    for (int d = 0; d < BD; ++d)
+                    {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int k = 0; k < RESTRICT_SMALL_MAX; ++k)
+                            if (q[k] >= 0)
+                                acc += static_cast<double>(v[k][d]);
+                        out[static_cast<size_t>(task.x) * BD + d] = static_cast<T>(acc);
+                    }
+                }
+                // A piece (at most RESTRICT_PIECE entries) per wave: each lane loads its entries
+                // in one batch, then their values in one batch.
+                constexpr int PL = RESTRICT_PIECE / 64;
+                for (int i = t >> 6; i < sd.nbig; i += nt >> 6)
+                {
+                    const int4 task = sd.rtask[sd.nsmall + i];
+                    int q[PL];
+#pragma unroll
+                    for (int k = 0; k < PL; ++k)
+                    {
+                        const int e = task.y + lane + 64 * k;
+                        q[k] = e < task.z ? sd.rlist[e] : -1;
+                    }
+                    T v[PL][BD];
+#pragma unroll
+                    for (int k = 0; k < PL; ++k)
+#pragma unroll
+                        for (int d = 0; d < BD; ++d)
+                            v[k][d] = q[k] >= 0 ? seg1[static_cast<size_t>(q[k]) * BD + d] : T(0);
+#pragma unroll
+                    for (int d = 0; d < BD; ++d)
+                    {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int k = 0; k < PL; ++k)
+                            if (q[k] >= 0)
+                                acc += static_cast<double>(v[k][d]);
+                        acc = wave_sum(acc);
+                        if (lane == 0)
+                        {
+                            if (task.w < 0)
+                                out[static_cast<size_t>(task.x) * BD + d] = static_cast<T>(acc);
+                            else
+                                sd.rpart[static_cast<size_t>(task.w) * BD + d] = acc;
+                        }
+                    }
+                }
+            }
+            if (sd.dense_nd > 0)
+            {
+                const int nd = sd.dense_nd;
+                for (int row = t >> 6; row < nd; row += nt >> 6)
+                {
+                    const double *a = sd.dense_inv + static_cast<size_t>(row) * nd;
+                    double acc = 0.0;
+                    for (int c = lane; c < nd; c += 64)
+                    {
+                        const int node = sd.dense_seg + sd.dense_ids[c / BD];
+                        acc += a[c] * static_cast<double>(mr[static_cast<size_t>(node) * BD + c % BD]);
+                    }
+                    acc = wave_sum(acc);
+                    if (lane == 0)
+                    {
+                        const int node = sd.dense_seg + sd.dense_ids[row / BD];
+                        mz[static_cast<size_t>(node) * BD + row % BD] = static_cast<T>(acc);
+                    }
+                }
+            }
+        }
+
         // FP16 tiled storage of a scaled bank matrix S (N = 32 BD, T = N / 8 tiles of 8 x 8 per
         // side, unit diagonal implicit), N*N/2 values in blocks of 64:
         //   block q < Q = T(T-1)/2: off-diagonal tile (I, J), I < J, row-major; tiles ordered by
@@ -139,7 +246,8 @@ namespace psmas
                                                                               const float *__restrict__ mr,
                                                                               float *__restrict__ mz,
                                                                               const int *__restrict__ bank_sys,
-                                                                              const int *__restrict__ active)
+                                                                              const int *__restrict__ active,
+                                                                              ApplySide sd)
         {
             constexpr int N = 32 * BD;
             constexpr int T = N / 8;
@@ -163,16 +271,22 @@ namespace psmas
                 const int sy = bank_sys[m]; // -1: a coarse matrix (its nodes may span systems)
                 return sy >= 0 && active[sy] == 0;
             };
+            if (static_cast<int>(blockIdx.x) < sd.wgs)
+            {
+                apply_side_work<BD, float>(sd, mr, mz, blockIdx.x);
+                return;
+            }
+            const int nwg = static_cast<int>(gridDim.x) - sd.wgs; // apply workgroups
             const int end = mat_begin + count;
             auto step = [&](int m) {
-                m += gridDim.x;
+                m += nwg;
                 while (m < end && skip(m))
-                    m += gridDim.x;
+                    m += nwg;
                 return m;
             };
-            int cur = mat_begin + static_cast<int>(blockIdx.x);
+            int cur = mat_begin + static_cast<int>(blockIdx.x) - sd.wgs;
             while (cur < end && skip(cur))
-                cur += gridDim.x;
+                cur += nwg;
             if (cur >= end)
                 return;
 
@@ -287,7 +401,7 @@ namespace psmas
                 if (hf == 1)
                     part[row] = acc;
                 __syncthreads(); // also: sF and sx may be rewritten after this point
-                if (hf == 0)
+                if (hf == 0 && !dense_owned(sd, mm * 32 + row / BD))
                     mz[static_cast<size_t>(mm) * N + row] = (acc + part[row]) * dscale[static_cast<size_t>(mm) * N + row];
                 // part[] is next written after the following unpack's barrier
             };
@@ -333,7 +447,8 @@ namespace psmas
                                                                           const float *__restrict__ dscale,
                                                                           const T *__restrict__ mr, T *__restrict__ mz,
                                                                           const int *__restrict__ bank_sys,
-                                                                          const int *__restrict__ active)
+                                                                          const int *__restrict__ active,
+                                                                          ApplySide sd)
         {
             constexpr int N = 32 * BD;
             constexpr int L = N * (N + 1) / 2;
@@ -355,10 +470,16 @@ namespace psmas
                 const int sy = bank_sys[m]; // -1: a coarse matrix (its nodes may span systems)
                 return sy >= 0 && active[sy] == 0;
             };
+            if (static_cast<int>(blockIdx.x) < sd.wgs)
+            {
+                apply_side_work<BD, T>(sd, mr, mz, blockIdx.x);
+                return;
+            }
+            const int nwg = static_cast<int>(gridDim.x) - sd.wgs; // apply workgroups
             const int end = mat_begin + count;
-            int m = mat_begin + static_cast<int>(blockIdx.x);
+            int m = mat_begin + static_cast<int>(blockIdx.x) - sd.wgs;
             while (m < end && skip(m))
-                m += gridDim.x;
+                m += nwg;
             if (m >= end)
                 return;
             u32x4 reg[PER];
@@ -397,9 +518,9 @@ namespace psmas
                 if (threadIdx.x < N)
                     sx[threadIdx.x] = xr;
                 __syncthreads();
-                int next = m + gridDim.x;
+                int next = m + nwg;
                 while (next < end && skip(next))
-                    next += gridDim.x;
+                    next += nwg;
                 if (next < end)
                     load(next); // in flight while this matrix is computed
                 int rs_t = t0 * N - (t0 * (t0 + 1)) / 2;
@@ -417,7 +538,7 @@ namespace psmas
                 if (half == 1 && threadIdx.x < 2 * N)
                     part[row] = acc;
                 __syncthreads();
-                if (half == 0)
+                if (half == 0 && !dense_owned(sd, m * 32 + row / BD))
                 {
                     TA y = acc + part[row];
                     if (SCALED)
@@ -427,6 +548,22 @@ namespace psmas
                 __syncthreads(); // sA, sx and part are rewritten for the next matrix
                 m = next;
             }
+        }
+
+        template <int BD, typename T>
+        __global__ __launch_bounds__(256) void k_restrict_combine(int n, const int4 *__restrict__ comb,
+                                                                  const double *__restrict__ rpart,
+                                                                  T *__restrict__ out)
+        {
+            const int i = blockIdx.x * 256 + threadIdx.x;
+            if (i >= n * BD)
+                return;
+            const int4 c = comb[i / BD];
+            const int d = i % BD;
+            double acc = 0.0;
+            for (int k = 0; k < c.z; ++k)
+                acc += rpart[static_cast<size_t>(c.y + k) * BD + d];
+            out[static_cast<size_t>(c.x) * BD + d] = static_cast<T>(acc);
         }
 
         // ----------------------------------------------------------------------- prolongation
@@ -956,36 +1093,47 @@ namespace psmas
 
     template <int BD, typename TM, typename T>
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
-                          const int *bank_sys, const int *active, hipStream_t s)
+                          const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side)
     {
-        if (count <= 0)
+        if (count <= 0 && side.wgs <= 0)
             return;
-        // As many workgroups as fit on 256 CUs at once (LDS-limited, at most eight per CU); each
-        // walks its matrices grid-stride.
-        constexpr int N = 32 * BD;
+        // Apply workgroups: as many as are resident at once beside the side workgroups (registers
+        // and LDS decide); each walks its matrices grid-stride.  Side workgroups come first so the
+        // dispatcher starts them at once.
+        auto resident_of = [](const void *kernel) {
+            int per_cu = 0, cus = 0, dev = 0;
+            PSMAS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, apply_block<BD>(), 0));
+            PSMAS_HIP(hipGetDevice(&dev));
+            PSMAS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            return (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+        };
+        auto grid_for = [&](int resident) {
+            const int avail = resident - side.wgs > 1 ? resident - side.wgs : 1;
+            const int apply_wgs = count <= 0 ? 0 : (count < avail ? count : avail);
+            return side.wgs + apply_wgs;
+        };
         if constexpr (std::is_same<TM, __half>::value)
         {
-            // Exactly the workgroups that are resident at once (registers and LDS decide).
-            static const int resident = [] {
-                int per_cu = 0, cus = 0, dev = 0;
-                PSMAS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, reinterpret_cast<const void *>(&k_mas_apply_h16<BD>), apply_block<BD>(), 0));
-                PSMAS_HIP(hipGetDevice(&dev));
-                PSMAS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-                return (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
-            }();
-            const int grid = count < resident ? count : resident;
-            hipLaunchKernelGGL((k_mas_apply_h16<BD>), dim3(grid), dim3(apply_block<BD>()), 0, s, mat_begin, count,
-                               mats, dscale, mr, mz, bank_sys, active);
+            static const int resident = resident_of(reinterpret_cast<const void *>(&k_mas_apply_h16<BD>));
+            hipLaunchKernelGGL((k_mas_apply_h16<BD>), dim3(grid_for(resident)), dim3(apply_block<BD>()), 0, s,
+                               mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side);
         }
         else
         {
-            constexpr int lds = N * (N + 1) / 2 * static_cast<int>(sizeof(TM)) + 2 * N * 8;
-            constexpr int per_cu = (160 * 1024 / lds) < 8 ? (160 * 1024 / lds) : 8;
-            const int grid = count < 256 * per_cu ? count : 256 * per_cu;
-            hipLaunchKernelGGL((k_mas_apply<BD, TM, T>), dim3(grid), dim3(apply_block<BD>()), 0, s, mat_begin, count,
-                               mats, dscale, mr, mz, bank_sys, active);
+            static const int resident = resident_of(reinterpret_cast<const void *>(&k_mas_apply<BD, TM, T>));
+            hipLaunchKernelGGL((k_mas_apply<BD, TM, T>), dim3(grid_for(resident)), dim3(apply_block<BD>()), 0, s,
+                               mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side);
         }
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD, typename T>
+    void launch_restrict_combine(int n, const int4 *comb, const double *rpart, T *out, hipStream_t s)
+    {
+        if (n <= 0)
+            return;
+        hipLaunchKernelGGL((k_restrict_combine<BD, T>), dim3(div_up(n * BD, 256)), dim3(256), 0, s, n, comb, rpart,
+                           out);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -1113,11 +1261,16 @@ namespace psmas
     template void launch_restrict_coarse<BD, double>(const CoarseRestrict &, double *, hipStream_t);             \
     template void launch_restrict_coarse<BD, float>(const CoarseRestrict &, float *, hipStream_t);               \
     template void launch_mas_apply<BD, double, double>(int, int, const double *, const float *, const double *,   \
-                                                       double *, const int *, const int *, hipStream_t);         \
+                                                       double *, const int *, const int *, hipStream_t,          \
+                                                       const ApplySide &);                                       \
     template void launch_mas_apply<BD, double, float>(int, int, const double *, const float *, const float *,     \
-                                                      float *, const int *, const int *, hipStream_t);           \
+                                                      float *, const int *, const int *, hipStream_t,            \
+                                                      const ApplySide &);                                        \
     template void launch_mas_apply<BD, __half, float>(int, int, const __half *, const float *, const float *,     \
-                                                      float *, const int *, const int *, hipStream_t);           \
+                                                      float *, const int *, const int *, hipStream_t,            \
+                                                      const ApplySide &);                                        \
+    template void launch_restrict_combine<BD, double>(int, const int4 *, const double *, double *, hipStream_t);   \
+    template void launch_restrict_combine<BD, float>(int, const int4 *, const double *, float *, hipStream_t);     \
     template void launch_prolong_dot<BD, double>(int, int, const int4 *, const int *, const double *, double *,   \
                                                  const Reductions &, bool, const SysMap &, const LccApply &,     \
                                                  const FlexDot<double> &, hipStream_t);                          \

@@ -475,6 +475,7 @@ namespace psmas
         dense_note_.clear();
         d_dense_inv_.reset();
         d_dense_ids_.reset();
+        d_dense_index_.reset();
 
         upload_matrix(all);
         all = PaddedBsr();
@@ -723,6 +724,7 @@ namespace psmas
         d_mat_off_.allocate(MAX_COARSE_LEVELS + 1, "MAS matrix offsets");
         PSMAS_HIP(hipMemcpyAsync(d_lev_off_.data(), lev_off_, sizeof(lev_off_), hipMemcpyHostToDevice, stream_));
         PSMAS_HIP(hipMemcpyAsync(d_mat_off_.data(), mat_off_, sizeof(mat_off_), hipMemcpyHostToDevice, stream_));
+        build_restrict_chunks();
 
         const int dim = BANK * bd_;
         const size_t L = static_cast<size_t>(dim) * (dim + 1) / 2;
@@ -738,6 +740,121 @@ namespace psmas
         PSMAS_HIP(hipStreamSynchronize(stream_));
         if (h_fail)
             throw std::runtime_error("[MAS] MAS packed inverse failed.");
+    }
+
+    void Solver::build_restrict_chunks()
+    {
+        nrsmall_ = nrbig_ = nrcomb_ = 0;
+        d_rtask_.reset();
+        d_rlist_.reset();
+        d_rcomb_.reset();
+        d_rpart_.reset();
+        if (level_num_ < 2)
+            return;
+        // Ancestors of every segment-1 node in segments 2.. (the same for all its fine slots).
+        std::vector<int4> cm(npad_);
+        PSMAS_HIP(hipMemcpyAsync(cm.data(), d_cmap_.data(), sizeof(int4) * npad_, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        const int n1 = cco_[0];
+        std::vector<int4> anc(n1, int4{-1, -1, -1, -1});
+        for (int p = 0; p < npad_; ++p)
+            if (cm[p].x >= 0)
+                anc[cm[p].x] = cm[p];
+        const int node0 = lev_off_[2];
+        const int nrel = total_level_nodes_ - node0;
+        auto ancestor = [&](int q, int seg) { // seg >= 2
+            const int4 &a = anc[q];
+            return seg == 2 ? a.y : (seg == 3 ? a.z : a.w);
+        };
+        std::vector<int> eoff(static_cast<size_t>(nrel) + 1, 0);
+        for (int seg = 2; seg <= level_num_; ++seg)
+            for (int q = 0; q < n1; ++q)
+            {
+                const int a = ancestor(q, seg);
+                if (a >= 0)
+                    ++eoff[static_cast<size_t>(lev_off_[seg] - node0 + a) + 1];
+            }
+        for (int i = 0; i < nrel; ++i)
+            eoff[i + 1] += eoff[i];
+        std::vector<int> list(eoff[nrel]);
+        std::vector<int> fill(eoff.begin(), eoff.end() - 1);
+        for (int seg = 2; seg <= level_num_; ++seg)
+            for (int q = 0; q < n1; ++q) // ascending: each node's list is sorted
+            {
+                const int a = ancestor(q, seg);
+                if (a >= 0)
+                    list[fill[lev_off_[seg] - node0 + a]++] = q;
+            }
+        // Every node of segments 2.. gets a task (padding nodes: an empty list, i.e. zero); long
+        // lists are cut into pieces whose sums a combine step adds.
+        std::vector<int4> small, big, comb;
+        int nparts = 0;
+        for (int i = 0; i < nrel; ++i)
+        {
+            const int b = eoff[i], e = eoff[i + 1];
+            if (e - b <= RESTRICT_SMALL_MAX)
+                small.push_back(int4{node0 + i, b, e, -1});
+            else if (e - b <= RESTRICT_PIECE)
+                big.push_back(int4{node0 + i, b, e, -1});
+            else
+            {
+                const int first = nparts;
+                for (int k = b; k < e; k += RESTRICT_PIECE)
+                    big.push_back(int4{node0 + i, k, std::min(k + RESTRICT_PIECE, e), nparts++});
+                comb.push_back(int4{node0 + i, first, nparts - first, 0});
+            }
+        }
+        nrsmall_ = static_cast<int>(small.size());
+        nrbig_ = static_cast<int>(big.size());
+        nrcomb_ = static_cast<int>(comb.size());
+        if (nrcomb_ > 0)
+        {
+            d_rcomb_.allocate(comb.size(), "MAS restriction combines");
+            PSMAS_HIP(hipMemcpy(d_rcomb_.data(), comb.data(), d_rcomb_.bytes(), hipMemcpyHostToDevice));
+            d_rpart_.allocate(static_cast<size_t>(nparts) * bd_, "MAS restriction piece sums");
+        }
+        small.insert(small.end(), big.begin(), big.end());
+        d_rtask_.allocate(small.size(), "MAS restriction tasks");
+        PSMAS_HIP(hipMemcpy(d_rtask_.data(), small.data(), d_rtask_.bytes(), hipMemcpyHostToDevice));
+        if (!list.empty())
+        {
+            d_rlist_.allocate(list.size(), "MAS restriction lists");
+            PSMAS_HIP(hipMemcpy(d_rlist_.data(), list.data(), d_rlist_.bytes(), hipMemcpyHostToDevice));
+        }
+    }
+
+    ApplySide Solver::restrict_side(const void *seg1, void *out, int block) const
+    {
+        ApplySide sd;
+        if (nrsmall_ + nrbig_ == 0)
+            return sd;
+        sd.nsmall = nrsmall_;
+        sd.nbig = nrbig_;
+        sd.rtask = d_rtask_.data();
+        sd.rlist = d_rlist_.data();
+        sd.seg1 = seg1;
+        sd.out = out;
+        sd.rpart = const_cast<double *>(d_rpart_.data());
+        // A thread per small node and a wave per large node, at most 256 workgroups.
+        const int want = std::max(div_up(nrsmall_, block), div_up(nrbig_, block / 64));
+        sd.wgs = std::min(256, std::max(1, want));
+        return sd;
+    }
+
+    ApplySide Solver::coarse_side(int block) const
+    {
+        ApplySide sd;
+        if (params_.coarse_solver == CoarseSolver::Cholesky && dense_nd_ > 0)
+        {
+            sd.dense_nd = dense_nd_;
+            sd.dense_inv = d_dense_inv_.data();
+            sd.dense_ids = d_dense_ids_.data();
+            sd.dense_seg = lev_off_[level_num_];
+            sd.dense_index = d_dense_index_.data();
+            sd.dense_count = cco_[level_num_ - 1];
+            sd.wgs = std::min(256, div_up(dense_nd_, block / 64)); // wave per row
+        }
+        return sd;
     }
 
     void Solver::allocate_vectors()
@@ -836,6 +953,8 @@ namespace psmas
         const int nd = bd_ * dense_nodes_;
         DeviceBuffer<int> d_index(count, "MAS coarsest active index");
         PSMAS_HIP(hipMemcpy(d_index.data(), active_index.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+        d_dense_index_.allocate(count, "MAS coarsest dense index");
+        PSMAS_HIP(hipMemcpy(d_dense_index_.data(), active_index.data(), sizeof(int) * count, hipMemcpyHostToDevice));
         DeviceBuffer<double> dense(static_cast<size_t>(nd) * nd, "MAS coarsest Galerkin matrix");
         dense.zero(stream_);
         launch_dense_galerkin(bd_, npad_, level, sell(d_vals64_.data()), d_row_len_.data(), d_block_of_slot_.data(),
@@ -858,15 +977,6 @@ namespace psmas
         PSMAS_HIP(hipMemcpy(d_dense_ids_.data(), ids.data(), d_dense_ids_.bytes(), hipMemcpyHostToDevice));
         dense_nd_ = nd;
         dense_note_ = "dense FP64 Cholesky on " + std::to_string(dense_nodes_) + " coarsest nodes";
-    }
-
-    template <int BD, typename T>
-    void Solver::apply_dense_coarse(const T *mr, T *mz, hipStream_t s)
-    {
-        if (params_.coarse_solver != CoarseSolver::Cholesky || dense_nd_ == 0)
-            return;
-        const size_t off = static_cast<size_t>(lev_off_[level_num_]) * BD;
-        launch_dense_apply<BD, T>(dense_nd_, d_dense_inv_.data(), d_dense_ids_.data(), mr + off, mz + off, s);
     }
 
     void Solver::check_ready_single(int n) const
@@ -994,28 +1104,28 @@ namespace psmas
     template <int BD>
     void Solver::fp64_precondition(bool init)
     {
-        // Branch: coarse levels 2.. (restriction, their banks, the dense coarsest solve) beside the
-        // fine banks, which need only the fine residual.
-        const bool par = branches();
-        hipStream_t sc = par ? side_[0] : stream_;
-        const int nfine = mat_cnt_[0];
+        // Launch 1: fine and segment-1 banks, with the restriction of segments 2.. from segment 1
+        // in its leading workgroups; launch 2: banks of segments 2.. (inputs from the restriction
+        // sums) and the dense coarsest solve.
+        constexpr int block = 64 * BD;
+        const int n1 = first_apply_mats();
         const double per_mat = (BANK * BD) * (BANK * BD + 1) / 2 * 8.0 + 2.0 * BANK * BD * 8.0;
-        if (par)
-            fork(1);
-        tic("restrict_levels_fp64", static_cast<double>(cco_[0] + cco_[1] + cco_[2]) * BD * 8.0 * 2.0);
-        launch_restrict_coarse<BD, double>(coarse_restrict(), d_mr_.data(), sc);
+        tic("mas_apply_fp64", static_cast<double>(n1) * per_mat);
+        launch_mas_apply<BD, double, double>(0, n1, d_mats64_.data(), nullptr, d_mr_.data(), d_mz_.data(), nullptr,
+                                             nullptr, stream_,
+                                             restrict_side(d_mr_.data() + static_cast<size_t>(lev_off_[1]) * BD,
+                                                           d_mr_.data(), block));
+        launch_restrict_combine<BD, double>(nrcomb_, d_rcomb_.data(), d_rpart_.data(), d_mr_.data(), stream_);
         toc();
-        tic("mas_apply_fp64_coarse", static_cast<double>(total_mats_ - nfine) * per_mat);
-        launch_mas_apply<BD, double, double>(nfine, total_mats_ - nfine, d_mats64_.data(), nullptr, d_mr_.data(),
-                                             d_mz_.data(), nullptr, nullptr, sc);
-        toc();
-        apply_dense_coarse<BD, double>(d_mr_.data(), d_mz_.data(), sc);
-        tic("mas_apply_fp64", static_cast<double>(nfine) * per_mat);
-        launch_mas_apply<BD, double, double>(0, nfine, d_mats64_.data(), nullptr, d_mr_.data(), d_mz_.data(),
-                                             nullptr, nullptr, stream_);
-        toc();
-        if (par)
-            join(1);
+        const ApplySide cs = coarse_side(block);
+        if (total_mats_ > n1 || cs.wgs > 0)
+        {
+            tic("mas_apply_fp64_coarse", static_cast<double>(total_mats_ - n1) * per_mat +
+                                             static_cast<double>(dense_nd_) * dense_nd_ * 8.0);
+            launch_mas_apply<BD, double, double>(n1, total_mats_ - n1, d_mats64_.data(), nullptr, d_mr_.data(),
+                                                 d_mz_.data(), nullptr, nullptr, stream_, cs);
+            toc();
+        }
         launch_prolong_dot<BD, double>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr_.data(),
                                        d_mz_.data(), reductions(), init, sysmap(), LccApply{}, FlexDot<double>{},
                                        stream_);
@@ -1152,25 +1262,17 @@ namespace psmas
     {
         const long long nvec = static_cast<long long>(npad_) * BD;
         const int dim = BANK * BD;
-        // Branches beside the fine banks (which need only the fine residual): coarse levels 2..
-        // (restriction, their banks, the dense coarsest solve) and the learned correction
-        // (lattice restriction and network), joined before the prolongation.
-        const bool par = branches();
-        const int nside = learned ? 2 : 1;
-        hipStream_t sc = par ? side_[0] : stream_;
-        hipStream_t sl = par ? side_[1] : stream_;
-        const int nfine = mat_cnt_[0];
+        // Learned correction (lattice restriction and network) on a side stream beside the bank
+        // applies, joined before the prolongation.  Apply launch 1: fine and segment-1 banks, with
+        // the restriction of segments 2.. in its leading workgroups; launch 2: banks of segments
+        // 2.. (inputs from the restriction sums) and the dense coarsest solve.
+        const bool par = learned && branches();
+        hipStream_t sl = par ? side_[0] : stream_;
+        constexpr int block = 64 * BD;
+        const int n1 = first_apply_mats();
         const double per_mat = dim * dim * 1.0 + dim * 4.0 + 2.0 * dim * 4.0;
         if (par)
-            fork(nside);
-        tic("restrict_levels", static_cast<double>(cco_[0] + cco_[1] + cco_[2]) * BD * 4.0 * 2.0);
-        launch_restrict_coarse<BD, float>(coarse_restrict(), d_mr32_.data(), sc);
-        toc();
-        tic("mas_apply_fp16_coarse", (total_mats_ - nfine) * per_mat);
-        launch_mas_apply<BD, __half, float>(nfine, total_mats_ - nfine, d_mats16_.data(), d_dscale_.data(),
-                                            d_mr32_.data(), d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), sc);
-        toc();
-        apply_dense_coarse<BD, float>(d_mr32_.data(), d_mz32_.data(), sc);
+            fork(1);
         LccApply lv;
         FlexDot<float> flex;
         if (learned)
@@ -1193,12 +1295,24 @@ namespace psmas
                 flex.rzo = d_rzo_.data();
             }
         }
-        tic("mas_apply_fp16", nfine * per_mat);
-        launch_mas_apply<BD, __half, float>(0, nfine, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
-                                            d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_);
+        tic("mas_apply_fp16", n1 * per_mat);
+        launch_mas_apply<BD, __half, float>(0, n1, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
+                                            d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_,
+                                            restrict_side(d_mr32_.data() + static_cast<size_t>(lev_off_[1]) * BD,
+                                                          d_mr32_.data(), block));
+        launch_restrict_combine<BD, float>(nrcomb_, d_rcomb_.data(), d_rpart_.data(), d_mr32_.data(), stream_);
         toc();
+        const ApplySide cs = coarse_side(block);
+        if (total_mats_ > n1 || cs.wgs > 0)
+        {
+            tic("mas_apply_fp16_coarse", (total_mats_ - n1) * per_mat + static_cast<double>(dense_nd_) * dense_nd_ * 8.0);
+            launch_mas_apply<BD, __half, float>(n1, total_mats_ - n1, d_mats16_.data(), d_dscale_.data(),
+                                                d_mr32_.data(), d_mz32_.data(), d_bank_sys_.data(), d_active_.data(),
+                                                stream_, cs);
+            toc();
+        }
         if (par)
-            join(nside);
+            join(1);
         tic(learned ? "prolong_dot_learned" : "prolong_dot", static_cast<double>(n_total_) * 4.0 * 3 + nblocks_ * 256.0 * 16);
         launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
                                       d_mz32_.data(), reductions(), init, sysmap(), lv, flex, stream_);

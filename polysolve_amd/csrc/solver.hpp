@@ -173,6 +173,17 @@ namespace psmas
         std::string dense_note_;
         DeviceBuffer<double> d_dense_inv_;
         DeviceBuffer<int> d_dense_ids_;
+        DeviceBuffer<int> d_dense_index_; ///< per coarsest-segment node: dense index or -1
+
+        // Restriction of coarse segments 2.. straight from segment 1: every node's segment-1
+        // descendants (ascending), summed by the side workgroups of the first apply launch (a
+        // thread per node of at most RESTRICT_SMALL_MAX descendants, a wave per piece of at most
+        // RESTRICT_PIECE descendants of a larger node; pieces of one node are combined after).
+        int nrsmall_ = 0, nrbig_ = 0, nrcomb_ = 0;
+        DeviceBuffer<int4> d_rtask_;
+        DeviceBuffer<int> d_rlist_;
+        DeviceBuffer<int4> d_rcomb_;   ///< nodes cut into pieces: {node, first partial, pieces}
+        DeviceBuffer<double> d_rpart_; ///< piece sums of those nodes
 
         // vectors
         DeviceBuffer<double> d_b_, d_x_, d_mr_, d_mz_, d_p_, d_Ap_, d_io_;
@@ -212,10 +223,16 @@ namespace psmas
         void build_partition(System &S, const CscView &A);
         void upload_matrix(const PaddedBsr &bsr);
         void build_mas();
+        void build_restrict_chunks();
+        /// Side work of the first apply launch (restriction of segments 2.. from the segment-1
+        /// values seg1 into the multi-level vector out) and of the second (the dense coarsest
+        /// solve); `block` is the apply kernel's workgroup size.
+        ApplySide restrict_side(const void *seg1, void *out, int block) const;
+        ApplySide coarse_side(int block) const;
+        /// Matrices of the first apply launch: fine banks and segment-1 banks.
+        int first_apply_mats() const { return level_num_ >= 2 ? mat_off_[2] : total_mats_; }
         void ensure_mixed_data();
         void ensure_coarse_solver();
-        template <int BD, typename T>
-        void apply_dense_coarse(const T *mr, T *mz, hipStream_t s);
         /// Branch streams of the preconditioner: side streams unless kernels are being timed
         /// (then everything runs in order on stream_ so per-kernel events stay meaningful).
         bool branches() const { return !timing_on_; }
