@@ -76,6 +76,45 @@ namespace psmas
         return r;
     }
 
+    /// NV system totals at once (one pass over the partials, one reduction): out[v] equals
+    /// system_total(parts[v]) bit for bit (same per-thread order, wave reduction and wave order).
+    template <int BLOCK, int NV>
+    __device__ __forceinline__ void system_totals(const double *const (&parts)[NV], int sys, const SysMap &sm,
+                                                  double (&out)[NV])
+    {
+        static_assert(BLOCK % WAVE == 0 && BLOCK <= 1024, "block size");
+        constexpr int NW = BLOCK / WAVE;
+        __shared__ double part[NV][NW];
+        const int b0 = sm.blk_begin[sys], nb = sm.blk_count[sys];
+        double t[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            t[v] = 0.0;
+        for (int i = threadIdx.x; i < nb; i += BLOCK)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                t[v] += parts[v][b0 + i];
+        const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+        {
+            t[v] = wave_sum(t[v]);
+            if (lane == 0)
+                part[v][w] = t[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+        {
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < NW; ++i)
+                s += part[v][i];
+            out[v] = NW == 1 ? t[v] : s;
+        }
+        __syncthreads();
+    }
+
     /// alpha / beta division.  The reference returns 0 when |den| < 1e-20 (MASSolver.cu:43-56)
     /// because convergence is only checked every few iterations and the step must vanish once
     /// the solution is reached.  guard > 0 reproduces that absolute rule exactly; guard == 0

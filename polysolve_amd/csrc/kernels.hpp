@@ -2,6 +2,8 @@
 // returns immediately.
 #pragma once
 
+#include "lattice.hpp"
+
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -119,10 +121,11 @@ namespace psmas
     void launch_spmv_plain(const SellView<TV> &A, const TX *x, TX *y, hipStream_t s);
 
     // ----- vectors (node-blocked: nblocks workgroups of NODE_BLOCK nodes) ------------------
-    /// alpha = rz / pap; x += alpha p; r -= alpha Ap; partials of r . r -> red.rr; restrict r -> rc0.
+    /// alpha = rz / pap; x += alpha p; r -= alpha Ap; partials of r . r -> red.rr; restrict r -> rc0;
+    /// with lb, also the lattice block partials of the new r (learned correction).
     template <int BD, typename T>
     void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0,
-                          const Reductions &red, const SysMap &sm, hipStream_t s);
+                          const Reductions &red, const SysMap &sm, hipStream_t s, const LatticeBlocks *lb = nullptr);
 
     /// x += alpha p (alpha = rz / pap).
     template <int BD, typename T>
@@ -215,6 +218,13 @@ namespace psmas
         int dense_seg = 0;
         const int *dense_index = nullptr;
         int dense_count = 0;
+        // Lattice gather of the learned correction: u[J] = sum of lat_partial[lat_src[i]] over
+        // i in [lat_ptr[J], lat_ptr[J + 1]), J < lat_npts (wave per point, fixed-order reduction).
+        int lat_npts = 0;
+        const int *lat_ptr = nullptr;
+        const int *lat_src = nullptr;
+        const double *lat_partial = nullptr;
+        double *lat_u = nullptr;
     };
 
     template <int BD, typename TM, typename T>

@@ -3,6 +3,7 @@
 #include "lcc.hpp"
 #include "coarse_solve.hpp"
 #include "device.hpp"
+#include "lattice_device.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -14,7 +15,6 @@ namespace psmas
     {
         constexpr int LCC_CHUNK = 1024; ///< restriction entries per workgroup
         constexpr int GEMV_SYS = 8;     ///< systems per wave in the inference GEMV
-        constexpr int LCC_TMAX = 64;    ///< distinct lattice points one node block may touch
 
         __device__ __forceinline__ float wave_sum_f(float v)
         {
@@ -26,14 +26,6 @@ namespace psmas
 
         __device__ __forceinline__ float bf16_lo(unsigned v) { return __uint_as_float(v << 16); }
         __device__ __forceinline__ float bf16_hi(unsigned v) { return __uint_as_float(v & 0xffff0000u); }
-
-        __device__ __forceinline__ float trilinear_weight(int corner, float fx, float fy, float fz)
-        {
-            const float wx = (corner & 1) ? fx : 1.0f - fx;
-            const float wy = (corner & 2) ? fy : 1.0f - fy;
-            const float wz = (corner & 4) ? fz : 1.0f - fz;
-            return wx * wy * wz;
-        }
 
         /// partial[c][d] = sum over chunk c's entries (slot, w) of w * v[slot][d], in entry order.
         template <int BD, typename T>
@@ -65,59 +57,19 @@ namespace psmas
             }
         }
 
-        /// One workgroup per 256-node block: partial[b][t][d] = sum over the block's coupled slots
-        /// of w_q * v[slot][d] for the corners q whose block-local lattice point is t (t < nt[b]).
-        /// The host orders the block's (slot, corner) pairs target by target and gives every pair
-        /// its position (pos, 8 per slot); each thread writes its 8 weighted contributions to those
-        /// positions in LDS, so target t's terms are a contiguous range that thread (t, d) adds in
-        /// order.
+        /// One workgroup per 256-node block: the block's lattice partials from v (see
+        /// lattice_block_partials).
         template <int BD, typename T>
-        __global__ __launch_bounds__(256) void k_lcc_restrict_blocks(const int4 *__restrict__ cell,
-                                                                     const uint4 *__restrict__ pos,
-                                                                     const int *__restrict__ block_nt,
-                                                                     const int *__restrict__ tbase,
-                                                                     const int *__restrict__ tptr,
-                                                                     const T *__restrict__ v,
-                                                                     double *__restrict__ partial)
+        __global__ __launch_bounds__(256) void k_lcc_restrict_blocks(LatticeBlocks lb, const T *__restrict__ v)
         {
             __shared__ T vals[256 * 8 * BD];
             const int b = blockIdx.x;
-            const int nt = block_nt[b];
-            if (nt == 0)
-                return;
-            const int p = b * 256 + threadIdx.x;
-            const int4 c = cell[p];
-            if (c.x >= 0)
-            {
-                const float fx = __int_as_float(c.y), fy = __int_as_float(c.z), fz = __int_as_float(c.w);
-                const uint4 P = pos[p];
-                const unsigned pk[4] = {P.x, P.y, P.z, P.w};
-                T r[BD];
+            T r[BD];
+            const size_t p = static_cast<size_t>(b) * 256 + threadIdx.x;
 #pragma unroll
-                for (int d = 0; d < BD; ++d)
-                    r[d] = v[static_cast<size_t>(p) * BD + d];
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                {
-                    const int at = static_cast<int>((pk[q >> 1] >> (16 * (q & 1))) & 0xffffu);
-                    const T w = static_cast<T>(trilinear_weight(q, fx, fy, fz));
-#pragma unroll
-                    for (int d = 0; d < BD; ++d)
-                        vals[at * BD + d] = w * r[d];
-                }
-            }
-            __syncthreads();
-            const int base = tbase[b];
-            const int e_begin = tptr[base];
-            for (int i = threadIdx.x; i < nt * BD; i += 256)
-            {
-                const int t = i / BD, d = i % BD;
-                const int e0 = tptr[base + t] - e_begin, e1 = tptr[base + t + 1] - e_begin;
-                double s = 0.0;
-                for (int e = e0; e < e1; ++e)
-                    s += static_cast<double>(vals[e * BD + d]);
-                partial[(static_cast<size_t>(b) * LCC_TMAX + t) * BD + d] = s;
-            }
+            for (int d = 0; d < BD; ++d)
+                r[d] = v[p * BD + d];
+            lattice_block_partials<BD, T>(lb, b, lb.block_nt[b], lb.cell[p], lb.pos[p], r, vals);
         }
 
         /// One wave per (global) lattice point J: u[J][d] = sum of its partials, lanes striding over
@@ -593,12 +545,7 @@ namespace psmas
     void LearnedCoarse::restrict_to_lattice(const T *v, const int *active, hipStream_t s, float output_factor)
     {
         if (use_blocks_)
-        {
-            hipLaunchKernelGGL((k_lcc_restrict_blocks<BD, T>), dim3(nblocks_), dim3(256), 0, s, d_cell_.data(),
-                               d_pos_.data(), d_block_nt_.data(), d_tbase_.data(), d_tptr_.data(), v,
-                               d_partial_.data());
-            PSMAS_LAUNCH_CHECK();
-        }
+            restrict_blocks<BD, T>(v, s);
         else if (nchunks_ > 0)
         {
             hipLaunchKernelGGL((k_lcc_restrict_partial<BD, T>), dim3(nchunks_), dim3(256), 0, s, d_chunk_begin_.data(),
@@ -611,6 +558,36 @@ namespace psmas
         PSMAS_LAUNCH_CHECK();
         out_factor_ = output_factor;
         (void)active;
+    }
+
+    template <int BD, typename T>
+    void LearnedCoarse::restrict_blocks(const T *v, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_lcc_restrict_blocks<BD, T>), dim3(nblocks_), dim3(256), 0, s, block_view(), v);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    LatticeBlocks LearnedCoarse::block_view() const
+    {
+        LatticeBlocks lb;
+        if (!use_blocks_)
+            return lb;
+        lb.cell = d_cell_.data();
+        lb.pos = d_pos_.data();
+        lb.block_nt = d_block_nt_.data();
+        lb.tbase = d_tbase_.data();
+        lb.tptr = d_tptr_.data();
+        lb.partial = const_cast<double *>(d_partial_.data());
+        return lb;
+    }
+
+    void LearnedCoarse::gather_into(ApplySide &sd) const
+    {
+        sd.lat_npts = nsys_ * np_;
+        sd.lat_ptr = d_point_ptr_.data();
+        sd.lat_src = d_point_src_.data();
+        sd.lat_partial = d_partial_.data();
+        sd.lat_u = const_cast<double *>(d_u_.data());
     }
 
     void LearnedCoarse::infer(const int *active, hipStream_t s)
@@ -804,6 +781,8 @@ namespace psmas
     }
 
 #define PSMAS_INST_LCC(BD)                                                                                          \
+    template void LearnedCoarse::restrict_blocks<BD, double>(const double *, hipStream_t);                        \
+    template void LearnedCoarse::restrict_blocks<BD, float>(const float *, hipStream_t);                          \
     template void LearnedCoarse::restrict_to_lattice<BD, double>(const double *, const int *, hipStream_t, float); \
     template void LearnedCoarse::restrict_to_lattice<BD, float>(const float *, const int *, hipStream_t, float);   \
     template void LearnedCoarse::prolong_host_coef<BD>(const std::vector<double> &, double *, long long, hipStream_t);

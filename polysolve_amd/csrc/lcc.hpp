@@ -49,6 +49,16 @@ namespace psmas
         /// u = P^T v (FP64 sums) for all systems -> d_u_ [nsys][m] (the network normalises it).
         template <int BD, typename T>
         void restrict_to_lattice(const T *v, const int *active, hipStream_t s, float output_factor = 1.0f);
+        /// Block path only: the per-block lattice partials of v (the gather to u is separate).
+        template <int BD, typename T>
+        void restrict_blocks(const T *v, hipStream_t s);
+        /// Block-path view for the residual update that computes the partials (cell null when the
+        /// entry path is in use).
+        LatticeBlocks block_view() const;
+        /// Fills the lattice gather (partials -> u) of an apply launch's side work.
+        void gather_into(ApplySide &sd) const;
+        /// Output factor of the next inference (the restriction sets it when it runs here).
+        void set_output_factor(float f) { out_factor_ = f; }
         /// Network inference on the normalised inputs -> coef [nsys][m] (scaled back).
         void infer(const int *active, hipStream_t s);
         /// coef_ -> LccApply view.

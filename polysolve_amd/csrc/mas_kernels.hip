@@ -153,6 +153,33 @@ namespace psmas
                     }
                 }
             }
+            if (sd.lat_npts > 0)
+            {
+                // Lattice gather: a wave per point, counting waves from the far end so that the
+                // waves with restriction pieces get points last.
+                const int nw = nt >> 6;
+                for (int J = nw - 1 - (t >> 6); J < sd.lat_npts; J += nw)
+                {
+                    double a[BD];
+#pragma unroll
+                    for (int d = 0; d < BD; ++d)
+                        a[d] = 0.0;
+                    for (int i = sd.lat_ptr[J] + lane; i < sd.lat_ptr[J + 1]; i += 64)
+                    {
+                        const size_t c = static_cast<size_t>(sd.lat_src[i]);
+#pragma unroll
+                        for (int d = 0; d < BD; ++d)
+                            a[d] += sd.lat_partial[c * BD + d];
+                    }
+#pragma unroll
+                    for (int d = 0; d < BD; ++d)
+                    {
+                        const double v = wave_sum(a[d]);
+                        if (lane == 0)
+                            sd.lat_u[static_cast<size_t>(J) * BD + d] = v;
+                    }
+                }
+            }
             if (sd.dense_nd > 0)
             {
                 const int nd = sd.dense_nd;

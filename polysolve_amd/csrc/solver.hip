@@ -115,29 +115,7 @@ namespace psmas
         device_ = 0;
         PSMAS_HIP(hipSetDevice(device_));
         PSMAS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-        for (int i = 0; i < 2; ++i)
-        {
-            PSMAS_HIP(hipStreamCreateWithFlags(&side_[i], hipStreamNonBlocking));
-            PSMAS_HIP(hipEventCreateWithFlags(&join_ev_[i], hipEventDisableTiming));
-        }
-        PSMAS_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
         sys_.resize(1);
-    }
-
-    void Solver::fork(int nside)
-    {
-        PSMAS_HIP(hipEventRecord(fork_ev_, stream_));
-        for (int i = 0; i < nside; ++i)
-            PSMAS_HIP(hipStreamWaitEvent(side_[i], fork_ev_, 0));
-    }
-
-    void Solver::join(int nside)
-    {
-        for (int i = 0; i < nside; ++i)
-        {
-            PSMAS_HIP(hipEventRecord(join_ev_[i], side_[i]));
-            PSMAS_HIP(hipStreamWaitEvent(stream_, join_ev_[i], 0));
-        }
     }
 
     Solver::~Solver()
@@ -145,20 +123,9 @@ namespace psmas
         (void)hipSetDevice(device_);
         if (stream_)
             (void)hipStreamSynchronize(stream_);
-        for (hipStream_t st : side_)
-            if (st)
-                (void)hipStreamSynchronize(st);
         graphs_.reset();
         for (hipEvent_t ev : event_pool_)
             (void)hipEventDestroy(ev);
-        for (hipEvent_t ev : join_ev_)
-            if (ev)
-                (void)hipEventDestroy(ev);
-        if (fork_ev_)
-            (void)hipEventDestroy(fork_ev_);
-        for (hipStream_t st : side_)
-            if (st)
-                (void)hipStreamDestroy(st);
         if (stream_)
             (void)hipStreamDestroy(stream_);
     }
@@ -823,20 +790,26 @@ namespace psmas
         }
     }
 
-    ApplySide Solver::restrict_side(const void *seg1, void *out, int block) const
+    ApplySide Solver::restrict_side(const void *seg1, void *out, int block, bool lattice_gather) const
     {
         ApplySide sd;
-        if (nrsmall_ + nrbig_ == 0)
+        if (nrsmall_ + nrbig_ > 0)
+        {
+            sd.nsmall = nrsmall_;
+            sd.nbig = nrbig_;
+            sd.rtask = d_rtask_.data();
+            sd.rlist = d_rlist_.data();
+            sd.seg1 = seg1;
+            sd.out = out;
+            sd.rpart = const_cast<double *>(d_rpart_.data());
+        }
+        if (lattice_gather)
+            lcc_->gather_into(sd);
+        if (sd.nsmall + sd.nbig + sd.lat_npts == 0)
             return sd;
-        sd.nsmall = nrsmall_;
-        sd.nbig = nrbig_;
-        sd.rtask = d_rtask_.data();
-        sd.rlist = d_rlist_.data();
-        sd.seg1 = seg1;
-        sd.out = out;
-        sd.rpart = const_cast<double *>(d_rpart_.data());
-        // A thread per small node and a wave per large node, at most 256 workgroups.
-        const int want = std::max(div_up(nrsmall_, block), div_up(nrbig_, block / 64));
+        // A thread per small node, a wave per piece of a large node and per lattice point; at
+        // most 256 workgroups.
+        const int want = std::max(div_up(sd.nsmall, block), div_up(sd.nbig + sd.lat_npts, block / 64));
         sd.wgs = std::min(256, std::max(1, want));
         return sd;
     }
@@ -1246,15 +1219,20 @@ namespace psmas
 
     // ---- mixed path: FP32 inner PCG with FP64 reliable updates ------------------------------
     template <int BD>
-    void Solver::mixed_part_a()
+    void Solver::mixed_part_a(bool learned)
     {
         const SellView<float> A = sell(d_vals32_.data());
         tic("spmv_fp32", static_cast<double>(nnzb_) * (BD * BD * 4 + 4) + (nb_total_ + 1) * 4.0 + 2.0 * n_total_ * 4.0);
         launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), reductions(), sysmap(), stream_);
         toc();
+        // With the learned correction on the block path, the update also leaves the lattice block
+        // partials of the new r.
+        const LatticeBlocks lb = learned && lcc_ ? lcc_->block_view() : LatticeBlocks{};
+        tic(lb.cell ? "update_xr_lattice" : "update_xr", static_cast<double>(n_total_) * 4.0 * 6.0);
         launch_update_xr<BD, float>(nblocks_, d_p32_.data(), d_Ap32_.data(), d_x32_.data(), d_mr32_.data(),
                                     fine_restrict(), d_mr32_.data() + static_cast<size_t>(npad_) * BD, reductions(),
-                                    sysmap(), stream_);
+                                    sysmap(), stream_, &lb);
+        toc();
     }
 
     template <int BD>
@@ -1262,32 +1240,28 @@ namespace psmas
     {
         const long long nvec = static_cast<long long>(npad_) * BD;
         const int dim = BANK * BD;
-        // Learned correction (lattice restriction and network) on a side stream beside the bank
-        // applies, joined before the prolongation.  Apply launch 1: fine and segment-1 banks, with
-        // the restriction of segments 2.. in its leading workgroups; launch 2: banks of segments
-        // 2.. (inputs from the restriction sums) and the dense coarsest solve.
-        const bool par = learned && branches();
-        hipStream_t sl = par ? side_[0] : stream_;
+        // Apply launch 1: fine and segment-1 banks, with the restriction of segments 2.. (and, for
+        // the learned correction on the block path, the lattice gather of the block partials the
+        // residual update left) in its leading workgroups; launch 2: banks of segments 2.. and the
+        // dense coarsest solve; then the network.
         constexpr int block = 64 * BD;
         const int n1 = first_apply_mats();
         const double per_mat = dim * dim * 1.0 + dim * 4.0 + 2.0 * dim * 4.0;
-        if (par)
-            fork(1);
+        const bool lat_blocks = learned && lcc_->block_restriction();
         LccApply lv;
         FlexDot<float> flex;
         if (learned)
         {
-            // Block path: per slot the lattice cell (16 B) and r, 2 B per list entry; entry path: per
-            // coupled slot 8 {slot, weight} entries (8 B each) and 8 gathers of r.
-            tic("lcc_restrict", lcc_->block_restriction()
-                                    ? static_cast<double>(npad_) * (16.0 + BD * 4.0) + lcc_->restrict_entries() * 2.0
-                                    : static_cast<double>(lcc_->restrict_entries()) * (8.0 + BD * 4.0));
-            lcc_->restrict_to_lattice<BD, float>(d_mr32_.data(), d_active_.data(), sl,
-                                                 static_cast<float>(params_.learned_output_scale));
-            toc();
-            tic("lcc_infer", static_cast<double>(lcc_->inference_weight_bytes()));
-            lcc_->infer(d_active_.data(), sl);
-            toc();
+            if (lat_blocks)
+                lcc_->set_output_factor(static_cast<float>(params_.learned_output_scale));
+            else
+            {
+                // Entry path: per coupled slot 8 {slot, weight} entries (8 B each) and 8 gathers of r.
+                tic("lcc_restrict", static_cast<double>(lcc_->restrict_entries()) * (8.0 + BD * 4.0));
+                lcc_->restrict_to_lattice<BD, float>(d_mr32_.data(), d_active_.data(), stream_,
+                                                     static_cast<float>(params_.learned_output_scale));
+                toc();
+            }
             lv = lcc_->apply_view();
             if (!init)
             {
@@ -1299,7 +1273,7 @@ namespace psmas
         launch_mas_apply<BD, __half, float>(0, n1, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
                                             d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_,
                                             restrict_side(d_mr32_.data() + static_cast<size_t>(lev_off_[1]) * BD,
-                                                          d_mr32_.data(), block));
+                                                          d_mr32_.data(), block, lat_blocks));
         launch_restrict_combine<BD, float>(nrcomb_, d_rcomb_.data(), d_rpart_.data(), d_mr32_.data(), stream_);
         toc();
         const ApplySide cs = coarse_side(block);
@@ -1311,8 +1285,12 @@ namespace psmas
                                                 stream_, cs);
             toc();
         }
-        if (par)
-            join(1);
+        if (learned)
+        {
+            tic("lcc_infer", static_cast<double>(lcc_->inference_weight_bytes()));
+            lcc_->infer(d_active_.data(), stream_);
+            toc();
+        }
         tic(learned ? "prolong_dot_learned" : "prolong_dot", static_cast<double>(n_total_) * 4.0 * 3 + nblocks_ * 256.0 * 16);
         launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
                                       d_mz32_.data(), reductions(), init, sysmap(), lv, flex, stream_);
@@ -1335,7 +1313,7 @@ namespace psmas
     }
 
     template <int BD>
-    void Solver::mixed_reliable_residual()
+    void Solver::mixed_reliable_residual(bool learned)
     {
         const SellView<double> A = sell(d_vals64_.data());
         tic("spmv_res_fp64",
@@ -1344,6 +1322,10 @@ namespace psmas
                                                    d_mr32_.data() + static_cast<size_t>(npad_) * BD,
                                                    reductions().rr_true, sysmap(), true, stream_);
         toc();
+        // Block path of the learned correction: the preconditioner expects the lattice block
+        // partials of the current r (the residual update leaves them; here r was recomputed).
+        if (learned && lcc_ && lcc_->block_restriction())
+            lcc_->restrict_blocks<BD, float>(d_mr32_.data(), stream_);
     }
 
     void Solver::solve_mixed()
@@ -1393,7 +1375,7 @@ namespace psmas
             }
             launch_fill<double>(nvec, d_x_.data(), 0.0, stream_);
             launch_fill<float>(nvec, d_x32_.data(), 0.0f, stream_);
-            mixed_reliable_residual<BD>(); // every system selected: r = b
+            mixed_reliable_residual<BD>(learned); // every system selected: r = b
             mixed_part_b<BD>(true, learned);
             finalize_scalars();
             read_scalars();
@@ -1441,7 +1423,7 @@ namespace psmas
                         mixed_part_b<BD>(false, learned);
                     for (int i = 0; i < count; ++i)
                     {
-                        mixed_part_a<BD>();
+                        mixed_part_a<BD>(learned);
                         if (i + 1 < count)
                             mixed_part_b<BD>(false, learned);
                     }
@@ -1501,7 +1483,7 @@ namespace psmas
                 // Reliable update: fold the FP32 correction into x and recompute r in FP64.
                 upload_flags();
                 launch_accumulate_correction<BD>(nblocks_, d_x32_.data(), d_x_.data(), sysmap(), stream_);
-                mixed_reliable_residual<BD>();
+                mixed_reliable_residual<BD>(learned);
                 finalize_scalars();
                 read_scalars();
                 for (int s = 0; s < K; ++s)

@@ -123,12 +123,6 @@ namespace psmas
         double solve_ms_ = 0.0;
 
         hipStream_t stream_ = nullptr;
-        /// Side streams for the branches of the preconditioner (coarse levels, learned correction):
-        /// forked from and joined back into stream_ by events, so graph capture records them as
-        /// parallel branches.  Default priority, like stream_.
-        hipStream_t side_[2] = {nullptr, nullptr};
-        hipEvent_t fork_ev_ = nullptr;
-        hipEvent_t join_ev_[2] = {nullptr, nullptr};
         int device_ = 0;
 
         int bd_ = 0, npad_ = 0, nparts_ = 0, nblocks_ = 0;
@@ -225,19 +219,15 @@ namespace psmas
         void build_mas();
         void build_restrict_chunks();
         /// Side work of the first apply launch (restriction of segments 2.. from the segment-1
-        /// values seg1 into the multi-level vector out) and of the second (the dense coarsest
-        /// solve); `block` is the apply kernel's workgroup size.
-        ApplySide restrict_side(const void *seg1, void *out, int block) const;
+        /// values seg1 into the multi-level vector out; with lattice_gather also the learned
+        /// correction's lattice gather) and of the second (the dense coarsest solve); `block` is
+        /// the apply kernel's workgroup size.
+        ApplySide restrict_side(const void *seg1, void *out, int block, bool lattice_gather = false) const;
         ApplySide coarse_side(int block) const;
         /// Matrices of the first apply launch: fine banks and segment-1 banks.
         int first_apply_mats() const { return level_num_ >= 2 ? mat_off_[2] : total_mats_; }
         void ensure_mixed_data();
         void ensure_coarse_solver();
-        /// Branch streams of the preconditioner: side streams unless kernels are being timed
-        /// (then everything runs in order on stream_ so per-kernel events stay meaningful).
-        bool branches() const { return !timing_on_; }
-        void fork(int nside);
-        void join(int nside);
         void allocate_vectors();
         void check_ready_single(int n) const;
 
@@ -258,11 +248,11 @@ namespace psmas
         template <int BD>
         void fp64_precondition(bool init);
         template <int BD>
-        void mixed_part_a();
+        void mixed_part_a(bool learned);
         template <int BD>
         void mixed_part_b(bool init, bool learned = false);
         template <int BD>
-        void mixed_reliable_residual();
+        void mixed_reliable_residual(bool learned);
 
         void run_block(int key, const std::function<void()> &enqueue);
 

@@ -14,6 +14,7 @@
 //    bank-local component masks, so the preconditioner never re-reads r for that step;
 //  * systems that have converged (active == 0) skip their workgroups.
 #include "device.hpp"
+#include "lattice_device.hpp"
 #include "kernels.hpp"
 #include "common.hpp"
 
@@ -212,32 +213,57 @@ namespace psmas
         /// alpha = rz / pap of the block's system (reference scalar_division, MASSolver.cu:46-56).
         __device__ __forceinline__ double system_alpha(const Reductions &red, int sys, const SysMap &sm)
         {
-            const double rz = system_total<VEC_BLOCK>(red.rz, sys, sm);
-            const double pap = system_total<VEC_BLOCK>(red.pap, sys, sm);
-            return safe_div(rz, pap, red.sc[static_cast<size_t>(sys) * S_COUNT + S_GUARD]);
+            const double *const parts[2] = {red.rz, red.pap};
+            double t[2];
+            system_totals<VEC_BLOCK, 2>(parts, sys, sm, t);
+            return safe_div(t[0], t[1], red.sc[static_cast<size_t>(sys) * S_COUNT + S_GUARD]);
         }
 
         /// x += alpha p; r -= alpha Ap on the block's 256*BD contiguous scalars (coalesced), then
         /// r . r and the restriction of r to coarse level 0 from LDS.
-        template <int BD, typename T>
+        template <int BD, typename T, bool LAT>
         __global__ __launch_bounds__(VEC_BLOCK) void k_update_xr(const T *__restrict__ p, const T *__restrict__ Ap,
                                                                  T *__restrict__ x, T *__restrict__ r, FineRestrict fr,
-                                                                 T *__restrict__ rc0, Reductions red, SysMap sm)
+                                                                 T *__restrict__ rc0, Reductions red, SysMap sm,
+                                                                 LatticeBlocks lb)
         {
+            static_assert(VEC_BLOCK == 256, "lattice blocks are 256-node blocks");
             __shared__ T sv[VEC_BLOCK * BD];
             const int sys = block_system(sm);
             if (sm.active[sys] == 0)
                 return;
-            const T alpha = static_cast<T>(system_alpha(red, sys, sm));
+            // Lattice inputs first: their loads overlap the vector update.
+            int lnt = 0;
+            int4 lc{-1, 0, 0, 0};
+            uint4 lp{0u, 0u, 0u, 0u};
+            if constexpr (LAT)
+            {
+                const size_t slot = static_cast<size_t>(blockIdx.x) * VEC_BLOCK + threadIdx.x;
+                lnt = lb.block_nt[blockIdx.x];
+                lc = lb.cell[slot];
+                lp = lb.pos[slot];
+            }
+            // The block's vectors are loaded before alpha's reduction so both latencies overlap.
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
+            T pv[BD], apv[BD], xv[BD], rv0[BD];
+#pragma unroll
+            for (int k = 0; k < BD; ++k)
+            {
+                const size_t o = base + threadIdx.x + k * VEC_BLOCK;
+                pv[k] = p[o];
+                apv[k] = Ap[o];
+                xv[k] = x[o];
+                rv0[k] = r[o];
+            }
+            const T alpha = static_cast<T>(system_alpha(red, sys, sm));
             double dot = 0.0;
 #pragma unroll
             for (int k = 0; k < BD; ++k)
             {
                 const int j = threadIdx.x + k * VEC_BLOCK;
                 const size_t o = base + j;
-                x[o] = fma(alpha, p[o], x[o]);
-                const T rv = fma(-alpha, Ap[o], r[o]);
+                x[o] = fma(alpha, pv[k], xv[k]);
+                const T rv = fma(-alpha, apv[k], rv0[k]);
                 r[o] = rv;
                 sv[j] = rv;
                 dot += static_cast<double>(rv) * static_cast<double>(rv);
@@ -248,6 +274,15 @@ namespace psmas
             __syncthreads();
             restrict_bank<BD, T>(sv, (threadIdx.x / 32) * 32, mask, gid, rc0);
             store_partial<VEC_BLOCK>(dot, red.rr, blockIdx.x);
+            if constexpr (LAT)
+            {
+                __shared__ T vals[256 * 8 * BD];
+                T rn[BD];
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                    rn[d] = sv[threadIdx.x * BD + d];
+                lattice_block_partials<BD, T>(lb, blockIdx.x, lnt, lc, lp, rn, vals);
+            }
         }
 
         template <int BD, typename T>
@@ -275,30 +310,41 @@ namespace psmas
             const int sys = block_system(sm);
             if (sm.active[sys] == 0)
                 return;
-            double rz = system_total<VEC_BLOCK>(red.rz, sys, sm);
-            const double rz_old = system_total<VEC_BLOCK>(red.rz_old, sys, sm);
-            if (rzo != nullptr) // flexible (Polak-Ribiere): r_new . (z_new - z_old)
-                rz -= system_total<VEC_BLOCK>(rzo, sys, sm);
-            const T beta = static_cast<T>(safe_div(rz, rz_old, red.sc[static_cast<size_t>(sys) * S_COUNT + S_GUARD]));
-            if (zprev != nullptr)
-            {
-                const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
-#pragma unroll
-                for (int k = 0; k < BD; ++k)
-                {
-                    const size_t o = base + threadIdx.x + k * VEC_BLOCK;
-                    const T zo = z[o];
-                    p[o] = fma(beta, p[o], zo);
-                    zprev[o] = zo;
-                }
-                return;
-            }
+            // The block's vectors are loaded before beta's reduction so both latencies overlap.
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
+            T zv[BD], pv[BD];
 #pragma unroll
             for (int k = 0; k < BD; ++k)
             {
                 const size_t o = base + threadIdx.x + k * VEC_BLOCK;
-                p[o] = fma(beta, p[o], z[o]);
+                zv[k] = z[o];
+                pv[k] = p[o];
+            }
+            double rz, rz_old;
+            if (rzo != nullptr) // flexible (Polak-Ribiere): r_new . (z_new - z_old)
+            {
+                const double *const parts[3] = {red.rz, red.rz_old, rzo};
+                double t[3];
+                system_totals<VEC_BLOCK, 3>(parts, sys, sm, t);
+                rz = t[0] - t[2];
+                rz_old = t[1];
+            }
+            else
+            {
+                const double *const parts[2] = {red.rz, red.rz_old};
+                double t[2];
+                system_totals<VEC_BLOCK, 2>(parts, sys, sm, t);
+                rz = t[0];
+                rz_old = t[1];
+            }
+            const T beta = static_cast<T>(safe_div(rz, rz_old, red.sc[static_cast<size_t>(sys) * S_COUNT + S_GUARD]));
+#pragma unroll
+            for (int k = 0; k < BD; ++k)
+            {
+                const size_t o = base + threadIdx.x + k * VEC_BLOCK;
+                p[o] = fma(beta, pv[k], zv[k]);
+                if (zprev != nullptr)
+                    zprev[o] = zv[k];
             }
         }
 
@@ -389,11 +435,10 @@ namespace psmas
         __global__ __launch_bounds__(VEC_BLOCK) void k_finalize(Reductions red, SysMap sm)
         {
             const int sys = blockIdx.x;
-            const double pap = system_total<VEC_BLOCK>(red.pap, sys, sm);
-            const double rr = system_total<VEC_BLOCK>(red.rr, sys, sm);
-            const double rz = system_total<VEC_BLOCK>(red.rz, sys, sm);
-            const double rz_old = system_total<VEC_BLOCK>(red.rz_old, sys, sm);
-            const double rr_true = system_total<VEC_BLOCK>(red.rr_true, sys, sm);
+            const double *const parts[5] = {red.pap, red.rr, red.rz, red.rz_old, red.rr_true};
+            double t[5];
+            system_totals<VEC_BLOCK, 5>(parts, sys, sm, t);
+            const double pap = t[0], rr = t[1], rz = t[2], rz_old = t[3], rr_true = t[4];
             if (threadIdx.x == 0)
             {
                 double *scs = red.sc + static_cast<size_t>(sys) * S_COUNT;
@@ -472,9 +517,14 @@ namespace psmas
 
     template <int BD, typename T>
     void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0,
-                          const Reductions &red, const SysMap &sm, hipStream_t s)
+                          const Reductions &red, const SysMap &sm, hipStream_t s, const LatticeBlocks *lb)
     {
-        hipLaunchKernelGGL((k_update_xr<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, Ap, x, r, fr, rc0, red, sm);
+        if (lb != nullptr && lb->cell != nullptr)
+            hipLaunchKernelGGL((k_update_xr<BD, T, true>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, Ap, x, r, fr, rc0,
+                               red, sm, *lb);
+        else
+            hipLaunchKernelGGL((k_update_xr<BD, T, false>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, Ap, x, r, fr,
+                               rc0, red, sm, LatticeBlocks{});
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -576,9 +626,10 @@ namespace psmas
                                                       hipStream_t);                                               \
     template void launch_update_xr<BD, double>(int, const double *, const double *, double *, double *,           \
                                                FineRestrict, double *, const Reductions &, const SysMap &,        \
-                                               hipStream_t);                                                      \
+                                               hipStream_t, const LatticeBlocks *);                               \
     template void launch_update_xr<BD, float>(int, const float *, const float *, float *, float *, FineRestrict,  \
-                                              float *, const Reductions &, const SysMap &, hipStream_t);          \
+                                              float *, const Reductions &, const SysMap &, hipStream_t,           \
+                                              const LatticeBlocks *);                                             \
     template void launch_axpy_alpha<BD, double>(int, const double *, double *, const Reductions &, const SysMap &, \
                                                 hipStream_t);                                                     \
     template void launch_update_p<BD, double>(int, const double *, double *, const Reductions &, const SysMap &,   \
