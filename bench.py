@@ -252,8 +252,10 @@ def main() -> int:
             # Kernel durations do not depend on how well the network is trained.
             from polysolve_amd import problems as P
 
+            # The training solves run to convergence (pairs come from their solutions) whatever
+            # --max-iter caps the profiled solves at.
             solver.set_parameters({"MAS": {"learned_collect": True, "lattice_points": args.lattice,
-                                           "learned_random_pairs": 32}})
+                                           "learned_random_pairs": 32, "max_iter": 3000}})
             seq = P.config5_moduli(prob.ne, steps=3, seed=0)
             next(seq)
             for E in seq:
@@ -262,7 +264,8 @@ def main() -> int:
                 xt = np.zeros(pt.n)
                 solver.solve(pt.b, xt)
             solver.learned_train(epochs=20, batch=64, lr=3e-3, seed=0)
-            solver.set_parameters({"MAS": {"learned_collect": False, "learned_correction": True}})
+            solver.set_parameters({"MAS": {"learned_collect": False, "learned_correction": True,
+                                           "max_iter": args.max_iter}})
             solver.factorize(prob.A)
             learned_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
             log(f"  learned {learned_ms:.2f} ms/solve, {solver.get_info()['solver_iter']} it")

@@ -202,13 +202,19 @@ namespace psmas
         // thread each, [nsmall, nsmall + nbig) a wave each (lanes stride the list, fixed-order
         // reduction).  rtask[i].w < 0: the sum is multi-level node rtask[i].x's value (rounded to
         // the vector type, written to out); otherwise it is partial rtask[i].w of a node cut into
-        // pieces (written to rpart, combined by launch_restrict_combine).
+        // pieces (written to rpart; the launch that reads the node adds its pieces).
         int nsmall = 0, nbig = 0;
         const int4 *rtask = nullptr;
         const int *rlist = nullptr;
         const void *seg1 = nullptr; ///< segment-1 values, of the launch's vector type
         void *out = nullptr;        ///< multi-level vector the sums go to
         double *rpart = nullptr;    ///< [pieces][bd]
+        // Inputs of the launch that reads segments 2..: nodes whose restriction was cut into pieces
+        // take the sum of rpart[rcomb[i].y .. rcomb[i].y + rcomb[i].z) (in piece order) with
+        // i = split_of[node - node0] >= 0; all other nodes read mr.
+        const int *split_of = nullptr;
+        const int4 *rcomb = nullptr;
+        int node0 = 0;
         // Dense coarsest solve y = Inv x, one wave per row; its nodes are coarsest-segment nodes
         // dense_seg + dense_ids[i].  Bank outputs of nodes with dense_index[n - dense_seg] >= 0
         // (n - dense_seg < dense_count) are left to it.
@@ -231,10 +237,6 @@ namespace psmas
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
                           const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side = ApplySide{});
 
-    /// out[comb[i].x] = sum of rpart[comb[i].y .. comb[i].y + comb[i].z) for i < n (nodes whose
-    /// restriction was cut into pieces), in piece order.
-    template <int BD, typename T>
-    void launch_restrict_combine(int n, const int4 *comb, const double *rpart, T *out, hipStream_t s);
 
     /// z_p = mz_p + sum_l mz_l[cmap_p.l] (+ the lattice correction P c when lcc.cell is set);
     /// partials of r . z -> red.rz (the previous ones move to red.rz_old unless init) and, when

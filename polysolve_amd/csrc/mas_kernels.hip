@@ -68,6 +68,38 @@ namespace psmas
         template <int BD>
         constexpr int apply_block() { return BD == 1 ? 64 : (BD == 2 ? 128 : 192); }
 
+        /// Input component d of multi-level node `node`: mr, or for a node whose restriction was
+        /// cut into pieces the FP64 sum of its pieces in order (rounded like a stored value).  The
+        /// two loads are independent, so only split nodes pay a further round trip.
+        template <int BD, typename T>
+        __device__ __forceinline__ T coarse_input(const ApplySide &sd, const T *__restrict__ mr, int node, int d)
+        {
+            T v = mr[static_cast<size_t>(node) * BD + d];
+            if (sd.split_of != nullptr && node >= sd.node0)
+            {
+                const int si = sd.split_of[node - sd.node0];
+                if (si >= 0)
+                {
+                    // Pieces loaded eight at a time, added in order.
+                    const int4 c = sd.rcomb[si];
+                    double acc = 0.0;
+                    for (int k0 = 0; k0 < c.z; k0 += 8)
+                    {
+                        double pv[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            pv[j] = k0 + j < c.z ? sd.rpart[static_cast<size_t>(c.y + k0 + j) * BD + d] : 0.0;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            if (k0 + j < c.z)
+                                acc += pv[j];
+                    }
+                    v = static_cast<T>(acc);
+                }
+            }
+            return v;
+        }
+
         /// True when the dense coarsest solve owns node `node` (its bank output is not stored).
         __device__ __forceinline__ bool dense_owned(const ApplySide &sd, int node)
         {
@@ -78,9 +110,12 @@ namespace psmas
         }
 
         /// Side work of workgroup `wg` < sd.wgs: restriction tasks (thread per small node, wave per
-        /// large node), then dense coarsest rows (wave per row, lanes over columns).
+        /// large node), lattice gather, then dense coarsest rows (the inputs staged once per
+        /// workgroup in `stage`, LDS of `stage_cap` elements, when they fit; wave per row, lanes
+        /// over columns).
         template <int BD, typename T>
-        __device__ void apply_side_work(const ApplySide &sd, const T *__restrict__ mr, T *__restrict__ mz, int wg)
+        __device__ void apply_side_work(const ApplySide &sd, const T *__restrict__ mr, T *__restrict__ mz, int wg,
+                                        T *stage, int stage_cap)
         {
             const int nt = sd.wgs * static_cast<int>(blockDim.x);
             const int t = wg * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
@@ -183,14 +218,22 @@ namespace psmas
             if (sd.dense_nd > 0)
             {
                 const int nd = sd.dense_nd;
+                const bool staged = nd <= stage_cap;
+                if (staged)
+                {
+                    for (int c = threadIdx.x; c < nd; c += blockDim.x)
+                        stage[c] = coarse_input<BD, T>(sd, mr, sd.dense_seg + sd.dense_ids[c / BD], c % BD);
+                    __syncthreads();
+                }
                 for (int row = t >> 6; row < nd; row += nt >> 6)
                 {
                     const double *a = sd.dense_inv + static_cast<size_t>(row) * nd;
                     double acc = 0.0;
                     for (int c = lane; c < nd; c += 64)
                     {
-                        const int node = sd.dense_seg + sd.dense_ids[c / BD];
-                        acc += a[c] * static_cast<double>(mr[static_cast<size_t>(node) * BD + c % BD]);
+                        const T xc = staged ? stage[c]
+                                            : coarse_input<BD, T>(sd, mr, sd.dense_seg + sd.dense_ids[c / BD], c % BD);
+                        acc += a[c] * static_cast<double>(xc);
                     }
                     acc = wave_sum(acc);
                     if (lane == 0)
@@ -300,7 +343,8 @@ namespace psmas
             };
             if (static_cast<int>(blockIdx.x) < sd.wgs)
             {
-                apply_side_work<BD, float>(sd, mr, mz, blockIdx.x);
+                apply_side_work<BD, float>(sd, mr, mz, blockIdx.x, reinterpret_cast<float *>(sF),
+                                           static_cast<int>(sizeof(sF) / (sizeof(float))));
                 return;
             }
             const int nwg = static_cast<int>(gridDim.x) - sd.wgs; // apply workgroups
@@ -358,7 +402,8 @@ namespace psmas
                 for (int k = 0; k < PER; ++k)
                     reg[k] = __builtin_nontemporal_load(src + tid + k * BLOCK);
                 if (tid < N)
-                    xr = mr[static_cast<size_t>(mm) * N + tid] * dscale[static_cast<size_t>(mm) * N + tid];
+                    xr = coarse_input<BD, float>(sd, mr, mm * 32 + tid / BD, tid % BD) *
+                         dscale[static_cast<size_t>(mm) * N + tid];
             };
             auto unpack = [&](const u32x4(&reg)[PER], float xr) {
 #pragma unroll
@@ -499,7 +544,8 @@ namespace psmas
             };
             if (static_cast<int>(blockIdx.x) < sd.wgs)
             {
-                apply_side_work<BD, T>(sd, mr, mz, blockIdx.x);
+                apply_side_work<BD, T>(sd, mr, mz, blockIdx.x, reinterpret_cast<T *>(sA),
+                                       static_cast<int>(sizeof(sA) / (sizeof(T))));
                 return;
             }
             const int nwg = static_cast<int>(gridDim.x) - sd.wgs; // apply workgroups
@@ -522,7 +568,7 @@ namespace psmas
                 }
                 if (threadIdx.x < N)
                 {
-                    xr = static_cast<TA>(mr[static_cast<size_t>(mm) * N + threadIdx.x]);
+                    xr = static_cast<TA>(coarse_input<BD, T>(sd, mr, mm * 32 + threadIdx.x / BD, threadIdx.x % BD));
                     if (SCALED)
                         xr *= static_cast<TA>(dscale[static_cast<size_t>(mm) * N + threadIdx.x]);
                 }
@@ -575,22 +621,6 @@ namespace psmas
                 __syncthreads(); // sA, sx and part are rewritten for the next matrix
                 m = next;
             }
-        }
-
-        template <int BD, typename T>
-        __global__ __launch_bounds__(256) void k_restrict_combine(int n, const int4 *__restrict__ comb,
-                                                                  const double *__restrict__ rpart,
-                                                                  T *__restrict__ out)
-        {
-            const int i = blockIdx.x * 256 + threadIdx.x;
-            if (i >= n * BD)
-                return;
-            const int4 c = comb[i / BD];
-            const int d = i % BD;
-            double acc = 0.0;
-            for (int k = 0; k < c.z; ++k)
-                acc += rpart[static_cast<size_t>(c.y + k) * BD + d];
-            out[static_cast<size_t>(c.x) * BD + d] = static_cast<T>(acc);
         }
 
         // ----------------------------------------------------------------------- prolongation
@@ -1155,16 +1185,6 @@ namespace psmas
     }
 
     template <int BD, typename T>
-    void launch_restrict_combine(int n, const int4 *comb, const double *rpart, T *out, hipStream_t s)
-    {
-        if (n <= 0)
-            return;
-        hipLaunchKernelGGL((k_restrict_combine<BD, T>), dim3(div_up(n * BD, 256)), dim3(256), 0, s, n, comb, rpart,
-                           out);
-        PSMAS_LAUNCH_CHECK();
-    }
-
-    template <int BD, typename T>
     void launch_prolong_dot(int nblocks, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,
                             const Reductions &red, bool init, const SysMap &sm, const LccApply &lcc,
                             const FlexDot<T> &flex, hipStream_t s)
@@ -1296,8 +1316,6 @@ namespace psmas
     template void launch_mas_apply<BD, __half, float>(int, int, const __half *, const float *, const float *,     \
                                                       float *, const int *, const int *, hipStream_t,            \
                                                       const ApplySide &);                                        \
-    template void launch_restrict_combine<BD, double>(int, const int4 *, const double *, double *, hipStream_t);   \
-    template void launch_restrict_combine<BD, float>(int, const int4 *, const double *, float *, hipStream_t);     \
     template void launch_prolong_dot<BD, double>(int, int, const int4 *, const int *, const double *, double *,   \
                                                  const Reductions &, bool, const SysMap &, const LccApply &,     \
                                                  const FlexDot<double> &, hipStream_t);                          \

@@ -716,6 +716,7 @@ namespace psmas
         d_rlist_.reset();
         d_rcomb_.reset();
         d_rpart_.reset();
+        d_split_of_.reset();
         if (level_num_ < 2)
             return;
         // Ancestors of every segment-1 node in segments 2.. (the same for all its fine slots).
@@ -779,6 +780,11 @@ namespace psmas
             d_rcomb_.allocate(comb.size(), "MAS restriction combines");
             PSMAS_HIP(hipMemcpy(d_rcomb_.data(), comb.data(), d_rcomb_.bytes(), hipMemcpyHostToDevice));
             d_rpart_.allocate(static_cast<size_t>(nparts) * bd_, "MAS restriction piece sums");
+            std::vector<int> split_of(nrel, -1);
+            for (int i = 0; i < nrcomb_; ++i)
+                split_of[comb[i].x - node0] = i;
+            d_split_of_.allocate(split_of.size(), "MAS restriction split nodes");
+            PSMAS_HIP(hipMemcpy(d_split_of_.data(), split_of.data(), d_split_of_.bytes(), hipMemcpyHostToDevice));
         }
         small.insert(small.end(), big.begin(), big.end());
         d_rtask_.allocate(small.size(), "MAS restriction tasks");
@@ -817,6 +823,13 @@ namespace psmas
     ApplySide Solver::coarse_side(int block) const
     {
         ApplySide sd;
+        if (nrcomb_ > 0)
+        {
+            sd.split_of = d_split_of_.data();
+            sd.rcomb = d_rcomb_.data();
+            sd.rpart = const_cast<double *>(d_rpart_.data());
+            sd.node0 = lev_off_[2];
+        }
         if (params_.coarse_solver == CoarseSolver::Cholesky && dense_nd_ > 0)
         {
             sd.dense_nd = dense_nd_;
@@ -1088,7 +1101,6 @@ namespace psmas
                                              nullptr, stream_,
                                              restrict_side(d_mr_.data() + static_cast<size_t>(lev_off_[1]) * BD,
                                                            d_mr_.data(), block));
-        launch_restrict_combine<BD, double>(nrcomb_, d_rcomb_.data(), d_rpart_.data(), d_mr_.data(), stream_);
         toc();
         const ApplySide cs = coarse_side(block);
         if (total_mats_ > n1 || cs.wgs > 0)
@@ -1274,7 +1286,6 @@ namespace psmas
                                             d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_,
                                             restrict_side(d_mr32_.data() + static_cast<size_t>(lev_off_[1]) * BD,
                                                           d_mr32_.data(), block, lat_blocks));
-        launch_restrict_combine<BD, float>(nrcomb_, d_rcomb_.data(), d_rpart_.data(), d_mr32_.data(), stream_);
         toc();
         const ApplySide cs = coarse_side(block);
         if (total_mats_ > n1 || cs.wgs > 0)

@@ -172,12 +172,14 @@ namespace psmas
         // Restriction of coarse segments 2.. straight from segment 1: every node's segment-1
         // descendants (ascending), summed by the side workgroups of the first apply launch (a
         // thread per node of at most RESTRICT_SMALL_MAX descendants, a wave per piece of at most
-        // RESTRICT_PIECE descendants of a larger node; pieces of one node are combined after).
+        // RESTRICT_PIECE descendants of a larger node; the launch that reads such a node adds its
+        // pieces).
         int nrsmall_ = 0, nrbig_ = 0, nrcomb_ = 0;
         DeviceBuffer<int4> d_rtask_;
         DeviceBuffer<int> d_rlist_;
         DeviceBuffer<int4> d_rcomb_;   ///< nodes cut into pieces: {node, first partial, pieces}
         DeviceBuffer<double> d_rpart_; ///< piece sums of those nodes
+        DeviceBuffer<int> d_split_of_; ///< per node of segments 2..: index into d_rcomb_ or -1
 
         // vectors
         DeviceBuffer<double> d_b_, d_x_, d_mr_, d_mz_, d_p_, d_Ap_, d_io_;
