@@ -3,6 +3,9 @@
 // are interpreted.
 #include "params.hpp"
 #include "common.hpp"
+#include "kernels.hpp"
+
+static_assert(psmas::RESTRICT_SMALL_MAX == 8 && psmas::RESTRICT_PIECE == 512, "restrict_piece range message");
 
 #include <cctype>
 #include <cmath>
@@ -271,7 +274,8 @@ namespace psmas
             "use_preconditioned_residual_norm", "precision", "aggregation", "coarse_solver",
             "reliable_update_delta", "check_interval", "true_residual_period", "use_graphs",
             "learned_correction", "division_guard", "learned_collect", "lattice_points", "learned_hidden",
-            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll"};
+            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll",
+            "restrict_piece"};
     } // namespace
 
     void apply_json_parameters(Params &p, const std::string &json)
@@ -405,6 +409,12 @@ namespace psmas
             if (q.spmv_unroll != 2 && q.spmv_unroll != 4 && q.spmv_unroll != 8)
                 bad("spmv_unroll", "must be 2, 4 or 8");
         }
+        if (const JValue *v = obj->get("restrict_piece"))
+        {
+            q.restrict_piece = as_int(*v, "restrict_piece");
+            if (q.restrict_piece <= RESTRICT_SMALL_MAX || q.restrict_piece > RESTRICT_PIECE)
+                bad("restrict_piece", "must be in [9, 512]");
+        }
         if (const JValue *v = obj->get("learned_output_scale"))
         {
             q.learned_output_scale = as_double(*v, "learned_output_scale");
@@ -443,7 +453,7 @@ namespace psmas
           << ", \"learned_random_pairs\": " << p.learned_random_pairs
           << ", \"learned_snapshots\": " << p.learned_snapshots
           << ", \"learned_stall_segments\": " << p.learned_stall_segments
-          << ", \"learned_output_scale\": " << p.learned_output_scale << ", \"spmv_unroll\": " << p.spmv_unroll
+          << ", \"learned_output_scale\": " << p.learned_output_scale << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"restrict_piece\": " << p.restrict_piece
           << "}";
         return o.str();
     }

@@ -28,6 +28,9 @@
 //                        correction is switched off for the rest of the solve
 //   learned_output_scale factor on the learned correction (1 = as trained)
 //   spmv_unroll          SpMV row-loop batch depth 2 | 4 | 8 (chosen by psmas_autotune)
+//   restrict_piece       longest level-1 descendant list one wave sums when restricting coarse
+//                        levels 2.. (longer lists are cut into pieces added where they are read);
+//                        9..512, default 512; takes effect at the next factorize
 //   division_guard       alpha/beta are set to 0 when |denominator| < division_guard; 0 (default)
 //                        sets them to 0 only for a non-finite quotient; 1e-20 is the reference's
 //                        absolute rule (MASSolver.cu:43-56)
@@ -82,6 +85,7 @@ namespace psmas
         int learned_stall_segments = 3;
         double learned_output_scale = 1.0;
         int spmv_unroll = 4;
+        int restrict_piece = 512;
     };
 
     /// Apply a JSON object to params.  Accepts either {"MAS": {...}} (only the "MAS" member is

@@ -251,6 +251,8 @@ namespace psmas
           << ", \"solve_ms\": " << solve_ms_ << ", \"levels\": " << level_num_ << ", \"parts\": " << S.nparts
           << ", \"systems\": " << sys_.size() << ", \"aggregation\": \"" << S.aggregation << "\""
           << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\""
+          << ", \"coarse_restriction\": {\"thread_nodes\": " << nrsmall_ << ", \"wave_pieces\": " << nrbig_
+          << ", \"split_nodes\": " << nrcomb_ << "}"
           << ", \"learned\": {\"used\": " << (learned_used_ ? "true" : "false")
           << ", \"fallback\": " << (learned_fallback_ ? "true" : "false")
           << ", \"fallback_iteration\": " << learned_fallback_iter_
@@ -762,13 +764,13 @@ namespace psmas
             const int b = eoff[i], e = eoff[i + 1];
             if (e - b <= RESTRICT_SMALL_MAX)
                 small.push_back(int4{node0 + i, b, e, -1});
-            else if (e - b <= RESTRICT_PIECE)
+            else if (e - b <= params_.restrict_piece)
                 big.push_back(int4{node0 + i, b, e, -1});
             else
             {
                 const int first = nparts;
-                for (int k = b; k < e; k += RESTRICT_PIECE)
-                    big.push_back(int4{node0 + i, k, std::min(k + RESTRICT_PIECE, e), nparts++});
+                for (int k = b; k < e; k += params_.restrict_piece)
+                    big.push_back(int4{node0 + i, k, std::min(k + params_.restrict_piece, e), nparts++});
                 comb.push_back(int4{node0 + i, first, nparts - first, 0});
             }
         }

@@ -112,6 +112,32 @@ def test_apply_fp64_matches_oracle(gpu, cfg1, agg):
         assert np.linalg.norm(z - zr) <= 1e-10 * np.linalg.norm(zr)
 
 
+def test_split_restriction_lists_match(gpu, het12):
+    """Coarse levels 2.. are restricted straight from level 1; with restrict_piece = 9 every
+    descendant list longer than 9 is cut into pieces whose sums the second apply launch adds.
+    Both applies must equal the whole-list ones (FP64 to rounding) and the oracle."""
+    base = _factorized(het12, {"coarse_solver": "cholesky"})
+    cut = _factorized(het12, {"coarse_solver": "cholesky", "restrict_piece": 9})
+    assert base.level_info()["level_num"] >= 2
+    assert base.get_info()["coarse_restriction"]["split_nodes"] == 0
+    info = cut.get_info()["coarse_restriction"]
+    assert info["split_nodes"] > 0 and info["wave_pieces"] > info["split_nodes"], info
+    offsets, perm = cut.partition()
+    O = oracle.MasOracle(het12.A, 3, offsets, perm)
+    rng = np.random.default_rng(5)
+    for _ in range(2):
+        r = rng.standard_normal(het12.n)
+        z0, z1 = base.apply_preconditioner(r), cut.apply_preconditioner(r)
+        assert np.linalg.norm(z1 - z0) <= 1e-13 * np.linalg.norm(z0)
+        m0, m1 = base.apply_preconditioner(r, mixed=True), cut.apply_preconditioner(r, mixed=True)
+        assert np.linalg.norm(m1 - m0) <= 1e-6 * np.linalg.norm(m0)
+    # Without the dense coarsest solve the FP64 apply is the reference operator.
+    cut.set_parameters({"MAS": {"coarse_solver": "bank"}})
+    r = rng.standard_normal(het12.n)
+    zr = O.apply(r)
+    assert np.linalg.norm(cut.apply_preconditioner(r) - zr) <= 1e-10 * np.linalg.norm(zr)
+
+
 def test_apply_mixed_close_to_fp64(gpu, het12):
     s = _factorized(het12)
     rng = np.random.default_rng(3)
