@@ -13,12 +13,34 @@ namespace psmas
 {
     constexpr int WAVE = 64;
 
+    /// v from another lane of the same row, moved by DPP (CTRL: a DPP control word); a lane whose
+    /// source is inactive reads 0.
+    template <int CTRL>
+    __device__ __forceinline__ double dpp_move(double v)
+    {
+        const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+        return __hiloint2double(hi, lo);
+    }
+
+    __device__ __forceinline__ double read_lane(double v, int lane)
+    {
+        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                                __builtin_amdgcn_readlane(__double2loint(v), lane));
+    }
+
+    /// Sum over the 64 lanes of a wave, returned to every lane; all 64 lanes must be active.
+    /// Within each 16-lane row, DPP moves add lane pairs (xor 1), quads (xor 2), then mirror the
+    /// half rows and rows, so every lane holds its row's sum; the four row sums are then added
+    /// as (r0 + r1) + (r2 + r3).  A fixed order, so every wave gets the same bits for the same
+    /// values; no LDS traffic (a shuffle through ds_bpermute would be an LDS round trip per step).
     __device__ __forceinline__ double wave_sum(double v)
     {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
-            v += __shfl_xor(v, off, WAVE);
-        return v;
+        v += dpp_move<0xB1>(v);  // quad_perm [1, 0, 3, 2]
+        v += dpp_move<0x4E>(v);  // quad_perm [2, 3, 0, 1]
+        v += dpp_move<0x141>(v); // row_half_mirror
+        v += dpp_move<0x140>(v); // row_mirror
+        return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
     }
 
     /// Sum over the workgroup; the result is valid in thread 0.  Fixed combination order.
