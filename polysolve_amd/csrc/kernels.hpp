@@ -169,22 +169,9 @@ namespace psmas
     void launch_set_scalar(double *sc, int count, int slot, double value, hipStream_t s);
 
     // ----- MAS apply ----------------------------------------------------------------------
-    struct CoarseRestrict
-    {
-        int level_num = 0;                  ///< coarse levels
-        int counts[4] = {0, 0, 0, 0};       ///< coarse nodes per level
-        int offsets[5] = {0, 0, 0, 0, 0};   ///< node offset of multi-level segment (0 = fine)
-        const uint32_t *lead[4] = {nullptr, nullptr, nullptr, nullptr}; ///< index: level-(l-1) node, l >= 1
-        const int *gid[4] = {nullptr, nullptr, nullptr, nullptr};
-    };
-
-    /// Restrict coarse level 0 to levels 1..L-1: one launch per level, one thread per node; a
-    /// level-l node sums at most 32 level-(l-1) members (its component inside one bank).
-    template <int BD, typename T>
-    void launch_restrict_coarse(const CoarseRestrict &cr, T *mr, hipStream_t s);
-
     /// y_m = Inv_m x_m for matrices [mat_begin, mat_begin + count) of the multi-level vectors.
-    /// Storage: packed upper row-major.  Half storage is scaled: Inv = D S D with D per row.
+    /// FP64 storage: packed upper row-major.  FP16 storage is scaled, Inv = D S D with D per row
+    /// (dscale), and S is tiled in 8 x 8 blocks with an implicit unit diagonal (k_make_half).
     /// bank_sys/active (may be null): skip matrices of systems that stopped iterating
     /// (bank_sys[m] = system of matrix m, -1 for coarse matrices, which are never skipped).
     /// Longest descendant list of a restriction task that takes one thread; longer lists are cut

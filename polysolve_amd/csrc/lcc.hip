@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace psmas
 {
@@ -16,12 +17,20 @@ namespace psmas
         constexpr int LCC_CHUNK = 1024; ///< restriction entries per workgroup
         constexpr int GEMV_SYS = 8;     ///< systems per wave in the inference GEMV
 
+        /// Sum over the 64 lanes of a wave (all active), returned to every lane: the same DPP row
+        /// moves and fixed row order as wave_sum, in FP32.
         __device__ __forceinline__ float wave_sum_f(float v)
         {
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1)
-                v += __shfl_xor(v, off, WAVE);
-            return v;
+            auto mv = [](float x, auto ctrl) {
+                return __int_as_float(
+                    __builtin_amdgcn_update_dpp(0, __float_as_int(x), decltype(ctrl)::value, 0xF, 0xF, false));
+            };
+            v += mv(v, std::integral_constant<int, 0xB1>());  // quad_perm [1, 0, 3, 2]
+            v += mv(v, std::integral_constant<int, 0x4E>());  // quad_perm [2, 3, 0, 1]
+            v += mv(v, std::integral_constant<int, 0x141>()); // row_half_mirror
+            v += mv(v, std::integral_constant<int, 0x140>()); // row_mirror
+            auto rl = [&](int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); };
+            return (rl(0) + rl(16)) + (rl(32) + rl(48));
         }
 
         __device__ __forceinline__ float bf16_lo(unsigned v) { return __uint_as_float(v << 16); }
@@ -179,7 +188,8 @@ namespace psmas
             }
 #pragma unroll
             for (int j = 0; j < GEMV_SYS; ++j)
-                acc[j] = wave_sum_f(acc[j]);
+                if (j < ns) // ns is the same for the whole workgroup
+                    acc[j] = wave_sum_f(acc[j]);
             if (lane != 0)
                 return;
 #pragma unroll
@@ -266,7 +276,8 @@ namespace psmas
             }
 #pragma unroll
             for (int j = 0; j < GEMV_SYS; ++j)
-                acc[j] = wave_sum_f(acc[j]);
+                if (j < ns) // ns is the same for the whole workgroup
+                    acc[j] = wave_sum_f(acc[j]);
             if (lane != 0)
                 return;
 #pragma unroll

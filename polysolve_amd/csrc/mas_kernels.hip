@@ -31,39 +31,6 @@ namespace psmas
 
     namespace
     {
-        // ------------------------------------------------------------------------ restriction
-        /// Level l from level l-1: one thread per level-(l-1) node; a leader sums the members of its
-        /// component (bit b of its mask = node base+b of the same 32-id group) in bit order.
-        template <int BD, typename T>
-        __global__ __launch_bounds__(256) void k_restrict_level(int count, const uint32_t *__restrict__ lead,
-                                                                const int *__restrict__ gid,
-                                                                const T *__restrict__ src, T *__restrict__ dst)
-        {
-            const int q = blockIdx.x * 256 + threadIdx.x;
-            if (q >= count)
-                return;
-            uint32_t mask = lead[q];
-            if (mask == 0u)
-                return;
-            const int g = gid[q];
-            const int base = q & ~31;
-            double s[BD];
-#pragma unroll
-            for (int i = 0; i < BD; ++i)
-                s[i] = 0.0;
-            while (mask)
-            {
-                const int b = __builtin_ctz(mask);
-                mask &= mask - 1u;
-#pragma unroll
-                for (int i = 0; i < BD; ++i)
-                    s[i] += static_cast<double>(src[static_cast<size_t>(base + b) * BD + i]);
-            }
-#pragma unroll
-            for (int i = 0; i < BD; ++i)
-                dst[static_cast<size_t>(g) * BD + i] = static_cast<T>(s[i]);
-        }
-
         // ------------------------------------------------------------------------------ apply
         template <int BD>
         constexpr int apply_block() { return BD == 1 ? 64 : (BD == 2 ? 128 : 192); }
@@ -1131,23 +1098,6 @@ namespace psmas
         }
     } // namespace
 
-    template <int BD, typename T>
-    void launch_restrict_coarse(const CoarseRestrict &cr, T *mr, hipStream_t s)
-    {
-        if (cr.level_num <= 1)
-            return;
-        for (int l = 1; l < cr.level_num; ++l)
-        {
-            const int count = cr.counts[l - 1];
-            if (count <= 0)
-                continue;
-            hipLaunchKernelGGL((k_restrict_level<BD, T>), dim3(div_up(count, 256)), dim3(256), 0, s, count, cr.lead[l],
-                               cr.gid[l], mr + static_cast<size_t>(cr.offsets[l]) * BD,
-                               mr + static_cast<size_t>(cr.offsets[l + 1]) * BD);
-            PSMAS_LAUNCH_CHECK();
-        }
-    }
-
     template <int BD, typename TM, typename T>
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
                           const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side)
@@ -1305,8 +1255,6 @@ namespace psmas
     }
 
 #define PSMAS_INST_APPLY(BD)                                                                                     \
-    template void launch_restrict_coarse<BD, double>(const CoarseRestrict &, double *, hipStream_t);             \
-    template void launch_restrict_coarse<BD, float>(const CoarseRestrict &, float *, hipStream_t);               \
     template void launch_mas_apply<BD, double, double>(int, int, const double *, const float *, const double *,   \
                                                        double *, const int *, const int *, hipStream_t,          \
                                                        const ApplySide &);                                       \

@@ -317,22 +317,6 @@ namespace psmas
         return f;
     }
 
-    CoarseRestrict Solver::coarse_restrict() const
-    {
-        CoarseRestrict c;
-        c.level_num = level_num_;
-        for (int l = 0; l < MAX_COARSE_LEVELS; ++l)
-            c.counts[l] = cco_[l];
-        for (int l = 0; l <= MAX_COARSE_LEVELS; ++l)
-            c.offsets[l] = lev_off_[l];
-        for (int l = 1; l < MAX_COARSE_LEVELS; ++l)
-        {
-            c.lead[l] = d_leadc_[l].data();
-            c.gid[l] = d_gidc_[l].data();
-        }
-        return c;
-    }
-
     template <typename TV>
     SellView<TV> Solver::sell(const TV *vals) const
     {

@@ -212,7 +212,6 @@ namespace psmas
         void finalize_scalars();
         SysMap sysmap() const;
         FineRestrict fine_restrict() const;
-        CoarseRestrict coarse_restrict() const;
         template <typename TV>
         SellView<TV> sell(const TV *vals) const;
 
