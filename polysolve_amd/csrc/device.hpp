@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -42,6 +43,25 @@ namespace psmas
         v += dpp_move<0x140>(v); // row_mirror
         return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
     }
+
+    /// FP32 version of wave_sum (same DPP row moves and row order).
+    __device__ __forceinline__ float wave_sum(float v)
+    {
+        auto mv = [](float x, auto ctrl) {
+            return __int_as_float(
+                __builtin_amdgcn_update_dpp(0, __float_as_int(x), decltype(ctrl)::value, 0xF, 0xF, false));
+        };
+        v += mv(v, std::integral_constant<int, 0xB1>());  // quad_perm [1, 0, 3, 2]
+        v += mv(v, std::integral_constant<int, 0x4E>());  // quad_perm [2, 3, 0, 1]
+        v += mv(v, std::integral_constant<int, 0x141>()); // row_half_mirror
+        v += mv(v, std::integral_constant<int, 0x140>()); // row_mirror
+        auto rl = [&](int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); };
+        return (rl(0) + rl(16)) + (rl(32) + rl(48));
+    }
+
+    /// The two BF16 values packed in a 32-bit word, as FP32.
+    __device__ __forceinline__ float bf16_lo(unsigned v) { return __uint_as_float(v << 16); }
+    __device__ __forceinline__ float bf16_hi(unsigned v) { return __uint_as_float(v & 0xffff0000u); }
 
     /// Sum over the workgroup; the result is valid in thread 0.  Fixed combination order.
     template <int BLOCK>

@@ -211,6 +211,18 @@ namespace psmas
         int dense_seg = 0;
         const int *dense_index = nullptr;
         int dense_count = 0;
+        // Layer 1 and the skip layer of the learned correction's network for a single system: every
+        // side workgroup stages x = u sqrt(m) / |u| (zero-padded to lin_md) in LDS after the dense
+        // inputs; waves take rows (row < lin_h: h1 = relu(W1 x + b1), else skip = S x); workgroup 0
+        // writes scale = |u| lin_ascale / sqrt(m).  Skipped when lin_active[0] == 0.
+        int lin_rows = 0; ///< lin_h + lin_md (0: none)
+        int lin_m = 0, lin_md = 0, lin_h = 0;
+        const double *lin_u = nullptr;
+        const unsigned short *lin_w1 = nullptr, *lin_s = nullptr;
+        const float *lin_b1 = nullptr;
+        float *lin_h1 = nullptr, *lin_skip = nullptr, *lin_scale = nullptr;
+        float lin_ascale = 0.0f;
+        const int *lin_active = nullptr;
         // Lattice gather of the learned correction: u[J] = sum of lat_partial[lat_src[i]] over
         // i in [lat_ptr[J], lat_ptr[J + 1]), J < lat_npts (wave per point, fixed-order reduction).
         int lat_npts = 0;
@@ -223,6 +235,9 @@ namespace psmas
     template <int BD, typename TM, typename T>
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
                           const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side = ApplySide{});
+
+    /// Elements of the vector type the FP16 apply kernel's side workgroups can stage in LDS.
+    int apply_h16_stage_floats(int bd);
 
 
     /// z_p = mz_p + sum_l mz_l[cmap_p.l] (+ the lattice correction P c when lcc.cell is set);

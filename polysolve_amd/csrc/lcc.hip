@@ -17,24 +17,6 @@ namespace psmas
         constexpr int LCC_CHUNK = 1024; ///< restriction entries per workgroup
         constexpr int GEMV_SYS = 8;     ///< systems per wave in the inference GEMV
 
-        /// Sum over the 64 lanes of a wave (all active), returned to every lane: the same DPP row
-        /// moves and fixed row order as wave_sum, in FP32.
-        __device__ __forceinline__ float wave_sum_f(float v)
-        {
-            auto mv = [](float x, auto ctrl) {
-                return __int_as_float(
-                    __builtin_amdgcn_update_dpp(0, __float_as_int(x), decltype(ctrl)::value, 0xF, 0xF, false));
-            };
-            v += mv(v, std::integral_constant<int, 0xB1>());  // quad_perm [1, 0, 3, 2]
-            v += mv(v, std::integral_constant<int, 0x4E>());  // quad_perm [2, 3, 0, 1]
-            v += mv(v, std::integral_constant<int, 0x141>()); // row_half_mirror
-            v += mv(v, std::integral_constant<int, 0x140>()); // row_mirror
-            auto rl = [&](int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); };
-            return (rl(0) + rl(16)) + (rl(32) + rl(48));
-        }
-
-        __device__ __forceinline__ float bf16_lo(unsigned v) { return __uint_as_float(v << 16); }
-        __device__ __forceinline__ float bf16_hi(unsigned v) { return __uint_as_float(v & 0xffff0000u); }
 
         /// partial[c][d] = sum over chunk c's entries (slot, w) of w * v[slot][d], in entry order.
         template <int BD, typename T>
@@ -189,7 +171,7 @@ namespace psmas
 #pragma unroll
             for (int j = 0; j < GEMV_SYS; ++j)
                 if (j < ns) // ns is the same for the whole workgroup
-                    acc[j] = wave_sum_f(acc[j]);
+                    acc[j] = wave_sum(acc[j]);
             if (lane != 0)
                 return;
 #pragma unroll
@@ -277,7 +259,7 @@ namespace psmas
 #pragma unroll
             for (int j = 0; j < GEMV_SYS; ++j)
                 if (j < ns) // ns is the same for the whole workgroup
-                    acc[j] = wave_sum_f(acc[j]);
+                    acc[j] = wave_sum(acc[j]);
             if (lane != 0)
                 return;
 #pragma unroll
@@ -601,7 +583,7 @@ namespace psmas
         sd.lat_u = const_cast<double *>(d_u_.data());
     }
 
-    void LearnedCoarse::infer(const int *active, hipStream_t s)
+    void LearnedCoarse::check_network()
     {
         if (!net_)
             throw std::runtime_error("[MAS] learned correction: the network is not trained.");
@@ -616,9 +598,55 @@ namespace psmas
             d_h2_.allocate(static_cast<size_t>(nsys_) * h, "MAS network activations");
             d_skip_.allocate(static_cast<size_t>(nsys_) * md_, "MAS network skip output");
         }
+    }
+
+    bool LearnedCoarse::in_side(ApplySide &sd, int stage_floats, int block, const int *active)
+    {
+        check_network();
+        const int dense = sd.dense_nd > 0 ? sd.dense_nd : 0;
+        if (nsys_ != 1 || dense + md_ > stage_floats)
+            return false;
         const unsigned short *wb = net_->bf16_weights();
         const float *wf = net_->fp32_weights();
+        const int h = net_->hidden();
+        sd.lin_rows = h + md_;
+        sd.lin_m = m();
+        sd.lin_md = md_;
+        sd.lin_h = h;
+        sd.lin_u = d_u_.data();
+        sd.lin_w1 = wb + net_->offset(0);
+        sd.lin_b1 = wf + net_->offset(1);
+        sd.lin_s = wb + net_->offset(6);
+        sd.lin_h1 = d_h1_.data();
+        sd.lin_skip = d_skip_.data();
+        sd.lin_scale = d_scale_.data();
+        sd.lin_ascale = ascale_ * out_factor_;
+        sd.lin_active = active;
+        sd.wgs = std::max(sd.wgs, std::min(256, div_up(sd.lin_rows, block / 64))); // about a row per wave
+        return true;
+    }
+
+    void LearnedCoarse::infer_out(const int *active, hipStream_t s)
+    {
+        const unsigned short *wb = net_->bf16_weights();
+        const float *wf = net_->fp32_weights();
+        const int h = net_->hidden();
         const int ngroups = (nsys_ + GEMV_SYS - 1) / GEMV_SYS;
+        // Layer 2 fused into the output layer (only the m real outputs), plus the skip output,
+        // times the scale.
+        hipLaunchKernelGGL(k_lcc_out, dim3(div_up(m(), 4) * ngroups), dim3(256), 0, s, wb + net_->offset(2),
+                           wf + net_->offset(3), h, static_cast<const float *>(d_h1_.data()), wb + net_->offset(4),
+                           wf + net_->offset(5), m(), static_cast<const float *>(d_skip_.data()), md_,
+                           static_cast<const float *>(d_scale_.data()), d_coef_.data(), m(), nsys_, active);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void LearnedCoarse::infer(const int *active, hipStream_t s)
+    {
+        check_network();
+        const int h = net_->hidden();
+        const unsigned short *wb = net_->bf16_weights();
+        const float *wf = net_->fp32_weights();
         // Layer 1 and the skip layer read the same normalised input, built in the kernel.
         const int gsize = std::max(1, std::min(GEMV_SYS, static_cast<int>(60000 / (sizeof(float) * md_))));
         const int ngroups_in = div_up(nsys_, gsize);
@@ -628,13 +656,7 @@ namespace psmas
                            h, wb + net_->offset(6), d_h1_.data(), d_skip_.data(), d_scale_.data(), ascale_ * out_factor_,
                            nsys_, gsize, active);
         PSMAS_LAUNCH_CHECK();
-        // Layer 2 fused into the output layer (only the m real outputs), plus the skip output,
-        // times the scale.
-        hipLaunchKernelGGL(k_lcc_out, dim3(div_up(m(), 4) * ngroups), dim3(256), 0, s, wb + net_->offset(2),
-                           wf + net_->offset(3), h, static_cast<const float *>(d_h1_.data()), wb + net_->offset(4),
-                           wf + net_->offset(5), m(), static_cast<const float *>(d_skip_.data()), md_,
-                           static_cast<const float *>(d_scale_.data()), d_coef_.data(), m(), nsys_, active);
-        PSMAS_LAUNCH_CHECK();
+        infer_out(active, s);
     }
 
     LccApply LearnedCoarse::apply_view() const

@@ -57,6 +57,12 @@ namespace psmas
         LatticeBlocks block_view() const;
         /// Fills the lattice gather (partials -> u) of an apply launch's side work.
         void gather_into(ApplySide &sd) const;
+        /// Single system whose network input (and the dense inputs before it) fit in
+        /// `stage_floats`: fills the layer-1 / skip side work of an apply launch (more side
+        /// workgroups if its rows need them) and returns true; infer_out then finishes.
+        bool in_side(ApplySide &sd, int stage_floats, int block, const int *active);
+        /// The output layer only (after in_side's side work ran).
+        void infer_out(const int *active, hipStream_t s);
         /// Output factor of the next inference (the restriction sets it when it runs here).
         void set_output_factor(float f) { out_factor_ = f; }
         /// Network inference on the normalised inputs -> coef [nsys][m] (scaled back).
@@ -105,6 +111,7 @@ namespace psmas
         DeviceBuffer<float> d_h1_, d_h2_, d_skip_, d_coef_;
         std::vector<double> ptp_inv_;     ///< (P^T P)^-1 of system 0, np x np (on first use)
         void ensure_ptp();
+        void check_network(); ///< throws unless a network of the lattice's width is ready
         std::vector<int4> h_cell0_;       ///< host copy, system 0
 
         // network and training data

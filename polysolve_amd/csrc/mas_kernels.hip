@@ -210,6 +210,62 @@ namespace psmas
                     }
                 }
             }
+            if constexpr (std::is_same<T, float>::value)
+            {
+                if (sd.lin_rows > 0 && (sd.lin_active == nullptr || sd.lin_active[0] != 0))
+                {
+                    // Normalised lattice residual in LDS, after the dense inputs.
+                    __shared__ double lin_red[4];
+                    float *xs = stage + (sd.dense_nd > 0 ? sd.dense_nd : 0);
+                    const int m = sd.lin_m, md = sd.lin_md, h = sd.lin_h;
+                    const int nwb = static_cast<int>(blockDim.x) / 64;
+                    __syncthreads(); // the dense section may still read stage
+                    double q = 0.0;
+                    for (int i = threadIdx.x; i < m; i += blockDim.x)
+                        q += sd.lin_u[i] * sd.lin_u[i];
+                    q = wave_sum(q);
+                    if (lane == 0)
+                        lin_red[threadIdx.x / 64] = q;
+                    __syncthreads();
+                    double nu2 = 0.0;
+                    for (int w = 0; w < nwb; ++w)
+                        nu2 += lin_red[w];
+                    const double nu = sqrt(nu2);
+                    const double fac = nu > 0.0 ? sqrt(static_cast<double>(m)) / nu : 0.0;
+                    if (wg == 0 && threadIdx.x == 0)
+                        sd.lin_scale[0] = static_cast<float>(nu * sd.lin_ascale / sqrt(static_cast<double>(m)));
+                    for (int i = threadIdx.x; i < md; i += blockDim.x)
+                        xs[i] = i < m ? static_cast<float>(sd.lin_u[i] * fac) : 0.0f;
+                    __syncthreads();
+                    for (int row = t >> 6; row < sd.lin_rows; row += nt >> 6)
+                    {
+                        const bool first = row < h;
+                        const int r = first ? row : row - h;
+                        const uint4 *wrow =
+                            reinterpret_cast<const uint4 *>((first ? sd.lin_w1 : sd.lin_s) + static_cast<size_t>(r) * md);
+                        float acc = 0.0f;
+                        for (int c = lane; c < md / 8; c += 64)
+                        {
+                            const uint4 wv = wrow[c];
+                            const float *x = xs + c * 8;
+                            acc += bf16_lo(wv.x) * x[0] + bf16_hi(wv.x) * x[1] + bf16_lo(wv.y) * x[2] +
+                                   bf16_hi(wv.y) * x[3] + bf16_lo(wv.z) * x[4] + bf16_hi(wv.z) * x[5] +
+                                   bf16_lo(wv.w) * x[6] + bf16_hi(wv.w) * x[7];
+                        }
+                        acc = wave_sum(acc);
+                        if (lane == 0)
+                        {
+                            if (first)
+                            {
+                                const float y = acc + sd.lin_b1[r];
+                                sd.lin_h1[r] = y > 0.0f ? y : 0.0f;
+                            }
+                            else
+                                sd.lin_skip[r] = acc;
+                        }
+                    }
+                }
+            }
         }
 
         // FP16 tiled storage of a scaled bank matrix S (N = 32 BD, T = N / 8 tiles of 8 x 8 per
@@ -1132,6 +1188,12 @@ namespace psmas
                                mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side);
         }
         PSMAS_LAUNCH_CHECK();
+    }
+
+    int apply_h16_stage_floats(int bd)
+    {
+        const int n = 32 * bd;
+        return n * (n + 8) / 2; // the __half LDS copy of one matrix, as floats
     }
 
     template <int BD, typename T>

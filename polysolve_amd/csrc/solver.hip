@@ -1273,7 +1273,10 @@ namespace psmas
                                             restrict_side(d_mr32_.data() + static_cast<size_t>(lev_off_[1]) * BD,
                                                           d_mr32_.data(), block, lat_blocks));
         toc();
-        const ApplySide cs = coarse_side(block);
+        // The network's first layer and skip layer join the second launch's side work when they
+        // fit; the output layer follows it.
+        ApplySide cs = coarse_side(block);
+        const bool lin_side = learned && lcc_->in_side(cs, apply_h16_stage_floats(BD), block, d_active_.data());
         if (total_mats_ > n1 || cs.wgs > 0)
         {
             tic("mas_apply_fp16_coarse", (total_mats_ - n1) * per_mat + static_cast<double>(dense_nd_) * dense_nd_ * 8.0);
@@ -1284,8 +1287,11 @@ namespace psmas
         }
         if (learned)
         {
-            tic("lcc_infer", static_cast<double>(lcc_->inference_weight_bytes()));
-            lcc_->infer(d_active_.data(), stream_);
+            tic(lin_side ? "lcc_infer_out" : "lcc_infer", static_cast<double>(lcc_->inference_weight_bytes()));
+            if (lin_side)
+                lcc_->infer_out(d_active_.data(), stream_);
+            else
+                lcc_->infer(d_active_.data(), stream_);
             toc();
         }
         tic(learned ? "prolong_dot_learned" : "prolong_dot", static_cast<double>(n_total_) * 4.0 * 3 + nblocks_ * 256.0 * 16);
