@@ -78,31 +78,6 @@ namespace psmas
             flush();
         }
 
-        /// y[active] = Inv x[active] on the coarsest segment: one wave per dense row, the
-        /// restricted vector staged through the cache.  Rows of inactive nodes are not touched.
-        template <int BD, typename T>
-        __global__ __launch_bounds__(256) void k_dense_apply(int nd, const double *__restrict__ inv,
-                                                             const int *__restrict__ active_ids,
-                                                             const T *__restrict__ xseg, T *__restrict__ yseg)
-        {
-            const int row = blockIdx.x * 4 + threadIdx.x / WAVE;
-            const int lane = threadIdx.x % WAVE;
-            if (row >= nd)
-                return;
-            const double *a = inv + static_cast<size_t>(row) * nd;
-            double s = 0.0;
-            for (int c = lane; c < nd; c += WAVE)
-            {
-                const int node = active_ids[c / BD];
-                s += a[c] * static_cast<double>(xseg[static_cast<size_t>(node) * BD + c % BD]);
-            }
-            s = wave_sum(s);
-            if (lane == 0)
-            {
-                const int node = active_ids[row / BD];
-                yseg[static_cast<size_t>(node) * BD + row % BD] = static_cast<T>(s);
-            }
-        }
     } // namespace
 
     void launch_dense_galerkin(int bd, int npad, int level, const SellView<double> &A, const int *row_len,
@@ -119,16 +94,6 @@ namespace psmas
         else
             hipLaunchKernelGGL(k_dense_galerkin<3>, grid, block, 0, s, npad, level, A, row_len, block_of_slot, cmap,
                                active_index, nd, dense);
-        PSMAS_LAUNCH_CHECK();
-    }
-
-    template <int BD, typename T>
-    void launch_dense_apply(int nd, const double *inv, const int *active_ids, const T *xseg, T *yseg, hipStream_t s)
-    {
-        if (nd <= 0)
-            return;
-        hipLaunchKernelGGL((k_dense_apply<BD, T>), dim3(div_up(nd, 4)), dim3(256), 0, s, nd, inv, active_ids, xseg,
-                           yseg);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -179,13 +144,4 @@ namespace psmas
         return true;
     }
 
-#define PSMAS_INST_DENSE(BD)                                                                                       \
-    template void launch_dense_apply<BD, double>(int, const double *, const int *, const double *, double *,      \
-                                                 hipStream_t);                                                    \
-    template void launch_dense_apply<BD, float>(int, const double *, const int *, const float *, float *,         \
-                                                hipStream_t);
-    PSMAS_INST_DENSE(1)
-    PSMAS_INST_DENSE(2)
-    PSMAS_INST_DENSE(3)
-#undef PSMAS_INST_DENSE
 } // namespace psmas

@@ -1,4 +1,5 @@
-// Exact FP64 coarsest-level solve (see coarse_solve.hip).
+// Exact FP64 coarsest-level solve: setup here (coarse_solve.hip); the apply is side work of the
+// second bank-apply launch (mas_kernels.hip).
 #pragma once
 
 #include "kernels.hpp"
@@ -15,10 +16,6 @@ namespace psmas
     void launch_dense_galerkin(int bd, int npad, int level, const SellView<double> &A, const int *row_len,
                                const int *block_of_slot, const int4 *cmap, const int *active_index, int nd,
                                double *dense, hipStream_t s);
-
-    /// y = Inv x on the active rows of the coarsest segment (xseg / yseg point at its start).
-    template <int BD, typename T>
-    void launch_dense_apply(int nd, const double *inv, const int *active_ids, const T *xseg, T *yseg, hipStream_t s);
 
     /// In-place inverse of a dense SPD matrix via Cholesky (A = L L^T, A^-1 = L^-T L^-1).
     /// Returns false when the matrix is not numerically positive definite.
