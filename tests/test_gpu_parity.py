@@ -256,6 +256,30 @@ def test_prefactor_sequence(gpu, bd):
         assert np.linalg.norm(A @ x - b) < 1e-8
 
 
+@pytest.mark.parametrize("bd", [1, 2])
+def test_mixed_small_block_sizes(gpu, bd):
+    """The tiled FP16 apply and the side work of both apply launches at block sizes 1 and 2 (one
+    and two 16-byte chunks per thread, 4 and 8 tiles per side): close to the FP64 apply,
+    symmetric, and the mixed solve reaches the FP64 tolerance."""
+    pattern = helpers.grid_laplacian_pattern(17, 13, 5)  # 1,105 unknowns: odd for bd = 2
+    A = helpers.prefactor_matrix(pattern, seed=7)
+    rng = np.random.default_rng(8)
+    s = MASSolver()
+    s.set_parameters({"MAS": {"block_dim": bd, "precision": "mixed", "coarse_solver": "cholesky",
+                              "relative_tolerance": 1e-8, "absolute_tolerance": 0.0, "max_iter": 5000}})
+    s.factorize(A)
+    r, v = rng.standard_normal(A.shape[0]), rng.standard_normal(A.shape[0])
+    z64 = s.apply_preconditioner(r)
+    z16, zv = s.apply_preconditioner(r, mixed=True), s.apply_preconditioner(v, mixed=True)
+    assert np.linalg.norm(z16 - z64) <= 5e-3 * np.linalg.norm(z64)
+    assert abs(v @ z16 - r @ zv) <= 1e-5 * np.linalg.norm(v) * np.linalg.norm(z16)
+    b = rng.uniform(-1, 1, A.shape[0])
+    x = np.zeros_like(b)
+    s.solve(b, x)
+    assert s.get_info()["solver_status"] == "Reach relative tolerance"
+    assert np.linalg.norm(b - A @ x) <= 1e-8 * np.linalg.norm(b)
+
+
 def test_odd_sizes_trailing_block_padding(gpu):
     pattern = helpers.grid_laplacian_pattern(11, 7, 3)  # 231 unknowns
     A = helpers.prefactor_matrix(pattern, seed=5)
