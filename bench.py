@@ -6,12 +6,16 @@ bottom face fixed, gravity RHS — 823,875 DOF, 7,081,535 3x3 blocks.  A "step" 
 from b to x with b and x resident in HBM (``psmas_solve_device``); factorisation (partition, MAS
 build) is done once beforehand and reported separately.
 
-Two paths run on the same factorisation:
-  * fp64  — the reference algorithm: FP64 MAS-PCG, true residual every 4th iteration,
-            convergence checked every 10th, relative tolerance 1e-8;
-  * mixed — FP32 inner PCG with FP16 fine-level bank inverses and FP64 reliable updates;
-            converged only when the FP64 true residual ||b - A x|| <= 1e-8 ||b||.
-The headline ``value`` is the mixed path's ms per solve; the fp64 path is reported beside it.
+Three paths run on the same system:
+  * fp64    — the reference algorithm: FP64 MAS-PCG, true residual every 4th iteration,
+              convergence checked every 10th, relative tolerance 1e-8;
+  * mixed   — FP32 inner PCG with FP16 bank inverses and FP64 reliable updates; converged only
+              when the FP64 true residual ||b - A x|| <= 1e-8 ||b||;
+  * learned — the mixed path plus the learned lattice coarse correction (flexible PCG), its
+              network trained beforehand on pairs from solves of the following steps of a
+              drifting-modulus sequence (collection and training times reported separately).
+The headline ``value`` is the learned path's ms per solve (the mixed path's when the learned path
+is disabled or falls back); the other two are reported beside it.
 
 One JSON line is printed (driver contract), plus roofline and CPU-baseline objects.
 """
@@ -178,9 +182,10 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction path")
     ap.add_argument("--profile-learned", action="store_true",
                     help="profile mode: also train briefly (2 solves, 20 epochs) and time learned solves")
-    ap.add_argument("--train-solves", type=int, default=8,
+    ap.add_argument("--train-solves", type=int, default=16,
                     help="learned path: solves of the drifting sequence (steps 1..T) that provide training pairs")
-    ap.add_argument("--lattice", type=int, default=7, help="learned path: lattice points per axis")
+    ap.add_argument("--lattice", type=int, default=8, help="learned path: lattice points per axis")
+    ap.add_argument("--hidden", type=int, default=128, help="learned path: hidden width of the network")
     ap.add_argument("--random-pairs", type=int, default=200, help="learned path: Gaussian pairs per training solve")
     ap.add_argument("--epochs", type=int, default=150, help="learned path: training epochs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -292,7 +297,8 @@ def main() -> int:
         trainer = MASSolver()
         trainer.set_parameters({"MAS": dict(base, precision="mixed", coarse_solver=args.coarse,
                                             learned_collect=True, lattice_points=args.lattice,
-                                            learned_random_pairs=args.random_pairs, lazy_partitioning=True)})
+                                            learned_hidden=args.hidden, learned_random_pairs=args.random_pairs,
+                                            lazy_partitioning=True)})
         trainer.set_coordinates(prob.coords)
         seq = P.config5_moduli(prob.ne, steps=args.train_solves + 1, seed=0)
         next(seq)  # step 0 is the benchmarked system
@@ -332,7 +338,8 @@ def main() -> int:
                    "kernels": learned_kernels,
                    "reliable_updates": infol["reliable_updates"], "fallback": infol["learned"]["fallback"],
                    "train_solves": args.train_solves, "pairs": pairs, "collect_s": round(collect_s, 1),
-                   "train_s": round(train_s, 1), "train_loss": loss, "lattice_points": args.lattice}
+                   "train_s": round(train_s, 1), "train_loss": loss, "lattice_points": args.lattice,
+                   "hidden": args.hidden}
         del trainer
 
     # Accuracy reference: the FP64 path driven to relative residual 1e-11 (a direct FP64
