@@ -2,6 +2,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <climits>
 #include <cmath>
 #include <numeric>
@@ -401,6 +402,63 @@ namespace psmas
         out.vals.assign(out.cols.size() * bs, 0.0);
         if (!refresh_padded_bsr_values(A, out))
             throw InvalidArgumentError("[MAS] internal: BSR pattern mismatch.");
+    }
+
+    bool build_value_map(const CscView &A, int bd, const std::vector<int> &slot_of_block, const std::vector<int> &rows,
+                         const std::vector<int> &cols, const std::vector<int> &slice_ptr, std::vector<int> &map)
+    {
+        const int n = A.n, nb = div_up(n, bd), bs = bd * bd;
+        const size_t nvals = static_cast<size_t>(slice_ptr.back()) * bs * 64;
+        if (nvals > static_cast<size_t>(INT_MAX))
+            return false;
+        map.assign(static_cast<size_t>(A.nnz), -1);
+        std::vector<unsigned char> taken(nvals, 0);
+        std::atomic<bool> ok{true};
+        // Parallel over block columns: distinct block columns own distinct blocks, so the marks
+        // of one value are only touched by the thread that owns its block column.
+        parallel_for_blocks(nb, [&](int b0, int b1) {
+            for (int cb = b0; cb < b1; ++cb)
+            {
+                const int C = slot_of_block[cb];
+                for (int c = cb * bd; c < std::min(n, cb * bd + bd); ++c)
+                    for (int k = A.colptr[c]; k < A.colptr[c + 1]; ++k)
+                    {
+                        const int r = A.rowidx[k];
+                        const int R = slot_of_block[r / bd];
+                        const int *beg = cols.data() + rows[R];
+                        const int *end = cols.data() + rows[R + 1];
+                        const int *it = std::lower_bound(beg, end, C);
+                        if (it == end || *it != C)
+                        {
+                            ok = false;
+                            continue;
+                        }
+                        const size_t slot = static_cast<size_t>(slice_ptr[R / 64]) + static_cast<size_t>(it - beg);
+                        const size_t v = (slot * bs + static_cast<size_t>((r % bd) * bd + (c % bd))) * 64 + R % 64;
+                        if (taken[v])
+                            ok = false; // duplicate entry: values would have to be added
+                        taken[v] = 1;
+                        map[static_cast<size_t>(k)] = static_cast<int>(v);
+                    }
+            }
+        });
+        return ok;
+    }
+
+    bool same_ints(const int *a, const int *b, size_t n)
+    {
+        constexpr size_t CHUNK = 4096; // enough chunks for the parallel split
+        const int chunks = static_cast<int>((n + CHUNK - 1) / CHUNK);
+        std::atomic<bool> same{true};
+        parallel_for_blocks(chunks, [&](int c0, int c1) {
+            for (int c = c0; c < c1 && same; ++c)
+            {
+                const size_t lo = static_cast<size_t>(c) * CHUNK, hi = std::min(n, lo + CHUNK);
+                if (std::memcmp(a + lo, b + lo, (hi - lo) * sizeof(int)) != 0)
+                    same = false;
+            }
+        });
+        return same;
     }
 
     bool refresh_padded_bsr_values(const CscView &A, PaddedBsr &bsr)

@@ -77,4 +77,13 @@ namespace psmas
     void build_padded_bsr(const CscView &A, int bd, const Partition &part, PaddedBsr &out);
     /// Refresh only the values of an existing pattern (same sparsity), for re-factorisation.
     bool refresh_padded_bsr_values(const CscView &A, PaddedBsr &bsr);
+    /// Index of every non-zero of A in the SELL-64 FP64 value array of one system's padded BSR
+    /// (rows, cols; slot_of_block of its partition; slice_ptr = slot offset of every 64-row slice):
+    /// ((slice_ptr[R / 64] + position of the block in row R) * bd*bd + e) * 64 + R % 64, with R the
+    /// block row's slot and e the entry's place in the block.  Returns false when two non-zeros
+    /// land on one value (duplicate entries), whose values cannot be scattered.
+    bool build_value_map(const CscView &A, int bd, const std::vector<int> &slot_of_block, const std::vector<int> &rows,
+                         const std::vector<int> &cols, const std::vector<int> &slice_ptr, std::vector<int> &map);
+    /// a[0..n) == b[0..n), compared in parallel chunks.
+    bool same_ints(const int *a, const int *b, size_t n);
 } // namespace psmas

@@ -165,6 +165,9 @@ namespace psmas
     template <typename T>
     void launch_scatter_out(int nb, int bd, int n, const int *slot_of_block, const T *src, T *dst, hipStream_t s);
 
+    /// dst[map[k]] = src[k] for k < n (map injective): a values-only matrix refresh.
+    void launch_scatter_values(long long n, const int *map, const double *src, double *dst, hipStream_t s);
+
     /// sc[s][slot] = value for systems s < count.
     void launch_set_scalar(double *sc, int count, int slot, double value, hipStream_t s);
 

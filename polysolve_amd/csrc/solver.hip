@@ -251,6 +251,7 @@ namespace psmas
           << ", \"solve_ms\": " << solve_ms_ << ", \"levels\": " << level_num_ << ", \"parts\": " << S.nparts
           << ", \"systems\": " << sys_.size() << ", \"aggregation\": \"" << S.aggregation << "\""
           << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\""
+          << ", \"factorize_mode\": \"" << factorize_mode_ << "\""
           << ", \"coarse_restriction\": {\"thread_nodes\": " << nrsmall_ << ", \"wave_pieces\": " << nrbig_
           << ", \"split_nodes\": " << nrcomb_ << "}"
           << ", \"learned\": {\"used\": " << (learned_used_ ? "true" : "false")
@@ -354,8 +355,14 @@ namespace psmas
             throw InvalidArgumentError("[MAS] empty batch.");
         for (const auto &A : As)
             validate_csc(A);
+        if (As.size() == 1 && try_refresh_values(As[0]))
+            return;
         const int bd = params_.block_dim;
         factorized_ = false;
+        factorize_mode_ = "full";
+        vr_colptr_.clear();
+        vr_rowidx_.clear();
+        d_vmap_.reset();
         graphs_->clear();
         const int K = static_cast<int>(As.size());
         if (static_cast<int>(sys_.size()) != K)
@@ -431,6 +438,18 @@ namespace psmas
         d_dense_index_.reset();
 
         upload_matrix(all);
+        if (K == 1 && params_.lazy_partitioning)
+        {
+            // Remember the pattern and where each non-zero lives, for values-only refactorisation.
+            std::vector<int> map;
+            if (build_value_map(As[0], bd, sys_[0].slot_of_block, all.rows, all.cols, h_slice_ptr_, map))
+            {
+                vr_colptr_.assign(As[0].colptr, As[0].colptr + As[0].n + 1);
+                vr_rowidx_.assign(As[0].rowidx, As[0].rowidx + As[0].nnz);
+                d_vmap_.allocate(map.size(), "MAS value map");
+                PSMAS_HIP(hipMemcpy(d_vmap_.data(), map.data(), d_vmap_.bytes(), hipMemcpyHostToDevice));
+            }
+        }
         all = PaddedBsr();
 
         // System maps.
@@ -478,6 +497,40 @@ namespace psmas
             ensure_mixed_data();
         PSMAS_HIP(hipStreamSynchronize(stream_));
         factorized_ = true;
+    }
+
+    bool Solver::try_refresh_values(const CscView &A)
+    {
+        if (!params_.lazy_partitioning || !factorized_ || sys_.size() != 1 || !d_vmap_ ||
+            params_.block_dim != bd_ || A.n != sys_[0].n || static_cast<size_t>(A.nnz) != vr_rowidx_.size() ||
+            !same_ints(A.colptr, vr_colptr_.data(), vr_colptr_.size()) ||
+            !same_ints(A.rowidx, vr_rowidx_.data(), vr_rowidx_.size()))
+            return false;
+        factorized_ = false;
+        graphs_->clear();
+        if (d_csc_vals_.size() != static_cast<size_t>(A.nnz))
+            d_csc_vals_.allocate(static_cast<size_t>(A.nnz), "MAS input values");
+        PSMAS_HIP(hipMemcpyAsync(d_csc_vals_.data(), A.vals, d_csc_vals_.bytes(), hipMemcpyHostToDevice, stream_));
+        launch_scatter_values(A.nnz, d_vmap_.data(), d_csc_vals_.data(), d_vals64_.data(), stream_);
+        // Numeric parts only: the partition, SELL pattern, coarse levels, restriction lists and
+        // lattice depend on the pattern (and coordinates), which are unchanged.
+        dense_built_ = false;
+        dense_nd_ = 0;
+        dense_note_.clear();
+        d_dense_inv_.reset();
+        d_dense_ids_.reset();
+        d_dense_index_.reset();
+        d_vals32_.reset();
+        d_mats16_.reset();
+        d_dscale_.reset();
+        rhs_ready_ = false;
+        build_bank_inverses();
+        if (params_.precision == Precision::Mixed)
+            ensure_mixed_data();
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        factorize_mode_ = "values";
+        factorized_ = true;
+        return true;
     }
 
     void Solver::build_partition(System &S, const CscView &A)
@@ -547,6 +600,7 @@ namespace psmas
             slice_ptr[s + 1] = slice_ptr[s] + mx;
         }
         nslots_ = slice_ptr[nslices_];
+        h_slice_ptr_ = slice_ptr;
         std::vector<int> cols(static_cast<size_t>(nslots_) * WAVE);
         std::vector<double> vals(static_cast<size_t>(nslots_) * WAVE * bs);
         auto fill = [&](int s0, int s1) {
@@ -678,10 +732,16 @@ namespace psmas
         PSMAS_HIP(hipMemcpyAsync(d_lev_off_.data(), lev_off_, sizeof(lev_off_), hipMemcpyHostToDevice, stream_));
         PSMAS_HIP(hipMemcpyAsync(d_mat_off_.data(), mat_off_, sizeof(mat_off_), hipMemcpyHostToDevice, stream_));
         build_restrict_chunks();
+        build_bank_inverses();
+    }
 
+    void Solver::build_bank_inverses()
+    {
+        const SellView<double> A = sell(d_vals64_.data());
         const int dim = BANK * bd_;
         const size_t L = static_cast<size_t>(dim) * (dim + 1) / 2;
-        d_mats64_.allocate(static_cast<size_t>(total_mats_) * L, "MAS coarse_matrices");
+        if (d_mats64_.size() != static_cast<size_t>(total_mats_) * L)
+            d_mats64_.allocate(static_cast<size_t>(total_mats_) * L, "MAS coarse_matrices");
         d_mats64_.zero(stream_);
         launch_galerkin(npad_, bd_, level_num_, d_mat_off_.data(), A, d_row_len_.data(), d_block_of_slot_.data(),
                         d_cmap_.data(), d_mats64_.data(), stream_);

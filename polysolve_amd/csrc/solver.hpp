@@ -169,6 +169,14 @@ namespace psmas
         DeviceBuffer<int> d_dense_ids_;
         DeviceBuffer<int> d_dense_index_; ///< per coarsest-segment node: dense index or -1
 
+        // Values-only refactorisation: the last full factorisation's pattern and the position of
+        // each of its non-zeros in the SELL value array.
+        std::vector<int> vr_colptr_, vr_rowidx_;
+        std::vector<int> h_slice_ptr_;
+        DeviceBuffer<int> d_vmap_;
+        DeviceBuffer<double> d_csc_vals_;
+        const char *factorize_mode_ = "full";
+
         // Restriction of coarse segments 2.. straight from segment 1: every node's segment-1
         // descendants (ascending), summed by the side workgroups of the first apply launch (a
         // thread per node of at most RESTRICT_SMALL_MAX descendants, a wave per piece of at most
@@ -218,6 +226,11 @@ namespace psmas
         void build_partition(System &S, const CscView &A);
         void upload_matrix(const PaddedBsr &bsr);
         void build_mas();
+        void build_bank_inverses(); ///< Galerkin bank matrices and their inverses (numeric part)
+        /// Values-only refactorisation: one system, lazy_partitioning, the same n, block size and
+        /// sparsity pattern as the last full factorisation.  Scatters the new values into the SELL
+        /// arrays and rebuilds the numeric MAS parts; false when it does not apply.
+        bool try_refresh_values(const CscView &A);
         void build_restrict_chunks();
         /// Side work of the first apply launch (restriction of segments 2.. from the segment-1
         /// values seg1 into the multi-level vector out; with lattice_gather also the learned

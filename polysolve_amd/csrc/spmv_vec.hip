@@ -515,6 +515,26 @@ namespace psmas
     }
 #undef PSMAS_UNROLL_DISPATCH
 
+    namespace
+    {
+        __global__ __launch_bounds__(256) void k_scatter_values(long long n, const int *__restrict__ map,
+                                                                const double *__restrict__ src, double *__restrict__ dst)
+        {
+            const long long k = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+            if (k < n)
+                dst[map[k]] = src[k];
+        }
+    } // namespace
+
+    void launch_scatter_values(long long n, const int *map, const double *src, double *dst, hipStream_t s)
+    {
+        if (n <= 0)
+            return;
+        hipLaunchKernelGGL(k_scatter_values, dim3(static_cast<unsigned>(div_up_ll(n, 256))), dim3(256), 0, s, n, map,
+                           src, dst);
+        PSMAS_LAUNCH_CHECK();
+    }
+
     template <int BD, typename T>
     void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0,
                           const Reductions &red, const SysMap &sm, hipStream_t s, const LatticeBlocks *lb)

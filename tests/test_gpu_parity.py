@@ -312,6 +312,48 @@ def test_lazy_partitioning_reuses_partition(gpu, het12):
     assert np.array_equal(off1, off2) and np.array_equal(perm1, perm2)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_values_only_refactorisation_matches_full(gpu, het12, precision):
+    """With lazy_partitioning, a matrix with the last one's sparsity pattern only has its values
+    scattered into the device arrays and the numeric MAS parts rebuilt: its solve must equal a
+    fresh factorisation's (same Morton partition).  A changed pattern takes the full path."""
+    prm = {"block_dim": 3, "lazy_partitioning": True, "aggregation": "morton", "precision": precision,
+           "coarse_solver": "cholesky", "relative_tolerance": 1e-8, "absolute_tolerance": 0.0}
+    rng = np.random.default_rng(11)
+    d = sp.diags(rng.uniform(0.5, 2.0, het12.n))
+    A2 = (d @ het12.A @ d).tocsc()  # same pattern, other values, still SPD
+    s = MASSolver()
+    s.set_parameters({"MAS": prm})
+    s.set_coordinates(het12.coords)
+    s.factorize(het12.A)
+    assert s.get_info()["factorize_mode"] == "full"
+    x1 = np.zeros(het12.n)
+    s.solve(het12.b, x1)
+    s.factorize(A2)
+    assert s.get_info()["factorize_mode"] == "values"
+    x2 = np.zeros(het12.n)
+    s.solve(het12.b, x2)
+    ref = MASSolver()
+    ref.set_parameters({"MAS": prm})
+    ref.set_coordinates(het12.coords)
+    ref.factorize(A2)
+    xr = np.zeros(het12.n)
+    ref.solve(het12.b, xr)
+    # The same operator (the Galerkin fill adds with FP64 atomics, so two factorisations agree to
+    # rounding, and a solve's iteration count can move by one check interval).
+    r = rng.standard_normal(het12.n)
+    z, zr = s.apply_preconditioner(r), ref.apply_preconditioner(r)
+    assert np.linalg.norm(z - zr) <= 1e-12 * np.linalg.norm(zr)
+    assert abs(s.get_info()["solver_iter"] - ref.get_info()["solver_iter"]) <= 10
+    assert np.linalg.norm(het12.b - A2 @ x2) <= 1e-8 * np.linalg.norm(het12.b)
+    assert np.linalg.norm(x2 - xr) <= 1e-6 * np.linalg.norm(xr)
+    # One more off-diagonal pair: a new pattern, so a full factorisation.
+    A3 = A2.tolil()
+    A3[0, het12.n - 1] = A3[het12.n - 1, 0] = 1e-3
+    s.factorize(A3.tocsc())
+    assert s.get_info()["factorize_mode"] == "full"
+
+
 def test_inverse_failure_raises(gpu):
     pattern = helpers.grid_laplacian_pattern(8, 8, 1)
     A = helpers.prefactor_matrix(pattern, seed=1).tolil()
