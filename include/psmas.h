@@ -87,6 +87,15 @@ const char *psmas_last_error(psmas_handle h);
 /* Same as psmas_solve with b and x in device memory (HIP pointers on the solver's device). */
 int psmas_solve_device(psmas_handle h, int n, const double *d_b, double *d_x);
 
+/* Values-only refactorisation from device memory: d_vals holds nnz doubles (HIP pointer on the
+ * solver's device) in the order of the column-compressed pattern of the last full factorisation,
+ * which must have run with "lazy_partitioning": true on one system without duplicate entries.
+ * The values are scattered into the solver's matrix and the numeric MAS parts rebuilt, with no
+ * host transfer (for a caller that assembles A on the GPU).  Fails when no such pattern is
+ * recorded or nnz differs.  A refinement of the reference's lazy re-factorisation
+ * (MASSolver.cu:304-321), which has no device-value entry. */
+int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d_vals);
+
 /* Node coordinates for Morton-ordered aggregation into 32-node domains: nb tuples of dim
  * (1..3) doubles, one per block node (block_dim scalars).  Takes effect at the next factorize. */
 int psmas_set_coordinates(psmas_handle h, int nb, int dim, const double *xyz);

@@ -43,6 +43,7 @@ SIGNATURES = {
     "psmas_solve": (_i, [_vp, _i, _pd, _pd]),
     "psmas_last_error": (_cp, [_vp]),
     "psmas_solve_device": (_i, [_vp, _i, _vp, _vp]),
+    "psmas_factorize_values_device": (_i, [_vp, _ll, _vp]),
     "psmas_set_coordinates": (_i, [_vp, _i, _i, _pd]),
     "psmas_batch_factorize_csc": (_i, [_vp, _i, _pi, C.POINTER(_ll), C.POINTER(_pi), C.POINTER(_pi),
                                        C.POINTER(_pd)]),

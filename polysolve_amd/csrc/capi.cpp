@@ -155,6 +155,13 @@ extern "C"
         return guarded(h->last_error, [&] { h->solver.solve_device(n, d_b, d_x); });
     }
 
+    int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d_vals)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.refresh_values_device(nnz, d_vals); });
+    }
+
     const char *psmas_last_error(psmas_handle h)
     {
         return h ? h->last_error.c_str() : g_create_error.c_str();

@@ -21,14 +21,6 @@
 
 namespace psmas
 {
-    // Matrix-element conversion used by the apply kernel.
-    template <typename TA>
-    __device__ __forceinline__ TA to_float_or_double(double v) { return static_cast<TA>(v); }
-    template <typename TA>
-    __device__ __forceinline__ TA to_float_or_double(float v) { return static_cast<TA>(v); }
-    template <typename TA>
-    __device__ __forceinline__ TA to_float_or_double(__half v) { return static_cast<TA>(__half2float(v)); }
-
     namespace
     {
         // ------------------------------------------------------------------------------ apply
@@ -530,12 +522,13 @@ namespace psmas
                     break;
             }
         }
-        /// y_m = Inv_m x_m for matrices [mat_begin, mat_begin + count), grid-stride: each workgroup
-        /// takes every gridDim.x-th matrix.  The packed upper matrix moves global -> registers ->
-        /// LDS; the next matrix's loads are issued before the current one is computed, so every
-        /// workgroup keeps a matrix in flight.  Threads (row, half) each sum half of row `row`,
-        /// reading element (t, row) for t < row (packed row t, contiguous across threads) and
-        /// (row, t) otherwise.
+        /// y_m = Inv_m x_m for FP64 packed matrices [mat_begin, mat_begin + count), grid-stride:
+        /// each workgroup takes every nwg-th matrix.  The packed upper matrix moves global ->
+        /// registers -> LDS; the next matrix's loads are issued before the current one is
+        /// computed, so every workgroup keeps a matrix in flight.  Threads (row, half) each sum half
+        /// of row `row`, reading element (t, row) for t < row (packed row t, contiguous across
+        /// threads) and (row, t) otherwise.  (Bandwidth-bound in FP64; the FP16 copy uses the tiled
+        /// kernel above.)
         template <int BD, typename TM, typename T>
         __global__ __launch_bounds__(apply_block<BD>()) void k_mas_apply(int mat_begin, int count,
                                                                           const TM *__restrict__ mats,
@@ -548,8 +541,9 @@ namespace psmas
             constexpr int N = 32 * BD;
             constexpr int L = N * (N + 1) / 2;
             constexpr int BLOCK = apply_block<BD>();
-            constexpr bool SCALED = sizeof(TM) == 2;
-            using TA = typename std::conditional<sizeof(TM) == 8, double, float>::type;
+            static_assert(std::is_same<TM, double>::value && std::is_same<T, double>::value,
+                          "the packed apply is the FP64 path's");
+            using TA = double;
             using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
             static_assert((L * sizeof(TM)) % 16 == 0, "packed matrix must be a whole number of 16 B chunks");
             constexpr int CH = static_cast<int>(L * sizeof(TM) / 16);
@@ -590,11 +584,7 @@ namespace psmas
                         reg[k] = __builtin_nontemporal_load(src + i);
                 }
                 if (threadIdx.x < N)
-                {
-                    xr = static_cast<TA>(coarse_input<BD, T>(sd, mr, mm * 32 + threadIdx.x / BD, threadIdx.x % BD));
-                    if (SCALED)
-                        xr *= static_cast<TA>(dscale[static_cast<size_t>(mm) * N + threadIdx.x]);
-                }
+                    xr = coarse_input<BD, T>(sd, mr, mm * 32 + threadIdx.x / BD, threadIdx.x % BD);
             };
             load(m);
             const int row = threadIdx.x % N;
@@ -627,7 +617,7 @@ namespace psmas
                     for (int t = t0; t < t1; ++t)
                     {
                         const int idx = (t < row) ? (rs_t + row) : (rs_row + t);
-                        acc = fma(to_float_or_double<TA>(sA[idx]), sx[t], acc);
+                        acc = fma(sA[idx], sx[t], acc);
                         rs_t += N - t - 1;
                     }
                 }
@@ -635,12 +625,7 @@ namespace psmas
                     part[row] = acc;
                 __syncthreads();
                 if (half == 0 && !dense_owned(sd, m * 32 + row / BD))
-                {
-                    TA y = acc + part[row];
-                    if (SCALED)
-                        y *= static_cast<TA>(dscale[static_cast<size_t>(m) * N + row]);
-                    mz[static_cast<size_t>(m) * N + row] = static_cast<T>(y);
-                }
+                    mz[static_cast<size_t>(m) * N + row] = acc + part[row];
                 __syncthreads(); // sA, sx and part are rewritten for the next matrix
                 m = next;
             }
@@ -1320,9 +1305,6 @@ namespace psmas
     template void launch_mas_apply<BD, double, double>(int, int, const double *, const float *, const double *,   \
                                                        double *, const int *, const int *, hipStream_t,          \
                                                        const ApplySide &);                                       \
-    template void launch_mas_apply<BD, double, float>(int, int, const double *, const float *, const float *,     \
-                                                      float *, const int *, const int *, hipStream_t,            \
-                                                      const ApplySide &);                                        \
     template void launch_mas_apply<BD, __half, float>(int, int, const __half *, const float *, const float *,     \
                                                       float *, const int *, const int *, hipStream_t,            \
                                                       const ApplySide &);                                        \

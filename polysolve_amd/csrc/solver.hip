@@ -512,6 +512,26 @@ namespace psmas
             d_csc_vals_.allocate(static_cast<size_t>(A.nnz), "MAS input values");
         PSMAS_HIP(hipMemcpyAsync(d_csc_vals_.data(), A.vals, d_csc_vals_.bytes(), hipMemcpyHostToDevice, stream_));
         launch_scatter_values(A.nnz, d_vmap_.data(), d_csc_vals_.data(), d_vals64_.data(), stream_);
+        finish_value_refresh();
+        return true;
+    }
+
+    void Solver::refresh_values_device(long long nnz, const double *d_vals)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        if (!factorized_ || sys_.size() != 1 || !d_vmap_ || d_vals == nullptr ||
+            nnz != static_cast<long long>(vr_rowidx_.size()))
+            throw InvalidArgumentError(
+                "[MAS] device values need a recorded pattern: a full factorisation of one system with "
+                "lazy_partitioning, no duplicate entries, and the same number of non-zeros.");
+        factorized_ = false;
+        graphs_->clear();
+        launch_scatter_values(nnz, d_vmap_.data(), d_vals, d_vals64_.data(), stream_);
+        finish_value_refresh();
+    }
+
+    void Solver::finish_value_refresh()
+    {
         // Numeric parts only: the partition, SELL pattern, coarse levels, restriction lists and
         // lattice depend on the pattern (and coordinates), which are unchanged.
         dense_built_ = false;
@@ -530,7 +550,6 @@ namespace psmas
         PSMAS_HIP(hipStreamSynchronize(stream_));
         factorize_mode_ = "values";
         factorized_ = true;
-        return true;
     }
 
     void Solver::build_partition(System &S, const CscView &A)

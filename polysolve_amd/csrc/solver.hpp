@@ -93,6 +93,10 @@ namespace psmas
         int block_dim() const { return bd_; }
         int dim() const { return sys_.empty() ? 0 : sys_[0].n; }
 
+        /// Values-only refactorisation from device memory (values in the recorded CSC order of the
+        /// last full factorisation with lazy_partitioning).
+        void refresh_values_device(long long nnz, const double *d_vals);
+
     private:
         struct Graphs;
 
@@ -231,6 +235,9 @@ namespace psmas
         /// sparsity pattern as the last full factorisation.  Scatters the new values into the SELL
         /// arrays and rebuilds the numeric MAS parts; false when it does not apply.
         bool try_refresh_values(const CscView &A);
+        /// The numeric rebuild after the SELL values were refreshed.
+        void finish_value_refresh();
+
         void build_restrict_chunks();
         /// Side work of the first apply launch (restriction of segments 2.. from the segment-1
         /// values seg1 into the multi-level vector out; with lattice_gather also the learned

@@ -165,6 +165,11 @@ class MASSolver(Solver):
         """Solve with b and x already in device memory (raw HIP pointers)."""
         _check(N.lib().psmas_solve_device(self._h, int(n), C.c_void_p(b_ptr), C.c_void_p(x_ptr)), self._h)
 
+    def factorize_values_device(self, vals_ptr: int, nnz: int) -> None:
+        """Refactorise from nnz values already in device memory (raw HIP pointer), in the CSC order
+        of the last full factorisation, which must have used lazy_partitioning."""
+        _check(N.lib().psmas_factorize_values_device(self._h, int(nnz), C.c_void_p(vals_ptr)), self._h)
+
     def set_coordinates(self, xyz) -> None:
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
         if xyz.ndim != 2:

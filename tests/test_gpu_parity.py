@@ -354,6 +354,42 @@ def test_values_only_refactorisation_matches_full(gpu, het12, precision):
     assert s.get_info()["factorize_mode"] == "full"
 
 
+def test_values_from_device_memory(gpu, het12):
+    """psmas_factorize_values_device: the values of a same-pattern matrix handed over in device
+    memory give the same operator as the host values-only path; without a recorded pattern or
+    with the wrong count it fails."""
+    prm = {"block_dim": 3, "lazy_partitioning": True, "aggregation": "morton", "coarse_solver": "cholesky",
+           "relative_tolerance": 1e-8, "absolute_tolerance": 0.0}
+    rng = np.random.default_rng(12)
+    d = sp.diags(rng.uniform(0.5, 2.0, het12.n))
+    A2 = (d @ het12.A @ d).tocsc()
+    A2.sort_indices()
+    dev = MASSolver()
+    dev.set_parameters({"MAS": prm})
+    dev.set_coordinates(het12.coords)
+    dev.factorize(het12.A)
+    vals = DeviceArray(A2.nnz, A2.data)
+    with pytest.raises(MASError, match="recorded pattern"):
+        dev.factorize_values_device(vals.ptr, A2.nnz - 1)
+    dev.factorize_values_device(vals.ptr, A2.nnz)
+    assert dev.get_info()["factorize_mode"] == "values"
+    host = MASSolver()
+    host.set_parameters({"MAS": prm})
+    host.set_coordinates(het12.coords)
+    host.factorize(het12.A)
+    host.factorize(A2)
+    r = rng.standard_normal(het12.n)
+    z, zh = dev.apply_preconditioner(r), host.apply_preconditioner(r)
+    assert np.linalg.norm(z - zh) <= 1e-12 * np.linalg.norm(zh)
+    x = np.zeros(het12.n)
+    dev.solve(het12.b, x)
+    assert np.linalg.norm(het12.b - A2 @ x) <= 1e-8 * np.linalg.norm(het12.b)
+    plain = _factorized(het12)  # no lazy_partitioning: no pattern recorded
+    with pytest.raises(MASError, match="recorded pattern"):
+        plain.factorize_values_device(vals.ptr, A2.nnz)
+    vals.free()
+
+
 def test_inverse_failure_raises(gpu):
     pattern = helpers.grid_laplacian_pattern(8, 8, 1)
     A = helpers.prefactor_matrix(pattern, seed=1).tolil()
