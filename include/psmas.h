@@ -82,6 +82,29 @@ int psmas_solve(psmas_handle h, int n, const double *b, double *x);
  * NULL).  Valid until the next call on h. */
 const char *psmas_last_error(psmas_handle h);
 
+/* ---- Dirichlet boundary conditions (src/polysolve/linear/FEMSolver.hpp:35-47) ------------ */
+
+/* prefactorize (FEMSolver.cpp:269-315): factorise the matrix A' obtained from A by dropping every
+ * entry in the row or column of a fixed dof and putting N(k) on each diagonal (1 for a fixed dof,
+ * a_kk otherwise; FEMSolver.cpp:283-306).  dirichlet_nodes: ndir scalar dof indices in [0, n)
+ * (duplicates allowed).  The reference overwrites A with A'; here A is left as it is, and the
+ * couplings a_ij (i free, j fixed) are kept on the device for the right-hand side lift.  With
+ * "lazy_partitioning": true, a later call with the same pattern and the same fixed dofs only
+ * refreshes the values (on the GPU), as does psmas_factorize_values_device with A's values. */
+int psmas_dirichlet_factorize_csc(psmas_handle h, int n, long long nnz, const int *colptr, const int *rowidx,
+                                  const double *vals, int ndir, const int *dirichlet_nodes);
+
+/* dirichlet_solve_prefactorized (FEMSolver.cpp:318-342): f (length n, host) is replaced by the
+ * lifted right-hand side g = f - (I - N) A N f, computed on the GPU with the A of the last
+ * psmas_dirichlet_factorize_csc, and x receives the solution of A' x = g (x[i] = f[i] at a fixed
+ * dof up to the solver tolerance).  The one-call dirichlet_solve (FEMSolver.cpp:97-266) is
+ * psmas_dirichlet_factorize_csc followed by this call.  Fails without a Dirichlet
+ * factorisation. */
+int psmas_dirichlet_solve(psmas_handle h, int n, double *f, double *x);
+/* Same with f and x in device memory (HIP pointers on the solver's device); f is overwritten
+ * by g in place. */
+int psmas_dirichlet_solve_device(psmas_handle h, int n, double *d_f, double *d_x);
+
 /* ---- Extensions (no counterpart in the reference) --------------------------------------- */
 
 /* Same as psmas_solve with b and x in device memory (HIP pointers on the solver's device). */
@@ -92,7 +115,9 @@ int psmas_solve_device(psmas_handle h, int n, const double *d_b, double *d_x);
  * which must have run with "lazy_partitioning": true on one system without duplicate entries.
  * The values are scattered into the solver's matrix and the numeric MAS parts rebuilt, with no
  * host transfer (for a caller that assembles A on the GPU).  Fails when no such pattern is
- * recorded or nnz differs.  A refinement of the reference's lazy re-factorisation
+ * recorded or nnz differs.  After psmas_dirichlet_factorize_csc the values are those of the
+ * matrix before elimination (its pattern): the eliminated entries are dropped and the couplings
+ * refreshed on the device.  A refinement of the reference's lazy re-factorisation
  * (MASSolver.cu:304-321), which has no device-value entry. */
 int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d_vals);
 

@@ -44,6 +44,9 @@ SIGNATURES = {
     "psmas_last_error": (_cp, [_vp]),
     "psmas_solve_device": (_i, [_vp, _i, _vp, _vp]),
     "psmas_factorize_values_device": (_i, [_vp, _ll, _vp]),
+    "psmas_dirichlet_factorize_csc": (_i, [_vp, _i, _ll, _pi, _pi, _pd, _i, _pi]),
+    "psmas_dirichlet_solve": (_i, [_vp, _i, _pd, _pd]),
+    "psmas_dirichlet_solve_device": (_i, [_vp, _i, _vp, _vp]),
     "psmas_set_coordinates": (_i, [_vp, _i, _i, _pd]),
     "psmas_batch_factorize_csc": (_i, [_vp, _i, _pi, C.POINTER(_ll), C.POINTER(_pi), C.POINTER(_pi),
                                        C.POINTER(_pd)]),
@@ -110,7 +113,7 @@ def problems_lib() -> C.CDLL:
             raise ImportError(f"{PROBLEMS_PATH} is missing: run python -m polysolve_amd.build")
         L = C.CDLL(str(PROBLEMS_PATH))
         L.psgen_hex_assemble.restype = _vp
-        L.psgen_hex_assemble.argtypes = [_i, _pd, C.c_double, C.c_double, _i, _pi, _pi, _pd, _pd,
+        L.psgen_hex_assemble.argtypes = [_i, _pd, C.c_double, C.c_double, _i, _pi, _pi, _pd, _pd, _i,
                                          _pi, C.POINTER(_ll)]
         L.psgen_fetch.restype = _i
         L.psgen_fetch.argtypes = [_vp, _pi, _pi, _pd, _pd]

@@ -155,6 +155,36 @@ extern "C"
         return guarded(h->last_error, [&] { h->solver.solve_device(n, d_b, d_x); });
     }
 
+    int psmas_dirichlet_factorize_csc(psmas_handle h, int n, long long nnz, const int *colptr, const int *rowidx,
+                                      const double *vals, int ndir, const int *dirichlet_nodes)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            psmas::CscView A;
+            A.n = n;
+            A.nnz = nnz;
+            A.colptr = colptr;
+            A.rowidx = rowidx;
+            A.vals = vals;
+            h->solver.dirichlet_factorize(A, dirichlet_nodes, ndir);
+        });
+    }
+
+    int psmas_dirichlet_solve(psmas_handle h, int n, double *f, double *x)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.dirichlet_solve_host(n, f, x); });
+    }
+
+    int psmas_dirichlet_solve_device(psmas_handle h, int n, double *d_f, double *d_x)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.dirichlet_solve_device(n, d_f, d_x); });
+    }
+
     int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d_vals)
     {
         if (!h)

@@ -445,6 +445,106 @@ namespace psmas
         return ok;
     }
 
+    void eliminate_dirichlet(const CscView &A, const std::vector<unsigned char> &fixed, DirichletSystem &out)
+    {
+        const int n = A.n;
+        out.colptr.assign(static_cast<size_t>(n) + 1, 0);
+        std::vector<unsigned char> add_diag(n, 0);
+        // Kept entries per column; a free column without a diagonal gets an explicit zero one.
+        parallel_for_blocks(n, [&](int c0, int c1) {
+            for (int c = c0; c < c1; ++c)
+            {
+                if (fixed[c])
+                {
+                    out.colptr[c + 1] = 1;
+                    continue;
+                }
+                int cnt = 0;
+                bool diag = false;
+                for (int k = A.colptr[c]; k < A.colptr[c + 1]; ++k)
+                {
+                    const int r = A.rowidx[k];
+                    if (!fixed[r])
+                    {
+                        ++cnt;
+                        diag |= (r == c);
+                    }
+                }
+                add_diag[c] = !diag;
+                out.colptr[c + 1] = cnt + (diag ? 0 : 1);
+            }
+        });
+        for (int c = 0; c < n; ++c)
+            out.colptr[c + 1] += out.colptr[c];
+        const size_t nnz = static_cast<size_t>(out.colptr[n]);
+        out.rowidx.resize(nnz);
+        out.vals.resize(nnz);
+        out.keep.assign(static_cast<size_t>(A.nnz), -1);
+        parallel_for_blocks(n, [&](int c0, int c1) {
+            for (int c = c0; c < c1; ++c)
+            {
+                int o = out.colptr[c];
+                if (fixed[c])
+                {
+                    out.rowidx[o] = c;
+                    out.vals[o] = 1.0;
+                    continue;
+                }
+                bool pending = add_diag[c];
+                for (int k = A.colptr[c]; k < A.colptr[c + 1]; ++k)
+                {
+                    const int r = A.rowidx[k];
+                    if (fixed[r])
+                        continue;
+                    if (pending && r > c)
+                    {
+                        out.rowidx[o] = c;
+                        out.vals[o++] = 0.0;
+                        pending = false;
+                    }
+                    out.rowidx[o] = r;
+                    out.vals[o] = A.vals[k];
+                    out.keep[k] = o++;
+                }
+                if (pending)
+                {
+                    out.rowidx[o] = c;
+                    out.vals[o] = 0.0;
+                }
+            }
+        });
+        // Couplings a_ij (i free, j fixed) live in the fixed columns; bucket them by row, columns
+        // ascending within a row.
+        std::vector<int> count(n, 0);
+        for (int j = 0; j < n; ++j)
+            if (fixed[j])
+                for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k)
+                    if (!fixed[A.rowidx[k]])
+                        ++count[A.rowidx[k]];
+        out.crow.clear();
+        out.cptr.assign(1, 0);
+        std::vector<int> pos(n, -1);
+        for (int i = 0; i < n; ++i)
+            if (count[i] > 0)
+            {
+                pos[i] = out.cptr.back();
+                out.crow.push_back(i);
+                out.cptr.push_back(out.cptr.back() + count[i]);
+            }
+        out.ccol.resize(static_cast<size_t>(out.cptr.back()));
+        out.cidx.resize(static_cast<size_t>(out.cptr.back()));
+        for (int j = 0; j < n; ++j)
+            if (fixed[j])
+                for (int k = A.colptr[j]; k < A.colptr[j + 1]; ++k)
+                {
+                    const int i = A.rowidx[k];
+                    if (fixed[i])
+                        continue;
+                    out.ccol[pos[i]] = j;
+                    out.cidx[pos[i]++] = k;
+                }
+    }
+
     bool same_ints(const int *a, const int *b, size_t n)
     {
         constexpr size_t CHUNK = 4096; // enough chunks for the parallel split

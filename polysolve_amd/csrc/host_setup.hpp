@@ -84,6 +84,22 @@ namespace psmas
     /// land on one value (duplicate entries), whose values cannot be scattered.
     bool build_value_map(const CscView &A, int bd, const std::vector<int> &slot_of_block, const std::vector<int> &rows,
                          const std::vector<int> &cols, const std::vector<int> &slice_ptr, std::vector<int> &map);
+    /// Dirichlet elimination (FEMSolver.cpp:123-160, prefactorize FEMSolver.cpp:269-306) with
+    /// N(k) = 1 for a fixed dof k.  The eliminated matrix keeps a_ij when neither i nor j is fixed
+    /// and has N(k) added on every diagonal: 1 on a fixed dof's otherwise empty row and column,
+    /// a_kk + 0 on a free one (an explicit zero where A has no diagonal).  keep[k] is the position
+    /// in it of A's k-th non-zero, or -1 for a dropped one.  The coupling list holds the entries
+    /// a_ij of A with i free and j fixed, grouped by row (crow, cptr) in ascending column order
+    /// (ccol, and cidx = their index among A's non-zeros): the lifted right-hand side is
+    /// g = f - (I - N) A N f, i.e. g_i = f_i - sum over the list of a_ij f_j.
+    struct DirichletSystem
+    {
+        std::vector<int> colptr, rowidx;
+        std::vector<double> vals;
+        std::vector<int> keep;
+        std::vector<int> crow, cptr, ccol, cidx;
+    };
+    void eliminate_dirichlet(const CscView &A, const std::vector<unsigned char> &fixed, DirichletSystem &out);
     /// a[0..n) == b[0..n), compared in parallel chunks.
     bool same_ints(const int *a, const int *b, size_t n);
 } // namespace psmas

@@ -165,8 +165,16 @@ namespace psmas
     template <typename T>
     void launch_scatter_out(int nb, int bd, int n, const int *slot_of_block, const T *src, T *dst, hipStream_t s);
 
-    /// dst[map[k]] = src[k] for k < n (map injective): a values-only matrix refresh.
+    /// dst[map[k]] = src[k] for k < n with map[k] >= 0 (injective on those; -1 = dropped entry):
+    /// a values-only matrix refresh.
     void launch_scatter_values(long long n, const int *map, const double *src, double *dst, hipStream_t s);
+    /// dst[k] = src[idx[k]] for k < n.
+    void launch_gather_values(int n, const int *idx, const double *src, double *dst, hipStream_t s);
+    /// Dirichlet right-hand side lift g = f - (I - N) A N f over the coupling rows (crow[t] with
+    /// couplings cptr[t]..cptr[t+1]: values cval at fixed columns ccol); rows without couplings
+    /// are left as they are in g.  g may be f (the coupled rows are free, the columns read fixed).
+    void launch_dirichlet_lift(int nrows, const int *crow, const int *cptr, const int *ccol, const double *cval,
+                               const double *f, double *g, hipStream_t s);
 
     /// sc[s][slot] = value for systems s < count.
     void launch_set_scalar(double *sc, int count, int slot, double value, hipStream_t s);

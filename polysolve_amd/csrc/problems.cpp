@@ -4,7 +4,7 @@
 // i + j*(ne+1) + k*(ne+1)^2 (x fastest), DOF id 3*node + component.  Element stiffness is
 // 2x2x2 Gauss quadrature of B^T D B for isotropic linear elasticity with a per-element Young's
 // modulus.  Optional SPD 6x6 pair blocks kappa*g*g^T (contact/barrier-like Hessian terms) are added
-// on top.  The bottom face z = 0 is Dirichlet-fixed using the convention of polysolve's
+// on top.  The bottom face z = 0 is Dirichlet-fixed (unless asked not to) using the convention of polysolve's
 // FEM helper (src/polysolve/linear/FEMSolver.cpp:97-160): every entry whose row or column is a
 // Dirichlet DOF is dropped, and the Dirichlet diagonal is set to 1; with a zero Dirichlet value
 // the right-hand side of those DOFs is 0.
@@ -118,7 +118,7 @@ namespace
 
     Problem *assemble(int ne, const double *E_elem, double nu, double body_force_z,
                       int npairs, const int *pair_i, const int *pair_j,
-                      const double *pair_g, const double *pair_kappa)
+                      const double *pair_g, const double *pair_kappa, bool fix_bottom)
     {
         const int N = ne + 1;
         const int nn = N * N * N;
@@ -222,7 +222,7 @@ namespace
         for (int a = 0; a < nn; ++a)
             extra_ptr[a + 1] += extra_ptr[a];
 
-        auto dirichlet = [&](int node) { return node < N * N; }; // bottom face k = 0
+        auto dirichlet = [&](int node) { return fix_bottom && node < N * N; }; // bottom face k = 0
 
         Problem *P = new Problem();
         P->n = 3 * nn;
@@ -298,10 +298,12 @@ extern "C"
     /// Assemble a Q1-hex elasticity system.  E_elem holds ne^3 per-element Young's moduli in
     /// element order i + ne*(j + ne*k).  body_force_z is rho*g_z per unit volume (lumped to the
     /// nodes).  Pairs (pair_i[p], pair_j[p]) add kappa_p * g_p g_p^T with g_p = pair_g[6p:6p+6].
+    /// fix_bottom != 0 applies the Dirichlet convention on the bottom face; 0 returns the bare
+    /// (singular) stiffness matrix and load.
     /// Returns an opaque handle (NULL on failure); sizes through n_out and nnz_out.
     void *psgen_hex_assemble(int ne, const double *E_elem, double nu, double body_force_z,
                              int npairs, const int *pair_i, const int *pair_j,
-                             const double *pair_g, const double *pair_kappa,
+                             const double *pair_g, const double *pair_kappa, int fix_bottom,
                              int *n_out, long long *nnz_out)
     {
         if (ne < 1 || E_elem == nullptr || npairs < 0)
@@ -311,7 +313,8 @@ extern "C"
             return nullptr;
         try
         {
-            Problem *P = assemble(ne, E_elem, nu, body_force_z, npairs, pair_i, pair_j, pair_g, pair_kappa);
+            Problem *P = assemble(ne, E_elem, nu, body_force_z, npairs, pair_i, pair_j, pair_g, pair_kappa,
+                                  fix_bottom != 0);
             *n_out = P->n;
             *nnz_out = static_cast<long long>(P->rowidx.size());
             return P;

@@ -252,6 +252,9 @@ namespace psmas
           << ", \"systems\": " << sys_.size() << ", \"aggregation\": \"" << S.aggregation << "\""
           << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\""
           << ", \"factorize_mode\": \"" << factorize_mode_ << "\""
+          << ", \"dirichlet\": {\"active\": " << (dirichlet_ ? "true" : "false")
+          << ", \"fixed_dofs\": " << dir_nodes_.size() << ", \"coupled_rows\": " << n_crow_
+          << ", \"couplings\": " << n_coupl_ << "}"
           << ", \"coarse_restriction\": {\"thread_nodes\": " << nrsmall_ << ", \"wave_pieces\": " << nrbig_
           << ", \"split_nodes\": " << nrcomb_ << "}"
           << ", \"learned\": {\"used\": " << (learned_used_ ? "true" : "false")
@@ -355,14 +358,22 @@ namespace psmas
             throw InvalidArgumentError("[MAS] empty batch.");
         for (const auto &A : As)
             validate_csc(A);
-        if (As.size() == 1 && try_refresh_values(As[0]))
+        if (As.size() == 1 && try_refresh_values(As[0], nullptr))
             return;
+        factorize_full(As, nullptr, nullptr);
+    }
+
+    void Solver::factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir)
+    {
         const int bd = params_.block_dim;
         factorized_ = false;
         factorize_mode_ = "full";
         vr_colptr_.clear();
         vr_rowidx_.clear();
         d_vmap_.reset();
+        dirichlet_ = false;
+        dir_nodes_.clear();
+        n_crow_ = 0;
         graphs_->clear();
         const int K = static_cast<int>(As.size());
         if (static_cast<int>(sys_.size()) != K)
@@ -444,8 +455,17 @@ namespace psmas
             std::vector<int> map;
             if (build_value_map(As[0], bd, sys_[0].slot_of_block, all.rows, all.cols, h_slice_ptr_, map))
             {
-                vr_colptr_.assign(As[0].colptr, As[0].colptr + As[0].n + 1);
-                vr_rowidx_.assign(As[0].rowidx, As[0].rowidx + As[0].nnz);
+                const CscView &P = raw != nullptr ? *raw : As[0];
+                if (dir != nullptr)
+                {
+                    // Map the non-zeros of the matrix before elimination; dropped ones get -1.
+                    std::vector<int> full(dir->keep.size());
+                    for (size_t k = 0; k < full.size(); ++k)
+                        full[k] = dir->keep[k] >= 0 ? map[static_cast<size_t>(dir->keep[k])] : -1;
+                    map.swap(full);
+                }
+                vr_colptr_.assign(P.colptr, P.colptr + P.n + 1);
+                vr_rowidx_.assign(P.rowidx, P.rowidx + P.nnz);
                 d_vmap_.allocate(map.size(), "MAS value map");
                 PSMAS_HIP(hipMemcpy(d_vmap_.data(), map.data(), d_vmap_.bytes(), hipMemcpyHostToDevice));
             }
@@ -499,10 +519,12 @@ namespace psmas
         factorized_ = true;
     }
 
-    bool Solver::try_refresh_values(const CscView &A)
+    bool Solver::try_refresh_values(const CscView &A, const std::vector<int> *dirichlet_nodes)
     {
         if (!params_.lazy_partitioning || !factorized_ || sys_.size() != 1 || !d_vmap_ ||
             params_.block_dim != bd_ || A.n != sys_[0].n || static_cast<size_t>(A.nnz) != vr_rowidx_.size() ||
+            (dirichlet_nodes != nullptr) != dirichlet_ ||
+            (dirichlet_nodes != nullptr && *dirichlet_nodes != dir_nodes_) ||
             !same_ints(A.colptr, vr_colptr_.data(), vr_colptr_.size()) ||
             !same_ints(A.rowidx, vr_rowidx_.data(), vr_rowidx_.size()))
             return false;
@@ -511,9 +533,104 @@ namespace psmas
         if (d_csc_vals_.size() != static_cast<size_t>(A.nnz))
             d_csc_vals_.allocate(static_cast<size_t>(A.nnz), "MAS input values");
         PSMAS_HIP(hipMemcpyAsync(d_csc_vals_.data(), A.vals, d_csc_vals_.bytes(), hipMemcpyHostToDevice, stream_));
-        launch_scatter_values(A.nnz, d_vmap_.data(), d_csc_vals_.data(), d_vals64_.data(), stream_);
+        scatter_new_values(d_csc_vals_.data());
         finish_value_refresh();
         return true;
+    }
+
+    void Solver::scatter_new_values(const double *d_vals)
+    {
+        launch_scatter_values(static_cast<long long>(vr_rowidx_.size()), d_vmap_.data(), d_vals, d_vals64_.data(),
+                              stream_);
+        if (dirichlet_)
+            launch_gather_values(static_cast<int>(d_cval_.size()), d_cidx_.data(), d_vals, d_cval_.data(), stream_);
+    }
+
+    void Solver::dirichlet_factorize(const CscView &A, const int *nodes, int ndir)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        validate_csc(A);
+        if (ndir < 0 || (ndir > 0 && nodes == nullptr))
+            throw InvalidArgumentError("[MAS] invalid Dirichlet node list.");
+        std::vector<int> list(nodes, nodes + ndir);
+        for (int d : list)
+            if (d < 0 || d >= A.n)
+                throw InvalidArgumentError("[MAS] Dirichlet node index out of range.");
+        std::sort(list.begin(), list.end());
+        list.erase(std::unique(list.begin(), list.end()), list.end());
+        if (try_refresh_values(A, &list))
+            return;
+        std::vector<unsigned char> fixed(A.n, 0);
+        for (int d : list)
+            fixed[d] = 1;
+        DirichletSystem D;
+        eliminate_dirichlet(A, fixed, D);
+        const CscView At{A.n, static_cast<long long>(D.rowidx.size()), D.colptr.data(), D.rowidx.data(),
+                         D.vals.data()};
+        factorize_full(std::vector<CscView>{At}, &A, &D);
+        n_crow_ = static_cast<int>(D.crow.size());
+        const size_t nc = D.ccol.size();
+        std::vector<double> cval(nc);
+        for (size_t k = 0; k < nc; ++k)
+            cval[k] = A.vals[D.cidx[k]];
+        d_crow_.allocate(std::max<size_t>(1, D.crow.size()), "MAS Dirichlet couplings");
+        d_cptr_.allocate(D.cptr.size(), "MAS Dirichlet couplings");
+        d_ccol_.allocate(std::max<size_t>(1, nc), "MAS Dirichlet couplings");
+        d_cidx_.allocate(std::max<size_t>(1, nc), "MAS Dirichlet couplings");
+        d_cval_.allocate(std::max<size_t>(1, nc), "MAS Dirichlet couplings");
+        PSMAS_HIP(hipMemcpy(d_cptr_.data(), D.cptr.data(), d_cptr_.bytes(), hipMemcpyHostToDevice));
+        if (nc > 0)
+        {
+            PSMAS_HIP(hipMemcpy(d_crow_.data(), D.crow.data(), sizeof(int) * D.crow.size(), hipMemcpyHostToDevice));
+            PSMAS_HIP(hipMemcpy(d_ccol_.data(), D.ccol.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+            PSMAS_HIP(hipMemcpy(d_cidx_.data(), D.cidx.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+            PSMAS_HIP(hipMemcpy(d_cval_.data(), cval.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+        }
+        n_coupl_ = static_cast<long long>(nc);
+        dir_nodes_.swap(list);
+        dirichlet_ = true;
+    }
+
+    void Solver::dirichlet_lift_device(const double *d_f, double *d_g)
+    {
+        const int n = sys_[0].n;
+        if (d_g != d_f)
+            PSMAS_HIP(hipMemcpyAsync(d_g, d_f, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+        launch_dirichlet_lift(n_crow_, d_crow_.data(), d_cptr_.data(), d_ccol_.data(), d_cval_.data(), d_f, d_g,
+                              stream_);
+    }
+
+    void Solver::check_dirichlet(int n) const
+    {
+        if (!factorized_ || !dirichlet_)
+            throw InvalidArgumentError("[MAS] no Dirichlet factorisation: call the Dirichlet factorize first.");
+        if (n != sys_[0].n)
+            throw InvalidArgumentError("[MAS] invalid f: size does not match the factorised matrix.");
+    }
+
+    void Solver::dirichlet_solve_host(int n, double *f, double *u)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        check_dirichlet(n);
+        if (f == nullptr || u == nullptr)
+            throw InvalidArgumentError("[MAS] invalid f or u: null pointer.");
+        if (d_dir_f_.size() != static_cast<size_t>(n))
+            d_dir_f_.allocate(static_cast<size_t>(n), "MAS Dirichlet right-hand side");
+        PSMAS_HIP(hipMemcpyAsync(d_dir_f_.data(), f, sizeof(double) * n, hipMemcpyHostToDevice, stream_));
+        dirichlet_lift_device(d_dir_f_.data(), d_dir_f_.data());
+        PSMAS_HIP(hipMemcpyAsync(f, d_dir_f_.data(), sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        solve_host(n, f, u);
+    }
+
+    void Solver::dirichlet_solve_device(int n, double *d_f, double *d_u)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        check_dirichlet(n);
+        if (d_f == nullptr || d_u == nullptr)
+            throw InvalidArgumentError("[MAS] invalid f or u: null pointer.");
+        dirichlet_lift_device(d_f, d_f);
+        solve_device(n, d_f, d_u);
     }
 
     void Solver::refresh_values_device(long long nnz, const double *d_vals)
@@ -526,7 +643,7 @@ namespace psmas
                 "lazy_partitioning, no duplicate entries, and the same number of non-zeros.");
         factorized_ = false;
         graphs_->clear();
-        launch_scatter_values(nnz, d_vmap_.data(), d_vals, d_vals64_.data(), stream_);
+        scatter_new_values(d_vals);
         finish_value_refresh();
     }
 

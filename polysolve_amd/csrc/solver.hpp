@@ -97,6 +97,16 @@ namespace psmas
         /// last full factorisation with lazy_partitioning).
         void refresh_values_device(long long nnz, const double *d_vals);
 
+        /// Dirichlet elimination on the factorisation (prefactorize, FEMSolver.cpp:269-315): factorise
+        /// the matrix with the rows and columns of the fixed dofs replaced by identity; remembers A's
+        /// couplings to the fixed dofs for the right-hand side lift.  With lazy_partitioning and the
+        /// same pattern and node set as the last such call, only the values are refreshed.
+        void dirichlet_factorize(const CscView &A, const int *nodes, int ndir);
+        /// dirichlet_solve_prefactorized (FEMSolver.cpp:318-342): f := g = f - (I - N) A N f with
+        /// the A of the last Dirichlet factorisation, then u = solve(g).
+        void dirichlet_solve_host(int n, double *f, double *u);
+        void dirichlet_solve_device(int n, double *d_f, double *d_u);
+
     private:
         struct Graphs;
 
@@ -181,6 +191,15 @@ namespace psmas
         DeviceBuffer<double> d_csc_vals_;
         const char *factorize_mode_ = "full";
 
+        // Dirichlet elimination: the fixed dofs (sorted, unique) and A's couplings a_ij (i free,
+        // j fixed) by row, for the right-hand side lift.
+        bool dirichlet_ = false;
+        std::vector<int> dir_nodes_;
+        int n_crow_ = 0;
+        long long n_coupl_ = 0;
+        DeviceBuffer<int> d_crow_, d_cptr_, d_ccol_, d_cidx_;
+        DeviceBuffer<double> d_cval_, d_dir_f_;
+
         // Restriction of coarse segments 2.. straight from segment 1: every node's segment-1
         // descendants (ascending), summed by the side workgroups of the first apply launch (a
         // thread per node of at most RESTRICT_SMALL_MAX descendants, a wave per piece of at most
@@ -232,9 +251,19 @@ namespace psmas
         void build_mas();
         void build_bank_inverses(); ///< Galerkin bank matrices and their inverses (numeric part)
         /// Values-only refactorisation: one system, lazy_partitioning, the same n, block size and
-        /// sparsity pattern as the last full factorisation.  Scatters the new values into the SELL
-        /// arrays and rebuilds the numeric MAS parts; false when it does not apply.
-        bool try_refresh_values(const CscView &A);
+        /// sparsity pattern as the last full factorisation, and the same fixed dofs (dirichlet_nodes,
+        /// sorted and unique) when that was a Dirichlet one (nullptr: a plain one).  Scatters the
+        /// new values into the SELL arrays and rebuilds the numeric MAS parts; false when it does
+        /// not apply.
+        bool try_refresh_values(const CscView &A, const std::vector<int> *dirichlet_nodes);
+        /// Full factorisation; for a Dirichlet one, As[0] is the eliminated matrix, raw the matrix
+        /// before elimination (whose non-zeros the value map then indexes) and dir the elimination.
+        void factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir);
+        /// New SELL values (and coupling values of a Dirichlet factorisation) from d_vals in the
+        /// recorded CSC order.
+        void scatter_new_values(const double *d_vals);
+        void dirichlet_lift_device(const double *d_f, double *d_g);
+        void check_dirichlet(int n) const;
         /// The numeric rebuild after the SELL values were refreshed.
         void finish_value_refresh();
 

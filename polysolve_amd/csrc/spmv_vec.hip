@@ -522,9 +522,62 @@ namespace psmas
         {
             const long long k = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
             if (k < n)
-                dst[map[k]] = src[k];
+            {
+                const int v = map[k];
+                if (v >= 0)
+                    dst[v] = src[k];
+            }
+        }
+
+        __global__ __launch_bounds__(256) void k_gather_values(int n, const int *__restrict__ idx,
+                                                               const double *__restrict__ src, double *__restrict__ dst)
+        {
+            const int k = blockIdx.x * 256 + threadIdx.x;
+            if (k < n)
+                dst[k] = src[idx[k]];
+        }
+
+        // g_i = f_i - sum_k a_k f_{col_k} over row i's couplings, summed in list order with
+        // separate roundings (no fused multiply-add), as a sequential sparse product does.
+        __global__ __launch_bounds__(256) void k_dirichlet_lift(int nrows, const int *__restrict__ crow,
+                                                                const int *__restrict__ cptr,
+                                                                const int *__restrict__ ccol,
+                                                                const double *__restrict__ cval, const double *f,
+                                                                double *g)
+        {
+#pragma clang fp contract(off)
+            const int t = blockIdx.x * 256 + threadIdx.x;
+            if (t >= nrows)
+                return;
+            const int i = crow[t];
+            double s = 0.0;
+            for (int k = cptr[t]; k < cptr[t + 1]; ++k)
+            {
+                const double prod = cval[k] * f[ccol[k]];
+                s = s + prod;
+            }
+            g[i] = f[i] - s;
         }
     } // namespace
+
+    void launch_gather_values(int n, const int *idx, const double *src, double *dst, hipStream_t s)
+    {
+        if (n <= 0)
+            return;
+        hipLaunchKernelGGL(k_gather_values, dim3(static_cast<unsigned>(div_up(n, 256))), dim3(256), 0, s, n, idx, src,
+                           dst);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_dirichlet_lift(int nrows, const int *crow, const int *cptr, const int *ccol, const double *cval,
+                               const double *f, double *g, hipStream_t s)
+    {
+        if (nrows <= 0)
+            return;
+        hipLaunchKernelGGL(k_dirichlet_lift, dim3(static_cast<unsigned>(div_up(nrows, 256))), dim3(256), 0, s, nrows,
+                           crow, cptr, ccol, cval, f, g);
+        PSMAS_LAUNCH_CHECK();
+    }
 
     void launch_scatter_values(long long n, const int *map, const double *src, double *dst, hipStream_t s)
     {

@@ -165,6 +165,32 @@ class MASSolver(Solver):
         """Solve with b and x already in device memory (raw HIP pointers)."""
         _check(N.lib().psmas_solve_device(self._h, int(n), C.c_void_p(b_ptr), C.c_void_p(x_ptr)), self._h)
 
+    def dirichlet_factorize(self, A, dirichlet_nodes) -> None:
+        """prefactorize (FEMSolver.cpp:269-315): factorise A with the rows and columns of the fixed
+        dofs replaced by identity; A itself is not modified."""
+        n, colptr, rowidx, vals = _as_csc(A)
+        nodes = np.ascontiguousarray(dirichlet_nodes, dtype=np.int32).ravel()
+        _check(N.lib().psmas_dirichlet_factorize_csc(self._h, n, int(rowidx.size), _ptr(colptr, C.c_int),
+                                                     _ptr(rowidx, C.c_int), _ptr(vals, C.c_double),
+                                                     int(nodes.size), _ptr(nodes, C.c_int)), self._h)
+        self._n = n
+
+    def dirichlet_solve(self, f, u) -> None:
+        """dirichlet_solve_prefactorized (FEMSolver.cpp:318-342): f := g = f - (I - N) A N f, then
+        u = solve(g); f and u are contiguous float64 arrays written in place."""
+        for name, a in (("f", f), ("u", u)):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+                raise MASError(f"[MAS] {name} must be a contiguous float64 numpy array")
+        if f.size != u.size:
+            raise MASError("[MAS] Size mismatch between f and u.")
+        _check(N.lib().psmas_dirichlet_solve(self._h, int(f.size), _ptr(f, C.c_double), _ptr(u, C.c_double)),
+               self._h)
+
+    def dirichlet_solve_device(self, f_ptr: int, u_ptr: int, n: int) -> None:
+        """dirichlet_solve with f and u in device memory (raw HIP pointers); f becomes g."""
+        _check(N.lib().psmas_dirichlet_solve_device(self._h, int(n), C.c_void_p(f_ptr), C.c_void_p(u_ptr)),
+               self._h)
+
     def factorize_values_device(self, vals_ptr: int, nnz: int) -> None:
         """Refactorise from nnz values already in device memory (raw HIP pointer), in the CSC order
         of the last full factorisation, which must have used lazy_partitioning."""
@@ -309,6 +335,42 @@ class MASSolver(Solver):
         rc, s = N.read_string(N.lib().psmas_get_kernel_timing, self._h)
         _check(rc, self._h)
         return _json.loads(s)
+
+
+def _mas_only(solver) -> "MASSolver":
+    if not isinstance(solver, MASSolver):
+        raise MASError("[MAS] the Dirichlet helpers here drive a MASSolver")
+    return solver
+
+
+def prefactorize(solver, A, dirichlet_nodes, precond_num: int = 0, save_path: str = "") -> None:
+    """FEMSolver.hpp:42-44.  The reference overwrites A with the eliminated matrix; A is kept
+    here (its couplings to the fixed dofs live on the device for the right-hand side lift)."""
+    if save_path:
+        raise MASError("[MAS] save_path is not supported: the eliminated matrix is built on the solver side")
+    s = _mas_only(solver)
+    s.analyze_pattern(A, precond_num)
+    s.dirichlet_factorize(A, dirichlet_nodes)
+
+
+def dirichlet_solve_prefactorized(solver, A, f, dirichlet_nodes, u) -> None:
+    """FEMSolver.hpp:46-47: f := g = f - (I - N) A N f and u = solve(g).  The lift uses the A
+    and the fixed dofs given to prefactorize; A and dirichlet_nodes are accepted for the
+    reference's signature."""
+    _mas_only(solver).dirichlet_solve(f, u)
+
+
+def dirichlet_solve(solver, A, f, dirichlet_nodes, u, precond_num: int = 0, save_path: str = "",
+                    compute_spectrum: bool = False, remove_zero_cols: bool = False,
+                    skip_last_cols: bool = False) -> np.ndarray:
+    """FEMSolver.hpp:35-40 (FEMSolver.cpp:97-266): prefactorize + dirichlet_solve_prefactorized.
+    Returns the spectrum slot of the reference (zeros; compute_spectrum needs Spectra there and is
+    refused here, as are remove_zero_cols and skip_last_cols)."""
+    if compute_spectrum or remove_zero_cols or skip_last_cols:
+        raise MASError("[MAS] compute_spectrum / remove_zero_cols / skip_last_cols are not supported")
+    prefactorize(solver, A, dirichlet_nodes, precond_num, save_path)
+    dirichlet_solve_prefactorized(solver, A, f, dirichlet_nodes, u)
+    return np.zeros(4)
 
 
 def parse_parameters(params) -> dict:

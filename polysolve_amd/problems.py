@@ -51,9 +51,12 @@ def unit_element_stiffness(nu: float = 0.3) -> np.ndarray:
 
 
 def hex_elasticity(ne: int, E=1e5, nu: float = 0.3, body_force_z: float = -9.81,
-                   pairs: tuple | None = None, rhs: np.ndarray | None = None, name: str = "") -> Problem:
+                   pairs: tuple | None = None, rhs: np.ndarray | None = None, name: str = "",
+                 fix_bottom: bool = True) -> Problem:
     """Assemble the system.  ``E`` is a scalar or ne^3 per-element moduli (element order
-    i + ne*(j + ne*k)); ``pairs`` = (i, j, g (p,6), kappa (p,)) adds kappa g g^T blocks."""
+    i + ne*(j + ne*k)); ``pairs`` = (i, j, g (p,6), kappa (p,)) adds kappa g g^T blocks.
+    ``fix_bottom=False`` returns the bare stiffness matrix (singular: rigid motions) and load,
+    whose bottom-face DOFs are bottom_face_dofs(ne)."""
     import scipy.sparse as sp
 
     L = N.problems_lib()
@@ -72,7 +75,7 @@ def hex_elasticity(ne: int, E=1e5, nu: float = 0.3, body_force_z: float = -9.81,
     n = C.c_int(0)
     nnz = C.c_longlong(0)
     h = L.psgen_hex_assemble(int(ne), E_arr.ctypes.data_as(C.POINTER(C.c_double)), float(nu),
-                             float(body_force_z), *args, C.byref(n), C.byref(nnz))
+                             float(body_force_z), *args, int(bool(fix_bottom)), C.byref(n), C.byref(nnz))
     if not h:
         raise MemoryError("hex assembly failed")
     try:
@@ -88,9 +91,14 @@ def hex_elasticity(ne: int, E=1e5, nu: float = 0.3, body_force_z: float = -9.81,
     A.has_sorted_indices = True
     if rhs is not None:
         b = np.ascontiguousarray(rhs, dtype=np.float64).copy()
-        N1 = ne + 1
-        b[: 3 * N1 * N1] = 0.0  # Dirichlet DOFs (bottom face) carry their zero value
+        if fix_bottom:
+            b[bottom_face_dofs(ne)] = 0.0  # Dirichlet DOFs (bottom face) carry their zero value
     return Problem(A=A, b=b, coords=grid_coords(ne), ne=ne, name=name, meta={"nu": nu})
+
+
+def bottom_face_dofs(ne: int) -> np.ndarray:
+    """DOF ids of the bottom face z = 0 (nodes k = 0, all three components)."""
+    return np.arange(3 * (ne + 1) ** 2, dtype=np.int32)
 
 
 def random_moduli(ne: int, seed: int = 0, lo: float = 3.0, hi: float = 6.0) -> np.ndarray:
