@@ -596,8 +596,10 @@ namespace psmas
         const int n = sys_[0].n;
         if (d_g != d_f)
             PSMAS_HIP(hipMemcpyAsync(d_g, d_f, sizeof(double) * n, hipMemcpyDeviceToDevice, stream_));
+        tic("dirichlet_lift", static_cast<double>(n_coupl_) * 20.0 + static_cast<double>(n_crow_) * 24.0);
         launch_dirichlet_lift(n_crow_, d_crow_.data(), d_cptr_.data(), d_ccol_.data(), d_cval_.data(), d_f, d_g,
                               stream_);
+        toc();
     }
 
     void Solver::check_dirichlet(int n) const
