@@ -26,7 +26,19 @@ namespace psmas
         const int *cols = nullptr;
         const TV *vals = nullptr;
         int unroll = 4; ///< slots per batch in the SpMV row loop: 2, 4 or 8 (autotuned)
+        /// Halo form of the columns: per node block (256 rows, one SpMV workgroup) the distinct
+        /// columns its slots read, ascending (halo[halo_ptr[b] .. halo_ptr[b+1])), and every slot's
+        /// column as a 16-bit index into its block's list (lcols, same slot layout as cols).
+        /// halo_max = the longest list; 0 = not in use (x is gathered through cols).
+        const unsigned short *lcols = nullptr;
+        const int *halo_ptr = nullptr;
+        const int *halo = nullptr;
+        int halo_max = 0;
     };
+
+    /// Longest per-workgroup halo the SpMV stages in LDS (nodes); larger sets fall back to
+    /// gathering x through the 32-bit columns.
+    constexpr int SPMV_HALO_CAP = 2048;
 
     /// Restriction of fine (bank-slot) values to coarse level 0: slot p that leads its bank-local
     /// connected component carries the component's member mask (lanes of its bank) and the

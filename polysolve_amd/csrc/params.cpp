@@ -274,7 +274,7 @@ namespace psmas
             "use_preconditioned_residual_norm", "precision", "aggregation", "coarse_solver",
             "reliable_update_delta", "check_interval", "true_residual_period", "use_graphs",
             "learned_correction", "division_guard", "learned_collect", "lattice_points", "learned_hidden",
-            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll",
+            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll", "spmv_halo",
             "restrict_piece"};
     } // namespace
 
@@ -409,6 +409,8 @@ namespace psmas
             if (q.spmv_unroll != 2 && q.spmv_unroll != 4 && q.spmv_unroll != 8)
                 bad("spmv_unroll", "must be 2, 4 or 8");
         }
+        if (const JValue *v = obj->get("spmv_halo"))
+            q.spmv_halo = as_bool(*v, "spmv_halo");
         if (const JValue *v = obj->get("restrict_piece"))
         {
             q.restrict_piece = as_int(*v, "restrict_piece");
@@ -453,7 +455,7 @@ namespace psmas
           << ", \"learned_random_pairs\": " << p.learned_random_pairs
           << ", \"learned_snapshots\": " << p.learned_snapshots
           << ", \"learned_stall_segments\": " << p.learned_stall_segments
-          << ", \"learned_output_scale\": " << p.learned_output_scale << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"restrict_piece\": " << p.restrict_piece
+          << ", \"learned_output_scale\": " << p.learned_output_scale << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"spmv_halo\": " << (p.spmv_halo ? "true" : "false") << ", \"restrict_piece\": " << p.restrict_piece
           << "}";
         return o.str();
     }

@@ -28,6 +28,9 @@
 //                        correction is switched off for the rest of the solve
 //   learned_output_scale factor on the learned correction (1 = as trained)
 //   spmv_unroll          SpMV row-loop batch depth 2 | 4 | 8 (chosen by psmas_autotune)
+//   spmv_halo            SpMV gathers x from a per-workgroup copy of the entries its rows read,
+//                        staged in LDS, with 16-bit local columns (default true; used when every
+//                        workgroup's set fits in SPMV_HALO_CAP nodes)
 //   restrict_piece       longest level-1 descendant list one wave sums when restricting coarse
 //                        levels 2.. (longer lists are cut into pieces added where they are read);
 //                        9..512, default 512; takes effect at the next factorize
@@ -85,6 +88,7 @@ namespace psmas
         int learned_stall_segments = 3;
         double learned_output_scale = 1.0;
         int spmv_unroll = 4;
+        bool spmv_halo = true;
         int restrict_piece = 512;
     };
 

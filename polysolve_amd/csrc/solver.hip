@@ -31,6 +31,7 @@
 #include <cstring>
 #include <functional>
 #include <sstream>
+#include <atomic>
 #include <thread>
 
 namespace psmas
@@ -136,7 +137,8 @@ namespace psmas
         apply_json_parameters(params_, json);
         if (before.use_graphs != params_.use_graphs || before.check_interval != params_.check_interval ||
             before.true_residual_period != params_.true_residual_period || before.precision != params_.precision ||
-            before.coarse_solver != params_.coarse_solver || before.spmv_unroll != params_.spmv_unroll)
+            before.coarse_solver != params_.coarse_solver || before.spmv_unroll != params_.spmv_unroll ||
+            before.spmv_halo != params_.spmv_halo)
             graphs_->clear();
     }
 
@@ -330,7 +332,22 @@ namespace psmas
         v.cols = d_cols_.data();
         v.vals = vals;
         v.unroll = params_.spmv_unroll;
+        if (params_.spmv_halo && halo_max_ > 0)
+        {
+            v.lcols = d_lcols_.data();
+            v.halo_ptr = d_halo_ptr_.data();
+            v.halo = d_halo_.data();
+            v.halo_max = halo_max_;
+        }
         return v;
+    }
+
+    double Solver::spmv_bytes(int val_bytes, int x_bytes, int vec_passes) const
+    {
+        const bool halo = params_.spmv_halo && halo_max_ > 0;
+        return static_cast<double>(nnzb_) * (bd_ * bd_ * val_bytes + (halo ? 2 : 4)) +
+               (halo ? static_cast<double>(halo_total_) * 4.0 + (nblocks_ + 1) * 4.0 : (nb_total_ + 1) * 4.0) +
+               static_cast<double>(vec_passes) * n_total_ * x_bytes;
     }
 
     void Solver::read_scalars()
@@ -793,6 +810,85 @@ namespace psmas
         PSMAS_HIP(hipMemcpy(d_row_len_.data(), row_len.data(), d_row_len_.bytes(), hipMemcpyHostToDevice));
         PSMAS_HIP(hipMemcpy(d_block_of_slot_.data(), block_of_slot_.data(), d_block_of_slot_.bytes(),
                             hipMemcpyHostToDevice));
+        build_halo(slice_ptr, cols);
+    }
+
+    void Solver::build_halo(const std::vector<int> &slice_ptr, const std::vector<int> &cols)
+    {
+        halo_max_ = 0;
+        halo_total_ = 0;
+        d_lcols_.reset();
+        d_halo_ptr_.reset();
+        d_halo_.reset();
+        constexpr int SLICES = NODE_BLOCK / WAVE;
+        const int nblk = npad_ / NODE_BLOCK;
+        std::vector<std::vector<int>> lists(nblk);
+        std::vector<unsigned short> lcols(cols.size());
+        std::atomic<bool> fits{true};
+        auto work = [&](int b0, int b1) {
+            std::vector<int> local(npad_, -1);
+            for (int b = b0; b < b1 && fits; ++b)
+            {
+                std::vector<int> &u = lists[b];
+                const size_t i0 = static_cast<size_t>(slice_ptr[b * SLICES]) * WAVE;
+                const size_t i1 = static_cast<size_t>(slice_ptr[(b + 1) * SLICES]) * WAVE;
+                for (size_t i = i0; i < i1; ++i)
+                    if (local[cols[i]] < 0)
+                    {
+                        local[cols[i]] = 0;
+                        u.push_back(cols[i]);
+                    }
+                std::sort(u.begin(), u.end());
+                const bool ok = static_cast<int>(u.size()) <= SPMV_HALO_CAP;
+                if (ok)
+                {
+                    for (size_t k = 0; k < u.size(); ++k)
+                        local[u[k]] = static_cast<int>(k);
+                    for (size_t i = i0; i < i1; ++i)
+                        lcols[i] = static_cast<unsigned short>(local[cols[i]]);
+                }
+                else
+                {
+                    fits = false;
+                }
+                for (int c : u)
+                    local[c] = -1;
+            }
+        };
+        {
+            const int nt = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+            std::vector<std::thread> th;
+            const int chunk = div_up(nblk, nt);
+            for (int t = 0; t < nt; ++t)
+            {
+                const int b = t * chunk, e = std::min(nblk, b + chunk);
+                if (b < e)
+                    th.emplace_back(work, b, e);
+            }
+            for (auto &x : th)
+                x.join();
+        }
+        if (!fits)
+            return;
+        std::vector<int> ptr(static_cast<size_t>(nblk) + 1, 0);
+        int mx = 0;
+        for (int b = 0; b < nblk; ++b)
+        {
+            ptr[b + 1] = ptr[b] + static_cast<int>(lists[b].size());
+            mx = std::max(mx, static_cast<int>(lists[b].size()));
+        }
+        std::vector<int> all(static_cast<size_t>(ptr[nblk]));
+        for (int b = 0; b < nblk; ++b)
+            std::copy(lists[b].begin(), lists[b].end(), all.begin() + ptr[b]);
+        d_lcols_.allocate(lcols.size(), "MAS SELL local columns");
+        d_halo_ptr_.allocate(ptr.size(), "MAS SpMV halo lists");
+        d_halo_.allocate(std::max<size_t>(1, all.size()), "MAS SpMV halo lists");
+        PSMAS_HIP(hipMemcpy(d_lcols_.data(), lcols.data(), d_lcols_.bytes(), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_halo_ptr_.data(), ptr.data(), d_halo_ptr_.bytes(), hipMemcpyHostToDevice));
+        if (!all.empty())
+            PSMAS_HIP(hipMemcpy(d_halo_.data(), all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice));
+        halo_max_ = mx;
+        halo_total_ = static_cast<long long>(all.size());
     }
 
     void Solver::build_mas()
@@ -1304,15 +1400,14 @@ namespace psmas
     void Solver::fp64_iteration(int k)
     {
         const SellView<double> A = sell(d_vals64_.data());
-        const double spmv_bytes =
-            static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_total_ + 1) * 4.0 + 2.0 * n_total_ * 8.0;
-        tic("spmv_fp64", spmv_bytes);
+        const double bytes = spmv_bytes(8, 8, 2);
+        tic("spmv_fp64", bytes);
         launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), reductions(), sysmap(), stream_);
         toc();
         if (k % params_.true_residual_period == 0)
         {
             launch_axpy_alpha<BD, double>(nblocks_, d_p_.data(), d_x_.data(), reductions(), sysmap(), stream_);
-            tic("spmv_res_fp64", spmv_bytes + n_total_ * 8.0);
+            tic("spmv_res_fp64", bytes + n_total_ * 8.0);
             launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
                                                         d_mr_.data() + static_cast<size_t>(npad_) * BD,
                                                         reductions().rr, sysmap(), false, stream_);
@@ -1418,7 +1513,7 @@ namespace psmas
     void Solver::mixed_part_a(bool learned)
     {
         const SellView<float> A = sell(d_vals32_.data());
-        tic("spmv_fp32", static_cast<double>(nnzb_) * (BD * BD * 4 + 4) + (nb_total_ + 1) * 4.0 + 2.0 * n_total_ * 4.0);
+        tic("spmv_fp32", spmv_bytes(4, 4, 2));
         launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), reductions(), sysmap(), stream_);
         toc();
         // With the learned correction on the block path, the update also leaves the lattice block
@@ -1517,8 +1612,7 @@ namespace psmas
     void Solver::mixed_reliable_residual(bool learned)
     {
         const SellView<double> A = sell(d_vals64_.data());
-        tic("spmv_res_fp64",
-            static_cast<double>(nnzb_) * (BD * BD * 8 + 4) + (nb_total_ + 1) * 4.0 + 3.0 * n_total_ * 8.0);
+        tic("spmv_res_fp64", spmv_bytes(8, 8, 3));
         launch_spmv_res<BD, double, double, float>(A, d_x_.data(), d_b_.data(), d_mr32_.data(), fine_restrict(),
                                                    d_mr32_.data() + static_cast<size_t>(npad_) * BD,
                                                    reductions().rr_true, sysmap(), true, stream_);

@@ -149,6 +149,12 @@ namespace psmas
         int nslices_ = 0;
         long long nslots_ = 0;
         DeviceBuffer<int> d_slice_ptr_, d_cols_, d_row_len_, d_block_of_slot_, d_slot_of_block_;
+        // Halo form of the SELL columns (see SellView): 16-bit local columns and per-node-block
+        // lists; halo_max_ = 0 when some block reads more than SPMV_HALO_CAP distinct nodes.
+        DeviceBuffer<unsigned short> d_lcols_;
+        DeviceBuffer<int> d_halo_ptr_, d_halo_;
+        int halo_max_ = 0;
+        long long halo_total_ = 0;
         DeviceBuffer<double> d_vals64_;
         DeviceBuffer<float> d_vals32_;
 
@@ -248,6 +254,11 @@ namespace psmas
 
         void build_partition(System &S, const CscView &A);
         void upload_matrix(const PaddedBsr &bsr);
+        /// Per node block, the distinct columns of its SELL slots and the 16-bit local columns.
+        void build_halo(const std::vector<int> &slice_ptr, const std::vector<int> &cols);
+        /// Algorithmic bytes of one SpMV: values (val_bytes each), columns, the halo lists when in
+        /// use, and vec_passes row vectors of x_bytes per scalar.
+        double spmv_bytes(int val_bytes, int x_bytes, int vec_passes) const;
         void build_mas();
         void build_bank_inverses(); ///< Galerkin bank matrices and their inverses (numeric part)
         /// Values-only refactorisation: one system, lazy_partitioning, the same n, block size and

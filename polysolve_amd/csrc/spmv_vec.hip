@@ -20,6 +20,9 @@
 
 namespace psmas
 {
+    /// Dynamic LDS of the SpMV kernels: the workgroup's halo of x.
+    extern __shared__ __align__(16) unsigned char spmv_halo_lds[];
+
     namespace
     {
         constexpr int SPMV_BLOCK = NODE_BLOCK; // 4 waves = 4 slices of 64 rows
@@ -37,9 +40,13 @@ namespace psmas
         /// unconditionally (slot indices past the slice end are clamped to its last slot, whose
         /// lines are in cache, and their values are zeroed), which lets the compiler count the
         /// loads in flight and keep the column prefetch outstanding across the FMAs.
-        template <int BD, int U, typename TV, typename TX, typename TA>
+        ///
+        /// HALO: x comes from the workgroup's LDS copy xs of the entries its rows read, indexed by
+        /// 16-bit local columns (stage_halo fills it); otherwise x is gathered from global memory
+        /// through the 32-bit columns.
+        template <int BD, int U, bool HALO, typename TV, typename TX, typename TA>
         __device__ __forceinline__ void sell_row(const SellView<TV> &A, int slice, int lane, const TX *__restrict__ x,
-                                                 TA acc[BD])
+                                                 const TX *xs, TA acc[BD])
         {
             const int beg = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice]);
             const int end = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice + 1]);
@@ -47,11 +54,18 @@ namespace psmas
                 return;
             const int last = end - 1;
             const int *__restrict__ cols = A.cols + lane;
+            const unsigned short *__restrict__ lcols = A.lcols + lane;
             const TV *__restrict__ vals = A.vals + lane;
+            auto column = [&](int slot) -> int {
+                if constexpr (HALO)
+                    return static_cast<int>(ldg_stream(lcols + static_cast<size_t>(slot) * WAVE));
+                else
+                    return ldg_stream(cols + static_cast<size_t>(slot) * WAVE);
+            };
             int c[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                c[u] = ldg_stream(cols + static_cast<size_t>(min(beg + u, last)) * WAVE);
+                c[u] = column(min(beg + u, last));
             for (int k = beg; k < end; k += U)
             {
                 TA xv[U][BD];
@@ -60,19 +74,30 @@ namespace psmas
                 for (int u = 0; u < U; ++u)
                 {
                     const size_t slot = static_cast<size_t>(min(k + u, last));
+                    if constexpr (!HALO)
+                    {
 #pragma unroll
-                    for (int j = 0; j < BD; ++j)
-                        xv[u][j] = static_cast<TA>(x[static_cast<size_t>(c[u]) * BD + j]);
+                        for (int j = 0; j < BD; ++j)
+                            xv[u][j] = static_cast<TA>(x[static_cast<size_t>(c[u]) * BD + j]);
+                    }
 #pragma unroll
                     for (int e = 0; e < BD * BD; ++e)
                         v[u][e] = ldg_stream(vals + (slot * (BD * BD) + e) * WAVE);
+                }
+                if constexpr (HALO)
+                {
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int j = 0; j < BD; ++j)
+                            xv[u][j] = static_cast<TA>(xs[c[u] * BD + j]);
                 }
                 // Keep the next batch's column loads behind this batch's loads and ahead of the
                 // FMAs, so that they stay in flight while the FMAs wait for this batch.
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    c[u] = ldg_stream(cols + static_cast<size_t>(min(k + U + u, last)) * WAVE);
+                    c[u] = column(min(k + U + u, last));
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int u = 0; u < U; ++u)
@@ -100,7 +125,25 @@ namespace psmas
 
         __device__ __forceinline__ int block_system(const SysMap &sm) { return sm.block_sys[blockIdx.x]; }
 
-        template <int BD, int U, typename TV, typename TX>
+        /// Copy x at node block blk's halo into the dynamic LDS array and wait for the workgroup;
+        /// every wave of the workgroup must call it (uniform control flow).
+        template <int BD, typename TV, typename TX>
+        __device__ __forceinline__ const TX *stage_halo(const SellView<TV> &A, int blk, const TX *__restrict__ x)
+        {
+            TX *xs = reinterpret_cast<TX *>(spmv_halo_lds);
+            const int hb = A.halo_ptr[blk], hn = A.halo_ptr[blk + 1] - hb;
+            for (int i = threadIdx.x; i < hn; i += SPMV_BLOCK)
+            {
+                const size_t g = static_cast<size_t>(A.halo[hb + i]) * BD;
+#pragma unroll
+                for (int j = 0; j < BD; ++j)
+                    xs[i * BD + j] = x[g + j];
+            }
+            __syncthreads();
+            return xs;
+        }
+
+        template <int BD, int U, bool HALO, typename TV, typename TX>
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap(SellView<TV> A, const TX *__restrict__ x,
                                                                 TX *__restrict__ y, Reductions red, SysMap sm)
         {
@@ -109,13 +152,16 @@ namespace psmas
             const int sys = sm.block_sys[blk];
             if (sm.active[sys] == 0)
                 return;
+            const TX *xs = nullptr;
+            if constexpr (HALO)
+                xs = stage_halo<BD>(A, blk, x);
             const int lane = threadIdx.x % WAVE;
             const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
             TA acc[BD];
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD, U>(A, slice, lane, x, acc);
+            sell_row<BD, U, HALO>(A, slice, lane, x, xs, acc);
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
             double dot = 0.0;
 #pragma unroll
@@ -152,7 +198,7 @@ namespace psmas
                 yc0[static_cast<size_t>(gid) * BD + i] = static_cast<T>(s[i]);
         }
 
-        template <int BD, int U, typename TV, typename TX, typename TY>
+        template <int BD, int U, bool HALO, typename TV, typename TX, typename TY>
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_res(SellView<TV> A, const TX *__restrict__ x,
                                                                  const TX *__restrict__ b, TY *__restrict__ y,
                                                                  FineRestrict fr, TY *__restrict__ yc0,
@@ -165,6 +211,9 @@ namespace psmas
             const int sys = sm.block_sys[blk];
             if (sm.active[sys] == 0 || (selected_only && sm.select[sys] == 0))
                 return;
+            const TX *xs = nullptr;
+            if constexpr (HALO)
+                xs = stage_halo<BD>(A, blk, x);
             const int lane = threadIdx.x % WAVE;
             const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
@@ -172,7 +221,7 @@ namespace psmas
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD, U>(A, slice, lane, x, acc);
+            sell_row<BD, U, HALO>(A, slice, lane, x, xs, acc);
             double dot = 0.0;
 #pragma unroll
             for (int i = 0; i < BD; ++i)
@@ -203,7 +252,7 @@ namespace psmas
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD, U>(A, slice, lane, x, acc);
+            sell_row<BD, U, false>(A, slice, lane, x, static_cast<const TX *>(nullptr), acc);
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
 #pragma unroll
             for (int i = 0; i < BD; ++i)
@@ -489,8 +538,17 @@ namespace psmas
                         hipStream_t s)
     {
         const int grid = A.nslices / (SPMV_BLOCK / WAVE);
-        PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_ap<BD, U, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), 0,
-                                                           s, A, x, y, red, sm));
+        if (A.halo_max > 0)
+        {
+            const size_t lds = static_cast<size_t>(A.halo_max) * BD * sizeof(TX);
+            PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_ap<BD, U, true, TV, TX>), dim3(grid),
+                                                               dim3(SPMV_BLOCK), lds, s, A, x, y, red, sm));
+        }
+        else
+        {
+            PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_ap<BD, U, false, TV, TX>), dim3(grid),
+                                                               dim3(SPMV_BLOCK), 0, s, A, x, y, red, sm));
+        }
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -499,9 +557,20 @@ namespace psmas
                          double *rr_partials, const SysMap &sm, bool selected_only, hipStream_t s)
     {
         const int grid = A.nslices / (SPMV_BLOCK / WAVE);
-        PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_res<BD, U, TV, TX, TY>), dim3(grid),
-                                                           dim3(SPMV_BLOCK), 0, s, A, x, b, y, fr, yc0, rr_partials,
-                                                           sm, selected_only ? 1 : 0));
+        const int sel = selected_only ? 1 : 0;
+        if (A.halo_max > 0)
+        {
+            const size_t lds = static_cast<size_t>(A.halo_max) * BD * sizeof(TX);
+            PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_res<BD, U, true, TV, TX, TY>), dim3(grid),
+                                                               dim3(SPMV_BLOCK), lds, s, A, x, b, y, fr, yc0,
+                                                               rr_partials, sm, sel));
+        }
+        else
+        {
+            PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_res<BD, U, false, TV, TX, TY>), dim3(grid),
+                                                               dim3(SPMV_BLOCK), 0, s, A, x, b, y, fr, yc0,
+                                                               rr_partials, sm, sel));
+        }
         PSMAS_LAUNCH_CHECK();
     }
 
