@@ -110,6 +110,16 @@ int psmas_dirichlet_solve_device(psmas_handle h, int n, double *d_f, double *d_x
 /* Same as psmas_solve with b and x in device memory (HIP pointers on the solver's device). */
 int psmas_solve_device(psmas_handle h, int n, const double *d_b, double *d_x);
 
+/* Same as psmas_factorize_csc with the CSC arrays in device memory (HIP pointers on the solver's
+ * device): d_colptr n+1, d_rowidx and d_vals nnz entries.  The matrix is built on the GPU; the
+ * arrays are validated there first (the same errors as psmas_factorize_csc).  The host copies
+ * the pattern back only for a graph aggregation (no coordinates set).  With "lazy_partitioning"
+ * the values of later Newton steps can follow through psmas_factorize_values_device.  Replaces
+ * the reference's two host-to-device uploads of the CSC arrays (BSRAdjacency.cu:180-197,
+ * BSRMatrix.cu:214-231) for callers that assemble on the GPU. */
+int psmas_factorize_csc_device(psmas_handle h, int n, long long nnz, const int *d_colptr, const int *d_rowidx,
+                               const double *d_vals);
+
 /* Values-only refactorisation from device memory: d_vals holds nnz doubles (HIP pointer on the
  * solver's device) in the order of the column-compressed pattern of the last full factorisation,
  * which must have run with "lazy_partitioning": true on one system without duplicate entries.

@@ -185,6 +185,14 @@ extern "C"
         return guarded(h->last_error, [&] { h->solver.dirichlet_solve_device(n, d_f, d_x); });
     }
 
+    int psmas_factorize_csc_device(psmas_handle h, int n, long long nnz, const int *d_colptr, const int *d_rowidx,
+                                   const double *d_vals)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.factorize_device(n, nnz, d_colptr, d_rowidx, d_vals); });
+    }
+
     int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d_vals)
     {
         if (!h)

@@ -324,12 +324,8 @@ namespace psmas
         }
     } // namespace
 
-    void build_padded_bsr(const CscView &A, int bd, const Partition &part, PaddedBsr &out)
+    void assign_slots(int nb, int bd, const Partition &part, PaddedBsr &out)
     {
-        const int n = A.n;
-        const int nb = div_up(n, bd);
-        const int bs = bd * bd;
-        const int padded_scalars = bd * nb - n;
         out.bd = bd;
         out.nb = nb;
         out.nparts = div_up(std::max(part.nparts, 1), 8) * 8; // whole 256-node blocks
@@ -355,6 +351,15 @@ namespace psmas
             out.slot_of_block[b] = slot;
             out.block_of_slot[slot] = b;
         }
+    }
+
+    void build_padded_bsr(const CscView &A, int bd, const Partition &part, PaddedBsr &out)
+    {
+        const int n = A.n;
+        const int nb = div_up(n, bd);
+        const int bs = bd * bd;
+        const int padded_scalars = bd * nb - n;
+        assign_slots(nb, bd, part, out);
         const std::vector<int> &slot = out.slot_of_block;
 
         // Pattern: per block column, unique block rows; transpose into rows.

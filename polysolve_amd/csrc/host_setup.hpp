@@ -2,6 +2,7 @@
 // <=32-node domains, and the permuted, bank-padded BSR matrix.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
@@ -75,6 +76,9 @@ namespace psmas
         std::vector<double> vals;
     };
     void build_padded_bsr(const CscView &A, int bd, const Partition &part, PaddedBsr &out);
+    /// Only the slot assignment of build_padded_bsr (bd, nb, nparts, npad, slot_of_block,
+    /// block_of_slot), for the device matrix build.
+    void assign_slots(int nb, int bd, const Partition &part, PaddedBsr &out);
     /// Refresh only the values of an existing pattern (same sparsity), for re-factorisation.
     bool refresh_padded_bsr_values(const CscView &A, PaddedBsr &bsr);
     /// Index of every non-zero of A in the SELL-64 FP64 value array of one system's padded BSR

@@ -274,7 +274,7 @@ namespace psmas
             "use_preconditioned_residual_norm", "precision", "aggregation", "coarse_solver",
             "reliable_update_delta", "check_interval", "true_residual_period", "use_graphs",
             "learned_correction", "division_guard", "learned_collect", "lattice_points", "learned_hidden",
-            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll", "spmv_halo",
+            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll", "spmv_halo", "device_setup",
             "restrict_piece"};
     } // namespace
 
@@ -411,6 +411,8 @@ namespace psmas
         }
         if (const JValue *v = obj->get("spmv_halo"))
             q.spmv_halo = as_bool(*v, "spmv_halo");
+        if (const JValue *v = obj->get("device_setup"))
+            q.device_setup = as_bool(*v, "device_setup");
         if (const JValue *v = obj->get("restrict_piece"))
         {
             q.restrict_piece = as_int(*v, "restrict_piece");
@@ -455,7 +457,7 @@ namespace psmas
           << ", \"learned_random_pairs\": " << p.learned_random_pairs
           << ", \"learned_snapshots\": " << p.learned_snapshots
           << ", \"learned_stall_segments\": " << p.learned_stall_segments
-          << ", \"learned_output_scale\": " << p.learned_output_scale << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"spmv_halo\": " << (p.spmv_halo ? "true" : "false") << ", \"restrict_piece\": " << p.restrict_piece
+          << ", \"learned_output_scale\": " << p.learned_output_scale << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"spmv_halo\": " << (p.spmv_halo ? "true" : "false") << ", \"device_setup\": " << (p.device_setup ? "true" : "false") << ", \"restrict_piece\": " << p.restrict_piece
           << "}";
         return o.str();
     }

@@ -28,6 +28,8 @@
 //                        correction is switched off for the rest of the solve
 //   learned_output_scale factor on the learned correction (1 = as trained)
 //   spmv_unroll          SpMV row-loop batch depth 2 | 4 | 8 (chosen by psmas_autotune)
+//   device_setup         build the SELL matrix of a single system on the GPU from the CSC arrays
+//                        (default true; false: the host build)
 //   spmv_halo            SpMV gathers x from a per-workgroup copy of the entries its rows read,
 //                        staged in LDS, with 16-bit local columns (default true; used when every
 //                        workgroup's set fits in SPMV_HALO_CAP nodes)
@@ -89,6 +91,7 @@ namespace psmas
         double learned_output_scale = 1.0;
         int spmv_unroll = 4;
         bool spmv_halo = true;
+        bool device_setup = true;
         int restrict_piece = 512;
     };
 

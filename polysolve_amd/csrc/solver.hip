@@ -22,6 +22,7 @@
 // residual b - A x is recomputed with the FP64 matrix; convergence is decided on that FP64
 // residual only.
 #include "solver.hpp"
+#include "sell_build.hpp"
 #include "coarse_solve.hpp"
 #include "device.hpp"
 
@@ -254,9 +255,18 @@ namespace psmas
           << ", \"systems\": " << sys_.size() << ", \"aggregation\": \"" << S.aggregation << "\""
           << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\""
           << ", \"factorize_mode\": \"" << factorize_mode_ << "\""
+          << ", \"setup_ms\": {" << [&] {
+                 std::ostringstream q;
+                 q.precision(6);
+                 for (size_t i = 0; i < setup_ms_.size(); ++i)
+                     q << (i ? ", " : "") << "\"" << setup_ms_[i].first << "\": " << setup_ms_[i].second;
+                 return q.str();
+             }() << "}"
           << ", \"dirichlet\": {\"active\": " << (dirichlet_ ? "true" : "false")
           << ", \"fixed_dofs\": " << dir_nodes_.size() << ", \"coupled_rows\": " << n_crow_
           << ", \"couplings\": " << n_coupl_ << "}"
+          << ", \"spmv_halo\": {\"active\": " << (params_.spmv_halo && halo_max_ > 0 ? "true" : "false")
+          << ", \"max_nodes\": " << halo_max_ << ", \"total_nodes\": " << halo_total_ << "}"
           << ", \"coarse_restriction\": {\"thread_nodes\": " << nrsmall_ << ", \"wave_pieces\": " << nrbig_
           << ", \"split_nodes\": " << nrcomb_ << "}"
           << ", \"learned\": {\"used\": " << (learned_used_ ? "true" : "false")
@@ -380,11 +390,21 @@ namespace psmas
         factorize_full(As, nullptr, nullptr);
     }
 
-    void Solver::factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir)
+    void Solver::factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
+                                const DeviceCsc *dev)
     {
         const int bd = params_.block_dim;
         factorized_ = false;
         factorize_mode_ = "full";
+        setup_ms_.clear();
+        auto t_mark = std::chrono::steady_clock::now();
+        auto phase = [&](const char *name) {
+            // Host wall time of one setup phase (the stream is drained so GPU work is counted).
+            PSMAS_HIP(hipStreamSynchronize(stream_));
+            const auto now = std::chrono::steady_clock::now();
+            setup_ms_.emplace_back(name, std::chrono::duration<double, std::milli>(now - t_mark).count());
+            t_mark = now;
+        };
         vr_colptr_.clear();
         vr_rowidx_.clear();
         d_vmap_.reset();
@@ -397,6 +417,10 @@ namespace psmas
             sys_.resize(K);
         const bool bd_changed = (bd != bd_);
         bd_ = bd;
+        // One system without elimination: the SELL matrix is built on the GPU from the CSC arrays
+        // (uploaded here, or the caller's device arrays).
+        const bool gpu_build = K == 1 && dir == nullptr && (dev != nullptr || params_.device_setup);
+        vr_nnz_ = -1;
 
         // Per-system partition and bank-padded BSR, concatenated.
         PaddedBsr all;
@@ -421,8 +445,13 @@ namespace psmas
             S.nb = nb;
             if (rebuild)
                 build_partition(S, A);
+            if (K == 1)
+                phase("partition");
             PaddedBsr bsr;
-            build_padded_bsr(A, bd, S.part, bsr);
+            if (gpu_build)
+                assign_slots(nb, bd, S.part, bsr);
+            else
+                build_padded_bsr(A, bd, S.part, bsr);
             S.nparts = bsr.nparts;
             S.npad = bsr.npad;
             S.node_off = node_off;
@@ -433,12 +462,15 @@ namespace psmas
             sob_off += nb;
             h_slot_of_block.insert(h_slot_of_block.end(), bsr.slot_of_block.begin(), bsr.slot_of_block.end());
             block_of_slot_.insert(block_of_slot_.end(), bsr.block_of_slot.begin(), bsr.block_of_slot.end());
-            const int row_base = all.rows.back();
-            for (int r = 0; r < bsr.npad; ++r)
-                all.rows.push_back(row_base + bsr.rows[r + 1]);
-            for (int c : bsr.cols)
-                all.cols.push_back(c + node_off);
-            all.vals.insert(all.vals.end(), bsr.vals.begin(), bsr.vals.end());
+            if (!gpu_build) // (the device build makes the matrix itself; bsr holds only the slots)
+            {
+                const int row_base = all.rows.back();
+                for (int r = 0; r < bsr.npad; ++r)
+                    all.rows.push_back(row_base + bsr.rows[r + 1]);
+                for (int c : bsr.cols)
+                    all.cols.push_back(c + node_off);
+                all.vals.insert(all.vals.end(), bsr.vals.begin(), bsr.vals.end());
+            }
             node_off += bsr.npad;
             nnzb_ += S.nnzb;
             nnz_scalar_ += A.nnz;
@@ -450,14 +482,6 @@ namespace psmas
         nblocks_ = npad_ / NODE_BLOCK;
         all.npad = npad_;
         all.nparts = nparts_;
-        nonisolated_.assign(npad_, 0);
-        for (int r = 0; r < npad_; ++r)
-            for (int k = all.rows[r]; k < all.rows[r + 1]; ++k)
-                if (all.cols[k] != r)
-                {
-                    nonisolated_[r] = 1;
-                    break;
-                }
         dense_built_ = false;
         dense_nd_ = 0;
         dense_note_.clear();
@@ -465,8 +489,25 @@ namespace psmas
         d_dense_ids_.reset();
         d_dense_index_.reset();
 
-        upload_matrix(all);
-        if (K == 1 && params_.lazy_partitioning)
+        if (gpu_build)
+        {
+            device_matrix_build(As[0], dev, h_slot_of_block, phase);
+        }
+        else
+        {
+            nonisolated_.assign(npad_, 0);
+            for (int r = 0; r < npad_; ++r)
+                for (int k = all.rows[r]; k < all.rows[r + 1]; ++k)
+                    if (all.cols[k] != r)
+                    {
+                        nonisolated_[r] = 1;
+                        break;
+                    }
+            phase(K == 1 ? "padded_bsr" : "partition_and_bsr");
+            upload_matrix(all);
+            phase("sell_upload_and_halo");
+        }
+        if (!gpu_build && K == 1 && params_.lazy_partitioning)
         {
             // Remember the pattern and where each non-zero lives, for values-only refactorisation.
             std::vector<int> map;
@@ -483,9 +524,11 @@ namespace psmas
                 }
                 vr_colptr_.assign(P.colptr, P.colptr + P.n + 1);
                 vr_rowidx_.assign(P.rowidx, P.rowidx + P.nnz);
+                vr_nnz_ = P.nnz;
                 d_vmap_.allocate(map.size(), "MAS value map");
                 PSMAS_HIP(hipMemcpy(d_vmap_.data(), map.data(), d_vmap_.bytes(), hipMemcpyHostToDevice));
             }
+            phase("value_map");
         }
         all = PaddedBsr();
 
@@ -521,7 +564,9 @@ namespace psmas
         d_dscale_.reset();
         lattice_stale_ = true;
         rhs_ready_ = false;
+        phase("system_maps");
         build_mas();
+        phase("coarse_levels_and_inverses");
         {
             // Matrix -> system for the apply's skip test: fine banks as above, coarse matrices -1.
             std::vector<int> mat_sys(total_mats_, -1);
@@ -532,8 +577,110 @@ namespace psmas
         allocate_vectors();
         if (params_.precision == Precision::Mixed)
             ensure_mixed_data();
-        PSMAS_HIP(hipStreamSynchronize(stream_));
+        phase("vectors_and_mixed_data");
         factorized_ = true;
+    }
+
+    template <typename Phase>
+    void Solver::device_matrix_build(const CscView &A, const DeviceCsc *dev, const std::vector<int> &h_slot_of_block,
+                                     Phase &phase)
+    {
+        System &S = sys_[0];
+        const int *d_cp = nullptr, *d_ri = nullptr;
+        const double *d_v = nullptr;
+        DeviceBuffer<int> up_cp, up_ri;
+        if (dev != nullptr)
+        {
+            d_cp = dev->colptr;
+            d_ri = dev->rowidx;
+            d_v = dev->vals;
+        }
+        else
+        {
+            up_cp.allocate(static_cast<size_t>(A.n) + 1, "MAS input pattern");
+            up_ri.allocate(static_cast<size_t>(std::max<long long>(A.nnz, 1)), "MAS input pattern");
+            d_csc_vals_.ensure(static_cast<size_t>(std::max<long long>(A.nnz, 1)), "MAS input values");
+            PSMAS_HIP(hipMemcpyAsync(up_cp.data(), A.colptr, sizeof(int) * (A.n + 1), hipMemcpyHostToDevice, stream_));
+            if (A.nnz > 0)
+            {
+                PSMAS_HIP(hipMemcpyAsync(up_ri.data(), A.rowidx, sizeof(int) * A.nnz, hipMemcpyHostToDevice, stream_));
+                PSMAS_HIP(hipMemcpyAsync(d_csc_vals_.data(), A.vals, sizeof(double) * A.nnz, hipMemcpyHostToDevice,
+                                         stream_));
+            }
+            d_cp = up_cp.data();
+            d_ri = up_ri.data();
+            d_v = d_csc_vals_.data();
+            phase("csc_upload");
+        }
+        const long long nnz = dev != nullptr ? dev->nnz : A.nnz;
+        DeviceBuffer<int> d_sob(h_slot_of_block.size(), "MAS block map");
+        PSMAS_HIP(hipMemcpyAsync(d_sob.data(), h_slot_of_block.data(), d_sob.bytes(), hipMemcpyHostToDevice, stream_));
+        SellDevice sd;
+        build_sell_device(S.n, nnz, bd_, d_cp, d_ri, d_v, d_sob.data(), npad_, params_.lazy_partitioning, stream_, sd);
+        nnzb_ = S.nnzb = sd.nnzb;
+        nonisolated_ = std::move(sd.nonisolated);
+        nslices_ = npad_ / WAVE;
+        nslots_ = sd.nslots;
+        h_slice_ptr_ = sd.slice_ptr;
+        d_slice_ptr_ = std::move(sd.d_slice_ptr);
+        d_cols_ = std::move(sd.d_cols);
+        d_vals64_ = std::move(sd.d_vals);
+        d_row_len_ = std::move(sd.d_row_len);
+        d_block_of_slot_.allocate(block_of_slot_.size(), "MAS slot map");
+        PSMAS_HIP(hipMemcpyAsync(d_block_of_slot_.data(), block_of_slot_.data(), d_block_of_slot_.bytes(),
+                                 hipMemcpyHostToDevice, stream_));
+        phase("device_matrix_build");
+        std::vector<int> cols(static_cast<size_t>(nslots_) * WAVE);
+        PSMAS_HIP(hipMemcpyAsync(cols.data(), d_cols_.data(), sizeof(int) * cols.size(), hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        build_halo(h_slice_ptr_, cols);
+        phase("halo");
+        if (params_.lazy_partitioning && sd.map_ok)
+        {
+            d_vmap_ = std::move(sd.d_map);
+            vr_nnz_ = nnz;
+            if (dev == nullptr)
+            {
+                vr_colptr_.assign(A.colptr, A.colptr + A.n + 1);
+                vr_rowidx_.assign(A.rowidx, A.rowidx + A.nnz);
+            }
+        }
+    }
+
+    void Solver::factorize_device(int n, long long nnz, const int *d_colptr, const int *d_rowidx, const double *d_vals)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        validate_csc_device(n, nnz, d_colptr, d_rowidx, stream_);
+        if (nnz > 0 && d_vals == nullptr)
+            throw InvalidArgumentError("[MAS] invalid A: empty matrix or null arrays.");
+        // The host needs the pattern only for a graph aggregation (no coordinates).
+        CscView A{n, nnz, nullptr, nullptr, nullptr};
+        std::vector<int> hc, hr;
+        std::vector<double> hv;
+        const int nb = div_up(n, params_.block_dim);
+        System *S0 = sys_.empty() ? nullptr : &sys_[0];
+        const bool reuse = params_.lazy_partitioning && S0 != nullptr && sys_.size() == 1 && S0->have_partition &&
+                           static_cast<int>(S0->part.perm.size()) == nb && params_.block_dim == bd_;
+        const bool have_coords = S0 != nullptr && S0->coord_dim > 0 &&
+                                 static_cast<int>(S0->coords.size()) == nb * S0->coord_dim;
+        if (!reuse && (params_.aggregation == Aggregation::Graph ||
+                       (params_.aggregation == Aggregation::Auto && !have_coords)))
+        {
+            hc.resize(static_cast<size_t>(n) + 1);
+            hr.resize(static_cast<size_t>(nnz));
+            hv.resize(static_cast<size_t>(nnz));
+            PSMAS_HIP(hipMemcpy(hc.data(), d_colptr, sizeof(int) * hc.size(), hipMemcpyDeviceToHost));
+            if (nnz > 0)
+            {
+                PSMAS_HIP(hipMemcpy(hr.data(), d_rowidx, sizeof(int) * hr.size(), hipMemcpyDeviceToHost));
+                PSMAS_HIP(hipMemcpy(hv.data(), d_vals, sizeof(double) * hv.size(), hipMemcpyDeviceToHost));
+            }
+            A.colptr = hc.data();
+            A.rowidx = hr.data();
+            A.vals = hv.data();
+        }
+        const DeviceCsc dv{n, nnz, d_colptr, d_rowidx, d_vals};
+        factorize_full(std::vector<CscView>{A}, nullptr, nullptr, &dv);
     }
 
     bool Solver::try_refresh_values(const CscView &A, const std::vector<int> *dirichlet_nodes)
@@ -557,8 +704,7 @@ namespace psmas
 
     void Solver::scatter_new_values(const double *d_vals)
     {
-        launch_scatter_values(static_cast<long long>(vr_rowidx_.size()), d_vmap_.data(), d_vals, d_vals64_.data(),
-                              stream_);
+        launch_scatter_values(vr_nnz_, d_vmap_.data(), d_vals, d_vals64_.data(), stream_);
         if (dirichlet_)
             launch_gather_values(static_cast<int>(d_cval_.size()), d_cidx_.data(), d_vals, d_cval_.data(), stream_);
     }
@@ -655,8 +801,7 @@ namespace psmas
     void Solver::refresh_values_device(long long nnz, const double *d_vals)
     {
         PSMAS_HIP(hipSetDevice(device_));
-        if (!factorized_ || sys_.size() != 1 || !d_vmap_ || d_vals == nullptr ||
-            nnz != static_cast<long long>(vr_rowidx_.size()))
+        if (!factorized_ || sys_.size() != 1 || !d_vmap_ || d_vals == nullptr || nnz != vr_nnz_)
             throw InvalidArgumentError(
                 "[MAS] device values need a recorded pattern: a full factorisation of one system with "
                 "lazy_partitioning, no duplicate entries, and the same number of non-zeros.");

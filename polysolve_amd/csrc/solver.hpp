@@ -43,6 +43,16 @@ namespace psmas
         double bytes_per_launch = 0.0; ///< algorithmic bytes
     };
 
+    /// CSC arrays in device memory.
+    struct DeviceCsc
+    {
+        int n = 0;
+        long long nnz = 0;
+        const int *colptr = nullptr;
+        const int *rowidx = nullptr;
+        const double *vals = nullptr;
+    };
+
     class Solver
     {
     public:
@@ -96,6 +106,9 @@ namespace psmas
         /// Values-only refactorisation from device memory (values in the recorded CSC order of the
         /// last full factorisation with lazy_partitioning).
         void refresh_values_device(long long nnz, const double *d_vals);
+        /// Full factorisation of one system from CSC arrays in device memory (the matrix is built
+        /// on the GPU; the host sees the pattern only for a graph aggregation).
+        void factorize_device(int n, long long nnz, const int *d_colptr, const int *d_rowidx, const double *d_vals);
 
         /// Dirichlet elimination on the factorisation (prefactorize, FEMSolver.cpp:269-315): factorise
         /// the matrix with the rows and columns of the fixed dofs replaced by identity; remembers A's
@@ -191,11 +204,13 @@ namespace psmas
 
         // Values-only refactorisation: the last full factorisation's pattern and the position of
         // each of its non-zeros in the SELL value array.
-        std::vector<int> vr_colptr_, vr_rowidx_;
+        std::vector<int> vr_colptr_, vr_rowidx_; ///< host pattern (empty after a device-input factorisation)
+        long long vr_nnz_ = -1;                   ///< non-zeros the value map covers (-1: none)
         std::vector<int> h_slice_ptr_;
         DeviceBuffer<int> d_vmap_;
         DeviceBuffer<double> d_csc_vals_;
         const char *factorize_mode_ = "full";
+        std::vector<std::pair<std::string, double>> setup_ms_; ///< phases of the last full factorisation
 
         // Dirichlet elimination: the fixed dofs (sorted, unique) and A's couplings a_ij (i free,
         // j fixed) by row, for the right-hand side lift.
@@ -269,7 +284,14 @@ namespace psmas
         bool try_refresh_values(const CscView &A, const std::vector<int> *dirichlet_nodes);
         /// Full factorisation; for a Dirichlet one, As[0] is the eliminated matrix, raw the matrix
         /// before elimination (whose non-zeros the value map then indexes) and dir the elimination.
-        void factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir);
+        void factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
+                            const DeviceCsc *dev = nullptr);
+        /// The SELL matrix of one system built on the GPU (sell_build.hpp) from A (uploaded) or
+        /// from the caller's device arrays dev; also the halo lists and, with lazy_partitioning,
+        /// the value map.  phase(name) closes a timed setup phase.
+        template <typename Phase>
+        void device_matrix_build(const CscView &A, const DeviceCsc *dev, const std::vector<int> &h_slot_of_block,
+                                 Phase &phase);
         /// New SELL values (and coupling values of a Dirichlet factorisation) from d_vals in the
         /// recorded CSC order.
         void scatter_new_values(const double *d_vals);

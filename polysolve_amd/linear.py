@@ -191,6 +191,13 @@ class MASSolver(Solver):
         _check(N.lib().psmas_dirichlet_solve_device(self._h, int(n), C.c_void_p(f_ptr), C.c_void_p(u_ptr)),
                self._h)
 
+    def factorize_device(self, colptr_ptr: int, rowidx_ptr: int, vals_ptr: int, n: int, nnz: int) -> None:
+        """Full factorisation from CSC arrays already in device memory (raw HIP pointers: int32
+        colptr of n+1, int32 rowidx and float64 vals of nnz entries)."""
+        _check(N.lib().psmas_factorize_csc_device(self._h, int(n), int(nnz), C.c_void_p(colptr_ptr),
+                                                  C.c_void_p(rowidx_ptr), C.c_void_p(vals_ptr)), self._h)
+        self._n = int(n)
+
     def factorize_values_device(self, vals_ptr: int, nnz: int) -> None:
         """Refactorise from nnz values already in device memory (raw HIP pointer), in the CSC order
         of the last full factorisation, which must have used lazy_partitioning."""
@@ -391,28 +398,31 @@ def synchronize() -> None:
 
 
 class DeviceArray:
-    """A float64 vector in device memory owned through the C ABI (psmas_device_alloc).
+    """A vector in device memory (float64 unless ``dtype`` says otherwise) owned through the C ABI
+    (psmas_device_alloc).
 
-    For callers without their own GPU allocator: ``solve_device`` takes ``.ptr``.
+    For callers without their own GPU allocator: ``solve_device`` and ``factorize_device`` take
+    ``.ptr``.
     """
 
-    def __init__(self, n: int, host: np.ndarray | None = None) -> None:
+    def __init__(self, n: int, host: np.ndarray | None = None, dtype=np.float64) -> None:
         self.n = int(n)
+        self.dtype = np.dtype(dtype)
         p = C.c_void_p()
-        _check(N.lib().psmas_device_alloc(max(1, self.n) * 8, C.byref(p)), None)
+        _check(N.lib().psmas_device_alloc(max(1, self.n) * self.dtype.itemsize, C.byref(p)), None)
         self.ptr = int(p.value)
         if host is not None:
             self.upload(host)
 
     def upload(self, host) -> None:
-        h = np.ascontiguousarray(host, dtype=np.float64)
+        h = np.ascontiguousarray(host, dtype=self.dtype)
         if h.size != self.n:
             raise MASError("[MAS] invalid host array size")
-        _check(N.lib().psmas_memcpy(C.c_void_p(self.ptr), h.ctypes.data_as(C.c_void_p), self.n * 8, 1), None)
+        _check(N.lib().psmas_memcpy(C.c_void_p(self.ptr), h.ctypes.data_as(C.c_void_p), h.nbytes, 1), None)
 
     def download(self) -> np.ndarray:
-        out = np.empty(self.n, dtype=np.float64)
-        _check(N.lib().psmas_memcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.n * 8, 2), None)
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(N.lib().psmas_memcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes, 2), None)
         return out
 
     def free(self) -> None:
