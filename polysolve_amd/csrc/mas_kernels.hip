@@ -411,14 +411,25 @@ namespace psmas
             for (int i = tid; i < N; i += BLOCK)
                 sF[i * LDW + i] = __float2half_rn(1.0f); // never overwritten below
 
-            auto load = [&](u32x4(&reg)[PER], float &xr, int mm) {
+            // A matrix in flight: its tile in registers, and for thread tid < N (the one that also
+            // stores row tid) the scaled input x_tid, the row's scale and whether the row's output
+            // is stored, so that nothing global is read between the barriers.
+            struct Row
+            {
+                float x = 0.0f, d = 0.0f;
+                bool store = false;
+            };
+            auto load = [&](u32x4(&reg)[PER], Row &rw, int mm) {
                 const u32x4 *src = reinterpret_cast<const u32x4 *>(mats + static_cast<size_t>(mm) * LEN);
 #pragma unroll
                 for (int k = 0; k < PER; ++k)
                     reg[k] = __builtin_nontemporal_load(src + tid + k * BLOCK);
                 if (tid < N)
-                    xr = coarse_input<BD, float>(sd, mr, mm * 32 + tid / BD, tid % BD) *
-                         dscale[static_cast<size_t>(mm) * N + tid];
+                {
+                    rw.d = dscale[static_cast<size_t>(mm) * N + tid];
+                    rw.x = coarse_input<BD, float>(sd, mr, mm * 32 + tid / BD, tid % BD) * rw.d;
+                    rw.store = !dense_owned(sd, mm * 32 + tid / BD);
+                }
             };
             auto unpack = [&](const u32x4(&reg)[PER], float xr) {
 #pragma unroll
@@ -464,7 +475,7 @@ namespace psmas
                 __syncthreads();
             };
             const int row = tid % N, hf = tid / N;
-            auto compute = [&](int mm) {
+            auto compute = [&](int mm, const Row &rw) {
                 const __half *arow = sF + row * LDW + hf * HALF;
                 const float *xs = sx + hf * HALF;
                 float acc = 0.0f;
@@ -488,34 +499,36 @@ namespace psmas
                 if (hf == 1)
                     part[row] = acc;
                 __syncthreads(); // also: sF and sx may be rewritten after this point
-                if (hf == 0 && !dense_owned(sd, mm * 32 + row / BD))
-                    mz[static_cast<size_t>(mm) * N + row] = (acc + part[row]) * dscale[static_cast<size_t>(mm) * N + row];
+                if (hf == 0 && rw.store) // (hf == 0: row == tid)
+                    mz[static_cast<size_t>(mm) * N + row] = (acc + part[row]) * rw.d;
                 // part[] is next written after the following unpack's barrier
             };
 
             // Invariant: regA holds matrix cur, regB matrix nxt (when nxt < end).
             u32x4 regA[PER], regB[PER];
-            float xa = 0.0f, xb = 0.0f;
+            Row ra, rb;
             int nxt = step(cur);
-            load(regA, xa, cur);
+            load(regA, ra, cur);
             if (nxt < end)
-                load(regB, xb, nxt);
+                load(regB, rb, nxt);
             while (true)
             {
                 int nn = nxt < end ? step(nxt) : end;
-                unpack(regA, xa);
+                unpack(regA, ra.x);
+                Row now = ra;
                 if (nn < end)
-                    load(regA, xa, nn);
-                compute(cur);
+                    load(regA, ra, nn);
+                compute(cur, now);
                 cur = nxt;
                 nxt = nn;
                 if (cur >= end)
                     break;
                 nn = nxt < end ? step(nxt) : end;
-                unpack(regB, xb);
+                unpack(regB, rb.x);
+                now = rb;
                 if (nn < end)
-                    load(regB, xb, nn);
-                compute(cur);
+                    load(regB, rb, nn);
+                compute(cur, now);
                 cur = nxt;
                 nxt = nn;
                 if (cur >= end)
