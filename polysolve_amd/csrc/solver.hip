@@ -23,6 +23,7 @@
 // residual only.
 #include "solver.hpp"
 #include "sell_build.hpp"
+#include "aggregate.hpp"
 #include "coarse_solve.hpp"
 #include "device.hpp"
 
@@ -421,6 +422,28 @@ namespace psmas
         // (uploaded here, or the caller's device arrays).
         const bool gpu_build = K == 1 && dir == nullptr && (dev != nullptr || params_.device_setup);
         vr_nnz_ = -1;
+        // The CSC arrays in device memory for the device aggregation and build (host input is
+        // uploaded once here).
+        DeviceCsc dcsc;
+        DeviceBuffer<int> up_cp, up_ri;
+        if (gpu_build && dev != nullptr)
+            dcsc = *dev;
+        else if (gpu_build)
+        {
+            const CscView &A = As[0];
+            up_cp.allocate(static_cast<size_t>(A.n) + 1, "MAS input pattern");
+            up_ri.allocate(static_cast<size_t>(std::max<long long>(A.nnz, 1)), "MAS input pattern");
+            d_csc_vals_.ensure(static_cast<size_t>(std::max<long long>(A.nnz, 1)), "MAS input values");
+            PSMAS_HIP(hipMemcpyAsync(up_cp.data(), A.colptr, sizeof(int) * (A.n + 1), hipMemcpyHostToDevice, stream_));
+            if (A.nnz > 0)
+            {
+                PSMAS_HIP(hipMemcpyAsync(up_ri.data(), A.rowidx, sizeof(int) * A.nnz, hipMemcpyHostToDevice, stream_));
+                PSMAS_HIP(hipMemcpyAsync(d_csc_vals_.data(), A.vals, sizeof(double) * A.nnz, hipMemcpyHostToDevice,
+                                         stream_));
+            }
+            dcsc = DeviceCsc{A.n, A.nnz, up_cp.data(), up_ri.data(), d_csc_vals_.data()};
+            phase("csc_upload");
+        }
 
         // Per-system partition and bank-padded BSR, concatenated.
         PaddedBsr all;
@@ -444,7 +467,7 @@ namespace psmas
             S.n = A.n;
             S.nb = nb;
             if (rebuild)
-                build_partition(S, A);
+                build_partition(S, A, gpu_build ? &dcsc : nullptr);
             if (K == 1)
                 phase("partition");
             PaddedBsr bsr;
@@ -491,7 +514,7 @@ namespace psmas
 
         if (gpu_build)
         {
-            device_matrix_build(As[0], dev, h_slot_of_block, phase);
+            device_matrix_build(As[0], dcsc, dev != nullptr, h_slot_of_block, phase);
         }
         else
         {
@@ -582,37 +605,13 @@ namespace psmas
     }
 
     template <typename Phase>
-    void Solver::device_matrix_build(const CscView &A, const DeviceCsc *dev, const std::vector<int> &h_slot_of_block,
-                                     Phase &phase)
+    void Solver::device_matrix_build(const CscView &A, const DeviceCsc &dcsc, bool device_input,
+                                     const std::vector<int> &h_slot_of_block, Phase &phase)
     {
         System &S = sys_[0];
-        const int *d_cp = nullptr, *d_ri = nullptr;
-        const double *d_v = nullptr;
-        DeviceBuffer<int> up_cp, up_ri;
-        if (dev != nullptr)
-        {
-            d_cp = dev->colptr;
-            d_ri = dev->rowidx;
-            d_v = dev->vals;
-        }
-        else
-        {
-            up_cp.allocate(static_cast<size_t>(A.n) + 1, "MAS input pattern");
-            up_ri.allocate(static_cast<size_t>(std::max<long long>(A.nnz, 1)), "MAS input pattern");
-            d_csc_vals_.ensure(static_cast<size_t>(std::max<long long>(A.nnz, 1)), "MAS input values");
-            PSMAS_HIP(hipMemcpyAsync(up_cp.data(), A.colptr, sizeof(int) * (A.n + 1), hipMemcpyHostToDevice, stream_));
-            if (A.nnz > 0)
-            {
-                PSMAS_HIP(hipMemcpyAsync(up_ri.data(), A.rowidx, sizeof(int) * A.nnz, hipMemcpyHostToDevice, stream_));
-                PSMAS_HIP(hipMemcpyAsync(d_csc_vals_.data(), A.vals, sizeof(double) * A.nnz, hipMemcpyHostToDevice,
-                                         stream_));
-            }
-            d_cp = up_cp.data();
-            d_ri = up_ri.data();
-            d_v = d_csc_vals_.data();
-            phase("csc_upload");
-        }
-        const long long nnz = dev != nullptr ? dev->nnz : A.nnz;
+        const int *d_cp = dcsc.colptr, *d_ri = dcsc.rowidx;
+        const double *d_v = dcsc.vals;
+        const long long nnz = dcsc.nnz;
         DeviceBuffer<int> d_sob(h_slot_of_block.size(), "MAS block map");
         PSMAS_HIP(hipMemcpyAsync(d_sob.data(), h_slot_of_block.data(), d_sob.bytes(), hipMemcpyHostToDevice, stream_));
         SellDevice sd;
@@ -639,7 +638,7 @@ namespace psmas
         {
             d_vmap_ = std::move(sd.d_map);
             vr_nnz_ = nnz;
-            if (dev == nullptr)
+            if (!device_input)
             {
                 vr_colptr_.assign(A.colptr, A.colptr + A.n + 1);
                 vr_rowidx_.assign(A.rowidx, A.rowidx + A.nnz);
@@ -653,32 +652,8 @@ namespace psmas
         validate_csc_device(n, nnz, d_colptr, d_rowidx, stream_);
         if (nnz > 0 && d_vals == nullptr)
             throw InvalidArgumentError("[MAS] invalid A: empty matrix or null arrays.");
-        // The host needs the pattern only for a graph aggregation (no coordinates).
-        CscView A{n, nnz, nullptr, nullptr, nullptr};
-        std::vector<int> hc, hr;
-        std::vector<double> hv;
-        const int nb = div_up(n, params_.block_dim);
-        System *S0 = sys_.empty() ? nullptr : &sys_[0];
-        const bool reuse = params_.lazy_partitioning && S0 != nullptr && sys_.size() == 1 && S0->have_partition &&
-                           static_cast<int>(S0->part.perm.size()) == nb && params_.block_dim == bd_;
-        const bool have_coords = S0 != nullptr && S0->coord_dim > 0 &&
-                                 static_cast<int>(S0->coords.size()) == nb * S0->coord_dim;
-        if (!reuse && (params_.aggregation == Aggregation::Graph ||
-                       (params_.aggregation == Aggregation::Auto && !have_coords)))
-        {
-            hc.resize(static_cast<size_t>(n) + 1);
-            hr.resize(static_cast<size_t>(nnz));
-            hv.resize(static_cast<size_t>(nnz));
-            PSMAS_HIP(hipMemcpy(hc.data(), d_colptr, sizeof(int) * hc.size(), hipMemcpyDeviceToHost));
-            if (nnz > 0)
-            {
-                PSMAS_HIP(hipMemcpy(hr.data(), d_rowidx, sizeof(int) * hr.size(), hipMemcpyDeviceToHost));
-                PSMAS_HIP(hipMemcpy(hv.data(), d_vals, sizeof(double) * hv.size(), hipMemcpyDeviceToHost));
-            }
-            A.colptr = hc.data();
-            A.rowidx = hr.data();
-            A.vals = hv.data();
-        }
+        // The host never sees the arrays: domains come from coordinates or the device aggregation.
+        const CscView A{n, nnz, nullptr, nullptr, nullptr};
         const DeviceCsc dv{n, nnz, d_colptr, d_rowidx, d_vals};
         factorize_full(std::vector<CscView>{A}, nullptr, nullptr, &dv);
     }
@@ -833,7 +808,7 @@ namespace psmas
         factorized_ = true;
     }
 
-    void Solver::build_partition(System &S, const CscView &A)
+    void Solver::build_partition(System &S, const CscView &A, const DeviceCsc *dcsc)
     {
         const int nb = S.nb;
         Aggregation agg = params_.aggregation;
@@ -874,6 +849,11 @@ namespace psmas
             PSMAS_HIP(hipStreamSynchronize(stream_));
             S.part = chunk_partition(order, BANK);
             S.aggregation = "morton";
+        }
+        else if (dcsc != nullptr)
+        {
+            S.part = gpu_aggregate(dcsc->n, dcsc->nnz, bd_, dcsc->colptr, dcsc->rowidx, dcsc->vals, BANK, stream_);
+            S.aggregation = "graph_gpu";
         }
         else
         {

@@ -267,7 +267,9 @@ namespace psmas
         template <typename TV>
         SellView<TV> sell(const TV *vals) const;
 
-        void build_partition(System &S, const CscView &A);
+        /// Domains of system S: Morton chunks (coordinates), else the graph aggregation on the
+        /// GPU when dcsc (A in device memory) is given, else on the host.
+        void build_partition(System &S, const CscView &A, const DeviceCsc *dcsc = nullptr);
         void upload_matrix(const PaddedBsr &bsr);
         /// Per node block, the distinct columns of its SELL slots and the 16-bit local columns.
         void build_halo(const std::vector<int> &slice_ptr, const std::vector<int> &cols);
@@ -286,12 +288,13 @@ namespace psmas
         /// before elimination (whose non-zeros the value map then indexes) and dir the elimination.
         void factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
                             const DeviceCsc *dev = nullptr);
-        /// The SELL matrix of one system built on the GPU (sell_build.hpp) from A (uploaded) or
-        /// from the caller's device arrays dev; also the halo lists and, with lazy_partitioning,
-        /// the value map.  phase(name) closes a timed setup phase.
+        /// The SELL matrix of one system built on the GPU (sell_build.hpp) from the CSC arrays in
+        /// device memory (A uploaded, or the caller's arrays when device_input); also the halo
+        /// lists and, with lazy_partitioning, the value map (and A's host pattern for host input).
+        /// phase(name) closes a timed setup phase.
         template <typename Phase>
-        void device_matrix_build(const CscView &A, const DeviceCsc *dev, const std::vector<int> &h_slot_of_block,
-                                 Phase &phase);
+        void device_matrix_build(const CscView &A, const DeviceCsc &dcsc, bool device_input,
+                                 const std::vector<int> &h_slot_of_block, Phase &phase);
         /// New SELL values (and coupling values of a Dirichlet factorisation) from d_vals in the
         /// recorded CSC order.
         void scatter_new_values(const double *d_vals);
