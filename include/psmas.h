@@ -111,9 +111,10 @@ int psmas_dirichlet_solve_device(psmas_handle h, int n, double *d_f, double *d_x
 int psmas_solve_device(psmas_handle h, int n, const double *d_b, double *d_x);
 
 /* Same as psmas_factorize_csc with the CSC arrays in device memory (HIP pointers on the solver's
- * device): d_colptr n+1, d_rowidx and d_vals nnz entries.  The matrix is built on the GPU; the
- * arrays are validated there first (the same errors as psmas_factorize_csc).  The host copies
- * the pattern back only for a graph aggregation (no coordinates set).  With "lazy_partitioning"
+ * device): d_colptr n+1, d_rowidx and d_vals nnz entries.  The arrays are validated on the GPU
+ * first (the same errors as psmas_factorize_csc); the domains (Morton chunks with coordinates,
+ * the GPU graph aggregation without) and the matrix are built there, and the host never copies
+ * A back.  With "lazy_partitioning"
  * the values of later Newton steps can follow through psmas_factorize_values_device.  Replaces
  * the reference's two host-to-device uploads of the CSC arrays (BSRAdjacency.cu:180-197,
  * BSRMatrix.cu:214-231) for callers that assemble on the GPU. */
