@@ -132,6 +132,17 @@ int psmas_factorize_csc_device(psmas_handle h, int n, long long nnz, const int *
  * (MASSolver.cu:304-321), which has no device-value entry. */
 int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d_vals);
 
+/* Several right-hand sides of one matrix (no counterpart in the reference; for Newton or IPC
+ * callers that solve one Hessian for several vectors): factorise nrhs (1..8) copies of A as a
+ * batch whose SpMV reads the matrix once for all of them, then solve all right-hand sides in one
+ * launch sequence, each with its own CG scalars, convergence test and status
+ * (psmas_batch_get_info(h, c, ...) for copy c).  B and X hold nrhs vectors of length n one after
+ * the other (column-major n x nrhs), host arrays or device pointers (_device). */
+int psmas_factorize_multi_csc(psmas_handle h, int nrhs, int n, long long nnz, const int *colptr, const int *rowidx,
+                              const double *vals);
+int psmas_solve_multi(psmas_handle h, int n, int nrhs, const double *B, double *X);
+int psmas_solve_multi_device(psmas_handle h, int n, int nrhs, const double *d_B, double *d_X);
+
 /* Node coordinates for Morton-ordered aggregation into 32-node domains: nb tuples of dim
  * (1..3) doubles, one per block node (block_dim scalars).  Takes effect at the next factorize. */
 int psmas_set_coordinates(psmas_handle h, int nb, int dim, const double *xyz);

@@ -193,6 +193,53 @@ extern "C"
         return guarded(h->last_error, [&] { h->solver.factorize_device(n, nnz, d_colptr, d_rowidx, d_vals); });
     }
 
+    int psmas_factorize_multi_csc(psmas_handle h, int nrhs, int n, long long nnz, const int *colptr, const int *rowidx,
+                                  const double *vals)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            psmas::CscView A;
+            A.n = n;
+            A.nnz = nnz;
+            A.colptr = colptr;
+            A.rowidx = rowidx;
+            A.vals = vals;
+            h->solver.factorize_copies(A, nrhs);
+        });
+    }
+
+    static int solve_multi(psmas_handle h, int n, int nrhs, const double *B, double *X, bool device)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            if (nrhs != h->solver.num_systems() || B == nullptr || X == nullptr)
+                throw psmas::InvalidArgumentError(
+                    "[MAS] solve_multi: nrhs must equal the right-hand sides of the last factorize_multi.");
+            std::vector<const double *> b(nrhs);
+            std::vector<double *> x(nrhs);
+            for (int c = 0; c < nrhs; ++c)
+            {
+                b[c] = B + static_cast<size_t>(c) * n;
+                x[c] = X + static_cast<size_t>(c) * n;
+            }
+            if (n != h->solver.dim())
+                throw psmas::InvalidArgumentError("[MAS] Size mismatch. Did you forget to call factorize?");
+            h->solver.solve_batch(b, x, device);
+        });
+    }
+
+    int psmas_solve_multi(psmas_handle h, int n, int nrhs, const double *B, double *X)
+    {
+        return solve_multi(h, n, nrhs, B, X, false);
+    }
+
+    int psmas_solve_multi_device(psmas_handle h, int n, int nrhs, const double *d_B, double *d_X)
+    {
+        return solve_multi(h, n, nrhs, d_B, d_X, true);
+    }
+
     int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d_vals)
     {
         if (!h)

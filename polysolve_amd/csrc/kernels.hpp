@@ -131,6 +131,11 @@ namespace psmas
     /// Plain y = A x without reductions (introspection path).
     template <int BD, typename TV, typename TX>
     void launch_spmv_plain(const SellView<TV> &A, const TX *x, TX *y, hipStream_t s);
+    /// launch_spmv_ap for a batch of `copies` (<= 8) systems that repeat one matrix of npad1 slots:
+    /// each block of copy 0's slots is read once for all copies.
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_ap_copies(const SellView<TV> &A, int copies, int npad1, const TX *x, TX *y, const Reductions &red,
+                               const SysMap &sm, hipStream_t s);
 
     // ----- vectors (node-blocked: nblocks workgroups of NODE_BLOCK nodes) ------------------
     /// alpha = rz / pap; x += alpha p; r -= alpha Ap; partials of r . r -> red.rr; restrict r -> rc0;

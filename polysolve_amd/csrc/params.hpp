@@ -28,6 +28,8 @@
 //                        correction is switched off for the rest of the solve
 //   learned_output_scale factor on the learned correction (1 = as trained)
 //   spmv_unroll          SpMV row-loop batch depth 2 | 4 | 8 (chosen by psmas_autotune)
+//   spmv_copies          after factorize_copies, one SpMV launch reads the matrix once for every
+//                        right-hand side (default true; false: the batch's per-system SpMV)
 //   device_setup         build the SELL matrix of a single system on the GPU from the CSC arrays
 //                        (default true; false: the host build)
 //   spmv_halo            SpMV gathers x from a per-workgroup copy of the entries its rows read,
@@ -92,6 +94,7 @@ namespace psmas
         int spmv_unroll = 4;
         bool spmv_halo = true;
         bool device_setup = true;
+        bool spmv_copies = true;
         int restrict_piece = 512;
     };
 

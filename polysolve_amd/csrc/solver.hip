@@ -140,7 +140,7 @@ namespace psmas
         if (before.use_graphs != params_.use_graphs || before.check_interval != params_.check_interval ||
             before.true_residual_period != params_.true_residual_period || before.precision != params_.precision ||
             before.coarse_solver != params_.coarse_solver || before.spmv_unroll != params_.spmv_unroll ||
-            before.spmv_halo != params_.spmv_halo)
+            before.spmv_halo != params_.spmv_halo || before.spmv_copies != params_.spmv_copies)
             graphs_->clear();
     }
 
@@ -256,6 +256,7 @@ namespace psmas
           << ", \"systems\": " << sys_.size() << ", \"aggregation\": \"" << S.aggregation << "\""
           << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\""
           << ", \"factorize_mode\": \"" << factorize_mode_ << "\""
+          << ", \"copies\": " << copies_
           << ", \"setup_ms\": {" << [&] {
                  std::ostringstream q;
                  q.precision(6);
@@ -397,6 +398,7 @@ namespace psmas
         const int bd = params_.block_dim;
         factorized_ = false;
         factorize_mode_ = "full";
+        copies_ = 1;
         setup_ms_.clear();
         auto t_mark = std::chrono::steady_clock::now();
         auto phase = [&](const char *name) {
@@ -656,6 +658,30 @@ namespace psmas
         const CscView A{n, nnz, nullptr, nullptr, nullptr};
         const DeviceCsc dv{n, nnz, d_colptr, d_rowidx, d_vals};
         factorize_full(std::vector<CscView>{A}, nullptr, nullptr, &dv);
+    }
+
+    void Solver::factorize_copies(const CscView &A, int copies)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        if (copies < 1 || copies > MAX_COPIES)
+            throw InvalidArgumentError("[MAS] invalid number of right-hand sides (1 .. 8).");
+        validate_csc(A);
+        // System 0's coordinates (if any) go to every copy, so that all copies get one partition.
+        if (!sys_.empty() && sys_[0].coord_dim > 0)
+        {
+            const System S0 = sys_[0];
+            sys_.resize(copies);
+            for (int c = 1; c < copies; ++c)
+            {
+                sys_[c].coords = S0.coords;
+                sys_[c].coord_dim = S0.coord_dim;
+            }
+        }
+        factorize_full(std::vector<CscView>(copies, A), nullptr, nullptr);
+        for (int c = 1; c < copies; ++c)
+            if (sys_[c].npad != sys_[0].npad || sys_[c].part.perm != sys_[0].part.perm)
+                return; // (not expected: the copies keep their own SpMV then)
+        copies_ = copies;
     }
 
     bool Solver::try_refresh_values(const CscView &A, const std::vector<int> *dirichlet_nodes)
@@ -1527,7 +1553,11 @@ namespace psmas
         const SellView<double> A = sell(d_vals64_.data());
         const double bytes = spmv_bytes(8, 8, 2);
         tic("spmv_fp64", bytes);
-        launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), reductions(), sysmap(), stream_);
+        if (copies_ > 1 && params_.spmv_copies)
+            launch_spmv_ap_copies<BD, double, double>(A, copies_, sys_[0].npad, d_p_.data(), d_Ap_.data(), reductions(),
+                                                      sysmap(), stream_);
+        else
+            launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), reductions(), sysmap(), stream_);
         toc();
         if (k % params_.true_residual_period == 0)
         {
@@ -1639,7 +1669,11 @@ namespace psmas
     {
         const SellView<float> A = sell(d_vals32_.data());
         tic("spmv_fp32", spmv_bytes(4, 4, 2));
-        launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), reductions(), sysmap(), stream_);
+        if (copies_ > 1 && params_.spmv_copies)
+            launch_spmv_ap_copies<BD, float, float>(A, copies_, sys_[0].npad, d_p32_.data(), d_Ap32_.data(), reductions(),
+                                                    sysmap(), stream_);
+        else
+            launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), reductions(), sysmap(), stream_);
         toc();
         // With the learned correction on the block path, the update also leaves the lattice block
         // partials of the new r.

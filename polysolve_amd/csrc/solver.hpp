@@ -106,6 +106,11 @@ namespace psmas
         /// Values-only refactorisation from device memory (values in the recorded CSC order of the
         /// last full factorisation with lazy_partitioning).
         void refresh_values_device(long long nnz, const double *d_vals);
+        /// Several right-hand sides of one matrix: a batch of `copies` (<= MAX_COPIES) systems
+        /// that repeat A, whose SpMV reads the matrix once for all of them.  solve_batch then
+        /// takes the right-hand sides.
+        void factorize_copies(const CscView &A, int copies);
+        static constexpr int MAX_COPIES = 8;
         /// Full factorisation of one system from CSC arrays in device memory (the matrix is built
         /// on the GPU; the host sees the pattern only for a graph aggregation).
         void factorize_device(int n, long long nnz, const int *d_colptr, const int *d_rowidx, const double *d_vals);
@@ -210,6 +215,7 @@ namespace psmas
         DeviceBuffer<int> d_vmap_;
         DeviceBuffer<double> d_csc_vals_;
         const char *factorize_mode_ = "full";
+        int copies_ = 1; ///< > 1: the batch repeats one matrix (factorize_copies)
         std::vector<std::pair<std::string, double>> setup_ms_; ///< phases of the last full factorisation
 
         // Dirichlet elimination: the fixed dofs (sorted, unique) and A's couplings a_ij (i free,

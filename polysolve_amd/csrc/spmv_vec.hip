@@ -174,6 +174,95 @@ namespace psmas
             store_partial<SPMV_BLOCK>(dot, red.pap, blk);
         }
 
+        /// y_c = A x_c for the `copies` systems of a batch that repeats one matrix: system c's node
+        /// block b is block c * nblocks1 + b and its slots are copy 0's with c * npad1 added to the
+        /// columns.  Workgroup b reads each block of copy 0's slots once and applies it to every
+        /// copy's x (KC >= copies accumulators per row); copy c's pAp partial goes to its block.
+        /// Per copy the FMAs run in the same order as in k_spmv_ap.
+        template <int BD, int KC, typename TV, typename TX>
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap_copies(SellView<TV> A, int copies, int npad1,
+                                                                       int nblocks1, const TX *__restrict__ x,
+                                                                       TX *__restrict__ y, Reductions red, SysMap sm)
+        {
+            using TA = typename Acc<TV>::type;
+            constexpr int U = 2;
+            const int b = xcd_block();
+            const int lane = threadIdx.x % WAVE;
+            const int slice = b * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            bool act[KC];
+#pragma unroll
+            for (int c = 0; c < KC; ++c)
+                act[c] = c < copies && sm.active[sm.block_sys[c * nblocks1 + b]] != 0;
+            TA acc[KC][BD];
+#pragma unroll
+            for (int c = 0; c < KC; ++c)
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                    acc[c][i] = TA(0);
+            const int beg = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice]);
+            const int end = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice + 1]);
+            const int last = end - 1;
+            const int *__restrict__ cols = A.cols + lane;
+            const TV *__restrict__ vals = A.vals + lane;
+            for (int k = beg; k < end; k += U)
+            {
+                int col[U];
+                TV v[U][BD * BD];
+                TA xv[U][KC][BD];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    const size_t slot = static_cast<size_t>(min(k + u, last));
+                    col[u] = ldg_stream(cols + slot * WAVE);
+#pragma unroll
+                    for (int e = 0; e < BD * BD; ++e)
+                        v[u][e] = ldg_stream(vals + (slot * (BD * BD) + e) * WAVE);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int c = 0; c < KC; ++c)
+#pragma unroll
+                        for (int j = 0; j < BD; ++j)
+                            xv[u][c][j] = c < copies
+                                              ? static_cast<TA>(x[(static_cast<size_t>(c) * npad1 + col[u]) * BD + j])
+                                              : TA(0);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    const bool ok = k + u < end; // wave-uniform
+#pragma unroll
+                    for (int c = 0; c < KC; ++c)
+#pragma unroll
+                        for (int i = 0; i < BD; ++i)
+#pragma unroll
+                            for (int j = 0; j < BD; ++j)
+                                acc[c][i] = fma(ok ? static_cast<TA>(v[u][i * BD + j]) : TA(0), xv[u][c][j], acc[c][i]);
+                }
+            }
+            const size_t row = static_cast<size_t>(slice) * WAVE + lane;
+#pragma unroll
+            for (int c = 0; c < KC; ++c)
+            {
+                if (c >= copies)
+                    break;
+                double dot = 0.0;
+                const size_t g = static_cast<size_t>(c) * npad1 + row;
+                if (act[c])
+                {
+#pragma unroll
+                    for (int i = 0; i < BD; ++i)
+                    {
+                        const TX yi = static_cast<TX>(acc[c][i]);
+                        y[g * BD + i] = yi;
+                        dot += static_cast<double>(x[g * BD + i]) * static_cast<double>(yi);
+                    }
+                }
+                if (act[c]) // uniform in the workgroup
+                    store_partial<SPMV_BLOCK>(dot, red.pap, c * nblocks1 + b);
+            }
+        }
+
         /// Leader lanes of a bank sum their component's members from LDS into the coarse vector.
         template <int BD, typename T>
         __device__ __forceinline__ void restrict_bank(const T *sv, int bank_base_in_block, uint32_t mask, int gid,
@@ -582,6 +671,22 @@ namespace psmas
                                                            0, s, A, x, y));
         PSMAS_LAUNCH_CHECK();
     }
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_ap_copies(const SellView<TV> &A, int copies, int npad1, const TX *x, TX *y, const Reductions &red,
+                               const SysMap &sm, hipStream_t s)
+    {
+        const int nblocks1 = npad1 / SPMV_BLOCK;
+        if (copies <= 2)
+            hipLaunchKernelGGL((k_spmv_ap_copies<BD, 2, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), 0, s, A, copies,
+                               npad1, nblocks1, x, y, red, sm);
+        else if (copies <= 4)
+            hipLaunchKernelGGL((k_spmv_ap_copies<BD, 4, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), 0, s, A, copies,
+                               npad1, nblocks1, x, y, red, sm);
+        else
+            hipLaunchKernelGGL((k_spmv_ap_copies<BD, 8, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), 0, s, A, copies,
+                               npad1, nblocks1, x, y, red, sm);
+        PSMAS_LAUNCH_CHECK();
+    }
 #undef PSMAS_UNROLL_DISPATCH
 
     namespace
@@ -762,6 +867,11 @@ namespace psmas
     template void launch_spmv_res<BD, double, double, float>(const SellView<double> &, const double *,            \
                                                              const double *, float *, FineRestrict, float *,      \
                                                              double *, const SysMap &, bool, hipStream_t);        \
+    template void launch_spmv_ap_copies<BD, double, double>(const SellView<double> &, int, int, const double *,   \
+                                                            double *, const Reductions &, const SysMap &,         \
+                                                            hipStream_t);                                         \
+    template void launch_spmv_ap_copies<BD, float, float>(const SellView<float> &, int, int, const float *, float *, \
+                                                          const Reductions &, const SysMap &, hipStream_t);       \
     template void launch_spmv_plain<BD, double, double>(const SellView<double> &, const double *, double *,       \
                                                         hipStream_t);                                             \
     template void launch_spmv_plain<BD, float, float>(const SellView<float> &, const float *, float *,            \

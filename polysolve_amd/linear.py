@@ -198,6 +198,29 @@ class MASSolver(Solver):
                                                   C.c_void_p(rowidx_ptr), C.c_void_p(vals_ptr)), self._h)
         self._n = int(n)
 
+    def factorize_multi(self, A, nrhs: int) -> None:
+        """Factorise A for nrhs (1..8) right-hand sides solved together (solve_multi)."""
+        n, colptr, rowidx, vals = _as_csc(A)
+        _check(N.lib().psmas_factorize_multi_csc(self._h, int(nrhs), n, int(rowidx.size), _ptr(colptr, C.c_int),
+                                                 _ptr(rowidx, C.c_int), _ptr(vals, C.c_double)), self._h)
+        self._n = n
+
+    def solve_multi(self, B) -> np.ndarray:
+        """Solve A X = B for the nrhs columns of B (n x nrhs) after factorize_multi."""
+        B = np.asfortranarray(B, dtype=np.float64)
+        if B.ndim != 2:
+            raise MASError("[MAS] B must be n x nrhs")
+        n, k = B.shape
+        X = np.zeros((n, k), dtype=np.float64, order="F")
+        _check(N.lib().psmas_solve_multi(self._h, n, k, B.ctypes.data_as(C.POINTER(C.c_double)),
+                                         X.ctypes.data_as(C.POINTER(C.c_double))), self._h)
+        return X
+
+    def solve_multi_device(self, b_ptr: int, x_ptr: int, n: int, nrhs: int) -> None:
+        """solve_multi with B and X (n x nrhs, column-major) in device memory."""
+        _check(N.lib().psmas_solve_multi_device(self._h, int(n), int(nrhs), C.c_void_p(b_ptr), C.c_void_p(x_ptr)),
+               self._h)
+
     def factorize_values_device(self, vals_ptr: int, nnz: int) -> None:
         """Refactorise from nnz values already in device memory (raw HIP pointer), in the CSC order
         of the last full factorisation, which must have used lazy_partitioning."""

@@ -1,0 +1,64 @@
+"""Several right-hand sides of one matrix (factorize_multi / solve_multi): every column solved to
+tolerance; the copies SpMV (matrix read once for all right-hand sides) gives the iterates of the
+batch's per-system SpMV bit for bit; device arrays give the host result; each column matches a
+solve of that right-hand side alone."""
+import numpy as np
+import pytest
+
+from polysolve_amd import problems
+from polysolve_amd.linear import DeviceArray, MASError, MASSolver
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def het12():
+    return problems.hex_elasticity(12, E=problems.random_moduli(12, 3), name="het12")
+
+
+def _prm(precision):
+    return {"MAS": {"block_dim": 3, "precision": precision, "relative_tolerance": 1e-8, "absolute_tolerance": 0.0,
+                    "coarse_solver": "cholesky"}}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("k", [3, 8])
+def test_multi_rhs(gpu, het12, precision, k):
+    rng = np.random.default_rng(k)
+    B = rng.standard_normal((het12.n, k))
+    B[:, 0] = het12.b
+    s = MASSolver()
+    s.set_parameters(_prm(precision))
+    s.set_coordinates(het12.coords)
+    s.factorize_multi(het12.A, k)
+    assert s.get_info()["copies"] == k
+    X = s.solve_multi(B)
+    for c in range(k):
+        assert s.batch_info(c)["solver_status"] == "Reach relative tolerance"
+        assert np.linalg.norm(B[:, c] - het12.A @ X[:, c]) <= 1e-8 * np.linalg.norm(B[:, c])
+    s.set_parameters({"MAS": {"spmv_copies": False}})
+    assert np.array_equal(s.solve_multi(B), X)
+    s.set_parameters({"MAS": {"spmv_copies": True}})
+    dB, dX = DeviceArray(het12.n * k, B.ravel(order="F")), DeviceArray(het12.n * k)
+    s.solve_multi_device(dB.ptr, dX.ptr, het12.n, k)
+    assert np.array_equal(dX.download().reshape((het12.n, k), order="F"), X)
+    dB.free()
+    dX.free()
+    one = MASSolver()
+    one.set_parameters(_prm(precision))
+    one.set_coordinates(het12.coords)
+    one.factorize(het12.A)
+    for c in (0, k - 1):
+        x1 = np.zeros(het12.n)
+        one.solve(np.ascontiguousarray(B[:, c]), x1)
+        assert np.linalg.norm(X[:, c] - x1) <= 1e-6 * np.linalg.norm(x1)
+
+
+def test_multi_rhs_errors(gpu, het12):
+    s = MASSolver()
+    s.set_parameters(_prm("mixed"))
+    with pytest.raises(MASError, match="right-hand sides"):
+        s.factorize_multi(het12.A, 9)
+    s.factorize_multi(het12.A, 2)
+    with pytest.raises(MASError, match="nrhs"):
+        s.solve_multi(np.zeros((het12.n, 3)))

@@ -455,3 +455,22 @@ def test_reference_division_guard_reproduces_stall(gpu):
     s.set_parameters({"MAS": {"division_guard": 0.0}})
     s.solve(p.b, x)
     assert s.get_info()["solver_status"] == "Reach relative tolerance"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_spmv_halo_matches_global_gathers(gpu, het12, precision):
+    """x staged per workgroup in LDS (16-bit local columns) gives the same products, bit for bit,
+    as gathering x through the 32-bit columns: a whole solve (its A p and b - A x kernels) ends
+    with the same iterate."""
+    s = _factorized(het12, {"precision": precision, "relative_tolerance": 1e-8, "absolute_tolerance": 0.0})
+    info = s.get_info()["spmv_halo"]
+    assert info["active"] and 0 < info["max_nodes"] <= 2048 and info["total_nodes"] >= het12.n // 3
+    xs_halo = np.zeros(het12.n)
+    s.solve(het12.b, xs_halo)
+    it_halo = s.get_info()["solver_iter"]
+    s.set_parameters({"MAS": {"spmv_halo": False}})
+    assert not s.get_info()["spmv_halo"]["active"]
+    xs = np.zeros(het12.n)
+    s.solve(het12.b, xs)
+    assert s.get_info()["solver_iter"] == it_halo
+    assert np.array_equal(xs, xs_halo)
