@@ -179,7 +179,11 @@ namespace psmas
         /// columns.  Workgroup b reads each block of copy 0's slots once and applies it to every
         /// copy's x (KC >= copies accumulators per row); copy c's pAp partial goes to its block.
         /// Per copy the FMAs run in the same order as in k_spmv_ap.
-        template <int BD, int KC, typename TV, typename TX>
+        ///
+        /// HALO: every copy's x at block b's halo (copy 0's list plus c * npad1) is first staged in
+        /// the dynamic LDS array (copy after copy, hn entries each) and read through the 16-bit
+        /// local columns.
+        template <int BD, int KC, bool HALO, typename TV, typename TX>
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap_copies(SellView<TV> A, int copies, int npad1,
                                                                        int nblocks1, const TX *__restrict__ x,
                                                                        TX *__restrict__ y, Reductions red, SysMap sm)
@@ -189,6 +193,22 @@ namespace psmas
             const int b = xcd_block();
             const int lane = threadIdx.x % WAVE;
             const int slice = b * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            TX *xs = reinterpret_cast<TX *>(spmv_halo_lds);
+            int hn = 0;
+            if constexpr (HALO)
+            {
+                const int hb = A.halo_ptr[b];
+                hn = A.halo_ptr[b + 1] - hb;
+                for (int i = threadIdx.x; i < copies * hn; i += SPMV_BLOCK)
+                {
+                    const int c = i / hn, t = i - c * hn;
+                    const size_t g = (static_cast<size_t>(c) * npad1 + A.halo[hb + t]) * BD;
+#pragma unroll
+                    for (int j = 0; j < BD; ++j)
+                        xs[i * BD + j] = x[g + j];
+                }
+                __syncthreads();
+            }
             bool act[KC];
 #pragma unroll
             for (int c = 0; c < KC; ++c)
@@ -203,6 +223,7 @@ namespace psmas
             const int end = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice + 1]);
             const int last = end - 1;
             const int *__restrict__ cols = A.cols + lane;
+            const unsigned short *__restrict__ lcols = A.lcols + lane;
             const TV *__restrict__ vals = A.vals + lane;
             for (int k = beg; k < end; k += U)
             {
@@ -213,7 +234,10 @@ namespace psmas
                 for (int u = 0; u < U; ++u)
                 {
                     const size_t slot = static_cast<size_t>(min(k + u, last));
-                    col[u] = ldg_stream(cols + slot * WAVE);
+                    if constexpr (HALO)
+                        col[u] = static_cast<int>(ldg_stream(lcols + slot * WAVE));
+                    else
+                        col[u] = ldg_stream(cols + slot * WAVE);
 #pragma unroll
                     for (int e = 0; e < BD * BD; ++e)
                         v[u][e] = ldg_stream(vals + (slot * (BD * BD) + e) * WAVE);
@@ -224,9 +248,14 @@ namespace psmas
                     for (int c = 0; c < KC; ++c)
 #pragma unroll
                         for (int j = 0; j < BD; ++j)
-                            xv[u][c][j] = c < copies
-                                              ? static_cast<TA>(x[(static_cast<size_t>(c) * npad1 + col[u]) * BD + j])
-                                              : TA(0);
+                        {
+                            if (c >= copies)
+                                xv[u][c][j] = TA(0);
+                            else if constexpr (HALO)
+                                xv[u][c][j] = static_cast<TA>(xs[(c * hn + col[u]) * BD + j]);
+                            else
+                                xv[u][c][j] = static_cast<TA>(x[(static_cast<size_t>(c) * npad1 + col[u]) * BD + j]);
+                        }
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                 {
@@ -676,15 +705,29 @@ namespace psmas
                                const SysMap &sm, hipStream_t s)
     {
         const int nblocks1 = npad1 / SPMV_BLOCK;
+        // The halo lists of the batch's first nblocks1 node blocks are copy 0's.
+        const size_t lds = static_cast<size_t>(copies) * A.halo_max * BD * sizeof(TX);
+        const bool halo = A.halo_max > 0 && lds <= 65536;
+#define PSMAS_COPIES_LAUNCH(KC)                                                                                        \
+    if (halo)                                                                                                          \
+        hipLaunchKernelGGL((k_spmv_ap_copies<BD, KC, true, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), lds, s, A,    \
+                           copies, npad1, nblocks1, x, y, red, sm);                                                    \
+    else                                                                                                               \
+        hipLaunchKernelGGL((k_spmv_ap_copies<BD, KC, false, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), 0, s, A,     \
+                           copies, npad1, nblocks1, x, y, red, sm);
         if (copies <= 2)
-            hipLaunchKernelGGL((k_spmv_ap_copies<BD, 2, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), 0, s, A, copies,
-                               npad1, nblocks1, x, y, red, sm);
+        {
+            PSMAS_COPIES_LAUNCH(2)
+        }
         else if (copies <= 4)
-            hipLaunchKernelGGL((k_spmv_ap_copies<BD, 4, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), 0, s, A, copies,
-                               npad1, nblocks1, x, y, red, sm);
+        {
+            PSMAS_COPIES_LAUNCH(4)
+        }
         else
-            hipLaunchKernelGGL((k_spmv_ap_copies<BD, 8, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), 0, s, A, copies,
-                               npad1, nblocks1, x, y, red, sm);
+        {
+            PSMAS_COPIES_LAUNCH(8)
+        }
+#undef PSMAS_COPIES_LAUNCH
         PSMAS_LAUNCH_CHECK();
     }
 #undef PSMAS_UNROLL_DISPATCH
