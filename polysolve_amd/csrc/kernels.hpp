@@ -261,8 +261,11 @@ namespace psmas
     };
 
     template <int BD, typename TM, typename T>
+    /// copies > 1 (FP16 only): the batch repeats one matrix and its first nfine1 banks are copy 0's
+    /// fine banks, applied once for every copy.
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
-                          const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side = ApplySide{});
+                          const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side = ApplySide{},
+                          int copies = 1, int nfine1 = 0);
 
     /// Elements of the vector type the FP16 apply kernel's side workgroups can stage in LDS.
     int apply_h16_stage_floats(int bd);

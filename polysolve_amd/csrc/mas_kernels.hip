@@ -324,7 +324,13 @@ namespace psmas
         /// (tid / N, tid % N) then sums half of row `row` with 16-byte LDS reads.  Two matrices are
         /// in flight per workgroup: the loads of the one after next are issued before the current
         /// matrix is computed.
-        template <int BD>
+        ///
+        /// KC > 1 (a batch of `copies` <= KC systems that repeat one matrix, nfine1 fine banks each):
+        /// fine bank m < nfine1 stands for banks m + c nfine1 of every copy c; its tile is unpacked
+        /// once and one pass over each LDS row multiplies it with every copy's input (k
+        /// accumulators).  Those other copies' fine banks are not visited; coarse banks, which may
+        /// span copies, go one by one as before.
+        template <int BD, int KC>
         __global__ __launch_bounds__(apply_block<BD>()) void k_mas_apply_h16(int mat_begin, int count,
                                                                               const __half *__restrict__ mats,
                                                                               const float *__restrict__ dscale,
@@ -332,7 +338,7 @@ namespace psmas
                                                                               float *__restrict__ mz,
                                                                               const int *__restrict__ bank_sys,
                                                                               const int *__restrict__ active,
-                                                                              ApplySide sd)
+                                                                              ApplySide sd, int copies, int nfine1)
         {
             constexpr int N = 32 * BD;
             constexpr int T = N / 8;
@@ -347,10 +353,16 @@ namespace psmas
             using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
 
             __shared__ __attribute__((aligned(16))) __half sF[N * LDW];
-            __shared__ __attribute__((aligned(16))) float sx[N];
-            __shared__ float part[N];
+            __shared__ __attribute__((aligned(16))) float sx[KC][N];
+            __shared__ float part[KC][N];
 
+            // Copies of fine bank m: copies when m is copy 0's fine bank (KC > 1), else 1.
+            auto ncopies = [&](int m) { return (KC > 1 && m < nfine1) ? copies : 1; };
             auto skip = [&](int m) {
+                if (KC > 1 && m >= nfine1 && m < copies * nfine1)
+                    return true; // done together with copy 0's bank
+                if (KC > 1 && m < nfine1)
+                    return false; // the copies converge at different times
                 if (bank_sys == nullptr)
                     return false;
                 const int sy = bank_sys[m]; // -1: a coarse matrix (its nodes may span systems)
@@ -416,7 +428,8 @@ namespace psmas
             // is stored, so that nothing global is read between the barriers.
             struct Row
             {
-                float x = 0.0f, d = 0.0f;
+                float x[KC] = {};
+                float d = 0.0f;
                 bool store = false;
             };
             auto load = [&](u32x4(&reg)[PER], Row &rw, int mm) {
@@ -427,11 +440,15 @@ namespace psmas
                 if (tid < N)
                 {
                     rw.d = dscale[static_cast<size_t>(mm) * N + tid];
-                    rw.x = coarse_input<BD, float>(sd, mr, mm * 32 + tid / BD, tid % BD) * rw.d;
+                    const int nc = ncopies(mm);
+#pragma unroll
+                    for (int c = 0; c < KC; ++c)
+                        if (c < nc)
+                            rw.x[c] = coarse_input<BD, float>(sd, mr, (mm + c * nfine1) * 32 + tid / BD, tid % BD) * rw.d;
                     rw.store = !dense_owned(sd, mm * 32 + tid / BD);
                 }
             };
-            auto unpack = [&](const u32x4(&reg)[PER], float xr) {
+            auto unpack = [&](const u32x4(&reg)[PER], const Row &rw) {
 #pragma unroll
                 for (int k = 0; k < PER; ++k)
                 {
@@ -471,36 +488,62 @@ namespace psmas
                     }
                 }
                 if (tid < N)
-                    sx[tid] = xr;
+                {
+#pragma unroll
+                    for (int c = 0; c < KC; ++c)
+                        sx[c][tid] = rw.x[c];
+                }
                 __syncthreads();
             };
             const int row = tid % N, hf = tid / N;
             auto compute = [&](int mm, const Row &rw) {
                 const __half *arow = sF + row * LDW + hf * HALF;
-                const float *xs = sx + hf * HALF;
-                float acc = 0.0f;
+                const int nc = ncopies(mm);
+                float acc[KC];
+#pragma unroll
+                for (int c = 0; c < KC; ++c)
+                    acc[c] = 0.0f;
 #pragma unroll
                 for (int t = 0; t < HALF; t += 8)
                 {
                     const u32x4 av = *reinterpret_cast<const u32x4 *>(arow + t);
                     __half h[8];
                     __builtin_memcpy(h, &av, 16);
-                    const float4 x0 = *reinterpret_cast<const float4 *>(xs + t);
-                    const float4 x1 = *reinterpret_cast<const float4 *>(xs + t + 4);
-                    acc = fmaf(__half2float(h[0]), x0.x, acc);
-                    acc = fmaf(__half2float(h[1]), x0.y, acc);
-                    acc = fmaf(__half2float(h[2]), x0.z, acc);
-                    acc = fmaf(__half2float(h[3]), x0.w, acc);
-                    acc = fmaf(__half2float(h[4]), x1.x, acc);
-                    acc = fmaf(__half2float(h[5]), x1.y, acc);
-                    acc = fmaf(__half2float(h[6]), x1.z, acc);
-                    acc = fmaf(__half2float(h[7]), x1.w, acc);
+#pragma unroll
+                    for (int c = 0; c < KC; ++c)
+                    {
+                        if (c >= nc)
+                            break;
+                        const float *xs = sx[c] + hf * HALF;
+                        const float4 x0 = *reinterpret_cast<const float4 *>(xs + t);
+                        const float4 x1 = *reinterpret_cast<const float4 *>(xs + t + 4);
+                        acc[c] = fmaf(__half2float(h[0]), x0.x, acc[c]);
+                        acc[c] = fmaf(__half2float(h[1]), x0.y, acc[c]);
+                        acc[c] = fmaf(__half2float(h[2]), x0.z, acc[c]);
+                        acc[c] = fmaf(__half2float(h[3]), x0.w, acc[c]);
+                        acc[c] = fmaf(__half2float(h[4]), x1.x, acc[c]);
+                        acc[c] = fmaf(__half2float(h[5]), x1.y, acc[c]);
+                        acc[c] = fmaf(__half2float(h[6]), x1.z, acc[c]);
+                        acc[c] = fmaf(__half2float(h[7]), x1.w, acc[c]);
+                    }
                 }
                 if (hf == 1)
-                    part[row] = acc;
+                {
+#pragma unroll
+                    for (int c = 0; c < KC; ++c)
+                        part[c][row] = acc[c];
+                }
                 __syncthreads(); // also: sF and sx may be rewritten after this point
                 if (hf == 0 && rw.store) // (hf == 0: row == tid)
-                    mz[static_cast<size_t>(mm) * N + row] = (acc + part[row]) * rw.d;
+                {
+#pragma unroll
+                    for (int c = 0; c < KC; ++c)
+                    {
+                        if (c >= nc)
+                            break;
+                        mz[static_cast<size_t>(mm + c * nfine1) * N + row] = (acc[c] + part[c][row]) * rw.d;
+                    }
+                }
                 // part[] is next written after the following unpack's barrier
             };
 
@@ -514,7 +557,7 @@ namespace psmas
             while (true)
             {
                 int nn = nxt < end ? step(nxt) : end;
-                unpack(regA, ra.x);
+                unpack(regA, ra);
                 Row now = ra;
                 if (nn < end)
                     load(regA, ra, nn);
@@ -524,7 +567,7 @@ namespace psmas
                 if (cur >= end)
                     break;
                 nn = nxt < end ? step(nxt) : end;
-                unpack(regB, rb.x);
+                unpack(regB, rb);
                 now = rb;
                 if (nn < end)
                     load(regB, rb, nn);
@@ -1154,7 +1197,8 @@ namespace psmas
 
     template <int BD, typename TM, typename T>
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
-                          const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side)
+                          const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side, int copies,
+                          int nfine1)
     {
         if (count <= 0 && side.wgs <= 0)
             return;
@@ -1175,9 +1219,21 @@ namespace psmas
         };
         if constexpr (std::is_same<TM, __half>::value)
         {
-            static const int resident = resident_of(reinterpret_cast<const void *>(&k_mas_apply_h16<BD>));
-            hipLaunchKernelGGL((k_mas_apply_h16<BD>), dim3(grid_for(resident)), dim3(apply_block<BD>()), 0, s,
-                               mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side);
+#define PSMAS_H16_LAUNCH(KC)                                                                                           \
+    {                                                                                                                  \
+        static const int resident = resident_of(reinterpret_cast<const void *>(&k_mas_apply_h16<BD, KC>));             \
+        hipLaunchKernelGGL((k_mas_apply_h16<BD, KC>), dim3(grid_for(resident)), dim3(apply_block<BD>()), 0, s,         \
+                           mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side, copies, nfine1);            \
+    }
+            if (copies <= 1 || nfine1 <= 0)
+                PSMAS_H16_LAUNCH(1)
+            else if (copies <= 2)
+                PSMAS_H16_LAUNCH(2)
+            else if (copies <= 4)
+                PSMAS_H16_LAUNCH(4)
+            else
+                PSMAS_H16_LAUNCH(8)
+#undef PSMAS_H16_LAUNCH
         }
         else
         {
@@ -1317,10 +1373,10 @@ namespace psmas
 #define PSMAS_INST_APPLY(BD)                                                                                     \
     template void launch_mas_apply<BD, double, double>(int, int, const double *, const float *, const double *,   \
                                                        double *, const int *, const int *, hipStream_t,          \
-                                                       const ApplySide &);                                       \
+                                                       const ApplySide &, int, int);                             \
     template void launch_mas_apply<BD, __half, float>(int, int, const __half *, const float *, const float *,     \
                                                       float *, const int *, const int *, hipStream_t,            \
-                                                      const ApplySide &);                                        \
+                                                      const ApplySide &, int, int);                              \
     template void launch_prolong_dot<BD, double>(int, int, const int4 *, const int *, const double *, double *,   \
                                                  const Reductions &, bool, const SysMap &, const LccApply &,     \
                                                  const FlexDot<double> &, hipStream_t);                          \

@@ -1720,10 +1720,12 @@ namespace psmas
             }
         }
         tic("mas_apply_fp16", n1 * per_mat);
+        const bool shared_fine = copies_ > 1 && params_.spmv_copies; // one matrix read for every copy
         launch_mas_apply<BD, __half, float>(0, n1, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
                                             d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_,
                                             restrict_side(d_mr32_.data() + static_cast<size_t>(lev_off_[1]) * BD,
-                                                          d_mr32_.data(), block, lat_blocks));
+                                                          d_mr32_.data(), block, lat_blocks),
+                                            shared_fine ? copies_ : 1, shared_fine ? sys_[0].nparts : 0);
         toc();
         // The network's first layer and skip layer join the second launch's side work when they
         // fit; the output layer follows it.
