@@ -184,10 +184,13 @@ namespace psmas
         /// the dynamic LDS array (copy after copy, hn entries each) and read through the 16-bit
         /// local columns.
         template <int BD, int KC, bool HALO, typename TV, typename TX>
-        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap_copies(SellView<TV> A, int copies, int npad1,
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap_copies(SellView<TV> A, int c0, int copies, int npad1,
                                                                        int nblocks1, const TX *__restrict__ x,
                                                                        TX *__restrict__ y, Reductions red, SysMap sm)
         {
+            // This launch's copies are c0 .. c0 + copies - 1 of the batch.
+            x += static_cast<size_t>(c0) * npad1 * BD;
+            y += static_cast<size_t>(c0) * npad1 * BD;
             using TA = typename Acc<TV>::type;
             constexpr int U = 2;
             const int b = xcd_block();
@@ -212,7 +215,7 @@ namespace psmas
             bool act[KC];
 #pragma unroll
             for (int c = 0; c < KC; ++c)
-                act[c] = c < copies && sm.active[sm.block_sys[c * nblocks1 + b]] != 0;
+                act[c] = c < copies && sm.active[sm.block_sys[(c0 + c) * nblocks1 + b]] != 0;
             TA acc[KC][BD];
 #pragma unroll
             for (int c = 0; c < KC; ++c)
@@ -288,7 +291,7 @@ namespace psmas
                     }
                 }
                 if (act[c]) // uniform in the workgroup
-                    store_partial<SPMV_BLOCK>(dot, red.pap, c * nblocks1 + b);
+                    store_partial<SPMV_BLOCK>(dot, red.pap, (c0 + c) * nblocks1 + b);
             }
         }
 
@@ -704,31 +707,32 @@ namespace psmas
     void launch_spmv_ap_copies(const SellView<TV> &A, int copies, int npad1, const TX *x, TX *y, const Reductions &red,
                                const SysMap &sm, hipStream_t s)
     {
+        // Groups of at most 4 copies per launch: their halos (copy 0's lists plus c * npad1) stay
+        // within 64 KB of LDS, and each group reads the matrix once.
         const int nblocks1 = npad1 / SPMV_BLOCK;
-        // The halo lists of the batch's first nblocks1 node blocks are copy 0's.
-        const size_t lds = static_cast<size_t>(copies) * A.halo_max * BD * sizeof(TX);
-        const bool halo = A.halo_max > 0 && lds <= 65536;
+        for (int c0 = 0; c0 < copies; c0 += 4)
+        {
+            const int cn = std::min(4, copies - c0);
+            const size_t lds = static_cast<size_t>(cn) * A.halo_max * BD * sizeof(TX);
+            const bool halo = A.halo_max > 0 && lds <= 65536;
 #define PSMAS_COPIES_LAUNCH(KC)                                                                                        \
     if (halo)                                                                                                          \
-        hipLaunchKernelGGL((k_spmv_ap_copies<BD, KC, true, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), lds, s, A,    \
-                           copies, npad1, nblocks1, x, y, red, sm);                                                    \
+        hipLaunchKernelGGL((k_spmv_ap_copies<BD, KC, true, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), lds, s, A, c0, \
+                           cn, npad1, nblocks1, x, y, red, sm);                                                        \
     else                                                                                                               \
-        hipLaunchKernelGGL((k_spmv_ap_copies<BD, KC, false, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), 0, s, A,     \
-                           copies, npad1, nblocks1, x, y, red, sm);
-        if (copies <= 2)
-        {
-            PSMAS_COPIES_LAUNCH(2)
-        }
-        else if (copies <= 4)
-        {
-            PSMAS_COPIES_LAUNCH(4)
-        }
-        else
-        {
-            PSMAS_COPIES_LAUNCH(8)
-        }
+        hipLaunchKernelGGL((k_spmv_ap_copies<BD, KC, false, TV, TX>), dim3(nblocks1), dim3(SPMV_BLOCK), 0, s, A, c0, \
+                           cn, npad1, nblocks1, x, y, red, sm);
+            if (cn <= 2)
+            {
+                PSMAS_COPIES_LAUNCH(2)
+            }
+            else
+            {
+                PSMAS_COPIES_LAUNCH(4)
+            }
 #undef PSMAS_COPIES_LAUNCH
-        PSMAS_LAUNCH_CHECK();
+            PSMAS_LAUNCH_CHECK();
+        }
     }
 #undef PSMAS_UNROLL_DISPATCH
 
