@@ -134,7 +134,8 @@ int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d
 
 /* Several right-hand sides of one matrix (no counterpart in the reference; for Newton or IPC
  * callers that solve one Hessian for several vectors): factorise nrhs (1..8) copies of A as a
- * batch whose SpMV reads the matrix once for all of them, then solve all right-hand sides in one
+ * batch whose SpMV and fine-level preconditioner apply read the matrix once for all of them
+ * (parameter spmv_copies), then solve all right-hand sides in one
  * launch sequence, each with its own CG scalars, convergence test and status
  * (psmas_batch_get_info(h, c, ...) for copy c).  B and X hold nrhs vectors of length n one after
  * the other (column-major n x nrhs), host arrays or device pointers (_device). */
