@@ -15,7 +15,9 @@ Three paths run on the same system:
               network trained beforehand on pairs from solves of the following steps of a
               drifting-modulus sequence (collection and training times reported separately).
 The headline ``value`` is the learned path's ms per solve (the mixed path's when the learned path
-is disabled or falls back); the other two are reported beside it.
+is disabled or falls back); the other two are reported beside it.  ``multi_rhs`` reports
+``--multi-rhs`` (default 4) seeded right-hand sides of the same matrix solved together
+(factorize_multi, mixed path) against the same right-hand sides solved one by one.
 
 One JSON line is printed (driver contract), plus roofline and CPU-baseline objects.
 """
@@ -188,6 +190,8 @@ def main() -> int:
     ap.add_argument("--hidden", type=int, default=128, help="learned path: hidden width of the network")
     ap.add_argument("--random-pairs", type=int, default=200, help="learned path: Gaussian pairs per training solve")
     ap.add_argument("--epochs", type=int, default=150, help="learned path: training epochs")
+    ap.add_argument("--multi-rhs", type=int, default=4,
+                    help="also solve this many right-hand sides together (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="profile mode: first run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -405,6 +409,53 @@ def main() -> int:
                          f"{info64['solver_iter']} iterations",
                "cpu": cpu_info(), "nproc": os.cpu_count()}
 
+    # Several right-hand sides of this matrix solved together (mixed path, factorize_multi): the
+    # SpMV and the fine-bank apply read the matrix once for all of them.  Compared with solving
+    # the same right-hand sides one by one with the same parameters.
+    multi = None
+    if args.multi_rhs > 1 and not args.profile_only:
+        k = args.multi_rhs
+        rng = np.random.default_rng(7)
+        B = rng.standard_normal((n, k))
+        B[:, 0] = prob.b
+        mprm = {"MAS": dict(base, precision="mixed", coarse_solver=args.coarse, learned_correction=False,
+                            learned_collect=False)}
+        solver.set_parameters(mprm)
+        bc = DeviceArray(n)
+        solver.solve_device(b_dev.ptr, x_dev.ptr, n)
+        synchronize()
+        t = time.perf_counter()
+        for c in range(k):
+            bc.upload(B[:, c])
+            solver.solve_device(bc.ptr, x_dev.ptr, n)
+        synchronize()
+        singles_ms = (time.perf_counter() - t) * 1e3
+        bc.free()
+        ms_solver = MASSolver()
+        ms_solver.set_parameters(mprm)
+        ms_solver.set_coordinates(prob.coords)
+        t = time.perf_counter()
+        ms_solver.factorize_multi(prob.A, k)
+        multi_setup = (time.perf_counter() - t) * 1e3
+        dB, dX = DeviceArray(n * k, B.ravel(order="F")), DeviceArray(n * k)
+        ms_solver.solve_multi_device(dB.ptr, dX.ptr, n, k)
+        synchronize()
+        t = time.perf_counter()
+        for _ in range(max(args.steps // 2, 1)):
+            ms_solver.solve_multi_device(dB.ptr, dX.ptr, n, k)
+        synchronize()
+        multi_ms = (time.perf_counter() - t) * 1e3 / max(args.steps // 2, 1)
+        X = dX.download().reshape((n, k), order="F")
+        worst = max(float(np.linalg.norm(B[:, c] - prob.A @ X[:, c]) / np.linalg.norm(B[:, c])) for c in range(k))
+        multi = {"rhs": k, "ms": round(multi_ms, 3), "ms_per_rhs": round(multi_ms / k, 3),
+                 "one_by_one_ms": round(singles_ms, 3), "throughput_vs_one_by_one": round(singles_ms / multi_ms, 3),
+                 "iters": [ms_solver.batch_info(c)["solver_iter"] for c in range(k)],
+                 "max_true_rel_residual": worst, "setup_ms": round(multi_setup, 1)}
+        log(f"  multi-RHS: {k} right-hand sides in {multi_ms:.1f} ms against {singles_ms:.1f} ms one by one")
+        dB.free()
+        dX.free()
+        del ms_solver
+
     head_ms = learned["ms"] if learned is not None and not learned["fallback"] else mixed_ms
     head_path = ("learned-correction mixed MAS-PCG (flexible, reliable updates)" if head_ms != mixed_ms
                  else "mixed MAS-PCG (reliable updates)")
@@ -437,6 +488,7 @@ def main() -> int:
         "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
         "setup_ms": round(setup_ms, 1),
         "roofline": roofline,
+        "multi_rhs": multi,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
