@@ -39,4 +39,9 @@ namespace psmas
     void build_sell_device(int n, long long nnz, int bd, const int *d_colptr, const int *d_rowidx,
                            const double *d_vals, const int *d_slot_of_block, int npad, bool want_map,
                            hipStream_t s, SellDevice &out);
+
+    /// The block-diagonal matrix of `copies` copies of m (built with npad slots): copy c's rows
+    /// and columns are those of copy 0 moved by c * npad, its slices follow copy c-1's, as the
+    /// concatenated build of identical systems lays them out.  Drops the value map.
+    void replicate_sell(SellDevice &m, int copies, int npad, hipStream_t s);
 } // namespace psmas

@@ -393,7 +393,7 @@ namespace psmas
     }
 
     void Solver::factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
-                                const DeviceCsc *dev)
+                                const DeviceCsc *dev, bool replicate)
     {
         const int bd = params_.block_dim;
         factorized_ = false;
@@ -420,9 +420,10 @@ namespace psmas
             sys_.resize(K);
         const bool bd_changed = (bd != bd_);
         bd_ = bd;
-        // One system without elimination: the SELL matrix is built on the GPU from the CSC arrays
-        // (uploaded here, or the caller's device arrays).
-        const bool gpu_build = K == 1 && dir == nullptr && (dev != nullptr || params_.device_setup);
+        // One system (or copies of one) without elimination: the SELL matrix is built on the GPU
+        // from the CSC arrays (uploaded here, or the caller's device arrays).
+        replicate = replicate && K > 1;
+        const bool gpu_build = (K == 1 || replicate) && dir == nullptr && (dev != nullptr || params_.device_setup);
         vr_nnz_ = -1;
         // The CSC arrays in device memory for the device aggregation and build (host input is
         // uploaded once here).
@@ -468,9 +469,15 @@ namespace psmas
                                  static_cast<int>(S.part.perm.size()) != nb || bd_changed;
             S.n = A.n;
             S.nb = nb;
-            if (rebuild)
+            if (replicate && s > 0)
+            {
+                S.part = sys_[0].part;
+                S.aggregation = sys_[0].aggregation;
+                S.have_partition = true;
+            }
+            else if (rebuild)
                 build_partition(S, A, gpu_build ? &dcsc : nullptr);
-            if (K == 1)
+            if (s == 0 && (K == 1 || replicate))
                 phase("partition");
             PaddedBsr bsr;
             if (gpu_build)
@@ -617,8 +624,13 @@ namespace psmas
         DeviceBuffer<int> d_sob(h_slot_of_block.size(), "MAS block map");
         PSMAS_HIP(hipMemcpyAsync(d_sob.data(), h_slot_of_block.data(), d_sob.bytes(), hipMemcpyHostToDevice, stream_));
         SellDevice sd;
-        build_sell_device(S.n, nnz, bd_, d_cp, d_ri, d_v, d_sob.data(), npad_, params_.lazy_partitioning, stream_, sd);
-        nnzb_ = S.nnzb = sd.nnzb;
+        const int K = static_cast<int>(sys_.size());
+        build_sell_device(S.n, nnz, bd_, d_cp, d_ri, d_v, d_sob.data(), S.npad, params_.lazy_partitioning && K == 1,
+                          stream_, sd);
+        for (System &C : sys_)
+            C.nnzb = sd.nnzb;
+        replicate_sell(sd, K, S.npad, stream_); // (copies of system 0 for several right-hand sides)
+        nnzb_ = sd.nnzb;
         nonisolated_ = std::move(sd.nonisolated);
         nslices_ = npad_ / WAVE;
         nslots_ = sd.nslots;
@@ -677,7 +689,7 @@ namespace psmas
                 sys_[c].coord_dim = S0.coord_dim;
             }
         }
-        factorize_full(std::vector<CscView>(copies, A), nullptr, nullptr);
+        factorize_full(std::vector<CscView>(copies, A), nullptr, nullptr, nullptr, true);
         for (int c = 1; c < copies; ++c)
             if (sys_[c].npad != sys_[0].npad || sys_[c].part.perm != sys_[0].part.perm)
                 return; // (not expected: the copies keep their own SpMV then)

@@ -292,8 +292,10 @@ namespace psmas
         bool try_refresh_values(const CscView &A, const std::vector<int> *dirichlet_nodes);
         /// Full factorisation; for a Dirichlet one, As[0] is the eliminated matrix, raw the matrix
         /// before elimination (whose non-zeros the value map then indexes) and dir the elimination.
+        /// replicate: every As[s] is As[0] (copies for several right-hand sides); system 0's
+        /// partition and matrix are built once and copied.
         void factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
-                            const DeviceCsc *dev = nullptr);
+                            const DeviceCsc *dev = nullptr, bool replicate = false);
         /// The SELL matrix of one system built on the GPU (sell_build.hpp) from the CSC arrays in
         /// device memory (A uploaded, or the caller's arrays when device_input); also the halo
         /// lists and, with lazy_partitioning, the value map (and A's host pattern for host input).

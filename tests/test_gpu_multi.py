@@ -62,3 +62,28 @@ def test_multi_rhs_errors(gpu, het12):
     s.factorize_multi(het12.A, 2)
     with pytest.raises(MASError, match="nrhs"):
         s.solve_multi(np.zeros((het12.n, 3)))
+
+
+@pytest.mark.parametrize("coords", [True, False])
+def test_multi_rhs_replicated_device_build(gpu, het12, coords):
+    """factorize_multi builds system 0's matrix on the GPU once and copies it to the other right-hand
+    sides; the host build of every copy gives the same solutions (to the FP64 Galerkin's rounding)."""
+    k = 4
+    B = np.random.default_rng(7).standard_normal((het12.n, k))
+    out = []
+    for device_setup in (True, False):
+        s = MASSolver()
+        s.set_parameters({"MAS": {**_prm("mixed")["MAS"], "device_setup": device_setup}})
+        if coords:
+            s.set_coordinates(het12.coords)
+        s.factorize_multi(het12.A, k)
+        info = s.get_info()
+        assert info["copies"] == k
+        assert ("device_matrix_build" in info["setup_ms"]) == device_setup
+        out.append((s.solve_multi(B), [s.batch_info(c)["solver_iter"] for c in range(k)]))
+    (Xd, itd), (Xh, ith) = out
+    for c in range(k):
+        assert np.linalg.norm(B[:, c] - het12.A @ Xd[:, c]) <= 1e-8 * np.linalg.norm(B[:, c])
+        if coords:  # the same domains on both sides
+            assert abs(itd[c] - ith[c]) <= 10
+            assert np.linalg.norm(Xd[:, c] - Xh[:, c]) <= 1e-6 * np.linalg.norm(Xh[:, c])
