@@ -171,6 +171,14 @@ namespace psmas
                 out[static_cast<long long>(c) * len + i] = v + c * npad;
         }
 
+        __global__ __launch_bounds__(TB) void k_compose_map(long long n, const int *__restrict__ keep,
+                                                            const int *__restrict__ map, int *__restrict__ out)
+        {
+            const long long k = static_cast<long long>(blockIdx.x) * TB + threadIdx.x;
+            if (k < n)
+                out[k] = keep[k] >= 0 ? map[keep[k]] : -1;
+        }
+
         /// Flags: 1 colptr[0] != 0 or colptr[n] != nnz; 2 a decreasing column pointer; 4 a row
         /// index out of [0, n).
         __global__ __launch_bounds__(TB) void k_validate(int n, long long nnz, const int *__restrict__ colptr,
@@ -355,5 +363,21 @@ namespace psmas
         m.nslots *= copies;
         m.d_map.reset();
         m.map_ok = false;
+    }
+
+    DeviceBuffer<int> compose_value_map(const std::vector<int> &keep, const DeviceBuffer<int> &map, hipStream_t s)
+    {
+        const size_t n = keep.size();
+        DeviceBuffer<int> d_keep(std::max<size_t>(n, 1), "MAS Dirichlet value map");
+        DeviceBuffer<int> out(std::max<size_t>(n, 1), "MAS value map");
+        if (n > 0)
+        {
+            PSMAS_HIP(hipMemcpyAsync(d_keep.data(), keep.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_compose_map, dim3(grid_for(static_cast<long long>(n))), dim3(TB), 0, s,
+                               static_cast<long long>(n), d_keep.data(), map.data(), out.data());
+            PSMAS_LAUNCH_CHECK();
+        }
+        PSMAS_HIP(hipStreamSynchronize(s));
+        return out;
     }
 } // namespace psmas

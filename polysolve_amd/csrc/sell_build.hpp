@@ -44,4 +44,8 @@ namespace psmas
     /// and columns are those of copy 0 moved by c * npad, its slices follow copy c-1's, as the
     /// concatenated build of identical systems lays them out.  Drops the value map.
     void replicate_sell(SellDevice &m, int copies, int npad, hipStream_t s);
+
+    /// map[keep[k]] for every k (-1 where keep[k] < 0): the value index of each non-zero of a
+    /// matrix whose non-zero k became non-zero keep[k] of the built one (or was dropped).
+    DeviceBuffer<int> compose_value_map(const std::vector<int> &keep, const DeviceBuffer<int> &map, hipStream_t s);
 } // namespace psmas

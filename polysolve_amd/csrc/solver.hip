@@ -423,7 +423,7 @@ namespace psmas
         // One system (or copies of one) without elimination: the SELL matrix is built on the GPU
         // from the CSC arrays (uploaded here, or the caller's device arrays).
         replicate = replicate && K > 1;
-        const bool gpu_build = (K == 1 || replicate) && dir == nullptr && (dev != nullptr || params_.device_setup);
+        const bool gpu_build = (K == 1 || replicate) && (dev != nullptr || params_.device_setup);
         vr_nnz_ = -1;
         // The CSC arrays in device memory for the device aggregation and build (host input is
         // uploaded once here).
@@ -523,7 +523,7 @@ namespace psmas
 
         if (gpu_build)
         {
-            device_matrix_build(As[0], dcsc, dev != nullptr, h_slot_of_block, phase);
+            device_matrix_build(As[0], dcsc, dev != nullptr, h_slot_of_block, raw, dir, phase);
         }
         else
         {
@@ -615,7 +615,8 @@ namespace psmas
 
     template <typename Phase>
     void Solver::device_matrix_build(const CscView &A, const DeviceCsc &dcsc, bool device_input,
-                                     const std::vector<int> &h_slot_of_block, Phase &phase)
+                                     const std::vector<int> &h_slot_of_block, const CscView *raw,
+                                     const DirichletSystem *dir, Phase &phase)
     {
         System &S = sys_[0];
         const int *d_cp = dcsc.colptr, *d_ri = dcsc.rowidx;
@@ -652,11 +653,19 @@ namespace psmas
         {
             d_vmap_ = std::move(sd.d_map);
             vr_nnz_ = nnz;
+            const CscView &P = raw != nullptr ? *raw : A;
+            if (dir != nullptr)
+            {
+                // Index the non-zeros of the matrix before elimination; dropped ones get -1.
+                vr_nnz_ = P.nnz;
+                d_vmap_ = compose_value_map(dir->keep, d_vmap_, stream_);
+            }
             if (!device_input)
             {
-                vr_colptr_.assign(A.colptr, A.colptr + A.n + 1);
-                vr_rowidx_.assign(A.rowidx, A.rowidx + A.nnz);
+                vr_colptr_.assign(P.colptr, P.colptr + P.n + 1);
+                vr_rowidx_.assign(P.rowidx, P.rowidx + P.nnz);
             }
+            phase("value_map");
         }
     }
 

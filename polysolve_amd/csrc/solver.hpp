@@ -298,11 +298,13 @@ namespace psmas
                             const DeviceCsc *dev = nullptr, bool replicate = false);
         /// The SELL matrix of one system built on the GPU (sell_build.hpp) from the CSC arrays in
         /// device memory (A uploaded, or the caller's arrays when device_input); also the halo
-        /// lists and, with lazy_partitioning, the value map (and A's host pattern for host input).
-        /// phase(name) closes a timed setup phase.
+        /// lists and, with lazy_partitioning, the value map (and A's host pattern for host input);
+        /// for a Dirichlet factorisation (raw, dir as in factorize_full) the map indexes raw's
+        /// non-zeros.  phase(name) closes a timed setup phase.
         template <typename Phase>
         void device_matrix_build(const CscView &A, const DeviceCsc &dcsc, bool device_input,
-                                 const std::vector<int> &h_slot_of_block, Phase &phase);
+                                 const std::vector<int> &h_slot_of_block, const CscView *raw,
+                                 const DirichletSystem *dir, Phase &phase);
         /// New SELL values (and coupling values of a Dirichlet factorisation) from d_vals in the
         /// recorded CSC order.
         void scatter_new_values(const double *d_vals);

@@ -28,9 +28,10 @@ def nodes(bare):
     return np.concatenate([face, extra, face[:3]]).astype(np.int32)
 
 
-def _solver(bare, precision="fp64", lazy=False):
+def _solver(bare, precision="fp64", lazy=False, device_setup=True):
     s = MASSolver()
     s.set_parameters({"MAS": {"block_dim": 3, "precision": precision, "lazy_partitioning": lazy,
+                              "device_setup": device_setup,
                               "aggregation": "morton", "relative_tolerance": 1e-10, "absolute_tolerance": 0.0}})
     s.set_coordinates(bare.coords)
     return s
@@ -47,18 +48,23 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("device_setup", [True, False])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_dirichlet_factorize_matches_eliminated_matrix(gpu, bare, nodes, precision):
-    s = _solver(bare, precision)
+def test_dirichlet_factorize_matches_eliminated_matrix(gpu, bare, nodes, precision, device_setup):
+    s = _solver(bare, precision, device_setup=device_setup)
     s.dirichlet_factorize(bare.A, nodes)
-    info = s.get_info()["dirichlet"]
-    assert info["active"] and info["fixed_dofs"] == np.unique(nodes).size
-    ref = _solver(bare, precision)
+    info = s.get_info()
+    assert info["dirichlet"]["active"] and info["dirichlet"]["fixed_dofs"] == np.unique(nodes).size
+    assert ("device_matrix_build" in info["setup_ms"]) == device_setup
+    ref = _solver(bare, precision, device_setup=False)
     At = DO.eliminate(bare.A, nodes)
     ref.factorize(At)
     r = np.random.default_rng(1).standard_normal(bare.n)
     assert _rel(s.apply_preconditioner(r), ref.apply_preconditioner(r)) <= 1e-12
     assert _rel(s.spmv(r), At @ r) <= 1e-13
+    host = _solver(bare, precision, device_setup=False)
+    host.dirichlet_factorize(bare.A, nodes)
+    assert np.array_equal(s.spmv(r), host.spmv(r))  # the GPU build gives the host build's matrix
 
 
 def test_lift_is_bit_exact_and_solution_matches_reduced_solve(gpu, bare, nodes):
@@ -74,13 +80,14 @@ def test_lift_is_bit_exact_and_solution_matches_reduced_solve(gpu, bare, nodes):
     assert np.abs(u[nodes] - f[nodes]).max() <= 1e-8 * np.abs(u).max()
 
 
+@pytest.mark.parametrize("device_setup", [True, False])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_newton_steps_refresh_values_only(gpu, bare, nodes, precision):
+def test_newton_steps_refresh_values_only(gpu, bare, nodes, precision, device_setup):
     rng = np.random.default_rng(3)
     d = rng.uniform(0.5, 2.0, bare.n)
     A2 = (bare.A.multiply(d[:, None]).multiply(d[None, :])).tocsc()
     A2.sort_indices()
-    s = _solver(bare, precision, lazy=True)
+    s = _solver(bare, precision, lazy=True, device_setup=device_setup)
     s.dirichlet_factorize(bare.A, nodes)
     assert s.get_info()["factorize_mode"] == "full"
     s.dirichlet_factorize(A2, nodes)
