@@ -179,6 +179,14 @@ namespace psmas
                 out[k] = keep[k] >= 0 ? map[keep[k]] : -1;
         }
 
+        __global__ __launch_bounds__(TB) void k_shift_cols(long long len, int off, const int *__restrict__ cols,
+                                                           int *__restrict__ out)
+        {
+            const long long i = static_cast<long long>(blockIdx.x) * TB + threadIdx.x;
+            if (i < len)
+                out[i] = cols[i] + off;
+        }
+
         /// Flags: 1 colptr[0] != 0 or colptr[n] != nnz; 2 a decreasing column pointer; 4 a row
         /// index out of [0, n).
         __global__ __launch_bounds__(TB) void k_validate(int n, long long nnz, const int *__restrict__ colptr,
@@ -379,5 +387,54 @@ namespace psmas
         }
         PSMAS_HIP(hipStreamSynchronize(s));
         return out;
+    }
+
+    void concat_sell(std::vector<SellDevice> &parts, const std::vector<int> &npads, hipStream_t s, SellDevice &out)
+    {
+        long long slots = 0, blocks = 0, rows = 0;
+        size_t nvals = 0;
+        for (size_t p = 0; p < parts.size(); ++p)
+        {
+            slots += parts[p].nslots;
+            blocks += parts[p].nnzb;
+            rows += npads[p];
+            nvals += parts[p].d_vals.size();
+        }
+        if (slots > INT_MAX || rows > INT_MAX)
+            throw InvalidArgumentError("[MAS] the batch is too large for 32-bit slot indices.");
+        out = SellDevice();
+        out.nnzb = blocks;
+        out.nslots = slots;
+        out.slice_ptr.assign(1, 0);
+        out.d_slice_ptr.allocate(static_cast<size_t>(rows / 64 + 1), "MAS SELL slice pointers");
+        out.d_cols.allocate(static_cast<size_t>(slots) * 64, "MAS SELL columns");
+        out.d_vals.allocate(std::max<size_t>(nvals, 1), "MAS SELL values (FP64)");
+        out.d_row_len.allocate(static_cast<size_t>(rows), "MAS SELL row lengths");
+        long long slot_off = 0;
+        size_t val_off = 0;
+        int row_off = 0;
+        for (size_t p = 0; p < parts.size(); ++p)
+        {
+            SellDevice &m = parts[p];
+            const int nslices = npads[p] / 64;
+            for (int k = 1; k <= nslices; ++k)
+                out.slice_ptr.push_back(m.slice_ptr[k] + static_cast<int>(slot_off));
+            out.nonisolated.insert(out.nonisolated.end(), m.nonisolated.begin(), m.nonisolated.end());
+            const long long ncols = m.nslots * 64;
+            hipLaunchKernelGGL(k_shift_cols, dim3(grid_for(ncols)), dim3(TB), 0, s, ncols, row_off, m.d_cols.data(),
+                               out.d_cols.data() + slot_off * 64);
+            PSMAS_LAUNCH_CHECK();
+            PSMAS_HIP(hipMemcpyAsync(out.d_vals.data() + val_off, m.d_vals.data(), m.d_vals.bytes(),
+                                     hipMemcpyDeviceToDevice, s));
+            PSMAS_HIP(hipMemcpyAsync(out.d_row_len.data() + row_off, m.d_row_len.data(), sizeof(int) * npads[p],
+                                     hipMemcpyDeviceToDevice, s));
+            slot_off += m.nslots;
+            val_off += m.d_vals.size();
+            row_off += npads[p];
+        }
+        PSMAS_HIP(hipMemcpyAsync(out.d_slice_ptr.data(), out.slice_ptr.data(), out.d_slice_ptr.bytes(),
+                                 hipMemcpyHostToDevice, s));
+        PSMAS_HIP(hipStreamSynchronize(s));
+        parts.clear();
     }
 } // namespace psmas

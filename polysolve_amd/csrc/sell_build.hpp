@@ -48,4 +48,9 @@ namespace psmas
     /// map[keep[k]] for every k (-1 where keep[k] < 0): the value index of each non-zero of a
     /// matrix whose non-zero k became non-zero keep[k] of the built one (or was dropped).
     DeviceBuffer<int> compose_value_map(const std::vector<int> &keep, const DeviceBuffer<int> &map, hipStream_t s);
+
+    /// The block-diagonal matrix of the systems parts[p] (each built with npads[p] slots), one
+    /// after another as the concatenated host build lays them out: system p's rows and columns
+    /// moved by the slots of the systems before it, its slices after theirs.  Frees parts.
+    void concat_sell(std::vector<SellDevice> &parts, const std::vector<int> &npads, hipStream_t s, SellDevice &out);
 } // namespace psmas

@@ -420,31 +420,41 @@ namespace psmas
             sys_.resize(K);
         const bool bd_changed = (bd != bd_);
         bd_ = bd;
-        // One system (or copies of one) without elimination: the SELL matrix is built on the GPU
-        // from the CSC arrays (uploaded here, or the caller's device arrays).
+        // The SELL matrix is built on the GPU from the CSC arrays (uploaded here, or the caller's
+        // device arrays): one system, copies of one, or every system of a batch.
         replicate = replicate && K > 1;
-        const bool gpu_build = (K == 1 || replicate) && (dev != nullptr || params_.device_setup);
+        const bool gpu_build = dev != nullptr || params_.device_setup;
         vr_nnz_ = -1;
         // The CSC arrays in device memory for the device aggregation and build (host input is
-        // uploaded once here).
-        DeviceCsc dcsc;
-        DeviceBuffer<int> up_cp, up_ri;
+        // uploaded once here; one system's values stay for values-only refactorisations).
+        std::vector<DeviceCsc> dcsc(K);
+        const int nup = replicate ? 1 : K;
+        std::vector<DeviceBuffer<int>> up_cp(nup), up_ri(nup);
+        std::vector<DeviceBuffer<double>> up_v(nup);
         if (gpu_build && dev != nullptr)
-            dcsc = *dev;
+            dcsc[0] = *dev;
         else if (gpu_build)
         {
-            const CscView &A = As[0];
-            up_cp.allocate(static_cast<size_t>(A.n) + 1, "MAS input pattern");
-            up_ri.allocate(static_cast<size_t>(std::max<long long>(A.nnz, 1)), "MAS input pattern");
-            d_csc_vals_.ensure(static_cast<size_t>(std::max<long long>(A.nnz, 1)), "MAS input values");
-            PSMAS_HIP(hipMemcpyAsync(up_cp.data(), A.colptr, sizeof(int) * (A.n + 1), hipMemcpyHostToDevice, stream_));
-            if (A.nnz > 0)
+            for (int u = 0; u < nup; ++u)
             {
-                PSMAS_HIP(hipMemcpyAsync(up_ri.data(), A.rowidx, sizeof(int) * A.nnz, hipMemcpyHostToDevice, stream_));
-                PSMAS_HIP(hipMemcpyAsync(d_csc_vals_.data(), A.vals, sizeof(double) * A.nnz, hipMemcpyHostToDevice,
+                const CscView &A = As[u];
+                const size_t nz = static_cast<size_t>(std::max<long long>(A.nnz, 1));
+                up_cp[u].allocate(static_cast<size_t>(A.n) + 1, "MAS input pattern");
+                up_ri[u].allocate(nz, "MAS input pattern");
+                DeviceBuffer<double> &v = (K == 1) ? d_csc_vals_ : up_v[u];
+                v.ensure(nz, "MAS input values");
+                PSMAS_HIP(hipMemcpyAsync(up_cp[u].data(), A.colptr, sizeof(int) * (A.n + 1), hipMemcpyHostToDevice,
                                          stream_));
+                if (A.nnz > 0)
+                {
+                    PSMAS_HIP(hipMemcpyAsync(up_ri[u].data(), A.rowidx, sizeof(int) * A.nnz, hipMemcpyHostToDevice,
+                                             stream_));
+                    PSMAS_HIP(hipMemcpyAsync(v.data(), A.vals, sizeof(double) * A.nnz, hipMemcpyHostToDevice, stream_));
+                }
+                dcsc[u] = DeviceCsc{A.n, A.nnz, up_cp[u].data(), up_ri[u].data(), v.data()};
             }
-            dcsc = DeviceCsc{A.n, A.nnz, up_cp.data(), up_ri.data(), d_csc_vals_.data()};
+            for (int s = nup; s < K; ++s)
+                dcsc[s] = dcsc[0];
             phase("csc_upload");
         }
 
@@ -476,7 +486,7 @@ namespace psmas
                 S.have_partition = true;
             }
             else if (rebuild)
-                build_partition(S, A, gpu_build ? &dcsc : nullptr);
+                build_partition(S, A, gpu_build ? &dcsc[s] : nullptr);
             if (s == 0 && (K == 1 || replicate))
                 phase("partition");
             PaddedBsr bsr;
@@ -523,7 +533,9 @@ namespace psmas
 
         if (gpu_build)
         {
-            device_matrix_build(As[0], dcsc, dev != nullptr, h_slot_of_block, raw, dir, phase);
+            if (K > 1 && !replicate)
+                phase("partition");
+            device_matrix_build(As[0], dcsc, dev != nullptr, replicate, h_slot_of_block, raw, dir, phase);
         }
         else
         {
@@ -614,23 +626,40 @@ namespace psmas
     }
 
     template <typename Phase>
-    void Solver::device_matrix_build(const CscView &A, const DeviceCsc &dcsc, bool device_input,
-                                     const std::vector<int> &h_slot_of_block, const CscView *raw,
+    void Solver::device_matrix_build(const CscView &A, const std::vector<DeviceCsc> &dcsc, bool device_input,
+                                     bool replicate, const std::vector<int> &h_slot_of_block, const CscView *raw,
                                      const DirichletSystem *dir, Phase &phase)
     {
         System &S = sys_[0];
-        const int *d_cp = dcsc.colptr, *d_ri = dcsc.rowidx;
-        const double *d_v = dcsc.vals;
-        const long long nnz = dcsc.nnz;
+        const long long nnz = dcsc[0].nnz;
         DeviceBuffer<int> d_sob(h_slot_of_block.size(), "MAS block map");
         PSMAS_HIP(hipMemcpyAsync(d_sob.data(), h_slot_of_block.data(), d_sob.bytes(), hipMemcpyHostToDevice, stream_));
         SellDevice sd;
         const int K = static_cast<int>(sys_.size());
-        build_sell_device(S.n, nnz, bd_, d_cp, d_ri, d_v, d_sob.data(), S.npad, params_.lazy_partitioning && K == 1,
-                          stream_, sd);
-        for (System &C : sys_)
-            C.nnzb = sd.nnzb;
-        replicate_sell(sd, K, S.npad, stream_); // (copies of system 0 for several right-hand sides)
+        if (K == 1 || replicate)
+        {
+            build_sell_device(S.n, nnz, bd_, dcsc[0].colptr, dcsc[0].rowidx, dcsc[0].vals, d_sob.data(), S.npad,
+                              params_.lazy_partitioning && K == 1, stream_, sd);
+            for (System &C : sys_)
+                C.nnzb = sd.nnzb;
+            replicate_sell(sd, K, S.npad, stream_); // (copies of system 0 for several right-hand sides)
+        }
+        else
+        {
+            // Every system on its own (its slot map is its segment of the concatenated one), then
+            // one block-diagonal matrix.
+            std::vector<SellDevice> parts(K);
+            std::vector<int> npads(K);
+            for (int s = 0; s < K; ++s)
+            {
+                System &C = sys_[s];
+                build_sell_device(C.n, dcsc[s].nnz, bd_, dcsc[s].colptr, dcsc[s].rowidx, dcsc[s].vals,
+                                  d_sob.data() + C.sob_off, C.npad, false, stream_, parts[s]);
+                C.nnzb = parts[s].nnzb;
+                npads[s] = C.npad;
+            }
+            concat_sell(parts, npads, stream_, sd);
+        }
         nnzb_ = sd.nnzb;
         nonisolated_ = std::move(sd.nonisolated);
         nslices_ = npad_ / WAVE;

@@ -296,14 +296,15 @@ namespace psmas
         /// partition and matrix are built once and copied.
         void factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
                             const DeviceCsc *dev = nullptr, bool replicate = false);
-        /// The SELL matrix of one system built on the GPU (sell_build.hpp) from the CSC arrays in
-        /// device memory (A uploaded, or the caller's arrays when device_input); also the halo
+        /// The SELL matrix built on the GPU (sell_build.hpp) from the CSC arrays in device memory
+        /// (dcsc[s] per system: uploaded, or the caller's arrays when device_input), for one
+        /// system, `replicate` copies of system 0, or every system of a batch; also the halo
         /// lists and, with lazy_partitioning, the value map (and A's host pattern for host input);
         /// for a Dirichlet factorisation (raw, dir as in factorize_full) the map indexes raw's
         /// non-zeros.  phase(name) closes a timed setup phase.
         template <typename Phase>
-        void device_matrix_build(const CscView &A, const DeviceCsc &dcsc, bool device_input,
-                                 const std::vector<int> &h_slot_of_block, const CscView *raw,
+        void device_matrix_build(const CscView &A, const std::vector<DeviceCsc> &dcsc, bool device_input,
+                                 bool replicate, const std::vector<int> &h_slot_of_block, const CscView *raw,
                                  const DirichletSystem *dir, Phase &phase);
         /// New SELL values (and coupling values of a Dirichlet factorisation) from d_vals in the
         /// recorded CSC order.

@@ -80,3 +80,29 @@ def test_spmv_unroll_variants_match(gpu, unroll):
     assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
     y32 = s.spmv(x, fp32=True)
     assert np.linalg.norm(y32 - ref) <= 2e-6 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("coords", [True, False])
+def test_batch_device_build_matches_host_build(gpu, uneven_batch, coords):
+    """The batch's matrices built on the GPU system by system and joined (device_setup) against the
+    host build of the whole batch: the same layout and, on the same domains, the same solutions."""
+    out = []
+    for device_setup in (True, False):
+        s = MASSolver()
+        s.set_parameters({"MAS": {"block_dim": 3, "precision": "fp64", "relative_tolerance": 1e-8,
+                                  "absolute_tolerance": 0.0, "max_iter": 5000, "device_setup": device_setup}})
+        s.factorize_batch([p.A for p in uneven_batch],
+                          coords=[p.coords for p in uneven_batch] if coords else None)
+        assert ("device_matrix_build" in s.get_info()["setup_ms"]) == device_setup
+        xs = [np.zeros(p.n) for p in uneven_batch]
+        s.solve_batch([p.b for p in uneven_batch], xs)
+        for i, p in enumerate(uneven_batch):
+            assert np.linalg.norm(p.b - p.A @ xs[i]) <= 1e-8 * np.linalg.norm(p.b)
+        out.append((s.level_info(), xs, [s.batch_info(i)["solver_iter"] for i in range(len(uneven_batch))]))
+    (ld, xd, itd), (lh, xh, ith) = out
+    if coords:  # the same domains on both sides
+        # (slot counts may differ: the GPU build gives every 64-row slice a slot, empty ones too)
+        assert ld["nnzb"] == lh["nnzb"] and ld["npad"] == lh["npad"]
+        for i in range(len(xd)):
+            assert abs(itd[i] - ith[i]) <= 10
+            assert np.linalg.norm(xd[i] - xh[i]) <= 1e-6 * np.linalg.norm(xh[i])
