@@ -30,8 +30,9 @@
 //   spmv_unroll          SpMV row-loop batch depth 2 | 4 | 8 (chosen by psmas_autotune)
 //   spmv_copies          after factorize_copies, one SpMV launch reads the matrix once for every
 //                        right-hand side (default true; false: the batch's per-system SpMV)
-//   device_setup         build the SELL matrix of a single system on the GPU from the CSC arrays
-//                        (default true; false: the host build)
+//   device_setup         build the SELL matrix on the GPU from the CSC arrays, for one system, a
+//                        batch or the copies of factorize_copies, and aggregate there when no
+//                        coordinates are given (default true; false: the host build)
 //   spmv_halo            SpMV gathers x from a per-workgroup copy of the entries its rows read,
 //                        staged in LDS, with 16-bit local columns (default true; used when every
 //                        workgroup's set fits in SPMV_HALO_CAP nodes)
