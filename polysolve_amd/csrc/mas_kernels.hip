@@ -703,6 +703,15 @@ namespace psmas
             const int p = blockIdx.x * NODE_BLOCK + threadIdx.x;
             const int4 c = cmap[p];
             const int cc[4] = {c.x, c.y, c.z, c.w};
+            // r (and the previous z) do not depend on the coarse gathers: load them with the first
+            // round trip.
+            T rv[BD], zp[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+            {
+                rv[i] = r[static_cast<size_t>(p) * BD + i];
+                zp[i] = flex.zprev != nullptr ? flex.zprev[static_cast<size_t>(p) * BD + i] : T(0);
+            }
             double z[BD];
 #pragma unroll
             for (int i = 0; i < BD; ++i)
@@ -742,10 +751,10 @@ namespace psmas
             {
                 const T zi = static_cast<T>(z[i]);
                 mz[static_cast<size_t>(p) * BD + i] = zi;
-                const double ri = static_cast<double>(r[static_cast<size_t>(p) * BD + i]);
+                const double ri = static_cast<double>(rv[i]);
                 dot += ri * static_cast<double>(zi);
                 if (flex.zprev != nullptr)
-                    dot_old += ri * static_cast<double>(flex.zprev[static_cast<size_t>(p) * BD + i]);
+                    dot_old += ri * static_cast<double>(zp[i]);
             }
             store_partial<NODE_BLOCK>(dot, red.rz, blockIdx.x);
             if (flex.zprev != nullptr)

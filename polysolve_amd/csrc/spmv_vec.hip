@@ -413,7 +413,11 @@ namespace psmas
                 lc = lb.cell[slot];
                 lp = lb.pos[slot];
             }
-            // The block's vectors are loaded before alpha's reduction so both latencies overlap.
+            // The block's vectors and restriction map are loaded before alpha's reduction (whose
+            // barriers the loads would not cross) so that all these latencies overlap.
+            const size_t node = static_cast<size_t>(blockIdx.x) * VEC_BLOCK + threadIdx.x;
+            const uint32_t mask = fr.lead[node];
+            const int gid = fr.gid[node];
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
             T pv[BD], apv[BD], xv[BD], rv0[BD];
 #pragma unroll
@@ -438,9 +442,6 @@ namespace psmas
                 sv[j] = rv;
                 dot += static_cast<double>(rv) * static_cast<double>(rv);
             }
-            const size_t node = static_cast<size_t>(blockIdx.x) * VEC_BLOCK + threadIdx.x;
-            const uint32_t mask = fr.lead[node];
-            const int gid = fr.gid[node];
             __syncthreads();
             restrict_bank<BD, T>(sv, (threadIdx.x / 32) * 32, mask, gid, rc0);
             store_partial<VEC_BLOCK>(dot, red.rr, blockIdx.x);
