@@ -22,7 +22,7 @@ INCLUDE = HERE.parent / "include"
 BUILD_DIR = HERE.parent / "_build" / "psmas"
 ARCH = os.environ.get("PSMAS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["spmv_vec.hip", "mas_kernels.hip", "coarse_solve.hip", "mlp.hip", "lcc.hip", "sell_build.hip", "aggregate.hip", "solver.hip"]
+HIP_SOURCES = ["spmv_vec.hip", "mas_kernels.hip", "coarse_solve.hip", "mlp.hip", "lcc.hip", "sell_build.hip", "aggregate.hip", "dirichlet.hip", "solver.hip"]
 CXX_SOURCES = ["host_setup.cpp", "params.cpp", "capi.cpp", "mlp_capi.cpp"]
 
 # Packed-FP32 VALU forms are disabled for every HIP translation unit (hipcc warns that the host

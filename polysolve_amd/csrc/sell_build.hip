@@ -373,6 +373,20 @@ namespace psmas
         m.map_ok = false;
     }
 
+    DeviceBuffer<int> compose_value_map(const DeviceBuffer<int> &keep, size_t n, const DeviceBuffer<int> &map,
+                                        hipStream_t s)
+    {
+        DeviceBuffer<int> out(std::max<size_t>(n, 1), "MAS value map");
+        if (n > 0)
+        {
+            hipLaunchKernelGGL(k_compose_map, dim3(grid_for(static_cast<long long>(n))), dim3(TB), 0, s,
+                               static_cast<long long>(n), keep.data(), map.data(), out.data());
+            PSMAS_LAUNCH_CHECK();
+        }
+        PSMAS_HIP(hipStreamSynchronize(s));
+        return out;
+    }
+
     DeviceBuffer<int> compose_value_map(const std::vector<int> &keep, const DeviceBuffer<int> &map, hipStream_t s)
     {
         const size_t n = keep.size();

@@ -48,6 +48,9 @@ namespace psmas
     /// map[keep[k]] for every k (-1 where keep[k] < 0): the value index of each non-zero of a
     /// matrix whose non-zero k became non-zero keep[k] of the built one (or was dropped).
     DeviceBuffer<int> compose_value_map(const std::vector<int> &keep, const DeviceBuffer<int> &map, hipStream_t s);
+    /// The same with keep (n entries) in device memory.
+    DeviceBuffer<int> compose_value_map(const DeviceBuffer<int> &keep, size_t n, const DeviceBuffer<int> &map,
+                                        hipStream_t s);
 
     /// The block-diagonal matrix of the systems parts[p] (each built with npads[p] slots), one
     /// after another as the concatenated host build lays them out: system p's rows and columns

@@ -22,6 +22,7 @@
 // residual b - A x is recomputed with the FP64 matrix; convergence is decided on that FP64
 // residual only.
 #include "solver.hpp"
+#include "dirichlet.hpp"
 #include "sell_build.hpp"
 #include "aggregate.hpp"
 #include "coarse_solve.hpp"
@@ -393,7 +394,7 @@ namespace psmas
     }
 
     void Solver::factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
-                                const DeviceCsc *dev, bool replicate)
+                                const DeviceCsc *dev, bool replicate, const DeviceBuffer<int> *d_keep)
     {
         const int bd = params_.block_dim;
         factorized_ = false;
@@ -535,7 +536,7 @@ namespace psmas
         {
             if (K > 1 && !replicate)
                 phase("partition");
-            device_matrix_build(As[0], dcsc, dev != nullptr, replicate, h_slot_of_block, raw, dir, phase);
+            device_matrix_build(As[0], dcsc, dev != nullptr, replicate, h_slot_of_block, raw, dir, d_keep, phase);
         }
         else
         {
@@ -628,7 +629,7 @@ namespace psmas
     template <typename Phase>
     void Solver::device_matrix_build(const CscView &A, const std::vector<DeviceCsc> &dcsc, bool device_input,
                                      bool replicate, const std::vector<int> &h_slot_of_block, const CscView *raw,
-                                     const DirichletSystem *dir, Phase &phase)
+                                     const DirichletSystem *dir, const DeviceBuffer<int> *d_keep, Phase &phase)
     {
         System &S = sys_[0];
         const long long nnz = dcsc[0].nnz;
@@ -683,13 +684,14 @@ namespace psmas
             d_vmap_ = std::move(sd.d_map);
             vr_nnz_ = nnz;
             const CscView &P = raw != nullptr ? *raw : A;
-            if (dir != nullptr)
+            if (dir != nullptr || d_keep != nullptr)
             {
                 // Index the non-zeros of the matrix before elimination; dropped ones get -1.
                 vr_nnz_ = P.nnz;
-                d_vmap_ = compose_value_map(dir->keep, d_vmap_, stream_);
+                d_vmap_ = d_keep != nullptr ? compose_value_map(*d_keep, static_cast<size_t>(P.nnz), d_vmap_, stream_)
+                                            : compose_value_map(dir->keep, d_vmap_, stream_);
             }
-            if (!device_input)
+            if (!device_input || raw != nullptr) // (a host matrix before elimination)
             {
                 vr_colptr_.assign(P.colptr, P.colptr + P.n + 1);
                 vr_rowidx_.assign(P.rowidx, P.rowidx + P.nnz);
@@ -756,8 +758,8 @@ namespace psmas
     void Solver::scatter_new_values(const double *d_vals)
     {
         launch_scatter_values(vr_nnz_, d_vmap_.data(), d_vals, d_vals64_.data(), stream_);
-        if (dirichlet_)
-            launch_gather_values(static_cast<int>(d_cval_.size()), d_cidx_.data(), d_vals, d_cval_.data(), stream_);
+        if (dirichlet_ && n_coupl_ > 0) // (the coupling buffers hold one unused entry when there are none)
+            launch_gather_values(static_cast<int>(n_coupl_), d_cidx_.data(), d_vals, d_cval_.data(), stream_);
     }
 
     void Solver::dirichlet_factorize(const CscView &A, const int *nodes, int ndir)
@@ -774,6 +776,11 @@ namespace psmas
         list.erase(std::unique(list.begin(), list.end()), list.end());
         if (try_refresh_values(A, &list))
             return;
+        if (params_.device_setup)
+        {
+            dirichlet_factorize_device(A, list);
+            return;
+        }
         std::vector<unsigned char> fixed(A.n, 0);
         for (int d : list)
             fixed[d] = 1;
@@ -801,6 +808,48 @@ namespace psmas
             PSMAS_HIP(hipMemcpy(d_cval_.data(), cval.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
         }
         n_coupl_ = static_cast<long long>(nc);
+        dir_nodes_.swap(list);
+        dirichlet_ = true;
+    }
+
+    void Solver::dirichlet_factorize_device(const CscView &A, std::vector<int> &list)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        // A once in device memory: its values stay for values-only refactorisations.
+        const size_t nz = static_cast<size_t>(std::max<long long>(A.nnz, 1));
+        DeviceBuffer<int> cp(static_cast<size_t>(A.n) + 1, "MAS input pattern"), ri(nz, "MAS input pattern");
+        DeviceBuffer<int> dn(std::max<size_t>(list.size(), 1), "MAS Dirichlet nodes");
+        d_csc_vals_.ensure(nz, "MAS input values");
+        PSMAS_HIP(hipMemcpyAsync(cp.data(), A.colptr, sizeof(int) * (A.n + 1), hipMemcpyHostToDevice, stream_));
+        if (A.nnz > 0)
+        {
+            PSMAS_HIP(hipMemcpyAsync(ri.data(), A.rowidx, sizeof(int) * A.nnz, hipMemcpyHostToDevice, stream_));
+            PSMAS_HIP(hipMemcpyAsync(d_csc_vals_.data(), A.vals, sizeof(double) * A.nnz, hipMemcpyHostToDevice,
+                                     stream_));
+        }
+        if (!list.empty())
+            PSMAS_HIP(hipMemcpyAsync(dn.data(), list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice,
+                                     stream_));
+        DirichletDevice D;
+        eliminate_dirichlet_device(A.n, A.nnz, cp.data(), ri.data(), d_csc_vals_.data(), dn.data(),
+                                   static_cast<int>(list.size()), stream_, D);
+        const double elim_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        const CscView At{A.n, D.nnz, nullptr, nullptr, nullptr};
+        const DeviceCsc dv{A.n, D.nnz, D.colptr.data(), D.rowidx.data(), D.vals.data()};
+        factorize_full(std::vector<CscView>{At}, &A, nullptr, &dv, false, &D.keep);
+        setup_ms_.insert(setup_ms_.begin(), {"dirichlet_elimination", elim_ms});
+        n_crow_ = D.nrows;
+        n_coupl_ = D.ncoupl;
+        d_crow_ = std::move(D.crow);
+        d_cptr_ = std::move(D.cptr);
+        d_ccol_ = std::move(D.ccol);
+        d_cidx_ = std::move(D.cidx);
+        d_cval_.allocate(static_cast<size_t>(std::max<long long>(n_coupl_, 1)), "MAS Dirichlet couplings");
+        if (n_coupl_ > 0)
+            launch_gather_values(static_cast<int>(n_coupl_), d_cidx_.data(), d_csc_vals_.data(), d_cval_.data(),
+                                 stream_);
+        PSMAS_HIP(hipStreamSynchronize(stream_));
         dir_nodes_.swap(list);
         dirichlet_ = true;
     }

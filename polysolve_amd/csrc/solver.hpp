@@ -293,9 +293,11 @@ namespace psmas
         /// Full factorisation; for a Dirichlet one, As[0] is the eliminated matrix, raw the matrix
         /// before elimination (whose non-zeros the value map then indexes) and dir the elimination.
         /// replicate: every As[s] is As[0] (copies for several right-hand sides); system 0's
-        /// partition and matrix are built once and copied.
+        /// partition and matrix are built once and copied.  d_keep: a Dirichlet elimination made
+        /// on the GPU (dev = A′, raw = A, dir = nullptr) with its raw -> A′ map in device memory.
         void factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
-                            const DeviceCsc *dev = nullptr, bool replicate = false);
+                            const DeviceCsc *dev = nullptr, bool replicate = false,
+                            const DeviceBuffer<int> *d_keep = nullptr);
         /// The SELL matrix built on the GPU (sell_build.hpp) from the CSC arrays in device memory
         /// (dcsc[s] per system: uploaded, or the caller's arrays when device_input), for one
         /// system, `replicate` copies of system 0, or every system of a batch; also the halo
@@ -305,10 +307,13 @@ namespace psmas
         template <typename Phase>
         void device_matrix_build(const CscView &A, const std::vector<DeviceCsc> &dcsc, bool device_input,
                                  bool replicate, const std::vector<int> &h_slot_of_block, const CscView *raw,
-                                 const DirichletSystem *dir, Phase &phase);
+                                 const DirichletSystem *dir, const DeviceBuffer<int> *d_keep, Phase &phase);
         /// New SELL values (and coupling values of a Dirichlet factorisation) from d_vals in the
         /// recorded CSC order.
         void scatter_new_values(const double *d_vals);
+        /// The Dirichlet factorisation with the elimination on the GPU (device_setup); list holds
+        /// the sorted, unique fixed dofs and is taken over.
+        void dirichlet_factorize_device(const CscView &A, std::vector<int> &list);
         void dirichlet_lift_device(const double *d_f, double *d_g);
         void check_dirichlet(int n) const;
         /// The numeric rebuild after the SELL values were refreshed.

@@ -65,6 +65,8 @@ def test_dirichlet_factorize_matches_eliminated_matrix(gpu, bare, nodes, precisi
     host = _solver(bare, precision, device_setup=False)
     host.dirichlet_factorize(bare.A, nodes)
     assert np.array_equal(s.spmv(r), host.spmv(r))  # the GPU build gives the host build's matrix
+    assert info["dirichlet"] == host.get_info()["dirichlet"]  # the same coupling rows and entries
+    assert ("dirichlet_elimination" in info["setup_ms"]) == device_setup
 
 
 def test_lift_is_bit_exact_and_solution_matches_reduced_solve(gpu, bare, nodes):
@@ -158,3 +160,23 @@ def test_dirichlet_errors(gpu, bare, nodes):
     s.factorize(DO.eliminate(bare.A, nodes))  # a plain factorisation ends the Dirichlet mode
     with pytest.raises(MASError, match="no Dirichlet factorisation"):
         s.dirichlet_solve(bare.b.copy(), np.zeros(bare.n))
+
+
+@pytest.mark.parametrize("device_setup", [True, False])
+def test_no_fixed_dofs_and_values_refresh(gpu, device_setup):
+    """An empty fixed set (no couplings) on a non-singular system (bottom face fixed in the
+    assembly) factorises, refreshes its values and solves like a plain factorisation."""
+    p = problems.hex_elasticity(8, E=problems.random_moduli(8, 6), name="fixed8")
+    d = np.random.default_rng(6).uniform(0.5, 2.0, p.n)
+    A2 = (p.A.multiply(d[:, None]).multiply(d[None, :])).tocsc()
+    A2.sort_indices()
+    empty = np.zeros(0, dtype=np.int32)
+    s = _solver(p, lazy=True, device_setup=device_setup)
+    s.dirichlet_factorize(p.A, empty)
+    assert s.get_info()["dirichlet"]["couplings"] == 0
+    s.dirichlet_factorize(A2, empty)
+    assert s.get_info()["factorize_mode"] == "values"
+    g, u = p.b.copy(), np.zeros(p.n)
+    s.dirichlet_solve(g, u)
+    assert np.array_equal(g, p.b)
+    assert np.linalg.norm(p.b - A2 @ u) <= 1e-9 * np.linalg.norm(p.b)
