@@ -94,6 +94,15 @@ const char *psmas_last_error(psmas_handle h);
 int psmas_dirichlet_factorize_csc(psmas_handle h, int n, long long nnz, const int *colptr, const int *rowidx,
                                   const double *vals, int ndir, const int *dirichlet_nodes);
 
+/* The same with A's CSC arrays in device memory (HIP pointers on the solver's device; validated
+ * there, as in psmas_factorize_csc_device); dirichlet_nodes stays a host array.  The elimination
+ * and the matrix build run on the GPU and the host never receives A; the values of later Newton
+ * steps follow through psmas_factorize_values_device ("lazy_partitioning": true).  Replaces
+ * prefactorize (FEMSolver.cpp:269-315) for callers that assemble on the GPU. */
+int psmas_dirichlet_factorize_csc_device(psmas_handle h, int n, long long nnz, const int *d_colptr,
+                                         const int *d_rowidx, const double *d_vals, int ndir,
+                                         const int *dirichlet_nodes);
+
 /* dirichlet_solve_prefactorized (FEMSolver.cpp:318-342): f (length n, host) is replaced by the
  * lifted right-hand side g = f - (I - N) A N f, computed on the GPU with the A of the last
  * psmas_dirichlet_factorize_csc, and x receives the solution of A' x = g (x[i] = f[i] at a fixed

@@ -49,6 +49,7 @@ SIGNATURES = {
     "psmas_solve_multi": (_i, [_vp, _i, _i, _pd, _pd]),
     "psmas_solve_multi_device": (_i, [_vp, _i, _i, _vp, _vp]),
     "psmas_dirichlet_factorize_csc": (_i, [_vp, _i, _ll, _pi, _pi, _pd, _i, _pi]),
+    "psmas_dirichlet_factorize_csc_device": (_i, [_vp, _i, _ll, _vp, _vp, _vp, _i, _pi]),
     "psmas_dirichlet_solve": (_i, [_vp, _i, _pd, _pd]),
     "psmas_dirichlet_solve_device": (_i, [_vp, _i, _vp, _vp]),
     "psmas_set_coordinates": (_i, [_vp, _i, _i, _pd]),

@@ -171,6 +171,17 @@ extern "C"
         });
     }
 
+    int psmas_dirichlet_factorize_csc_device(psmas_handle h, int n, long long nnz, const int *d_colptr,
+                                             const int *d_rowidx, const double *d_vals, int ndir,
+                                             const int *dirichlet_nodes)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            h->solver.dirichlet_factorize_device_input(n, nnz, d_colptr, d_rowidx, d_vals, dirichlet_nodes, ndir);
+        });
+    }
+
     int psmas_dirichlet_solve(psmas_handle h, int n, double *f, double *x)
     {
         if (!h)

@@ -691,7 +691,7 @@ namespace psmas
                 d_vmap_ = d_keep != nullptr ? compose_value_map(*d_keep, static_cast<size_t>(P.nnz), d_vmap_, stream_)
                                             : compose_value_map(dir->keep, d_vmap_, stream_);
             }
-            if (!device_input || raw != nullptr) // (a host matrix before elimination)
+            if (P.colptr != nullptr) // (host input: the pattern for values-only refactorisations)
             {
                 vr_colptr_.assign(P.colptr, P.colptr + P.n + 1);
                 vr_rowidx_.assign(P.rowidx, P.rowidx + P.nnz);
@@ -766,14 +766,7 @@ namespace psmas
     {
         PSMAS_HIP(hipSetDevice(device_));
         validate_csc(A);
-        if (ndir < 0 || (ndir > 0 && nodes == nullptr))
-            throw InvalidArgumentError("[MAS] invalid Dirichlet node list.");
-        std::vector<int> list(nodes, nodes + ndir);
-        for (int d : list)
-            if (d < 0 || d >= A.n)
-                throw InvalidArgumentError("[MAS] Dirichlet node index out of range.");
-        std::sort(list.begin(), list.end());
-        list.erase(std::unique(list.begin(), list.end()), list.end());
+        std::vector<int> list = dirichlet_list(A.n, nodes, ndir);
         if (try_refresh_values(A, &list))
             return;
         if (params_.device_setup)
@@ -818,7 +811,6 @@ namespace psmas
         // A once in device memory: its values stay for values-only refactorisations.
         const size_t nz = static_cast<size_t>(std::max<long long>(A.nnz, 1));
         DeviceBuffer<int> cp(static_cast<size_t>(A.n) + 1, "MAS input pattern"), ri(nz, "MAS input pattern");
-        DeviceBuffer<int> dn(std::max<size_t>(list.size(), 1), "MAS Dirichlet nodes");
         d_csc_vals_.ensure(nz, "MAS input values");
         PSMAS_HIP(hipMemcpyAsync(cp.data(), A.colptr, sizeof(int) * (A.n + 1), hipMemcpyHostToDevice, stream_));
         if (A.nnz > 0)
@@ -827,11 +819,46 @@ namespace psmas
             PSMAS_HIP(hipMemcpyAsync(d_csc_vals_.data(), A.vals, sizeof(double) * A.nnz, hipMemcpyHostToDevice,
                                      stream_));
         }
+        dirichlet_eliminate_and_factorize(A, DeviceCsc{A.n, A.nnz, cp.data(), ri.data(), d_csc_vals_.data()}, list,
+                                          t0);
+    }
+
+    void Solver::dirichlet_factorize_device_input(int n, long long nnz, const int *d_colptr, const int *d_rowidx,
+                                                  const double *d_vals, const int *nodes, int ndir)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        const auto t0 = std::chrono::steady_clock::now();
+        validate_csc_device(n, nnz, d_colptr, d_rowidx, stream_);
+        if (nnz > 0 && d_vals == nullptr)
+            throw InvalidArgumentError("[MAS] invalid A: empty matrix or null arrays.");
+        std::vector<int> list = dirichlet_list(n, nodes, ndir);
+        // The host never sees A: later values come through psmas_factorize_values_device.
+        dirichlet_eliminate_and_factorize(CscView{n, nnz, nullptr, nullptr, nullptr},
+                                          DeviceCsc{n, nnz, d_colptr, d_rowidx, d_vals}, list, t0);
+    }
+
+    std::vector<int> Solver::dirichlet_list(int n, const int *nodes, int ndir)
+    {
+        if (ndir < 0 || (ndir > 0 && nodes == nullptr))
+            throw InvalidArgumentError("[MAS] invalid Dirichlet node list.");
+        std::vector<int> list(nodes, nodes + ndir);
+        for (int d : list)
+            if (d < 0 || d >= n)
+                throw InvalidArgumentError("[MAS] Dirichlet node index out of range.");
+        std::sort(list.begin(), list.end());
+        list.erase(std::unique(list.begin(), list.end()), list.end());
+        return list;
+    }
+
+    void Solver::dirichlet_eliminate_and_factorize(const CscView &A, const DeviceCsc &dA, std::vector<int> &list,
+                                                   std::chrono::steady_clock::time_point t0)
+    {
+        DeviceBuffer<int> dn(std::max<size_t>(list.size(), 1), "MAS Dirichlet nodes");
         if (!list.empty())
             PSMAS_HIP(hipMemcpyAsync(dn.data(), list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice,
                                      stream_));
         DirichletDevice D;
-        eliminate_dirichlet_device(A.n, A.nnz, cp.data(), ri.data(), d_csc_vals_.data(), dn.data(),
+        eliminate_dirichlet_device(dA.n, dA.nnz, dA.colptr, dA.rowidx, dA.vals, dn.data(),
                                    static_cast<int>(list.size()), stream_, D);
         const double elim_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -847,8 +874,7 @@ namespace psmas
         d_cidx_ = std::move(D.cidx);
         d_cval_.allocate(static_cast<size_t>(std::max<long long>(n_coupl_, 1)), "MAS Dirichlet couplings");
         if (n_coupl_ > 0)
-            launch_gather_values(static_cast<int>(n_coupl_), d_cidx_.data(), d_csc_vals_.data(), d_cval_.data(),
-                                 stream_);
+            launch_gather_values(static_cast<int>(n_coupl_), d_cidx_.data(), dA.vals, d_cval_.data(), stream_);
         PSMAS_HIP(hipStreamSynchronize(stream_));
         dir_nodes_.swap(list);
         dirichlet_ = true;

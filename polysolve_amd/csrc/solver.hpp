@@ -20,6 +20,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <chrono>
 #include <vector>
 
 namespace psmas
@@ -120,6 +121,9 @@ namespace psmas
         /// couplings to the fixed dofs for the right-hand side lift.  With lazy_partitioning and the
         /// same pattern and node set as the last such call, only the values are refreshed.
         void dirichlet_factorize(const CscView &A, const int *nodes, int ndir);
+        /// The same with A's CSC arrays in device memory (validated there); nodes on the host.
+        void dirichlet_factorize_device_input(int n, long long nnz, const int *d_colptr, const int *d_rowidx,
+                                              const double *d_vals, const int *nodes, int ndir);
         /// dirichlet_solve_prefactorized (FEMSolver.cpp:318-342): f := g = f - (I - N) A N f with
         /// the A of the last Dirichlet factorisation, then u = solve(g).
         void dirichlet_solve_host(int n, double *f, double *u);
@@ -314,6 +318,13 @@ namespace psmas
         /// The Dirichlet factorisation with the elimination on the GPU (device_setup); list holds
         /// the sorted, unique fixed dofs and is taken over.
         void dirichlet_factorize_device(const CscView &A, std::vector<int> &list);
+        /// Sorted, unique fixed dofs from a caller's list (checked against n).
+        static std::vector<int> dirichlet_list(int n, const int *nodes, int ndir);
+        /// Elimination of the fixed dofs in list (taken over) from dA (A in device memory; A's
+        /// host arrays, when not null, give the pattern for values-only refactorisations), then
+        /// the factorisation of A′; t0 starts the elimination phase's clock.
+        void dirichlet_eliminate_and_factorize(const CscView &A, const DeviceCsc &dA, std::vector<int> &list,
+                                               std::chrono::steady_clock::time_point t0);
         void dirichlet_lift_device(const double *d_f, double *d_g);
         void check_dirichlet(int n) const;
         /// The numeric rebuild after the SELL values were refreshed.

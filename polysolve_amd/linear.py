@@ -175,6 +175,17 @@ class MASSolver(Solver):
                                                      int(nodes.size), _ptr(nodes, C.c_int)), self._h)
         self._n = n
 
+    def dirichlet_factorize_device(self, colptr_ptr: int, rowidx_ptr: int, vals_ptr: int, n: int, nnz: int,
+                                   dirichlet_nodes) -> None:
+        """dirichlet_factorize with A's CSC arrays already in device memory (raw HIP pointers:
+        int32 colptr of n+1, int32 rowidx and float64 vals of nnz entries); the fixed dofs are a
+        host array."""
+        nodes = np.ascontiguousarray(dirichlet_nodes, dtype=np.int32).ravel()
+        _check(N.lib().psmas_dirichlet_factorize_csc_device(self._h, int(n), int(nnz), C.c_void_p(colptr_ptr),
+                                                            C.c_void_p(rowidx_ptr), C.c_void_p(vals_ptr),
+                                                            int(nodes.size), _ptr(nodes, C.c_int)), self._h)
+        self._n = int(n)
+
     def dirichlet_solve(self, f, u) -> None:
         """dirichlet_solve_prefactorized (FEMSolver.cpp:318-342): f := g = f - (I - N) A N f, then
         u = solve(g); f and u are contiguous float64 arrays written in place."""

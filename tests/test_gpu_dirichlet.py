@@ -180,3 +180,39 @@ def test_no_fixed_dofs_and_values_refresh(gpu, device_setup):
     s.dirichlet_solve(g, u)
     assert np.array_equal(g, p.b)
     assert np.linalg.norm(p.b - A2 @ u) <= 1e-9 * np.linalg.norm(p.b)
+
+
+def test_device_input_dirichlet_factorisation(gpu, bare, nodes):
+    """A's CSC arrays in device memory: the same eliminated matrix as the host-input call, the same
+    lift, and Newton values from device memory."""
+    n, colptr, rowidx, vals = _as_csc(bare.A)
+    dc, dr, dv = DeviceArray(n + 1, colptr, np.int32), DeviceArray(rowidx.size, rowidx, np.int32), \
+        DeviceArray(vals.size, vals)
+    s = _solver(bare, lazy=True)
+    s.dirichlet_factorize_device(dc.ptr, dr.ptr, dv.ptr, n, rowidx.size, nodes)
+    host = _solver(bare, lazy=True)
+    host.dirichlet_factorize(bare.A, nodes)
+    assert s.get_info()["dirichlet"] == host.get_info()["dirichlet"]
+    r = np.random.default_rng(8).standard_normal(n)
+    assert np.array_equal(s.spmv(r), host.spmv(r))
+    f = _f(bare, nodes, 4)
+    g, u = f.copy(), np.zeros(n)
+    s.dirichlet_solve(g, u)
+    assert np.array_equal(g, DO.lift(bare.A, nodes, f))
+    assert _rel(u, DO.reduced_solution(bare.A, nodes, f)) <= 1e-6
+    d = np.random.default_rng(9).uniform(0.5, 2.0, n)
+    A2 = (bare.A.multiply(d[:, None]).multiply(d[None, :])).tocsc()
+    A2.sort_indices()
+    dv.upload(_as_csc(A2)[3])
+    s.factorize_values_device(dv.ptr, rowidx.size)
+    assert s.get_info()["factorize_mode"] == "values"
+    g, u = f.copy(), np.zeros(n)
+    s.dirichlet_solve(g, u)
+    assert np.array_equal(g, DO.lift(A2, nodes, f))
+    bad = rowidx.copy()
+    bad[5] = n
+    dr.upload(bad)
+    with pytest.raises(MASError, match="row index out of range"):
+        s.dirichlet_factorize_device(dc.ptr, dr.ptr, dv.ptr, n, rowidx.size, nodes)
+    for a in (dc, dr, dv):
+        a.free()
