@@ -150,6 +150,10 @@ int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d
  * the other (column-major n x nrhs), host arrays or device pointers (_device). */
 int psmas_factorize_multi_csc(psmas_handle h, int nrhs, int n, long long nnz, const int *colptr, const int *rowidx,
                               const double *vals);
+/* The same with A's CSC arrays in device memory (HIP pointers on the solver's device, validated
+ * there); the matrix is built once on the GPU and copied for every right-hand side. */
+int psmas_factorize_multi_csc_device(psmas_handle h, int nrhs, int n, long long nnz, const int *d_colptr,
+                                     const int *d_rowidx, const double *d_vals);
 int psmas_solve_multi(psmas_handle h, int n, int nrhs, const double *B, double *X);
 int psmas_solve_multi_device(psmas_handle h, int n, int nrhs, const double *d_B, double *d_X);
 

@@ -46,6 +46,7 @@ SIGNATURES = {
     "psmas_factorize_values_device": (_i, [_vp, _ll, _vp]),
     "psmas_factorize_csc_device": (_i, [_vp, _i, _ll, _vp, _vp, _vp]),
     "psmas_factorize_multi_csc": (_i, [_vp, _i, _i, _ll, _pi, _pi, _pd]),
+    "psmas_factorize_multi_csc_device": (_i, [_vp, _i, _i, _ll, _vp, _vp, _vp]),
     "psmas_solve_multi": (_i, [_vp, _i, _i, _pd, _pd]),
     "psmas_solve_multi_device": (_i, [_vp, _i, _i, _vp, _vp]),
     "psmas_dirichlet_factorize_csc": (_i, [_vp, _i, _ll, _pi, _pi, _pd, _i, _pi]),

@@ -220,6 +220,18 @@ extern "C"
         });
     }
 
+    int psmas_factorize_multi_csc_device(psmas_handle h, int nrhs, int n, long long nnz, const int *d_colptr,
+                                         const int *d_rowidx, const double *d_vals)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            const psmas::CscView A{n, nnz, nullptr, nullptr, nullptr};
+            const psmas::DeviceCsc dv{n, nnz, d_colptr, d_rowidx, d_vals};
+            h->solver.factorize_copies(A, nrhs, &dv);
+        });
+    }
+
     static int solve_multi(psmas_handle h, int n, int nrhs, const double *B, double *X, bool device)
     {
         if (!h)

@@ -433,7 +433,7 @@ namespace psmas
         std::vector<DeviceBuffer<int>> up_cp(nup), up_ri(nup);
         std::vector<DeviceBuffer<double>> up_v(nup);
         if (gpu_build && dev != nullptr)
-            dcsc[0] = *dev;
+            std::fill(dcsc.begin(), dcsc.end(), *dev); // (one system, or copies of it)
         else if (gpu_build)
         {
             for (int u = 0; u < nup; ++u)
@@ -712,12 +712,19 @@ namespace psmas
         factorize_full(std::vector<CscView>{A}, nullptr, nullptr, &dv);
     }
 
-    void Solver::factorize_copies(const CscView &A, int copies)
+    void Solver::factorize_copies(const CscView &A, int copies, const DeviceCsc *dev)
     {
         PSMAS_HIP(hipSetDevice(device_));
         if (copies < 1 || copies > MAX_COPIES)
             throw InvalidArgumentError("[MAS] invalid number of right-hand sides (1 .. 8).");
-        validate_csc(A);
+        if (dev != nullptr)
+        {
+            validate_csc_device(dev->n, dev->nnz, dev->colptr, dev->rowidx, stream_);
+            if (dev->nnz > 0 && dev->vals == nullptr)
+                throw InvalidArgumentError("[MAS] invalid A: empty matrix or null arrays.");
+        }
+        else
+            validate_csc(A);
         // System 0's coordinates (if any) go to every copy, so that all copies get one partition.
         if (!sys_.empty() && sys_[0].coord_dim > 0)
         {
@@ -729,7 +736,7 @@ namespace psmas
                 sys_[c].coord_dim = S0.coord_dim;
             }
         }
-        factorize_full(std::vector<CscView>(copies, A), nullptr, nullptr, nullptr, true);
+        factorize_full(std::vector<CscView>(copies, A), nullptr, nullptr, dev, true);
         for (int c = 1; c < copies; ++c)
             if (sys_[c].npad != sys_[0].npad || sys_[c].part.perm != sys_[0].part.perm)
                 return; // (not expected: the copies keep their own SpMV then)

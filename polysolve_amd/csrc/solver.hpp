@@ -110,7 +110,7 @@ namespace psmas
         /// Several right-hand sides of one matrix: a batch of `copies` (<= MAX_COPIES) systems
         /// that repeat A, whose SpMV reads the matrix once for all of them.  solve_batch then
         /// takes the right-hand sides.
-        void factorize_copies(const CscView &A, int copies);
+        void factorize_copies(const CscView &A, int copies, const DeviceCsc *dev = nullptr);
         static constexpr int MAX_COPIES = 8;
         /// Full factorisation of one system from CSC arrays in device memory (the matrix is built
         /// on the GPU; the host sees the pattern only for a graph aggregation).

@@ -216,6 +216,13 @@ class MASSolver(Solver):
                                                  _ptr(rowidx, C.c_int), _ptr(vals, C.c_double)), self._h)
         self._n = n
 
+
+    def factorize_multi_device(self, colptr_ptr: int, rowidx_ptr: int, vals_ptr: int, n: int, nnz: int,
+                               nrhs: int) -> None:
+        """factorize_multi with A's CSC arrays already in device memory (raw HIP pointers)."""
+        _check(N.lib().psmas_factorize_multi_csc_device(self._h, int(nrhs), int(n), int(nnz), C.c_void_p(colptr_ptr),
+                                                        C.c_void_p(rowidx_ptr), C.c_void_p(vals_ptr)), self._h)
+        self._n = int(n)
     def solve_multi(self, B) -> np.ndarray:
         """Solve A X = B for the nrhs columns of B (n x nrhs) after factorize_multi."""
         B = np.asfortranarray(B, dtype=np.float64)

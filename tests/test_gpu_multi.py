@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from polysolve_amd import problems
-from polysolve_amd.linear import DeviceArray, MASError, MASSolver
+from polysolve_amd.linear import DeviceArray, MASError, MASSolver, _as_csc
 
 pytestmark = pytest.mark.gpu
 
@@ -87,3 +87,30 @@ def test_multi_rhs_replicated_device_build(gpu, het12, coords):
         if coords:  # the same domains on both sides
             assert abs(itd[c] - ith[c]) <= 10
             assert np.linalg.norm(Xd[:, c] - Xh[:, c]) <= 1e-6 * np.linalg.norm(Xh[:, c])
+
+
+def test_multi_rhs_device_input(gpu, het12):
+    """factorize_multi from CSC arrays in device memory solves like the host-input call."""
+    k = 3
+    B = np.random.default_rng(11).standard_normal((het12.n, k))
+    n, colptr, rowidx, vals = _as_csc(het12.A)
+    dc, dr, dv = DeviceArray(n + 1, colptr, np.int32), DeviceArray(rowidx.size, rowidx, np.int32), \
+        DeviceArray(vals.size, vals)
+    s = MASSolver()
+    s.set_parameters(_prm("mixed"))
+    s.set_coordinates(het12.coords)
+    s.factorize_multi_device(dc.ptr, dr.ptr, dv.ptr, n, rowidx.size, k)
+    assert s.get_info()["copies"] == k
+    X = s.solve_multi(B)
+    ref = MASSolver()
+    ref.set_parameters(_prm("mixed"))
+    ref.set_coordinates(het12.coords)
+    ref.factorize_multi(het12.A, k)
+    Xr = ref.solve_multi(B)
+    for c in range(k):
+        assert np.linalg.norm(B[:, c] - het12.A @ X[:, c]) <= 1e-8 * np.linalg.norm(B[:, c])
+        assert np.linalg.norm(X[:, c] - Xr[:, c]) <= 1e-6 * np.linalg.norm(Xr[:, c])
+    with pytest.raises(MASError, match="right-hand sides"):
+        s.factorize_multi_device(dc.ptr, dr.ptr, dv.ptr, n, rowidx.size, 0)
+    for a in (dc, dr, dv):
+        a.free()
