@@ -19,6 +19,12 @@ namespace psmas
     {
         if (nt == 0)
             return; // uniform over the block
+        // The targets' term ranges (nt <= LCC_TMAX) go to LDS with one load per thread, issued
+        // before the terms are written, so the waves' loops below make no global round trips.
+        __shared__ int tp[LCC_TMAX + 1];
+        const int base = lb.tbase[b];
+        if (threadIdx.x <= nt)
+            tp[threadIdx.x] = lb.tptr[base + threadIdx.x];
         if (c.x >= 0)
         {
             const float fx = __int_as_float(c.y), fy = __int_as_float(c.z), fz = __int_as_float(c.w);
@@ -34,12 +40,11 @@ namespace psmas
             }
         }
         __syncthreads();
-        const int base = lb.tbase[b];
-        const int e_begin = lb.tptr[base];
+        const int e_begin = tp[0];
         const int lane = threadIdx.x % WAVE;
         for (int t = threadIdx.x / WAVE; t < nt; t += 256 / WAVE)
         {
-            const int e0 = lb.tptr[base + t] - e_begin, e1 = lb.tptr[base + t + 1] - e_begin;
+            const int e0 = tp[t] - e_begin, e1 = tp[t + 1] - e_begin;
             double a[BD];
 #pragma unroll
             for (int d = 0; d < BD; ++d)
