@@ -557,8 +557,8 @@ extern "C"
             if (!first)
                 o << ", ";
             first = false;
-            o << "\"" << t.name << "\": {\"launches\": " << t.launches << ", \"total_ms\": " << t.total_ms
-              << ", \"bytes\": " << t.bytes_per_launch << "}";
+            o << "\"" << t.name << "\": {\"launches\": " << t.launches << ", \"total_ms\": " << psmas::json_number(t.total_ms)
+              << ", \"bytes\": " << psmas::json_number(t.bytes_per_launch) << "}";
         }
         o << "}";
         return write_string(o.str(), buf, cap, needed);

@@ -7,8 +7,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <initializer_list>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -45,6 +48,34 @@ namespace psmas
 
     /// Kernel launch check: reports launch-configuration errors at the launch site.
 #define PSMAS_LAUNCH_CHECK() PSMAS_HIP(hipGetLastError())
+
+    /// Host-side check before a kernel launch: every listed device pointer must be set.  A kernel
+    /// that reads or writes through a null pointer faults the GPU (a pointer taken from a buffer
+    /// before its allocation is the typical cause), so the launch is refused with an error that
+    /// names it and the argument's position in the list (0-based).
+    inline void require_device_ptrs(const char *launch, std::initializer_list<const void *> ptrs)
+    {
+        int i = 0;
+        for (const void *p : ptrs)
+        {
+            if (p == nullptr)
+                throw std::logic_error(std::string("[MAS] internal error: null device pointer (argument ") +
+                                       std::to_string(i) + ") passed to " + launch);
+            ++i;
+        }
+    }
+
+    /// A double as JSON text: a number with round-trip precision, or null when it is not finite
+    /// (JSON has no NaN or infinity; nlohmann's serialiser writes null for them too).
+    inline std::string json_number(double v, int precision = 17)
+    {
+        if (!std::isfinite(v))
+            return "null";
+        std::ostringstream o;
+        o.precision(precision);
+        o << v;
+        return o.str();
+    }
 
     constexpr int div_up(int a, int b) { return (a + b - 1) / b; }
     constexpr long long div_up_ll(long long a, long long b) { return (a + b - 1) / b; }

@@ -212,6 +212,7 @@ namespace psmas
     struct ApplySide
     {
         int wgs = 0;
+        int apply_cap = 0; ///< > 0: at most this many matrix workgroups (host launch choice only)
         // Restriction of coarse segments 2.. from segment 1: task i sums the segment-1 values of
         // rlist[rtask[i].y .. rtask[i].z) (ascending node ids) in FP64.  Tasks [0, nsmall) take a
         // thread each, [nsmall, nsmall + nbig) a wave each (lanes stride the list, fixed-order

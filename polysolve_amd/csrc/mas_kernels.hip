@@ -1211,6 +1211,11 @@ namespace psmas
     {
         if (count <= 0 && side.wgs <= 0)
             return;
+        require_device_ptrs("mas_apply", {mr, mz});
+        if (count > 0)
+            require_device_ptrs("mas_apply (matrices)",
+                                {mats, std::is_same<TM, __half>::value ? static_cast<const void *>(dscale)
+                                                                       : static_cast<const void *>(mats)});
         // Apply workgroups: as many as are resident at once beside the side workgroups (registers
         // and LDS decide); each walks its matrices grid-stride.  Side workgroups come first so the
         // dispatcher starts them at once.
@@ -1222,7 +1227,9 @@ namespace psmas
             return (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
         };
         auto grid_for = [&](int resident) {
-            const int avail = resident - side.wgs > 1 ? resident - side.wgs : 1;
+            int avail = resident - side.wgs > 1 ? resident - side.wgs : 1;
+            if (side.apply_cap > 0 && side.apply_cap < avail)
+                avail = side.apply_cap;
             const int apply_wgs = count <= 0 ? 0 : (count < avail ? count : avail);
             return side.wgs + apply_wgs;
         };
@@ -1264,6 +1271,7 @@ namespace psmas
                             const Reductions &red, bool init, const SysMap &sm, const LccApply &lcc,
                             const FlexDot<T> &flex, hipStream_t s)
     {
+        require_device_ptrs("prolong_dot", {cmap, lev_off, r, mz});
         hipLaunchKernelGGL((k_prolong_dot<BD, T>), dim3(nblocks), dim3(NODE_BLOCK), 0, s, level_num, cmap, lev_off, r,
                            mz, red, init, sm, lcc, flex);
         PSMAS_LAUNCH_CHECK();

@@ -187,6 +187,7 @@ namespace psmas
         void gemm(int M, int N, int K, const AT *A, int lda, const BT *B, int ldb, float *C, int ldc,
                   const float *bias, bool relu, hipStream_t s, bool accum = false)
         {
+            require_device_ptrs("mlp gemm", {A, B, C});
             const dim3 grid(div_up(N, TN), div_up(M, TM));
             hipLaunchKernelGGL((k_gemm<AT, TA, BT, TB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias,
                                relu ? 1 : 0, accum ? 1 : 0);
@@ -300,6 +301,7 @@ namespace psmas
         const long long cnt = static_cast<long long>(B) * d_out_;
         // loss = sum (Y - T)^2 / (B * d_out); dL/dY = 2 (Y - T) / (B * d_out)
         const float scale = 2.0f / static_cast<float>(cnt);
+        require_device_ptrs("mlp mse", {y_.data(), T, dy_.data(), loss_.data()});
         hipLaunchKernelGGL(k_mse, dim3(1), dim3(1024), 0, stream_, cnt, y_.data(), T, scale, dy_.data(), loss_.data());
         PSMAS_LAUNCH_CHECK();
         float *g = g_.data();
@@ -341,6 +343,7 @@ namespace psmas
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
         const float c1 = 1.0f - std::pow(b1, static_cast<float>(step_));
         const float c2 = 1.0f - std::pow(b2, static_cast<float>(step_));
+        require_device_ptrs("mlp adam", {w_.data(), g_.data(), m_.data(), v_.data()});
         hipLaunchKernelGGL(k_adam, dim3(grid1(nparams_)), dim3(256), 0, stream_, static_cast<long long>(nparams_),
                            w_.data(), wb_.data(), g_.data(), m_.data(), v_.data(), lr, b1, b2, eps, c1, c2,
                            weight_decay);
@@ -380,6 +383,7 @@ namespace psmas
             {
                 PSMAS_HIP(hipMemcpyAsync(idx.data(), order.data() + b0, sizeof(int) * batch, hipMemcpyHostToDevice,
                                          stream_));
+                require_device_ptrs("mlp gather rows", {idx.data(), X, T, xb.data(), tb.data()});
                 hipLaunchKernelGGL(k_gather_rows, dim3(batch), dim3(256), 0, stream_, idx.data(), X, d_in_,
                                    xb.data());
                 hipLaunchKernelGGL(k_gather_rows, dim3(batch), dim3(256), 0, stream_, idx.data(), T, d_out_,

@@ -254,6 +254,8 @@ namespace psmas
         {
             if (v.kind != JValue::Number)
                 bad(key, "expected a number");
+            if (!std::isfinite(v.num)) // e.g. 1e999: out of double range
+                bad(key, "expected a finite number");
             return v.num;
         }
         bool as_bool(const JValue &v, const std::string &key)
@@ -275,7 +277,7 @@ namespace psmas
             "reliable_update_delta", "check_interval", "true_residual_period", "use_graphs",
             "learned_correction", "division_guard", "learned_collect", "lattice_points", "learned_hidden",
             "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll", "spmv_halo", "device_setup", "spmv_copies",
-            "restrict_piece"};
+            "restrict_piece", "apply_workgroups"};
     } // namespace
 
     void apply_json_parameters(Params &p, const std::string &json)
@@ -421,6 +423,12 @@ namespace psmas
             if (q.restrict_piece <= RESTRICT_SMALL_MAX || q.restrict_piece > RESTRICT_PIECE)
                 bad("restrict_piece", "must be in [9, 512]");
         }
+        if (const JValue *v = obj->get("apply_workgroups"))
+        {
+            q.apply_workgroups = as_int(*v, "apply_workgroups");
+            if (q.apply_workgroups < 0 || q.apply_workgroups > 1000000)
+                bad("apply_workgroups", "must be in [0, 1000000]");
+        }
         if (const JValue *v = obj->get("learned_output_scale"))
         {
             q.learned_output_scale = as_double(*v, "learned_output_scale");
@@ -441,7 +449,7 @@ namespace psmas
         std::ostringstream o;
         o.precision(17);
         o << "{\"block_dim\": " << p.block_dim << ", \"max_iter\": " << p.max_iter
-          << ", \"relative_tolerance\": " << p.rel_tol << ", \"absolute_tolerance\": " << p.abs_tol
+          << ", \"relative_tolerance\": " << json_number(p.rel_tol) << ", \"absolute_tolerance\": " << json_number(p.abs_tol)
           << ", \"lazy_partitioning\": " << (p.lazy_partitioning ? "true" : "false")
           << ", \"use_preconditioned_residual_norm\": " << (p.use_preconditioned_residual_norm ? "true" : "false")
           << ", \"true_residual_period\": " << p.true_residual_period
@@ -449,18 +457,18 @@ namespace psmas
           << ", \"aggregation\": \""
           << (p.aggregation == Aggregation::Auto ? "auto" : (p.aggregation == Aggregation::Morton ? "morton" : "graph"))
           << "\", \"coarse_solver\": \"" << (p.coarse_solver == CoarseSolver::Bank ? "bank" : "cholesky") << "\""
-          << ", \"reliable_update_delta\": " << p.reliable_update_delta
+          << ", \"reliable_update_delta\": " << json_number(p.reliable_update_delta)
           << ", \"check_interval\": " << p.check_interval
           << ", \"use_graphs\": " << (p.use_graphs ? "true" : "false")
           << ", \"learned_correction\": " << (p.learned_correction ? "true" : "false")
-          << ", \"division_guard\": " << p.division_guard
+          << ", \"division_guard\": " << json_number(p.division_guard)
           << ", \"learned_collect\": " << (p.learned_collect ? "true" : "false")
           << ", \"lattice_points\": " << p.lattice_points << ", \"learned_hidden\": " << p.learned_hidden
           << ", \"learned_random_pairs\": " << p.learned_random_pairs
           << ", \"learned_snapshots\": " << p.learned_snapshots
           << ", \"learned_stall_segments\": " << p.learned_stall_segments
-          << ", \"learned_output_scale\": " << p.learned_output_scale << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"spmv_halo\": " << (p.spmv_halo ? "true" : "false") << ", \"device_setup\": " << (p.device_setup ? "true" : "false") << ", \"spmv_copies\": " << (p.spmv_copies ? "true" : "false") << ", \"restrict_piece\": " << p.restrict_piece
-          << "}";
+          << ", \"learned_output_scale\": " << json_number(p.learned_output_scale) << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"spmv_halo\": " << (p.spmv_halo ? "true" : "false") << ", \"device_setup\": " << (p.device_setup ? "true" : "false") << ", \"spmv_copies\": " << (p.spmv_copies ? "true" : "false") << ", \"restrict_piece\": " << p.restrict_piece
+          << ", \"apply_workgroups\": " << p.apply_workgroups << "}";
         return o.str();
     }
 } // namespace psmas

@@ -39,6 +39,10 @@
 //   restrict_piece       longest level-1 descendant list one wave sums when restricting coarse
 //                        levels 2.. (longer lists are cut into pieces added where they are read);
 //                        9..512, default 512; takes effect at the next factorize
+//   apply_workgroups     at most this many workgroups apply bank matrices in one apply launch
+//                        (0, the default: as many as are resident at once); each walks its
+//                        matrices grid-stride, so a small cap makes every workgroup take many
+//                        matrices (parity tests of that loop on small systems)
 //   division_guard       alpha/beta are set to 0 when |denominator| < division_guard; 0 (default)
 //                        sets them to 0 only for a non-finite quotient; 1e-20 is the reference's
 //                        absolute rule (MASSolver.cu:43-56)
@@ -97,6 +101,7 @@ namespace psmas
         bool device_setup = true;
         bool spmv_copies = true;
         int restrict_piece = 512;
+        int apply_workgroups = 0;
     };
 
     /// Apply a JSON object to params.  Accepts either {"MAS": {...}} (only the "MAS" member is

@@ -141,7 +141,8 @@ namespace psmas
         if (before.use_graphs != params_.use_graphs || before.check_interval != params_.check_interval ||
             before.true_residual_period != params_.true_residual_period || before.precision != params_.precision ||
             before.coarse_solver != params_.coarse_solver || before.spmv_unroll != params_.spmv_unroll ||
-            before.spmv_halo != params_.spmv_halo || before.spmv_copies != params_.spmv_copies)
+            before.spmv_halo != params_.spmv_halo || before.spmv_copies != params_.spmv_copies ||
+            before.apply_workgroups != params_.apply_workgroups)
             graphs_->clear();
     }
 
@@ -195,7 +196,7 @@ namespace psmas
             float ms = 0.0f;
             PSMAS_HIP(hipEventElapsedTime(&ms, e0, e1));
             const float us = ms * 1e3f / reps;
-            o << (ci ? ", " : "") << "\"" << cands[ci] << "\": " << us;
+            o << (ci ? ", " : "") << "\"" << cands[ci] << "\": " << json_number(us, 6);
             if (us < best_t)
             {
                 best_t = us;
@@ -226,7 +227,7 @@ namespace psmas
                 for (const auto &S : sys_)
                     ok = ok && (S.status == Status::ReachRelativeTolerance || S.status == Status::ReachAbsoluteTolerance);
                 o << (firstc ? "" : ", ") << "\"" << (cs == CoarseSolver::Bank ? "bank" : "cholesky") << "/" << iv
-                  << "\": " << (ok ? solve_ms_ : -1.0);
+                  << "\": " << json_number(ok ? solve_ms_ : -1.0, 6);
                 firstc = false;
                 if (ok && solve_ms_ < best_ms)
                 {
@@ -238,7 +239,7 @@ namespace psmas
         graphs_->clear();
         o << "}, \"check_interval\": " << params_.check_interval << ", \"coarse_solver\": \""
           << (params_.coarse_solver == CoarseSolver::Bank ? "bank" : "cholesky") << "\", \"best_solve_ms\": "
-          << best_ms << "}";
+          << json_number(best_ms < 1e300 ? best_ms : NAN, 6) << "}";
         return o.str();
     }
 
@@ -249,11 +250,11 @@ namespace psmas
         const System &S = sys_[s];
         std::ostringstream o;
         o.precision(17);
-        o << "{\"solver_iter\": " << S.iterations << ", \"solver_error\": " << S.residual_norm
+        o << "{\"solver_iter\": " << S.iterations << ", \"solver_error\": " << json_number(S.residual_norm)
           << ", \"solver_status\": \"" << status_string(S.status) << "\""
           << ", \"precision\": \"" << (params_.precision == Precision::FP64 ? "fp64" : "mixed") << "\""
-          << ", \"reliable_updates\": " << S.reliable_updates << ", \"relative_residual\": " << S.rel_residual
-          << ", \"solve_ms\": " << solve_ms_ << ", \"levels\": " << level_num_ << ", \"parts\": " << S.nparts
+          << ", \"reliable_updates\": " << S.reliable_updates << ", \"relative_residual\": " << json_number(S.rel_residual)
+          << ", \"solve_ms\": " << json_number(solve_ms_) << ", \"levels\": " << level_num_ << ", \"parts\": " << S.nparts
           << ", \"systems\": " << sys_.size() << ", \"aggregation\": \"" << S.aggregation << "\""
           << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\""
           << ", \"factorize_mode\": \"" << factorize_mode_ << "\""
@@ -262,7 +263,7 @@ namespace psmas
                  std::ostringstream q;
                  q.precision(6);
                  for (size_t i = 0; i < setup_ms_.size(); ++i)
-                     q << (i ? ", " : "") << "\"" << setup_ms_[i].first << "\": " << setup_ms_[i].second;
+                     q << (i ? ", " : "") << "\"" << setup_ms_[i].first << "\": " << json_number(setup_ms_[i].second, 6);
                  return q.str();
              }() << "}"
           << ", \"dirichlet\": {\"active\": " << (dirichlet_ ? "true" : "false")
@@ -277,7 +278,7 @@ namespace psmas
           << ", \"fallback_iteration\": " << learned_fallback_iter_
           << ", \"trained\": " << (lcc_ && lcc_->trained() ? "true" : "false")
           << ", \"pairs\": " << (lcc_ ? lcc_->num_pairs() : 0)
-          << ", \"train_loss\": " << (lcc_ ? lcc_->last_loss() : 0.0)
+          << ", \"train_loss\": " << json_number(lcc_ ? lcc_->last_loss() : 0.0)
           << ", \"lattice_points\": " << (lcc_ && lcc_->lattice_ready() ? lcc_->k() : 0)
           << ", \"block_restriction\": " << (lcc_ && lcc_->block_restriction() ? "true" : "false")
           << ", \"note\": \"" << learned_note_ << "\"}}";
@@ -1362,6 +1363,7 @@ namespace psmas
     ApplySide Solver::restrict_side(const void *seg1, void *out, int block, bool lattice_gather) const
     {
         ApplySide sd;
+        sd.apply_cap = params_.apply_workgroups;
         if (nrsmall_ + nrbig_ > 0)
         {
             sd.nsmall = nrsmall_;
@@ -1386,6 +1388,7 @@ namespace psmas
     ApplySide Solver::coarse_side(int block) const
     {
         ApplySide sd;
+        sd.apply_cap = params_.apply_workgroups;
         if (nrcomb_ > 0)
         {
             sd.split_of = d_split_of_.data();

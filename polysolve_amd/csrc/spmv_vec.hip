@@ -659,6 +659,7 @@ namespace psmas
     void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, const Reductions &red, const SysMap &sm,
                         hipStream_t s)
     {
+        require_device_ptrs("spmv_ap", {A.slice_ptr, A.cols, A.vals, x, y});
         const int grid = A.nslices / (SPMV_BLOCK / WAVE);
         if (A.halo_max > 0)
         {
@@ -678,6 +679,7 @@ namespace psmas
     void launch_spmv_res(const SellView<TV> &A, const TX *x, const TX *b, TY *y, FineRestrict fr, TY *yc0,
                          double *rr_partials, const SysMap &sm, bool selected_only, hipStream_t s)
     {
+        require_device_ptrs("spmv_res", {A.slice_ptr, A.cols, A.vals, x, b, y});
         const int grid = A.nslices / (SPMV_BLOCK / WAVE);
         const int sel = selected_only ? 1 : 0;
         if (A.halo_max > 0)
@@ -699,6 +701,7 @@ namespace psmas
     template <int BD, typename TV, typename TX>
     void launch_spmv_plain(const SellView<TV> &A, const TX *x, TX *y, hipStream_t s)
     {
+        require_device_ptrs("spmv_plain", {A.slice_ptr, A.cols, A.vals, x, y});
         const int grid = div_up(A.nslices, SPMV_BLOCK / WAVE);
         PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_plain<BD, U, TV, TX>), dim3(grid), dim3(SPMV_BLOCK),
                                                            0, s, A, x, y));
@@ -708,6 +711,7 @@ namespace psmas
     void launch_spmv_ap_copies(const SellView<TV> &A, int copies, int npad1, const TX *x, TX *y, const Reductions &red,
                                const SysMap &sm, hipStream_t s)
     {
+        require_device_ptrs("spmv_ap_copies", {A.slice_ptr, A.cols, A.vals, x, y});
         // Groups of at most 4 copies per launch: their halos (copy 0's lists plus c * npad1) stay
         // within 64 KB of LDS, and each group reads the matrix once.
         const int nblocks1 = npad1 / SPMV_BLOCK;
@@ -786,6 +790,7 @@ namespace psmas
     {
         if (n <= 0)
             return;
+        require_device_ptrs("gather_values", {idx, src, dst});
         hipLaunchKernelGGL(k_gather_values, dim3(static_cast<unsigned>(div_up(n, 256))), dim3(256), 0, s, n, idx, src,
                            dst);
         PSMAS_LAUNCH_CHECK();
@@ -805,6 +810,7 @@ namespace psmas
     {
         if (n <= 0)
             return;
+        require_device_ptrs("scatter_values", {map, src, dst});
         hipLaunchKernelGGL(k_scatter_values, dim3(static_cast<unsigned>(div_up_ll(n, 256))), dim3(256), 0, s, n, map,
                            src, dst);
         PSMAS_LAUNCH_CHECK();
@@ -814,6 +820,7 @@ namespace psmas
     void launch_update_xr(int nblocks, const T *p, const T *Ap, T *x, T *r, FineRestrict fr, T *rc0,
                           const Reductions &red, const SysMap &sm, hipStream_t s, const LatticeBlocks *lb)
     {
+        require_device_ptrs("update_xr", {p, Ap, x, r});
         if (lb != nullptr && lb->cell != nullptr)
             hipLaunchKernelGGL((k_update_xr<BD, T, true>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, Ap, x, r, fr, rc0,
                                red, sm, *lb);
@@ -826,6 +833,7 @@ namespace psmas
     template <int BD, typename T>
     void launch_axpy_alpha(int nblocks, const T *p, T *x, const Reductions &red, const SysMap &sm, hipStream_t s)
     {
+        require_device_ptrs("axpy_alpha", {p, x});
         hipLaunchKernelGGL((k_axpy_alpha<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, p, x, red, sm);
         PSMAS_LAUNCH_CHECK();
     }
@@ -834,6 +842,7 @@ namespace psmas
     void launch_update_p(int nblocks, const T *z, T *p, const Reductions &red, const SysMap &sm, hipStream_t s,
                          const double *rzo, T *zprev)
     {
+        require_device_ptrs("update_p", {z, p});
         hipLaunchKernelGGL((k_update_p<BD, T>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, z, p, red, sm, rzo, zprev);
         PSMAS_LAUNCH_CHECK();
     }
@@ -841,6 +850,7 @@ namespace psmas
     template <int BD>
     void launch_accumulate_correction(int nblocks, float *x32, double *x64, const SysMap &sm, hipStream_t s)
     {
+        require_device_ptrs("accumulate_correction", {x32, x64});
         hipLaunchKernelGGL((k_accumulate_correction<BD>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, x32, x64, sm);
         PSMAS_LAUNCH_CHECK();
     }
@@ -850,6 +860,7 @@ namespace psmas
     {
         if (n <= 0)
             return;
+        require_device_ptrs("copy_convert", {src, dst});
         hipLaunchKernelGGL((k_copy_convert<TS, TD>), dim3(stream_grid(n)), dim3(256), 0, s, n, src, dst);
         PSMAS_LAUNCH_CHECK();
     }
@@ -859,6 +870,7 @@ namespace psmas
     {
         if (n <= 0)
             return;
+        require_device_ptrs("fill", {dst});
         hipLaunchKernelGGL((k_fill<T>), dim3(stream_grid(n)), dim3(256), 0, s, n, dst, value);
         PSMAS_LAUNCH_CHECK();
     }
@@ -866,6 +878,7 @@ namespace psmas
     template <typename T>
     void launch_gather_in(int npad, int bd, int n, const int *block_of_slot, const T *src, T *dst, hipStream_t s)
     {
+        require_device_ptrs("gather_in", {block_of_slot, src, dst});
         hipLaunchKernelGGL((k_gather_in<T>), dim3(stream_grid(static_cast<long long>(npad) * bd)), dim3(256), 0, s,
                            npad, bd, n, block_of_slot, src, dst);
         PSMAS_LAUNCH_CHECK();
@@ -874,6 +887,7 @@ namespace psmas
     template <typename T>
     void launch_scatter_out(int nb, int bd, int n, const int *slot_of_block, const T *src, T *dst, hipStream_t s)
     {
+        require_device_ptrs("scatter_out", {slot_of_block, src, dst});
         hipLaunchKernelGGL((k_scatter_out<T>), dim3(stream_grid(n)), dim3(256), 0, s, nb, bd, n, slot_of_block, src,
                            dst);
         PSMAS_LAUNCH_CHECK();
@@ -881,6 +895,7 @@ namespace psmas
 
     void launch_set_scalar(double *sc, int count, int slot, double value, hipStream_t s)
     {
+        require_device_ptrs("set_scalar", {sc});
         hipLaunchKernelGGL(k_set_scalar, dim3(div_up(count, 256)), dim3(256), 0, s, sc, static_cast<int>(S_COUNT),
                            count, slot, value);
         PSMAS_LAUNCH_CHECK();
@@ -888,12 +903,14 @@ namespace psmas
 
     void launch_snapshot(long long n, const double *x64, const float *x32, double *out, hipStream_t s)
     {
+        require_device_ptrs("snapshot", {x64, x32, out});
         hipLaunchKernelGGL(k_snapshot, dim3(stream_grid(n)), dim3(256), 0, s, n, x64, x32, out);
         PSMAS_LAUNCH_CHECK();
     }
 
     void launch_difference(long long n, const double *a, const double *b, double *out, hipStream_t s)
     {
+        require_device_ptrs("difference", {a, b, out});
         hipLaunchKernelGGL(k_difference, dim3(stream_grid(n)), dim3(256), 0, s, n, a, b, out);
         PSMAS_LAUNCH_CHECK();
     }

@@ -66,6 +66,7 @@ def test_inner_object_extension():
     '{"MAS": {"reliable_update_delta": 2}}',
     '{"MAS": {"block_dim": 3',
     '[1, 2]',
+    '{"MAS": {"relative_tolerance": 1e999}}',
 ])
 def test_invalid_parameters_raise(bad):
     with pytest.raises(MASError, match=r"\[MAS\]"):
@@ -90,3 +91,18 @@ def test_null_handle_calls_fail_cleanly():
     assert lib.psmas_solve(None, 3, None, None) == _native.PSMAS_ERROR_INVALID_ARGUMENT
     assert lib.psmas_factorize_csc(None, 3, 0, None, None, None) == _native.PSMAS_ERROR_INVALID_ARGUMENT
     assert lib.psmas_destroy(None) == _native.PSMAS_OK
+
+
+def test_parameter_json_is_strict_json():
+    """Every number the library writes is valid JSON (no nan / inf tokens): parsed with a decoder
+    that refuses the non-standard constants."""
+    import json
+
+    def refuse(tok):
+        raise ValueError(f"non-standard JSON constant {tok}")
+
+    for prm in ({}, {"MAS": {"relative_tolerance": 1e-300, "division_guard": 1e-20, "learned_output_scale": -0.5}}):
+        text = json.dumps(prm)
+        rc, out = _native.read_string(_native.lib().psmas_parse_parameters, text.encode())
+        assert rc == _native.PSMAS_OK
+        json.loads(out, parse_constant=refuse)

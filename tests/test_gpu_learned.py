@@ -153,3 +153,29 @@ def test_learned_correction_without_network_is_plain(gpu, drift_sequence):
     info = s.get_info()
     assert info["solver_status"] == "Reach relative tolerance"
     assert not info["learned"]["used"] and "no trained network" in info["learned"]["note"]
+
+
+def test_info_stays_json_when_training_diverges(gpu):
+    """A learning rate far too large drives the training loss to a non-finite value; get_info must
+    still be strict JSON (the value becomes null), since a polysolve caller parses it after every
+    solve."""
+    import json
+
+    from polysolve_amd import _native
+
+    p = problems.hex_elasticity(6, E=problems.random_moduli(6, 21))
+    s = _solver({"lattice_points": 3, "learned_collect": True, "learned_random_pairs": 16})
+    s.set_coordinates(p.coords)
+    s.factorize(p.A)
+    x = np.zeros(p.n)
+    s.solve(p.b, x)
+    loss = s.learned_train(epochs=3, batch=16, lr=1e30, seed=0)
+    assert not np.isfinite(loss)
+
+    def refuse(tok):
+        raise ValueError(f"non-standard JSON constant {tok}")
+
+    rc, text = _native.read_string(_native.lib().psmas_get_info, s._h)
+    assert rc == _native.PSMAS_OK
+    info = json.loads(text, parse_constant=refuse)
+    assert info["learned"]["train_loss"] is None

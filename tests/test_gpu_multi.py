@@ -114,3 +114,29 @@ def test_multi_rhs_device_input(gpu, het12):
         s.factorize_multi_device(dc.ptr, dr.ptr, dv.ptr, n, rowidx.size, 0)
     for a in (dc, dr, dv):
         a.free()
+
+
+@pytest.mark.parametrize("k", [3, 8])
+def test_multi_rhs_apply_cap(gpu, het12, k):
+    """apply_workgroups = 8: every apply workgroup walks many matrices, so the shared fine-bank
+    loop (a copy-0 bank stands for every copy's bank, the other copies' fine banks are stepped
+    over, coarse banks go one by one) runs its skip and step logic across many matrices.  The
+    iterates equal the uncapped launch's bit for bit, and every column matches a direct solve."""
+    import scipy.sparse.linalg as spla
+
+    rng = np.random.default_rng(40 + k)
+    B = rng.standard_normal((het12.n, k))
+    out = []
+    for cap in (0, 8):
+        s = MASSolver()
+        s.set_parameters({"MAS": {**_prm("mixed")["MAS"], "apply_workgroups": cap}})
+        s.set_coordinates(het12.coords)
+        s.factorize_multi(het12.A, k)
+        out.append((s.solve_multi(B), [s.batch_info(c)["solver_iter"] for c in range(k)]))
+    (X0, it0), (X8, it8) = out
+    assert it0 == it8
+    assert np.array_equal(X0, X8)
+    lu = spla.splu(het12.A.tocsc())
+    for c in range(k):
+        xd = lu.solve(B[:, c])
+        assert np.linalg.norm(X8[:, c] - xd) <= 1e-6 * np.linalg.norm(xd)

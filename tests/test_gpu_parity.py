@@ -99,17 +99,27 @@ def test_bank_inverses_match_oracle(gpu, cfg1, agg):
     assert err.max() < 1e-9
 
 
+@pytest.mark.parametrize("cap", [0, 8])
 @pytest.mark.parametrize("agg", ["morton", "graph"])
-def test_apply_fp64_matches_oracle(gpu, cfg1, agg):
-    s = _factorized(cfg1, {"aggregation": agg})
+def test_apply_fp64_matches_oracle(gpu, cfg1, agg, cap):
+    """cap = 8 workgroups apply all ~190 bank matrices, so each walks ~24 of them through the
+    grid-stride loop with two matrices in flight (the path every workgroup takes on 64^3)."""
+    s = _factorized(cfg1, {"aggregation": agg, "apply_workgroups": cap})
     offsets, perm = s.partition()
     O = oracle.MasOracle(cfg1.A, 3, offsets, perm)
+    assert O.info["total_mats"] > 20 * 8
     rng = np.random.default_rng(2)
     for _ in range(3):
         r = rng.standard_normal(cfg1.n)
         z = s.apply_preconditioner(r)
         zr = O.apply(r)
         assert np.linalg.norm(z - zr) <= 1e-10 * np.linalg.norm(zr)
+    if cap:
+        # Which workgroup applies a matrix changes nothing in its arithmetic.
+        z16 = s.apply_preconditioner(r, mixed=True)
+        s.set_parameters({"MAS": {"apply_workgroups": 0}})
+        assert np.array_equal(s.apply_preconditioner(r), z)
+        assert np.array_equal(s.apply_preconditioner(r, mixed=True), z16)
 
 
 def test_split_restriction_lists_match(gpu, het12):
@@ -138,8 +148,9 @@ def test_split_restriction_lists_match(gpu, het12):
     assert np.linalg.norm(cut.apply_preconditioner(r) - zr) <= 1e-10 * np.linalg.norm(zr)
 
 
-def test_apply_mixed_close_to_fp64(gpu, het12):
-    s = _factorized(het12)
+@pytest.mark.parametrize("cap", [0, 8])
+def test_apply_mixed_close_to_fp64(gpu, het12, cap):
+    s = _factorized(het12, {"apply_workgroups": cap})
     rng = np.random.default_rng(3)
     r = rng.standard_normal(het12.n)
     z64 = s.apply_preconditioner(r)
