@@ -119,6 +119,19 @@ int psmas_dirichlet_solve_device(psmas_handle h, int n, double *d_f, double *d_x
 /* Same as psmas_solve with b and x in device memory (HIP pointers on the solver's device). */
 int psmas_solve_device(psmas_handle h, int n, const double *d_b, double *d_x);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same solve as
+ * psmas_solve_device (bit-identical results), but it allocates nothing: the solve's reduction
+ * partials and per-system scalars live in a static 1 MiB array of the library, one per GPU, held
+ * until the process ends, and kernels are launched one by one (no graph replay).  Limit: systems
+ * of at most 21,840 node blocks (256 padded nodes each, about 5.5 M block nodes); larger ones are
+ * refused with PSMAS_ERROR_INVALID_ARGUMENT.  Parts that a factorisation builds at its first
+ * solve (the mixed path's FP32/FP16 copies, the dense coarsest solve, the learned correction's
+ * lattice) must exist: call psmas_solve_device once after factorize, otherwise the call fails
+ * with PSMAS_ERROR_RUNTIME; learned_collect and kernel timing are refused.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (several threads or handles) race. */
+int psmas_solve_device_noalloc(psmas_handle h, int n, const double *d_b, double *d_x);
+
 /* Same as psmas_factorize_csc with the CSC arrays in device memory (HIP pointers on the solver's
  * device): d_colptr n+1, d_rowidx and d_vals nnz entries.  The arrays are validated on the GPU
  * first (the same errors as psmas_factorize_csc); the domains (Morton chunks with coordinates,

@@ -43,6 +43,7 @@ SIGNATURES = {
     "psmas_solve": (_i, [_vp, _i, _pd, _pd]),
     "psmas_last_error": (_cp, [_vp]),
     "psmas_solve_device": (_i, [_vp, _i, _vp, _vp]),
+    "psmas_solve_device_noalloc": (_i, [_vp, _i, _vp, _vp]),
     "psmas_factorize_values_device": (_i, [_vp, _ll, _vp]),
     "psmas_factorize_csc_device": (_i, [_vp, _i, _ll, _vp, _vp, _vp]),
     "psmas_factorize_multi_csc": (_i, [_vp, _i, _i, _ll, _pi, _pi, _pd]),

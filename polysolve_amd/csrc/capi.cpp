@@ -155,6 +155,13 @@ extern "C"
         return guarded(h->last_error, [&] { h->solver.solve_device(n, d_b, d_x); });
     }
 
+    int psmas_solve_device_noalloc(psmas_handle h, int n, const double *d_b, double *d_x)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] { h->solver.solve_device_noalloc(n, d_b, d_x); });
+    }
+
     int psmas_dirichlet_factorize_csc(psmas_handle h, int n, long long nnz, const int *colptr, const int *rowidx,
                                       const double *vals, int ndir, const int *dirichlet_nodes)
     {

@@ -33,6 +33,8 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <sstream>
 #include <atomic>
 #include <thread>
@@ -91,6 +93,39 @@ namespace psmas
     {
         /// Graph cache keys: precision path, iteration pattern, iterations per graph.
         int graph_key(int path, int pattern, int count) { return (path << 24) | (pattern << 16) | count; }
+
+        // Scratch of solve_device_noalloc (1 MiB per GPU, held until the process ends): the
+        // per-system scalars in the first SCRATCH_SC_PAD doubles, then the five reduction
+        // partial arrays and the flexible-PCG partials, one double per 256-node block each.
+        // Host code never passes `scratch` itself to a kernel: it is a host-side stand-in, and
+        // the device address comes from hipGetSymbolAddress (scratch_address).
+        constexpr size_t SCRATCH_DOUBLES = size_t(1) << 17;
+        constexpr size_t SCRATCH_SC_PAD = 32; ///< >= S_COUNT doubles for the one system
+        static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+        /// This device's address of `scratch`, looked up once per device with that device current.
+        double *scratch_address(int device, std::string &error)
+        {
+            static std::mutex mtx;
+            static std::map<int, std::pair<double *, std::string>> addr;
+            std::lock_guard<std::mutex> lock(mtx);
+            auto it = addr.find(device);
+            if (it == addr.end())
+            {
+                void *p = nullptr;
+                const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+                std::string msg;
+                if (e != hipSuccess)
+                {
+                    (void)hipGetLastError();
+                    msg = std::string("hipGetSymbolAddress of the static scratch failed: ") + hipGetErrorString(e);
+                    p = nullptr;
+                }
+                it = addr.emplace(device, std::make_pair(static_cast<double *>(p), msg)).first;
+            }
+            error = it->second.second;
+            return it->second.first;
+        }
     } // namespace
 
     // ------------------------------------------------------------------------------ graphs
@@ -119,6 +154,7 @@ namespace psmas
         device_ = 0;
         PSMAS_HIP(hipSetDevice(device_));
         PSMAS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        scratch_ = scratch_address(device_, scratch_error_);
         sys_.resize(1);
     }
 
@@ -299,16 +335,29 @@ namespace psmas
         S.have_partition = false;
     }
 
+    double *Solver::sc_ptr() const { return use_scratch_ ? scratch_ : d_sc_.data(); }
+
+    double *Solver::rzo_ptr() const
+    {
+        return use_scratch_ ? scratch_ + SCRATCH_SC_PAD + static_cast<size_t>(R_COUNT) * partial_cap_ : d_rzo_.data();
+    }
+
+    int Solver::noalloc_max_blocks()
+    {
+        return static_cast<int>((SCRATCH_DOUBLES - SCRATCH_SC_PAD) / (R_COUNT + 1));
+    }
+
     Reductions Solver::reductions() const
     {
         Reductions r;
-        auto site = [&](int i) { return d_partials_.data() + static_cast<size_t>(i) * partial_cap_; };
+        double *const base = use_scratch_ ? scratch_ + SCRATCH_SC_PAD : d_partials_.data();
+        auto site = [&](int i) { return base + static_cast<size_t>(i) * partial_cap_; };
         r.pap = site(R_PAP);
         r.rr = site(R_RR);
         r.rz = site(R_RZ);
         r.rz_old = site(R_RZ_OLD);
         r.rr_true = site(R_RR_TRUE);
-        r.sc = d_sc_.data();
+        r.sc = sc_ptr();
         return r;
     }
 
@@ -367,7 +416,7 @@ namespace psmas
     void Solver::read_scalars()
     {
         h_sc_.resize(static_cast<size_t>(S_COUNT) * sys_.size());
-        PSMAS_HIP(hipMemcpyAsync(h_sc_.data(), d_sc_.data(), sizeof(double) * h_sc_.size(), hipMemcpyDeviceToHost,
+        PSMAS_HIP(hipMemcpyAsync(h_sc_.data(), sc_ptr(), sizeof(double) * h_sc_.size(), hipMemcpyDeviceToHost,
                                  stream_));
         PSMAS_HIP(hipStreamSynchronize(stream_));
     }
@@ -1552,6 +1601,53 @@ namespace psmas
         solve_batch({d_b}, {d_x}, true);
     }
 
+    void Solver::solve_device_noalloc(int n, const double *d_b, double *d_x)
+    {
+        PSMAS_HIP(hipSetDevice(device_));
+        sys_[0].status = Status::Running;
+        check_ready_single(n);
+        if (scratch_ == nullptr)
+            throw std::runtime_error("[MAS] solve_device_noalloc: " + scratch_error_);
+        if (d_b == nullptr || d_x == nullptr)
+            throw InvalidArgumentError("[MAS] null b or x.");
+        if (nblocks_ > noalloc_max_blocks())
+            throw InvalidArgumentError("[MAS] solve_device_noalloc: " + std::to_string(nblocks_) +
+                                       " node blocks of 256 padded nodes exceed the static scratch (1 MiB: at most " +
+                                       std::to_string(noalloc_max_blocks()) + " blocks); use psmas_solve_device.");
+        // Everything the solve would otherwise build or allocate on the way must exist already.
+        std::string missing;
+        if (params_.coarse_solver == CoarseSolver::Cholesky && !dense_built_)
+            missing = "the dense coarsest-level solve";
+        if (params_.precision == Precision::Mixed && (!d_vals32_ || !d_mats16_ || !d_x32_))
+            missing = "the FP32/FP16 data of the mixed path";
+        if (params_.precision == Precision::Mixed && params_.learned_correction && lcc_ && lcc_->trained() &&
+            (lattice_stale_ || !lcc_->lattice_ready() || lcc_->k() != params_.lattice_points ||
+             d_zprev32_.size() < static_cast<size_t>(npad_) * bd_))
+            missing = "the learned correction's lattice and flexible-PCG buffers";
+        if (!missing.empty())
+            throw std::runtime_error("[MAS] solve_device_noalloc: " + missing +
+                                     " is built at the first solve of a factorisation; call psmas_solve_device once "
+                                     "after factorize and these parameters.");
+        if (params_.learned_collect)
+            throw std::runtime_error("[MAS] solve_device_noalloc: learned_collect allocates snapshots; not available.");
+        if (timing_on_)
+            throw std::runtime_error("[MAS] solve_device_noalloc: kernel timing is on; not available.");
+        use_scratch_ = true;
+        try
+        {
+            // The scratch keeps the previous call's contents: clear the part this system uses.
+            const long long used = static_cast<long long>(SCRATCH_SC_PAD) + static_cast<long long>(R_COUNT + 1) * partial_cap_;
+            launch_fill<double>(used, scratch_, 0.0, stream_);
+            solve_batch({d_b}, {d_x}, true);
+        }
+        catch (...)
+        {
+            use_scratch_ = false;
+            throw;
+        }
+        use_scratch_ = false;
+    }
+
     void Solver::solve_batch(const std::vector<const double *> &b, const std::vector<double *> &x, bool device)
     {
         PSMAS_HIP(hipSetDevice(device_));
@@ -1601,7 +1697,7 @@ namespace psmas
         h_active_.assign(K, 1);
         h_select_.assign(K, 1);
         upload_flags();
-        launch_set_scalar(d_sc_.data(), K, S_GUARD, params_.division_guard, stream_);
+        launch_set_scalar(sc_ptr(), K, S_GUARD, params_.division_guard, stream_);
         ensure_coarse_solver();
         if (params_.precision == Precision::Mixed)
         {
@@ -1619,7 +1715,8 @@ namespace psmas
 
     void Solver::run_block(int key, const std::function<void()> &enqueue)
     {
-        if (!params_.use_graphs || timing_on_)
+        // Captured graphs hold the allocated partial buffers; the scratch solve launches directly.
+        if (!params_.use_graphs || timing_on_ || use_scratch_)
         {
             enqueue();
             return;
@@ -1851,7 +1948,7 @@ namespace psmas
             if (!init)
             {
                 flex.zprev = d_zprev32_.data();
-                flex.rzo = d_rzo_.data();
+                flex.rzo = rzo_ptr();
             }
         }
         tic("mas_apply_fp16", n1 * per_mat);
@@ -1896,7 +1993,7 @@ namespace psmas
         else if (learned)
         {
             launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), reductions(), sysmap(), stream_,
-                                       d_rzo_.data(), d_zprev32_.data());
+                                       rzo_ptr(), d_zprev32_.data());
         }
         else
         {
@@ -1960,7 +2057,7 @@ namespace psmas
             {
                 if (d_zprev32_.size() < static_cast<size_t>(nvec))
                     d_zprev32_.allocate(nvec, "MAS flexible z_prev");
-                if (d_rzo_.size() < static_cast<size_t>(nblocks_))
+                if (!use_scratch_ && d_rzo_.size() < static_cast<size_t>(nblocks_))
                     d_rzo_.allocate(nblocks_, "MAS flexible partials");
                 learned_used_ = true;
             }

@@ -71,6 +71,13 @@ namespace psmas
         void factorize_batch(const std::vector<CscView> &As);
         void solve_host(int n, const double *b, double *x);
         void solve_device(int n, const double *d_b, double *d_x);
+        /// solve_device without any device allocation: the reduction partials and per-system
+        /// scalars live in the library's static per-GPU scratch array (shared by every solver on
+        /// that GPU: overlapping calls race), kernels are launched one by one, and a state that
+        /// would need an allocation (parts built lazily at the first solve, pair collection,
+        /// kernel timing, a system above noalloc_max_blocks() node blocks) is refused.
+        void solve_device_noalloc(int n, const double *d_b, double *d_x);
+        static int noalloc_max_blocks();
         void solve_batch(const std::vector<const double *> &b, const std::vector<double *> &x, bool device);
         void set_coordinates(int sys, int nb, int dim, const double *xyz);
         int num_systems() const { return static_cast<int>(sys_.size()); }
@@ -252,6 +259,15 @@ namespace psmas
         std::vector<double> h_sc_;
 
         std::unique_ptr<Graphs> graphs_;
+
+        // Static scratch of solve_device_noalloc: this device's address of the library's array
+        // (null when the lookup failed, with the reason), and whether the running solve uses it
+        // for its partials, scalars and flexible-PCG partials instead of the buffers above.
+        double *scratch_ = nullptr;
+        std::string scratch_error_;
+        bool use_scratch_ = false;
+        double *sc_ptr() const;
+        double *rzo_ptr() const;
 
         // learned coarse correction
         std::unique_ptr<LearnedCoarse> lcc_;

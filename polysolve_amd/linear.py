@@ -165,6 +165,12 @@ class MASSolver(Solver):
         """Solve with b and x already in device memory (raw HIP pointers)."""
         _check(N.lib().psmas_solve_device(self._h, int(n), C.c_void_p(b_ptr), C.c_void_p(x_ptr)), self._h)
 
+    def solve_device_noalloc(self, b_ptr: int, x_ptr: int, n: int) -> None:
+        """solve_device without any device allocation (psmas_solve_device_noalloc): the solve's
+        partials and scalars live in the library's static per-GPU scratch; run one solve_device
+        after factorize first so that the parts built at the first solve exist."""
+        _check(N.lib().psmas_solve_device_noalloc(self._h, int(n), C.c_void_p(b_ptr), C.c_void_p(x_ptr)), self._h)
+
     def dirichlet_factorize(self, A, dirichlet_nodes) -> None:
         """prefactorize (FEMSolver.cpp:269-315): factorise A with the rows and columns of the fixed
         dofs replaced by identity; A itself is not modified."""
