@@ -50,6 +50,12 @@ def lib() -> C.CDLL:
         L.orc_mas_pcg.argtypes = [_vp, _pd, _pd, _i, _d, _d, _i, _i, _d, _pi, _pd, _pi]
         L.orc_jacobi_cg.restype = _i
         L.orc_jacobi_cg.argtypes = [_i, _pi, _pi, _pd, _pd, _pd, _d, _i, _pd]
+        L.orc_jacobi_cg_omp.restype = _i
+        L.orc_jacobi_cg_omp.argtypes = [_i, _pi, _pi, _pd, _pd, _pd, _d, _i, _i, _pd]
+        L.orc_omp_max_threads.restype = _i
+        L.orc_omp_max_threads.argtypes = []
+        L.orc_extend_add.restype = None
+        L.orc_extend_add.argtypes = [_i, _pd, _pi, _i, _i, _pd, _pd, _pd]
         _lib = L
     return _lib
 
@@ -152,6 +158,23 @@ def jacobi_cg(A, b, tol=1e-8, max_iter=1000, x0=None):
     it = lib().orc_jacobi_cg(n, _p(colptr, C.c_int), _p(rowidx, C.c_int), _p(vals, C.c_double), _p(b, C.c_double),
                              _p(x, C.c_double), float(tol), int(max_iter), C.byref(err))
     return x, it, err.value
+
+
+def jacobi_cg_omp(A, b, tol=1e-8, max_iter=1000, threads=0, x0=None):
+    """The same Eigen CG semantics with OpenMP over ``threads`` threads (0: all); A symmetric.
+    Returns (x, iterations, ||r|| / ||b||)."""
+    n, colptr, rowidx, vals = _csc(A)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    err = C.c_double(0.0)
+    it = lib().orc_jacobi_cg_omp(n, _p(colptr, C.c_int), _p(rowidx, C.c_int), _p(vals, C.c_double),
+                                 _p(b, C.c_double), _p(x, C.c_double), float(tol), int(max_iter), int(threads),
+                                 C.byref(err))
+    return x, it, err.value
+
+
+def omp_max_threads() -> int:
+    return int(lib().orc_omp_max_threads())
 
 
 STATUS = {0: "Running", 1: "Reach relative tolerance", 2: "Reach absolute tolerance", 3: "Reach max iterations"}

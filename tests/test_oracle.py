@@ -249,6 +249,26 @@ def test_jacobi_cg_closed_forms():
     assert it2 <= distinct and err2 < 1e-10
 
 
+@pytest.mark.parametrize("threads", [1, 4])
+def test_jacobi_cg_omp_matches_serial_and_closed_forms(threads):
+    """The OpenMP restatement (rows of the symmetric matrix in parallel) runs Eigen's CG steps:
+    the closed forms of the serial version hold, and on heterogeneous elasticity it takes the
+    serial version's iterations (up to summation order) to the same solution."""
+    A = sp.diags(np.linspace(1.0, 50.0, 40)).tocsc()
+    b = np.random.default_rng(3).standard_normal(40)
+    x, it, err = oracle.jacobi_cg_omp(A, b, tol=1e-12, max_iter=100, threads=threads)
+    assert it == 0 and np.allclose(x, b / A.diagonal(), rtol=1e-14)
+    p = problems.hex_elasticity(6, E=problems.random_moduli(6, 8))
+    xs, its, errs = oracle.jacobi_cg(p.A, p.b, tol=1e-9, max_iter=5000)
+    xo, ito, erro = oracle.jacobi_cg_omp(p.A, p.b, tol=1e-9, max_iter=5000, threads=threads)
+    assert abs(ito - its) <= 2 and erro < 1e-9
+    assert np.linalg.norm(xo - xs) <= 1e-7 * np.linalg.norm(xs)
+    assert np.linalg.norm(p.A @ xo - p.b) <= 1.01e-9 * np.linalg.norm(p.b)
+    # max_iter caps the run (Eigen returns after maxIters steps with the residual reached)
+    _, itc, errc = oracle.jacobi_cg_omp(p.A, p.b, tol=1e-9, max_iter=7, threads=threads)
+    assert itc == 7 and errc > 1e-9
+
+
 def test_jacobi_cg_solves_elasticity():
     p = problems.hex_elasticity(4, E=1e5)
     x, it, err = oracle.jacobi_cg(p.A, p.b, tol=1e-10, max_iter=2000)
