@@ -2,24 +2,35 @@
 """Benchmark: time to FP64 relative residual 1e-8 of MAS-PCG on one MI355X.
 
 Workload (default): 64^3 Q1-hex linear elasticity, per-element E = 10^U(3,6) (seed 0), nu = 0.3,
-bottom face fixed, gravity RHS — 823,875 DOF, 7,081,535 3x3 blocks.  A "step" is one full solve
-from b to x with b and x resident in HBM (``psmas_solve_device``); factorisation (partition, MAS
-build) is done once beforehand and reported separately.
+bottom face fixed, gravity RHS — 823,875 DOF, 7,081,535 3x3 blocks — and the 200-step sequence
+of such systems whose moduli drift by a 5% multiplicative random walk (step 0 is the system
+above).  Right-hand sides and solutions stay in HBM (``psmas_solve_device``); factorisations are
+done before the timed solves and reported separately.
 
-Three paths run on the same system:
+Paths:
   * fp64    — the reference algorithm: FP64 MAS-PCG, true residual every 4th iteration,
               convergence checked every 10th, relative tolerance 1e-8;
   * mixed   — FP32 inner PCG with FP16 bank inverses and FP64 reliable updates; converged only
               when the FP64 true residual ||b - A x|| <= 1e-8 ||b||;
-  * learned — the mixed path plus the learned lattice coarse correction (flexible PCG), its
-              network trained beforehand on pairs from solves of the following steps of a
-              drifting-modulus sequence (collection and training times reported separately).
-The headline ``value`` is the learned path's ms per solve (the mixed path's when the learned path
-is disabled or falls back); the other two are reported beside it.  ``multi_rhs`` reports
-``--multi-rhs`` (default 4) seeded right-hand sides of the same matrix solved together
-(factorize_multi, mixed path) against the same right-hand sides solved one by one.
+  * learned — the mixed path plus the learned lattice coarse correction (flexible PCG, plain MAS
+              as fallback), its network trained on pairs from the solves of steps 1..50.
 
-One JSON line is printed (driver contract), plus roofline and CPU-baseline objects.
+A "step" is one solve of one held-out system of the sequence: ``--steps K`` systems spread over
+steps 51..199, each refactorised (values only), solved ``--warmup`` times untimed (first system)
+or once (the others), then solved once timed between device synchronisations by each of the
+three paths.  The headline ``value`` is the learned path's mean ms over the K held-out systems;
+the mixed and FP64 means on the same systems are reported beside it, with iterations, fallbacks
+and true residuals per step, and ablations of the network on the first held-out steps (the exact
+lattice coarse solve it approximates, and a network with only its linear skip layer).
+
+The plain paths are also timed on step 0 itself, where the kernels are timed with HIP events
+(the ``roofline`` object) and the solutions are compared with an FP64 direct solve (CPU sparse
+Cholesky, the CHOLMOD stand-in) in ``accuracy``.  ``cpu_baseline`` is the reference's CPU
+iterative solver (Eigen's Jacobi-preconditioned CG, restated with OpenMP) run to 1e-8 on this
+machine's host cores; the direct solver's times are beside it.  ``multi_rhs`` reports
+``--multi-rhs`` (default 4) right-hand sides of the step-0 matrix solved together.
+
+One JSON line is printed (driver contract).
 """
 from __future__ import annotations
 
@@ -167,14 +178,218 @@ def bench_batch(args) -> int:
     return 0
 
 
+def timed_solve(solver, b_dev, x_dev, n, warmup, sync):
+    """warmup untimed solves, then one timed solve bracketed by device synchronisations (ms)."""
+    for _ in range(warmup):
+        solver.solve_device(b_dev.ptr, x_dev.ptr, n)
+    sync()
+    t0 = time.perf_counter()
+    solver.solve_device(b_dev.ptr, x_dev.ptr, n)
+    sync()
+    return (time.perf_counter() - t0) * 1e3
+
+
+def kernel_table(kt):
+    out = {}
+    for name, rec in kt.items():
+        if rec["launches"]:
+            avg = rec["total_ms"] / rec["launches"]
+            out[name] = {"launches": rec["launches"], "avg_us": avg * 1e3, "bytes": rec["bytes"],
+                         "GBps": rec["bytes"] / (avg * 1e-3) / 1e9}
+    return out
+
+
+def learned_sequence(args, prob, base, sync):
+    """The drifting-modulus protocol on the 64^3 system: the network is trained on pairs from the
+    solves of steps 1..T of the sequence, then every path is timed on held-out later steps
+    (a spread sample of 51..199).  Returns the per-step records, the summary and the ablations."""
+    from polysolve_amd import MASSolver, problems as P
+    from polysolve_amd.linear import DeviceArray
+
+    T = args.train_solves
+    K = max(args.steps, 1)
+    eval_steps = sorted({int(v) for v in np.linspace(T + 1, 199, K).round()})
+    while len(eval_steps) < K:  # K > 149: repeat the sample
+        eval_steps += eval_steps[:K - len(eval_steps)]
+    need = set(range(1, T + 1)) | set(eval_steps)
+    moduli = {}
+    for t, E in enumerate(P.config5_moduli(prob.ne, steps=max(need) + 1, seed=0)):
+        if t in need:
+            moduli[t] = E
+    n = prob.n
+    trainer = MASSolver()
+    trainer.set_parameters({"MAS": dict(base, precision="mixed", coarse_solver=args.coarse, learned_collect=True,
+                                        lattice_points=args.lattice, learned_hidden=args.hidden,
+                                        learned_random_pairs=args.random_pairs, lazy_partitioning=True)})
+    trainer.set_coordinates(prob.coords)
+    log(f"learned path: pairs from steps 1..{T} of the drifting sequence, held-out steps {eval_steps}")
+    t0 = time.perf_counter()
+    train_iters = []
+    for t in range(1, T + 1):
+        pt = P.hex_elasticity(prob.ne, E=moduli[t])
+        trainer.factorize(pt.A)
+        xt = np.zeros(pt.n)
+        trainer.solve(pt.b, xt)
+        train_iters.append(trainer.get_info()["solver_iter"])
+        if t % 10 == 0:
+            log(f"  collected steps 1..{t} ({time.perf_counter() - t0:.0f}s)")
+    collect_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = trainer.learned_train(epochs=args.epochs, batch=64, lr=3e-3, seed=0)
+    train_s = time.perf_counter() - t0
+    pairs = trainer.get_info()["learned"]["pairs"]
+    log(f"  {pairs} pairs in {collect_s:.1f}s, trained in {train_s:.1f}s, loss {loss:.3g}")
+    trainer.set_parameters({"MAS": {"learned_collect": False}})
+
+    b_dev, x_dev = DeviceArray(n), DeviceArray(n)
+    modes = {"learned": {"precision": "mixed", "learned_correction": True},
+             "mixed": {"precision": "mixed", "learned_correction": False},
+             "fp64": {"precision": "fp64", "learned_correction": False}}
+
+    def run(pt, mode, warmup):
+        trainer.set_parameters({"MAS": modes[mode]})
+        ms = timed_solve(trainer, b_dev, x_dev, n, warmup, sync)
+        info = trainer.get_info()
+        x = x_dev.download()
+        rel = float(np.linalg.norm(pt.b - pt.A @ x) / np.linalg.norm(pt.b))
+        return {"ms": ms, "iters": info["solver_iter"], "rel": rel, "status": info["solver_status"],
+                "fallback": bool(info["learned"]["fallback"]), "used": bool(info["learned"]["used"]),
+                "reliable_updates": info["reliable_updates"]}, x
+
+    records = []
+    systems = {}
+    for i, t in enumerate(eval_steps):
+        pt = P.hex_elasticity(prob.ne, E=moduli[t])
+        trainer.factorize(pt.A)
+        b_dev.upload(pt.b)
+        rec = {"step": t, "factorize_mode": trainer.get_info()["factorize_mode"]}
+        xs = {}
+        for mode in ("learned", "mixed", "fp64"):
+            rec[mode], xs[mode] = run(pt, mode, max(args.warmup if i == 0 else 1, 1))
+        rec["learned_vs_fp64_rel_diff"] = float(np.linalg.norm(xs["learned"] - xs["fp64"]) /
+                                                np.linalg.norm(xs["fp64"]))
+        records.append(rec)
+        if len(systems) < args.ablation_steps:
+            systems[t] = pt
+        log(f"  step {t}: learned {rec['learned']['ms']:.2f} ms / {rec['learned']['iters']} it, mixed "
+            f"{rec['mixed']['ms']:.2f} / {rec['mixed']['iters']}, fp64 {rec['fp64']['ms']:.2f} / "
+            f"{rec['fp64']['iters']}, res {rec['learned']['rel']:.2e}")
+
+    # The learned path on step 0 (the sequence's origin, not trained on) for the accuracy check
+    # against the direct solve.
+    trainer.factorize(prob.A)
+    b_dev.upload(prob.b)
+    step0, x0_learned = run(prob, "learned", 1)
+
+    # Ablations on the first held-out steps: the network against the linear two-level method
+    # it approximates (the exact lattice coarse solve) and against a skip-layer-only network
+    # trained on the same pairs.
+    abl = {"steps": list(systems), "mlp": [], "linear": [], "exact": []}
+    for t in systems:
+        r = next(rr for rr in records if rr["step"] == t)
+        abl["mlp"].append({"ms": r["learned"]["ms"], "iters": r["learned"]["iters"],
+                           "fallback": r["learned"]["fallback"]})
+    trainer.set_parameters({"MAS": {"learned_network": "linear"}})
+    t1 = time.perf_counter()
+    lin_loss = trainer.learned_train(epochs=args.epochs, batch=64, lr=3e-3, seed=0)
+    abl["linear_train_s"] = round(time.perf_counter() - t1, 1)
+    abl["linear_train_loss"] = lin_loss
+    for t, pt in systems.items():
+        trainer.factorize(pt.A)
+        b_dev.upload(pt.b)
+        r, _ = run(pt, "learned", 1)
+        abl["linear"].append({"ms": r["ms"], "iters": r["iters"], "fallback": r["fallback"], "rel": r["rel"]})
+    exact_setup = []
+    for t, pt in systems.items():
+        trainer.factorize(pt.A)
+        exact_setup.append(trainer.learned_exact())
+        b_dev.upload(pt.b)
+        r, _ = run(pt, "learned", 1)
+        abl["exact"].append({"ms": r["ms"], "iters": r["iters"], "fallback": r["fallback"], "rel": r["rel"]})
+    abl["exact_setup_ms"] = [round(v, 1) for v in exact_setup]
+    for key in ("mlp", "linear", "exact"):
+        abl[key + "_mean_iters"] = float(np.mean([r["iters"] for r in abl[key]]))
+        abl[key + "_mean_ms"] = float(np.mean([r["ms"] for r in abl[key]]))
+    log(f"  ablation on steps {abl['steps']}: mlp {abl['mlp_mean_iters']:.0f} it / {abl['mlp_mean_ms']:.2f} ms, "
+        f"linear {abl['linear_mean_iters']:.0f} / {abl['linear_mean_ms']:.2f}, exact lattice solve "
+        f"{abl['exact_mean_iters']:.0f} / {abl['exact_mean_ms']:.2f}")
+    b_dev.free()
+    x_dev.free()
+    del trainer
+
+    def stat(mode, key):
+        v = [r[mode][key] for r in records]
+        return float(np.mean(v)), float(np.max(v))
+
+    summary = {"train_steps": [1, T], "eval_steps": eval_steps, "pairs": pairs, "collect_s": round(collect_s, 1),
+               "train_s": round(train_s, 1), "train_loss": loss, "train_iters_mean": float(np.mean(train_iters)),
+               "lattice_points": args.lattice, "hidden": args.hidden}
+    for mode in ("learned", "mixed", "fp64"):
+        ms_mean, ms_max = stat(mode, "ms")
+        it_mean, it_max = stat(mode, "iters")
+        summary[mode] = {"ms_mean": round(ms_mean, 3), "ms_max": round(ms_max, 3), "iters_mean": it_mean,
+                         "iters_max": it_max, "max_true_rel_residual": max(r[mode]["rel"] for r in records),
+                         "converged": sum(r[mode]["status"] == "Reach relative tolerance" for r in records)}
+    summary["learned"]["fallbacks"] = sum(r["learned"]["fallback"] for r in records)
+    summary["learned"]["max_rel_diff_vs_fp64"] = max(r["learned_vs_fp64_rel_diff"] for r in records)
+    return records, summary, abl, step0, x0_learned
+
+
+def cpu_baselines(args, prob, x_refs):
+    """The reference's CPU paths on this machine's host cores: Eigen's Jacobi-CG semantics (full
+    run to 1e-8 with OpenMP, plus a bounded one-thread sample as Eigen runs a column-major
+    product) and the direct solver (CHOLMOD stand-in: nested-dissection multifrontal Cholesky),
+    whose solution is also the FP64 accuracy reference of the GPU results."""
+    import oracle
+    from oracle.sparse_cholesky import SparseCholesky
+
+    threads = args.cpu_threads or oracle.omp_max_threads()
+    out = {"nproc": os.cpu_count(), "threads": threads, "cpu": cpu_info()}
+    log(f"CPU: Jacobi-CG (Eigen semantics) to 1e-8, {threads} threads")
+    t = time.perf_counter()
+    xj, itj, errj = oracle.jacobi_cg_omp(prob.A, prob.b, tol=1e-8, max_iter=args.cpu_max_iter, threads=threads)
+    jac_ms = (time.perf_counter() - t) * 1e3
+    rel_j = float(np.linalg.norm(prob.b - prob.A @ xj) / np.linalg.norm(prob.b))
+    out["jacobi_cg"] = {"ms": round(jac_ms, 1), "iters": itj, "tol_error": errj, "true_rel_residual": rel_j,
+                        "converged": errj < 1e-8, "threads": threads}
+    log(f"  {jac_ms:.0f} ms, {itj} iterations, rel res {rel_j:.2e}")
+    t = time.perf_counter()
+    _, it1, _ = oracle.jacobi_cg_omp(prob.A, prob.b, tol=0.0, max_iter=args.cpu_sample_iters, threads=1)
+    per_it = (time.perf_counter() - t) / max(it1, 1)
+    out["jacobi_cg_1thread_estimate_ms"] = round(per_it * itj * 1e3, 1)
+    out["jacobi_cg_1thread_sample"] = f"{it1} iterations on 1 thread ({per_it * 1e3:.1f} ms each) x {itj} iterations"
+    if args.no_direct:
+        return out, None
+    log("CPU: sparse Cholesky (nested dissection, multifrontal) — CHOLMOD stand-in")
+    t = time.perf_counter()
+    S = SparseCholesky(prob.A, 3, prob.coords)
+    S.factorize(progress=lambda f, s: log(f"  factorize {100 * f:.0f}% of the flops after {s:.0f}s"))
+    xd = S.solve(prob.b)
+    total_ms = (time.perf_counter() - t) * 1e3
+    rel_d = float(np.linalg.norm(prob.b - prob.A @ xd) / np.linalg.norm(prob.b))
+    out["cholmod_standin"] = {"total_ms": round(total_ms, 1), "analyze_ms": round(S.times["analyze_s"] * 1e3, 1),
+                              "factorize_ms": round(S.times["factorize_s"] * 1e3, 1),
+                              "solve_ms": round(S.times["solve_s"] * 1e3, 1), "nnz_L": S.nnz_L,
+                              "gflop": round(S.flops / 1e9, 1), "true_rel_residual": rel_d}
+    log(f"  {total_ms:.0f} ms (factorize {S.times['factorize_s']:.1f}s), rel res {rel_d:.2e}")
+    errs = {k: float(np.linalg.norm(x - xd) / np.linalg.norm(xd)) for k, x in x_refs.items()}
+    errs["jacobi_cg"] = float(np.linalg.norm(xj - xd) / np.linalg.norm(xd))
+    return out, {"reference": "FP64 sparse Cholesky (CHOLMOD stand-in)", "reference_rel_residual": rel_d,
+                 "rel_err": errs}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed solves: one per held-out system of the drifting sequence (64^3), else of the system")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed solves before the first timed one")
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=30, help="oracle PCG iterations in the CPU sample")
+    ap.add_argument("--no-direct", action="store_true", help="skip the CPU sparse Cholesky (accuracy reference)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU Jacobi-CG (0: all)")
+    ap.add_argument("--cpu-max-iter", type=int, default=100000)
+    ap.add_argument("--cpu-sample-iters", type=int, default=40, help="one-thread Jacobi-CG sample")
     ap.add_argument("--aggregation", default="morton", choices=["morton", "graph"])
     ap.add_argument("--coarse", default="cholesky", choices=["cholesky", "bank"],
                     help="coarsest-level solver of the mixed path")
@@ -184,11 +399,12 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction path")
     ap.add_argument("--profile-learned", action="store_true",
                     help="profile mode: also train briefly (2 solves, 20 epochs) and time learned solves")
-    ap.add_argument("--train-solves", type=int, default=16,
-                    help="learned path: solves of the drifting sequence (steps 1..T) that provide training pairs")
+    ap.add_argument("--train-solves", type=int, default=50,
+                    help="learned path: steps 1..T of the drifting sequence provide the training pairs")
+    ap.add_argument("--ablation-steps", type=int, default=4, help="held-out steps of the ablations")
     ap.add_argument("--lattice", type=int, default=8, help="learned path: lattice points per axis")
     ap.add_argument("--hidden", type=int, default=128, help="learned path: hidden width of the network")
-    ap.add_argument("--random-pairs", type=int, default=200, help="learned path: Gaussian pairs per training solve")
+    ap.add_argument("--random-pairs", type=int, default=100, help="learned path: Gaussian pairs per training solve")
     ap.add_argument("--epochs", type=int, default=150, help="learned path: training epochs")
     ap.add_argument("--multi-rhs", type=int, default=4,
                     help="also solve this many right-hand sides together (0: skip)")
@@ -228,6 +444,7 @@ def main() -> int:
     x_dev = DeviceArray(prob.n, np.zeros(prob.n))
     n = prob.n
     bnorm = float(np.linalg.norm(prob.b))
+    learned_seq = args.config == 2 and not args.no_learned and not args.profile_only
 
     def accuracy():
         x = x_dev.download()
@@ -247,9 +464,10 @@ def main() -> int:
         src.free()
         dst.free()
 
-    # FP64 reference path.
+    # The plain paths on this system (the held-out steps of the sequence are timed below).
+    path_steps = min(args.steps, 5) if learned_seq else args.steps
     log("fp64 path")
-    fp64_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
+    fp64_ms = run_path(solver, b_dev, x_dev, n, path_steps, args.warmup, synchronize)
     info64 = solver.get_info()
     x64, rel64 = accuracy()
     log(f"  fp64 {fp64_ms:.2f} ms/solve, {info64['solver_iter']} it, true rel res {rel64:.2e}")
@@ -280,91 +498,15 @@ def main() -> int:
             log(f"  learned {learned_ms:.2f} ms/solve, {solver.get_info()['solver_iter']} it")
         return 0
 
-    # Mixed path (headline).
     solver.set_parameters({"MAS": {"precision": "mixed", "coarse_solver": args.coarse}})
     log("mixed path")
-    mixed_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
+    mixed_ms = run_path(solver, b_dev, x_dev, n, path_steps, args.warmup, synchronize)
     infom = solver.get_info()
     xm, relm = accuracy()
     log(f"  mixed {mixed_ms:.2f} ms/solve, {infom['solver_iter']} it, {infom['reliable_updates']} updates, "
         f"true rel res {relm:.2e}")
 
-    # Learned path: the correction network is trained on pairs from solves of steps 1..T of the
-    # drifting-modulus sequence whose step 0 is this very system (it never sees step 0), then
-    # used on this system.  Training cost is reported separately from the time to solution.
-    learned = None
-    if not args.no_learned and args.config == 2:
-        from polysolve_amd import problems as P
-
-        log(f"learned path: pairs from steps 1..{args.train_solves} of the drifting sequence")
-        t0 = time.perf_counter()
-        trainer = MASSolver()
-        trainer.set_parameters({"MAS": dict(base, precision="mixed", coarse_solver=args.coarse,
-                                            learned_collect=True, lattice_points=args.lattice,
-                                            learned_hidden=args.hidden, learned_random_pairs=args.random_pairs,
-                                            lazy_partitioning=True)})
-        trainer.set_coordinates(prob.coords)
-        seq = P.config5_moduli(prob.ne, steps=args.train_solves + 1, seed=0)
-        next(seq)  # step 0 is the benchmarked system
-        train_iters = []
-        for E in seq:
-            pt = P.hex_elasticity(prob.ne, E=E)
-            trainer.factorize(pt.A)
-            xt = np.zeros(pt.n)
-            trainer.solve(pt.b, xt)
-            train_iters.append(trainer.get_info()["solver_iter"])
-        collect_s = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        loss = trainer.learned_train(epochs=args.epochs, batch=64, lr=3e-3, seed=0)
-        train_s = time.perf_counter() - t0
-        pairs = trainer.get_info()["learned"]["pairs"]
-        log(f"  {pairs} pairs in {collect_s:.1f}s (solves: {train_iters} it), trained in {train_s:.1f}s, "
-            f"loss {loss:.3g}")
-        trainer.set_parameters({"MAS": {"learned_collect": False, "learned_correction": True}})
-        trainer.factorize(prob.A)
-        learned_ms = run_path(trainer, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
-        infol = trainer.get_info()
-        xl, rell = accuracy()
-        log(f"  learned {learned_ms:.2f} ms/solve, {infol['solver_iter']} it, true rel res {rell:.2e}, "
-            f"learned {infol['learned']}")
-        trainer.set_kernel_timing(True)
-        trainer.solve_device(b_dev.ptr, x_dev.ptr, n)
-        synchronize()
-        ktl = trainer.kernel_timing()
-        trainer.set_kernel_timing(False)
-        learned_kernels = {}
-        for name, rec in ktl.items():
-            if rec["launches"]:
-                avg = rec["total_ms"] / rec["launches"]
-                learned_kernels[name] = {"launches": rec["launches"], "avg_us": round(avg * 1e3, 2),
-                                         "GBps": round(rec["bytes"] / (avg * 1e-3) / 1e9, 1)}
-        learned = {"ms": round(learned_ms, 3), "iters": infol["solver_iter"], "true_rel_residual": rell,
-                   "kernels": learned_kernels,
-                   "reliable_updates": infol["reliable_updates"], "fallback": infol["learned"]["fallback"],
-                   "train_solves": args.train_solves, "pairs": pairs, "collect_s": round(collect_s, 1),
-                   "train_s": round(train_s, 1), "train_loss": loss, "lattice_points": args.lattice,
-                   "hidden": args.hidden}
-        del trainer
-
-    # Accuracy reference: the FP64 path driven to relative residual 1e-11 (a direct FP64
-    # factorisation of this 3-D system does not fit the host in the time of a bench run; on the
-    # small workloads the tests compare against a direct solve).  The reference's own residual
-    # is reported: the solution-error figures are only meaningful while it is well below 1e-8.
-    solver.set_parameters({"MAS": {"precision": "fp64", "coarse_solver": args.coarse, "relative_tolerance": 1e-11,
-                                   "max_iter": 6000}})
-    solver.solve_device(b_dev.ptr, x_dev.ptr, n)
-    synchronize()
-    inforef = solver.get_info()
-    xref, relref = accuracy()
-    log(f"  reference: {inforef['solver_iter']} it, true rel res {relref:.2e}")
-    err_m = float(np.linalg.norm(xm - xref) / np.linalg.norm(xref))
-    if learned is not None:
-        learned["rel_err_vs_fp64_ref"] = float(np.linalg.norm(xl - xref) / np.linalg.norm(xref))
-    err_64 = float(np.linalg.norm(x64 - xref) / np.linalg.norm(xref))
-    solver.set_parameters({"MAS": {"relative_tolerance": tol, "max_iter": 20000}})
-
     # Roofline: dominant kernel of the mixed path, timed with HIP events on the solver stream.
-    solver.set_parameters({"MAS": {"precision": "mixed", "coarse_solver": args.coarse}})
     solver.set_kernel_timing(True)
     solver.solve_device(b_dev.ptr, x_dev.ptr, n)
     synchronize()
@@ -376,44 +518,29 @@ def main() -> int:
     synchronize()
     kt64 = solver.kernel_timing()
     solver.set_kernel_timing(False)
-    kernels = {}
-    for name, rec in list(kt.items()) + [(k + "@fp64path", v) for k, v in kt64.items()]:
-        if rec["launches"]:
-            avg = rec["total_ms"] / rec["launches"]
-            kernels[name] = {"launches": rec["launches"], "avg_us": avg * 1e3, "bytes": rec["bytes"],
-                             "GBps": rec["bytes"] / (avg * 1e-3) / 1e9}
+    kernels = kernel_table(kt)
+    kernels.update({k + "@fp64path": v for k, v in kernel_table(kt64).items()})
     dom_name = max(kt.items(), key=lambda kv: kv[1]["total_ms"])[0]
     dom = kernels[dom_name]
     roofline = {"bound": "hbm", "achieved": round(dom["GBps"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get((args.config, dom_name)), "kernel": dom_name,
-                "avg_us": round(dom["avg_us"], 2), "bytes_per_launch": dom["bytes"]}
+                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get((args.config, dom_name)),
+                "kernel": dom_name, "avg_us": round(dom["avg_us"], 2), "bytes_per_launch": dom["bytes"]}
 
-    # CPU baseline: the oracle's FP64 MAS-PCG (reference schedule), single thread, on a bounded
-    # sample of iterations of the same system with the same partition, scaled to the fp64
-    # iteration count.
-    cpu = None
-    if not args.no_cpu_baseline:
-        import oracle
-
-        offsets, perm = solver.partition()
-        t = time.perf_counter()
-        O = oracle.MasOracle(prob.A, 3, offsets, perm)
-        cpu_setup = time.perf_counter() - t
-        t = time.perf_counter()
-        O.pcg(prob.b, max_iter=args.cpu_iters, rel_tol=0.0, abs_tol=0.0)
-        per_it = (time.perf_counter() - t) / args.cpu_iters
-        est_ms = per_it * info64["solver_iter"] * 1e3
-        cpu = {"value": round(est_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
-               "sample": f"oracle FP64 MAS-PCG, {args.cpu_iters} iterations on this 64^3 system "
-                         f"({per_it * 1e3:.0f} ms/iteration, setup {cpu_setup:.1f}s excluded), scaled to "
-                         f"{info64['solver_iter']} iterations",
-               "cpu": cpu_info(), "nproc": os.cpu_count()}
+    # Values-only refactorisation (a Newton step's setup): same pattern, new values.
+    solver.set_parameters({"MAS": {"lazy_partitioning": True, "precision": "mixed", "coarse_solver": args.coarse}})
+    solver.factorize(prob.A)  # records the pattern
+    t = time.perf_counter()
+    solver.factorize(prob.A)
+    refactor_ms = (time.perf_counter() - t) * 1e3
+    refactor_info = solver.get_info()
+    log(f"  values-only refactorisation {refactor_ms:.1f} ms ({refactor_info['factorize_mode']}), "
+        f"phases {refactor_info['setup_ms']}")
 
     # Several right-hand sides of this matrix solved together (mixed path, factorize_multi): the
     # SpMV and the fine-bank apply read the matrix once for all of them.  Compared with solving
     # the same right-hand sides one by one with the same parameters.
     multi = None
-    if args.multi_rhs > 1 and not args.profile_only:
+    if args.multi_rhs > 1:
         k = args.multi_rhs
         rng = np.random.default_rng(7)
         B = rng.standard_normal((n, k))
@@ -440,11 +567,12 @@ def main() -> int:
         dB, dX = DeviceArray(n * k, B.ravel(order="F")), DeviceArray(n * k)
         ms_solver.solve_multi_device(dB.ptr, dX.ptr, n, k)
         synchronize()
+        reps = max(path_steps // 2, 1)
         t = time.perf_counter()
-        for _ in range(max(args.steps // 2, 1)):
+        for _ in range(reps):
             ms_solver.solve_multi_device(dB.ptr, dX.ptr, n, k)
         synchronize()
-        multi_ms = (time.perf_counter() - t) * 1e3 / max(args.steps // 2, 1)
+        multi_ms = (time.perf_counter() - t) * 1e3 / reps
         X = dX.download().reshape((n, k), order="F")
         worst = max(float(np.linalg.norm(B[:, c] - prob.A @ X[:, c]) / np.linalg.norm(B[:, c])) for c in range(k))
         multi = {"rhs": k, "ms": round(multi_ms, 3), "ms_per_rhs": round(multi_ms / k, 3),
@@ -455,10 +583,46 @@ def main() -> int:
         dB.free()
         dX.free()
         del ms_solver
+    b_dev.free()
+    x_dev.free()
+    del solver
 
-    head_ms = learned["ms"] if learned is not None and not learned["fallback"] else mixed_ms
-    head_path = ("learned-correction mixed MAS-PCG (flexible, reliable updates)" if head_ms != mixed_ms
-                 else "mixed MAS-PCG (reliable updates)")
+    learned = None
+    x_refs = {"fp64": x64, "mixed": xm}
+    if learned_seq:
+        records, summary, abl, step0, x0l = learned_sequence(args, prob, base, synchronize)
+        x_refs["learned"] = x0l
+        learned = dict(summary, per_step=records, ablation=abl, step0=step0)
+
+    cpu, acc = (None, None)
+    if not args.no_cpu_baseline:
+        cpu, acc = cpu_baselines(args, prob, x_refs)
+
+    if learned is not None:
+        head_ms = learned["learned"]["ms_mean"]
+        head_iters = learned["learned"]["iters_mean"]
+        head_path = ("learned-correction mixed MAS-PCG (flexible, reliable updates) on held-out steps of the "
+                     f"drifting sequence; network trained on steps 1-{args.train_solves}")
+        speed = {"learned_vs_fp64": round(learned["fp64"]["ms_mean"] / head_ms, 3),
+                 "mixed_vs_fp64": round(learned["fp64"]["ms_mean"] / learned["mixed"]["ms_mean"], 3),
+                 "learned_vs_mixed": round(learned["mixed"]["ms_mean"] / head_ms, 3)}
+        workload = (f"Q1-hex elasticity {prob.ne}^3 (n={n}, nnzb={levels['nnzb']}): {len(learned['eval_steps'])} "
+                    f"held-out steps of the 200-step 5% drifting-modulus sequence (seed 0)")
+        max_rel = learned["learned"]["max_true_rel_residual"]
+    else:
+        head_ms, head_iters = mixed_ms, infom["solver_iter"]
+        head_path = "mixed MAS-PCG (reliable updates)"
+        speed = {"mixed_vs_fp64": round(fp64_ms / mixed_ms, 3)}
+        workload = f"{prob.name}: Q1-hex elasticity {prob.ne}^3, n={n}, nnzb={levels['nnzb']}"
+        max_rel = relm
+    cpu_line = None
+    if cpu is not None:
+        cpu_line = {"value": cpu["jacobi_cg"]["ms"], "unit": "ms", "cores": cpu["threads"], "kind": "port",
+                    "sample": (f"full run of the Eigen ConjugateGradient<Lower|Upper, DiagonalPreconditioner> "
+                               f"restatement (OpenMP) on the 64^3 step-0 system to rel. residual 1e-8: "
+                               f"{cpu['jacobi_cg']['iters']} iterations" if args.config == 2 else
+                               f"full Jacobi-CG run to 1e-8: {cpu['jacobi_cg']['iters']} iterations"),
+                    **{k: v for k, v in cpu.items() if k != "threads"}}
     line = {
         "metric": "ms to 1e-8 rel. residual + PCG iters: learned/mixed MAS-PCG vs plain FP64",
         "value": round(head_ms, 3),
@@ -472,25 +636,24 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "fp32+fp16 inner / fp64 refinement",
         "data": "synthetic (seeded Q1-hex elasticity assembled in-repo)",
-        "config": {"workload": f"{prob.name}: Q1-hex elasticity {prob.ne}^3, n={n}, nnzb={levels['nnzb']}",
-                   "path": head_path, "aggregation": args.aggregation,
+        "config": {"workload": workload, "path": head_path, "aggregation": args.aggregation,
                    "parts": levels["parts"], "levels": levels["level_num"]},
-        "iters": learned["iters"] if head_ms != mixed_ms else infom["solver_iter"],
-        "true_rel_residual": learned["true_rel_residual"] if head_ms != mixed_ms else relm,
-        "rel_err_vs_fp64_ref": learned["rel_err_vs_fp64_ref"] if head_ms != mixed_ms else err_m,
-        "fp64_ref_rel_residual": relref,
+        "iters": head_iters,
+        "max_true_rel_residual": max_rel,
+        "speedup": speed,
         "learned": learned,
-        "mixed": {"ms": round(mixed_ms, 3), "iters": infom["solver_iter"], "reliable_updates": infom["reliable_updates"],
-                  "true_rel_residual": relm, "rel_err_vs_fp64_ref": err_m},
-        "fp64": {"ms": round(fp64_ms, 3), "iters": info64["solver_iter"], "true_rel_residual": rel64,
-                 "rel_err_vs_fp64_ref": err_64},
-        "speedup_vs_fp64": round(fp64_ms / head_ms, 3),
-        "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
+        "step0": {"mixed": {"ms": round(mixed_ms, 3), "iters": infom["solver_iter"],
+                            "reliable_updates": infom["reliable_updates"], "true_rel_residual": relm},
+                  "fp64": {"ms": round(fp64_ms, 3), "iters": info64["solver_iter"], "true_rel_residual": rel64},
+                  "learned": learned["step0"] if learned is not None else None},
+        "accuracy": acc,
         "setup_ms": round(setup_ms, 1),
+        "refactorize_values_ms": round(refactor_ms, 1),
+        "refactorize_phases_ms": refactor_info["setup_ms"],
         "roofline": roofline,
         "multi_rhs": multi,
         "kernels": kernels,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_line,
     }
     print(json.dumps(line))
     return 0

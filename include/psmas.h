@@ -197,6 +197,15 @@ int psmas_batch_set_coordinates(psmas_handle h, int sys, int nb, int dim, const 
  * the network (and the pairs when drop_pairs != 0). */
 int psmas_learned_train(psmas_handle h, int epochs, int batch, float lr, unsigned seed, double *loss);
 int psmas_learned_reset(psmas_handle h, int drop_pairs);
+/* Ablation of the learned correction (no counterpart in the reference): replace the network by
+ * the exact lattice coarse solve of the current single-system factorisation,
+ * c = (P^T A P)^-1 P^T r, the linear two-level method the network learns to approximate.
+ * P^T A P is assembled on the GPU one column per lattice unknown and inverted on the host; the
+ * inverse is held (BF16) in the network's skip layer with the nonlinear branch zero, so the solve
+ * uses it exactly as a trained network (parameter learned_correction).  *ms receives the time it
+ * took.  With the parameter learned_network = "linear", psmas_learned_train fits the skip layer
+ * alone instead (a learned linear map). */
+int psmas_learned_exact(psmas_handle h, double *ms);
 
 /* Autotuner (no counterpart in the reference): on the current factorisation (single system or
  * batch) and the right-hand sides of the last solve, times the SpMV row-loop batch depth (2, 4, 8;

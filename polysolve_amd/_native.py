@@ -66,6 +66,7 @@ SIGNATURES = {
     "psmas_lattice_prolong": (_i, [_vp, _i, _pd, _pd]),
     "psmas_learned_train": (_i, [_vp, _i, _i, C.c_float, C.c_uint, _pd]),
     "psmas_learned_reset": (_i, [_vp, _i]),
+    "psmas_learned_exact": (_i, [_vp, C.POINTER(C.c_double)]),
     "psmas_device_count": (_i, []),
     "psmas_device_alloc": (_i, [_sz, C.POINTER(_vp)]),
     "psmas_device_free": (_i, [_vp]),

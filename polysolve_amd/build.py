@@ -79,7 +79,7 @@ def build_problems() -> Path:
     out = HERE / "libpsproblems.so"
     tmp = HERE / "libpsproblems.so.tmp"
     cxx = shutil.which("g++") or "g++"
-    _run([cxx, *CXX_FLAGS, "-shared", str(CSRC / "problems.cpp"), "-o", str(tmp)])
+    _run([cxx, *CXX_FLAGS, "-fopenmp", "-shared", str(CSRC / "problems.cpp"), "-o", str(tmp)])
     tmp.replace(out)
     return out
 

@@ -541,6 +541,17 @@ extern "C"
         return guarded(h->last_error, [&] { h->solver.learned_reset(drop_pairs != 0); });
     }
 
+    int psmas_learned_exact(psmas_handle h, double *ms)
+    {
+        if (!h)
+            return null_handle();
+        return guarded(h->last_error, [&] {
+            const double t = h->solver.learned_exact();
+            if (ms)
+                *ms = t;
+        });
+    }
+
     int psmas_set_kernel_timing(psmas_handle h, int on)
     {
         if (!h)

@@ -118,29 +118,37 @@ namespace psmas
                 L[static_cast<size_t>(i) * n + j] = v / d;
             }
         }
-        // W = L^-1 (lower triangular), column by column.
+        // W = L^-1 (lower triangular), row by row: W_i = (e_i - sum_{k<i} L_ik W_k) / L_ii, where
+        // row k of W is non-zero in columns 0..k only (contiguous row updates).
         std::vector<double> W(static_cast<size_t>(n) * n, 0.0);
-        for (int c = 0; c < n; ++c)
+        for (int i = 0; i < n; ++i)
         {
-            W[static_cast<size_t>(c) * n + c] = 1.0 / L[static_cast<size_t>(c) * n + c];
-            for (int i = c + 1; i < n; ++i)
+            double *wi = &W[static_cast<size_t>(i) * n];
+            wi[i] = 1.0;
+            for (int k = 0; k < i; ++k)
             {
-                double v = 0.0;
-                for (int k = c; k < i; ++k)
-                    v -= L[static_cast<size_t>(i) * n + k] * W[static_cast<size_t>(k) * n + c];
-                W[static_cast<size_t>(i) * n + c] = v / L[static_cast<size_t>(i) * n + i];
+                const double l = L[static_cast<size_t>(i) * n + k];
+                const double *wk = &W[static_cast<size_t>(k) * n];
+                for (int c = 0; c <= k; ++c)
+                    wi[c] -= l * wk[c];
+            }
+            const double d = 1.0 / L[static_cast<size_t>(i) * n + i];
+            for (int c = 0; c <= i; ++c)
+                wi[c] *= d;
+        }
+        // Inv = W^T W = sum_k W_k^T W_k (outer products of the rows), row updates again.
+        std::fill(A.begin(), A.end(), 0.0);
+        for (int k = 0; k < n; ++k)
+        {
+            const double *wk = &W[static_cast<size_t>(k) * n];
+            for (int i = 0; i <= k; ++i)
+            {
+                const double a = wk[i];
+                double *ai = &A[static_cast<size_t>(i) * n];
+                for (int c = 0; c <= k; ++c)
+                    ai[c] += a * wk[c];
             }
         }
-        // Inv = W^T W.
-        for (int i = 0; i < n; ++i)
-            for (int j = i; j < n; ++j)
-            {
-                double v = 0.0;
-                for (int k = j; k < n; ++k)
-                    v += W[static_cast<size_t>(k) * n + i] * W[static_cast<size_t>(k) * n + j];
-                A[static_cast<size_t>(i) * n + j] = v;
-                A[static_cast<size_t>(j) * n + i] = v;
-            }
         return true;
     }
 

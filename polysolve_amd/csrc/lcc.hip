@@ -769,7 +769,7 @@ namespace psmas
         trained_ = false;
     }
 
-    double LearnedCoarse::train(int hidden, int epochs, int batch, float lr, unsigned seed)
+    double LearnedCoarse::train(int hidden, int epochs, int batch, float lr, unsigned seed, bool linear)
     {
         const int mm = m_pairs_;
         if (mm == 0 || pairs_u_.empty())
@@ -804,13 +804,41 @@ namespace psmas
         for (size_t i = 0; i < used * md; ++i)
             T[i] = static_cast<float>(T[i] / a);
         net_ = std::make_unique<Mlp>(md, md, hidden, seed, true);
+        if (linear)
+            net_->set_linear_only(true);
         DeviceBuffer<float> dX(used * md, "MAS training inputs"), dT(used * md, "MAS training targets");
         PSMAS_HIP(hipMemcpy(dX.data(), X.data(), dX.bytes(), hipMemcpyHostToDevice));
         PSMAS_HIP(hipMemcpy(dT.data(), T.data(), dT.bytes(), hipMemcpyHostToDevice));
         last_loss_ = net_->train(static_cast<int>(used), dX.data(), dT.data(), epochs, batch, lr, seed, true);
         ascale_ = static_cast<float>(a);
         trained_ = true;
+        kind_ = linear ? "linear" : "mlp";
         return last_loss_;
+    }
+
+    void LearnedCoarse::install_linear(const std::vector<double> &M, int hidden, unsigned seed, const char *kind)
+    {
+        const int mm = m(), md = width();
+        if (mm == 0 || M.size() != static_cast<size_t>(mm) * mm)
+            throw std::runtime_error("[MAS] learned correction: linear map does not match the lattice.");
+        if (!net_ || net_->d_in() != md || net_->hidden() != hidden)
+            net_ = std::make_unique<Mlp>(md, md, hidden, seed, true);
+        // With output scale a = 1 the correction is c = |u| / sqrt(m) * N(u sqrt(m) / |u|) = S u
+        // when N is the skip layer alone.
+        std::vector<float> theta(net_->num_parameters());
+        net_->get_parameters(theta.data());
+        std::fill(theta.begin() + static_cast<long>(net_->offset(4)), theta.begin() + static_cast<long>(net_->offset(6)),
+                  0.0f); // W3, b3
+        float *S = theta.data() + net_->offset(6);
+        std::fill(S, S + static_cast<size_t>(md) * md, 0.0f);
+        for (int i = 0; i < mm; ++i)
+            for (int j = 0; j < mm; ++j)
+                S[static_cast<size_t>(i) * md + j] = static_cast<float>(M[static_cast<size_t>(i) * mm + j]);
+        net_->set_parameters(theta.data());
+        ascale_ = 1.0f;
+        last_loss_ = 0.0;
+        trained_ = true;
+        kind_ = kind;
     }
 
 #define PSMAS_INST_LCC(BD)                                                                                          \

@@ -82,7 +82,14 @@ namespace psmas
         bool trained() const { return trained_; }
         void add_pair(const std::vector<double> &u, const std::vector<double> &w);
         size_t num_pairs() const { return pairs_u_.size() / (m_pairs_ ? m_pairs_ : 1); }
-        double train(int hidden, int epochs, int batch, float lr, unsigned seed);
+        /// Fit the network to the collected pairs; linear: only the skip layer (and the output
+        /// bias) is trained, the nonlinear branch is zero.
+        double train(int hidden, int epochs, int batch, float lr, unsigned seed, bool linear = false);
+        /// Install c = M u as the correction (M: m x m, row-major, coarse unknowns of one system):
+        /// the network's skip layer holds M (BF16), its nonlinear branch and output bias are zero.
+        void install_linear(const std::vector<double> &M, int hidden, unsigned seed, const char *kind);
+        /// "mlp", "linear" (skip layer trained alone) or what install_linear named.
+        const std::string &kind() const { return kind_; }
         void reset_network();
         void clear_pairs();
         float output_scale() const { return ascale_; }
@@ -117,6 +124,7 @@ namespace psmas
         // network and training data
         std::unique_ptr<Mlp> net_;
         bool trained_ = false;
+        std::string kind_;
         float ascale_ = 1.0f;
         float out_factor_ = 1.0f; ///< learned_output_scale of the current restriction
         double last_loss_ = 0.0;

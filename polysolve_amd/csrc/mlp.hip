@@ -5,6 +5,7 @@
 #include "mlp.hpp"
 #include "common.hpp"
 #include "device.hpp"
+#include "kernels.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -337,6 +338,20 @@ namespace psmas
         return loss / static_cast<double>(cnt);
     }
 
+    void Mlp::set_linear_only(bool on)
+    {
+        if (on && !skip_)
+            throw InvalidArgumentError("[MAS] a linear-only network needs the skip layer.");
+        linear_only_ = on;
+        if (on)
+        {
+            std::vector<float> theta(nparams_);
+            get_parameters(theta.data());
+            std::fill(theta.begin() + static_cast<long>(offs_[4]), theta.begin() + static_cast<long>(offs_[5]), 0.0f);
+            set_parameters(theta.data());
+        }
+    }
+
     void Mlp::adam_step(float lr, float weight_decay)
     {
         ++step_;
@@ -390,6 +405,8 @@ namespace psmas
                                    tb.data());
                 PSMAS_LAUNCH_CHECK();
                 sum += loss_and_grad_device(batch, xb.data(), tb.data());
+                if (linear_only_) // W1, b1, W2, b2, W3 stay as they are (W3 = 0)
+                    launch_fill<float>(static_cast<long long>(offs_[5]), g_.data(), 0.0f, stream_);
                 adam_step(lr_ep, 0.0f);
                 ++nb;
             }

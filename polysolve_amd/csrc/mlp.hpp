@@ -43,6 +43,11 @@ namespace psmas
         /// learning rate decays as lr (1 + cos(pi epoch / epochs)) / 2.
         double train(int n, const float *X, const float *T, int epochs, int batch, float lr, unsigned seed,
                      bool cosine = false);
+        /// Linear mode: W3 is set to zero and only b3 and the skip layer S are trained (the
+        /// gradients of the hidden layers and of W3 are dropped), so y = S x + b3.  Needs a skip
+        /// layer.
+        void set_linear_only(bool on);
+        bool linear_only() const { return linear_only_; }
         hipStream_t stream() const { return stream_; }
         const unsigned short *bf16_weights() const { return wb_.data(); }
         const float *fp32_weights() const { return w_.data(); }
@@ -51,6 +56,7 @@ namespace psmas
     private:
         int d_in_, d_out_, h_;
         bool skip_ = false;
+        bool linear_only_ = false;
         size_t sizes_[7] = {}, offs_[7] = {};
         size_t nparams_ = 0;
         int step_ = 0;

@@ -277,7 +277,7 @@ namespace psmas
             "reliable_update_delta", "check_interval", "true_residual_period", "use_graphs",
             "learned_correction", "division_guard", "learned_collect", "lattice_points", "learned_hidden",
             "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll", "spmv_halo", "device_setup", "spmv_copies",
-            "restrict_piece", "apply_workgroups"};
+            "restrict_piece", "apply_workgroups", "learned_network"};
     } // namespace
 
     void apply_json_parameters(Params &p, const std::string &json)
@@ -423,6 +423,13 @@ namespace psmas
             if (q.restrict_piece <= RESTRICT_SMALL_MAX || q.restrict_piece > RESTRICT_PIECE)
                 bad("restrict_piece", "must be in [9, 512]");
         }
+        if (const JValue *v = obj->get("learned_network"))
+        {
+            const std::string k = as_string(*v, "learned_network");
+            if (k != "mlp" && k != "linear")
+                bad("learned_network", "must be \"mlp\" or \"linear\"");
+            q.learned_linear = k == "linear";
+        }
         if (const JValue *v = obj->get("apply_workgroups"))
         {
             q.apply_workgroups = as_int(*v, "apply_workgroups");
@@ -468,7 +475,8 @@ namespace psmas
           << ", \"learned_snapshots\": " << p.learned_snapshots
           << ", \"learned_stall_segments\": " << p.learned_stall_segments
           << ", \"learned_output_scale\": " << json_number(p.learned_output_scale) << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"spmv_halo\": " << (p.spmv_halo ? "true" : "false") << ", \"device_setup\": " << (p.device_setup ? "true" : "false") << ", \"spmv_copies\": " << (p.spmv_copies ? "true" : "false") << ", \"restrict_piece\": " << p.restrict_piece
-          << ", \"apply_workgroups\": " << p.apply_workgroups << "}";
+          << ", \"apply_workgroups\": " << p.apply_workgroups
+          << ", \"learned_network\": \"" << (p.learned_linear ? "linear" : "mlp") << "\"}";
         return o.str();
     }
 } // namespace psmas

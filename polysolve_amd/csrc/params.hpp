@@ -27,6 +27,8 @@
 //   learned_stall_segments  check intervals without a new smallest residual before the learned
 //                        correction is switched off for the rest of the solve
 //   learned_output_scale factor on the learned correction (1 = as trained)
+//   learned_network      "mlp" (default: hidden layers plus the linear skip layer) | "linear"
+//                        (only the skip layer is trained: a learned linear coarse map)
 //   spmv_unroll          SpMV row-loop batch depth 2 | 4 | 8 (chosen by psmas_autotune)
 //   spmv_copies          after factorize_copies, one SpMV launch reads the matrix once for every
 //                        right-hand side (default true; false: the batch's per-system SpMV)
@@ -102,6 +104,7 @@ namespace psmas
         bool spmv_copies = true;
         int restrict_piece = 512;
         int apply_workgroups = 0;
+        bool learned_linear = false;
     };
 
     /// Apply a JSON object to params.  Accepts either {"MAS": {...}} (only the "MAS" member is

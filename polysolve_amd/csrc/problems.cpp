@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -29,9 +30,10 @@ namespace
     struct Problem
     {
         int n = 0;
+        size_t nnz = 0;
         std::vector<int> colptr;
-        std::vector<int> rowidx;
-        std::vector<double> vals;
+        std::unique_ptr<int[]> rowidx;  // nnz, written in parallel (no serial zero fill)
+        std::unique_ptr<double[]> vals; // nnz
         std::vector<double> rhs;
     };
 
@@ -128,35 +130,44 @@ namespace
 
         // 27-point stencil blocks per node, slot s = (dk+1)*9 + (dj+1)*3 + (di+1); slot order
         // equals ascending neighbour id.  blocks[node][slot] holds A(node, neighbour).
-        std::vector<double> blocks(static_cast<size_t>(nn) * 27 * 9, 0.0);
+        // Left uninitialised here: each node's thread zeroes its own blocks first (parallel first
+        // touch of ~0.5 GB on 64^3).
+        std::unique_ptr<double[]> blocks(new double[static_cast<size_t>(nn) * 27 * 9]);
         std::vector<double> rhs(static_cast<size_t>(nn) * 3, 0.0);
         const double fnode = body_force_z * h * h * h / 8.0;
 
-        for (int k = 0; k < ne; ++k)
-            for (int j = 0; j < ne; ++j)
-                for (int i = 0; i < ne; ++i)
-                {
-                    const int e = i + ne * (j + ne * k);
-                    const double s = E_elem[e] * h;
-                    int nodes[8];
-                    for (int a = 0; a < 8; ++a)
-                        nodes[a] = (i + (a & 1)) + N * ((j + ((a >> 1) & 1)) + N * (k + ((a >> 2) & 1)));
-                    for (int a = 0; a < 8; ++a)
+        // Element contributions gathered per node (one thread per node, no write conflicts): node
+        // a visits its adjacent elements in ascending element id, which is the order an element
+        // loop would add them in, so every block and load entry gets the same sums bit for bit.
+#pragma omp parallel for schedule(static)
+        for (int a = 0; a < nn; ++a)
+        {
+            std::fill(&blocks[static_cast<size_t>(a) * 27 * 9], &blocks[static_cast<size_t>(a + 1) * 27 * 9], 0.0);
+            const int ai = a % N, aj = (a / N) % N, ak = a / (N * N);
+            for (int ek = ak - 1; ek <= ak; ++ek)
+                for (int ej = aj - 1; ej <= aj; ++ej)
+                    for (int ei = ai - 1; ei <= ai; ++ei)
                     {
-                        rhs[3 * nodes[a] + 2] += fnode;
+                        if (ei < 0 || ei >= ne || ej < 0 || ej >= ne || ek < 0 || ek >= ne)
+                            continue;
+                        const int e = ei + ne * (ej + ne * ek);
+                        const double sc = E_elem[e] * h;
+                        // local index of node a in element e
+                        const int la = (ai - ei) | ((aj - ej) << 1) | ((ak - ek) << 2);
+                        rhs[3 * static_cast<size_t>(a) + 2] += fnode;
                         for (int b = 0; b < 8; ++b)
                         {
-                            const int di = (b & 1) - (a & 1);
-                            const int dj = ((b >> 1) & 1) - ((a >> 1) & 1);
-                            const int dk = ((b >> 2) & 1) - ((a >> 2) & 1);
+                            const int di = (b & 1) - (la & 1);
+                            const int dj = ((b >> 1) & 1) - ((la >> 1) & 1);
+                            const int dk = ((b >> 2) & 1) - ((la >> 2) & 1);
                             const int slot = (dk + 1) * 9 + (dj + 1) * 3 + (di + 1);
-                            double *blk = &blocks[(static_cast<size_t>(nodes[a]) * 27 + slot) * 9];
+                            double *blk = &blocks[(static_cast<size_t>(a) * 27 + slot) * 9];
                             for (int r = 0; r < 3; ++r)
                                 for (int c = 0; c < 3; ++c)
-                                    blk[r * 3 + c] += s * Ku[3 * a + r][3 * b + c];
+                                    blk[r * 3 + c] += sc * Ku[3 * la + r][3 * b + c];
                         }
                     }
-                }
+        }
 
         // Pair blocks: kappa * g g^T with g = (g_i, g_j) in R^6.
         std::vector<ExtraBlock> extra;
@@ -233,60 +244,84 @@ namespace
                     P->rhs[3 * a + c] = 0.0;
 
         // Emit CSC.  Column (a, comp) lists rows (b, r) for neighbours b of a in ascending order,
-        // value A(b, a)[r][comp] = A(a, b)[comp][r] (the matrix is symmetric).
-        P->colptr.resize(static_cast<size_t>(P->n) + 1);
-        size_t nnz_est = static_cast<size_t>(nn) * 27 * 9 + merged.size() * 9;
-        P->rowidx.reserve(nnz_est);
-        P->vals.reserve(nnz_est);
-        P->colptr[0] = 0;
-        for (int a = 0; a < nn; ++a)
-        {
+        // value A(b, a)[r][comp] = A(a, b)[comp][r] (the matrix is symmetric).  Two passes over
+        // the nodes (entry counts, then the entries at their scanned offsets), each parallel.
+        auto neighbours = [&](int a, std::vector<std::pair<int, const double *>> &all) {
             const int ai = a % N, aj = (a / N) % N, ak = a / (N * N);
-            // Gather (neighbour, block pointer) in ascending neighbour order.
-            struct Nb
-            {
-                int node;
-                const double *blk;
-            };
-            Nb list[27 + 64];
-            int cnt = 0;
+            all.clear();
             for (int s = 0; s < 27; ++s)
             {
                 const int di = s % 3 - 1, dj = (s / 3) % 3 - 1, dk = s / 9 - 1;
                 const int bi = ai + di, bj = aj + dj, bk = ak + dk;
                 if (bi < 0 || bi >= N || bj < 0 || bj >= N || bk < 0 || bk >= N)
                     continue;
-                list[cnt++] = Nb{bi + N * (bj + N * bk), &blocks[(static_cast<size_t>(a) * 27 + s) * 9]};
+                all.emplace_back(bi + N * (bj + N * bk), &blocks[(static_cast<size_t>(a) * 27 + s) * 9]);
             }
-            std::vector<Nb> all(list, list + cnt);
-            for (int t = extra_ptr[a]; t < extra_ptr[a + 1]; ++t)
+            if (extra_ptr[a + 1] > extra_ptr[a])
             {
-                // Off-stencil blocks: merge into the stencil order.
-                all.push_back(Nb{merged[t].col, merged[t].v});
+                for (int t = extra_ptr[a]; t < extra_ptr[a + 1]; ++t) // off-stencil blocks
+                    all.emplace_back(merged[t].col, merged[t].v);
+                std::stable_sort(all.begin(), all.end(),
+                                 [](const std::pair<int, const double *> &x, const std::pair<int, const double *> &y) {
+                                     return x.first < y.first;
+                                 });
             }
-            std::sort(all.begin(), all.end(), [](const Nb &x, const Nb &y) { return x.node < y.node; });
-            for (int comp = 0; comp < 3; ++comp)
+        };
+        // entries of column (a, comp): 3 per free neighbour, 1 (the diagonal) for a Dirichlet node
+        std::vector<long long> per_node(static_cast<size_t>(nn) + 1, 0);
+#pragma omp parallel
+        {
+            std::vector<std::pair<int, const double *>> all;
+#pragma omp for schedule(static)
+            for (int a = 0; a < nn; ++a)
             {
+                neighbours(a, all);
+                long long cnt = 0;
                 for (const auto &nb : all)
+                    cnt += (dirichlet(a) || dirichlet(nb.first)) ? (a == nb.first ? 1 : 0) : 3;
+                per_node[a + 1] = cnt; // per component
+            }
+        }
+        for (int a = 0; a < nn; ++a)
+            per_node[a + 1] = per_node[a] + 3 * per_node[a + 1];
+        if (per_node[nn] > 2147483647LL)
+            throw std::bad_alloc();
+        P->colptr.resize(static_cast<size_t>(P->n) + 1);
+        P->nnz = static_cast<size_t>(per_node[nn]);
+        P->rowidx.reset(new int[P->nnz]);
+        P->vals.reset(new double[P->nnz]);
+        P->colptr[0] = 0;
+#pragma omp parallel
+        {
+            std::vector<std::pair<int, const double *>> all;
+#pragma omp for schedule(static)
+            for (int a = 0; a < nn; ++a)
+            {
+                neighbours(a, all);
+                size_t at = static_cast<size_t>(per_node[a]);
+                for (int comp = 0; comp < 3; ++comp)
                 {
-                    const int b = nb.node;
-                    if (dirichlet(a) || dirichlet(b))
+                    for (const auto &nb : all)
                     {
-                        if (a == b)
+                        const int b = nb.first;
+                        if (dirichlet(a) || dirichlet(b))
                         {
-                            // Dirichlet diagonal: only the (comp, comp) entry, value 1.
-                            P->rowidx.push_back(3 * b + comp);
-                            P->vals.push_back(1.0);
+                            if (a == b)
+                            {
+                                // Dirichlet diagonal: only the (comp, comp) entry, value 1.
+                                P->rowidx[at] = 3 * b + comp;
+                                P->vals[at++] = 1.0;
+                            }
+                            continue;
                         }
-                        continue;
+                        for (int r = 0; r < 3; ++r)
+                        {
+                            P->rowidx[at] = 3 * b + r;
+                            P->vals[at++] = nb.second[comp * 3 + r];
+                        }
                     }
-                    for (int r = 0; r < 3; ++r)
-                    {
-                        P->rowidx.push_back(3 * b + r);
-                        P->vals.push_back(nb.blk[comp * 3 + r]);
-                    }
+                    P->colptr[3 * static_cast<size_t>(a) + comp + 1] = static_cast<int>(at);
                 }
-                P->colptr[3 * a + comp + 1] = static_cast<int>(P->rowidx.size());
             }
         }
         return P;
@@ -316,7 +351,7 @@ extern "C"
             Problem *P = assemble(ne, E_elem, nu, body_force_z, npairs, pair_i, pair_j, pair_g, pair_kappa,
                                   fix_bottom != 0);
             *n_out = P->n;
-            *nnz_out = static_cast<long long>(P->rowidx.size());
+            *nnz_out = static_cast<long long>(P->nnz);
             return P;
         }
         catch (const std::bad_alloc &)
@@ -332,8 +367,15 @@ extern "C"
             return 1;
         const Problem *P = static_cast<const Problem *>(handle);
         std::memcpy(colptr, P->colptr.data(), sizeof(int) * P->colptr.size());
-        std::memcpy(rowidx, P->rowidx.data(), sizeof(int) * P->rowidx.size());
-        std::memcpy(vals, P->vals.data(), sizeof(double) * P->vals.size());
+        const long long nnz = static_cast<long long>(P->nnz);
+        const long long chunk = 1 << 20;
+#pragma omp parallel for schedule(static)
+        for (long long c0 = 0; c0 < nnz; c0 += chunk)
+        {
+            const size_t len = static_cast<size_t>(std::min(chunk, nnz - c0));
+            std::memcpy(rowidx + c0, P->rowidx.get() + c0, sizeof(int) * len);
+            std::memcpy(vals + c0, P->vals.get() + c0, sizeof(double) * len);
+        }
         std::memcpy(rhs, P->rhs.data(), sizeof(double) * P->rhs.size());
         return 0;
     }

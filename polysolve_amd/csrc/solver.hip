@@ -313,6 +313,7 @@ namespace psmas
           << ", \"fallback\": " << (learned_fallback_ ? "true" : "false")
           << ", \"fallback_iteration\": " << learned_fallback_iter_
           << ", \"trained\": " << (lcc_ && lcc_->trained() ? "true" : "false")
+          << ", \"network\": \"" << (lcc_ && lcc_->trained() ? lcc_->kind() : std::string()) << "\""
           << ", \"pairs\": " << (lcc_ ? lcc_->num_pairs() : 0)
           << ", \"train_loss\": " << json_number(lcc_ ? lcc_->last_loss() : 0.0)
           << ", \"lattice_points\": " << (lcc_ && lcc_->lattice_ready() ? lcc_->k() : 0)
@@ -2301,9 +2302,58 @@ namespace psmas
             throw std::runtime_error("[MAS] learned correction: no training pairs collected.");
         if (epochs < 1 || batch < 1 || !(lr > 0.0f))
             throw InvalidArgumentError("[MAS] learned correction: epochs, batch and lr must be positive.");
-        const double loss = lcc_->train(params_.learned_hidden, epochs, batch, lr, seed);
+        const double loss = lcc_->train(params_.learned_hidden, epochs, batch, lr, seed, params_.learned_linear);
         graphs_->clear(); // captured launches hold the network's buffers
         return loss;
+    }
+
+    double Solver::learned_exact()
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        PSMAS_HIP(hipSetDevice(device_));
+        check_ready_single(dim());
+        ensure_lattice();
+        const int m = lcc_->m();
+        std::vector<double> G(static_cast<size_t>(m) * m);
+        auto body = [&](auto bdc) {
+            constexpr int BD = decltype(bdc)::value;
+            const long long nvec = static_cast<long long>(npad_) * BD;
+            std::vector<double> w(m, 0.0);
+            for (int j = 0; j < m; ++j)
+            {
+                // column j: P^T A P e_j
+                w[j] = 1.0;
+                lcc_->prolong_host_coef<BD>(w, d_p_.data(), nvec, stream_);
+                launch_spmv_plain<BD, double, double>(sell(d_vals64_.data()), d_p_.data(), d_Ap_.data(), stream_);
+                lcc_->restrict_to_lattice<BD, double>(d_Ap_.data(), nullptr, stream_);
+                const std::vector<double> u = lcc_->download_u(stream_);
+                for (int i = 0; i < m; ++i)
+                    G[static_cast<size_t>(i) * m + j] = u[i];
+                w[j] = 0.0;
+            }
+        };
+        if (bd_ == 1)
+            body(std::integral_constant<int, 1>());
+        else if (bd_ == 2)
+            body(std::integral_constant<int, 2>());
+        else
+            body(std::integral_constant<int, 3>());
+        for (int i = 0; i < m; ++i)
+        {
+            for (int j = i + 1; j < m; ++j) // symmetric part (the sums differ in rounding only)
+            {
+                const double v = 0.5 * (G[static_cast<size_t>(i) * m + j] + G[static_cast<size_t>(j) * m + i]);
+                G[static_cast<size_t>(i) * m + j] = G[static_cast<size_t>(j) * m + i] = v;
+            }
+            // A lattice unknown no coupled node touches has a zero row: its u entry is always 0.
+            if (G[static_cast<size_t>(i) * m + i] == 0.0)
+                G[static_cast<size_t>(i) * m + i] = 1.0;
+        }
+        if (!cholesky_inverse(m, G))
+            throw std::runtime_error("[MAS] learned correction: lattice Galerkin matrix not positive definite.");
+        lcc_->install_linear(G, params_.learned_hidden, 1u, "galerkin");
+        graphs_->clear(); // captured launches hold the network's buffers
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 
     void Solver::learned_reset(bool drop_pairs)

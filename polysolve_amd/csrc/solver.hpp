@@ -100,6 +100,11 @@ namespace psmas
         double learned_train(int epochs, int batch, float lr, unsigned seed);
         /// Drop the trained network (and the collected pairs when drop_pairs).
         void learned_reset(bool drop_pairs);
+        /// Replace the network by the exact lattice coarse solve of the current factorisation,
+        /// c = (P^T A P)^-1 P^T r (the linear two-level method the network approximates):
+        /// P^T A P is assembled column by column on the GPU, inverted on the host (Cholesky) and
+        /// held in the skip layer.  Returns the wall time in ms.
+        double learned_exact();
 
         /// Time the tile / schedule candidates on this factorisation and the last right-hand side,
         /// keep the fastest in the parameters, return a JSON report.

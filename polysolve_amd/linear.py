@@ -382,6 +382,13 @@ class MASSolver(Solver):
     def learned_reset(self, drop_pairs: bool = False) -> None:
         _check(N.lib().psmas_learned_reset(self._h, int(bool(drop_pairs))), self._h)
 
+    def learned_exact(self) -> float:
+        """Replace the network by the exact lattice coarse solve (P^T A P)^-1 P^T of the current
+        factorisation (ablation); returns the ms it took."""
+        ms = C.c_double(0.0)
+        _check(N.lib().psmas_learned_exact(self._h, C.byref(ms)), self._h)
+        return ms.value
+
     def set_kernel_timing(self, on: bool) -> None:
         _check(N.lib().psmas_set_kernel_timing(self._h, int(on)), self._h)
 

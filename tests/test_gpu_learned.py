@@ -128,6 +128,35 @@ def test_learned_correction_reduces_iterations(gpu, drift_sequence, trained_solv
     assert sum(learned) <= 0.75 * sum(plain), (plain, learned)
 
 
+def test_exact_and_linear_corrections(gpu, drift_sequence):
+    """Ablations of the network: the exact lattice coarse solve (P^T A P)^-1 P^T r installed in its
+    skip layer, and a skip-layer-only network trained on the same pairs, both inside the same
+    flexible PCG.  Each reaches the FP64 tolerance with fewer iterations than plain MAS."""
+    p = drift_sequence[EVAL[0]]
+    s = _solver({"lattice_points": 5, "learned_collect": True, "learned_random_pairs": 120,
+                 "learned_snapshots": 64, "learned_network": "linear"})
+    for t in TRAIN:
+        q = drift_sequence[t]
+        s.set_coordinates(q.coords)
+        s.factorize(q.A)
+        s.solve(q.b, np.zeros(q.n))
+    s.learned_train(epochs=200, batch=64, lr=3e-3, seed=0)
+    s.set_parameters({"MAS": {"learned_collect": False}})
+    plain, _, x0 = _solve(s, p, False)
+    lin, r_lin, x_lin = _solve(s, p, True)
+    assert lin["learned"]["network"] == "linear" and lin["learned"]["used"]
+    ms = s.learned_exact()
+    assert ms > 0
+    ex, r_ex, x_ex = _solve(s, p, True)  # refactorises the same matrix: the installed map stays
+    assert ex["learned"]["network"] == "galerkin" and ex["learned"]["used"]
+    print("plain", plain["solver_iter"], "linear", lin["solver_iter"], "exact", ex["solver_iter"])
+    for info, rel, x in ((lin, r_lin, x_lin), (ex, r_ex, x_ex)):
+        assert info["solver_status"] == "Reach relative tolerance" and rel <= 1e-8
+        assert not info["learned"]["fallback"]
+        assert info["solver_iter"] < plain["solver_iter"]
+        assert np.linalg.norm(x - x0) <= 1e-6 * np.linalg.norm(x0)
+
+
 def test_learned_fallback_to_plain_mas_on_stall(gpu, drift_sequence, trained_solver):
     """A correction scaled far beyond what was trained stops the residual from decreasing; plain
     MAS takes over and the solve still reaches the FP64 tolerance."""

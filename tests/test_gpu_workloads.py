@@ -1,0 +1,120 @@
+"""The benchmark's workloads at their stated sizes, on the GPU.
+
+* Contact-like 48^3 elasticity with 20k random barrier pairs (n = 352,947): the mixed path
+  reaches FP64 relative residual 1e-8 and its FP64 MAS apply equals the oracle's.
+* 64 uneven systems (edges U{8..32}, about 2.5 M DOF) solved as one batch: every system reaches
+  the tolerance, and three sampled systems factorised alone match the oracle apply.
+* The drifting 64^3 sequence: a network trained on pairs from early steps cuts the iterations of
+  held-out later steps (51 and beyond) against plain MAS at the same FP64 accuracy.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from polysolve_amd import MASSolver, problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(p, x, b=None):
+    b = p.b if b is None else b
+    return float(np.linalg.norm(b - p.A @ x) / np.linalg.norm(b))
+
+
+def test_contact_48_with_20k_pairs(gpu):
+    p = problems.config3(seed=0)
+    assert p.n == 352947
+    s = MASSolver()
+    s.set_parameters({"MAS": {"block_dim": 3, "precision": "mixed", "coarse_solver": "cholesky",
+                              "relative_tolerance": 1e-8, "absolute_tolerance": 0.0, "max_iter": 20000}})
+    s.set_coordinates(p.coords)
+    s.factorize(p.A)
+    x = np.zeros(p.n)
+    s.solve(p.b, x)
+    info = s.get_info()
+    print("contact 48^3:", info["solver_iter"], "it,", info["reliable_updates"], "updates, halo", info["spmv_halo"])
+    assert info["solver_status"] == "Reach relative tolerance"
+    assert _rel(p, x) <= 1e-8
+    # The FP64 apply of the bank-only preconditioner equals the oracle's on this matrix.
+    s.set_parameters({"MAS": {"coarse_solver": "bank"}})
+    offsets, perm = s.partition()
+    O = oracle.MasOracle(p.A, 3, offsets, perm)
+    r = np.random.default_rng(3).standard_normal(p.n)
+    zr = O.apply(r)
+    assert np.linalg.norm(s.apply_preconditioner(r) - zr) <= 1e-10 * np.linalg.norm(zr)
+
+
+def test_batch_of_64_uneven_systems(gpu):
+    probs = problems.config4(seed=0)
+    assert len(probs) == 64
+    prm = {"block_dim": 3, "precision": "mixed", "coarse_solver": "cholesky", "relative_tolerance": 1e-8,
+           "absolute_tolerance": 0.0, "max_iter": 20000}
+    s = MASSolver()
+    s.set_parameters({"MAS": prm})
+    s.factorize_batch([p.A for p in probs], coords=[p.coords for p in probs])
+    xs = [np.zeros(p.n) for p in probs]
+    s.solve_batch([p.b for p in probs], xs)
+    worst = 0.0
+    for i, p in enumerate(probs):
+        assert s.batch_info(i)["solver_status"] == "Reach relative tolerance", i
+        worst = max(worst, _rel(p, xs[i]))
+    print("batch of 64: worst true relative residual", worst)
+    assert worst <= 1e-8
+    # Three systems (smallest, largest, one between) factorised alone: FP64 apply vs the oracle.
+    sizes = np.array([p.n for p in probs])
+    rng = np.random.default_rng(4)
+    for i in (int(np.argmin(sizes)), int(np.argmax(sizes)), 17):
+        p = probs[i]
+        one = MASSolver()
+        one.set_parameters({"MAS": {"block_dim": 3}})
+        one.set_coordinates(p.coords)
+        one.factorize(p.A)
+        offsets, perm = one.partition()
+        O = oracle.MasOracle(p.A, 3, offsets, perm)
+        r = rng.standard_normal(p.n)
+        zr = O.apply(r)
+        assert np.linalg.norm(one.apply_preconditioner(r) - zr) <= 1e-10 * np.linalg.norm(zr), i
+
+
+def test_drifting_64_learned_on_held_out_steps(gpu):
+    """Pairs from solves of steps 1..8 (the network never sees the evaluated systems), evaluation
+    on steps 60 and 150 of the 200-step sequence."""
+    ne = 64
+    train_steps, eval_steps = range(1, 9), (60, 150)
+    moduli = {}
+    for t, E in enumerate(problems.config5_moduli(ne, steps=max(eval_steps) + 1, seed=0)):
+        if t in train_steps or t in eval_steps:
+            moduli[t] = E
+    s = MASSolver()
+    s.set_parameters({"MAS": {"block_dim": 3, "precision": "mixed", "coarse_solver": "cholesky",
+                              "relative_tolerance": 1e-8, "absolute_tolerance": 0.0, "max_iter": 20000,
+                              "lazy_partitioning": True, "learned_collect": True, "lattice_points": 8,
+                              "learned_random_pairs": 200}})
+    s.set_coordinates(problems.grid_coords(ne))
+    for t in train_steps:
+        p = problems.hex_elasticity(ne, E=moduli[t])
+        s.factorize(p.A)
+        x = np.zeros(p.n)
+        s.solve(p.b, x)
+    loss = s.learned_train(epochs=150, batch=64, lr=3e-3, seed=0)
+    assert np.isfinite(loss)
+    s.set_parameters({"MAS": {"learned_collect": False}})
+    for t in eval_steps:
+        p = problems.hex_elasticity(ne, E=moduli[t])
+        s.factorize(p.A)
+        assert s.get_info()["factorize_mode"] == "values"
+        res = {}
+        for learned in (False, True):
+            s.set_parameters({"MAS": {"learned_correction": learned}})
+            x = np.zeros(p.n)
+            s.solve(p.b, x)
+            info = s.get_info()
+            assert info["solver_status"] == "Reach relative tolerance"
+            assert _rel(p, x) <= 1e-8
+            res[learned] = (info["solver_iter"], info["learned"], x)
+        print(f"step {t}: plain {res[False][0]} it, learned {res[True][0]} it")
+        assert res[True][1]["used"] and not res[True][1]["fallback"]
+        assert res[True][0] < res[False][0]
+        assert np.linalg.norm(res[True][2] - res[False][2]) <= 1e-6 * np.linalg.norm(res[False][2])
