@@ -1057,82 +1057,243 @@ namespace psmas
         }
 
         // ---------------------------------------------------------------------- build: inverse
-        /// Gauss-Jordan inverse (no pivoting; the matrices are SPD) of one packed matrix per
-        /// workgroup on the full matrix in LDS.  A zero diagonal entry is first replaced by 1
-        /// (virtual slots become identity rows, MASPreconditioner.cu:779-796).  Flags a zero or
-        /// non-finite pivot, a non-finite entry or a non-positive diagonal of the inverse.
+        using f64x4 = double __attribute__((ext_vector_type(4)));
+
+        /// In-place inverse of one packed SPD matrix per workgroup by block Gauss-Jordan with 4x4
+        /// pivot blocks, the rank-4 updates on FP64 MFMA (v_mfma_f64_16x16x4f64).  A zero diagonal
+        /// entry is first replaced by 1 (virtual slots become identity rows,
+        /// MASPreconditioner.cu:779-796).
+        ///
+        /// The N x N matrix (N = 32 BD) stays in registers as (N/16)^2 tiles of 16 x 16, each wave
+        /// of the four owning a BD x BD quadrant of tiles in the MFMA result layout (lane l,
+        /// register r: row 4r + l/16, column l%16).  Step K (N/4 steps) sweeps the rows and columns
+        /// s = 4K..4K+3: with the strip rows R = A[s, :] as they are (a panel in LDS, the strip rows
+        /// being register K%4 of the owning tiles) and P = A[s, s],
+        ///   A[i, j] -= A[i, s] P^-1 R[:, j]      (all i, j; one MFMA per tile),
+        ///   A[s, j] = P^-1 R[:, j],  A[i, s] = -A[i, s] P^-1,  A[s, s] = P^-1,
+        /// where A[i, s] = R[:, i]^T for rows not yet swept and -R[:, i]^T for swept rows (the
+        /// Gauss-Jordan sweep keeps that relation), so the panel alone feeds both MFMA operands:
+        /// A operand -(+-)R[k][i] (natural layout: lane i + 16k) and B operand X = P^-1 R (lane
+        /// j + 16k).  P^-1 comes from a 4x4 Cholesky computed by the wave that owns the pivot tile,
+        /// which gathers P with lane shuffles.  Panel and P^-1 are double-buffered, so a step
+        /// needs one barrier.  Flags a non-positive or non-finite pivot, a non-finite entry or a
+        /// non-positive diagonal of the inverse.
         template <int BD>
         __global__ __launch_bounds__(256) void k_invert(double *__restrict__ mats, int *fail)
         {
             constexpr int N = 32 * BD;
             constexpr int L = N * (N + 1) / 2;
-            constexpr int LD = N + 1; // padded leading dimension against bank conflicts
-            __shared__ double sA[N * LD];
-            __shared__ double rowk[N];
-            __shared__ double colk[N];
-            __shared__ double pinv;
+            constexpr int H = BD;     // tiles per wave and dimension
+            constexpr int PW = N + 4; // panel row stride (doubles)
+            __shared__ double panel[2][4 * PW];
+            __shared__ double pinv_s[2][16];
             __shared__ int bad;
             double *g = mats + static_cast<size_t>(blockIdx.x) * L;
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            const int lr = lane >> 4, lc = lane & 15; // result layout: row 4r + lr, column lc
+            const int ti0 = H * (wave >> 1), tj0 = H * (wave & 1);
             if (threadIdx.x == 0)
                 bad = 0;
-            for (int i = threadIdx.x; i < N; i += 256)
-            {
-                const int rs = i * N - (i * (i + 1)) / 2;
-                for (int j = i; j < N; ++j)
+            auto packed = [&](int i, int j) {
+                if (i > j)
                 {
-                    double v = g[rs + j];
-                    if (i == j && v == 0.0)
-                        v = 1.0;
-                    sA[i * LD + j] = v;
-                    sA[j * LD + i] = v;
+                    const int t = i;
+                    i = j;
+                    j = t;
                 }
-            }
-            __syncthreads();
-            for (int k = 0; k < N; ++k)
-            {
-                if (threadIdx.x == 0)
-                {
-                    const double piv = sA[k * LD + k];
-                    if (!isfinite(piv) || piv == 0.0)
-                        bad = 1;
-                    pinv = 1.0 / piv;
-                }
+                return i * N - (i * (i + 1)) / 2 + j;
+            };
+            f64x4 C[H][H];
+#pragma unroll
+            for (int a = 0; a < H; ++a)
+#pragma unroll
+                for (int b = 0; b < H; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                    {
+                        const int row = 16 * (ti0 + a) + 4 * r + lr, col = 16 * (tj0 + b) + lc;
+                        double v = g[packed(row, col)];
+                        if (row == col && v == 0.0)
+                            v = 1.0;
+                        C[a][b][r] = v;
+                    }
+            int flag = 0;
+            // One sweep step; q = K % 4 is a compile-time constant and tiles are selected by
+            // predicates, so every register array index is static (no scratch indexing).
+            auto step = [&](const int K, auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const int buf = K & 1, tK = K >> 2, s0 = 4 * K;
+                double *pn = panel[buf];
+                // Strip rows -> panel (register q of the tiles in tile row tK).
+#pragma unroll
+                for (int a = 0; a < H; ++a)
+                    if (ti0 + a == tK)
+                    {
+#pragma unroll
+                        for (int b = 0; b < H; ++b)
+                            pn[lr * PW + 16 * (tj0 + b) + lc] = C[a][b][q];
+                    }
+                // P^-1 by the wave owning the pivot tile: P[m][c] is register q of lane 16m + 4q + c.
+#pragma unroll
+                for (int a = 0; a < H; ++a)
+                    if (ti0 + a == tK && tj0 + a == tK)
+                    {
+                        double P[4][4];
+#pragma unroll
+                        for (int m = 0; m < 4; ++m)
+#pragma unroll
+                            for (int c = 0; c < 4; ++c)
+                                P[m][c] = __shfl(C[a][a][q], 16 * m + 4 * q + c);
+                        // Cholesky P = L L^T (lower triangle read), W = L^-1, P^-1 = W^T W.
+                        double l[4][4] = {};
+                        int ok = 1;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                        {
+                            double d = P[j][j];
+#pragma unroll
+                            for (int k = 0; k < j; ++k)
+                                d -= l[j][k] * l[j][k];
+                            ok &= (d > 0.0 && isfinite(d)) ? 1 : 0;
+                            d = sqrt(d > 0.0 ? d : 1.0);
+                            l[j][j] = d;
+                            const double inv = 1.0 / d;
+#pragma unroll
+                            for (int i = j + 1; i < 4; ++i)
+                            {
+                                double v = P[i][j];
+#pragma unroll
+                                for (int k = 0; k < j; ++k)
+                                    v -= l[i][k] * l[j][k];
+                                l[i][j] = v * inv;
+                            }
+                        }
+                        double w[4][4] = {};
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                        {
+                            w[i][i] = 1.0 / l[i][i];
+#pragma unroll
+                            for (int j = 0; j < i; ++j)
+                            {
+                                double v = 0.0;
+#pragma unroll
+                                for (int k = j; k < i; ++k)
+                                    v -= l[i][k] * w[k][j];
+                                w[i][j] = v * w[i][i];
+                            }
+                        }
+                        if (lane == 0) // static indices only (no scratch)
+                        {
+#pragma unroll
+                            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                                for (int c = 0; c < 4; ++c)
+                                {
+                                    double v = 0.0;
+#pragma unroll
+                                    for (int k = 0; k < 4; ++k)
+                                        v += w[k][m] * w[k][c];
+                                    pinv_s[buf][4 * m + c] = v;
+                                }
+                        }
+                        flag |= ok ? 0 : 1;
+                    }
                 __syncthreads();
-                const double pi = pinv;
-                for (int i = threadIdx.x; i < N; i += 256)
+                // Row lr of P^-1, X = P^-1 R for this wave's tile columns (B operands), the A
+                // operands for its tile rows, then the rank-4 update.
+                double pr[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    pr[m] = pinv_s[buf][4 * lr + m];
+                double X[H], Aop[H];
+#pragma unroll
+                for (int b = 0; b < H; ++b)
                 {
-                    colk[i] = sA[i * LD + k];
-                    rowk[i] = sA[k * LD + i] * pi;
+                    const int col = 16 * (tj0 + b) + lc;
+                    double v = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 4; ++m)
+                        v = fma(pr[m], pn[m * PW + col], v);
+                    X[b] = v;
                 }
-                __syncthreads();
-                for (int e = threadIdx.x; e < N * N; e += 256)
+#pragma unroll
+                for (int a = 0; a < H; ++a)
                 {
-                    const int i = e / N, j = e - (e / N) * N;
-                    double v;
-                    if (i == k)
-                        v = (j == k) ? pi : rowk[j];
-                    else if (j == k)
-                        v = -colk[i] * pi;
-                    else
-                        v = fma(-colk[i], rowk[j], sA[i * LD + j]);
-                    sA[i * LD + j] = v;
+                    const int row = 16 * (ti0 + a) + lc; // A operand: row lc, k = lr
+                    const double v = pn[lr * PW + row];
+                    Aop[a] = row < s0 ? v : -v;
                 }
-                __syncthreads();
-            }
-            // Symmetrise, check and write back packed.
-            for (int i = threadIdx.x; i < N; i += 256)
+#pragma unroll
+                for (int a = 0; a < H; ++a)
+#pragma unroll
+                    for (int b = 0; b < H; ++b)
+                        C[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(Aop[a], X[b], C[a][b], 0, 0, 0);
+                // Strip rows become X (this lane's X[b] is exactly its register-q entry), the
+                // strip columns -(+-)X^T, the pivot block P^-1.
+#pragma unroll
+                for (int a = 0; a < H; ++a)
+                    if (ti0 + a == tK)
+                    {
+#pragma unroll
+                        for (int b = 0; b < H; ++b)
+                        {
+                            const int col = 16 * (tj0 + b) + lc;
+                            C[a][b][q] = (col >= s0 && col < s0 + 4) ? pinv_s[buf][4 * lr + (col - s0)] : X[b];
+                        }
+                    }
+                if (lc >= 4 * q && lc < 4 * q + 4)
+                {
+                    const int c = lc - 4 * q;
+#pragma unroll
+                    for (int b = 0; b < H; ++b)
+                        if (tj0 + b == tK)
+                        {
+                            double pc[4];
+#pragma unroll
+                            for (int m = 0; m < 4; ++m)
+                                pc[m] = pinv_s[buf][4 * c + m];
+#pragma unroll
+                            for (int a = 0; a < H; ++a)
+#pragma unroll
+                                for (int r = 0; r < 4; ++r)
+                                {
+                                    const int row = 16 * (ti0 + a) + 4 * r + lr;
+                                    if (row >= s0 && row < s0 + 4)
+                                        continue;
+                                    double v = 0.0;
+#pragma unroll
+                                    for (int m = 0; m < 4; ++m)
+                                        v = fma(pc[m], pn[m * PW + row], v);
+                                    C[a][b][r] = row < s0 ? v : -v;
+                                }
+                        }
+                }
+            };
+            for (int K4 = 0; K4 < N / 16; ++K4)
             {
-                const int rs = i * N - (i * (i + 1)) / 2;
-                if (!(sA[i * LD + i] > 0.0))
-                    bad = 1;
-                for (int j = i; j < N; ++j)
-                {
-                    const double v = 0.5 * (sA[i * LD + j] + sA[j * LD + i]);
-                    if (!isfinite(v))
-                        bad = 1;
-                    g[rs + j] = v;
-                }
+                step(4 * K4 + 0, std::integral_constant<int, 0>());
+                step(4 * K4 + 1, std::integral_constant<int, 1>());
+                step(4 * K4 + 2, std::integral_constant<int, 2>());
+                step(4 * K4 + 3, std::integral_constant<int, 3>());
             }
+            // Check and write back the upper triangle.
+#pragma unroll
+            for (int a = 0; a < H; ++a)
+#pragma unroll
+                for (int b = 0; b < H; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                    {
+                        const int row = 16 * (ti0 + a) + 4 * r + lr, col = 16 * (tj0 + b) + lc;
+                        const double v = C[a][b][r];
+                        if (!isfinite(v) || (row == col && !(v > 0.0)))
+                            flag = 1;
+                        if (row <= col)
+                            g[packed(row, col)] = v;
+                    }
+            if (flag)
+                bad = 1; // benign race: every writer stores 1
             __syncthreads();
             if (threadIdx.x == 0 && bad)
                 atomicExch(fail, 1);
