@@ -122,6 +122,27 @@ namespace psmas
     void launch_spmv_ap(const SellView<TV> &A, const TX *x, TX *y, const Reductions &red, const SysMap &sm,
                         hipStream_t s);
 
+    /// The update of the search direction folded into the SpMV that uses it: the SpMV computes
+    /// p = z + beta p_old itself (beta from the partials of r . z as launch_update_p would, or
+    /// p = z with init) for every node it reads, multiplies A p and writes p to p_new for its own
+    /// rows (p_old and p_new must differ: other workgroups still read p_old).  Flexible PCG (rzo
+    /// set): beta = r . (z - z_prev) / r_old . z_old, and zprev receives z.  Bit-identical to
+    /// launch_update_p followed by launch_spmv_ap.  Needs the halo form of the matrix.
+    template <typename TX>
+    struct FusedDirection
+    {
+        const TX *z = nullptr;
+        const TX *p_old = nullptr;
+        TX *p_new = nullptr;
+        TX *zprev = nullptr;
+        const double *rzo = nullptr;
+        int init = 0;
+    };
+
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_ap_fused(const SellView<TV> &A, const FusedDirection<TX> &fd, TX *y, const Reductions &red,
+                              const SysMap &sm, hipStream_t s);
+
     /// y = b - A x, partials of y . y -> rr_partials; restrict y -> yc0.  TX: x,b; TY: y.
     /// selected_only: only systems with select == 1.
     template <int BD, typename TV, typename TX, typename TY>

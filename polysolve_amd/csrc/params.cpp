@@ -276,7 +276,7 @@ namespace psmas
             "use_preconditioned_residual_norm", "precision", "aggregation", "coarse_solver",
             "reliable_update_delta", "check_interval", "true_residual_period", "use_graphs",
             "learned_correction", "division_guard", "learned_collect", "lattice_points", "learned_hidden",
-            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll", "spmv_halo", "device_setup", "spmv_copies",
+            "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll", "spmv_halo", "device_setup", "spmv_copies", "spmv_fuse_direction",
             "restrict_piece", "apply_workgroups", "learned_network"};
     } // namespace
 
@@ -417,6 +417,8 @@ namespace psmas
             q.device_setup = as_bool(*v, "device_setup");
         if (const JValue *v = obj->get("spmv_copies"))
             q.spmv_copies = as_bool(*v, "spmv_copies");
+        if (const JValue *v = obj->get("spmv_fuse_direction"))
+            q.spmv_fuse_direction = as_bool(*v, "spmv_fuse_direction");
         if (const JValue *v = obj->get("restrict_piece"))
         {
             q.restrict_piece = as_int(*v, "restrict_piece");
@@ -474,7 +476,7 @@ namespace psmas
           << ", \"learned_random_pairs\": " << p.learned_random_pairs
           << ", \"learned_snapshots\": " << p.learned_snapshots
           << ", \"learned_stall_segments\": " << p.learned_stall_segments
-          << ", \"learned_output_scale\": " << json_number(p.learned_output_scale) << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"spmv_halo\": " << (p.spmv_halo ? "true" : "false") << ", \"device_setup\": " << (p.device_setup ? "true" : "false") << ", \"spmv_copies\": " << (p.spmv_copies ? "true" : "false") << ", \"restrict_piece\": " << p.restrict_piece
+          << ", \"learned_output_scale\": " << json_number(p.learned_output_scale) << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"spmv_halo\": " << (p.spmv_halo ? "true" : "false") << ", \"device_setup\": " << (p.device_setup ? "true" : "false") << ", \"spmv_copies\": " << (p.spmv_copies ? "true" : "false") << ", \"spmv_fuse_direction\": " << (p.spmv_fuse_direction ? "true" : "false") << ", \"restrict_piece\": " << p.restrict_piece
           << ", \"apply_workgroups\": " << p.apply_workgroups
           << ", \"learned_network\": \"" << (p.learned_linear ? "linear" : "mlp") << "\"}";
         return o.str();

@@ -38,6 +38,11 @@
 //   spmv_halo            SpMV gathers x from a per-workgroup copy of the entries its rows read,
 //                        staged in LDS, with 16-bit local columns (default true; used when every
 //                        workgroup's set fits in SPMV_HALO_CAP nodes)
+//   spmv_fuse_direction  the SpMV forms the search direction p = z + beta p itself while staging
+//                        its halo, instead of a separate update kernel (bit-identical iterates;
+//                        used with the halo form and without copies).  Default false: on MI355X
+//                        the longer halo stage (two gathers and beta's reduction ahead of the
+//                        row loop) costs more than the update kernel it removes
 //   restrict_piece       longest level-1 descendant list one wave sums when restricting coarse
 //                        levels 2.. (longer lists are cut into pieces added where they are read);
 //                        9..512, default 512; takes effect at the next factorize
@@ -102,6 +107,7 @@ namespace psmas
         bool spmv_halo = true;
         bool device_setup = true;
         bool spmv_copies = true;
+        bool spmv_fuse_direction = false;
         int restrict_piece = 512;
         int apply_workgroups = 0;
         bool learned_linear = false;

@@ -91,8 +91,14 @@ namespace psmas
 
     namespace
     {
-        /// Graph cache keys: precision path, iteration pattern, iterations per graph.
-        int graph_key(int path, int pattern, int count) { return (path << 24) | (pattern << 16) | count; }
+        /// Graph cache keys: precision path, iteration pattern, iterations per graph, and flags
+        /// (bit 0: fused direction update, bit 1: direction buffer at the segment's start, bit 2:
+        /// the segment's first SpMV starts the direction from z).
+        long long graph_key(int path, int pattern, int count, int flags = 0)
+        {
+            return (static_cast<long long>(path) << 56) | (static_cast<long long>(flags) << 48) |
+                   (static_cast<long long>(pattern) << 24) | static_cast<long long>(count);
+        }
 
         // Scratch of solve_device_noalloc (1 MiB per GPU, held until the process ends): the
         // per-system scalars in the first SCRATCH_SC_PAD doubles, then the five reduction
@@ -131,7 +137,7 @@ namespace psmas
     // ------------------------------------------------------------------------------ graphs
     struct Solver::Graphs
     {
-        std::map<int, hipGraphExec_t> execs;
+        std::map<long long, hipGraphExec_t> execs;
         ~Graphs() { clear(); }
         void clear()
         {
@@ -178,6 +184,7 @@ namespace psmas
             before.true_residual_period != params_.true_residual_period || before.precision != params_.precision ||
             before.coarse_solver != params_.coarse_solver || before.spmv_unroll != params_.spmv_unroll ||
             before.spmv_halo != params_.spmv_halo || before.spmv_copies != params_.spmv_copies ||
+            before.spmv_fuse_direction != params_.spmv_fuse_direction ||
             before.apply_workgroups != params_.apply_workgroups)
             graphs_->clear();
     }
@@ -1469,6 +1476,7 @@ namespace psmas
         d_b_.allocate(nvec, "device_buffers b");
         d_x_.allocate(nvec, "device_buffers x");
         d_p_.allocate(nvec, "device_buffers p");
+        d_pb_.allocate(nvec, "device_buffers p (second)");
         d_Ap_.allocate(nvec, "device_buffers Ap");
         d_mr_.allocate(nml, "MAS coarse_vectors r");
         d_mz_.allocate(nml, "MAS coarse_vectors z");
@@ -1479,6 +1487,7 @@ namespace psmas
         d_mr32_.reset();
         d_mz32_.reset();
         d_p32_.reset();
+        d_p32b_.reset();
         d_Ap32_.reset();
         const int K = static_cast<int>(sys_.size());
         d_sc_.allocate(static_cast<size_t>(S_COUNT) * K, "MAS device scalars");
@@ -1512,6 +1521,7 @@ namespace psmas
         {
             d_x32_.allocate(nvec, "device_buffers x32");
             d_p32_.allocate(nvec, "device_buffers p32");
+            d_p32b_.allocate(nvec, "device_buffers p32 (second)");
             d_Ap32_.allocate(nvec, "device_buffers Ap32");
             d_mr32_.allocate(nml, "MAS coarse_vectors r32");
             d_mz32_.allocate(nml, "MAS coarse_vectors z32");
@@ -1714,7 +1724,7 @@ namespace psmas
         solve_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 
-    void Solver::run_block(int key, const std::function<void()> &enqueue)
+    void Solver::run_block(long long key, const std::function<void()> &enqueue)
     {
         // Captured graphs hold the allocated partial buffers; the scratch solve launches directly.
         if (!params_.use_graphs || timing_on_ || use_scratch_)
@@ -1780,22 +1790,44 @@ namespace psmas
                                        stream_);
     }
 
+    bool Solver::fuse_direction() const
+    {
+        return params_.spmv_fuse_direction && params_.spmv_halo && halo_max_ > 0 && !(copies_ > 1 && params_.spmv_copies);
+    }
+
     template <int BD>
-    void Solver::fp64_iteration(int k)
+    void Solver::fp64_iteration(int k, bool init_direction)
     {
         const SellView<double> A = sell(d_vals64_.data());
-        const double bytes = spmv_bytes(8, 8, 2);
-        tic("spmv_fp64", bytes);
-        if (copies_ > 1 && params_.spmv_copies)
-            launch_spmv_ap_copies<BD, double, double>(A, copies_, sys_[0].npad, d_p_.data(), d_Ap_.data(), reductions(),
-                                                      sysmap(), stream_);
+        const bool fuse = fuse_direction();
+        if (fuse)
+        {
+            // p = z + beta p (or z) formed by the SpMV; the new p lands in the other buffer.
+            FusedDirection<double> fd;
+            fd.z = d_mz_.data();
+            fd.p_old = p64(p_cur_);
+            fd.p_new = p64(p_cur_ ^ 1);
+            fd.init = init_direction ? 1 : 0;
+            tic("spmv_dir_fp64", spmv_bytes(8, 8, 4));
+            launch_spmv_ap_fused<BD, double, double>(A, fd, d_Ap_.data(), reductions(), sysmap(), stream_);
+            toc();
+            p_cur_ ^= 1;
+        }
         else
-            launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), reductions(), sysmap(), stream_);
-        toc();
+        {
+            tic("spmv_fp64", spmv_bytes(8, 8, 2));
+            if (copies_ > 1 && params_.spmv_copies)
+                launch_spmv_ap_copies<BD, double, double>(A, copies_, sys_[0].npad, d_p_.data(), d_Ap_.data(),
+                                                          reductions(), sysmap(), stream_);
+            else
+                launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), reductions(), sysmap(), stream_);
+            toc();
+        }
+        double *p = fuse ? p64(p_cur_) : d_p_.data();
         if (k % params_.true_residual_period == 0)
         {
-            launch_axpy_alpha<BD, double>(nblocks_, d_p_.data(), d_x_.data(), reductions(), sysmap(), stream_);
-            tic("spmv_res_fp64", bytes + n_total_ * 8.0);
+            launch_axpy_alpha<BD, double>(nblocks_, p, d_x_.data(), reductions(), sysmap(), stream_);
+            tic("spmv_res_fp64", spmv_bytes(8, 8, 2) + n_total_ * 8.0);
             launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
                                                         d_mr_.data() + static_cast<size_t>(npad_) * BD,
                                                         reductions().rr, sysmap(), false, stream_);
@@ -1803,12 +1835,13 @@ namespace psmas
         }
         else
         {
-            launch_update_xr<BD, double>(nblocks_, d_p_.data(), d_Ap_.data(), d_x_.data(), d_mr_.data(),
-                                         fine_restrict(), d_mr_.data() + static_cast<size_t>(npad_) * BD,
-                                         reductions(), sysmap(), stream_);
+            launch_update_xr<BD, double>(nblocks_, p, d_Ap_.data(), d_x_.data(), d_mr_.data(), fine_restrict(),
+                                         d_mr_.data() + static_cast<size_t>(npad_) * BD, reductions(), sysmap(),
+                                         stream_);
         }
         fp64_precondition<BD>(false);
-        launch_update_p<BD, double>(nblocks_, d_mz_.data(), d_p_.data(), reductions(), sysmap(), stream_);
+        if (!fuse)
+            launch_update_p<BD, double>(nblocks_, d_mz_.data(), d_p_.data(), reductions(), sysmap(), stream_);
     }
 
     void Solver::solve_fp64()
@@ -1824,7 +1857,10 @@ namespace psmas
                                                         d_mr_.data() + static_cast<size_t>(npad_) * BD,
                                                         reductions().rr, sysmap(), false, stream_);
             fp64_precondition<BD>(true);
-            launch_copy_convert<double, double>(nvec, d_mz_.data(), d_p_.data(), stream_);
+            const bool fuse = fuse_direction();
+            p_cur_ = 0;
+            if (!fuse) // otherwise the first SpMV starts the direction from z
+                launch_copy_convert<double, double>(nvec, d_mz_.data(), d_p_.data(), stream_);
             finalize_scalars();
             read_scalars();
             std::vector<double> rz0(K), rr0(K), bb(K);
@@ -1850,11 +1886,16 @@ namespace psmas
                 const int last = std::min(next_check, params_.max_iter);
                 const int count = last - k + 1;
                 const int k0 = k;
-                run_block(graph_key(0, k % trp, count), [&]() {
+                // The lambda runs only when a graph is captured: the direction buffer parity is
+                // set from the segment's start either way.
+                const int par0 = p_cur_;
+                run_block(graph_key(0, k % trp, count, (fuse ? 1 : 0) | (par0 << 1) | (k0 == 1 ? 4 : 0)), [&]() {
+                    p_cur_ = par0;
                     for (int i = k0; i <= last; ++i)
-                        fp64_iteration<BD>(i);
+                        fp64_iteration<BD>(i, i == 1);
                     finalize_scalars();
                 });
+                p_cur_ = fuse ? par0 ^ (count & 1) : 0;
                 for (int s = 0; s < K; ++s)
                     if (h_active_[s])
                         sys_[s].iterations = last;
@@ -1898,23 +1939,46 @@ namespace psmas
 
     // ---- mixed path: FP32 inner PCG with FP64 reliable updates ------------------------------
     template <int BD>
-    void Solver::mixed_part_a(bool learned)
+    void Solver::mixed_part_a(bool learned, bool init_direction)
     {
         const SellView<float> A = sell(d_vals32_.data());
-        tic("spmv_fp32", spmv_bytes(4, 4, 2));
-        if (copies_ > 1 && params_.spmv_copies)
-            launch_spmv_ap_copies<BD, float, float>(A, copies_, sys_[0].npad, d_p32_.data(), d_Ap32_.data(), reductions(),
-                                                    sysmap(), stream_);
+        const bool fuse = fuse_direction();
+        if (fuse)
+        {
+            // p = z + beta p (flexible beta with the learned correction; p = z at the start) formed
+            // by the SpMV; the new p lands in the other buffer.
+            FusedDirection<float> fd;
+            fd.z = d_mz32_.data();
+            fd.p_old = p32(p_cur_);
+            fd.p_new = p32(p_cur_ ^ 1);
+            fd.init = init_direction ? 1 : 0;
+            if (learned)
+            {
+                fd.zprev = d_zprev32_.data();
+                fd.rzo = init_direction ? nullptr : rzo_ptr();
+            }
+            tic("spmv_dir_fp32", spmv_bytes(4, 4, 4) + (learned ? n_total_ * 4.0 : 0.0));
+            launch_spmv_ap_fused<BD, float, float>(A, fd, d_Ap32_.data(), reductions(), sysmap(), stream_);
+            toc();
+            p_cur_ ^= 1;
+        }
         else
-            launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), reductions(), sysmap(), stream_);
-        toc();
+        {
+            tic("spmv_fp32", spmv_bytes(4, 4, 2));
+            if (copies_ > 1 && params_.spmv_copies)
+                launch_spmv_ap_copies<BD, float, float>(A, copies_, sys_[0].npad, d_p32_.data(), d_Ap32_.data(),
+                                                        reductions(), sysmap(), stream_);
+            else
+                launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), reductions(), sysmap(), stream_);
+            toc();
+        }
         // With the learned correction on the block path, the update also leaves the lattice block
         // partials of the new r.
         const LatticeBlocks lb = learned && lcc_ ? lcc_->block_view() : LatticeBlocks{};
         tic(lb.cell ? "update_xr_lattice" : "update_xr", static_cast<double>(n_total_) * 4.0 * 6.0);
-        launch_update_xr<BD, float>(nblocks_, d_p32_.data(), d_Ap32_.data(), d_x32_.data(), d_mr32_.data(),
-                                    fine_restrict(), d_mr32_.data() + static_cast<size_t>(npad_) * BD, reductions(),
-                                    sysmap(), stream_, &lb);
+        launch_update_xr<BD, float>(nblocks_, fuse ? p32(p_cur_) : d_p32_.data(), d_Ap32_.data(), d_x32_.data(),
+                                    d_mr32_.data(), fine_restrict(), d_mr32_.data() + static_cast<size_t>(npad_) * BD,
+                                    reductions(), sysmap(), stream_, &lb);
         toc();
     }
 
@@ -1954,8 +2018,12 @@ namespace psmas
         }
         tic("mas_apply_fp16", n1 * per_mat);
         const bool shared_fine = copies_ > 1 && params_.spmv_copies; // one matrix read for every copy
+        // A single system is active for the whole apply: no per-matrix system lookups (two dependent
+        // loads per matrix on each workgroup's critical path).
+        const int *bsys = sys_.size() > 1 ? d_bank_sys_.data() : nullptr;
+        const int *bact = sys_.size() > 1 ? d_active_.data() : nullptr;
         launch_mas_apply<BD, __half, float>(0, n1, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
-                                            d_mz32_.data(), d_bank_sys_.data(), d_active_.data(), stream_,
+                                            d_mz32_.data(), bsys, bact, stream_,
                                             restrict_side(d_mr32_.data() + static_cast<size_t>(lev_off_[1]) * BD,
                                                           d_mr32_.data(), block, lat_blocks),
                                             shared_fine ? copies_ : 1, shared_fine ? sys_[0].nparts : 0);
@@ -1968,8 +2036,7 @@ namespace psmas
         {
             tic("mas_apply_fp16_coarse", (total_mats_ - n1) * per_mat + static_cast<double>(dense_nd_) * dense_nd_ * 8.0);
             launch_mas_apply<BD, __half, float>(n1, total_mats_ - n1, d_mats16_.data(), d_dscale_.data(),
-                                                d_mr32_.data(), d_mz32_.data(), d_bank_sys_.data(), d_active_.data(),
-                                                stream_, cs);
+                                                d_mr32_.data(), d_mz32_.data(), bsys, bact, stream_, cs);
             toc();
         }
         if (learned)
@@ -1985,6 +2052,8 @@ namespace psmas
         launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
                                       d_mz32_.data(), reductions(), init, sysmap(), lv, flex, stream_);
         toc();
+        if (fuse_direction())
+            return; // the next SpMV forms p (and z_prev)
         if (init)
         {
             launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_p32_.data(), stream_);
@@ -2065,6 +2134,7 @@ namespace psmas
             launch_fill<double>(nvec, d_x_.data(), 0.0, stream_);
             launch_fill<float>(nvec, d_x32_.data(), 0.0f, stream_);
             mixed_reliable_residual<BD>(learned); // every system selected: r = b
+            p_cur_ = 0;
             mixed_part_b<BD>(true, learned);
             finalize_scalars();
             read_scalars();
@@ -2107,17 +2177,23 @@ namespace psmas
                 }
                 const int count = std::min(ci, params_.max_iter - k);
                 const bool lead_b = !first;
-                run_block(graph_key(learned ? 2 : 1, first ? 1 : 2, count), [&]() {
+                // The lambda runs only when a graph is captured: the direction buffer parity is
+                // set from the segment's start either way.
+                const bool fuse = fuse_direction();
+                const int par0 = p_cur_;
+                run_block(graph_key(learned ? 2 : 1, first ? 1 : 2, count, (fuse ? 1 : 0) | (par0 << 1)), [&]() {
+                    p_cur_ = par0;
                     if (lead_b)
                         mixed_part_b<BD>(false, learned);
                     for (int i = 0; i < count; ++i)
                     {
-                        mixed_part_a<BD>(learned);
+                        mixed_part_a<BD>(learned, first && i == 0);
                         if (i + 1 < count)
                             mixed_part_b<BD>(false, learned);
                     }
                     finalize_scalars();
                 });
+                p_cur_ = fuse ? par0 ^ (count & 1) : 0;
                 first = false;
                 k += count;
                 read_scalars();

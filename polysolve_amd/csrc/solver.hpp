@@ -258,6 +258,14 @@ namespace psmas
         // vectors
         DeviceBuffer<double> d_b_, d_x_, d_mr_, d_mz_, d_p_, d_Ap_, d_io_;
         DeviceBuffer<float> d_x32_, d_mr32_, d_mz32_, d_p32_, d_Ap32_;
+        // Second search-direction buffers: with the direction update folded into the SpMV, p
+        // alternates between the two (other workgroups still read the previous p).
+        DeviceBuffer<double> d_pb_;
+        DeviceBuffer<float> d_p32b_;
+        int p_cur_ = 0; ///< which buffer holds the current p (0: d_p_ / d_p32_)
+        bool fuse_direction() const;
+        double *p64(int which) { return which ? d_pb_.data() : d_p_.data(); }
+        float *p32(int which) { return which ? d_p32b_.data() : d_p32_.data(); }
         DeviceBuffer<double> d_sc_;
         DeviceBuffer<double> d_partials_;
         int partial_cap_ = 0;
@@ -378,17 +386,17 @@ namespace psmas
 
         // per-iteration pieces
         template <int BD>
-        void fp64_iteration(int k);
+        void fp64_iteration(int k, bool init_direction);
         template <int BD>
         void fp64_precondition(bool init);
         template <int BD>
-        void mixed_part_a(bool learned);
+        void mixed_part_a(bool learned, bool init_direction);
         template <int BD>
         void mixed_part_b(bool init, bool learned = false);
         template <int BD>
         void mixed_reliable_residual(bool learned);
 
-        void run_block(int key, const std::function<void()> &enqueue);
+        void run_block(long long key, const std::function<void()> &enqueue);
 
         // timing helpers
         void tic(const char *name, double bytes);

@@ -174,6 +174,106 @@ namespace psmas
             store_partial<SPMV_BLOCK>(dot, red.pap, blk);
         }
 
+        /// beta of the search-direction update for system sys (the same arithmetic as k_update_p).
+        template <typename T>
+        __device__ __forceinline__ T direction_beta(const Reductions &red, int sys, const SysMap &sm,
+                                                    const double *rzo)
+        {
+            double rz, rz_old;
+            if (rzo != nullptr) // flexible (Polak-Ribiere): r_new . (z_new - z_old)
+            {
+                const double *const parts[3] = {red.rz, red.rz_old, rzo};
+                double t[3];
+                system_totals<VEC_BLOCK, 3>(parts, sys, sm, t);
+                rz = t[0] - t[2];
+                rz_old = t[1];
+            }
+            else
+            {
+                const double *const parts[2] = {red.rz, red.rz_old};
+                double t[2];
+                system_totals<VEC_BLOCK, 2>(parts, sys, sm, t);
+                rz = t[0];
+                rz_old = t[1];
+            }
+            return static_cast<T>(safe_div(rz, rz_old, red.sc[static_cast<size_t>(sys) * S_COUNT + S_GUARD]));
+        }
+
+        /// k_spmv_ap with the direction update folded in (FusedDirection): the halo receives
+        /// z + beta p_old, own rows write p_new (and z_prev).
+        template <int BD, int U, typename TV, typename TX>
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_ap_fused(SellView<TV> A, FusedDirection<TX> fd,
+                                                                      TX *__restrict__ y, Reductions red, SysMap sm)
+        {
+            using TA = typename Acc<TV>::type;
+            const int blk = xcd_block();
+            const int sys = sm.block_sys[blk];
+            if (sm.active[sys] == 0)
+                return;
+            TX *xs = reinterpret_cast<TX *>(spmv_halo_lds);
+            const int hb = A.halo_ptr[blk], hn = A.halo_ptr[blk + 1] - hb;
+            // The first PRE rounds of halo values are loaded before beta's reduction so that both
+            // latencies overlap.
+            constexpr int PRE = 3;
+            TX zr[PRE][BD], pr[PRE][BD];
+#pragma unroll
+            for (int r = 0; r < PRE; ++r)
+            {
+                const int i = threadIdx.x + r * SPMV_BLOCK;
+                if (i < hn)
+                {
+                    const size_t g = static_cast<size_t>(A.halo[hb + i]) * BD;
+#pragma unroll
+                    for (int j = 0; j < BD; ++j)
+                    {
+                        zr[r][j] = fd.z[g + j];
+                        pr[r][j] = fd.init ? TX(0) : fd.p_old[g + j];
+                    }
+                }
+            }
+            const TX beta = fd.init ? TX(0) : direction_beta<TX>(red, sys, sm, fd.rzo);
+            auto dir = [&](size_t o) { return fd.init ? fd.z[o] : fma(beta, fd.p_old[o], fd.z[o]); };
+#pragma unroll
+            for (int r = 0; r < PRE; ++r)
+            {
+                const int i = threadIdx.x + r * SPMV_BLOCK;
+                if (i < hn)
+#pragma unroll
+                    for (int j = 0; j < BD; ++j)
+                        xs[i * BD + j] = fd.init ? zr[r][j] : fma(beta, pr[r][j], zr[r][j]);
+            }
+            for (int i = threadIdx.x + PRE * SPMV_BLOCK; i < hn; i += SPMV_BLOCK)
+            {
+                const size_t g = static_cast<size_t>(A.halo[hb + i]) * BD;
+#pragma unroll
+                for (int j = 0; j < BD; ++j)
+                    xs[i * BD + j] = dir(g + j);
+            }
+            __syncthreads();
+            const int lane = threadIdx.x % WAVE;
+            const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            TA acc[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                acc[i] = TA(0);
+            sell_row<BD, U, true>(A, slice, lane, static_cast<const TX *>(nullptr), xs, acc);
+            const size_t row = static_cast<size_t>(slice) * WAVE + lane;
+            double dot = 0.0;
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+            {
+                const size_t o = row * BD + i;
+                const TX pn = dir(o);
+                fd.p_new[o] = pn;
+                if (fd.zprev != nullptr)
+                    fd.zprev[o] = fd.z[o];
+                const TX yi = static_cast<TX>(acc[i]);
+                y[o] = yi;
+                dot += static_cast<double>(pn) * static_cast<double>(yi);
+            }
+            store_partial<SPMV_BLOCK>(dot, red.pap, blk);
+        }
+
         /// y_c = A x_c for the `copies` systems of a batch that repeats one matrix: system c's node
         /// block b is block c * nblocks1 + b and its slots are copy 0's with c * npad1 added to the
         /// columns.  Workgroup b reads each block of copy 0's slots once and applies it to every
@@ -675,6 +775,24 @@ namespace psmas
         PSMAS_LAUNCH_CHECK();
     }
 
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_ap_fused(const SellView<TV> &A, const FusedDirection<TX> &fd, TX *y, const Reductions &red,
+                              const SysMap &sm, hipStream_t s)
+    {
+        require_device_ptrs("spmv_ap_fused", {A.slice_ptr, A.lcols, A.halo_ptr, A.halo, A.vals, fd.z, fd.p_new, y});
+        if (!fd.init)
+            require_device_ptrs("spmv_ap_fused (previous direction)", {fd.p_old});
+        if (A.halo_max <= 0)
+            throw std::logic_error("[MAS] internal error: the fused direction update needs the SpMV halo lists.");
+        if (fd.p_old == fd.p_new)
+            throw std::logic_error("[MAS] internal error: the fused direction update needs two direction buffers.");
+        const int grid = A.nslices / (SPMV_BLOCK / WAVE);
+        const size_t lds = static_cast<size_t>(A.halo_max) * BD * sizeof(TX);
+        PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_ap_fused<BD, U, TV, TX>), dim3(grid),
+                                                           dim3(SPMV_BLOCK), lds, s, A, fd, y, red, sm));
+        PSMAS_LAUNCH_CHECK();
+    }
+
     template <int BD, typename TV, typename TX, typename TY>
     void launch_spmv_res(const SellView<TV> &A, const TX *x, const TX *b, TY *y, FineRestrict fr, TY *yc0,
                          double *rr_partials, const SysMap &sm, bool selected_only, hipStream_t s)
@@ -926,6 +1044,11 @@ namespace psmas
                                                      const Reductions &, const SysMap &, hipStream_t);            \
     template void launch_spmv_ap<BD, float, float>(const SellView<float> &, const float *, float *,               \
                                                    const Reductions &, const SysMap &, hipStream_t);              \
+    template void launch_spmv_ap_fused<BD, double, double>(const SellView<double> &,                             \
+                                                           const FusedDirection<double> &, double *,              \
+                                                           const Reductions &, const SysMap &, hipStream_t);      \
+    template void launch_spmv_ap_fused<BD, float, float>(const SellView<float> &, const FusedDirection<float> &,  \
+                                                         float *, const Reductions &, const SysMap &, hipStream_t); \
     template void launch_spmv_res<BD, double, double, double>(const SellView<double> &, const double *,           \
                                                               const double *, double *, FineRestrict, double *,   \
                                                               double *, const SysMap &, bool, hipStream_t);       \

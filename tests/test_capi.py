@@ -58,6 +58,13 @@ def test_inner_object_extension():
     assert p["max_iter"] == 12 and p["precision"] == "mixed"
 
 
+def test_fused_direction_switch():
+    assert parse_parameters({})["spmv_fuse_direction"] is False
+    assert parse_parameters({"MAS": {"spmv_fuse_direction": True}})["spmv_fuse_direction"] is True
+    with pytest.raises(MASError, match="spmv_fuse_direction"):
+        parse_parameters({"MAS": {"spmv_fuse_direction": 0}})
+
+
 @pytest.mark.parametrize("bad", [
     '{"MAS": {"block_dim": 4}}',
     '{"MAS": {"precision": "bf8"}}',
