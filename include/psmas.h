@@ -155,16 +155,17 @@ int psmas_factorize_csc_device(psmas_handle h, int n, long long nnz, const int *
 int psmas_factorize_values_device(psmas_handle h, long long nnz, const double *d_vals);
 
 /* Several right-hand sides of one matrix (no counterpart in the reference; for Newton or IPC
- * callers that solve one Hessian for several vectors): factorise nrhs (1..8) copies of A as a
- * batch whose SpMV and fine-level preconditioner apply read the matrix once for all of them
- * (parameter spmv_copies), then solve all right-hand sides in one
- * launch sequence, each with its own CG scalars, convergence test and status
- * (psmas_batch_get_info(h, c, ...) for copy c).  B and X hold nrhs vectors of length n one after
- * the other (column-major n x nrhs), host arrays or device pointers (_device). */
+ * callers that solve one Hessian for several vectors): factorise A once (one SELL matrix, one set
+ * of bank inverses, one coarsest inverse) for nrhs (1..8) right-hand sides whose vectors, coarse
+ * numbering and CG scalars are per copy; the SpMV (parameter spmv_copies) and every bank apply
+ * read each matrix once for all of them.  Then solve all right-hand sides in one launch sequence,
+ * each with its own convergence test and status (psmas_batch_get_info(h, c, ...) for copy c);
+ * per copy the arithmetic is that of the single system.  B and X hold nrhs vectors of length n
+ * one after the other (column-major n x nrhs), host arrays or device pointers (_device). */
 int psmas_factorize_multi_csc(psmas_handle h, int nrhs, int n, long long nnz, const int *colptr, const int *rowidx,
                               const double *vals);
 /* The same with A's CSC arrays in device memory (HIP pointers on the solver's device, validated
- * there); the matrix is built once on the GPU and copied for every right-hand side. */
+ * there); the matrix is built on the GPU. */
 int psmas_factorize_multi_csc_device(psmas_handle h, int nrhs, int n, long long nnz, const int *d_colptr,
                                      const int *d_rowidx, const double *d_vals);
 int psmas_solve_multi(psmas_handle h, int n, int nrhs, const double *B, double *X);

@@ -261,6 +261,10 @@ namespace psmas
         int dense_seg = 0;
         const int *dense_index = nullptr;
         int dense_count = 0;
+        // Several right-hand sides of one factorisation: copy c's coarsest segment starts at
+        // dense_seg + c dense_stride; one pass over each inverse row serves every copy.
+        int dense_copies = 1;
+        int dense_stride = 0;
         // Layer 1 and the skip layer of the learned correction's network for a single system: every
         // side workgroup stages x = u sqrt(m) / |u| (zero-padded to lin_md) in LDS after the dense
         // inputs; waves take rows (row < lin_h: h1 = relu(W1 x + b1), else skip = S x); workgroup 0
@@ -282,12 +286,28 @@ namespace psmas
         double *lat_u = nullptr;
     };
 
+    /// Right-hand sides that share one factorisation (factorize_copies): the bank matrices are
+    /// stored once, in the single system's order (level l's cnt[l] matrices from soff[l]); the
+    /// multi-level vectors hold every copy level by level, copy c's banks of level l being
+    /// moff[l] + c cnt[l] .. + cnt[l] - 1.  Levels: the fine banks, then each coarse level.
+    /// copies == 1: one system, matrices and banks coincide.
+    struct CopyLayout
+    {
+        static constexpr int MAXL = 5; ///< fine + 4 coarse levels
+        int copies = 1;
+        int levels = 0;
+        int moff[MAXL + 1] = {}; ///< first bank of each level (all copies), then the total
+        int cnt[MAXL] = {};      ///< banks of each level per copy
+        int soff[MAXL + 1] = {}; ///< first stored matrix of each level, then the total
+    };
+
     template <int BD, typename TM, typename T>
-    /// copies > 1 (FP16 only): the batch repeats one matrix and its first nfine1 banks are copy 0's
-    /// fine banks, applied once for every copy.
+    /// cl.copies > 1: the FP16 apply takes [mat_begin, mat_begin + count) as stored matrices and
+    /// applies each to every copy's bank (one matrix read for all right-hand sides); the FP64
+    /// apply takes them as banks and reads each bank's stored matrix.
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
                           const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side = ApplySide{},
-                          int copies = 1, int nfine1 = 0);
+                          const CopyLayout &cl = CopyLayout{});
 
     /// Elements of the vector type the FP16 apply kernel's side workgroups can stage in LDS.
     int apply_h16_stage_floats(int bd);

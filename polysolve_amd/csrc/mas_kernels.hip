@@ -64,8 +64,33 @@ namespace psmas
         {
             if (sd.dense_index == nullptr)
                 return false;
-            const int rel = node - sd.dense_seg;
+            int rel = node - sd.dense_seg;
+            if (sd.dense_copies > 1 && rel >= 0 && rel < sd.dense_copies * sd.dense_stride)
+                rel %= sd.dense_stride; // the copies' segments repeat copy 0's
             return rel >= 0 && rel < sd.dense_count && sd.dense_index[rel] >= 0;
+        }
+
+        /// Stored matrix of bank m (CopyLayout).
+        __device__ __forceinline__ int stored_of_bank(const CopyLayout &cl, int m)
+        {
+            int l = 0;
+#pragma unroll
+            for (int i = 1; i < CopyLayout::MAXL; ++i)
+                if (i < cl.levels && m >= cl.moff[i])
+                    l = i;
+            return cl.soff[l] + (m - cl.moff[l]) % cl.cnt[l];
+        }
+
+        /// Copy 0's bank of stored matrix s and the bank stride between copies (CopyLayout).
+        __device__ __forceinline__ void bank_of_stored(const CopyLayout &cl, int s, int &bank, int &stride)
+        {
+            int l = 0;
+#pragma unroll
+            for (int i = 1; i < CopyLayout::MAXL; ++i)
+                if (i < cl.levels && s >= cl.soff[i])
+                    l = i;
+            bank = cl.moff[l] + (s - cl.soff[l]);
+            stride = cl.cnt[l];
         }
 
         /// Side work of workgroup `wg` < sd.wgs: restriction tasks (thread per small node, wave per
@@ -174,7 +199,7 @@ namespace psmas
                     }
                 }
             }
-            if (sd.dense_nd > 0)
+            if (sd.dense_nd > 0 && sd.dense_copies <= 1)
             {
                 const int nd = sd.dense_nd;
                 const bool staged = nd <= stage_cap;
@@ -199,6 +224,52 @@ namespace psmas
                     {
                         const int node = sd.dense_seg + sd.dense_ids[row / BD];
                         mz[static_cast<size_t>(node) * BD + row % BD] = static_cast<T>(acc);
+                    }
+                }
+            }
+            else if (sd.dense_nd > 0)
+            {
+                // Every copy's dense rows from one read of each inverse row (copy c's inputs at
+                // stage[c nd ..] when they all fit); per copy the same arithmetic as one system.
+                constexpr int MC = 8;
+                const int nd = sd.dense_nd, nc = sd.dense_copies;
+                const bool staged = nd * nc <= stage_cap;
+                auto input = [&](int c, int col) {
+                    return coarse_input<BD, T>(sd, mr, sd.dense_seg + c * sd.dense_stride + sd.dense_ids[col / BD],
+                                               col % BD);
+                };
+                if (staged)
+                {
+                    for (int i = threadIdx.x; i < nd * nc; i += blockDim.x)
+                        stage[i] = input(i / nd, i % nd);
+                    __syncthreads();
+                }
+                for (int row = t >> 6; row < nd; row += nt >> 6)
+                {
+                    const double *a = sd.dense_inv + static_cast<size_t>(row) * nd;
+                    double acc[MC];
+#pragma unroll
+                    for (int c = 0; c < MC; ++c)
+                        acc[c] = 0.0;
+                    for (int col = lane; col < nd; col += 64)
+                    {
+                        const double av = a[col];
+#pragma unroll
+                        for (int c = 0; c < MC; ++c)
+                            if (c < nc)
+                                acc[c] += av * static_cast<double>(staged ? stage[c * nd + col] : input(c, col));
+                    }
+#pragma unroll
+                    for (int c = 0; c < MC; ++c)
+                    {
+                        if (c >= nc)
+                            break;
+                        const double v = wave_sum(acc[c]);
+                        if (lane == 0)
+                        {
+                            const int node = sd.dense_seg + c * sd.dense_stride + sd.dense_ids[row / BD];
+                            mz[static_cast<size_t>(node) * BD + row % BD] = static_cast<T>(v);
+                        }
                     }
                 }
             }
@@ -325,11 +396,11 @@ namespace psmas
         /// in flight per workgroup: the loads of the one after next are issued before the current
         /// matrix is computed.
         ///
-        /// KC > 1 (a batch of `copies` <= KC systems that repeat one matrix, nfine1 fine banks each):
-        /// fine bank m < nfine1 stands for banks m + c nfine1 of every copy c; its tile is unpacked
-        /// once and one pass over each LDS row multiplies it with every copy's input (k
-        /// accumulators).  Those other copies' fine banks are not visited; coarse banks, which may
-        /// span copies, go one by one as before.
+        /// KC > 1 (cl.copies <= KC right-hand sides of one factorisation, CopyLayout): the items
+        /// [mat_begin, mat_begin + count) are stored matrices; each tile is unpacked once and one
+        /// pass over each LDS row multiplies it with every copy's input (KC accumulators), copy
+        /// c's bank being copy 0's plus c times its level's bank count.  Per copy the arithmetic
+        /// is that of a single system.
         template <int BD, int KC>
         __global__ __launch_bounds__(apply_block<BD>()) void k_mas_apply_h16(int mat_begin, int count,
                                                                               const __half *__restrict__ mats,
@@ -338,7 +409,7 @@ namespace psmas
                                                                               float *__restrict__ mz,
                                                                               const int *__restrict__ bank_sys,
                                                                               const int *__restrict__ active,
-                                                                              ApplySide sd, int copies, int nfine1)
+                                                                              ApplySide sd, CopyLayout cl)
         {
             constexpr int N = 32 * BD;
             constexpr int T = N / 8;
@@ -356,15 +427,10 @@ namespace psmas
             __shared__ __attribute__((aligned(16))) float sx[KC][N];
             __shared__ float part[KC][N];
 
-            // Copies of fine bank m: copies when m is copy 0's fine bank (KC > 1), else 1.
-            auto ncopies = [&](int m) { return (KC > 1 && m < nfine1) ? copies : 1; };
+            const int nc = KC > 1 ? cl.copies : 1;
             auto skip = [&](int m) {
-                if (KC > 1 && m >= nfine1 && m < copies * nfine1)
-                    return true; // done together with copy 0's bank
-                if (KC > 1 && m < nfine1)
-                    return false; // the copies converge at different times
-                if (bank_sys == nullptr)
-                    return false;
+                if (KC > 1 || bank_sys == nullptr)
+                    return false; // (copies converge at different times: every copy is applied)
                 const int sy = bank_sys[m]; // -1: a coarse matrix (its nodes may span systems)
                 return sy >= 0 && active[sy] == 0;
             };
@@ -431,6 +497,7 @@ namespace psmas
                 float x[KC] = {};
                 float d = 0.0f;
                 bool store = false;
+                int bank = 0, stride = 0; ///< copy 0's bank of the item, bank stride between copies
             };
             auto load = [&](u32x4(&reg)[PER], Row &rw, int mm) {
                 const u32x4 *src = reinterpret_cast<const u32x4 *>(mats + static_cast<size_t>(mm) * LEN);
@@ -440,12 +507,17 @@ namespace psmas
                 if (tid < N)
                 {
                     rw.d = dscale[static_cast<size_t>(mm) * N + tid];
-                    const int nc = ncopies(mm);
+                    if constexpr (KC > 1)
+                        bank_of_stored(cl, mm, rw.bank, rw.stride);
+                    else
+                        rw.bank = mm;
 #pragma unroll
                     for (int c = 0; c < KC; ++c)
                         if (c < nc)
-                            rw.x[c] = coarse_input<BD, float>(sd, mr, (mm + c * nfine1) * 32 + tid / BD, tid % BD) * rw.d;
-                    rw.store = !dense_owned(sd, mm * 32 + tid / BD);
+                            rw.x[c] = coarse_input<BD, float>(sd, mr, (rw.bank + c * rw.stride) * 32 + tid / BD,
+                                                              tid % BD) *
+                                      rw.d;
+                    rw.store = !dense_owned(sd, rw.bank * 32 + tid / BD);
                 }
             };
             auto unpack = [&](const u32x4(&reg)[PER], const Row &rw) {
@@ -496,9 +568,8 @@ namespace psmas
                 __syncthreads();
             };
             const int row = tid % N, hf = tid / N;
-            auto compute = [&](int mm, const Row &rw) {
+            auto compute = [&](const Row &rw) {
                 const __half *arow = sF + row * LDW + hf * HALF;
-                const int nc = ncopies(mm);
                 float acc[KC];
 #pragma unroll
                 for (int c = 0; c < KC; ++c)
@@ -541,7 +612,7 @@ namespace psmas
                     {
                         if (c >= nc)
                             break;
-                        mz[static_cast<size_t>(mm + c * nfine1) * N + row] = (acc[c] + part[c][row]) * rw.d;
+                        mz[static_cast<size_t>(rw.bank + c * rw.stride) * N + row] = (acc[c] + part[c][row]) * rw.d;
                     }
                 }
                 // part[] is next written after the following unpack's barrier
@@ -561,7 +632,7 @@ namespace psmas
                 Row now = ra;
                 if (nn < end)
                     load(regA, ra, nn);
-                compute(cur, now);
+                compute(now);
                 cur = nxt;
                 nxt = nn;
                 if (cur >= end)
@@ -571,7 +642,7 @@ namespace psmas
                 now = rb;
                 if (nn < end)
                     load(regB, rb, nn);
-                compute(cur, now);
+                compute(now);
                 cur = nxt;
                 nxt = nn;
                 if (cur >= end)
@@ -592,7 +663,7 @@ namespace psmas
                                                                           const T *__restrict__ mr, T *__restrict__ mz,
                                                                           const int *__restrict__ bank_sys,
                                                                           const int *__restrict__ active,
-                                                                          ApplySide sd)
+                                                                          ApplySide sd, CopyLayout cl)
         {
             constexpr int N = 32 * BD;
             constexpr int L = N * (N + 1) / 2;
@@ -631,7 +702,9 @@ namespace psmas
             u32x4 reg[PER];
             TA xr = TA(0);
             auto load = [&](int mm) {
-                const u32x4 *src = reinterpret_cast<const u32x4 *>(mats + static_cast<size_t>(mm) * L);
+                // Several right-hand sides of one factorisation: bank mm's matrix is stored once.
+                const int sm = cl.copies > 1 ? stored_of_bank(cl, mm) : mm;
+                const u32x4 *src = reinterpret_cast<const u32x4 *>(mats + static_cast<size_t>(sm) * L);
 #pragma unroll
                 for (int k = 0; k < PER; ++k)
                 {
@@ -1367,8 +1440,8 @@ namespace psmas
 
     template <int BD, typename TM, typename T>
     void launch_mas_apply(int mat_begin, int count, const TM *mats, const float *dscale, const T *mr, T *mz,
-                          const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side, int copies,
-                          int nfine1)
+                          const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side,
+                          const CopyLayout &cl)
     {
         if (count <= 0 && side.wgs <= 0)
             return;
@@ -1400,13 +1473,15 @@ namespace psmas
     {                                                                                                                  \
         static const int resident = resident_of(reinterpret_cast<const void *>(&k_mas_apply_h16<BD, KC>));             \
         hipLaunchKernelGGL((k_mas_apply_h16<BD, KC>), dim3(grid_for(resident)), dim3(apply_block<BD>()), 0, s,         \
-                           mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side, copies, nfine1);            \
+                           mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side, cl);                        \
     }
-            if (copies <= 1 || nfine1 <= 0)
+            if (cl.copies > 8)
+                throw std::logic_error("[MAS] internal error: at most 8 right-hand sides per factorisation.");
+            if (cl.copies <= 1)
                 PSMAS_H16_LAUNCH(1)
-            else if (copies <= 2)
+            else if (cl.copies <= 2)
                 PSMAS_H16_LAUNCH(2)
-            else if (copies <= 4)
+            else if (cl.copies <= 4)
                 PSMAS_H16_LAUNCH(4)
             else
                 PSMAS_H16_LAUNCH(8)
@@ -1416,7 +1491,7 @@ namespace psmas
         {
             static const int resident = resident_of(reinterpret_cast<const void *>(&k_mas_apply<BD, TM, T>));
             hipLaunchKernelGGL((k_mas_apply<BD, TM, T>), dim3(grid_for(resident)), dim3(apply_block<BD>()), 0, s,
-                               mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side);
+                               mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side, cl);
         }
         PSMAS_LAUNCH_CHECK();
     }
@@ -1551,10 +1626,10 @@ namespace psmas
 #define PSMAS_INST_APPLY(BD)                                                                                     \
     template void launch_mas_apply<BD, double, double>(int, int, const double *, const float *, const double *,   \
                                                        double *, const int *, const int *, hipStream_t,          \
-                                                       const ApplySide &, int, int);                             \
+                                                       const ApplySide &, const CopyLayout &);                   \
     template void launch_mas_apply<BD, __half, float>(int, int, const __half *, const float *, const float *,     \
                                                       float *, const int *, const int *, hipStream_t,            \
-                                                      const ApplySide &, int, int);                              \
+                                                      const ApplySide &, const CopyLayout &);                    \
     template void launch_prolong_dot<BD, double>(int, int, const int4 *, const int *, const double *, double *,   \
                                                  const Reductions &, bool, const SysMap &, const LccApply &,     \
                                                  const FlexDot<double> &, hipStream_t);                          \

@@ -31,7 +31,7 @@
 //                        (only the skip layer is trained: a learned linear coarse map)
 //   spmv_unroll          SpMV row-loop batch depth 2 | 4 | 8 (chosen by psmas_autotune)
 //   spmv_copies          after factorize_copies, one SpMV launch reads the matrix once for every
-//                        right-hand side (default true; false: the batch's per-system SpMV)
+//                        right-hand side (default true; false: one SpMV launch per right-hand side)
 //   device_setup         build the SELL matrix on the GPU from the CSC arrays, for one system, a
 //                        batch or the copies of factorize_copies, and aggregate there when no
 //                        coordinates are given (default true; false: the host build)

@@ -158,18 +158,6 @@ namespace psmas
             if (d)
                 *dup = 1;
         }
-        /// Copy c of the columns: every column (padding slots hold their own row) moved to copy
-        /// c's rows, c * npad further on.
-        __global__ __launch_bounds__(TB) void k_replicate_cols(long long len, int copies, int npad,
-                                                               const int *__restrict__ cols, int *__restrict__ out)
-        {
-            const long long i = static_cast<long long>(blockIdx.x) * TB + threadIdx.x;
-            if (i >= len)
-                return;
-            const int v = cols[i];
-            for (int c = 0; c < copies; ++c)
-                out[static_cast<long long>(c) * len + i] = v + c * npad;
-        }
 
         __global__ __launch_bounds__(TB) void k_compose_map(long long n, const int *__restrict__ keep,
                                                             const int *__restrict__ map, int *__restrict__ out)
@@ -326,51 +314,6 @@ namespace psmas
         if (!out.map_ok)
             out.d_map.reset();
         out.d_row_len = std::move(row_len);
-    }
-
-    void replicate_sell(SellDevice &m, int copies, int npad, hipStream_t s)
-    {
-        if (copies <= 1)
-            return;
-        if (static_cast<long long>(npad) * copies > INT_MAX ||
-            m.nslots * copies > static_cast<long long>(INT_MAX))
-            throw InvalidArgumentError("[MAS] too many right-hand sides for this matrix size.");
-        const int nslices = npad / 64;
-        const long long ncols = m.nslots * 64;
-        const size_t nvals = m.d_vals.size();
-        m.slice_ptr.resize(static_cast<size_t>(nslices) * copies + 1);
-        for (int c = 1; c < copies; ++c)
-            for (int k = 1; k <= nslices; ++k)
-                m.slice_ptr[static_cast<size_t>(c) * nslices + k] =
-                    m.slice_ptr[k] + static_cast<int>(c * m.nslots);
-        std::vector<char> iso(m.nonisolated);
-        m.nonisolated.resize(static_cast<size_t>(npad) * copies);
-        for (int c = 1; c < copies; ++c)
-            std::copy(iso.begin(), iso.end(), m.nonisolated.begin() + static_cast<size_t>(c) * npad);
-        DeviceBuffer<int> sp(m.slice_ptr.size(), "MAS SELL slice pointers");
-        PSMAS_HIP(hipMemcpyAsync(sp.data(), m.slice_ptr.data(), sp.bytes(), hipMemcpyHostToDevice, s));
-        DeviceBuffer<int> cols(static_cast<size_t>(ncols) * copies, "MAS SELL columns");
-        hipLaunchKernelGGL(k_replicate_cols, dim3(grid_for(ncols)), dim3(TB), 0, s, ncols, copies, npad,
-                           m.d_cols.data(), cols.data());
-        PSMAS_LAUNCH_CHECK();
-        DeviceBuffer<double> vals(nvals * copies, "MAS SELL values (FP64)");
-        DeviceBuffer<int> row_len(static_cast<size_t>(npad) * copies, "MAS SELL row lengths");
-        for (int c = 0; c < copies; ++c)
-        {
-            PSMAS_HIP(hipMemcpyAsync(vals.data() + c * nvals, m.d_vals.data(), sizeof(double) * nvals,
-                                     hipMemcpyDeviceToDevice, s));
-            PSMAS_HIP(hipMemcpyAsync(row_len.data() + static_cast<size_t>(c) * npad, m.d_row_len.data(),
-                                     sizeof(int) * npad, hipMemcpyDeviceToDevice, s));
-        }
-        PSMAS_HIP(hipStreamSynchronize(s));
-        m.d_slice_ptr = std::move(sp);
-        m.d_cols = std::move(cols);
-        m.d_vals = std::move(vals);
-        m.d_row_len = std::move(row_len);
-        m.nnzb *= copies;
-        m.nslots *= copies;
-        m.d_map.reset();
-        m.map_ok = false;
     }
 
     DeviceBuffer<int> compose_value_map(const DeviceBuffer<int> &keep, size_t n, const DeviceBuffer<int> &map,

@@ -40,11 +40,6 @@ namespace psmas
                            const double *d_vals, const int *d_slot_of_block, int npad, bool want_map,
                            hipStream_t s, SellDevice &out);
 
-    /// The block-diagonal matrix of `copies` copies of m (built with npad slots): copy c's rows
-    /// and columns are those of copy 0 moved by c * npad, its slices follow copy c-1's, as the
-    /// concatenated build of identical systems lays them out.  Drops the value map.
-    void replicate_sell(SellDevice &m, int copies, int npad, hipStream_t s);
-
     /// map[keep[k]] for every k (-1 where keep[k] < 0): the value index of each non-zero of a
     /// matrix whose non-zero k became non-zero keep[k] of the built one (or was dropped).
     DeviceBuffer<int> compose_value_map(const std::vector<int> &keep, const DeviceBuffer<int> &map, hipStream_t s);

@@ -386,6 +386,25 @@ namespace psmas
         return m;
     }
 
+    Reductions Solver::reductions_of_copy(int c) const
+    {
+        Reductions r = reductions();
+        const size_t o = static_cast<size_t>(c) * nblocks1_;
+        r.pap += o;
+        r.rr += o;
+        r.rz += o;
+        r.rz_old += o;
+        r.rr_true += o;
+        return r;
+    }
+
+    SysMap Solver::sysmap_of_copy(int c) const
+    {
+        SysMap m = sysmap();
+        m.block_sys += static_cast<size_t>(c) * nblocks1_;
+        return m;
+    }
+
     FineRestrict Solver::fine_restrict() const
     {
         FineRestrict f;
@@ -452,7 +471,7 @@ namespace psmas
     }
 
     void Solver::factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
-                                const DeviceCsc *dev, bool replicate, const DeviceBuffer<int> *d_keep)
+                                const DeviceCsc *dev, const DeviceBuffer<int> *d_keep)
     {
         const int bd = params_.block_dim;
         factorized_ = false;
@@ -480,21 +499,19 @@ namespace psmas
         const bool bd_changed = (bd != bd_);
         bd_ = bd;
         // The SELL matrix is built on the GPU from the CSC arrays (uploaded here, or the caller's
-        // device arrays): one system, copies of one, or every system of a batch.
-        replicate = replicate && K > 1;
+        // device arrays): one system or every system of a batch.
         const bool gpu_build = dev != nullptr || params_.device_setup;
         vr_nnz_ = -1;
         // The CSC arrays in device memory for the device aggregation and build (host input is
         // uploaded once here; one system's values stay for values-only refactorisations).
         std::vector<DeviceCsc> dcsc(K);
-        const int nup = replicate ? 1 : K;
-        std::vector<DeviceBuffer<int>> up_cp(nup), up_ri(nup);
-        std::vector<DeviceBuffer<double>> up_v(nup);
+        std::vector<DeviceBuffer<int>> up_cp(K), up_ri(K);
+        std::vector<DeviceBuffer<double>> up_v(K);
         if (gpu_build && dev != nullptr)
-            std::fill(dcsc.begin(), dcsc.end(), *dev); // (one system, or copies of it)
+            dcsc[0] = *dev; // (one system)
         else if (gpu_build)
         {
-            for (int u = 0; u < nup; ++u)
+            for (int u = 0; u < K; ++u)
             {
                 const CscView &A = As[u];
                 const size_t nz = static_cast<size_t>(std::max<long long>(A.nnz, 1));
@@ -512,8 +529,6 @@ namespace psmas
                 }
                 dcsc[u] = DeviceCsc{A.n, A.nnz, up_cp[u].data(), up_ri[u].data(), v.data()};
             }
-            for (int s = nup; s < K; ++s)
-                dcsc[s] = dcsc[0];
             phase("csc_upload");
         }
 
@@ -538,15 +553,9 @@ namespace psmas
                                  static_cast<int>(S.part.perm.size()) != nb || bd_changed;
             S.n = A.n;
             S.nb = nb;
-            if (replicate && s > 0)
-            {
-                S.part = sys_[0].part;
-                S.aggregation = sys_[0].aggregation;
-                S.have_partition = true;
-            }
-            else if (rebuild)
+            if (rebuild)
                 build_partition(S, A, gpu_build ? &dcsc[s] : nullptr);
-            if (s == 0 && (K == 1 || replicate))
+            if (s == 0 && K == 1)
                 phase("partition");
             PaddedBsr bsr;
             if (gpu_build)
@@ -592,9 +601,9 @@ namespace psmas
 
         if (gpu_build)
         {
-            if (K > 1 && !replicate)
+            if (K > 1)
                 phase("partition");
-            device_matrix_build(As[0], dcsc, dev != nullptr, replicate, h_slot_of_block, raw, dir, d_keep, phase);
+            device_matrix_build(As[0], dcsc, dev != nullptr, h_slot_of_block, raw, dir, d_keep, phase);
         }
         else
         {
@@ -686,7 +695,7 @@ namespace psmas
 
     template <typename Phase>
     void Solver::device_matrix_build(const CscView &A, const std::vector<DeviceCsc> &dcsc, bool device_input,
-                                     bool replicate, const std::vector<int> &h_slot_of_block, const CscView *raw,
+                                     const std::vector<int> &h_slot_of_block, const CscView *raw,
                                      const DirichletSystem *dir, const DeviceBuffer<int> *d_keep, Phase &phase)
     {
         System &S = sys_[0];
@@ -695,13 +704,11 @@ namespace psmas
         PSMAS_HIP(hipMemcpyAsync(d_sob.data(), h_slot_of_block.data(), d_sob.bytes(), hipMemcpyHostToDevice, stream_));
         SellDevice sd;
         const int K = static_cast<int>(sys_.size());
-        if (K == 1 || replicate)
+        if (K == 1)
         {
             build_sell_device(S.n, nnz, bd_, dcsc[0].colptr, dcsc[0].rowidx, dcsc[0].vals, d_sob.data(), S.npad,
-                              params_.lazy_partitioning && K == 1, stream_, sd);
-            for (System &C : sys_)
-                C.nnzb = sd.nnzb;
-            replicate_sell(sd, K, S.npad, stream_); // (copies of system 0 for several right-hand sides)
+                              params_.lazy_partitioning, stream_, sd);
+            S.nnzb = sd.nnzb;
         }
         else
         {
@@ -783,22 +790,125 @@ namespace psmas
         }
         else
             validate_csc(A);
-        // System 0's coordinates (if any) go to every copy, so that all copies get one partition.
-        if (!sys_.empty() && sys_[0].coord_dim > 0)
+        // One factorisation (system 0 keeps the coordinates), then per-copy node maps.
+        if (sys_.size() > 1)
+            sys_.resize(1);
+        factorize_full(std::vector<CscView>{A}, nullptr, nullptr, dev);
+        if (copies == 1)
+            return;
+        const auto t0 = std::chrono::steady_clock::now();
+        // The coarsest-level inverse is built now, from the single system's maps, whatever the
+        // coarse_solver parameter (it may be switched on later).
+        build_dense_coarse();
+        expand_copies(copies);
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        setup_ms_.emplace_back("expand_copies",
+                               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+
+    void Solver::expand_copies(int k)
+    {
+        const int npad1 = npad_, nblocks1 = nblocks_, levels = cl_.levels;
+        const System S0 = sys_[0];
+        // Level-major multi-level numbering: level l holds copy 0's cnt[l] banks, then copy 1's...
+        CopyLayout cl = cl_;
+        cl.copies = k;
+        int P[CopyLayout::MAXL] = {}; // one copy's (bank-padded) nodes of each level
+        cl.moff[0] = 0;
+        for (int l = 0; l < levels; ++l)
         {
-            const System S0 = sys_[0];
-            sys_.resize(copies);
-            for (int c = 1; c < copies; ++c)
-            {
-                sys_[c].coords = S0.coords;
-                sys_[c].coord_dim = S0.coord_dim;
-            }
+            P[l] = BANK * cl.cnt[l];
+            cl.moff[l + 1] = cl.moff[l] + k * cl.cnt[l];
         }
-        factorize_full(std::vector<CscView>(copies, A), nullptr, nullptr, dev, true);
-        for (int c = 1; c < copies; ++c)
-            if (sys_[c].npad != sys_[0].npad || sys_[c].part.perm != sys_[0].part.perm)
-                return; // (not expected: the copies keep their own SpMV then)
-        copies_ = copies;
+        for (int l = 0; l <= MAX_COARSE_LEVELS; ++l)
+        {
+            mat_off_[l] = l < levels ? cl.moff[l] : cl.moff[levels];
+            mat_cnt_[l] = l < levels ? k * cl.cnt[l] : 0;
+            lev_off_[l] = BANK * mat_off_[l];
+        }
+        total_mats_ = cl.moff[levels];
+        total_level_nodes_ = BANK * total_mats_;
+        cl_ = cl;
+        npad_ = k * npad1;
+        nparts_ = npad_ / BANK;
+        nblocks_ = k * nblocks1;
+        n_total_ *= k;
+        nb_total_ *= k;
+        sys_.assign(k, S0);
+        for (int c = 0; c < k; ++c)
+        {
+            System &S = sys_[c];
+            S.node_off = c * npad1;
+            S.blk_begin = c * nblocks1;
+            S.sob_off = c * S0.nb;
+        }
+
+        // Node maps: copy c's coarse ids are copy 0's plus c times the level's padded size.
+        std::vector<int4> cm(npad1);
+        std::vector<uint32_t> lead(npad1);
+        std::vector<int> gid(npad1), sob(S0.slot_of_block);
+        PSMAS_HIP(hipMemcpyAsync(cm.data(), d_cmap_.data(), sizeof(int4) * npad1, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipMemcpyAsync(lead.data(), d_lead0_.data(), sizeof(uint32_t) * npad1, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipMemcpyAsync(gid.data(), d_gid0_.data(), sizeof(int) * npad1, hipMemcpyDeviceToHost, stream_));
+        PSMAS_HIP(hipStreamSynchronize(stream_));
+        std::vector<int4> cmk(static_cast<size_t>(npad_));
+        std::vector<uint32_t> leadk(static_cast<size_t>(npad_));
+        std::vector<int> gidk(static_cast<size_t>(npad_)), bosk(static_cast<size_t>(npad_)),
+            sobk(static_cast<size_t>(k) * sob.size());
+        std::vector<char> nonisok(static_cast<size_t>(npad_));
+        auto shift = [](int id, int by) { return id >= 0 ? id + by : id; };
+        for (int c = 0; c < k; ++c)
+        {
+            const size_t o = static_cast<size_t>(c) * npad1;
+            for (int p = 0; p < npad1; ++p)
+            {
+                const int4 v = cm[p];
+                cmk[o + p] = int4{shift(v.x, c * P[1]), shift(v.y, c * P[2]), shift(v.z, c * P[3]), shift(v.w, c * P[4])};
+                leadk[o + p] = lead[p];
+                gidk[o + p] = shift(gid[p], c * P[1]);
+                bosk[o + p] = block_of_slot_[p];
+                nonisok[o + p] = nonisolated_[p];
+            }
+            std::copy(sob.begin(), sob.end(), sobk.begin() + static_cast<size_t>(c) * sob.size());
+        }
+        block_of_slot_.swap(bosk);
+        nonisolated_.swap(nonisok);
+        auto upload = [&](auto &buf, const auto &host, const char *what) {
+            buf.allocate(host.size(), what);
+            PSMAS_HIP(hipMemcpy(buf.data(), host.data(), buf.bytes(), hipMemcpyHostToDevice));
+        };
+        upload(d_cmap_, cmk, "MAS coarse map");
+        upload(d_lead0_, leadk, "MAS level-0 leader masks");
+        upload(d_gid0_, gidk, "MAS level-0 ids");
+        upload(d_block_of_slot_, block_of_slot_, "MAS slot map");
+        upload(d_slot_of_block_, sobk, "MAS block map");
+
+        // Systems: copy c's node blocks and fine banks.
+        std::vector<int> block_sys(nblocks_), blk_begin(k), blk_count(k), bank_sys(total_mats_, -1);
+        for (int c = 0; c < k; ++c)
+        {
+            blk_begin[c] = c * nblocks1;
+            blk_count[c] = nblocks1;
+            std::fill(block_sys.begin() + c * nblocks1, block_sys.begin() + (c + 1) * nblocks1, c);
+            std::fill(bank_sys.begin() + c * cl.cnt[0], bank_sys.begin() + (c + 1) * cl.cnt[0], c);
+        }
+        upload(d_block_sys_, block_sys, "MAS block systems");
+        upload(d_blk_begin_, blk_begin, "MAS system blocks");
+        upload(d_blk_count_, blk_count, "MAS system blocks");
+        upload(d_bank_sys_, bank_sys, "MAS matrix systems");
+        d_active_.allocate(k, "MAS system flags");
+        d_select_.allocate(k, "MAS system flags");
+        h_active_.assign(k, 1);
+        h_select_.assign(k, 1);
+        PSMAS_HIP(hipMemcpy(d_lev_off_.data(), lev_off_, sizeof(lev_off_), hipMemcpyHostToDevice));
+        PSMAS_HIP(hipMemcpy(d_mat_off_.data(), mat_off_, sizeof(mat_off_), hipMemcpyHostToDevice));
+        build_restrict_chunks(); // from the copies' coarse map: each copy's lists are copy 0's, shifted
+        allocate_vectors();
+        if (d_vals32_)
+            ensure_mixed_data();
+        copies_ = k;
+        lattice_stale_ = true;
+        graphs_->clear();
     }
 
     bool Solver::try_refresh_values(const CscView &A, const std::vector<int> *dirichlet_nodes)
@@ -929,7 +1039,7 @@ namespace psmas
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         const CscView At{A.n, D.nnz, nullptr, nullptr, nullptr};
         const DeviceCsc dv{A.n, D.nnz, D.colptr.data(), D.rowidx.data(), D.vals.data()};
-        factorize_full(std::vector<CscView>{At}, &A, nullptr, &dv, false, &D.keep);
+        factorize_full(std::vector<CscView>{At}, &A, nullptr, &dv, &D.keep);
         setup_ms_.insert(setup_ms_.begin(), {"dirichlet_elimination", elim_ms});
         n_crow_ = D.nrows;
         n_coupl_ = D.ncoupl;
@@ -1306,6 +1416,18 @@ namespace psmas
         d_mat_off_.allocate(MAX_COARSE_LEVELS + 1, "MAS matrix offsets");
         PSMAS_HIP(hipMemcpyAsync(d_lev_off_.data(), lev_off_, sizeof(lev_off_), hipMemcpyHostToDevice, stream_));
         PSMAS_HIP(hipMemcpyAsync(d_mat_off_.data(), mat_off_, sizeof(mat_off_), hipMemcpyHostToDevice, stream_));
+        // One copy: banks and stored matrices coincide.
+        stored_mats_ = total_mats_;
+        npad1_ = npad_;
+        nblocks1_ = nblocks_;
+        cl_ = CopyLayout{};
+        cl_.levels = level_num_ + 1;
+        for (int l = 0; l < cl_.levels; ++l)
+        {
+            cl_.moff[l] = cl_.soff[l] = mat_off_[l];
+            cl_.cnt[l] = mat_cnt_[l];
+        }
+        cl_.moff[cl_.levels] = cl_.soff[cl_.levels] = total_mats_;
         build_restrict_chunks();
         build_bank_inverses();
     }
@@ -1344,7 +1466,7 @@ namespace psmas
         std::vector<int4> cm(npad_);
         PSMAS_HIP(hipMemcpyAsync(cm.data(), d_cmap_.data(), sizeof(int4) * npad_, hipMemcpyDeviceToHost, stream_));
         PSMAS_HIP(hipStreamSynchronize(stream_));
-        const int n1 = cco_[0];
+        const int n1 = lev_off_[2] - lev_off_[1]; // segment-1 slots (every copy's, bank-padded)
         std::vector<int4> anc(n1, int4{-1, -1, -1, -1});
         for (int p = 0; p < npad_; ++p)
             if (cm[p].x >= 0)
@@ -1461,6 +1583,8 @@ namespace psmas
             sd.dense_seg = lev_off_[level_num_];
             sd.dense_index = d_dense_index_.data();
             sd.dense_count = cco_[level_num_ - 1];
+            sd.dense_copies = copies_;
+            sd.dense_stride = BANK * cl_.cnt[level_num_];
             sd.wgs = std::min(256, div_up(dense_nd_, block / 64)); // wave per row
         }
         return sd;
@@ -1511,9 +1635,9 @@ namespace psmas
         {
             // Every level's inverses in scaled FP16 (S = D^-1 Inv D^-1, unit diagonal): one apply
             // launch covers all matrices.
-            d_mats16_.allocate(static_cast<size_t>(total_mats_) * (dim * dim / 2), "MAS bank inverses (FP16)");
-            d_dscale_.allocate(static_cast<size_t>(total_mats_) * dim, "MAS bank scaling");
-            launch_make_half(bd_, total_mats_, d_mats64_.data(), d_mats16_.data(), d_dscale_.data(), stream_);
+            d_mats16_.allocate(static_cast<size_t>(stored_mats_) * (dim * dim / 2), "MAS bank inverses (FP16)");
+            d_dscale_.allocate(static_cast<size_t>(stored_mats_) * dim, "MAS bank scaling");
+            launch_make_half(bd_, stored_mats_, d_mats64_.data(), d_mats16_.data(), d_dscale_.data(), stream_);
         }
         const size_t nvec = static_cast<size_t>(npad_) * bd_;
         const size_t nml = static_cast<size_t>(total_level_nodes_) * bd_;
@@ -1534,6 +1658,13 @@ namespace psmas
     {
         if (params_.coarse_solver != CoarseSolver::Cholesky || dense_built_)
             return;
+        build_dense_coarse();
+    }
+
+    void Solver::build_dense_coarse()
+    {
+        if (copies_ > 1)
+            throw std::logic_error("[MAS] internal error: the coarsest inverse is built before the copies' maps.");
         dense_built_ = true;
         dense_nd_ = 0;
         const int level = level_num_ - 1; // coarsest coarse level (segment level_num_)
@@ -1760,6 +1891,37 @@ namespace psmas
         PSMAS_HIP(hipGraphLaunch(it->second, stream_));
     }
 
+    template <int BD, typename T>
+    void Solver::spmv_ap_all(const SellView<T> &A, const T *x, T *y)
+    {
+        if (copies_ > 1 && params_.spmv_copies)
+            launch_spmv_ap_copies<BD, T, T>(A, copies_, npad1_, x, y, reductions(), sysmap(), stream_);
+        else if (copies_ > 1)
+            for (int c = 0; c < copies_; ++c)
+            {
+                const size_t o = static_cast<size_t>(c) * npad1_ * BD;
+                launch_spmv_ap<BD, T, T>(A, x + o, y + o, reductions_of_copy(c), sysmap_of_copy(c), stream_);
+            }
+        else
+            launch_spmv_ap<BD, T, T>(A, x, y, reductions(), sysmap(), stream_);
+    }
+
+    template <int BD, typename TX, typename TY>
+    void Solver::spmv_res_all(const SellView<double> &A, const TX *x, const TX *b, TY *y, double *rr, bool selected)
+    {
+        TY *const level1 = y + static_cast<size_t>(npad_) * BD; // every copy's (copy-shifted ids)
+        for (int c = 0; c < copies_; ++c)
+        {
+            const size_t o = static_cast<size_t>(c) * npad1_;
+            FineRestrict fr = fine_restrict();
+            fr.lead += o;
+            fr.gid += o;
+            launch_spmv_res<BD, double, TX, TY>(A, x + o * BD, b + o * BD, y + o * BD, fr, level1,
+                                                rr + static_cast<size_t>(c) * nblocks1_, sysmap_of_copy(c), selected,
+                                                stream_);
+        }
+    }
+
     // ---- FP64 path: the reference schedule -------------------------------------------------
     template <int BD>
     void Solver::fp64_precondition(bool init)
@@ -1774,7 +1936,8 @@ namespace psmas
         launch_mas_apply<BD, double, double>(0, n1, d_mats64_.data(), nullptr, d_mr_.data(), d_mz_.data(), nullptr,
                                              nullptr, stream_,
                                              restrict_side(d_mr_.data() + static_cast<size_t>(lev_off_[1]) * BD,
-                                                           d_mr_.data(), block));
+                                                           d_mr_.data(), block),
+                                             cl_);
         toc();
         const ApplySide cs = coarse_side(block);
         if (total_mats_ > n1 || cs.wgs > 0)
@@ -1782,7 +1945,7 @@ namespace psmas
             tic("mas_apply_fp64_coarse", static_cast<double>(total_mats_ - n1) * per_mat +
                                              static_cast<double>(dense_nd_) * dense_nd_ * 8.0);
             launch_mas_apply<BD, double, double>(n1, total_mats_ - n1, d_mats64_.data(), nullptr, d_mr_.data(),
-                                                 d_mz_.data(), nullptr, nullptr, stream_, cs);
+                                                 d_mz_.data(), nullptr, nullptr, stream_, cs, cl_);
             toc();
         }
         launch_prolong_dot<BD, double>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr_.data(),
@@ -1792,7 +1955,7 @@ namespace psmas
 
     bool Solver::fuse_direction() const
     {
-        return params_.spmv_fuse_direction && params_.spmv_halo && halo_max_ > 0 && !(copies_ > 1 && params_.spmv_copies);
+        return params_.spmv_fuse_direction && params_.spmv_halo && halo_max_ > 0 && copies_ == 1;
     }
 
     template <int BD>
@@ -1816,11 +1979,7 @@ namespace psmas
         else
         {
             tic("spmv_fp64", spmv_bytes(8, 8, 2));
-            if (copies_ > 1 && params_.spmv_copies)
-                launch_spmv_ap_copies<BD, double, double>(A, copies_, sys_[0].npad, d_p_.data(), d_Ap_.data(),
-                                                          reductions(), sysmap(), stream_);
-            else
-                launch_spmv_ap<BD, double, double>(A, d_p_.data(), d_Ap_.data(), reductions(), sysmap(), stream_);
+            spmv_ap_all<BD, double>(A, d_p_.data(), d_Ap_.data());
             toc();
         }
         double *p = fuse ? p64(p_cur_) : d_p_.data();
@@ -1828,9 +1987,7 @@ namespace psmas
         {
             launch_axpy_alpha<BD, double>(nblocks_, p, d_x_.data(), reductions(), sysmap(), stream_);
             tic("spmv_res_fp64", spmv_bytes(8, 8, 2) + n_total_ * 8.0);
-            launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
-                                                        d_mr_.data() + static_cast<size_t>(npad_) * BD,
-                                                        reductions().rr, sysmap(), false, stream_);
+            spmv_res_all<BD, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), reductions().rr, false);
             toc();
         }
         else
@@ -1853,9 +2010,7 @@ namespace psmas
             const long long nvec = static_cast<long long>(npad_) * BD;
             launch_fill<double>(nvec, d_x_.data(), 0.0, stream_);
             // r = b - A x (x = 0), rr, restriction.
-            launch_spmv_res<BD, double, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), fine_restrict(),
-                                                        d_mr_.data() + static_cast<size_t>(npad_) * BD,
-                                                        reductions().rr, sysmap(), false, stream_);
+            spmv_res_all<BD, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), reductions().rr, false);
             fp64_precondition<BD>(true);
             const bool fuse = fuse_direction();
             p_cur_ = 0;
@@ -1965,11 +2120,7 @@ namespace psmas
         else
         {
             tic("spmv_fp32", spmv_bytes(4, 4, 2));
-            if (copies_ > 1 && params_.spmv_copies)
-                launch_spmv_ap_copies<BD, float, float>(A, copies_, sys_[0].npad, d_p32_.data(), d_Ap32_.data(),
-                                                        reductions(), sysmap(), stream_);
-            else
-                launch_spmv_ap<BD, float, float>(A, d_p32_.data(), d_Ap32_.data(), reductions(), sysmap(), stream_);
+            spmv_ap_all<BD, float>(A, d_p32_.data(), d_Ap32_.data());
             toc();
         }
         // With the learned correction on the block path, the update also leaves the lattice block
@@ -1992,8 +2143,8 @@ namespace psmas
         // residual update left) in its leading workgroups; launch 2: banks of segments 2.. and the
         // dense coarsest solve; then the network.
         constexpr int block = 64 * BD;
-        const int n1 = first_apply_mats();
-        const double per_mat = dim * dim * 1.0 + dim * 4.0 + 2.0 * dim * 4.0;
+        const int n1 = first_apply_stored(); // (the FP16 apply walks stored matrices)
+        const double per_mat = dim * dim * 1.0 + dim * 4.0 + 2.0 * dim * 4.0 * copies_;
         const bool lat_blocks = learned && lcc_->block_restriction();
         LccApply lv;
         FlexDot<float> flex;
@@ -2017,26 +2168,27 @@ namespace psmas
             }
         }
         tic("mas_apply_fp16", n1 * per_mat);
-        const bool shared_fine = copies_ > 1 && params_.spmv_copies; // one matrix read for every copy
         // A single system is active for the whole apply: no per-matrix system lookups (two dependent
-        // loads per matrix on each workgroup's critical path).
-        const int *bsys = sys_.size() > 1 ? d_bank_sys_.data() : nullptr;
-        const int *bact = sys_.size() > 1 ? d_active_.data() : nullptr;
+        // loads per matrix on each workgroup's critical path).  Copies of one factorisation apply
+        // every stored matrix to all of them.
+        const bool batch = sys_.size() > 1 && copies_ == 1;
+        const int *bsys = batch ? d_bank_sys_.data() : nullptr;
+        const int *bact = batch ? d_active_.data() : nullptr;
         launch_mas_apply<BD, __half, float>(0, n1, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
                                             d_mz32_.data(), bsys, bact, stream_,
                                             restrict_side(d_mr32_.data() + static_cast<size_t>(lev_off_[1]) * BD,
                                                           d_mr32_.data(), block, lat_blocks),
-                                            shared_fine ? copies_ : 1, shared_fine ? sys_[0].nparts : 0);
+                                            cl_);
         toc();
         // The network's first layer and skip layer join the second launch's side work when they
         // fit; the output layer follows it.
         ApplySide cs = coarse_side(block);
         const bool lin_side = learned && lcc_->in_side(cs, apply_h16_stage_floats(BD), block, d_active_.data());
-        if (total_mats_ > n1 || cs.wgs > 0)
+        if (stored_mats_ > n1 || cs.wgs > 0)
         {
-            tic("mas_apply_fp16_coarse", (total_mats_ - n1) * per_mat + static_cast<double>(dense_nd_) * dense_nd_ * 8.0);
-            launch_mas_apply<BD, __half, float>(n1, total_mats_ - n1, d_mats16_.data(), d_dscale_.data(),
-                                                d_mr32_.data(), d_mz32_.data(), bsys, bact, stream_, cs);
+            tic("mas_apply_fp16_coarse", (stored_mats_ - n1) * per_mat + static_cast<double>(dense_nd_) * dense_nd_ * 8.0);
+            launch_mas_apply<BD, __half, float>(n1, stored_mats_ - n1, d_mats16_.data(), d_dscale_.data(),
+                                                d_mr32_.data(), d_mz32_.data(), bsys, bact, stream_, cs, cl_);
             toc();
         }
         if (learned)
@@ -2076,9 +2228,7 @@ namespace psmas
     {
         const SellView<double> A = sell(d_vals64_.data());
         tic("spmv_res_fp64", spmv_bytes(8, 8, 3));
-        launch_spmv_res<BD, double, double, float>(A, d_x_.data(), d_b_.data(), d_mr32_.data(), fine_restrict(),
-                                                   d_mr32_.data() + static_cast<size_t>(npad_) * BD,
-                                                   reductions().rr_true, sysmap(), true, stream_);
+        spmv_res_all<BD, double, float>(A, d_x_.data(), d_b_.data(), d_mr32_.data(), reductions().rr_true, true);
         toc();
         // Block path of the learned correction: the preconditioner expects the lattice block
         // partials of the current r (the residual update leaves them; here r was recomputed).
@@ -2459,7 +2609,7 @@ namespace psmas
     std::vector<int> Solver::level_info() const
     {
         const System &S = sys_[0];
-        return {level_num_, cco_[0], cco_[1], cco_[2], cco_[3], S.nparts, S.npad, total_mats_,
+        return {level_num_, cco_[0], cco_[1], cco_[2], cco_[3], S.nparts, S.npad, stored_mats_,
                 static_cast<int>(std::min<long long>(S.nnzb, 2147483647LL)), static_cast<int>(nslots_), S.nb, S.n};
     }
 

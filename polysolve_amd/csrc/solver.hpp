@@ -119,9 +119,10 @@ namespace psmas
         /// Values-only refactorisation from device memory (values in the recorded CSC order of the
         /// last full factorisation with lazy_partitioning).
         void refresh_values_device(long long nnz, const double *d_vals);
-        /// Several right-hand sides of one matrix: a batch of `copies` (<= MAX_COPIES) systems
-        /// that repeat A, whose SpMV reads the matrix once for all of them.  solve_batch then
-        /// takes the right-hand sides.
+        /// Several right-hand sides of one matrix: one factorisation of A (one SELL matrix, one
+        /// set of bank inverses and coarsest inverse) shared by `copies` (<= MAX_COPIES) systems
+        /// whose vectors, coarse numbering and scalars are per copy; the SpMV and the bank apply
+        /// read each matrix once for all of them.  solve_batch then takes the right-hand sides.
         void factorize_copies(const CscView &A, int copies, const DeviceCsc *dev = nullptr);
         static constexpr int MAX_COPIES = 8;
         /// Full factorisation of one system from CSC arrays in device memory (the matrix is built
@@ -231,7 +232,10 @@ namespace psmas
         DeviceBuffer<int> d_vmap_;
         DeviceBuffer<double> d_csc_vals_;
         const char *factorize_mode_ = "full";
-        int copies_ = 1; ///< > 1: the batch repeats one matrix (factorize_copies)
+        int copies_ = 1; ///< > 1: right-hand sides sharing one factorisation (factorize_copies)
+        CopyLayout cl_;  ///< bank and stored-matrix numbering of those copies
+        int stored_mats_ = 0; ///< bank matrices held (one copy's)
+        int npad1_ = 0, nblocks1_ = 0; ///< one copy's padded nodes and node blocks
         std::vector<std::pair<std::string, double>> setup_ms_; ///< phases of the last full factorisation
 
         // Dirichlet elimination: the fixed dofs (sorted, unique) and A's couplings a_ij (i free,
@@ -300,6 +304,18 @@ namespace psmas
         size_t event_pool_used_ = 0;
 
         Reductions reductions() const;
+        /// The partials of copy c's node blocks (pointers offset to its first block), for one
+        /// launch over that copy's rows with copy 0's matrix.
+        Reductions reductions_of_copy(int c) const;
+        SysMap sysmap_of_copy(int c) const;
+        /// y = A x with the PCG partials, for every system: one launch, the copies launch (matrix
+        /// read once for all copies) or one launch per copy (spmv_copies false).
+        template <int BD, typename T>
+        void spmv_ap_all(const SellView<T> &A, const T *x, T *y);
+        /// y = b - A x (FP64 matrix) with its level-0 restriction and r . r partials into rr, for
+        /// every system (one launch per copy with copies); y is the multi-level vector.
+        template <int BD, typename TX, typename TY>
+        void spmv_res_all(const SellView<double> &A, const TX *x, const TX *b, TY *y, double *rr, bool selected);
         void finalize_scalars();
         SysMap sysmap() const;
         FineRestrict fine_restrict() const;
@@ -316,6 +332,11 @@ namespace psmas
         /// use, and vec_passes row vectors of x_bytes per scalar.
         double spmv_bytes(int val_bytes, int x_bytes, int vec_passes) const;
         void build_mas();
+        /// Several right-hand sides of the single-system factorisation just built: per-copy node
+        /// maps (coarse map, restriction lists and masks, slot maps, systems), level-major
+        /// multi-level numbering, vectors; the matrices stay as built.
+        void expand_copies(int copies);
+        void build_dense_coarse();
         void build_bank_inverses(); ///< Galerkin bank matrices and their inverses (numeric part)
         /// Values-only refactorisation: one system, lazy_partitioning, the same n, block size and
         /// sparsity pattern as the last full factorisation, and the same fixed dofs (dirichlet_nodes,
@@ -325,21 +346,19 @@ namespace psmas
         bool try_refresh_values(const CscView &A, const std::vector<int> *dirichlet_nodes);
         /// Full factorisation; for a Dirichlet one, As[0] is the eliminated matrix, raw the matrix
         /// before elimination (whose non-zeros the value map then indexes) and dir the elimination.
-        /// replicate: every As[s] is As[0] (copies for several right-hand sides); system 0's
-        /// partition and matrix are built once and copied.  d_keep: a Dirichlet elimination made
-        /// on the GPU (dev = A′, raw = A, dir = nullptr) with its raw -> A′ map in device memory.
+        /// d_keep: a Dirichlet elimination made on the GPU (dev = A′, raw = A, dir = nullptr) with
+        /// its raw -> A′ map in device memory.
         void factorize_full(const std::vector<CscView> &As, const CscView *raw, const DirichletSystem *dir,
-                            const DeviceCsc *dev = nullptr, bool replicate = false,
-                            const DeviceBuffer<int> *d_keep = nullptr);
+                            const DeviceCsc *dev = nullptr, const DeviceBuffer<int> *d_keep = nullptr);
         /// The SELL matrix built on the GPU (sell_build.hpp) from the CSC arrays in device memory
         /// (dcsc[s] per system: uploaded, or the caller's arrays when device_input), for one
-        /// system, `replicate` copies of system 0, or every system of a batch; also the halo
+        /// system or every system of a batch; also the halo
         /// lists and, with lazy_partitioning, the value map (and A's host pattern for host input);
         /// for a Dirichlet factorisation (raw, dir as in factorize_full) the map indexes raw's
         /// non-zeros.  phase(name) closes a timed setup phase.
         template <typename Phase>
         void device_matrix_build(const CscView &A, const std::vector<DeviceCsc> &dcsc, bool device_input,
-                                 bool replicate, const std::vector<int> &h_slot_of_block, const CscView *raw,
+                                 const std::vector<int> &h_slot_of_block, const CscView *raw,
                                  const DirichletSystem *dir, const DeviceBuffer<int> *d_keep, Phase &phase);
         /// New SELL values (and coupling values of a Dirichlet factorisation) from d_vals in the
         /// recorded CSC order.
@@ -368,6 +387,8 @@ namespace psmas
         ApplySide coarse_side(int block) const;
         /// Matrices of the first apply launch: fine banks and segment-1 banks.
         int first_apply_mats() const { return level_num_ >= 2 ? mat_off_[2] : total_mats_; }
+        /// The same boundary in stored matrices (differs with copies).
+        int first_apply_stored() const { return level_num_ >= 2 ? cl_.soff[2] : stored_mats_; }
         void ensure_mixed_data();
         void ensure_coarse_solver();
         void allocate_vectors();

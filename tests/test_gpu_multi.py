@@ -1,7 +1,8 @@
-"""Several right-hand sides of one matrix (factorize_multi / solve_multi): every column solved to
-tolerance; the copies SpMV (matrix read once for all right-hand sides) gives the iterates of the
-batch's per-system SpMV bit for bit; device arrays give the host result; each column matches a
-solve of that right-hand side alone."""
+"""Several right-hand sides of one matrix (factorize_multi / solve_multi): one factorisation (the
+single system's bank matrices, coarse map and coarsest inverse) serves every right-hand side;
+every column solved to tolerance and against a direct solve; the copies SpMV (matrix read once for
+all right-hand sides) gives the iterates of one SpMV launch per copy bit for bit; device arrays
+give the host result; each column matches a solve of that right-hand side alone."""
 import numpy as np
 import pytest
 
@@ -54,6 +55,43 @@ def test_multi_rhs(gpu, het12, precision, k):
         assert np.linalg.norm(X[:, c] - x1) <= 1e-6 * np.linalg.norm(x1)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_multi_rhs_one_factorisation(gpu, het12, precision):
+    """factorize_multi keeps the single system's factorisation (bank matrices, coarse map) once,
+    not per right-hand side; per copy the solve does the single system's arithmetic, so each
+    column takes the iterations of that right-hand side solved alone, and matches a direct solve."""
+    import scipy.sparse.linalg as spla
+
+    k = 5
+    B = np.random.default_rng(21).standard_normal((het12.n, k))
+    one = MASSolver()
+    one.set_parameters(_prm(precision))
+    one.set_coordinates(het12.coords)
+    one.factorize(het12.A)
+    s = MASSolver()
+    s.set_parameters(_prm(precision))
+    s.set_coordinates(het12.coords)
+    s.factorize_multi(het12.A, k)
+    li1, lik = one.level_info(), s.level_info()
+    assert li1["level_num"] >= 2  # (coarse segments 2.. and their restriction lists are exercised)
+    assert lik["bank_matrices"] == li1["bank_matrices"]
+    assert np.array_equal(s.coarse_map(), one.coarse_map())
+    M1 = one.bank_inverses(3)
+    assert np.linalg.norm(s.bank_inverses(3) - M1) <= 1e-12 * np.linalg.norm(M1)
+    assert "expand_copies" in s.get_info()["setup_ms"]
+    X = s.solve_multi(B)
+    lu = spla.splu(het12.A.tocsc())
+    for c in range(k):
+        info = s.batch_info(c)
+        assert info["solver_status"] == "Reach relative tolerance"
+        xd = lu.solve(B[:, c])
+        assert np.linalg.norm(X[:, c] - xd) <= 1e-6 * np.linalg.norm(xd)
+        x1 = np.zeros(het12.n)
+        one.solve(np.ascontiguousarray(B[:, c]), x1)
+        assert info["solver_iter"] == one.get_info()["solver_iter"], c
+        assert np.linalg.norm(X[:, c] - x1) <= 1e-7 * np.linalg.norm(x1)
+
+
 def test_multi_rhs_errors(gpu, het12):
     s = MASSolver()
     s.set_parameters(_prm("mixed"))
@@ -66,8 +104,8 @@ def test_multi_rhs_errors(gpu, het12):
 
 @pytest.mark.parametrize("coords", [True, False])
 def test_multi_rhs_replicated_device_build(gpu, het12, coords):
-    """factorize_multi builds system 0's matrix on the GPU once and copies it to the other right-hand
-    sides; the host build of every copy gives the same solutions (to the FP64 Galerkin's rounding)."""
+    """factorize_multi builds the matrix on the GPU (or on the host with device_setup off) once for
+    all right-hand sides; both builds give the same solutions (to the FP64 Galerkin's rounding)."""
     k = 4
     B = np.random.default_rng(7).standard_normal((het12.n, k))
     out = []
@@ -118,9 +156,8 @@ def test_multi_rhs_device_input(gpu, het12):
 
 @pytest.mark.parametrize("k", [3, 8])
 def test_multi_rhs_apply_cap(gpu, het12, k):
-    """apply_workgroups = 8: every apply workgroup walks many matrices, so the shared fine-bank
-    loop (a copy-0 bank stands for every copy's bank, the other copies' fine banks are stepped
-    over, coarse banks go one by one) runs its skip and step logic across many matrices.  The
+    """apply_workgroups = 8: every apply workgroup walks many stored matrices (each applied to
+    every copy's bank), so the grid-stride loop runs across many matrices and levels.  The
     iterates equal the uncapped launch's bit for bit, and every column matches a direct solve."""
     import scipy.sparse.linalg as spla
 
