@@ -252,6 +252,10 @@ namespace psmas
         const int *split_of = nullptr;
         const int4 *rcomb = nullptr;
         int node0 = 0;
+        int split_count = 0; ///< entries of split_of
+        /// Any readable device int (the apply kernels load from it unconditionally, in place of a
+        /// split or dense index that does not exist, and ignore the value).
+        const int *dummy = nullptr;
         // Dense coarsest solve y = Inv x, one wave per row; its nodes are coarsest-segment nodes
         // dense_seg + dense_ids[i].  Bank outputs of nodes with dense_index[n - dense_seg] >= 0
         // (n - dense_seg < dense_count) are left to it.

@@ -30,6 +30,27 @@ namespace psmas
         /// Input component d of multi-level node `node`: mr, or for a node whose restriction was
         /// cut into pieces the FP64 sum of its pieces in order (rounded like a stored value).  The
         /// two loads are independent, so only split nodes pay a further round trip.
+        /// Component d of split node si's input: the FP64 sum of its pieces in order (pieces loaded
+        /// eight at a time), rounded like a stored value.
+        template <int BD, typename T>
+        __device__ __forceinline__ T split_value(const ApplySide &sd, int si, int d)
+        {
+            const int4 c = sd.rcomb[si];
+            double acc = 0.0;
+            for (int k0 = 0; k0 < c.z; k0 += 8)
+            {
+                double pv[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    pv[j] = k0 + j < c.z ? sd.rpart[static_cast<size_t>(c.y + k0 + j) * BD + d] : 0.0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (k0 + j < c.z)
+                        acc += pv[j];
+            }
+            return static_cast<T>(acc);
+        }
+
         template <int BD, typename T>
         __device__ __forceinline__ T coarse_input(const ApplySide &sd, const T *__restrict__ mr, int node, int d)
         {
@@ -38,36 +59,56 @@ namespace psmas
             {
                 const int si = sd.split_of[node - sd.node0];
                 if (si >= 0)
-                {
-                    // Pieces loaded eight at a time, added in order.
-                    const int4 c = sd.rcomb[si];
-                    double acc = 0.0;
-                    for (int k0 = 0; k0 < c.z; k0 += 8)
-                    {
-                        double pv[8];
-#pragma unroll
-                        for (int j = 0; j < 8; ++j)
-                            pv[j] = k0 + j < c.z ? sd.rpart[static_cast<size_t>(c.y + k0 + j) * BD + d] : 0.0;
-#pragma unroll
-                        for (int j = 0; j < 8; ++j)
-                            if (k0 + j < c.z)
-                                acc += pv[j];
-                    }
-                    v = static_cast<T>(acc);
-                }
+                    v = split_value<BD, T>(sd, si, d);
             }
             return v;
         }
 
-        /// True when the dense coarsest solve owns node `node` (its bank output is not stored).
-        __device__ __forceinline__ bool dense_owned(const ApplySide &sd, int node)
+        /// The loads of a bank row's input, issued one matrix ahead and used only after the next
+        /// barrier: a use in the load phase would make the in-order memory counter wait for the
+        /// matrix tiles in flight as well.  Every load is unconditional (indices clamped, absent
+        /// arrays replaced by sd.dummy), so the compiler can count the loads in flight and wait
+        /// for exactly the earlier matrix's.  split: node >= node0 of a launch with split nodes
+        /// (its split index then decides the input).
+        template <typename T>
+        struct RawInput
         {
-            if (sd.dense_index == nullptr)
-                return false;
+            T x = T(0);
+            int si = -1;
+            bool split = false;
+        };
+
+        template <int BD, typename T>
+        __device__ __forceinline__ RawInput<T> raw_input(const ApplySide &sd, const T *__restrict__ mr, int node, int d)
+        {
+            RawInput<T> in;
+            in.x = mr[static_cast<size_t>(node) * BD + d];
+            const int rel = node - sd.node0;
+            const bool has = sd.split_of != nullptr;
+            in.split = has && rel >= 0;
+            const int last = has ? sd.split_count - 1 : 0;
+            in.si = (has ? sd.split_of : sd.dummy)[min(max(rel, 0), last)];
+            return in;
+        }
+
+        template <int BD, typename T>
+        __device__ __forceinline__ T input_value(const ApplySide &sd, const RawInput<T> &in, int d)
+        {
+            return (in.split && in.si >= 0) ? split_value<BD, T>(sd, in.si, d) : in.x;
+        }
+
+        /// Whether the dense coarsest solve owns node `node` (its bank output is then not stored),
+        /// in two halves: dense_probe issues the (unconditional, clamped) load, the caller tests
+        /// `in && dix >= 0` after the next barrier.
+        __device__ __forceinline__ void dense_probe(const ApplySide &sd, int node, bool &in, int &dix)
+        {
+            const bool has = sd.dense_index != nullptr;
             int rel = node - sd.dense_seg;
             if (sd.dense_copies > 1 && rel >= 0 && rel < sd.dense_copies * sd.dense_stride)
-                rel %= sd.dense_stride; // the copies' segments repeat copy 0's
-            return rel >= 0 && rel < sd.dense_count && sd.dense_index[rel] >= 0;
+                rel %= sd.dense_stride;
+            in = has && rel >= 0 && rel < sd.dense_count;
+            const int last = has ? sd.dense_count - 1 : 0;
+            dix = (has ? sd.dense_index : sd.dummy)[min(max(rel, 0), last)];
         }
 
         /// Stored matrix of bank m (CopyLayout).
@@ -424,8 +465,11 @@ namespace psmas
             using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
 
             __shared__ __attribute__((aligned(16))) __half sF[N * LDW];
-            __shared__ __attribute__((aligned(16))) float sx[KC][N];
-            __shared__ float part[KC][N];
+            // Copy rows 4 floats apart in bank order (N is a multiple of 32): the MFMA path reads
+            // and writes one row per lane group, which would otherwise hit the same banks.
+            constexpr int SXW = KC > 1 ? N + 4 : N;
+            __shared__ __attribute__((aligned(16))) float sx[KC][SXW];
+            __shared__ float part[KC][SXW];
 
             const int nc = KC > 1 ? cl.copies : 1;
             auto skip = [&](int m) {
@@ -490,37 +534,47 @@ namespace psmas
                 sF[i * LDW + i] = __float2half_rn(1.0f); // never overwritten below
 
             // A matrix in flight: its tile in registers, and for thread tid < N (the one that also
-            // stores row tid) the scaled input x_tid, the row's scale and whether the row's output
-            // is stored, so that nothing global is read between the barriers.
+            // stores row tid) the raw loads of the row's scale, each copy's input and the dense
+            // ownership of its node (RawInput), combined after the next barrier, so that nothing
+            // global is read or waited for between the barriers.
             struct Row
             {
-                float x[KC] = {};
+                RawInput<float> in[KC];
                 float d = 0.0f;
-                bool store = false;
+                bool dense = false;
+                int dix = -1;
                 int bank = 0, stride = 0; ///< copy 0's bank of the item, bank stride between copies
+                __device__ bool store() const { return !(dense && dix >= 0); }
             };
+            // The row loads go first and the tile loads last, so that waiting for this matrix's
+            // row values later leaves the next matrix's tile loads in flight.
             auto load = [&](u32x4(&reg)[PER], Row &rw, int mm) {
+                const int r = tid % N; // (threads past N repeat row tid - N's loads, unused)
+                if constexpr (KC > 1)
+                    bank_of_stored(cl, mm, rw.bank, rw.stride);
+                else
+                    rw.bank = mm;
+                rw.d = dscale[static_cast<size_t>(mm) * N + r];
+#pragma unroll
+                for (int c = 0; c < KC; ++c)
+                    rw.in[c] = raw_input<BD, float>(sd, mr, (rw.bank + (c < nc ? c : nc - 1) * rw.stride) * 32 + r / BD,
+                                                    r % BD);
+                dense_probe(sd, rw.bank * 32 + r / BD, rw.dense, rw.dix);
                 const u32x4 *src = reinterpret_cast<const u32x4 *>(mats + static_cast<size_t>(mm) * LEN);
 #pragma unroll
                 for (int k = 0; k < PER; ++k)
                     reg[k] = __builtin_nontemporal_load(src + tid + k * BLOCK);
-                if (tid < N)
-                {
-                    rw.d = dscale[static_cast<size_t>(mm) * N + tid];
-                    if constexpr (KC > 1)
-                        bank_of_stored(cl, mm, rw.bank, rw.stride);
-                    else
-                        rw.bank = mm;
-#pragma unroll
-                    for (int c = 0; c < KC; ++c)
-                        if (c < nc)
-                            rw.x[c] = coarse_input<BD, float>(sd, mr, (rw.bank + c * rw.stride) * 32 + tid / BD,
-                                                              tid % BD) *
-                                      rw.d;
-                    rw.store = !dense_owned(sd, rw.bank * 32 + tid / BD);
-                }
             };
-            auto unpack = [&](const u32x4(&reg)[PER], const Row &rw) {
+            // What the compute phase needs of a matrix's row loads, taken out in unpack (which
+            // waits for them anyway), so that the next load() may reuse the Row's registers
+            // without waiting for anything.
+            struct Out
+            {
+                float d;
+                bool store;
+                int bank, stride;
+            };
+            auto unpack = [&](const u32x4(&reg)[PER], const Row &rw) -> Out {
 #pragma unroll
                 for (int k = 0; k < PER; ++k)
                 {
@@ -563,12 +617,14 @@ namespace psmas
                 {
 #pragma unroll
                     for (int c = 0; c < KC; ++c)
-                        sx[c][tid] = rw.x[c];
+                        sx[c][tid] = c < nc ? input_value<BD, float>(sd, rw.in[c], tid % BD) * rw.d : 0.0f;
                 }
+                const Out o{rw.d, rw.store(), rw.bank, rw.stride};
                 __syncthreads();
+                return o;
             };
             const int row = tid % N, hf = tid / N;
-            auto compute = [&](const Row &rw) {
+            auto compute_valu = [&](const Out &rw) {
                 const __half *arow = sF + row * LDW + hf * HALF;
                 float acc[KC];
 #pragma unroll
@@ -580,11 +636,11 @@ namespace psmas
                     const u32x4 av = *reinterpret_cast<const u32x4 *>(arow + t);
                     __half h[8];
                     __builtin_memcpy(h, &av, 16);
+                    // Every accumulator runs (copies past nc have zero inputs): a loop without an
+                    // exit unrolls fully and its LDS reads issue ahead of the FMAs.
 #pragma unroll
                     for (int c = 0; c < KC; ++c)
                     {
-                        if (c >= nc)
-                            break;
                         const float *xs = sx[c] + hf * HALF;
                         const float4 x0 = *reinterpret_cast<const float4 *>(xs + t);
                         const float4 x1 = *reinterpret_cast<const float4 *>(xs + t + 4);
@@ -617,6 +673,81 @@ namespace psmas
                 }
                 // part[] is next written after the following unpack's barrier
             };
+            // KC = 8: Y = S X on MFMA (v_mfma_f32_16x16x16f16, FP32 accumulation).  Wave w owns
+            // rows 32w .. 32w + 31 (two 16-row tiles) over all N / 16 k-steps; the B operand is
+            // X (N x 16, columns past the copies zero) in FP16 as hi + lo parts of x 2^-e_j, e_j
+            // the exponent of copy j's largest |x| (exact power-of-two scaling), so the two
+            // products carry about 22 bits of x.  Lane l: A[i = l % 16][k = 4 (l / 16) + v],
+            // B[k = 4 (l / 16) + v][j = l % 16], D[i = 4 (l / 16) + v][j = l % 16].
+            auto compute_mfma = [&](const Out &rw) {
+                using h4 = _Float16 __attribute__((ext_vector_type(4)));
+                using f4 = float __attribute__((ext_vector_type(4)));
+                const int lane = tid & 63, w = tid >> 6, q = lane >> 4, j = lane & 15;
+                // Largest |x| of copy lane % 8 over the rows (lanes l, l + 8, ... share a copy).
+                float mx = 0.0f;
+                for (int i = lane >> 3; i < N; i += 8)
+                    mx = fmaxf(mx, fabsf(sx[lane & 7][i]));
+                mx = fmaxf(mx, __shfl_xor(mx, 8));
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                mx = fmaxf(mx, __shfl_xor(mx, 32));
+                int ex = 0;
+                (void)frexpf(mx, &ex);
+                const int e = __shfl(ex, j & 7); // copy j's exponent (lanes with j >= 8: unused)
+                const bool jv = j < nc;
+                f4 acc[2];
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < N / 16; ++ks)
+                {
+                    const int k = 16 * ks + 4 * q;
+                    h4 bh = h4{0, 0, 0, 0}, bl = h4{0, 0, 0, 0};
+                    if (jv)
+                    {
+                        const float4 xv = *reinterpret_cast<const float4 *>(sx[j] + k);
+                        const float xs[4] = {ldexpf(xv.x, -e), ldexpf(xv.y, -e), ldexpf(xv.z, -e), ldexpf(xv.w, -e)};
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                        {
+                            const _Float16 hi = static_cast<_Float16>(xs[v]);
+                            bh[v] = hi;
+                            bl[v] = static_cast<_Float16>(xs[v] - static_cast<float>(hi));
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                    {
+                        const int r = 32 * w + 16 * t + j;
+                        const h4 a = *reinterpret_cast<const h4 *>(sF + r * LDW + k);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(a, bh, acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(a, bl, acc[t], 0, 0, 0);
+                    }
+                }
+                if (jv)
+                {
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            part[j][32 * w + 16 * t + 4 * q + v] = ldexpf(acc[t][v], e);
+                }
+                __syncthreads(); // also: sF and sx may be rewritten after this point
+                if (tid < N && rw.store)
+                {
+#pragma unroll
+                    for (int c = 0; c < KC; ++c)
+                        if (c < nc)
+                            mz[static_cast<size_t>(rw.bank + c * rw.stride) * N + tid] = part[c][tid] * rw.d;
+                }
+                // part[] is next written after the following unpack's barrier
+            };
+            auto compute = [&](const Out &rw) {
+                if constexpr (KC == 8)
+                    compute_mfma(rw);
+                else
+                    compute_valu(rw);
+            };
 
             // Invariant: regA holds matrix cur, regB matrix nxt (when nxt < end).
             u32x4 regA[PER], regB[PER];
@@ -628,8 +759,7 @@ namespace psmas
             while (true)
             {
                 int nn = nxt < end ? step(nxt) : end;
-                unpack(regA, ra);
-                Row now = ra;
+                Out now = unpack(regA, ra);
                 if (nn < end)
                     load(regA, ra, nn);
                 compute(now);
@@ -638,8 +768,7 @@ namespace psmas
                 if (cur >= end)
                     break;
                 nn = nxt < end ? step(nxt) : end;
-                unpack(regB, rb);
-                now = rb;
+                now = unpack(regB, rb);
                 if (nn < end)
                     load(regB, rb, nn);
                 compute(now);
@@ -700,8 +829,15 @@ namespace psmas
             if (m >= end)
                 return;
             u32x4 reg[PER];
-            TA xr = TA(0);
+            // Row tid's raw input and dense probe (used after the next barrier, RawInput); the
+            // row loads go before the tile loads.
+            RawInput<T> rin;
+            bool dn = false;
+            int dx = -1;
             auto load = [&](int mm) {
+                const int r = threadIdx.x % N; // (threads past N repeat row tid - N's loads, unused)
+                rin = raw_input<BD, T>(sd, mr, mm * 32 + r / BD, r % BD);
+                dense_probe(sd, mm * 32 + r / BD, dn, dx);
                 // Several right-hand sides of one factorisation: bank mm's matrix is stored once.
                 const int sm = cl.copies > 1 ? stored_of_bank(cl, mm) : mm;
                 const u32x4 *src = reinterpret_cast<const u32x4 *>(mats + static_cast<size_t>(sm) * L);
@@ -712,8 +848,6 @@ namespace psmas
                     if (i < CH)
                         reg[k] = __builtin_nontemporal_load(src + i);
                 }
-                if (threadIdx.x < N)
-                    xr = coarse_input<BD, T>(sd, mr, mm * 32 + threadIdx.x / BD, threadIdx.x % BD);
             };
             load(m);
             const int row = threadIdx.x % N;
@@ -731,7 +865,8 @@ namespace psmas
                         dst[i] = reg[k];
                 }
                 if (threadIdx.x < N)
-                    sx[threadIdx.x] = xr;
+                    sx[threadIdx.x] = input_value<BD, T>(sd, rin, threadIdx.x % BD);
+                const bool store_row = !(dn && dx >= 0); // (row tid's, for the threads that store)
                 __syncthreads();
                 int next = m + nwg;
                 while (next < end && skip(next))
@@ -753,7 +888,7 @@ namespace psmas
                 if (half == 1 && threadIdx.x < 2 * N)
                     part[row] = acc;
                 __syncthreads();
-                if (half == 0 && !dense_owned(sd, m * 32 + row / BD))
+                if (half == 0 && store_row) // (half == 0: row == threadIdx.x)
                     mz[static_cast<size_t>(m) * N + row] = acc + part[row];
                 __syncthreads(); // sA, sx and part are rewritten for the next matrix
                 m = next;
@@ -1446,6 +1581,8 @@ namespace psmas
         if (count <= 0 && side.wgs <= 0)
             return;
         require_device_ptrs("mas_apply", {mr, mz});
+        if (count > 0)
+            require_device_ptrs("mas_apply (index placeholder)", {side.dummy});
         if (count > 0)
             require_device_ptrs("mas_apply (matrices)",
                                 {mats, std::is_same<TM, __half>::value ? static_cast<const void *>(dscale)

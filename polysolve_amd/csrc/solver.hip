@@ -1542,6 +1542,7 @@ namespace psmas
     ApplySide Solver::restrict_side(const void *seg1, void *out, int block, bool lattice_gather) const
     {
         ApplySide sd;
+        sd.dummy = d_blk_begin_.data();
         sd.apply_cap = params_.apply_workgroups;
         if (nrsmall_ + nrbig_ > 0)
         {
@@ -1567,10 +1568,12 @@ namespace psmas
     ApplySide Solver::coarse_side(int block) const
     {
         ApplySide sd;
+        sd.dummy = d_blk_begin_.data();
         sd.apply_cap = params_.apply_workgroups;
         if (nrcomb_ > 0)
         {
             sd.split_of = d_split_of_.data();
+            sd.split_count = static_cast<int>(d_split_of_.size());
             sd.rcomb = d_rcomb_.data();
             sd.rpart = const_cast<double *>(d_rpart_.data());
             sd.node0 = lev_off_[2];

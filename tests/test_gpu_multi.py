@@ -56,13 +56,16 @@ def test_multi_rhs(gpu, het12, precision, k):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_multi_rhs_one_factorisation(gpu, het12, precision):
+@pytest.mark.parametrize("k", [4, 6])
+def test_multi_rhs_one_factorisation(gpu, het12, precision, k):
     """factorize_multi keeps the single system's factorisation (bank matrices, coarse map) once,
-    not per right-hand side; per copy the solve does the single system's arithmetic, so each
-    column takes the iterations of that right-hand side solved alone, and matches a direct solve."""
+    not per right-hand side.  Up to 4 right-hand sides the solve does the single system's
+    arithmetic per copy, so each column takes exactly the iterations of that right-hand side
+    solved alone; from 5 on the mixed path's bank products run on MFMA (inputs split into FP16
+    hi + lo parts, FP32 accumulation in another order), within one check interval of them.  Every
+    column matches a direct solve."""
     import scipy.sparse.linalg as spla
 
-    k = 5
     B = np.random.default_rng(21).standard_normal((het12.n, k))
     one = MASSolver()
     one.set_parameters(_prm(precision))
@@ -88,8 +91,12 @@ def test_multi_rhs_one_factorisation(gpu, het12, precision):
         assert np.linalg.norm(X[:, c] - xd) <= 1e-6 * np.linalg.norm(xd)
         x1 = np.zeros(het12.n)
         one.solve(np.ascontiguousarray(B[:, c]), x1)
-        assert info["solver_iter"] == one.get_info()["solver_iter"], c
-        assert np.linalg.norm(X[:, c] - x1) <= 1e-7 * np.linalg.norm(x1)
+        if k <= 4 or precision == "fp64":
+            assert info["solver_iter"] == one.get_info()["solver_iter"], c
+            assert np.linalg.norm(X[:, c] - x1) <= 1e-7 * np.linalg.norm(x1)
+        else:
+            assert abs(info["solver_iter"] - one.get_info()["solver_iter"]) <= 10, c
+            assert np.linalg.norm(X[:, c] - x1) <= 1e-6 * np.linalg.norm(x1)
 
 
 def test_multi_rhs_errors(gpu, het12):
