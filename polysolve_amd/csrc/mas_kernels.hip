@@ -1582,11 +1582,11 @@ namespace psmas
             return;
         require_device_ptrs("mas_apply", {mr, mz});
         if (count > 0)
-            require_device_ptrs("mas_apply (index placeholder)", {side.dummy});
-        if (count > 0)
             require_device_ptrs("mas_apply (matrices)",
                                 {mats, std::is_same<TM, __half>::value ? static_cast<const void *>(dscale)
                                                                        : static_cast<const void *>(mats)});
+        if (count > 0)
+            require_device_ptrs("mas_apply (index placeholder)", {side.dummy});
         // Apply workgroups: as many as are resident at once beside the side workgroups (registers
         // and LDS decide); each walks its matrices grid-stride.  Side workgroups come first so the
         // dispatcher starts them at once.
