@@ -536,9 +536,9 @@ def main() -> int:
     log(f"  values-only refactorisation {refactor_ms:.1f} ms ({refactor_info['factorize_mode']}), "
         f"phases {refactor_info['setup_ms']}")
 
-    # Several right-hand sides of this matrix solved together (mixed path, factorize_multi): the
-    # SpMV and the fine-bank apply read the matrix once for all of them.  Compared with solving
-    # the same right-hand sides one by one with the same parameters.
+    # Several right-hand sides of this matrix solved together (mixed path, factorize_multi): one
+    # factorisation, whose SpMV and bank applies read each matrix once for all of them.  Compared
+    # with solving the same right-hand sides one by one with the same parameters.
     multi = None
     if args.multi_rhs > 1:
         k = args.multi_rhs
