@@ -77,6 +77,25 @@ def test_lift_equals_sparse_product_form():
     assert np.allclose(g, dense_form, rtol=1e-14, atol=1e-14 * np.abs(f).max())
 
 
+def test_reference_prefactorized_sequence_lifts_nothing():
+    """The reference's call sequence: prefactorize (FEMSolver.cpp:268-308) overwrites the caller's
+    A with the eliminated A′, and dirichlet_solve_prefactorized (FEMSolver.cpp:317-342) then lifts
+    with the A it is given, i.e. A′, whose free rows have no fixed columns: g = f exactly, and the
+    solve ignores the couplings to the prescribed values (u differs from the constrained
+    solution).  Lifting with the A before elimination, as psmas_dirichlet_solve does and the
+    one-shot dirichlet_solve (FEMSolver.cpp:97-137) does, gives the constrained solution."""
+    p = _bare(3)
+    nodes = _fixed_set(3)
+    f = np.random.default_rng(5).standard_normal(p.n)
+    At = DO.eliminate(p.A, nodes)
+    g_seq = DO.lift(At, nodes, f)
+    assert np.array_equal(g_seq, f)
+    u_seq = sla.spsolve(sp.csc_matrix(At), g_seq)
+    ref = DO.reduced_solution(p.A, nodes, f)
+    assert np.linalg.norm(u_seq - ref) > 1e-3 * np.linalg.norm(ref)
+    assert _solution_error(p.A, nodes, f, DO.lift(p.A, nodes, f)) <= 1e-10
+
+
 def test_missing_diagonal_gets_explicit_zero():
     # 4x4: column 2 has no diagonal; dof 1 fixed.
     dense = np.array([[4.0, 1.0, 2.0, 0.0],

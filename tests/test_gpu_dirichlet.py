@@ -82,6 +82,29 @@ def test_lift_is_bit_exact_and_solution_matches_reduced_solve(gpu, bare, nodes):
     assert np.abs(u[nodes] - f[nodes]).max() <= 1e-8 * np.abs(u).max()
 
 
+def test_reference_prefactorized_sequence(gpu, bare, nodes):
+    """The reference's prefactorize + dirichlet_solve_prefactorized sequence lifts with the
+    overwritten A′, so g = f (tests/test_dirichlet_oracle.py); here that sequence is the Dirichlet
+    factorisation followed by a plain solve of f, which gives A′⁻¹ f, while psmas_dirichlet_solve
+    lifts with A and gives the constrained solution."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    s = _solver(bare)
+    s.dirichlet_factorize(bare.A, nodes)
+    f = _f(bare, nodes)
+    At = DO.eliminate(bare.A, nodes)
+    assert np.array_equal(DO.lift(At, nodes, f), f)
+    u_seq = np.zeros(bare.n)
+    s.solve(f.copy(), u_seq)
+    assert _rel(u_seq, sla.spsolve(sp.csc_matrix(At), f)) <= 1e-6
+    g = f.copy()
+    u = np.zeros(bare.n)
+    s.dirichlet_solve(g, u)
+    assert _rel(u, DO.reduced_solution(bare.A, nodes, f)) <= 1e-6
+    assert _rel(u_seq, u) > 1e-3
+
+
 @pytest.mark.parametrize("device_setup", [True, False])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_newton_steps_refresh_values_only(gpu, bare, nodes, precision, device_setup):
