@@ -257,6 +257,16 @@ int psmas_parse_parameters(const char *json, char *buf, size_t cap, size_t *need
 
 /* ---- Introspection used by the parity tests ---------------------------------------------- */
 
+/* The block graph of a CSC matrix with block size bd (no device needed): the undirected pairs of
+ * distinct block nodes joined by a stored entry, in both directions, each row's neighbours in
+ * ascending order (ptr[nb+1], adj[edges]), with the L1 weight sum of |a_ij| over the entries of
+ * both blocks (w[edges]).  The graph and weights of build_adjacency_from_csc
+ * (BSRAdjacency.cu:184-460, keys and weights :60-84) before its quantisation (:131-162); the host
+ * graph aggregation matches on these weights.  *nedges receives the edge count; a call with
+ * edge_cap smaller than it writes ptr only and returns PSMAS_ERROR_BUFFER_TOO_SMALL. */
+int psmas_block_graph(int n, long long nnz, const int *colptr, const int *rowidx, const double *vals, int bd,
+                      int *ptr, int ptr_cap, int *adj, double *w, long long edge_cap, long long *nedges);
+
 /* info[0..11] = level_num, cco_nums[4], parts (even), padded nodes, bank matrices, nnz blocks,
  * SELL slots, block nodes, n.  cap >= 12. */
 int psmas_get_level_info(psmas_handle h, int *info, int cap);

@@ -85,6 +85,7 @@ SIGNATURES = {
                              _pd]),
     "psmas_gemm_bf16": (_i, [_i, _i, _i, _i, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "psmas_parse_parameters": (_i, [_cp, C.c_char_p, _sz, C.POINTER(_sz)]),
+    "psmas_block_graph": (_i, [_i, _ll, _pi, _pi, _pd, _i, _pi, _i, _pi, _pd, _ll, C.POINTER(_ll)]),
     "psmas_get_level_info": (_i, [_vp, _pi, _i]),
     "psmas_get_partition": (_i, [_vp, _pi, _pi, _i, _pi, _i]),
     "psmas_get_coarse_map": (_i, [_vp, _pi, _ll]),

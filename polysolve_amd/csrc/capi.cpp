@@ -2,8 +2,10 @@
 // into status codes (reference exception classes: CudaUtils.cuh:33-80, MASSolver.cu).
 #include "../../include/psmas.h"
 
+#include "host_setup.hpp"
 #include "solver.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <sstream>
@@ -395,6 +397,32 @@ extern "C"
             return rc;
         }
         return write_string(psmas::params_to_json(p), buf, cap, needed);
+    }
+
+    int psmas_block_graph(int n, long long nnz, const int *colptr, const int *rowidx, const double *vals, int bd,
+                          int *ptr, int ptr_cap, int *adj, double *w, long long edge_cap, long long *nedges)
+    {
+        bool fits = true;
+        const int rc = guarded(g_create_error, [&] { // (handle-less: psmas_last_error(NULL) reports it)
+            if (bd < 1 || bd > 3)
+                throw psmas::InvalidArgumentError("[MAS] invalid block size (1, 2 or 3).");
+            if (ptr == nullptr || nedges == nullptr)
+                throw psmas::InvalidArgumentError("[MAS] null output array.");
+            const psmas::CscView A{n, nnz, colptr, rowidx, vals};
+            psmas::validate_csc(A);
+            const psmas::BlockGraph g = psmas::build_block_graph(A, bd);
+            if (ptr_cap < g.nb + 1)
+                throw psmas::InvalidArgumentError("[MAS] block graph pointer buffer too small.");
+            std::copy(g.ptr.begin(), g.ptr.end(), ptr);
+            *nedges = static_cast<long long>(g.adj.size());
+            fits = edge_cap >= *nedges && (*nedges == 0 || (adj != nullptr && w != nullptr));
+            if (fits)
+            {
+                std::copy(g.adj.begin(), g.adj.end(), adj);
+                std::copy(g.w.begin(), g.w.end(), w);
+            }
+        });
+        return rc != PSMAS_OK ? rc : (fits ? PSMAS_OK : PSMAS_ERROR_BUFFER_TOO_SMALL);
     }
 
     int psmas_get_level_info(psmas_handle h, int *info, int cap)

@@ -443,6 +443,26 @@ def parse_parameters(params) -> dict:
     return _json.loads(s)
 
 
+def block_graph(A, block_dim: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """The block graph the host aggregation partitions (psmas_block_graph): CSR (ptr, adj) of the
+    distinct block pairs joined by a stored entry, both directions, neighbours ascending, and the
+    L1 weight of every edge (sum of |a| over both blocks)."""
+    n, colptr, rowidx, vals = _as_csc(A)
+    nb = -(-n // int(block_dim))
+    ptr = np.zeros(nb + 1, dtype=np.int32)
+    ne = C.c_longlong(0)
+    lib = N.lib()
+    args = (n, rowidx.size, _ptr(colptr, C.c_int), _ptr(rowidx, C.c_int), _ptr(vals, C.c_double), int(block_dim),
+            _ptr(ptr, C.c_int), ptr.size)
+    rc = lib.psmas_block_graph(*args, None, None, 0, C.byref(ne))
+    if rc not in (N.PSMAS_OK, N.PSMAS_ERROR_BUFFER_TOO_SMALL):
+        _check(rc, None)
+    adj = np.zeros(max(ne.value, 1), dtype=np.int32)
+    w = np.zeros(max(ne.value, 1), dtype=np.float64)
+    _check(lib.psmas_block_graph(*args, _ptr(adj, C.c_int), _ptr(w, C.c_double), adj.size, C.byref(ne)), None)
+    return ptr, adj[:ne.value], w[:ne.value]
+
+
 def device_count() -> int:
     return int(N.lib().psmas_device_count())
 
