@@ -227,6 +227,12 @@ int psmas_device_free(void *ptr);
 int psmas_memcpy(void *dst, const void *src, size_t bytes, int kind);
 int psmas_device_synchronize(void);
 
+/* Streaming-read probe (no counterpart in the reference; measurement only): reads `bytes` of
+ * device memory once per repetition with 16-byte nontemporal loads and writes the best rate in
+ * GB/s to *gbps — the read ceiling the SpMV and the bank apply are compared with, beside the
+ * 8 TB/s specification.  bytes >= 1 MiB. */
+int psmas_read_bandwidth(long long bytes, int reps, double *gbps);
+
 /* ---- Learned coarse-correction network (no counterpart in the reference) ------------------ */
 
 /* MLP d_in -> hidden -> hidden -> d_out (ReLU) with an optional linear skip layer S (skip != 0),

@@ -72,6 +72,7 @@ SIGNATURES = {
     "psmas_device_free": (_i, [_vp]),
     "psmas_memcpy": (_i, [_vp, _vp, _sz, _i]),
     "psmas_device_synchronize": (_i, []),
+    "psmas_read_bandwidth": (_i, [_ll, _i, C.POINTER(C.c_double)]),
     "psmas_mlp_create": (_i, [_i, _i, _i, _i, C.c_uint, C.POINTER(_vp)]),
     "psmas_mlp_destroy": (_i, [_vp]),
     "psmas_mlp_last_error": (_cp, [_vp]),

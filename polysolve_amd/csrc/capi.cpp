@@ -3,6 +3,7 @@
 #include "../../include/psmas.h"
 
 #include "host_setup.hpp"
+#include "kernels.hpp"
 #include "solver.hpp"
 
 #include <algorithm>
@@ -380,6 +381,15 @@ extern "C"
     int psmas_device_synchronize(void)
     {
         return guarded(g_create_error, [&] { PSMAS_HIP(hipDeviceSynchronize()); });
+    }
+
+    int psmas_read_bandwidth(long long bytes, int reps, double *gbps)
+    {
+        return guarded(g_create_error, [&] {
+            if (gbps == nullptr)
+                throw psmas::InvalidArgumentError("[MAS] null output.");
+            *gbps = psmas::read_bandwidth_probe(bytes, reps);
+        });
     }
 
     int psmas_parse_parameters(const char *json, char *buf, size_t cap, size_t *needed)

@@ -313,6 +313,10 @@ namespace psmas
                           const int *bank_sys, const int *active, hipStream_t s, const ApplySide &side = ApplySide{},
                           const CopyLayout &cl = CopyLayout{});
 
+    /// Best streaming-read rate (GB/s) over `reps` passes of `bytes` of device memory
+    /// (16-byte nontemporal loads, grid-stride); allocates and frees its buffer.
+    double read_bandwidth_probe(long long bytes, int reps);
+
     /// Elements of the vector type the FP16 apply kernel's side workgroups can stage in LDS.
     int apply_h16_stage_floats(int bd);
 

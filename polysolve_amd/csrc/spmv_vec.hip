@@ -174,6 +174,23 @@ namespace psmas
             store_partial<SPMV_BLOCK>(dot, red.pap, blk);
         }
 
+        /// Streaming-read probe: every 16-byte chunk once (nontemporal); the sum is stored only
+        /// under an impossible condition, which keeps the loads.
+        typedef float probe_f4 __attribute__((ext_vector_type(4)));
+        __global__ __launch_bounds__(256) void k_read_probe(const probe_f4 *__restrict__ a, long long n4,
+                                                            float *__restrict__ sink)
+        {
+            float acc = 0.0f;
+            const long long stride = static_cast<long long>(gridDim.x) * 256;
+            for (long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; i < n4; i += stride)
+            {
+                const probe_f4 v = __builtin_nontemporal_load(a + i);
+                acc += v.x + v.y + v.z + v.w;
+            }
+            if (acc == -1.2345e-37f)
+                sink[0] = acc;
+        }
+
         /// beta of the search-direction update for system sys (the same arithmetic as k_update_p).
         template <typename T>
         __device__ __forceinline__ T direction_beta(const Reductions &red, int sys, const SysMap &sm,
@@ -1093,4 +1110,41 @@ namespace psmas
     template void launch_fill<unsigned>(long long, unsigned *, unsigned, hipStream_t);
     template void launch_gather_in<double>(int, int, int, const int *, const double *, double *, hipStream_t);
     template void launch_scatter_out<double>(int, int, int, const int *, const double *, double *, hipStream_t);
+
+    double read_bandwidth_probe(long long bytes, int reps)
+    {
+        if (bytes < (1LL << 20) || reps < 1)
+            throw InvalidArgumentError("[MAS] read probe: at least 1 MiB and one repetition.");
+        const long long n4 = bytes / 16;
+        DeviceBuffer<float4> buf(static_cast<size_t>(n4), "MAS read probe");
+        DeviceBuffer<float> sink(1, "MAS read probe");
+        PSMAS_HIP(hipMemset(buf.data(), 0, static_cast<size_t>(n4) * 16));
+        struct Events // destroyed on every exit, exceptions included
+        {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~Events()
+            {
+                for (hipEvent_t x : e)
+                    if (x != nullptr)
+                        (void)hipEventDestroy(x);
+            }
+        } ev;
+        PSMAS_HIP(hipEventCreate(&ev.e[0]));
+        PSMAS_HIP(hipEventCreate(&ev.e[1]));
+        const probe_f4 *a = reinterpret_cast<const probe_f4 *>(buf.data());
+        float best = 1e30f;
+        for (int r = 0; r <= reps; ++r) // (the first pass warms up)
+        {
+            PSMAS_HIP(hipEventRecord(ev.e[0], nullptr));
+            hipLaunchKernelGGL(k_read_probe, dim3(2048), dim3(256), 0, nullptr, a, n4, sink.data());
+            PSMAS_LAUNCH_CHECK();
+            PSMAS_HIP(hipEventRecord(ev.e[1], nullptr));
+            PSMAS_HIP(hipEventSynchronize(ev.e[1]));
+            float ms = 0.0f;
+            PSMAS_HIP(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+            if (r > 0 && ms < best)
+                best = ms;
+        }
+        return static_cast<double>(n4) * 16.0 / (static_cast<double>(best) * 1e-3) / 1e9;
+    }
 } // namespace psmas
