@@ -35,17 +35,18 @@ namespace psmas
     /// half rows and rows, so every lane holds its row's sum; the four row sums are then added
     /// as (r0 + r1) + (r2 + r3).  A fixed order, so every wave gets the same bits for the same
     /// values; no LDS traffic (a shuffle through ds_bpermute would be an LDS round trip per step).
-    __device__ __forceinline__ double wave_sum(double v)
+    /// Sum over each 16-lane row of the wave, in every lane of the row (all lanes active).
+    __device__ __forceinline__ double row_sum16(double v)
     {
         v += dpp_move<0xB1>(v);  // quad_perm [1, 0, 3, 2]
         v += dpp_move<0x4E>(v);  // quad_perm [2, 3, 0, 1]
         v += dpp_move<0x141>(v); // row_half_mirror
         v += dpp_move<0x140>(v); // row_mirror
-        return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+        return v;
     }
 
-    /// FP32 version of wave_sum (same DPP row moves and row order).
-    __device__ __forceinline__ float wave_sum(float v)
+    /// FP32 version of row_sum16 (same DPP row moves).
+    __device__ __forceinline__ float row_sum16(float v)
     {
         auto mv = [](float x, auto ctrl) {
             return __int_as_float(
@@ -55,6 +56,19 @@ namespace psmas
         v += mv(v, std::integral_constant<int, 0x4E>());  // quad_perm [2, 3, 0, 1]
         v += mv(v, std::integral_constant<int, 0x141>()); // row_half_mirror
         v += mv(v, std::integral_constant<int, 0x140>()); // row_mirror
+        return v;
+    }
+
+    __device__ __forceinline__ double wave_sum(double v)
+    {
+        v = row_sum16(v);
+        return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+    }
+
+    /// FP32 version of wave_sum (same DPP row moves and row order).
+    __device__ __forceinline__ float wave_sum(float v)
+    {
+        v = row_sum16(v);
         auto rl = [&](int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); };
         return (rl(0) + rl(16)) + (rl(32) + rl(48));
     }

@@ -20,14 +20,16 @@ namespace psmas
 
     /// Block path of the lattice restriction: per slot the lattice cell {base point, fx, fy, fz}
     /// (base < 0: not restricted) and the LDS position of each of its 8 corner terms (16 bits
-    /// each); per block the number of distinct targets and the entry ranges of each target.
+    /// each); per block the number of distinct targets nt and, at a fixed stride of LCC_TMAX,
+    /// the block-local end of each target's term range (target t covers [tend[t-1], tend[t]),
+    /// tend[-1] = 0; entries past nt repeat the block's term count), so that a thread's entry is
+    /// addressed without a prior load.
     struct LatticeBlocks
     {
         const int4 *cell = nullptr;
         const uint4 *pos = nullptr;
         const int *block_nt = nullptr;
-        const int *tbase = nullptr;
-        const int *tptr = nullptr;
+        const int *tend = nullptr; ///< [block][LCC_TMAX]
         double *partial = nullptr; ///< [block][LCC_TMAX][bd]
     };
 

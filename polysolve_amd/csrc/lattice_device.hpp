@@ -7,28 +7,49 @@
 
 namespace psmas
 {
+    /// What a slot's thread needs of the block path, loaded by the caller before anything else
+    /// so that the loads overlap its own (no address depends on a loaded value).
+    struct LatticeSlot
+    {
+        int nt = 0;
+        int4 c{-1, 0, 0, 0};
+        uint4 P{0u, 0u, 0u, 0u};
+        int tend = 0; ///< entry threadIdx.x % LCC_TMAX of the block's target ends
+    };
+    __device__ __forceinline__ LatticeSlot lattice_slot(const LatticeBlocks &lb, int b)
+    {
+        LatticeSlot s;
+        const size_t slot = static_cast<size_t>(b) * 256 + threadIdx.x;
+        s.nt = lb.block_nt[b];
+        s.c = lb.cell[slot];
+        s.P = lb.pos[slot];
+        s.tend = lb.tend[static_cast<size_t>(b) * LCC_TMAX + threadIdx.x % LCC_TMAX];
+        return s;
+    }
+
     /// partial[b][t][d] = sum over block b's restricted slots of w_q * r[d] for the corners q
-    /// whose block-local lattice point is t (t < nt = block_nt[b]).  Thread i of the 256 owns slot
-    /// b*256 + i with values r, lattice cell c and corner positions P (loaded by the caller); each
-    /// writes its 8 weighted terms to their positions in `vals` (LDS, 256*8*BD), so target t's
-    /// terms form a contiguous range.  Wave w then sums targets w, w+4, ...: lanes stride the
-    /// range, a fixed-order wave reduction finishes.  Called by all 256 threads (has a barrier).
+    /// whose block-local lattice point is t (t < nt).  Thread i of the 256 owns slot b*256 + i
+    /// with values r and its LatticeSlot s; each writes its 8 weighted terms to their positions
+    /// in `vals` (LDS, 256*8*BD), so target t's terms form a contiguous range.  Each 16-lane row
+    /// of a wave then sums one target (wave w takes targets 4w .. 4w+3, then 16 further): lanes
+    /// stride the range, accumulating in T, and a fixed-order row reduction finishes.  Called by
+    /// all 256 threads (has a barrier).
     template <int BD, typename T>
-    __device__ __forceinline__ void lattice_block_partials(const LatticeBlocks &lb, int b, int nt, int4 c, uint4 P,
+    __device__ __forceinline__ void lattice_block_partials(const LatticeBlocks &lb, int b, const LatticeSlot &s,
                                                            const T (&r)[BD], T *vals)
     {
+        const int nt = s.nt;
         if (nt == 0)
             return; // uniform over the block
-        // The targets' term ranges (nt <= LCC_TMAX) go to LDS with one load per thread, issued
-        // before the terms are written, so the waves' loops below make no global round trips.
         __shared__ int tp[LCC_TMAX + 1];
-        const int base = lb.tbase[b];
-        if (threadIdx.x <= nt)
-            tp[threadIdx.x] = lb.tptr[base + threadIdx.x];
-        if (c.x >= 0)
+        if (threadIdx.x == 0)
+            tp[0] = 0;
+        if (threadIdx.x < LCC_TMAX)
+            tp[threadIdx.x + 1] = s.tend;
+        if (s.c.x >= 0)
         {
-            const float fx = __int_as_float(c.y), fy = __int_as_float(c.z), fz = __int_as_float(c.w);
-            const unsigned pk[4] = {P.x, P.y, P.z, P.w};
+            const float fx = __int_as_float(s.c.y), fy = __int_as_float(s.c.z), fz = __int_as_float(s.c.w);
+            const unsigned pk[4] = {s.P.x, s.P.y, s.P.z, s.P.w};
 #pragma unroll
             for (int q = 0; q < 8; ++q)
             {
@@ -40,26 +61,27 @@ namespace psmas
             }
         }
         __syncthreads();
-        const int e_begin = tp[0];
-        const int lane = threadIdx.x % WAVE;
-        for (int t = threadIdx.x / WAVE; t < nt; t += 256 / WAVE)
+        const int lane = threadIdx.x % WAVE, rl = lane & 15;
+        for (int t0 = static_cast<int>(threadIdx.x / WAVE) * 4; t0 < nt; t0 += 16) // wave-uniform
         {
-            const int e0 = tp[t] - e_begin, e1 = tp[t + 1] - e_begin;
-            double a[BD];
+            const int t = t0 + (lane >> 4);
+            const bool tv = t < nt;
+            const int e0 = tv ? tp[t] : 0, e1 = tv ? tp[t + 1] : 0;
+            T a[BD];
 #pragma unroll
             for (int d = 0; d < BD; ++d)
-                a[d] = 0.0;
-            for (int e = e0 + lane; e < e1; e += WAVE)
+                a[d] = T(0);
+            for (int e = e0 + rl; e < e1; e += 16)
 #pragma unroll
                 for (int d = 0; d < BD; ++d)
-                    a[d] += static_cast<double>(vals[e * BD + d]);
+                    a[d] += vals[e * BD + d];
 #pragma unroll
             for (int d = 0; d < BD; ++d)
-            {
-                const double v = wave_sum(a[d]);
-                if (lane == 0)
-                    lb.partial[(static_cast<size_t>(b) * LCC_TMAX + t) * BD + d] = v;
-            }
+                a[d] = row_sum16(a[d]);
+            if (tv && rl == 0)
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                    lb.partial[(static_cast<size_t>(b) * LCC_TMAX + t) * BD + d] = static_cast<double>(a[d]);
         }
     }
 } // namespace psmas

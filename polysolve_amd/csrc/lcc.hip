@@ -55,12 +55,13 @@ namespace psmas
         {
             __shared__ T vals[256 * 8 * BD];
             const int b = blockIdx.x;
+            const LatticeSlot ls = lattice_slot(lb, b);
             T r[BD];
             const size_t p = static_cast<size_t>(b) * 256 + threadIdx.x;
 #pragma unroll
             for (int d = 0; d < BD; ++d)
                 r[d] = v[p * BD + d];
-            lattice_block_partials<BD, T>(lb, b, lb.block_nt[b], lb.cell[p], lb.pos[p], r, vals);
+            lattice_block_partials<BD, T>(lb, b, ls, r, vals);
         }
 
         /// One wave per (global) lattice point J: u[J][d] = sum of its partials, lanes striding over
@@ -411,7 +412,7 @@ namespace psmas
         // stores its 8 corners' local ids as bytes.  Used when no block touches more than
         // LCC_TMAX points; otherwise the chunked entry list above is used.
         const int nblocks = total_pad / 256;
-        std::vector<int> block_nt(nblocks, 0), tbase(nblocks + 1, 0), tptr{0};
+        std::vector<int> block_nt(nblocks, 0), tend(static_cast<size_t>(nblocks) * LCC_TMAX, 0);
         std::vector<uint4> pos(total_pad, uint4{0u, 0u, 0u, 0u});
         std::vector<std::vector<int>> point_lists(npts);
         bool blocks_ok = true;
@@ -450,20 +451,23 @@ namespace psmas
                         const int t = static_cast<int>(std::lower_bound(pts.begin(), pts.end(), id) - pts.begin());
                         lists[t].push_back(((p - b * 256) << 3) | q);
                     }
-            tbase[b] = static_cast<int>(tptr.size()) - 1;
-            const int e_begin = tptr.back();
-            for (const auto &l : lists)
+            int at_end = 0; // block-local end of the targets so far
+            for (size_t t = 0; t < lists.size(); ++t)
             {
+                const auto &l = lists[t];
                 for (size_t k = 0; k < l.size(); ++k)
                 {
                     const int pq = l[k];
-                    const int at = tptr.back() - e_begin + static_cast<int>(k);
+                    const int at = at_end + static_cast<int>(k);
                     const int p = b * 256 + (pq >> 3), q = pq & 7;
                     unsigned *word = (q >> 1) == 0 ? &pos[p].x : (q >> 1) == 1 ? &pos[p].y : (q >> 1) == 2 ? &pos[p].z : &pos[p].w;
                     *word |= static_cast<unsigned>(at) << (16 * (q & 1));
                 }
-                tptr.push_back(tptr.back() + static_cast<int>(l.size()));
+                at_end += static_cast<int>(l.size());
+                tend[static_cast<size_t>(b) * LCC_TMAX + t] = at_end;
             }
+            for (int t = block_nt[b]; t < LCC_TMAX; ++t)
+                tend[static_cast<size_t>(b) * LCC_TMAX + t] = at_end;
         }
         std::vector<int> point_ptr(npts + 1, 0), point_src;
         if (blocks_ok)
@@ -508,20 +512,17 @@ namespace psmas
         if (use_blocks_)
         {
             d_block_nt_.allocate(block_nt.size(), "MAS lattice block targets");
-            d_tbase_.allocate(tbase.size(), "MAS lattice block lists");
-            d_tptr_.allocate(tptr.size(), "MAS lattice block lists");
+            d_tend_.allocate(tend.size(), "MAS lattice block lists");
             d_pos_.allocate(pos.size(), "MAS lattice block positions");
             PSMAS_HIP(hipMemcpy(d_block_nt_.data(), block_nt.data(), d_block_nt_.bytes(), hipMemcpyHostToDevice));
-            PSMAS_HIP(hipMemcpy(d_tbase_.data(), tbase.data(), d_tbase_.bytes(), hipMemcpyHostToDevice));
-            PSMAS_HIP(hipMemcpy(d_tptr_.data(), tptr.data(), d_tptr_.bytes(), hipMemcpyHostToDevice));
+            PSMAS_HIP(hipMemcpy(d_tend_.data(), tend.data(), d_tend_.bytes(), hipMemcpyHostToDevice));
             PSMAS_HIP(hipMemcpy(d_pos_.data(), pos.data(), d_pos_.bytes(), hipMemcpyHostToDevice));
             d_partial_.allocate(static_cast<size_t>(nblocks) * LCC_TMAX * bd, "MAS lattice partials");
         }
         else
         {
             d_block_nt_.reset();
-            d_tbase_.reset();
-            d_tptr_.reset();
+            d_tend_.reset();
             d_pos_.reset();
             d_partial_.allocate(static_cast<size_t>(std::max(nchunks_, 1)) * bd, "MAS lattice partials");
         }
@@ -568,8 +569,7 @@ namespace psmas
         lb.cell = d_cell_.data();
         lb.pos = d_pos_.data();
         lb.block_nt = d_block_nt_.data();
-        lb.tbase = d_tbase_.data();
-        lb.tptr = d_tptr_.data();
+        lb.tend = d_tend_.data();
         lb.partial = const_cast<double *>(d_partial_.data());
         return lb;
     }

@@ -105,8 +105,7 @@ namespace psmas
         DeviceBuffer<int> d_point_ptr_;   ///< per global lattice point: range of point_src (nsys*np + 1)
         DeviceBuffer<int> d_point_src_;   ///< partial indices summed into each lattice point, in order
         DeviceBuffer<int> d_block_nt_;    ///< block path: distinct lattice points per node block
-        DeviceBuffer<int> d_tbase_;       ///< block path: first (block, target) list of each block
-        DeviceBuffer<int> d_tptr_;        ///< block path: entry offsets of every (block, target) list
+        DeviceBuffer<int> d_tend_;        ///< block path: [block][LCC_TMAX] block-local ends of the target lists
         DeviceBuffer<uint4> d_pos_;       ///< block path: position of each (slot, corner) term, 8 x 16 bit
         bool use_blocks_ = false;
         int nblocks_ = 0;

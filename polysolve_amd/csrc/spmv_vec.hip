@@ -34,83 +34,102 @@ namespace psmas
             using type = T;
         };
 
-        /// Row `lane` of SELL slice `slice`: acc += A_row x.  Slots are processed U at a time: the
-        /// batch's x gathers use column indices loaded one batch earlier, so per batch the wave
-        /// waits for one memory round trip instead of two (column, then x).  Every load is issued
+        /// Row `lane` of SELL slice `slice` as a stream of batches of U slots: acc += A_row x.
+        /// open() issues the first batch's column and value loads and returns without waiting for
+        /// them, so that they overlap what the caller does next (the halo stage); run() then
+        /// consumes one batch per step.  Per step the x reads use column indices loaded one step
+        /// earlier, the next columns are issued ahead of the FMAs and the next values after them,
+        /// so a wave waits for one memory round trip per batch.  Every load is issued
         /// unconditionally (slot indices past the slice end are clamped to its last slot, whose
         /// lines are in cache, and their values are zeroed), which lets the compiler count the
-        /// loads in flight and keep the column prefetch outstanding across the FMAs.
+        /// loads in flight.
         ///
         /// HALO: x comes from the workgroup's LDS copy xs of the entries its rows read, indexed by
-        /// 16-bit local columns (stage_halo fills it); otherwise x is gathered from global memory
+        /// 16-bit local columns (halo_fill writes it); otherwise x is gathered from global memory
         /// through the 32-bit columns.
-        template <int BD, int U, bool HALO, typename TV, typename TX, typename TA>
-        __device__ __forceinline__ void sell_row(const SellView<TV> &A, int slice, int lane, const TX *__restrict__ x,
-                                                 const TX *xs, TA acc[BD])
+        template <int BD, int U, bool HALO, typename TV>
+        struct SellRow
         {
-            const int beg = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice]);
-            const int end = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice + 1]);
-            if (end <= beg)
-                return;
-            const int last = end - 1;
-            const int *__restrict__ cols = A.cols + lane;
-            const unsigned short *__restrict__ lcols = A.lcols + lane;
-            const TV *__restrict__ vals = A.vals + lane;
-            auto column = [&](int slot) -> int {
+            int beg = 0, end = 0, last = 0;
+            const int *__restrict__ cols = nullptr;
+            const unsigned short *__restrict__ lcols = nullptr;
+            const TV *__restrict__ vals = nullptr;
+            int c[U];
+            TV v[U][BD * BD];
+
+            __device__ __forceinline__ int column(int slot) const
+            {
                 if constexpr (HALO)
                     return static_cast<int>(ldg_stream(lcols + static_cast<size_t>(slot) * WAVE));
                 else
                     return ldg_stream(cols + static_cast<size_t>(slot) * WAVE);
-            };
-            int c[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                c[u] = column(min(beg + u, last));
-            for (int k = beg; k < end; k += U)
+            }
+            __device__ __forceinline__ void load_values(int k)
             {
-                TA xv[U][BD];
-                TV v[U][BD * BD];
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                 {
                     const size_t slot = static_cast<size_t>(min(k + u, last));
-                    if constexpr (!HALO)
-                    {
-#pragma unroll
-                        for (int j = 0; j < BD; ++j)
-                            xv[u][j] = static_cast<TA>(x[static_cast<size_t>(c[u]) * BD + j]);
-                    }
 #pragma unroll
                     for (int e = 0; e < BD * BD; ++e)
                         v[u][e] = ldg_stream(vals + (slot * (BD * BD) + e) * WAVE);
                 }
-                if constexpr (HALO)
+            }
+            __device__ __forceinline__ void open(const SellView<TV> &A, int slice, int lane)
+            {
+                beg = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice]);
+                end = __builtin_amdgcn_readfirstlane(A.slice_ptr[slice + 1]);
+                last = end - 1;
+                cols = A.cols + lane;
+                lcols = A.lcols + lane;
+                vals = A.vals + lane;
+                if (end > beg) // wave-uniform
                 {
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        c[u] = column(min(beg + u, last));
+                    load_values(beg);
+                }
+            }
+            template <typename TX, typename TA>
+            __device__ __forceinline__ void run(const TX *__restrict__ x, const TX *xs, TA acc[BD])
+            {
+                for (int k = beg; k < end; k += U)
+                {
+                    TA xv[U][BD];
 #pragma unroll
                     for (int u = 0; u < U; ++u)
 #pragma unroll
                         for (int j = 0; j < BD; ++j)
-                            xv[u][j] = static_cast<TA>(xs[c[u] * BD + j]);
-                }
-                // Keep the next batch's column loads behind this batch's loads and ahead of the
-                // FMAs, so that they stay in flight while the FMAs wait for this batch.
-                __builtin_amdgcn_sched_barrier(0);
+                        {
+                            if constexpr (HALO)
+                                xv[u][j] = static_cast<TA>(xs[c[u] * BD + j]);
+                            else
+                                xv[u][j] = static_cast<TA>(x[static_cast<size_t>(c[u]) * BD + j]);
+                        }
+                    // The next batch's column loads go behind this batch's loads and ahead of the
+                    // FMAs, so that they stay in flight while the FMAs wait for this batch.
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    c[u] = column(min(k + U + u, last));
-                __builtin_amdgcn_sched_barrier(0);
+                    for (int u = 0; u < U; ++u)
+                        c[u] = column(min(k + U + u, last));
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                {
-                    const bool ok = k + u < end; // wave-uniform
+                    for (int u = 0; u < U; ++u)
+                    {
+                        const bool ok = k + u < end; // wave-uniform
 #pragma unroll
-                    for (int i = 0; i < BD; ++i)
+                        for (int i = 0; i < BD; ++i)
 #pragma unroll
-                        for (int j = 0; j < BD; ++j)
-                            acc[i] = fma(ok ? static_cast<TA>(v[u][i * BD + j]) : TA(0), xv[u][j], acc[i]);
+                            for (int j = 0; j < BD; ++j)
+                                acc[i] = fma(ok ? static_cast<TA>(v[u][i * BD + j]) : TA(0), xv[u][j], acc[i]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (k + U < end) // wave-uniform
+                        load_values(k + U);
                 }
             }
-        }
+        };
 
         /// Node block of this workgroup for the SpMV kernels.  Workgroups are dealt to the 8 XCDs
         /// round-robin; this gives each XCD one contiguous range of node blocks, so the x gathers
@@ -125,19 +144,58 @@ namespace psmas
 
         __device__ __forceinline__ int block_system(const SysMap &sm) { return sm.block_sys[blockIdx.x]; }
 
-        /// Copy x at node block blk's halo into the dynamic LDS array and wait for the workgroup;
-        /// every wave of the workgroup must call it (uniform control flow).
+        /// Halo stage of node block blk in two parts: halo_indices issues the loads of the first
+        /// HALO_R rounds of halo node indices (clamped, unconditional), halo_fill gathers x at
+        /// them (one round trip for all rounds), stores the values in the dynamic LDS array,
+        /// handles longer halos in a loop, and waits for the workgroup.  Every wave of the
+        /// workgroup must call both (uniform control flow).
+        constexpr int HALO_R = 4;
+        struct HaloStage
+        {
+            int hb = 0, hn = 0;
+            int g[HALO_R];
+        };
+        template <typename TV>
+        __device__ __forceinline__ HaloStage halo_indices(const SellView<TV> &A, int blk)
+        {
+            HaloStage h;
+            h.hb = A.halo_ptr[blk];
+            h.hn = A.halo_ptr[blk + 1] - h.hb;
+            const int lastl = max(h.hn - 1, 0);
+#pragma unroll
+            for (int r = 0; r < HALO_R; ++r)
+                h.g[r] = h.hn > 0 ? A.halo[h.hb + min(static_cast<int>(threadIdx.x) + r * SPMV_BLOCK, lastl)] : 0;
+            return h;
+        }
         template <int BD, typename TV, typename TX>
-        __device__ __forceinline__ const TX *stage_halo(const SellView<TV> &A, int blk, const TX *__restrict__ x)
+        __device__ __forceinline__ const TX *halo_fill(const HaloStage &h, const SellView<TV> &A,
+                                                       const TX *__restrict__ x)
         {
             TX *xs = reinterpret_cast<TX *>(spmv_halo_lds);
-            const int hb = A.halo_ptr[blk], hn = A.halo_ptr[blk + 1] - hb;
-            for (int i = threadIdx.x; i < hn; i += SPMV_BLOCK)
+            if (h.hn > 0)
             {
-                const size_t g = static_cast<size_t>(A.halo[hb + i]) * BD;
+                TX v[HALO_R][BD];
 #pragma unroll
-                for (int j = 0; j < BD; ++j)
-                    xs[i * BD + j] = x[g + j];
+                for (int r = 0; r < HALO_R; ++r)
+#pragma unroll
+                    for (int j = 0; j < BD; ++j)
+                        v[r][j] = x[static_cast<size_t>(h.g[r]) * BD + j];
+#pragma unroll
+                for (int r = 0; r < HALO_R; ++r)
+                {
+                    const int i = threadIdx.x + r * SPMV_BLOCK;
+                    if (i < h.hn)
+#pragma unroll
+                        for (int j = 0; j < BD; ++j)
+                            xs[i * BD + j] = v[r][j];
+                }
+                for (int i = threadIdx.x + HALO_R * SPMV_BLOCK; i < h.hn; i += SPMV_BLOCK)
+                {
+                    const size_t g = static_cast<size_t>(A.halo[h.hb + i]) * BD;
+#pragma unroll
+                    for (int j = 0; j < BD; ++j)
+                        xs[i * BD + j] = x[g + j];
+                }
             }
             __syncthreads();
             return xs;
@@ -152,16 +210,21 @@ namespace psmas
             const int sys = sm.block_sys[blk];
             if (sm.active[sys] == 0)
                 return;
-            const TX *xs = nullptr;
-            if constexpr (HALO)
-                xs = stage_halo<BD>(A, blk, x);
             const int lane = threadIdx.x % WAVE;
             const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            HaloStage hs;
+            if constexpr (HALO)
+                hs = halo_indices(A, blk);
+            const TX *xs = nullptr;
+            if constexpr (HALO)
+                xs = halo_fill<BD>(hs, A, x);
+            SellRow<BD, U, HALO, TV> sr;
+            sr.open(A, slice, lane);
             TA acc[BD];
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD, U, HALO>(A, slice, lane, x, xs, acc);
+            sr.run(x, xs, acc);
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
             double dot = 0.0;
 #pragma unroll
@@ -266,14 +329,16 @@ namespace psmas
                 for (int j = 0; j < BD; ++j)
                     xs[i * BD + j] = dir(g + j);
             }
-            __syncthreads();
             const int lane = threadIdx.x % WAVE;
             const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            __syncthreads();
+            SellRow<BD, U, true, TV> sr;
+            sr.open(A, slice, lane);
             TA acc[BD];
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD, U, true>(A, slice, lane, static_cast<const TX *>(nullptr), xs, acc);
+            sr.run(static_cast<const TX *>(nullptr), xs, acc);
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
             double dot = 0.0;
 #pragma unroll
@@ -449,17 +514,22 @@ namespace psmas
             const int sys = sm.block_sys[blk];
             if (sm.active[sys] == 0 || (selected_only && sm.select[sys] == 0))
                 return;
-            const TX *xs = nullptr;
-            if constexpr (HALO)
-                xs = stage_halo<BD>(A, blk, x);
             const int lane = threadIdx.x % WAVE;
             const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
+            HaloStage hs;
+            if constexpr (HALO)
+                hs = halo_indices(A, blk);
+            const TX *xs = nullptr;
+            if constexpr (HALO)
+                xs = halo_fill<BD>(hs, A, x);
+            SellRow<BD, U, HALO, TV> sr;
+            sr.open(A, slice, lane);
             TA acc[BD];
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD, U, HALO>(A, slice, lane, x, xs, acc);
+            sr.run(x, xs, acc);
             double dot = 0.0;
 #pragma unroll
             for (int i = 0; i < BD; ++i)
@@ -490,7 +560,9 @@ namespace psmas
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            sell_row<BD, U, false>(A, slice, lane, x, static_cast<const TX *>(nullptr), acc);
+            SellRow<BD, U, false, TV> sr;
+            sr.open(A, slice, lane);
+            sr.run(x, static_cast<const TX *>(nullptr), acc);
             const size_t row = static_cast<size_t>(slice) * WAVE + lane;
 #pragma unroll
             for (int i = 0; i < BD; ++i)
@@ -520,16 +592,9 @@ namespace psmas
             if (sm.active[sys] == 0)
                 return;
             // Lattice inputs first: their loads overlap the vector update.
-            int lnt = 0;
-            int4 lc{-1, 0, 0, 0};
-            uint4 lp{0u, 0u, 0u, 0u};
+            LatticeSlot ls;
             if constexpr (LAT)
-            {
-                const size_t slot = static_cast<size_t>(blockIdx.x) * VEC_BLOCK + threadIdx.x;
-                lnt = lb.block_nt[blockIdx.x];
-                lc = lb.cell[slot];
-                lp = lb.pos[slot];
-            }
+                ls = lattice_slot(lb, blockIdx.x);
             // The block's vectors and restriction map are loaded before alpha's reduction (whose
             // barriers the loads would not cross) so that all these latencies overlap.
             const size_t node = static_cast<size_t>(blockIdx.x) * VEC_BLOCK + threadIdx.x;
@@ -569,7 +634,7 @@ namespace psmas
 #pragma unroll
                 for (int d = 0; d < BD; ++d)
                     rn[d] = sv[threadIdx.x * BD + d];
-                lattice_block_partials<BD, T>(lb, blockIdx.x, lnt, lc, lp, rn, vals);
+                lattice_block_partials<BD, T>(lb, blockIdx.x, ls, rn, vals);
             }
         }
 
