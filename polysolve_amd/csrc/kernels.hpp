@@ -226,6 +226,12 @@ namespace psmas
     /// Longest descendant list of a restriction task that takes one thread; longer lists are cut
     /// into pieces of at most RESTRICT_PIECE entries that take a wave each.
     constexpr int RESTRICT_SMALL_MAX = 8;
+    // Network layer 1 in the second apply launch: at most LIN_WGS_MAX workgroups, each lane holds
+    // LIN_CHUNKS 16-byte chunks of a weight row (rows up to 64 * 8 * LIN_CHUNKS wide) and each
+    // thread LIN_U_PER_THREAD values of u.
+    constexpr int LIN_WGS_MAX = 768;
+    constexpr int LIN_CHUNKS = 4;
+    constexpr int LIN_U_PER_THREAD = 16;
     constexpr int RESTRICT_PIECE = 512;
 
     /// Work done by the leading `wgs` workgroups of an apply launch beside the bank matrices (the
@@ -269,11 +275,13 @@ namespace psmas
         // dense_seg + c dense_stride; one pass over each inverse row serves every copy.
         int dense_copies = 1;
         int dense_stride = 0;
-        // Layer 1 and the skip layer of the learned correction's network for a single system: every
-        // side workgroup stages x = u sqrt(m) / |u| (zero-padded to lin_md) in LDS after the dense
-        // inputs; waves take rows (row < lin_h: h1 = relu(W1 x + b1), else skip = S x); workgroup 0
-        // writes scale = |u| lin_ascale / sqrt(m).  Skipped when lin_active[0] == 0.
+        // Layer 1 and the skip layer of the learned correction's network for a single system, done
+        // by side workgroups [lin_wg0, wgs) (the others do the work above): each stages
+        // x = u sqrt(m) / |u| (zero-padded to lin_md) in LDS; waves take rows (row < lin_h:
+        // h1 = relu(W1 x + b1), else skip = S x); workgroup lin_wg0 writes
+        // scale = |u| lin_ascale / sqrt(m).  Skipped when lin_active[0] == 0.
         int lin_rows = 0; ///< lin_h + lin_md (0: none)
+        int lin_wg0 = 0;
         int lin_m = 0, lin_md = 0, lin_h = 0;
         const double *lin_u = nullptr;
         const unsigned short *lin_w1 = nullptr, *lin_s = nullptr;

@@ -195,6 +195,97 @@ namespace psmas
 
         constexpr int LCC_HMAX = 1024; ///< largest hidden width the fused output kernel holds in LDS
 
+        /// k_lcc_out for hidden width 128 in one memory round trip: every load that depends on
+        /// nothing (this workgroup's W2 rows, its output row of W3, h1, the biases, the skip
+        /// outputs and the scales) is issued at the start.  A 16-lane unit u of the workgroup
+        /// computes h2 rows u, u + 16, ... (lane l: columns 8l .. 8l + 7, then a row reduction),
+        /// then output row 16 chunk + u the same way.  Workgroup (row chunk, system group).
+        __global__ __launch_bounds__(256) void k_lcc_out128(const unsigned short *__restrict__ W2,
+                                                            const float *__restrict__ b2,
+                                                            const float *__restrict__ H1,
+                                                            const unsigned short *__restrict__ W3,
+                                                            const float *__restrict__ b3, int rows,
+                                                            const float *__restrict__ skip, int ld_skip,
+                                                            const float *__restrict__ scale, float *__restrict__ coef,
+                                                            int ldc, int nsys, const int *__restrict__ active)
+        {
+            constexpr int H = 128, RPU = H / 16, HV = GEMV_SYS * H / 256;
+            __shared__ __attribute__((aligned(16))) float h1s[GEMV_SYS][H];
+            __shared__ __attribute__((aligned(16))) float h2s[GEMV_SYS][H];
+            const int ngroups = (nsys + GEMV_SYS - 1) / GEMV_SYS;
+            const int g = blockIdx.x % ngroups, chunk = blockIdx.x / ngroups;
+            const int s0 = g * GEMV_SYS;
+            const int ns = min(GEMV_SYS, nsys - s0);
+            const int unit = threadIdx.x >> 4, rl = threadIdx.x & 15;
+            const int orow = chunk * 16 + unit;
+            const int orc = min(orow, rows - 1);
+            // Loads first (addresses clamped where out of range; unused values are ignored).
+            uint4 w2[RPU];
+#pragma unroll
+            for (int i = 0; i < RPU; ++i)
+                w2[i] = reinterpret_cast<const uint4 *>(W2 + static_cast<size_t>(unit + 16 * i) * H)[rl];
+            const uint4 w3 = reinterpret_cast<const uint4 *>(W3 + static_cast<size_t>(orc) * H)[rl];
+            float h1v[HV];
+#pragma unroll
+            for (int i = 0; i < HV; ++i)
+                h1v[i] = H1[static_cast<size_t>(s0) * H + min(static_cast<int>(threadIdx.x) + 256 * i, ns * H - 1)];
+            float b2v[RPU];
+#pragma unroll
+            for (int i = 0; i < RPU; ++i)
+                b2v[i] = b2[unit + 16 * i];
+            const float b3v = b3[orc];
+            float skv[GEMV_SYS], scv[GEMV_SYS];
+            int actv[GEMV_SYS];
+#pragma unroll
+            for (int j = 0; j < GEMV_SYS; ++j)
+            {
+                const int sj = s0 + min(j, ns - 1);
+                skv[j] = skip[static_cast<size_t>(sj) * ld_skip + orc];
+                scv[j] = scale[sj];
+                actv[j] = active != nullptr ? active[sj] : 1;
+            }
+#pragma unroll
+            for (int i = 0; i < HV; ++i)
+            {
+                const int idx = threadIdx.x + 256 * i;
+                if (idx < ns * H)
+                    h1s[idx / H][idx % H] = h1v[i];
+            }
+            __syncthreads();
+            auto dot8 = [](const uint4 &wv, const float *x) {
+                const float4 x0 = *reinterpret_cast<const float4 *>(x);
+                const float4 x1 = *reinterpret_cast<const float4 *>(x + 4);
+                return bf16_lo(wv.x) * x0.x + bf16_hi(wv.x) * x0.y + bf16_lo(wv.y) * x0.z + bf16_hi(wv.y) * x0.w +
+                       bf16_lo(wv.z) * x1.x + bf16_hi(wv.z) * x1.y + bf16_lo(wv.w) * x1.z + bf16_hi(wv.w) * x1.w;
+            };
+#pragma unroll
+            for (int i = 0; i < RPU; ++i)
+            {
+                const int r = unit + 16 * i;
+#pragma unroll
+                for (int j = 0; j < GEMV_SYS; ++j)
+                {
+                    if (j >= ns)
+                        break; // uniform
+                    const float a = row_sum16(dot8(w2[i], &h1s[j][8 * rl])) + b2v[i];
+                    if (rl == 0)
+                        h2s[j][r] = a > 0.0f ? a : 0.0f;
+                }
+            }
+            __syncthreads();
+            if (orow >= rows)
+                return; // whole 16-lane units
+#pragma unroll
+            for (int j = 0; j < GEMV_SYS; ++j)
+            {
+                if (j >= ns)
+                    break;
+                const float a = row_sum16(dot8(w3, &h2s[j][8 * rl]));
+                if (rl == 0 && actv[j] != 0)
+                    coef[static_cast<size_t>(s0 + j) * ldc + orow] = (a + b3v + skv[j]) * scv[j];
+            }
+        }
+
         /// Hidden layer 2 and the output layer in one launch.  Workgroup (row chunk, system group):
         /// the 256 threads first compute h2 = relu(W2 h1 + b2) for the group's systems into LDS (every
         /// workgroup of the group computes the same values), then each wave produces one output
@@ -603,8 +694,7 @@ namespace psmas
     bool LearnedCoarse::in_side(ApplySide &sd, int stage_floats, int block, const int *active)
     {
         check_network();
-        const int dense = sd.dense_nd > 0 ? sd.dense_nd : 0;
-        if (nsys_ != 1 || dense + md_ > stage_floats)
+        if (nsys_ != 1 || md_ > stage_floats || md_ / 8 > 64 * LIN_CHUNKS || m() > LIN_U_PER_THREAD * block)
             return false;
         const unsigned short *wb = net_->bf16_weights();
         const float *wf = net_->fp32_weights();
@@ -622,7 +712,9 @@ namespace psmas
         sd.lin_scale = d_scale_.data();
         sd.lin_ascale = ascale_ * out_factor_;
         sd.lin_active = active;
-        sd.wgs = std::max(sd.wgs, std::min(256, div_up(sd.lin_rows, block / 64))); // about a row per wave
+        // Workgroups of their own after the others' (a row per wave).
+        sd.lin_wg0 = sd.wgs;
+        sd.wgs += std::min(LIN_WGS_MAX, div_up(sd.lin_rows, block / 64));
         return true;
     }
 
@@ -634,10 +726,16 @@ namespace psmas
         const int ngroups = (nsys_ + GEMV_SYS - 1) / GEMV_SYS;
         // Layer 2 fused into the output layer (only the m real outputs), plus the skip output,
         // times the scale.
-        hipLaunchKernelGGL(k_lcc_out, dim3(div_up(m(), 4) * ngroups), dim3(256), 0, s, wb + net_->offset(2),
-                           wf + net_->offset(3), h, static_cast<const float *>(d_h1_.data()), wb + net_->offset(4),
-                           wf + net_->offset(5), m(), static_cast<const float *>(d_skip_.data()), md_,
-                           static_cast<const float *>(d_scale_.data()), d_coef_.data(), m(), nsys_, active);
+        if (h == 128)
+            hipLaunchKernelGGL(k_lcc_out128, dim3(div_up(m(), 16) * ngroups), dim3(256), 0, s, wb + net_->offset(2),
+                               wf + net_->offset(3), static_cast<const float *>(d_h1_.data()), wb + net_->offset(4),
+                               wf + net_->offset(5), m(), static_cast<const float *>(d_skip_.data()), md_,
+                               static_cast<const float *>(d_scale_.data()), d_coef_.data(), m(), nsys_, active);
+        else
+            hipLaunchKernelGGL(k_lcc_out, dim3(div_up(m(), 4) * ngroups), dim3(256), 0, s, wb + net_->offset(2),
+                               wf + net_->offset(3), h, static_cast<const float *>(d_h1_.data()), wb + net_->offset(4),
+                               wf + net_->offset(5), m(), static_cast<const float *>(d_skip_.data()), md_,
+                               static_cast<const float *>(d_scale_.data()), d_coef_.data(), m(), nsys_, active);
         PSMAS_LAUNCH_CHECK();
     }
 

@@ -134,6 +134,106 @@ namespace psmas
             stride = cl.cnt[l];
         }
 
+        /// Layer 1 and the skip layer (ApplySide lin_*) by side workgroup w of nw.  Everything that
+        /// depends on nothing is loaded first: the weights of this wave's first row (rows go to
+        /// waves round-robin), u (once: |u| and the normalised input come from the same
+        /// registers) and the activity flag; each later row's weights are issued before the
+        /// previous row is computed.
+        __device__ void lin_side_work(const ApplySide &sd, int w, int nw, float *xs)
+        {
+            __shared__ double lin_red[4];
+            const int lane = threadIdx.x & 63;
+            const int m = sd.lin_m, md = sd.lin_md, h = sd.lin_h;
+            const int nch = md / 8; // 16-byte chunks per row (<= 64 LIN_CHUNKS)
+            const int nwb = static_cast<int>(blockDim.x) / 64;
+            const int wave_id = w * nwb + static_cast<int>(threadIdx.x) / 64, waves = nw * nwb;
+            auto row_ptr = [&](int row) {
+                const bool first = row < h;
+                return reinterpret_cast<const uint4 *>((first ? sd.lin_w1 : sd.lin_s) +
+                                                       static_cast<size_t>(first ? row : row - h) * md);
+            };
+            // A row's weights and, for rows of layer 1, its bias.
+            auto load_row = [&](int row, uint4(&wv)[LIN_CHUNKS], float &bias) {
+                const int r = min(row, sd.lin_rows - 1);
+                const uint4 *wr = row_ptr(r);
+#pragma unroll
+                for (int k = 0; k < LIN_CHUNKS; ++k)
+                    wv[k] = wr[min(lane + 64 * k, nch - 1)];
+                bias = sd.lin_b1[min(r, h - 1)];
+            };
+            uint4 wv[LIN_CHUNKS];
+            float bv;
+            int row = wave_id;
+            load_row(row, wv, bv);
+            double uv[LIN_U_PER_THREAD];
+#pragma unroll
+            for (int k = 0; k < LIN_U_PER_THREAD; ++k)
+                uv[k] = sd.lin_u[min(static_cast<int>(threadIdx.x + k * blockDim.x), m - 1)];
+            const int act = sd.lin_active != nullptr ? sd.lin_active[0] : 1;
+            if (act == 0)
+                return; // uniform
+            double q = 0.0;
+#pragma unroll
+            for (int k = 0; k < LIN_U_PER_THREAD; ++k)
+                if (static_cast<int>(threadIdx.x + k * blockDim.x) < m)
+                    q += uv[k] * uv[k];
+            q = wave_sum(q);
+            if (lane == 0)
+                lin_red[threadIdx.x / 64] = q;
+            __syncthreads();
+            double nu2 = 0.0;
+            for (int i = 0; i < nwb; ++i)
+                nu2 += lin_red[i];
+            const double nu = sqrt(nu2);
+            const double fac = nu > 0.0 ? sqrt(static_cast<double>(m)) / nu : 0.0;
+            if (w == 0 && threadIdx.x == 0)
+                sd.lin_scale[0] = static_cast<float>(nu * sd.lin_ascale / sqrt(static_cast<double>(m)));
+#pragma unroll
+            for (int k = 0; k < LIN_U_PER_THREAD; ++k)
+            {
+                const int i = threadIdx.x + k * blockDim.x;
+                if (i < m)
+                    xs[i] = static_cast<float>(uv[k] * fac);
+            }
+            for (int i = m + threadIdx.x; i < md; i += blockDim.x)
+                xs[i] = 0.0f;
+            __syncthreads();
+            for (; row < sd.lin_rows; row += waves) // wave-uniform
+            {
+                uint4 cur[LIN_CHUNKS];
+#pragma unroll
+                for (int k = 0; k < LIN_CHUNKS; ++k)
+                    cur[k] = wv[k];
+                const float bcur = bv;
+                if (row + waves < sd.lin_rows)
+                    load_row(row + waves, wv, bv);
+                float acc = 0.0f;
+#pragma unroll
+                for (int k = 0; k < LIN_CHUNKS; ++k)
+                {
+                    const int c = lane + 64 * k;
+                    if (c < nch)
+                    {
+                        const uint4 v = cur[k];
+                        const float *x = xs + c * 8;
+                        acc += bf16_lo(v.x) * x[0] + bf16_hi(v.x) * x[1] + bf16_lo(v.y) * x[2] + bf16_hi(v.y) * x[3] +
+                               bf16_lo(v.z) * x[4] + bf16_hi(v.z) * x[5] + bf16_lo(v.w) * x[6] + bf16_hi(v.w) * x[7];
+                    }
+                }
+                acc = wave_sum(acc);
+                if (lane == 0)
+                {
+                    if (row < h)
+                    {
+                        const float y = acc + bcur;
+                        sd.lin_h1[row] = y > 0.0f ? y : 0.0f;
+                    }
+                    else
+                        sd.lin_skip[row - h] = acc;
+                }
+            }
+        }
+
         /// Side work of workgroup `wg` < sd.wgs: restriction tasks (thread per small node, wave per
         /// large node), lattice gather, then dense coarsest rows (the inputs staged once per
         /// workgroup in `stage`, LDS of `stage_cap` elements, when they fit; wave per row, lanes
@@ -142,7 +242,16 @@ namespace psmas
         __device__ void apply_side_work(const ApplySide &sd, const T *__restrict__ mr, T *__restrict__ mz, int wg,
                                         T *stage, int stage_cap)
         {
-            const int nt = sd.wgs * static_cast<int>(blockDim.x);
+            if constexpr (std::is_same<T, float>::value)
+            {
+                if (sd.lin_rows > 0 && wg >= sd.lin_wg0)
+                {
+                    lin_side_work(sd, wg - sd.lin_wg0, sd.wgs - sd.lin_wg0, stage);
+                    return;
+                }
+            }
+            const int nside = sd.lin_rows > 0 ? sd.lin_wg0 : sd.wgs;
+            const int nt = nside * static_cast<int>(blockDim.x);
             const int t = wg * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
             const int lane = threadIdx.x & 63;
             if (sd.nsmall + sd.nbig > 0)
@@ -310,62 +419,6 @@ namespace psmas
                         {
                             const int node = sd.dense_seg + c * sd.dense_stride + sd.dense_ids[row / BD];
                             mz[static_cast<size_t>(node) * BD + row % BD] = static_cast<T>(v);
-                        }
-                    }
-                }
-            }
-            if constexpr (std::is_same<T, float>::value)
-            {
-                if (sd.lin_rows > 0 && (sd.lin_active == nullptr || sd.lin_active[0] != 0))
-                {
-                    // Normalised lattice residual in LDS, after the dense inputs.
-                    __shared__ double lin_red[4];
-                    float *xs = stage + (sd.dense_nd > 0 ? sd.dense_nd : 0);
-                    const int m = sd.lin_m, md = sd.lin_md, h = sd.lin_h;
-                    const int nwb = static_cast<int>(blockDim.x) / 64;
-                    __syncthreads(); // the dense section may still read stage
-                    double q = 0.0;
-                    for (int i = threadIdx.x; i < m; i += blockDim.x)
-                        q += sd.lin_u[i] * sd.lin_u[i];
-                    q = wave_sum(q);
-                    if (lane == 0)
-                        lin_red[threadIdx.x / 64] = q;
-                    __syncthreads();
-                    double nu2 = 0.0;
-                    for (int w = 0; w < nwb; ++w)
-                        nu2 += lin_red[w];
-                    const double nu = sqrt(nu2);
-                    const double fac = nu > 0.0 ? sqrt(static_cast<double>(m)) / nu : 0.0;
-                    if (wg == 0 && threadIdx.x == 0)
-                        sd.lin_scale[0] = static_cast<float>(nu * sd.lin_ascale / sqrt(static_cast<double>(m)));
-                    for (int i = threadIdx.x; i < md; i += blockDim.x)
-                        xs[i] = i < m ? static_cast<float>(sd.lin_u[i] * fac) : 0.0f;
-                    __syncthreads();
-                    for (int row = t >> 6; row < sd.lin_rows; row += nt >> 6)
-                    {
-                        const bool first = row < h;
-                        const int r = first ? row : row - h;
-                        const uint4 *wrow =
-                            reinterpret_cast<const uint4 *>((first ? sd.lin_w1 : sd.lin_s) + static_cast<size_t>(r) * md);
-                        float acc = 0.0f;
-                        for (int c = lane; c < md / 8; c += 64)
-                        {
-                            const uint4 wv = wrow[c];
-                            const float *x = xs + c * 8;
-                            acc += bf16_lo(wv.x) * x[0] + bf16_hi(wv.x) * x[1] + bf16_lo(wv.y) * x[2] +
-                                   bf16_hi(wv.y) * x[3] + bf16_lo(wv.z) * x[4] + bf16_hi(wv.z) * x[5] +
-                                   bf16_lo(wv.w) * x[6] + bf16_hi(wv.w) * x[7];
-                        }
-                        acc = wave_sum(acc);
-                        if (lane == 0)
-                        {
-                            if (first)
-                            {
-                                const float y = acc + sd.lin_b1[r];
-                                sd.lin_h1[r] = y > 0.0f ? y : 0.0f;
-                            }
-                            else
-                                sd.lin_skip[r] = acc;
                         }
                     }
                 }
