@@ -233,6 +233,10 @@ namespace psmas
     constexpr int LIN_CHUNKS = 4;
     constexpr int LIN_U_PER_THREAD = 16;
     constexpr int RESTRICT_PIECE = 512;
+    /// Threads per workgroup of the FP16 apply of one right-hand side (a wave per matrix, four
+    /// matrices per workgroup, k_mas_apply_w16).
+    constexpr int APPLY_W16_BLOCK = 256;
+    constexpr int APPLY_W16_WAVES = APPLY_W16_BLOCK / 64;
 
     /// Work done by the leading `wgs` workgroups of an apply launch beside the bank matrices (the
     /// other workgroups apply the matrices).  Either part may be empty.
@@ -240,6 +244,10 @@ namespace psmas
     {
         int wgs = 0;
         int apply_cap = 0; ///< > 0: at most this many matrix workgroups (host launch choice only)
+        /// FP16 storage and one right-hand side: a wave per matrix (k_mas_apply_w16, workgroups of
+        /// APPLY_W16_BLOCK threads) instead of the persistent kernel that unpacks each matrix in
+        /// LDS (host launch choice only; apply_threads gives the side workgroups' size).
+        bool wave_apply = false;
         // Restriction of coarse segments 2.. from segment 1: task i sums the segment-1 values of
         // rlist[rtask[i].y .. rtask[i].z) (ascending node ids) in FP64.  Tasks [0, nsmall) take a
         // thread each, [nsmall, nsmall + nbig) a wave each (lanes stride the list, fixed-order
@@ -325,8 +333,11 @@ namespace psmas
     /// (16-byte nontemporal loads, grid-stride); allocates and frees its buffer.
     double read_bandwidth_probe(long long bytes, int reps);
 
-    /// Elements of the vector type the FP16 apply kernel's side workgroups can stage in LDS.
-    int apply_h16_stage_floats(int bd);
+    /// Elements of the vector type the FP16 apply kernel's side workgroups can stage in LDS
+    /// (wave: the wave-per-matrix kernel of one right-hand side).
+    int apply_h16_stage_floats(int bd, bool wave);
+    /// Threads per workgroup of an apply launch (side workgroups included).
+    int apply_threads(int bd, bool wave);
 
 
     /// z_p = mz_p + sum_l mz_l[cmap_p.l] (+ the lattice correction P c when lcc.cell is set);

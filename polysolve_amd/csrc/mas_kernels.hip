@@ -18,6 +18,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <type_traits>
+#include <utility>
 
 namespace psmas
 {
@@ -237,12 +238,14 @@ namespace psmas
         /// Side work of workgroup `wg` < sd.wgs: restriction tasks (thread per small node, wave per
         /// large node), lattice gather, then dense coarsest rows (the inputs staged once per
         /// workgroup in `stage`, LDS of `stage_cap` elements, when they fit; wave per row, lanes
-        /// over columns).
-        template <int BD, typename T>
+        /// over columns).  PART 1 compiles only the work of a first apply launch (restriction,
+        /// lattice gather), PART 2 only that of a second one (dense rows, network layer 1),
+        /// PART 0 all of it.
+        template <int BD, typename T, int PART = 0>
         __device__ void apply_side_work(const ApplySide &sd, const T *__restrict__ mr, T *__restrict__ mz, int wg,
                                         T *stage, int stage_cap)
         {
-            if constexpr (std::is_same<T, float>::value)
+            if constexpr (std::is_same<T, float>::value && PART != 1)
             {
                 if (sd.lin_rows > 0 && wg >= sd.lin_wg0)
                 {
@@ -254,7 +257,7 @@ namespace psmas
             const int nt = nside * static_cast<int>(blockDim.x);
             const int t = wg * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
             const int lane = threadIdx.x & 63;
-            if (sd.nsmall + sd.nbig > 0)
+            if (PART != 2 && sd.nsmall + sd.nbig > 0)
             {
                 const T *seg1 = static_cast<const T *>(sd.seg1);
                 T *out = static_cast<T *>(sd.out);
@@ -322,7 +325,7 @@ namespace psmas
                     }
                 }
             }
-            if (sd.lat_npts > 0)
+            if (PART != 2 && sd.lat_npts > 0)
             {
                 // Lattice gather: a wave per point, counting waves from the far end so that the
                 // waves with restriction pieces get points last.
@@ -349,6 +352,8 @@ namespace psmas
                     }
                 }
             }
+            if (PART == 1)
+                return;
             if (sd.dense_nd > 0 && sd.dense_copies <= 1)
             {
                 const int nd = sd.dense_nd;
@@ -831,6 +836,277 @@ namespace psmas
                     break;
             }
         }
+        /// Sum over the 8 lanes of an aligned 8-lane group of v[c]: lane r (= lane % 8) receives
+        /// sum over the group's lanes r' of v_r'[r].  Three halving exchanges (half-row mirror,
+        /// quad_perm [2, 3, 0, 1], quad_perm [1, 0, 3, 2]); at each a lane keeps the half of its
+        /// values whose index bit matches its own lane bit and adds the partner's copy of that
+        /// half.  Fixed order: the result is reproducible.
+        __device__ __forceinline__ float group8_transpose_sum(const float (&v)[8], int lane)
+        {
+            auto mv = [](float x, auto ctrl) {
+                return __int_as_float(
+                    __builtin_amdgcn_update_dpp(0, __float_as_int(x), decltype(ctrl)::value, 0xF, 0xF, false));
+            };
+            const bool b2 = (lane & 4) != 0, b1 = (lane & 2) != 0, b0 = (lane & 1) != 0;
+            float k4[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const float keep = b2 ? v[j + 4] : v[j];
+                const float send = b2 ? v[j] : v[j + 4];
+                k4[j] = keep + mv(send, std::integral_constant<int, 0x141>()); // lane i <-> 7 - i
+            }
+            float k2[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+            {
+                const float keep = b1 ? k4[j + 2] : k4[j];
+                const float send = b1 ? k4[j] : k4[j + 2];
+                k2[j] = keep + mv(send, std::integral_constant<int, 0x4E>()); // lane i <-> i ^ 2
+            }
+            const float keep = b0 ? k2[1] : k2[0];
+            const float send = b0 ? k2[0] : k2[1];
+            return keep + mv(send, std::integral_constant<int, 0xB1>()); // lane i <-> i ^ 1
+        }
+
+        /// Tiles of the blocks g + 8 k (g = 0..7) of the tiled FP16 layout with NT tiles per side:
+        /// byte g holds I | J << 4 of off-diagonal tile (I, J), or p | 15 << 4 for folded diagonal
+        /// block p.  Evaluated at compile time.
+        template <int NT>
+        constexpr unsigned long long w16_tiles(int k)
+        {
+            constexpr int Q = NT * (NT - 1) / 2;
+            unsigned long long v = 0;
+            for (int g = 0; g < 8; ++g)
+            {
+                int q = g + 8 * k, I = 0, J = 15;
+                if (q < Q)
+                {
+                    int d = 1;
+                    while (q >= NT - d)
+                    {
+                        q -= NT - d;
+                        ++d;
+                    }
+                    I = q;
+                    J = q + d;
+                }
+                else
+                    I = q - Q;
+                v |= static_cast<unsigned long long>(I | (J << 4)) << (8 * g);
+            }
+            return v;
+        }
+
+        /// Calls f(std::integral_constant<int, k>()) for k = 0 .. K-1 (compile-time loop index).
+        template <int K, typename F, int... Ks>
+        __device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Ks...>)
+        {
+            (f(std::integral_constant<int, Ks>()), ...);
+        }
+        template <int K, typename F>
+        __device__ __forceinline__ void static_for(F &&f)
+        {
+            static_for_impl<K>(f, std::make_integer_sequence<int, K>());
+        }
+
+        /// Shape of the wave-per-matrix FP16 apply: N = 32 BD, NT = N / 8 tiles per side, NB = NT^2 / 2
+        /// blocks of 64 halves (Q off-diagonal tiles, then NT / 2 folded diagonal blocks), KB = NB / 8
+        /// blocks per 8-lane group; per wave the scaled input (N floats) and the partial sums
+        /// (N rows of NT slots, odd row stride PS) in LDS.
+        template <int BD>
+        struct W16Shape
+        {
+            static constexpr int N = 32 * BD;
+            static constexpr int NT = N / 8;
+            static constexpr int Q = NT * (NT - 1) / 2;
+            static constexpr int NB = NT * NT / 2;
+            static constexpr int KB = NB / 8;
+            static constexpr int PS = NT + 1;
+            static constexpr int WAVE_FLOATS = N + N * PS;
+            static constexpr int LDS_FLOATS = APPLY_W16_WAVES * WAVE_FLOATS;
+            static_assert(NB % 8 == 0 && NB == Q + NT / 2 && (N * PS) % 4 == 0, "tile layout");
+        };
+
+        /// y_m = D_m S_m D_m x_m for FP16 tiled matrices [mat_begin, mat_begin + count): a wave per
+        /// matrix, four waves per workgroup, each wave on its own (no workgroup barrier).  Lane l
+        /// loads rows l, l + 64, ... of the matrix's 16-byte tile rows in one batch (each load
+        /// instruction of the wave reads 1 KB contiguous), so that the tile stays in registers and
+        /// every resident wave keeps its whole matrix in flight; nothing is unpacked into LDS.
+        /// Lane (group g = l / 8, row r = l % 8) holds row r of blocks g, g + 8, ...:
+        ///  * off-diagonal tile (I, J), I < J: z[8I + r] += S_IJ[r][:] . x[8J ..] (its own row) and
+        ///    z[8J + c] += S_IJ[r][c] x[8I + r] (summed over the group's 8 lanes by
+        ///    group8_transpose_sum, lane c receiving column c);
+        ///  * folded diagonal block p: the strict upper triangle of tile 2p (c > r) and the strict
+        ///    lower one of tile 2p + 1 (c < r, stored transposed), each in both orientations.
+        /// Every tile contribution to row i lands in its own slot of the row's partials
+        /// (slot J for the row part of tile (I, J), slot I for the column part, slot I for the
+        /// diagonal tile), and z_i = d_i (x~_i + sum of the slots in slot order) with x~ = D x and
+        /// the unit diagonal of S: a fixed summation order.  Matrices past the first
+        /// 4 (gridDim - wgs) are walked grid-stride (apply_workgroups caps the grid).
+        template <int BD, bool COARSE>
+        __global__ __launch_bounds__(APPLY_W16_BLOCK) void k_mas_apply_w16(int mat_begin, int count,
+                                                                          const __half *__restrict__ mats,
+                                                                          const float *__restrict__ dscale,
+                                                                          const float *__restrict__ mr,
+                                                                          float *__restrict__ mz,
+                                                                          const int *__restrict__ bank_sys,
+                                                                          const int *__restrict__ active,
+                                                                          ApplySide sd)
+        {
+            using S = W16Shape<BD>;
+            constexpr int N = S::N, NT = S::NT, Q = S::Q, KB = S::KB, PS = S::PS;
+            constexpr int LEN = half_matrix_len<BD>();
+            constexpr int RPL = (N + 63) / 64; // rows of the input and output per lane
+            using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
+            __shared__ __attribute__((aligned(16))) float lds[S::LDS_FLOATS];
+            if (static_cast<int>(blockIdx.x) < sd.wgs)
+            {
+                apply_side_work<BD, float, COARSE ? 2 : 1>(sd, mr, mz, blockIdx.x, lds, S::LDS_FLOATS);
+                return;
+            }
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            float *sx = lds + wave * S::WAVE_FLOATS; // x~ = D x
+            float *sp = sx + N;                      // partials [N][PS]
+            const int end = mat_begin + count;
+            const int stride = (static_cast<int>(gridDim.x) - sd.wgs) * APPLY_W16_WAVES;
+            for (int m = mat_begin + (static_cast<int>(blockIdx.x) - sd.wgs) * APPLY_W16_WAVES + wave; m < end;
+                 m += stride) // wave-uniform
+            {
+                if (bank_sys != nullptr)
+                {
+                    const int sy = bank_sys[m]; // -1: a coarse matrix (its nodes may span systems)
+                    if (sy >= 0 && active[sy] == 0)
+                        continue;
+                }
+                // Row loads first (their wait then leaves the tile in flight), all unconditional.
+                // Split inputs and the dense coarsest rows exist only in a second launch (COARSE).
+                RawInput<float> in[RPL];
+                float d[RPL];
+                bool dn[RPL];
+                int dx[RPL];
+#pragma unroll
+                for (int j = 0; j < RPL; ++j)
+                {
+                    const int i = min(lane + 64 * j, N - 1);
+                    d[j] = dscale[static_cast<size_t>(m) * N + i];
+                    if constexpr (COARSE)
+                    {
+                        in[j] = raw_input<BD, float>(sd, mr, m * 32 + i / BD, i % BD);
+                        dense_probe(sd, m * 32 + i / BD, dn[j], dx[j]);
+                    }
+                    else
+                    {
+                        in[j].x = mr[static_cast<size_t>(m) * N + i];
+                        dn[j] = false;
+                        dx[j] = -1;
+                    }
+                }
+                const u32x4 *src = reinterpret_cast<const u32x4 *>(mats + static_cast<size_t>(m) * LEN);
+                u32x4 tile[KB];
+#pragma unroll
+                for (int k = 0; k < KB; ++k)
+                    tile[k] = __builtin_nontemporal_load(src + lane + 64 * k);
+                float xt[RPL];
+#pragma unroll
+                for (int j = 0; j < RPL; ++j)
+                {
+                    const int i = lane + 64 * j;
+                    xt[j] = (COARSE ? input_value<BD, float>(sd, in[j], i % BD) : in[j].x) * d[j];
+                    if (i < N)
+                        sx[i] = xt[j];
+                }
+                __builtin_amdgcn_wave_barrier(); // (LDS operations of one wave complete in order)
+                // The blocks' LDS addresses depend only on the lane: made opaque here so that they
+                // are computed per block, not hoisted out of the matrix loop (nine blocks' worth
+                // of addresses would stay live and cost occupancy).
+                int g = lane >> 3, r = lane & 7;
+                asm volatile("" : "+v"(g), "+v"(r));
+                static_for<KB>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    // This group's tile, from a compile-time table; blocks 8k .. 8k + 7 all lie
+                    // before the folded ones when 8k + 7 < Q.
+                    constexpr unsigned long long tab = w16_tiles<NT>(k);
+                    const int tij = static_cast<int>((tab >> (8 * g)) & 0xFFull);
+                    const int I = tij & 15, J = tij >> 4;
+                    float h[8];
+                    {
+                        __half hv[8];
+                        __builtin_memcpy(hv, &tile[k], 16);
+#pragma unroll
+                        for (int c = 0; c < 8; ++c)
+                            h[c] = __half2float(hv[c]);
+                    }
+                    if (8 * k + 7 < Q || J != 15)
+                    {
+                        const float4 x0 = *reinterpret_cast<const float4 *>(sx + 8 * J);
+                        const float4 x1 = *reinterpret_cast<const float4 *>(sx + 8 * J + 4);
+                        const float xi = sx[8 * I + r];
+                        float row = h[0] * x0.x;
+                        row = fmaf(h[1], x0.y, row);
+                        row = fmaf(h[2], x0.z, row);
+                        row = fmaf(h[3], x0.w, row);
+                        row = fmaf(h[4], x1.x, row);
+                        row = fmaf(h[5], x1.y, row);
+                        row = fmaf(h[6], x1.z, row);
+                        row = fmaf(h[7], x1.w, row);
+                        float v[8];
+#pragma unroll
+                        for (int c = 0; c < 8; ++c)
+                            v[c] = h[c] * xi;
+                        const float col = group8_transpose_sum(v, lane);
+                        sp[(8 * I + r) * PS + J] = row;
+                        sp[(8 * J + r) * PS + I] = col;
+                    }
+                    if (8 * k + 7 >= Q && J == 15)
+                    {
+                        const int p = I, b = 16 * p;
+                        const float4 a0 = *reinterpret_cast<const float4 *>(sx + b);
+                        const float4 a1 = *reinterpret_cast<const float4 *>(sx + b + 4);
+                        const float4 c0 = *reinterpret_cast<const float4 *>(sx + b + 8);
+                        const float4 c1 = *reinterpret_cast<const float4 *>(sx + b + 12);
+                        const float xa[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+                        const float xb[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+                        const float xar = sx[b + r], xbr = sx[b + 8 + r];
+                        float row1 = 0.0f, row2 = 0.0f, v1[8], v2[8];
+#pragma unroll
+                        for (int c = 0; c < 8; ++c)
+                        {
+                            // c > r: S(b + r, b + c) of tile 2p; c < r: S(b + 8 + c, b + 8 + r) of tile 2p + 1
+                            const bool up = c > r, lo = c < r;
+                            row1 = fmaf(up ? h[c] : 0.0f, xa[c], row1);
+                            row2 = fmaf(lo ? h[c] : 0.0f, xb[c], row2);
+                            v1[c] = up ? h[c] * xar : 0.0f;
+                            v2[c] = lo ? h[c] * xbr : 0.0f;
+                        }
+                        const float col1 = group8_transpose_sum(v1, lane);
+                        const float col2 = group8_transpose_sum(v2, lane);
+                        sp[(b + r) * PS + 2 * p] = row1 + col1;
+                        sp[(b + 8 + r) * PS + 2 * p + 1] = row2 + col2;
+                    }
+                    // One block at a time: interleaving the blocks' LDS reads and products costs
+                    // registers (occupancy) and gains nothing once the tile has arrived.
+                    __builtin_amdgcn_sched_barrier(0);
+                });
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int j = 0; j < RPL; ++j)
+                {
+                    const int i = lane + 64 * j;
+                    if (i < N)
+                    {
+                        float acc = xt[j];
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+                            acc += sp[i * PS + t];
+                        if (!(dn[j] && dx[j] >= 0))
+                            mz[static_cast<size_t>(m) * N + i] = acc * d[j];
+                    }
+                }
+                __builtin_amdgcn_wave_barrier(); // sx and sp are rewritten for the next matrix
+            }
+        }
+
         /// y_m = Inv_m x_m for FP64 packed matrices [mat_begin, mat_begin + count), grid-stride:
         /// each workgroup takes every nwg-th matrix.  The packed upper matrix moves global ->
         /// registers -> LDS; the next matrix's loads are issued before the current one is
@@ -1667,7 +1943,24 @@ namespace psmas
     }
             if (cl.copies > 8)
                 throw std::logic_error("[MAS] internal error: at most 8 right-hand sides per factorisation.");
-            if (cl.copies <= 1)
+            if (side.wave_apply && cl.copies > 1)
+                throw std::logic_error("[MAS] internal error: the wave apply takes one right-hand side.");
+            if (side.wave_apply)
+            {
+                // A wave per matrix, every matrix's workgroup launched at once (the hardware keeps
+                // as many resident as fit); apply_cap makes waves walk matrices grid-stride.
+                int apply_wgs = div_up(count > 0 ? count : 0, APPLY_W16_WAVES);
+                if (side.apply_cap > 0 && side.apply_cap < apply_wgs)
+                    apply_wgs = side.apply_cap;
+                // A second launch (split inputs, dense rows, network layer 1) compiles that work in.
+                if (side.split_of != nullptr || side.dense_nd > 0 || side.lin_rows > 0)
+                    hipLaunchKernelGGL((k_mas_apply_w16<BD, true>), dim3(side.wgs + apply_wgs), dim3(APPLY_W16_BLOCK),
+                                       0, s, mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side);
+                else
+                    hipLaunchKernelGGL((k_mas_apply_w16<BD, false>), dim3(side.wgs + apply_wgs), dim3(APPLY_W16_BLOCK),
+                                       0, s, mat_begin, count, mats, dscale, mr, mz, bank_sys, active, side);
+            }
+            else if (cl.copies <= 1)
                 PSMAS_H16_LAUNCH(1)
             else if (cl.copies <= 2)
                 PSMAS_H16_LAUNCH(2)
@@ -1686,11 +1979,15 @@ namespace psmas
         PSMAS_LAUNCH_CHECK();
     }
 
-    int apply_h16_stage_floats(int bd)
+    int apply_h16_stage_floats(int bd, bool wave)
     {
+        if (wave)
+            return bd == 1 ? W16Shape<1>::LDS_FLOATS : (bd == 2 ? W16Shape<2>::LDS_FLOATS : W16Shape<3>::LDS_FLOATS);
         const int n = 32 * bd;
         return n * (n + 8) / 2; // the __half LDS copy of one matrix, as floats
     }
+
+    int apply_threads(int bd, bool wave) { return wave ? APPLY_W16_BLOCK : 64 * bd; }
 
     template <int BD, typename T>
     void launch_prolong_dot(int nblocks, int level_num, const int4 *cmap, const int *lev_off, const T *r, T *mz,

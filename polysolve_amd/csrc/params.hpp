@@ -50,6 +50,10 @@
 //                        (0, the default: as many as are resident at once); each walks its
 //                        matrices grid-stride, so a small cap makes every workgroup take many
 //                        matrices (parity tests of that loop on small systems)
+//   apply_wave           the FP16 apply of one right-hand side takes a wave per bank matrix with
+//                        the matrix's tile rows in registers (default true); false: the persistent
+//                        kernel that unpacks each matrix into LDS (the one copies of a
+//                        factorisation always use)
 //   division_guard       alpha/beta are set to 0 when |denominator| < division_guard; 0 (default)
 //                        sets them to 0 only for a non-finite quotient; 1e-20 is the reference's
 //                        absolute rule (MASSolver.cu:43-56)
@@ -110,6 +114,7 @@ namespace psmas
         bool spmv_fuse_direction = false;
         int restrict_piece = 512;
         int apply_workgroups = 0;
+        bool apply_wave = true;
         bool learned_linear = false;
     };
 

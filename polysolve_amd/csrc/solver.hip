@@ -185,7 +185,7 @@ namespace psmas
             before.coarse_solver != params_.coarse_solver || before.spmv_unroll != params_.spmv_unroll ||
             before.spmv_halo != params_.spmv_halo || before.spmv_copies != params_.spmv_copies ||
             before.spmv_fuse_direction != params_.spmv_fuse_direction ||
-            before.apply_workgroups != params_.apply_workgroups)
+            before.apply_workgroups != params_.apply_workgroups || before.apply_wave != params_.apply_wave)
             graphs_->clear();
     }
 
@@ -2145,7 +2145,10 @@ namespace psmas
         // the learned correction on the block path, the lattice gather of the block partials the
         // residual update left) in its leading workgroups; launch 2: banks of segments 2.. and the
         // dense coarsest solve; then the network.
-        constexpr int block = 64 * BD;
+        // One right-hand side: a wave per bank matrix (k_mas_apply_w16); copies of one
+        // factorisation: the persistent kernel that unpacks each stored matrix once for all.
+        const bool wave = params_.apply_wave && copies_ == 1;
+        const int block = apply_threads(BD, wave);
         const int n1 = first_apply_stored(); // (the FP16 apply walks stored matrices)
         const double per_mat = dim * dim * 1.0 + dim * 4.0 + 2.0 * dim * 4.0 * copies_;
         const bool lat_blocks = learned && lcc_->block_restriction();
@@ -2177,16 +2180,18 @@ namespace psmas
         const bool batch = sys_.size() > 1 && copies_ == 1;
         const int *bsys = batch ? d_bank_sys_.data() : nullptr;
         const int *bact = batch ? d_active_.data() : nullptr;
+        ApplySide rs = restrict_side(d_mr32_.data() + static_cast<size_t>(lev_off_[1]) * BD, d_mr32_.data(), block,
+                                     lat_blocks);
+        rs.wave_apply = wave;
         launch_mas_apply<BD, __half, float>(0, n1, d_mats16_.data(), d_dscale_.data(), d_mr32_.data(),
-                                            d_mz32_.data(), bsys, bact, stream_,
-                                            restrict_side(d_mr32_.data() + static_cast<size_t>(lev_off_[1]) * BD,
-                                                          d_mr32_.data(), block, lat_blocks),
-                                            cl_);
+                                            d_mz32_.data(), bsys, bact, stream_, rs, cl_);
         toc();
         // The network's first layer and skip layer join the second launch's side work when they
         // fit; the output layer follows it.
         ApplySide cs = coarse_side(block);
-        const bool lin_side = learned && lcc_->in_side(cs, apply_h16_stage_floats(BD), block, d_active_.data());
+        cs.wave_apply = wave;
+        const bool lin_side =
+            learned && lcc_->in_side(cs, apply_h16_stage_floats(BD, wave), block, d_active_.data());
         if (stored_mats_ > n1 || cs.wgs > 0)
         {
             tic("mas_apply_fp16_coarse", (stored_mats_ - n1) * per_mat + static_cast<double>(dense_nd_) * dense_nd_ * 8.0);
