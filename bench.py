@@ -412,6 +412,8 @@ def main() -> int:
                     help="coarsest-level solver of the mixed path")
     ap.add_argument("--profile-only", action="store_true", help="time steps only (for rocprof runs)")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (no HIP graph replay)")
+    ap.add_argument("--pcg", default="classic", choices=["single", "classic"],
+                    help="mixed-path PCG iteration: the classic two-reduction one (default) or the single-reduction one")
     ap.add_argument("--max-iter", type=int, default=20000, help="iteration cap (short counter-collection runs)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction path")
     ap.add_argument("--profile-learned", action="store_true",
@@ -447,7 +449,8 @@ def main() -> int:
 
     tol = 1e-8
     base = {"block_dim": 3, "relative_tolerance": tol, "absolute_tolerance": 0.0, "max_iter": args.max_iter,
-            "aggregation": args.aggregation, "use_graphs": not args.no_graphs}
+            "aggregation": args.aggregation, "use_graphs": not args.no_graphs,
+            "single_reduction": args.pcg == "single"}
     solver = MASSolver()
     solver.set_parameters({"MAS": dict(base, precision="fp64")})
     solver.set_coordinates(prob.coords)

@@ -112,6 +112,51 @@ namespace psmas
             partials[block] = s;
     }
 
+    /// System of node block blk (SysMap; 0 for a single system, without a load).
+    __device__ __forceinline__ int system_of_block(const SysMap &sm, int blk)
+    {
+        return sm.single_blocks > 0 ? 0 : sm.block_sys[blk];
+    }
+
+    /// Whether system sys has stopped iterating (never, while a single system's kernels run).
+    __device__ __forceinline__ bool system_stopped(const SysMap &sm, int sys)
+    {
+        return sm.single_blocks == 0 && sm.active[sys] == 0;
+    }
+
+    /// NV workgroup sums at once -> parts[v][block] (parts[v] null: not stored); one barrier.
+    /// Each value equals store_partial's for the same v bit for bit (same wave and wave order).
+    template <int BLOCK, int NV>
+    __device__ __forceinline__ void store_partials(const double (&v)[NV], double *const (&parts)[NV], int block)
+    {
+        static_assert(BLOCK % WAVE == 0 && BLOCK <= 1024, "block size");
+        constexpr int NW = BLOCK / WAVE;
+        __shared__ double part[NV][NW];
+        const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+        double t[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+        {
+            t[k] = wave_sum(v[k]);
+            if (lane == 0)
+                part[k][w] = t[k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+            {
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < NW; ++i)
+                    s += part[k][i];
+                if (parts[k] != nullptr)
+                    parts[k][block] = NW == 1 ? t[k] : s;
+            }
+        }
+    }
+
     /// Sum of the partials of system sys's node blocks, in block order, returned to every thread.
     /// The partials come from an earlier kernel; every workgroup computes the same value bit
     /// for bit.
@@ -119,9 +164,22 @@ namespace psmas
     __device__ __forceinline__ double system_total(const double *__restrict__ partials, int sys, const SysMap &sm)
     {
         __shared__ double total;
-        const int b0 = sm.blk_begin[sys], nb = sm.blk_count[sys];
+        const int b0 = sm.single_blocks > 0 ? 0 : sm.blk_begin[sys];
+        const int nb = sm.single_blocks > 0 ? sm.single_blocks : sm.blk_count[sys];
+        // The first PRE strides are loaded in one batch (one cache round trip instead of
+        // one per stride; the partials come from another kernel, often from another XCD's
+        // cache), then added in stride order.
+        constexpr int PRE = 8;
+        double pv[PRE];
+#pragma unroll
+        for (int k = 0; k < PRE; ++k)
+            pv[k] = partials[b0 + min(static_cast<int>(threadIdx.x) + k * BLOCK, nb - 1)];
         double t = 0.0;
-        for (int i = threadIdx.x; i < nb; i += BLOCK)
+#pragma unroll
+        for (int k = 0; k < PRE; ++k)
+            if (static_cast<int>(threadIdx.x) + k * BLOCK < nb)
+                t += pv[k];
+        for (int i = threadIdx.x + PRE * BLOCK; i < nb; i += BLOCK)
             t += partials[b0 + i];
         t = block_sum<BLOCK>(t);
         if (threadIdx.x == 0)
@@ -141,12 +199,32 @@ namespace psmas
         static_assert(BLOCK % WAVE == 0 && BLOCK <= 1024, "block size");
         constexpr int NW = BLOCK / WAVE;
         __shared__ double part[NV][NW];
-        const int b0 = sm.blk_begin[sys], nb = sm.blk_count[sys];
+        const int b0 = sm.single_blocks > 0 ? 0 : sm.blk_begin[sys];
+        const int nb = sm.single_blocks > 0 ? sm.single_blocks : sm.blk_count[sys];
+        // The first PRE strides of every array are loaded in one batch (one cache round trip
+        // instead of one per stride: the partials come from another kernel, often from another
+        // XCD's cache), then added in stride order; further strides follow one by one.
+        constexpr int PRE = NV <= 2 ? 8 : 5;
+        double pv[NV][PRE];
+#pragma unroll
+        for (int k = 0; k < PRE; ++k)
+        {
+            const int i = b0 + min(static_cast<int>(threadIdx.x) + k * BLOCK, nb - 1);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                pv[v][k] = parts[v][i];
+        }
         double t[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v)
             t[v] = 0.0;
-        for (int i = threadIdx.x; i < nb; i += BLOCK)
+#pragma unroll
+        for (int k = 0; k < PRE; ++k)
+            if (static_cast<int>(threadIdx.x) + k * BLOCK < nb)
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    t[v] += pv[v][k];
+        for (int i = threadIdx.x + PRE * BLOCK; i < nb; i += BLOCK)
 #pragma unroll
             for (int v = 0; v < NV; ++v)
                 t[v] += parts[v][b0 + i];

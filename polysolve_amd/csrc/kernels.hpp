@@ -70,7 +70,11 @@ namespace psmas
         R_RZ = 2,
         R_RR_TRUE = 3,
         R_RZ_OLD = 4,
-        R_COUNT = 5,
+        R_DLT = 5,    ///< single-reduction PCG: u . A u
+        R_EPS = 6,    ///< single-reduction PCG: u . s_prev
+        R_PS = 7,     ///< single-reduction PCG: p . s of the update
+        R_PS_OLD = 8, ///< single-reduction PCG: the previous p . s
+        R_COUNT = 9,
     };
 
     /// Per-slot lattice data and the coarse coefficients, as the prolongation kernel reads them.
@@ -114,6 +118,10 @@ namespace psmas
         const int *active = nullptr;    ///< system -> 1 while it iterates
         const int *select = nullptr;    ///< system -> 1 when picked for a reliable update
         const int *bank_sys = nullptr;  ///< fine bank -> system
+        /// > 0: one system (not a batch) of this many node blocks.  Its kernels are launched only
+        /// while it iterates, so they skip the block -> system and activity lookups (two dependent
+        /// loads ahead of all other work) and its block range needs no lookup either.
+        int single_blocks = 0;
     };
 
     // ----- SpMV ---------------------------------------------------------------------------
@@ -157,6 +165,49 @@ namespace psmas
     template <int BD, typename TV, typename TX>
     void launch_spmv_ap_copies(const SellView<TV> &A, int copies, int npad1, const TX *x, TX *y, const Reductions &red,
                                const SysMap &sm, hipStream_t s);
+
+    // ----- single-reduction PCG (Chronopoulos & Gear 1989), mixed path ---------------------
+    /// Partials of one iteration of the single-reduction PCG.  The SpMV kernel writes
+    /// gamma = r . u (rz), delta = u . A u (dlt), eps = u . s_prev (eps) and, flexible, zeta =
+    /// r . u_prev (zeta), after moving its block's previous rz and ps to rz_old and ps_old; the
+    /// update kernel writes r . r (rr) and p . s (ps).
+    struct CgReductions
+    {
+        double *rz = nullptr, *rz_old = nullptr, *dlt = nullptr, *eps = nullptr;
+        double *ps = nullptr, *ps_old = nullptr, *zeta = nullptr, *rr = nullptr;
+        double *sc = nullptr;
+    };
+
+    /// The prolongation the single-reduction SpMV performs where it reads its input:
+    /// u_p = z_p + sum_l z_l[cmap_p.l] (+ P c), in FP64 and in level order as
+    /// launch_prolong_dot, from the multi-level z (mz).  own[b]: position of node block b's
+    /// first own node in its halo list (every own node is in it, in order).
+    struct CgProlong
+    {
+        const float *mz = nullptr;
+        const int4 *cmap = nullptr;
+        const int *lev_off = nullptr;
+        int level_num = 0;
+        const int *own = nullptr;
+        LccApply lcc;
+    };
+
+    /// u = prolongation of z (CgProlong) at every node the block's rows read, w = A u (FP32 SELL,
+    /// halo form), u written for the own rows; partials gamma = r . u, delta = w . u and, unless
+    /// init, eps = u . s_prev and (u_prev set) zeta = r . u_prev.
+    template <int BD>
+    void launch_spmv_cg(const SellView<float> &A, const CgProlong &pr, float *u, float *w, const float *r,
+                        const float *s_prev, const float *u_prev, const CgReductions &red, const SysMap &sm,
+                        bool init, hipStream_t s);
+
+    /// beta = gamma / gamma_old (flexible, u_prev set: (gamma - zeta) / gamma_old; init: 0),
+    /// p.s = delta + 2 beta eps + beta^2 ps_old (init: delta), alpha = gamma / p.s;
+    /// p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; partials of r . r and p . s;
+    /// restrict r -> rc0; u_prev = u when set; with lb the lattice block partials of r.
+    template <int BD>
+    void launch_update_cg(int nblocks, const float *u, const float *w, float *p, float *s_vec, float *x, float *r,
+                          FineRestrict fr, float *rc0, float *u_prev, const CgReductions &red, const SysMap &sm,
+                          bool init, hipStream_t s, const LatticeBlocks *lb = nullptr);
 
     // ----- vectors (node-blocked: nblocks workgroups of NODE_BLOCK nodes) ------------------
     /// alpha = rz / pap; x += alpha p; r -= alpha Ap; partials of r . r -> red.rr; restrict r -> rc0;

@@ -1232,8 +1232,8 @@ namespace psmas
                                                                     Reductions red, bool init, SysMap sm,
                                                                     LccApply lcc, FlexDot<T> flex)
         {
-            const int sys = sm.block_sys[blockIdx.x];
-            if (sm.active[sys] == 0)
+            const int sys = system_of_block(sm, blockIdx.x);
+            if (system_stopped(sm, sys))
                 return;
             if (!init && threadIdx.x == 0)
                 red.rz_old[blockIdx.x] = red.rz[blockIdx.x];

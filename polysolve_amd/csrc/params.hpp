@@ -50,6 +50,12 @@
 //                        (0, the default: as many as are resident at once); each walks its
 //                        matrices grid-stride, so a small cap makes every workgroup take many
 //                        matrices (parity tests of that loop on small systems)
+//   single_reduction     the mixed path's PCG iteration with one reduction phase (Chronopoulos &
+//                        Gear 1989): the SpMV multiplies u = M r, which it assembles itself from
+//                        the preconditioner's levels where it reads it, and one update kernel
+//                        forms p, s = A p (by recurrence), x and r (default false; used for one
+//                        right-hand side or a batch, with the halo form of the matrix; false: the
+//                        classic iteration with separate prolongation and direction kernels)
 //   apply_wave           the FP16 apply of one right-hand side takes a wave per bank matrix with
 //                        the matrix's tile rows in registers (default true); false: the persistent
 //                        kernel that unpacks each matrix into LDS (the one copies of a
@@ -115,6 +121,7 @@ namespace psmas
         int restrict_piece = 512;
         int apply_workgroups = 0;
         bool apply_wave = true;
+        bool single_reduction = false;
         bool learned_linear = false;
     };
 

@@ -93,7 +93,7 @@ namespace psmas
     {
         /// Graph cache keys: precision path, iteration pattern, iterations per graph, and flags
         /// (bit 0: fused direction update, bit 1: direction buffer at the segment's start, bit 2:
-        /// the segment's first SpMV starts the direction from z).
+        /// the segment's first SpMV starts the direction from z, bit 3: single-reduction iteration).
         long long graph_key(int path, int pattern, int count, int flags = 0)
         {
             return (static_cast<long long>(path) << 56) | (static_cast<long long>(flags) << 48) |
@@ -185,7 +185,8 @@ namespace psmas
             before.coarse_solver != params_.coarse_solver || before.spmv_unroll != params_.spmv_unroll ||
             before.spmv_halo != params_.spmv_halo || before.spmv_copies != params_.spmv_copies ||
             before.spmv_fuse_direction != params_.spmv_fuse_direction ||
-            before.apply_workgroups != params_.apply_workgroups || before.apply_wave != params_.apply_wave)
+            before.apply_workgroups != params_.apply_workgroups || before.apply_wave != params_.apply_wave ||
+            before.single_reduction != params_.single_reduction)
             graphs_->clear();
     }
 
@@ -302,6 +303,8 @@ namespace psmas
           << ", \"coarse_solver\": \"" << (dense_nd_ > 0 ? dense_note_ : std::string("bank")) << "\""
           << ", \"factorize_mode\": \"" << factorize_mode_ << "\""
           << ", \"copies\": " << copies_
+          << ", \"pcg_iteration\": \""
+          << (params_.precision == Precision::Mixed && single_reduction() ? "single_reduction" : "classic") << "\""
           << ", \"setup_ms\": {" << [&] {
                  std::ostringstream q;
                  q.precision(6);
@@ -369,6 +372,29 @@ namespace psmas
         return r;
     }
 
+    CgReductions Solver::cg_reductions() const
+    {
+        CgReductions c;
+        double *const base = use_scratch_ ? scratch_ + SCRATCH_SC_PAD : d_partials_.data();
+        auto site = [&](int i) { return base + static_cast<size_t>(i) * partial_cap_; };
+        c.rz = site(R_RZ);
+        c.rz_old = site(R_RZ_OLD);
+        c.dlt = site(R_DLT);
+        c.eps = site(R_EPS);
+        c.ps = site(R_PS);
+        c.ps_old = site(R_PS_OLD);
+        c.zeta = rzo_ptr();
+        c.rr = site(R_RR);
+        c.sc = sc_ptr();
+        return c;
+    }
+
+    bool Solver::single_reduction() const
+    {
+        return params_.single_reduction && !params_.spmv_fuse_direction && params_.spmv_halo && halo_max_ > 0 &&
+               halo_own_ok_ && copies_ == 1 && level_num_ <= 4;
+    }
+
     void Solver::finalize_scalars()
     {
         launch_finalize(static_cast<int>(sys_.size()), reductions(), sysmap(), stream_);
@@ -383,6 +409,8 @@ namespace psmas
         m.active = d_active_.data();
         m.select = d_select_.data();
         m.bank_sys = d_bank_sys_.data();
+        // One system, not copies of one: its kernels skip the system lookups.
+        m.single_blocks = sys_.size() == 1 && copies_ == 1 ? nblocks_ : 0;
         return m;
     }
 
@@ -1268,9 +1296,11 @@ namespace psmas
     {
         halo_max_ = 0;
         halo_total_ = 0;
+        halo_own_ok_ = false;
         d_lcols_.reset();
         d_halo_ptr_.reset();
         d_halo_.reset();
+        d_halo_own_.reset();
         constexpr int SLICES = NODE_BLOCK / WAVE;
         const int nblk = npad_ / NODE_BLOCK;
         std::vector<std::vector<int>> lists(nblk);
@@ -1331,6 +1361,23 @@ namespace psmas
         std::vector<int> all(static_cast<size_t>(ptr[nblk]));
         for (int b = 0; b < nblk; ++b)
             std::copy(lists[b].begin(), lists[b].end(), all.begin() + ptr[b]);
+        // Where each block's own nodes sit in its (ascending) list: every row reads its own node
+        // through its diagonal block (padded rows point at themselves), so they form one run.
+        std::vector<int> own(static_cast<size_t>(std::max(nblk, 1)), 0);
+        bool own_ok = true;
+        for (int b = 0; b < nblk && own_ok; ++b)
+        {
+            const std::vector<int> &u = lists[b];
+            const int first = b * NODE_BLOCK;
+            const auto it = std::lower_bound(u.begin(), u.end(), first);
+            const int pos = static_cast<int>(it - u.begin());
+            own[b] = pos;
+            if (pos + NODE_BLOCK > static_cast<int>(u.size()))
+                own_ok = false;
+            else
+                for (int t = 0; t < NODE_BLOCK && own_ok; ++t)
+                    own_ok = u[pos + t] == first + t;
+        }
         d_lcols_.allocate(lcols.size(), "MAS SELL local columns");
         d_halo_ptr_.allocate(ptr.size(), "MAS SpMV halo lists");
         d_halo_.allocate(std::max<size_t>(1, all.size()), "MAS SpMV halo lists");
@@ -1338,6 +1385,12 @@ namespace psmas
         PSMAS_HIP(hipMemcpy(d_halo_ptr_.data(), ptr.data(), d_halo_ptr_.bytes(), hipMemcpyHostToDevice));
         if (!all.empty())
             PSMAS_HIP(hipMemcpy(d_halo_.data(), all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice));
+        if (own_ok)
+        {
+            d_halo_own_.allocate(own.size(), "MAS SpMV halo own positions");
+            PSMAS_HIP(hipMemcpy(d_halo_own_.data(), own.data(), d_halo_own_.bytes(), hipMemcpyHostToDevice));
+        }
+        halo_own_ok_ = own_ok;
         halo_max_ = mx;
         halo_total_ = static_cast<long long>(all.size());
     }
@@ -1616,6 +1669,8 @@ namespace psmas
         d_p32_.reset();
         d_p32b_.reset();
         d_Ap32_.reset();
+        d_u32_.reset();
+        d_s32_.reset();
         const int K = static_cast<int>(sys_.size());
         d_sc_.allocate(static_cast<size_t>(S_COUNT) * K, "MAS device scalars");
         d_sc_.zero(stream_);
@@ -1654,6 +1709,11 @@ namespace psmas
             d_mz32_.allocate(nml, "MAS coarse_vectors z32");
             d_mr32_.zero(stream_);
             d_mz32_.zero(stream_);
+        }
+        if (!d_u32_)
+        {
+            d_u32_.allocate(nvec, "device_buffers u32");
+            d_s32_.allocate(nvec, "device_buffers s32");
         }
     }
 
@@ -1763,7 +1823,7 @@ namespace psmas
         std::string missing;
         if (params_.coarse_solver == CoarseSolver::Cholesky && !dense_built_)
             missing = "the dense coarsest-level solve";
-        if (params_.precision == Precision::Mixed && (!d_vals32_ || !d_mats16_ || !d_x32_))
+        if (params_.precision == Precision::Mixed && (!d_vals32_ || !d_mats16_ || !d_x32_ || !d_u32_))
             missing = "the FP32/FP16 data of the mixed path";
         if (params_.precision == Precision::Mixed && params_.learned_correction && lcc_ && lcc_->trained() &&
             (lattice_stale_ || !lcc_->lattice_ready() || lcc_->k() != params_.lattice_points ||
@@ -2140,6 +2200,75 @@ namespace psmas
     void Solver::mixed_part_b(bool init, bool learned)
     {
         const long long nvec = static_cast<long long>(npad_) * BD;
+        const LccApply lv = mixed_apply<BD>(init, learned);
+        FlexDot<float> flex;
+        if (learned && !init)
+        {
+            flex.zprev = d_zprev32_.data();
+            flex.rzo = rzo_ptr();
+        }
+        tic(learned ? "prolong_dot_learned" : "prolong_dot", static_cast<double>(n_total_) * 4.0 * 3 + nblocks_ * 256.0 * 16);
+        launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
+                                      d_mz32_.data(), reductions(), init, sysmap(), lv, flex, stream_);
+        toc();
+        if (fuse_direction())
+            return; // the next SpMV forms p (and z_prev)
+        if (init)
+        {
+            launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_p32_.data(), stream_);
+            if (learned)
+                launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_zprev32_.data(), stream_);
+        }
+        else if (learned)
+        {
+            launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), reductions(), sysmap(), stream_,
+                                       rzo_ptr(), d_zprev32_.data());
+        }
+        else
+        {
+            launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), reductions(), sysmap(), stream_);
+        }
+    }
+
+    // ---- mixed path, single-reduction iteration ------------------------------------------
+    // One iteration: update (alpha, beta from one set of partials; p, s, x, r) | apply launches
+    // (and the network) | SpMV of u = M r, which assembles u from the levels where it reads it.
+    template <int BD>
+    void Solver::cg_part_b(bool init, bool learned)
+    {
+        const LccApply lv = mixed_apply<BD>(init, learned);
+        CgProlong pr;
+        pr.mz = d_mz32_.data();
+        pr.cmap = d_cmap_.data();
+        pr.lev_off = d_lev_off_.data();
+        pr.level_num = level_num_;
+        pr.own = d_halo_own_.data();
+        pr.lcc = lv;
+        const SellView<float> A = sell(d_vals32_.data());
+        tic(learned ? "spmv_cg_learned" : "spmv_cg",
+            spmv_bytes(4, 4, 2) + static_cast<double>(n_total_) * 4.0 * (init ? 2.0 : (learned ? 4.0 : 3.0)));
+        launch_spmv_cg<BD>(A, pr, d_u32_.data(), d_Ap32_.data(), d_mr32_.data(), d_s32_.data(),
+                           learned ? d_zprev32_.data() : nullptr, cg_reductions(), sysmap(), init, stream_);
+        toc();
+    }
+
+    template <int BD>
+    void Solver::cg_part_a(bool learned, bool init)
+    {
+        const LatticeBlocks lb = learned && lcc_ ? lcc_->block_view() : LatticeBlocks{};
+        tic(lb.cell ? "update_cg_lattice" : "update_cg",
+            static_cast<double>(n_total_) * 4.0 * (init ? 8.0 : 10.0) + (learned ? n_total_ * 4.0 : 0.0));
+        launch_update_cg<BD>(nblocks_, d_u32_.data(), d_Ap32_.data(), d_p32_.data(), d_s32_.data(), d_x32_.data(),
+                             d_mr32_.data(), fine_restrict(), d_mr32_.data() + static_cast<size_t>(npad_) * BD,
+                             learned ? d_zprev32_.data() : nullptr, cg_reductions(), sysmap(), init, stream_, &lb);
+        toc();
+    }
+
+    /// Both apply launches (and the network) on the current r: mz holds every level's z, and the
+    /// returned view carries the learned correction's coefficients when it is used.
+    template <int BD>
+    LccApply Solver::mixed_apply(bool init, bool learned)
+    {
         const int dim = BANK * BD;
         // Apply launch 1: fine and segment-1 banks, with the restriction of segments 2.. (and, for
         // the learned correction on the block path, the lattice gather of the block partials the
@@ -2153,7 +2282,6 @@ namespace psmas
         const double per_mat = dim * dim * 1.0 + dim * 4.0 + 2.0 * dim * 4.0 * copies_;
         const bool lat_blocks = learned && lcc_->block_restriction();
         LccApply lv;
-        FlexDot<float> flex;
         if (learned)
         {
             if (lat_blocks)
@@ -2167,12 +2295,8 @@ namespace psmas
                 toc();
             }
             lv = lcc_->apply_view();
-            if (!init)
-            {
-                flex.zprev = d_zprev32_.data();
-                flex.rzo = rzo_ptr();
-            }
         }
+        (void)init;
         tic("mas_apply_fp16", n1 * per_mat);
         // A single system is active for the whole apply: no per-matrix system lookups (two dependent
         // loads per matrix on each workgroup's critical path).  Copies of one factorisation apply
@@ -2208,27 +2332,7 @@ namespace psmas
                 lcc_->infer(d_active_.data(), stream_);
             toc();
         }
-        tic(learned ? "prolong_dot_learned" : "prolong_dot", static_cast<double>(n_total_) * 4.0 * 3 + nblocks_ * 256.0 * 16);
-        launch_prolong_dot<BD, float>(nblocks_, level_num_, d_cmap_.data(), d_lev_off_.data(), d_mr32_.data(),
-                                      d_mz32_.data(), reductions(), init, sysmap(), lv, flex, stream_);
-        toc();
-        if (fuse_direction())
-            return; // the next SpMV forms p (and z_prev)
-        if (init)
-        {
-            launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_p32_.data(), stream_);
-            if (learned)
-                launch_copy_convert<float, float>(nvec, d_mz32_.data(), d_zprev32_.data(), stream_);
-        }
-        else if (learned)
-        {
-            launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), reductions(), sysmap(), stream_,
-                                       rzo_ptr(), d_zprev32_.data());
-        }
-        else
-        {
-            launch_update_p<BD, float>(nblocks_, d_mz32_.data(), d_p32_.data(), reductions(), sysmap(), stream_);
-        }
+        return lv;
     }
 
     template <int BD>
@@ -2293,7 +2397,20 @@ namespace psmas
             launch_fill<float>(nvec, d_x32_.data(), 0.0f, stream_);
             mixed_reliable_residual<BD>(learned); // every system selected: r = b
             p_cur_ = 0;
-            mixed_part_b<BD>(true, learned);
+            const bool cg = single_reduction();
+            auto part_a = [&](bool lrn, bool init) {
+                if (cg)
+                    cg_part_a<BD>(lrn, init);
+                else
+                    mixed_part_a<BD>(lrn, init);
+            };
+            auto part_b = [&](bool init, bool lrn) {
+                if (cg)
+                    cg_part_b<BD>(init, lrn);
+                else
+                    mixed_part_b<BD>(init, lrn);
+            };
+            part_b(true, learned);
             finalize_scalars();
             read_scalars();
             std::vector<double> bnorm(K), target(K), ref(K), best(K), seg_best(K);
@@ -2329,7 +2446,7 @@ namespace psmas
                 if (restart_plain)
                 {
                     // Restart from the current residual with plain MAS: z = M r, p = z.
-                    mixed_part_b<BD>(true, false);
+                    part_b(true, false);
                     first = true;
                     restart_plain = false;
                 }
@@ -2339,18 +2456,19 @@ namespace psmas
                 // set from the segment's start either way.
                 const bool fuse = fuse_direction();
                 const int par0 = p_cur_;
-                run_block(graph_key(learned ? 2 : 1, first ? 1 : 2, count, (fuse ? 1 : 0) | (par0 << 1)), [&]() {
-                    p_cur_ = par0;
-                    if (lead_b)
-                        mixed_part_b<BD>(false, learned);
-                    for (int i = 0; i < count; ++i)
-                    {
-                        mixed_part_a<BD>(learned, first && i == 0);
-                        if (i + 1 < count)
-                            mixed_part_b<BD>(false, learned);
-                    }
-                    finalize_scalars();
-                });
+                run_block(graph_key(learned ? 2 : 1, first ? 1 : 2, count, (fuse ? 1 : 0) | (par0 << 1) | (cg ? 8 : 0)),
+                          [&]() {
+                              p_cur_ = par0;
+                              if (lead_b)
+                                  part_b(false, learned);
+                              for (int i = 0; i < count; ++i)
+                              {
+                                  part_a(learned, first && i == 0);
+                                  if (i + 1 < count)
+                                      part_b(false, learned);
+                              }
+                              finalize_scalars();
+                          });
                 p_cur_ = fuse ? par0 ^ (count & 1) : 0;
                 first = false;
                 k += count;

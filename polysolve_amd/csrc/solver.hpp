@@ -188,6 +188,10 @@ namespace psmas
         // lists; halo_max_ = 0 when some block reads more than SPMV_HALO_CAP distinct nodes.
         DeviceBuffer<unsigned short> d_lcols_;
         DeviceBuffer<int> d_halo_ptr_, d_halo_;
+        /// Per node block: position of its first own node in its halo list, when every block's
+        /// own nodes appear there in order (halo_own_ok_; the single-reduction SpMV needs it).
+        DeviceBuffer<int> d_halo_own_;
+        bool halo_own_ok_ = false;
         int halo_max_ = 0;
         long long halo_total_ = 0;
         DeviceBuffer<double> d_vals64_;
@@ -266,6 +270,10 @@ namespace psmas
         // alternates between the two (other workgroups still read the previous p).
         DeviceBuffer<double> d_pb_;
         DeviceBuffer<float> d_p32b_;
+        // Single-reduction PCG (mixed path): u = M r assembled by the SpMV, and s = A p by recurrence.
+        DeviceBuffer<float> d_u32_, d_s32_;
+        bool single_reduction() const;
+        CgReductions cg_reductions() const;
         int p_cur_ = 0; ///< which buffer holds the current p (0: d_p_ / d_p32_)
         bool fuse_direction() const;
         double *p64(int which) { return which ? d_pb_.data() : d_p_.data(); }
@@ -414,6 +422,12 @@ namespace psmas
         void mixed_part_a(bool learned, bool init_direction);
         template <int BD>
         void mixed_part_b(bool init, bool learned = false);
+        template <int BD>
+        LccApply mixed_apply(bool init, bool learned);
+        template <int BD>
+        void cg_part_a(bool learned, bool init);
+        template <int BD>
+        void cg_part_b(bool init, bool learned);
         template <int BD>
         void mixed_reliable_residual(bool learned);
 

@@ -142,7 +142,7 @@ namespace psmas
             return xcd * q + (xcd < r ? xcd : r) + b / NXCD;
         }
 
-        __device__ __forceinline__ int block_system(const SysMap &sm) { return sm.block_sys[blockIdx.x]; }
+        __device__ __forceinline__ int block_system(const SysMap &sm) { return system_of_block(sm, blockIdx.x); }
 
         /// Halo stage of node block blk in two parts: halo_indices issues the loads of the first
         /// HALO_R rounds of halo node indices (clamped, unconditional), halo_fill gathers x at
@@ -207,8 +207,8 @@ namespace psmas
         {
             using TA = typename Acc<TV>::type;
             const int blk = xcd_block();
-            const int sys = sm.block_sys[blk];
-            if (sm.active[sys] == 0)
+            const int sys = system_of_block(sm, blk);
+            if (system_stopped(sm, sys))
                 return;
             const int lane = threadIdx.x % WAVE;
             const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
@@ -287,8 +287,8 @@ namespace psmas
         {
             using TA = typename Acc<TV>::type;
             const int blk = xcd_block();
-            const int sys = sm.block_sys[blk];
-            if (sm.active[sys] == 0)
+            const int sys = system_of_block(sm, blk);
+            if (system_stopped(sm, sys))
                 return;
             TX *xs = reinterpret_cast<TX *>(spmv_halo_lds);
             const int hb = A.halo_ptr[blk], hn = A.halo_ptr[blk + 1] - hb;
@@ -354,6 +354,174 @@ namespace psmas
                 dot += static_cast<double>(pn) * static_cast<double>(yi);
             }
             store_partial<SPMV_BLOCK>(dot, red.pap, blk);
+        }
+
+        /// u_g of node g from its preloaded coarse ids c, fine z zf and lattice cell lc: FP64 sum in
+        /// level order (then the lattice terms), as k_prolong_dot.  The coarse and lattice loads
+        /// are unconditional (clamped indices, zero weights where a node has no coarse ids or no
+        /// lattice cell), so that the compiler keeps every node's loads of a batch in flight.
+        template <int BD, bool LAT>
+        __device__ __forceinline__ void cg_prolong(const CgProlong &pr, const int4 &c, const float (&zf)[BD],
+                                                   const int4 &lc, float (&u)[BD])
+        {
+            double z[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                z[i] = static_cast<double>(zf[i]);
+            const bool has = c.x >= 0;
+            const int cc[4] = {c.x, c.y, c.z, c.w};
+            float cv[4][BD];
+#pragma unroll
+            for (int l = 0; l < 4; ++l)
+            {
+                const bool on = has && l < pr.level_num;
+                const size_t o = on ? static_cast<size_t>(pr.lev_off[l + 1] + cc[l]) * BD : 0;
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                    cv[l][i] = l < pr.level_num ? pr.mz[o + i] : 0.0f;
+            }
+#pragma unroll
+            for (int l = 0; l < 4; ++l)
+                if (has && l < pr.level_num)
+#pragma unroll
+                    for (int i = 0; i < BD; ++i)
+                        z[i] += static_cast<double>(cv[l][i]);
+            if constexpr (LAT)
+            {
+                const bool in = lc.x >= 0;
+                const float fx = __int_as_float(lc.y), fy = __int_as_float(lc.z), fz = __int_as_float(lc.w);
+                const int k = pr.lcc.k;
+                float cf[8][BD];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                {
+                    const int id = in ? lc.x + (q & 1) + ((q >> 1) & 1) * k + ((q >> 2) & 1) * k * k : 0;
+#pragma unroll
+                    for (int i = 0; i < BD; ++i)
+                        cf[q][i] = pr.lcc.coef[static_cast<size_t>(id) * BD + i];
+                }
+                if (in)
+                {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                    {
+                        const float w = ((q & 1) ? fx : 1.0f - fx) * ((q & 2) ? fy : 1.0f - fy) *
+                                        ((q & 4) ? fz : 1.0f - fz);
+#pragma unroll
+                        for (int i = 0; i < BD; ++i)
+                            z[i] += static_cast<double>(w) * static_cast<double>(cf[q][i]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                u[i] = static_cast<float>(z[i]);
+        }
+
+        /// Rounds of halo nodes whose inputs the single-reduction SpMV loads in one batch.
+        constexpr int HALO_CG_R = 3;
+
+        /// Single-reduction PCG SpMV (launch_spmv_cg): every workgroup stages u = prolongation of
+        /// z at its halo in LDS (its own rows' u among them, at own[blk] + row), multiplies, and
+        /// writes u and w = A u for its own rows with the partials of r . u, w . u, u . s_prev and
+        /// r . u_prev.  The block's previous rz and ps partials move to rz_old and ps_old first.
+        template <int BD, int U, bool LAT>
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_cg(SellView<float> A, CgProlong pr, float *__restrict__ u,
+                                                                float *__restrict__ w, const float *__restrict__ r,
+                                                                const float *__restrict__ s_prev,
+                                                                const float *__restrict__ u_prev, CgReductions red,
+                                                                SysMap sm, int init)
+        {
+            const int blk = xcd_block();
+            const int sys = system_of_block(sm, blk);
+            if (system_stopped(sm, sys))
+                return;
+            float *xs = reinterpret_cast<float *>(spmv_halo_lds);
+            const size_t row = static_cast<size_t>(blk) * SPMV_BLOCK + threadIdx.x;
+            // The own rows' inputs of the partials do not depend on the halo: loaded first.
+            float rv[BD], sv[BD], uv[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+            {
+                rv[i] = r[row * BD + i];
+                sv[i] = init ? 0.0f : s_prev[row * BD + i];
+                uv[i] = (init || u_prev == nullptr) ? 0.0f : u_prev[row * BD + i];
+            }
+            if (threadIdx.x == 0)
+            {
+                red.rz_old[blk] = red.rz[blk];
+                red.ps_old[blk] = red.ps[blk];
+            }
+            const int hb = A.halo_ptr[blk], hn = A.halo_ptr[blk + 1] - hb;
+            if (hn > 0)
+            {
+                const int last = hn - 1;
+                int g[HALO_CG_R];
+                int4 c[HALO_CG_R], lc[HALO_CG_R];
+                float zf[HALO_CG_R][BD];
+#pragma unroll
+                for (int k = 0; k < HALO_CG_R; ++k)
+                    g[k] = A.halo[hb + min(static_cast<int>(threadIdx.x) + k * SPMV_BLOCK, last)];
+#pragma unroll
+                for (int k = 0; k < HALO_CG_R; ++k)
+                {
+                    c[k] = pr.cmap[g[k]];
+#pragma unroll
+                    for (int i = 0; i < BD; ++i)
+                        zf[k][i] = pr.mz[static_cast<size_t>(g[k]) * BD + i];
+                    lc[k] = LAT ? pr.lcc.cell[g[k]] : int4{-1, 0, 0, 0};
+                }
+#pragma unroll
+                for (int k = 0; k < HALO_CG_R; ++k)
+                {
+                    const int i = threadIdx.x + k * SPMV_BLOCK;
+                    float uk[BD];
+                    cg_prolong<BD, LAT>(pr, c[k], zf[k], lc[k], uk);
+                    if (i < hn)
+#pragma unroll
+                        for (int j = 0; j < BD; ++j)
+                            xs[i * BD + j] = uk[j];
+                }
+                for (int i = threadIdx.x + HALO_CG_R * SPMV_BLOCK; i < hn; i += SPMV_BLOCK)
+                {
+                    const int gi = A.halo[hb + i];
+                    float zi[BD], ui[BD];
+#pragma unroll
+                    for (int j = 0; j < BD; ++j)
+                        zi[j] = pr.mz[static_cast<size_t>(gi) * BD + j];
+                    cg_prolong<BD, LAT>(pr, pr.cmap[gi], zi, LAT ? pr.lcc.cell[gi] : int4{-1, 0, 0, 0}, ui);
+#pragma unroll
+                    for (int j = 0; j < BD; ++j)
+                        xs[i * BD + j] = ui[j];
+                }
+            }
+            __syncthreads();
+            const int lane = threadIdx.x % WAVE;
+            const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            SellRow<BD, U, true, float> sr;
+            sr.open(A, slice, lane);
+            float acc[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                acc[i] = 0.0f;
+            sr.run(static_cast<const float *>(nullptr), xs, acc);
+            const int own = pr.own[blk] + static_cast<int>(threadIdx.x);
+            double dots[4] = {0.0, 0.0, 0.0, 0.0}; // gamma, delta, eps, zeta
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+            {
+                const float ui = xs[own * BD + i];
+                u[row * BD + i] = ui;
+                w[row * BD + i] = acc[i];
+                const double ud = static_cast<double>(ui);
+                dots[0] += static_cast<double>(rv[i]) * ud;
+                dots[1] += static_cast<double>(acc[i]) * ud;
+                dots[2] += ud * static_cast<double>(sv[i]);
+                dots[3] += static_cast<double>(rv[i]) * static_cast<double>(uv[i]);
+            }
+            double *const dst[4] = {red.rz, red.dlt, init ? nullptr : red.eps,
+                                    (init || u_prev == nullptr) ? nullptr : red.zeta};
+            store_partials<SPMV_BLOCK, 4>(dots, dst, blk);
         }
 
         /// y_c = A x_c for the `copies` systems of a batch that repeats one matrix: system c's node
@@ -511,8 +679,8 @@ namespace psmas
             using TA = typename Acc<TV>::type;
             __shared__ TY sv[SPMV_BLOCK * BD];
             const int blk = xcd_block();
-            const int sys = sm.block_sys[blk];
-            if (sm.active[sys] == 0 || (selected_only && sm.select[sys] == 0))
+            const int sys = system_of_block(sm, blk);
+            if (system_stopped(sm, sys) || (selected_only && sm.select[sys] == 0))
                 return;
             const int lane = threadIdx.x % WAVE;
             const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
@@ -589,7 +757,7 @@ namespace psmas
             static_assert(VEC_BLOCK == 256, "lattice blocks are 256-node blocks");
             __shared__ T sv[VEC_BLOCK * BD];
             const int sys = block_system(sm);
-            if (sm.active[sys] == 0)
+            if (system_stopped(sm, sys))
                 return;
             // Lattice inputs first: their loads overlap the vector update.
             LatticeSlot ls;
@@ -638,12 +806,106 @@ namespace psmas
             }
         }
 
+        /// Single-reduction PCG update (launch_update_cg).
+        template <int BD, bool LAT>
+        __global__ __launch_bounds__(VEC_BLOCK) void k_update_cg(const float *__restrict__ u, const float *__restrict__ w,
+                                                                 float *__restrict__ p, float *__restrict__ s,
+                                                                 float *__restrict__ x, float *__restrict__ r,
+                                                                 FineRestrict fr, float *__restrict__ rc0,
+                                                                 float *__restrict__ u_prev, CgReductions red,
+                                                                 SysMap sm, LatticeBlocks lb, int init)
+        {
+            static_assert(VEC_BLOCK == 256, "lattice blocks are 256-node blocks");
+            __shared__ float sv[VEC_BLOCK * BD];
+            const int sys = block_system(sm);
+            if (system_stopped(sm, sys))
+                return;
+            LatticeSlot ls;
+            if constexpr (LAT)
+                ls = lattice_slot(lb, blockIdx.x);
+            // The block's vectors and restriction map are loaded before the scalars' reduction
+            // (whose barriers the loads would not cross) so that the latencies overlap.
+            const size_t node = static_cast<size_t>(blockIdx.x) * VEC_BLOCK + threadIdx.x;
+            const uint32_t mask = fr.lead[node];
+            const int gid = fr.gid[node];
+            const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
+            float uv[BD], wv[BD], pv[BD], svv[BD], xv[BD], rv0[BD];
+#pragma unroll
+            for (int k = 0; k < BD; ++k)
+            {
+                const size_t o = base + threadIdx.x + k * VEC_BLOCK;
+                uv[k] = u[o];
+                wv[k] = w[o];
+                pv[k] = init ? 0.0f : p[o];
+                svv[k] = init ? 0.0f : s[o];
+                xv[k] = x[o];
+                rv0[k] = r[o];
+            }
+            // Every scalar comes from partials (no state kept between launches): gamma = r . u,
+            // delta = u . A u, eps = u . s_prev, gamma_old, (p . s)_old and zeta = r . u_prev.
+            const double guard = red.sc[static_cast<size_t>(sys) * S_COUNT + S_GUARD];
+            const bool flex = u_prev != nullptr;
+            double alpha, beta = 0.0;
+            if (init)
+            {
+                const double *const parts[2] = {red.rz, red.dlt};
+                double t[2];
+                system_totals<VEC_BLOCK, 2>(parts, sys, sm, t);
+                alpha = safe_div(t[0], t[1], guard);
+            }
+            else
+            {
+                const double *const parts[6] = {red.rz, red.dlt, red.eps, red.rz_old, red.ps_old,
+                                                 flex ? red.zeta : red.rz};
+                double t[6];
+                system_totals<VEC_BLOCK, 6>(parts, sys, sm, t);
+                beta = safe_div(flex ? t[0] - t[5] : t[0], t[3], guard);
+                // p . A p of the new direction from the recurrences: (u + beta p)(w + beta s).
+                const double pap = t[1] + 2.0 * beta * t[2] + beta * beta * t[4];
+                alpha = safe_div(t[0], pap, guard);
+            }
+            const float al = static_cast<float>(alpha), be = static_cast<float>(beta);
+            double rr = 0.0, ps = 0.0;
+#pragma unroll
+            for (int k = 0; k < BD; ++k)
+            {
+                const int j = threadIdx.x + k * VEC_BLOCK;
+                const size_t o = base + j;
+                const float pn = fmaf(be, pv[k], uv[k]);
+                const float sn = fmaf(be, svv[k], wv[k]);
+                p[o] = pn;
+                s[o] = sn;
+                x[o] = fmaf(al, pn, xv[k]);
+                const float rn = fmaf(-al, sn, rv0[k]);
+                r[o] = rn;
+                sv[j] = rn;
+                if (flex)
+                    u_prev[o] = uv[k];
+                rr += static_cast<double>(rn) * static_cast<double>(rn);
+                ps += static_cast<double>(pn) * static_cast<double>(sn);
+            }
+            __syncthreads();
+            restrict_bank<BD, float>(sv, (threadIdx.x / 32) * 32, mask, gid, rc0);
+            const double v2[2] = {rr, ps};
+            double *const dst[2] = {red.rr, red.ps};
+            store_partials<VEC_BLOCK, 2>(v2, dst, blockIdx.x);
+            if constexpr (LAT)
+            {
+                __shared__ float vals[256 * 8 * BD];
+                float rn[BD];
+#pragma unroll
+                for (int d = 0; d < BD; ++d)
+                    rn[d] = sv[threadIdx.x * BD + d];
+                lattice_block_partials<BD, float>(lb, blockIdx.x, ls, rn, vals);
+            }
+        }
+
         template <int BD, typename T>
         __global__ __launch_bounds__(VEC_BLOCK) void k_axpy_alpha(const T *__restrict__ p, T *__restrict__ x,
                                                                   Reductions red, SysMap sm)
         {
             const int sys = block_system(sm);
-            if (sm.active[sys] == 0)
+            if (system_stopped(sm, sys))
                 return;
             const T alpha = static_cast<T>(system_alpha(red, sys, sm));
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
@@ -661,7 +923,7 @@ namespace psmas
                                                                 const double *__restrict__ rzo, T *__restrict__ zprev)
         {
             const int sys = block_system(sm);
-            if (sm.active[sys] == 0)
+            if (system_stopped(sm, sys))
                 return;
             // The block's vectors are loaded before beta's reduction so both latencies overlap.
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
@@ -706,7 +968,7 @@ namespace psmas
                                                                              double *__restrict__ x64, SysMap sm)
         {
             const int sys = block_system(sm);
-            if (sm.active[sys] == 0 || sm.select[sys] == 0)
+            if (system_stopped(sm, sys) || sm.select[sys] == 0)
                 return;
             const size_t base = static_cast<size_t>(blockIdx.x) * VEC_BLOCK * BD;
 #pragma unroll
@@ -872,6 +1134,59 @@ namespace psmas
         const size_t lds = static_cast<size_t>(A.halo_max) * BD * sizeof(TX);
         PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_ap_fused<BD, U, TV, TX>), dim3(grid),
                                                            dim3(SPMV_BLOCK), lds, s, A, fd, y, red, sm));
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD>
+    void launch_spmv_cg(const SellView<float> &A, const CgProlong &pr, float *u, float *w, const float *r,
+                        const float *s_prev, const float *u_prev, const CgReductions &red, const SysMap &sm,
+                        bool init, hipStream_t s)
+    {
+        require_device_ptrs("spmv_cg", {A.slice_ptr, A.lcols, A.halo_ptr, A.halo, A.vals, pr.mz, pr.cmap, pr.lev_off,
+                                        pr.own, u, w, r, red.rz, red.rz_old, red.dlt, red.ps, red.ps_old});
+        if (!init)
+            require_device_ptrs("spmv_cg (previous direction)", {s_prev, red.eps});
+        if (!init && u_prev != nullptr)
+            require_device_ptrs("spmv_cg (flexible)", {red.zeta});
+        if (A.halo_max <= 0)
+            throw std::logic_error("[MAS] internal error: the single-reduction SpMV needs the SpMV halo lists.");
+        if (pr.level_num < 0 || pr.level_num > 4)
+            throw std::logic_error("[MAS] internal error: at most 4 coarse levels.");
+        const int grid = A.nslices / (SPMV_BLOCK / WAVE);
+        const size_t lds = static_cast<size_t>(A.halo_max) * BD * sizeof(float);
+        const int in = init ? 1 : 0;
+        if (pr.lcc.cell != nullptr)
+        {
+            require_device_ptrs("spmv_cg (lattice)", {pr.lcc.coef});
+            PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_cg<BD, U, true>), dim3(grid), dim3(SPMV_BLOCK),
+                                                               lds, s, A, pr, u, w, r, s_prev, u_prev, red, sm, in));
+        }
+        else
+        {
+            PSMAS_UNROLL_DISPATCH(A.unroll, hipLaunchKernelGGL((k_spmv_cg<BD, U, false>), dim3(grid), dim3(SPMV_BLOCK),
+                                                               lds, s, A, pr, u, w, r, s_prev, u_prev, red, sm, in));
+        }
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <int BD>
+    void launch_update_cg(int nblocks, const float *u, const float *w, float *p, float *s_vec, float *x, float *r,
+                          FineRestrict fr, float *rc0, float *u_prev, const CgReductions &red, const SysMap &sm,
+                          bool init, hipStream_t s, const LatticeBlocks *lb)
+    {
+        require_device_ptrs("update_cg", {u, w, p, s_vec, x, r, fr.lead, fr.gid, rc0, red.rz, red.dlt, red.rr, red.ps,
+                                          red.sc});
+        if (!init)
+            require_device_ptrs("update_cg (previous iteration)", {red.eps, red.rz_old, red.ps_old});
+        if (!init && u_prev != nullptr)
+            require_device_ptrs("update_cg (flexible)", {red.zeta});
+        const int in = init ? 1 : 0;
+        if (lb != nullptr && lb->cell != nullptr)
+            hipLaunchKernelGGL((k_update_cg<BD, true>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, u, w, p, s_vec, x, r, fr,
+                               rc0, u_prev, red, sm, *lb, in);
+        else
+            hipLaunchKernelGGL((k_update_cg<BD, false>), dim3(nblocks), dim3(VEC_BLOCK), 0, s, u, w, p, s_vec, x, r,
+                               fr, rc0, u_prev, red, sm, LatticeBlocks{}, in);
         PSMAS_LAUNCH_CHECK();
     }
 
@@ -1158,7 +1473,13 @@ namespace psmas
                                               hipStream_t, const double *, double *);                             \
     template void launch_update_p<BD, float>(int, const float *, float *, const Reductions &, const SysMap &,      \
                                              hipStream_t, const double *, float *);                               \
-    template void launch_accumulate_correction<BD>(int, float *, double *, const SysMap &, hipStream_t);
+    template void launch_accumulate_correction<BD>(int, float *, double *, const SysMap &, hipStream_t);            \
+    template void launch_spmv_cg<BD>(const SellView<float> &, const CgProlong &, float *, float *, const float *,   \
+                                     const float *, const float *, const CgReductions &, const SysMap &, bool,     \
+                                     hipStream_t);                                                                 \
+    template void launch_update_cg<BD>(int, const float *, const float *, float *, float *, float *, float *,       \
+                                       FineRestrict, float *, float *, const CgReductions &, const SysMap &, bool, \
+                                       hipStream_t, const LatticeBlocks *);
 
     PSMAS_INST_BD(1)
     PSMAS_INST_BD(2)

@@ -11,8 +11,9 @@ from polysolve_amd import MASSolver, problems
 
 pytestmark = pytest.mark.gpu
 
+# The fused update is a form of the classic iteration (two reductions per iteration).
 BASE = {"block_dim": 3, "coarse_solver": "cholesky", "relative_tolerance": 1e-8, "absolute_tolerance": 0.0,
-        "max_iter": 5000}
+        "max_iter": 5000, "single_reduction": False}
 
 
 def _solver(p, **extra):

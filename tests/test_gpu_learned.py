@@ -157,15 +157,20 @@ def test_exact_and_linear_corrections(gpu, drift_sequence):
         assert np.linalg.norm(x - x0) <= 1e-6 * np.linalg.norm(x0)
 
 
-def test_learned_fallback_to_plain_mas_on_stall(gpu, drift_sequence, trained_solver):
+@pytest.mark.parametrize("single_reduction", [False, True])
+def test_learned_fallback_to_plain_mas_on_stall(gpu, drift_sequence, trained_solver, single_reduction):
     """A correction scaled far beyond what was trained stops the residual from decreasing; plain
-    MAS takes over and the solve still reaches the FP64 tolerance."""
+    MAS takes over and the solve still reaches the FP64 tolerance.  Both PCG iterations (the
+    single-reduction one assembles u = M r + P c in its SpMV and restarts with beta = 0)."""
     s = trained_solver
-    s.set_parameters({"MAS": {"learned_output_scale": 1e6, "learned_stall_segments": 2}})
+    s.set_parameters({"MAS": {"learned_output_scale": 1e6, "learned_stall_segments": 2,
+                              "single_reduction": single_reduction}})
     try:
         info, rel, _ = _solve(s, drift_sequence[EVAL[0]], True)
     finally:
-        s.set_parameters({"MAS": {"learned_output_scale": 1.0, "learned_stall_segments": 3}})
+        s.set_parameters({"MAS": {"learned_output_scale": 1.0, "learned_stall_segments": 3,
+                                  "single_reduction": False}})
+    assert info["pcg_iteration"] == ("single_reduction" if single_reduction else "classic")
     assert info["learned"]["fallback"], info["learned"]
     assert info["learned"]["fallback_iteration"] > 0
     assert info["solver_status"] == "Reach relative tolerance"

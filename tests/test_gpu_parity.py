@@ -168,6 +168,39 @@ def test_apply_mixed_close_to_fp64(gpu, het12, cap):
     assert np.linalg.norm(zs - (z16 + 2.0 * zv)) <= 1e-5 * np.linalg.norm(zs)
 
 
+@pytest.mark.parametrize("bd", [1, 2, 3])
+@pytest.mark.parametrize("cap", [0, 3])
+def test_wave_apply_matches_persistent_apply(gpu, bd, cap):
+    """The wave-per-matrix FP16 apply (registers, DPP transposed products, slot partials) and the
+    persistent LDS-unpacked one compute the same operator from the same FP16 matrices: equal to
+    FP32 rounding (different summation orders), at block sizes 1-3, with the grid-stride loop
+    (cap = 3 workgroups walk every matrix), split level-2.. inputs (restrict_piece = 9) and the
+    dense coarsest rows of the second launch."""
+    if bd == 3:
+        p = problems.hex_elasticity(10, E=problems.random_moduli(10, 11))
+        A, coords = p.A, p.coords
+    else:
+        A = helpers.prefactor_matrix(helpers.grid_laplacian_pattern(17, 13, 5), seed=7)
+        coords = None
+    s = MASSolver()
+    s.set_parameters({"MAS": {"block_dim": bd, "coarse_solver": "cholesky", "restrict_piece": 9,
+                              "apply_workgroups": cap}})
+    if coords is not None:
+        s.set_coordinates(coords)
+    s.factorize(A)
+    rng = np.random.default_rng(21 + bd)
+    for _ in range(2):
+        r = rng.standard_normal(A.shape[0])
+        s.set_parameters({"MAS": {"apply_wave": True}})
+        zw = s.apply_preconditioner(r, mixed=True)
+        s.set_parameters({"MAS": {"apply_wave": False}})
+        zp = s.apply_preconditioner(r, mixed=True)
+        z64 = s.apply_preconditioner(r)
+        assert np.all(np.isfinite(zw))
+        assert np.linalg.norm(zw - zp) <= 2e-6 * np.linalg.norm(zp)
+        assert np.linalg.norm(zw - z64) <= 5e-3 * np.linalg.norm(z64)
+
+
 def test_apply_matches_dense_construction(gpu):
     p = problems.hex_elasticity(4, E=problems.random_moduli(4, 7))
     s = _factorized(p, {"aggregation": "graph"})
@@ -473,7 +506,9 @@ def test_spmv_halo_matches_global_gathers(gpu, het12, precision):
     """x staged per workgroup in LDS (16-bit local columns) gives the same products, bit for bit,
     as gathering x through the 32-bit columns: a whole solve (its A p and b - A x kernels) ends
     with the same iterate."""
-    s = _factorized(het12, {"precision": precision, "relative_tolerance": 1e-8, "absolute_tolerance": 0.0})
+    # (The single-reduction iteration needs the halo form: both solves use the classic one.)
+    s = _factorized(het12, {"precision": precision, "relative_tolerance": 1e-8, "absolute_tolerance": 0.0,
+                            "single_reduction": False})
     info = s.get_info()["spmv_halo"]
     assert info["active"] and 0 < info["max_nodes"] <= 2048 and info["total_nodes"] >= het12.n // 3
     xs_halo = np.zeros(het12.n)
