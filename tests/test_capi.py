@@ -58,11 +58,15 @@ def test_inner_object_extension():
     assert p["max_iter"] == 12 and p["precision"] == "mixed"
 
 
-def test_fused_direction_switch():
-    assert parse_parameters({})["spmv_fuse_direction"] is False
-    assert parse_parameters({"MAS": {"spmv_fuse_direction": True}})["spmv_fuse_direction"] is True
-    with pytest.raises(MASError, match="spmv_fuse_direction"):
-        parse_parameters({"MAS": {"spmv_fuse_direction": 0}})
+@pytest.mark.parametrize("key,default", [("spmv_fuse_direction", False), ("single_reduction", False),
+                                         ("apply_wave", True)])
+def test_kernel_variant_switches(key, default):
+    """Boolean switches between measured variants of the iteration's kernels: the default, the
+    other value, and a non-boolean refused with the key named."""
+    assert parse_parameters({})[key] is default
+    assert parse_parameters({"MAS": {key: not default}})[key] is (not default)
+    with pytest.raises(MASError, match=key):
+        parse_parameters({"MAS": {key: 0}})
 
 
 @pytest.mark.parametrize("bad", [
