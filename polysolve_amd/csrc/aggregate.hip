@@ -56,11 +56,15 @@ namespace psmas
                                                               const double *__restrict__ vals, double scale,
                                                               u64 sentinel, u64 *__restrict__ keys, u64 *__restrict__ w)
         {
-            const int c = blockIdx.x * TB + threadIdx.x;
+            // A wave per column, lanes over its entries: every read and write of a wave is
+            // contiguous (a thread per column wrote each lane's entries a column apart).
+            const int c = blockIdx.x * (TB / 64) + static_cast<int>(threadIdx.x) / 64; // wave-uniform
+            const int lane = threadIdx.x % 64;
             if (c >= n)
                 return;
             const u64 cb = static_cast<u64>(c / bd);
-            for (int k = colptr[c]; k < colptr[c + 1]; ++k)
+            const int k1 = colptr[c + 1];
+            for (int k = colptr[c] + lane; k < k1; k += 64)
             {
                 const u64 rb = static_cast<u64>(rowidx[k] / bd);
                 const size_t e = 2 * static_cast<size_t>(k);
@@ -288,8 +292,8 @@ namespace psmas
             DeviceBuffer<u64> keys(std::max<long long>(N, 1), "MAS aggregation keys"),
                 w(std::max<long long>(N, 1), "MAS aggregation weights");
             const u64 sentinel = (static_cast<u64>(nb) << bits) | static_cast<u64>(nb);
-            hipLaunchKernelGGL(k_graph_entries, dim3(grid_of(n)), dim3(TB), 0, s, n, bd, bits, d_colptr, d_rowidx,
-                               d_vals, scale, sentinel, keys.data(), w.data());
+            hipLaunchKernelGGL(k_graph_entries, dim3(grid_of(static_cast<long long>(n) * 64)), dim3(TB), 0, s, n, bd,
+                               bits, d_colptr, d_rowidx, d_vals, scale, sentinel, keys.data(), w.data());
             PSMAS_LAUNCH_CHECK();
             build_graph(N, keys, w, nb, bits, s, g);
         }

@@ -25,20 +25,24 @@ namespace psmas
         }
 
         /// Entry k of column c: key (slot of r's block) << cbits | (slot of c's block), its index k
-        /// and its element (r % bd, c % bd).  The trailing partial block's padded diagonal scalars
-        /// follow as entries nnz .. nnz+pad-1 (value 1).
+        /// and its element (r % bd, c % bd).  A wave per column, lanes over its entries, so that
+        /// every read and write of a wave is contiguous (a thread per column wrote each lane's
+        /// entries a column apart).  The trailing partial block's padded diagonal scalars follow
+        /// as entries nnz .. nnz+pad-1 (value 1).
         __global__ __launch_bounds__(TB) void k_entry_keys(int n, int nb, int bd, int cbits, long long nnz,
                                                            const int *__restrict__ colptr,
                                                            const int *__restrict__ rowidx,
                                                            const int *__restrict__ sob, uint64_t *__restrict__ keys,
                                                            int *__restrict__ idx, unsigned char *__restrict__ ecode)
         {
-            const int c = blockIdx.x * TB + threadIdx.x;
+            const int c = blockIdx.x * (TB / 64) + static_cast<int>(threadIdx.x) / 64; // wave-uniform
+            const int lane = threadIdx.x % 64;
             if (c < n)
             {
                 const uint64_t C = static_cast<uint64_t>(sob[c / bd]);
                 const int cm = c % bd;
-                for (int k = colptr[c]; k < colptr[c + 1]; ++k)
+                const int k1 = colptr[c + 1];
+                for (int k = colptr[c] + lane; k < k1; k += 64)
                 {
                     const int r = rowidx[k];
                     keys[k] = (static_cast<uint64_t>(sob[r / bd]) << cbits) | C;
@@ -229,8 +233,8 @@ namespace psmas
         DeviceBuffer<uint64_t> keys(N, "MAS matrix build keys"), keys_sorted(N, "MAS matrix build keys");
         DeviceBuffer<int> idx(N, "MAS matrix build order"), idx_sorted(N, "MAS matrix build order");
         DeviceBuffer<unsigned char> ecode(N, "MAS matrix build elements");
-        hipLaunchKernelGGL(k_entry_keys, dim3(grid_for(n)), dim3(TB), 0, s, n, nb, bd, cbits, nnz, d_colptr, d_rowidx,
-                           d_slot_of_block, keys.data(), idx.data(), ecode.data());
+        hipLaunchKernelGGL(k_entry_keys, dim3(grid_for(static_cast<long long>(n) * 64)), dim3(TB), 0, s, n, nb, bd,
+                           cbits, nnz, d_colptr, d_rowidx, d_slot_of_block, keys.data(), idx.data(), ecode.data());
         PSMAS_LAUNCH_CHECK();
 
         DeviceBuffer<int> flag(N, "MAS matrix build flags"), blk(N, "MAS matrix build blocks");
