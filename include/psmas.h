@@ -124,7 +124,7 @@ int psmas_solve_device(psmas_handle h, int n, const double *d_b, double *d_x);
  * psmas_solve_device (bit-identical results), but it allocates nothing: the solve's reduction
  * partials and per-system scalars live in a static 1 MiB array of the library, one per GPU, held
  * until the process ends, and kernels are launched one by one (no graph replay).  Limit: systems
- * of at most 21,840 node blocks (256 padded nodes each, about 5.5 M block nodes); larger ones are
+ * of at most 13,104 node blocks (256 padded nodes each, about 3.35 M block nodes); larger ones are
  * refused with PSMAS_ERROR_INVALID_ARGUMENT.  Parts that a factorisation builds at its first
  * solve (the mixed path's FP32/FP16 copies, the dense coarsest solve, the learned correction's
  * lattice) must exist: call psmas_solve_device once after factorize, otherwise the call fails
