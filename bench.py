@@ -427,6 +427,8 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=150, help="learned path: training epochs")
     ap.add_argument("--multi-rhs", type=int, default=4,
                     help="also solve this many right-hand sides together (0: skip)")
+    ap.add_argument("--profile-multi", type=int, default=0,
+                    help="profile mode: also solve this many right-hand sides together (kernel counters)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="profile mode: first run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -516,6 +518,21 @@ def main() -> int:
             solver.factorize(prob.A)
             learned_ms = run_path(solver, b_dev, x_dev, n, args.steps, args.warmup, synchronize)
             log(f"  learned {learned_ms:.2f} ms/solve, {solver.get_info()['solver_iter']} it")
+        if args.profile_multi > 1:
+            k = args.profile_multi
+            B = np.random.default_rng(7).standard_normal((n, k))
+            ms_solver = MASSolver()
+            ms_solver.set_parameters({"MAS": dict(base, precision="mixed", coarse_solver=args.coarse)})
+            ms_solver.set_coordinates(prob.coords)
+            ms_solver.factorize_multi(prob.A, k)
+            dB, dX = DeviceArray(n * k, B.ravel(order="F")), DeviceArray(n * k)
+            t = time.perf_counter()
+            for _ in range(max(args.steps, 1)):
+                ms_solver.solve_multi_device(dB.ptr, dX.ptr, n, k)
+            synchronize()
+            log(f"  {k} right-hand sides together {(time.perf_counter() - t) * 1e3 / max(args.steps, 1):.1f} ms/solve")
+            dB.free()
+            dX.free()
         return 0
 
     solver.set_parameters({"MAS": {"precision": "mixed", "coarse_solver": args.coarse}})
