@@ -614,4 +614,296 @@ namespace psmas
         }
         return ok;
     }
+    bool build_symmetric_sell(int npad, const std::vector<int> &slice_ptr, const std::vector<int> &cols,
+                              const std::vector<unsigned short> &lcols, const std::vector<int> &row_len, SymSell &out)
+    {
+        constexpr int NB = 256, W = 64, SL = NB / W;
+        constexpr uint32_t NONE = 0x7FFFu, FOREIGN = 1u << 31;
+        out = SymSell();
+        if (npad % NB != 0 || static_cast<int>(slice_ptr.size()) != npad / W + 1 ||
+            static_cast<int>(row_len.size()) != npad || lcols.size() != cols.size())
+        {
+            out.reason = "inconsistent SELL arrays";
+            return false;
+        }
+        if (cols.size() > static_cast<size_t>(INT_MAX))
+        {
+            out.reason = "too many slots for 32-bit slot indices";
+            return false;
+        }
+        const int nblk = npad / NB;
+        struct Ent
+        {
+            int col; ///< global column (ordering)
+            int src, mirror;
+            uint32_t meta;
+        };
+        struct Block
+        {
+            int len[SL] = {0, 0, 0, 0};
+            int incoming = 0;
+            std::vector<Ent> ent;              ///< position p's entries: ent[off[p] .. off[p + 1])
+            std::vector<int> off;
+            std::vector<uint32_t> info;
+        };
+        std::vector<Block> blocks(static_cast<size_t>(nblk));
+        std::atomic<bool> ok{true};
+        std::string reason;
+        std::atomic<int> failed_block{-1};
+        auto work = [&](int b0, int b1) {
+            // Per row of the block: diagonal slot, cross-block slots, intra-block (column, slot).
+            std::vector<std::pair<int, size_t>> intra[NB];
+            std::vector<size_t> cross[NB];
+            size_t diag[NB];
+            bool real_diag[NB];
+            int len[NB], cnt[NB], beg[NB];
+            struct Own
+            {
+                int other;
+                size_t src, mirror;
+                int rank;
+            };
+            std::vector<Own> owned[NB];
+            struct Foreign
+            {
+                int row;
+                size_t idx;
+                int rank;
+            };
+            std::vector<Foreign> foreign[NB], pool;
+            std::vector<Ent> rest;
+            for (int b = b0; b < b1 && ok; ++b)
+            {
+                const int r0 = b * NB;
+                bool fine = true;
+                for (int lr = 0; lr < NB && fine; ++lr)
+                {
+                    intra[lr].clear();
+                    cross[lr].clear();
+                    owned[lr].clear();
+                    const int r = r0 + lr, s = r / W, lane = r % W;
+                    const int sp = slice_ptr[s], L = row_len[r], slots = slice_ptr[s + 1] - sp;
+                    diag[lr] = SIZE_MAX;
+                    real_diag[lr] = false;
+                    for (int k = 0; k < L; ++k)
+                    {
+                        const size_t idx = static_cast<size_t>(sp + k) * W + lane;
+                        const int c = cols[idx];
+                        if (c == r)
+                        {
+                            diag[lr] = idx;
+                            real_diag[lr] = true;
+                        }
+                        else if (c / NB == b)
+                            intra[lr].emplace_back(c - r0, idx);
+                        else
+                            cross[lr].push_back(idx);
+                    }
+                    if (diag[lr] == SIZE_MAX)
+                    {
+                        // A row without a diagonal block: its first padding slot (own column,
+                        // zero values) stands in for it.
+                        if (L < slots)
+                            diag[lr] = static_cast<size_t>(sp + L) * W + lane;
+                        else
+                            fine = false;
+                    }
+                    len[lr] = 1 + static_cast<int>(cross[lr].size());
+                    cnt[lr] = 0;
+                }
+                if (!fine)
+                {
+                    ok = false;
+                    failed_block = b;
+                    break;
+                }
+                // Pairs (lr < c) in row order; the mirror (c, lr) must exist.
+                for (int lr = 0; lr < NB && fine; ++lr)
+                    for (const auto &e : intra[lr])
+                    {
+                        const int c = e.first;
+                        if (c < lr)
+                            continue;
+                        const auto &row = intra[c];
+                        const auto it = std::lower_bound(row.begin(), row.end(), std::make_pair(lr, size_t(0)),
+                                                         [](const std::pair<int, size_t> &a,
+                                                            const std::pair<int, size_t> &v) { return a.first < v.first; });
+                        if (it == row.end() || it->first != lr)
+                        {
+                            fine = false;
+                            break;
+                        }
+                        const bool mine = len[lr] <= len[c];
+                        const int owner = mine ? lr : c, other = mine ? c : lr;
+                        const size_t src = mine ? e.second : it->second, mir = mine ? it->second : e.second;
+                        ++len[owner];
+                        owned[owner].push_back({other, src, mir, cnt[other]++});
+                    }
+                // Every intra entry of a row must have been paired (a structurally symmetric block).
+                for (int lr = 0; lr < NB && fine; ++lr)
+                {
+                    int paired = static_cast<int>(owned[lr].size()) + cnt[lr];
+                    if (paired != static_cast<int>(intra[lr].size()) || cnt[lr] > 255)
+                        fine = false;
+                }
+                if (!fine)
+                {
+                    ok = false;
+                    failed_block = b;
+                    break;
+                }
+                // Rows longer than the block's mean length hand blocks of theirs that couple to other
+                // node blocks to the shortest rows, which compute those products for them and
+                // deliver them through the incoming slots (foreign entries).
+                int sum = 0;
+                for (int lr = 0; lr < NB; ++lr)
+                {
+                    sum += len[lr];
+                    foreign[lr].clear();
+                }
+                const int T = (sum + NB - 1) / NB;
+                pool.clear();
+                for (int lr = 0; lr < NB; ++lr)
+                    while (len[lr] > T && !cross[lr].empty() && cnt[lr] < 255)
+                    {
+                        pool.push_back({lr, cross[lr].back(), cnt[lr]++});
+                        cross[lr].pop_back();
+                        --len[lr];
+                    }
+                {
+                    int rorder[NB];
+                    std::iota(rorder, rorder + NB, 0);
+                    std::stable_sort(rorder, rorder + NB, [&](int a, int c) { return len[a] < len[c]; });
+                    size_t q = 0;
+                    for (int i = 0; i < NB && q < pool.size(); ++i)
+                        for (const int lr = rorder[i]; len[lr] < T && q < pool.size(); ++len[lr])
+                            foreign[lr].push_back(pool[q++]);
+                    for (int i = 0; q < pool.size(); i = (i + 1) % NB) // (sum <= NB * T: not reached)
+                    {
+                        foreign[rorder[i]].push_back(pool[q++]);
+                        ++len[rorder[i]];
+                    }
+                }
+                int total = 0;
+                for (int lr = 0; lr < NB; ++lr)
+                {
+                    beg[lr] = total;
+                    total += cnt[lr];
+                }
+                if (total > static_cast<int>(NONE) - 1)
+                {
+                    ok = false;
+                    failed_block = b;
+                    break;
+                }
+                Block &B = blocks[b];
+                B.incoming = total;
+                B.ent.clear();
+                B.off.assign(NB + 1, 0);
+                B.info.assign(NB, 0);
+                int order[NB];
+                std::iota(order, order + NB, 0);
+                std::stable_sort(order, order + NB, [&](int a, int c) { return len[a] > len[c]; });
+                for (int p = 0; p < NB; ++p)
+                {
+                    const int lr = order[p];
+                    const uint32_t self = lcols[diag[lr]];
+                    B.ent.push_back({r0 + lr, real_diag[lr] ? static_cast<int>(diag[lr]) : -1, -1, self | (NONE << 16)});
+                    rest.clear();
+                    for (const Own &o : owned[lr])
+                        rest.push_back({r0 + o.other, static_cast<int>(o.src), static_cast<int>(o.mirror),
+                                        static_cast<uint32_t>(lcols[o.src]) |
+                                            (static_cast<uint32_t>(beg[o.other] + o.rank) << 16)});
+                    for (size_t idx : cross[lr])
+                        rest.push_back({cols[idx], static_cast<int>(idx), -1, lcols[idx] | (NONE << 16)});
+                    for (const Foreign &f : foreign[lr])
+                        rest.push_back({cols[f.idx], static_cast<int>(f.idx), -1,
+                                        static_cast<uint32_t>(lcols[f.idx]) |
+                                            (static_cast<uint32_t>(beg[f.row] + f.rank) << 16) | FOREIGN});
+                    std::sort(rest.begin(), rest.end(), [](const Ent &a, const Ent &c) { return a.col < c.col; });
+                    B.ent.insert(B.ent.end(), rest.begin(), rest.end());
+                    B.off[p + 1] = static_cast<int>(B.ent.size());
+                    B.info[p] = static_cast<uint32_t>(lr) | (static_cast<uint32_t>(cnt[lr]) << 8) |
+                                (static_cast<uint32_t>(beg[lr]) << 16);
+                    B.len[p / W] = std::max(B.len[p / W], B.off[p + 1] - B.off[p]);
+                }
+            }
+        };
+        {
+            const int nt = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+            std::vector<std::thread> th;
+            const int chunk = div_up(nblk, nt);
+            for (int t = 0; t < nt; ++t)
+            {
+                const int b = t * chunk, e = std::min(nblk, b + chunk);
+                if (b < e)
+                    th.emplace_back(work, b, e);
+            }
+            for (auto &x : th)
+                x.join();
+        }
+        if (!ok)
+        {
+            out.reason = "node block " + std::to_string(failed_block.load()) +
+                         ": pattern not symmetric within the block, a row without a diagonal slot, or too many "
+                         "transposed products";
+            return false;
+        }
+        out.slice_ptr.assign(static_cast<size_t>(npad / W) + 1, 0);
+        for (int b = 0; b < nblk; ++b)
+        {
+            out.incoming_max = std::max(out.incoming_max, blocks[b].incoming);
+            for (int q = 0; q < SL; ++q)
+                out.slice_ptr[b * SL + q + 1] = out.slice_ptr[b * SL + q] + std::max(1, blocks[b].len[q]);
+        }
+        out.slots = out.slice_ptr.back();
+        for (int b = 0; b < nblk; ++b)
+            for (const Ent &e : blocks[b].ent)
+                out.stored += e.src >= 0 ? 1 : 0;
+        const size_t n = static_cast<size_t>(out.slots) * W;
+        out.meta.assign(n, 0);
+        out.src.assign(n, -1);
+        out.mirror.assign(n, -1);
+        out.row_info.assign(static_cast<size_t>(npad), 0);
+        auto fill = [&](int b0, int b1) {
+            for (int b = b0; b < b1; ++b)
+            {
+                const Block &B = blocks[b];
+                for (int p = 0; p < NB; ++p)
+                {
+                    const int s = b * SL + p / W, lane = p % W;
+                    const int sp = out.slice_ptr[s], slots = out.slice_ptr[s + 1] - sp;
+                    const Ent *E = B.ent.data() + B.off[p];
+                    const int ne = B.off[p + 1] - B.off[p];
+                    for (int k = 0; k < slots; ++k)
+                    {
+                        const size_t at = static_cast<size_t>(sp + k) * W + lane;
+                        if (k < ne)
+                        {
+                            out.meta[at] = E[k].meta;
+                            out.src[at] = E[k].src;
+                            out.mirror[at] = E[k].mirror;
+                        }
+                        else
+                            out.meta[at] = (E[0].meta & 0xFFFFu) | (NONE << 16); // zero block, own column
+                    }
+                    out.row_info[static_cast<size_t>(b) * NB + p] = B.info[p];
+                }
+            }
+        };
+        {
+            const int nt = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+            std::vector<std::thread> th;
+            const int chunk = div_up(nblk, nt);
+            for (int t = 0; t < nt; ++t)
+            {
+                const int b = t * chunk, e = std::min(nblk, b + chunk);
+                if (b < e)
+                    th.emplace_back(fill, b, e);
+            }
+            for (auto &x : th)
+                x.join();
+        }
+        return true;
+    }
 } // namespace psmas

@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace psmas
@@ -106,4 +107,35 @@ namespace psmas
     void eliminate_dirichlet(const CscView &A, const std::vector<unsigned char> &fixed, DirichletSystem &out);
     /// a[0..n) == b[0..n), compared in parallel chunks.
     bool same_ints(const int *a, const int *b, size_t n);
+
+    /// Symmetric form of a SELL-64 matrix whose rows come in node blocks of 256 (four slices):
+    /// a block coupling rows i != j of the same node block is kept in one of the two rows, its
+    /// owner, which applies it to its own row and its transpose to the other one; blocks on the
+    /// diagonal and blocks coupling two node blocks stay in their row.  The owner of each pair is
+    /// chosen greedily (pairs in row order, the row with fewer stored blocks so far takes it);
+    /// rows still longer than the node block's mean length then hand blocks that couple to other
+    /// node blocks to the shortest rows, which compute those products for them (foreign
+    /// entries); and each node block's rows are placed in its slices by decreasing stored length.
+    /// Per slot (slot * 64 + lane of the result): meta = local column | incoming slot << 16
+    /// (0x7FFF: none) | foreign << 31, src = the full matrix's slot * 64 + lane whose block it
+    /// holds (-1: padding), mirror = the transposed block's (where an owned block's transpose goes
+    /// to an incoming slot, else -1).  row_info per position: local row | incoming count << 8 |
+    /// first incoming slot << 16.  Fails (reason set) when the pattern is not structurally
+    /// symmetric within the node blocks, a row has no diagonal slot, or a node block has more
+    /// than 32,766 incoming slots or a row more than 255.
+    struct SymSell
+    {
+        std::vector<int> slice_ptr;
+        std::vector<uint32_t> meta;
+        std::vector<int> src, mirror;
+        std::vector<uint32_t> row_info;
+        int incoming_max = 0;
+        long long slots = 0;
+        long long stored = 0; ///< slots holding a block of the matrix (src >= 0)
+        std::string reason;
+    };
+    /// slice_ptr (npad / 64 + 1), cols and lcols (the full matrix's 32-bit and halo-local
+    /// columns, slot * 64 + lane), row_len (real blocks per row, npad).
+    bool build_symmetric_sell(int npad, const std::vector<int> &slice_ptr, const std::vector<int> &cols,
+                              const std::vector<unsigned short> &lcols, const std::vector<int> &row_len, SymSell &out);
 } // namespace psmas

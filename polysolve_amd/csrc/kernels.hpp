@@ -151,6 +151,48 @@ namespace psmas
     void launch_spmv_ap_fused(const SellView<TV> &A, const FusedDirection<TX> &fd, TX *y, const Reductions &red,
                               const SysMap &sm, hipStream_t s);
 
+    /// Symmetric form of a SELL-64 matrix with halo lists: a block coupling two rows of the same
+    /// node block is stored once, in the row chosen as its owner, which also applies its
+    /// transpose to the other row through the workgroup's LDS.  Per node block the 256 rows are
+    /// placed in the four slices by decreasing stored length (row_info maps a slice position to
+    /// its row).  Slot k of position (s, l): meta = local column | incoming slot << 16 (0x7FFF:
+    /// none) | foreign << 31 and the block's values, laid out as SellView's; slot 0 of every row
+    /// is its diagonal block.  A foreign slot holds a block of another row of the node block
+    /// (balancing the rows' lengths) and sends its product there.  row_info = local row |
+    /// incoming count << 8 | first incoming slot << 16; the incoming slots of a row hold the
+    /// transposed and foreign products addressed to it, in a fixed order.
+    template <typename TV>
+    struct SymView
+    {
+        int nslices = 0;
+        const int *slice_ptr = nullptr;
+        const uint32_t *meta = nullptr;
+        const TV *vals = nullptr;
+        const uint32_t *row_info = nullptr;
+        int incoming_max = 0; ///< longest incoming list of a node block (slots)
+        int unroll = 4;
+    };
+
+    /// y = A x and the partials of x . y -> red.pap, as launch_spmv_ap, from the symmetric form
+    /// S of A (the halo lists are A's).  The sums run in another order than launch_spmv_ap's.
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_sym(const SellView<TV> &A, const SymView<TV> &S, const TX *x, TX *y, const Reductions &red,
+                         const SysMap &sm, hipStream_t s);
+    /// Largest dynamic LDS launch_spmv_sym takes (bytes; of the 160 KB of a gfx950 CU).
+    constexpr size_t SPMV_SYM_LDS_CAP = 120 * 1024;
+    /// Dynamic LDS of launch_spmv_sym: the halo of x, then BD incoming arrays.
+    template <int BD, typename TV, typename TX>
+    size_t spmv_sym_lds(int halo_max, int incoming_max);
+    /// Values of the symmetric form: entry e of the sym slots takes the full matrix's block
+    /// src[e] (an index slot * 64 + lane of vals64; < 0: zero), converted to TV.
+    template <typename TV>
+    void launch_sym_gather(int bd, long long entries, const int *src, const double *vals64, TV *out, hipStream_t s);
+    /// flag = 1 if for some entry e with mirror[e] >= 0 the blocks src[e] and mirror[e] of vals64
+    /// are not each other's transpose within rel_tol of the pair's largest entry.  flag must be
+    /// zero before the launch.
+    void launch_sym_check(int bd, long long entries, const int *src, const int *mirror, const double *vals64,
+                          double rel_tol, int *flag, hipStream_t s);
+
     /// y = b - A x, partials of y . y -> rr_partials; restrict y -> yc0.  TX: x,b; TY: y.
     /// selected_only: only systems with select == 1.
     template <int BD, typename TV, typename TX, typename TY>

@@ -277,7 +277,7 @@ namespace psmas
             "reliable_update_delta", "check_interval", "true_residual_period", "use_graphs",
             "learned_correction", "division_guard", "learned_collect", "lattice_points", "learned_hidden",
             "learned_random_pairs", "learned_snapshots", "learned_stall_segments", "learned_output_scale", "spmv_unroll", "spmv_halo", "device_setup", "spmv_copies", "spmv_fuse_direction",
-            "restrict_piece", "apply_workgroups", "learned_network", "apply_wave", "single_reduction"};
+            "restrict_piece", "apply_workgroups", "learned_network", "apply_wave", "single_reduction", "spmv_symmetric"};
     } // namespace
 
     void apply_json_parameters(Params &p, const std::string &json)
@@ -436,6 +436,8 @@ namespace psmas
             q.apply_wave = as_bool(*v, "apply_wave");
         if (const JValue *v = obj->get("single_reduction"))
             q.single_reduction = as_bool(*v, "single_reduction");
+        if (const JValue *v = obj->get("spmv_symmetric"))
+            q.spmv_symmetric = as_bool(*v, "spmv_symmetric");
         if (const JValue *v = obj->get("apply_workgroups"))
         {
             q.apply_workgroups = as_int(*v, "apply_workgroups");
@@ -483,6 +485,7 @@ namespace psmas
           << ", \"learned_output_scale\": " << json_number(p.learned_output_scale) << ", \"spmv_unroll\": " << p.spmv_unroll << ", \"spmv_halo\": " << (p.spmv_halo ? "true" : "false") << ", \"device_setup\": " << (p.device_setup ? "true" : "false") << ", \"spmv_copies\": " << (p.spmv_copies ? "true" : "false") << ", \"spmv_fuse_direction\": " << (p.spmv_fuse_direction ? "true" : "false") << ", \"restrict_piece\": " << p.restrict_piece
           << ", \"apply_workgroups\": " << p.apply_workgroups << ", \"apply_wave\": " << (p.apply_wave ? "true" : "false")
           << ", \"single_reduction\": " << (p.single_reduction ? "true" : "false")
+          << ", \"spmv_symmetric\": " << (p.spmv_symmetric ? "true" : "false")
           << ", \"learned_network\": \"" << (p.learned_linear ? "linear" : "mlp") << "\"}";
         return o.str();
     }

@@ -60,6 +60,12 @@
 //                        the matrix's tile rows in registers (default true); false: the persistent
 //                        kernel that unpacks each matrix into LDS (the one copies of a
 //                        factorisation always use)
+//   spmv_symmetric       the single-matrix SpMV of the PCG loop (y = A p) reads the symmetric form
+//                        of the matrix: a block coupling two rows of the same 256-node block is
+//                        stored once and applied to both rows (default true; used when the
+//                        pattern is symmetric within the node blocks and the blocks stored once
+//                        are symmetric to 1e-12 of their largest entry, see get_info
+//                        "spmv_symmetric"); false: every block is read in its own row
 //   division_guard       alpha/beta are set to 0 when |denominator| < division_guard; 0 (default)
 //                        sets them to 0 only for a non-finite quotient; 1e-20 is the reference's
 //                        absolute rule (MASSolver.cu:43-56)
@@ -122,6 +128,7 @@ namespace psmas
         int apply_workgroups = 0;
         bool apply_wave = true;
         bool single_reduction = false;
+        bool spmv_symmetric = true;
         bool learned_linear = false;
     };
 

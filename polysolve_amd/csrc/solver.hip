@@ -186,7 +186,7 @@ namespace psmas
             before.spmv_halo != params_.spmv_halo || before.spmv_copies != params_.spmv_copies ||
             before.spmv_fuse_direction != params_.spmv_fuse_direction ||
             before.apply_workgroups != params_.apply_workgroups || before.apply_wave != params_.apply_wave ||
-            before.single_reduction != params_.single_reduction)
+            before.single_reduction != params_.single_reduction || before.spmv_symmetric != params_.spmv_symmetric)
             graphs_->clear();
     }
 
@@ -317,6 +317,14 @@ namespace psmas
           << ", \"couplings\": " << n_coupl_ << "}"
           << ", \"spmv_halo\": {\"active\": " << (params_.spmv_halo && halo_max_ > 0 ? "true" : "false")
           << ", \"max_nodes\": " << halo_max_ << ", \"total_nodes\": " << halo_total_ << "}"
+          << ", \"spmv_symmetric\": {\"fp32\": " << [this] {
+                 SymView<float> v;
+                 return sym_view(v) ? "true" : "false";
+             }() << ", \"fp64\": " << [this] {
+                 SymView<double> v;
+                 return sym_view(v) ? "true" : "false";
+             }() << ", \"slots\": " << sym_slots_ << ", \"full_slots\": " << nslots_
+          << ", \"incoming_max\": " << sym_incoming_max_ << ", \"note\": \"" << sym_note_ << "\"}"
           << ", \"coarse_restriction\": {\"thread_nodes\": " << nrsmall_ << ", \"wave_pieces\": " << nrbig_
           << ", \"split_nodes\": " << nrcomb_ << "}"
           << ", \"learned\": {\"used\": " << (learned_used_ ? "true" : "false")
@@ -466,6 +474,18 @@ namespace psmas
         return static_cast<double>(nnzb_) * (bd_ * bd_ * val_bytes + (halo ? 2 : 4)) +
                (halo ? static_cast<double>(halo_total_) * 4.0 + (nblocks_ + 1) * 4.0 : (nb_total_ + 1) * 4.0) +
                static_cast<double>(vec_passes) * n_total_ * x_bytes;
+    }
+
+    template <typename TV>
+    double Solver::spmv_ap_bytes() const
+    {
+        SymView<TV> v;
+        if (copies_ != 1 || !sym_view(v))
+            return spmv_bytes(sizeof(TV), sizeof(TV), 2);
+        // Stored blocks with their 32-bit metadata, the halo lists, one row word per row, x and y.
+        return static_cast<double>(sym_stored_) * (bd_ * bd_ * sizeof(TV) + 4) +
+               static_cast<double>(halo_total_) * 4.0 + (nblocks_ + 1) * 4.0 + static_cast<double>(npad_) * 4.0 +
+               2.0 * n_total_ * sizeof(TV);
     }
 
     void Solver::read_scalars()
@@ -700,6 +720,7 @@ namespace psmas
         h_select_.assign(K, 1);
 
         d_vals32_.reset();
+        reset_sym_values();
         d_mats16_.reset();
         d_dscale_.reset();
         lattice_stale_ = true;
@@ -717,6 +738,7 @@ namespace psmas
         allocate_vectors();
         if (params_.precision == Precision::Mixed)
             ensure_mixed_data();
+        prepare_sym<double>();
         phase("vectors_and_mixed_data");
         factorized_ = true;
     }
@@ -1151,12 +1173,14 @@ namespace psmas
         d_dense_ids_.reset();
         d_dense_index_.reset();
         d_vals32_.reset();
+        reset_sym_values();
         d_mats16_.reset();
         d_dscale_.reset();
         rhs_ready_ = false;
         build_bank_inverses();
         if (params_.precision == Precision::Mixed)
             ensure_mixed_data();
+        prepare_sym<double>();
         PSMAS_HIP(hipStreamSynchronize(stream_));
         factorize_mode_ = "values";
         factorized_ = true;
@@ -1228,7 +1252,7 @@ namespace psmas
             row_len[r] = bsr.rows[r + 1] - bsr.rows[r];
         for (int s = 0; s < nslices_; ++s)
         {
-            int mx = 0;
+            int mx = 1; // every slice at least one slot (as the device build): each row reads its own node
             for (int l = 0; l < WAVE; ++l)
                 mx = std::max(mx, row_len[s * WAVE + l]);
             slice_ptr[s + 1] = slice_ptr[s] + mx;
@@ -1297,6 +1321,17 @@ namespace psmas
         halo_max_ = 0;
         halo_total_ = 0;
         halo_own_ok_ = false;
+        sym_built_ = false;
+        sym_slots_ = 0;
+        sym_stored_ = 0;
+        sym_incoming_max_ = 0;
+        sym_note_ = "no halo lists";
+        d_sym_slice_ptr_.reset();
+        d_sym_src_.reset();
+        d_sym_mirror_.reset();
+        d_sym_meta_.reset();
+        d_sym_info_.reset();
+        reset_sym_values();
         d_lcols_.reset();
         d_halo_ptr_.reset();
         d_halo_.reset();
@@ -1393,6 +1428,107 @@ namespace psmas
         halo_own_ok_ = own_ok;
         halo_max_ = mx;
         halo_total_ = static_cast<long long>(all.size());
+        build_sym(slice_ptr, cols, lcols);
+    }
+
+    void Solver::build_sym(const std::vector<int> &slice_ptr, const std::vector<int> &cols,
+                           const std::vector<unsigned short> &lcols)
+    {
+        std::vector<int> row_len(static_cast<size_t>(npad_));
+        PSMAS_HIP(hipMemcpy(row_len.data(), d_row_len_.data(), sizeof(int) * row_len.size(), hipMemcpyDeviceToHost));
+        SymSell S;
+        if (!build_symmetric_sell(npad_, slice_ptr, cols, lcols, row_len, S))
+        {
+            sym_note_ = S.reason;
+            return;
+        }
+        auto up = [](auto &buf, const auto &v, const char *what) {
+            buf.allocate(std::max<size_t>(1, v.size()), what);
+            if (!v.empty())
+                PSMAS_HIP(hipMemcpy(buf.data(), v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice));
+        };
+        up(d_sym_slice_ptr_, S.slice_ptr, "MAS symmetric SELL slice pointers");
+        up(d_sym_meta_, S.meta, "MAS symmetric SELL columns");
+        up(d_sym_src_, S.src, "MAS symmetric SELL value map");
+        up(d_sym_mirror_, S.mirror, "MAS symmetric SELL value map");
+        up(d_sym_info_, S.row_info, "MAS symmetric SELL rows");
+        sym_slots_ = S.slots;
+        sym_stored_ = S.stored;
+        sym_incoming_max_ = S.incoming_max;
+        sym_built_ = true;
+        sym_note_.clear();
+    }
+
+    void Solver::reset_sym_values()
+    {
+        d_symv32_.reset();
+        d_symv64_.reset();
+        sym_values_ok_ = -1;
+    }
+
+    template <typename TV>
+    bool Solver::prepare_sym()
+    {
+        if (!params_.spmv_symmetric || !sym_built_ || copies_ != 1 || !params_.spmv_halo || halo_max_ <= 0)
+            return false;
+        const size_t lds = (static_cast<size_t>(halo_max_) * bd_ * sizeof(TV) + 15) / 16 * 16 +
+                           static_cast<size_t>(sym_incoming_max_) * bd_ * sizeof(TV);
+        if (lds > SPMV_SYM_LDS_CAP)
+            return false;
+        const long long entries = sym_slots_ * WAVE;
+        if (use_scratch_) // (the allocation-free solve uses what exists)
+        {
+            SymView<TV> v;
+            return sym_view(v);
+        }
+        if (sym_values_ok_ < 0)
+        {
+            DeviceBuffer<int> flag(1, "MAS symmetry check");
+            launch_fill<int>(1, flag.data(), 0, stream_);
+            launch_sym_check(bd_, entries, d_sym_src_.data(), d_sym_mirror_.data(), d_vals64_.data(), 1e-12,
+                             flag.data(), stream_);
+            int h = 0;
+            PSMAS_HIP(hipMemcpyAsync(&h, flag.data(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+            PSMAS_HIP(hipStreamSynchronize(stream_));
+            sym_values_ok_ = h == 0 ? 1 : 0;
+            sym_note_ = h == 0 ? "" : "the matrix is not symmetric to 1e-12 of its largest entries";
+        }
+        if (sym_values_ok_ == 0)
+            return false;
+        DeviceBuffer<TV> &buf = [this]() -> DeviceBuffer<TV> & {
+            if constexpr (std::is_same_v<TV, float>)
+                return d_symv32_;
+            else
+                return d_symv64_;
+        }();
+        if (!buf)
+        {
+            buf.allocate(static_cast<size_t>(entries) * bd_ * bd_, "MAS symmetric SELL values");
+            launch_sym_gather<TV>(bd_, entries, d_sym_src_.data(), d_vals64_.data(), buf.data(), stream_);
+        }
+        return true;
+    }
+
+    template <typename TV>
+    bool Solver::sym_view(SymView<TV> &out) const
+    {
+        const DeviceBuffer<TV> &buf = [this]() -> const DeviceBuffer<TV> & {
+            if constexpr (std::is_same_v<TV, float>)
+                return d_symv32_;
+            else
+                return d_symv64_;
+        }();
+        if (!params_.spmv_symmetric || !sym_built_ || sym_values_ok_ != 1 || copies_ != 1 || !params_.spmv_halo ||
+            halo_max_ <= 0 || !buf)
+            return false;
+        out.nslices = nslices_;
+        out.slice_ptr = d_sym_slice_ptr_.data();
+        out.meta = d_sym_meta_.data();
+        out.vals = buf.data();
+        out.row_info = d_sym_info_.data();
+        out.incoming_max = sym_incoming_max_;
+        out.unroll = params_.spmv_unroll;
+        return true;
     }
 
     void Solver::build_mas()
@@ -1688,6 +1824,7 @@ namespace psmas
             launch_copy_convert<double, float>(static_cast<long long>(d_vals32_.size()), d_vals64_.data(),
                                                d_vals32_.data(), stream_);
         }
+        prepare_sym<float>();
         const int dim = BANK * bd_;
         if (!d_mats16_)
         {
@@ -1911,6 +2048,7 @@ namespace psmas
         }
         else
         {
+            prepare_sym<double>(); // (built with the factorisation unless the parameters changed since)
             solve_fp64();
         }
         PSMAS_HIP(hipStreamSynchronize(stream_));
@@ -1966,7 +2104,13 @@ namespace psmas
                 launch_spmv_ap<BD, T, T>(A, x + o, y + o, reductions_of_copy(c), sysmap_of_copy(c), stream_);
             }
         else
-            launch_spmv_ap<BD, T, T>(A, x, y, reductions(), sysmap(), stream_);
+        {
+            SymView<T> S;
+            if (sym_view(S))
+                launch_spmv_sym<BD, T, T>(A, S, x, y, reductions(), sysmap(), stream_);
+            else
+                launch_spmv_ap<BD, T, T>(A, x, y, reductions(), sysmap(), stream_);
+        }
     }
 
     template <int BD, typename TX, typename TY>
@@ -2041,7 +2185,7 @@ namespace psmas
         }
         else
         {
-            tic("spmv_fp64", spmv_bytes(8, 8, 2));
+            tic("spmv_fp64", spmv_ap_bytes<double>());
             spmv_ap_all<BD, double>(A, d_p_.data(), d_Ap_.data());
             toc();
         }
@@ -2182,7 +2326,7 @@ namespace psmas
         }
         else
         {
-            tic("spmv_fp32", spmv_bytes(4, 4, 2));
+            tic("spmv_fp32", spmv_ap_bytes<float>());
             spmv_ap_all<BD, float>(A, d_p32_.data(), d_Ap32_.data());
             toc();
         }
@@ -2830,16 +2974,29 @@ namespace psmas
         launch_gather_in<double>(npad_, bd_, S.n, d_block_of_slot_.data(), d_io_.data(), d_p_.data(), stream_);
         if (fp32)
             ensure_mixed_data();
+        else
+            prepare_sym<double>();
+        // The kernel the PCG loop uses for y = A p (the symmetric form when it applies).
         auto body = [&](auto bdc) {
             constexpr int BD = decltype(bdc)::value;
             if (!fp32)
             {
-                launch_spmv_plain<BD, double, double>(sell(d_vals64_.data()), d_p_.data(), d_Ap_.data(), stream_);
+                SymView<double> S;
+                if (sym_view(S))
+                    launch_spmv_sym<BD, double, double>(sell(d_vals64_.data()), S, d_p_.data(), d_Ap_.data(),
+                                                        reductions(), sysmap(), stream_);
+                else
+                    launch_spmv_plain<BD, double, double>(sell(d_vals64_.data()), d_p_.data(), d_Ap_.data(), stream_);
             }
             else
             {
                 launch_copy_convert<double, float>(nvec, d_p_.data(), d_p32_.data(), stream_);
-                launch_spmv_plain<BD, float, float>(sell(d_vals32_.data()), d_p32_.data(), d_Ap32_.data(), stream_);
+                SymView<float> S;
+                if (sym_view(S))
+                    launch_spmv_sym<BD, float, float>(sell(d_vals32_.data()), S, d_p32_.data(), d_Ap32_.data(),
+                                                      reductions(), sysmap(), stream_);
+                else
+                    launch_spmv_plain<BD, float, float>(sell(d_vals32_.data()), d_p32_.data(), d_Ap32_.data(), stream_);
                 launch_copy_convert<float, double>(nvec, d_Ap32_.data(), d_Ap_.data(), stream_);
             }
         };

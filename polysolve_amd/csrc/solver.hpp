@@ -196,6 +196,18 @@ namespace psmas
         long long halo_total_ = 0;
         DeviceBuffer<double> d_vals64_;
         DeviceBuffer<float> d_vals32_;
+        // Symmetric form of the matrix (SymView; spmv_symmetric): the structure is built with the
+        // halo lists, the values of each type on first use (from d_vals64_), after a check that
+        // the blocks it stores once are symmetric (sym_values_ok_: -1 not checked yet).
+        DeviceBuffer<int> d_sym_slice_ptr_, d_sym_src_, d_sym_mirror_;
+        DeviceBuffer<uint32_t> d_sym_meta_, d_sym_info_;
+        DeviceBuffer<float> d_symv32_;
+        DeviceBuffer<double> d_symv64_;
+        long long sym_slots_ = 0, sym_stored_ = 0;
+        int sym_incoming_max_ = 0;
+        bool sym_built_ = false;
+        int sym_values_ok_ = -1;
+        std::string sym_note_;
 
         // systems
         DeviceBuffer<int> d_block_sys_, d_blk_begin_, d_blk_count_, d_active_, d_select_, d_bank_sys_;
@@ -329,6 +341,18 @@ namespace psmas
         FineRestrict fine_restrict() const;
         template <typename TV>
         SellView<TV> sell(const TV *vals) const;
+        /// The symmetric form for value type TV when the single-matrix SpMV may use it
+        /// (spmv_symmetric, structure built, values symmetric, LDS within 64 KB); builds its
+        /// values on first use.  Returns false otherwise.
+        template <typename TV>
+        bool sym_view(SymView<TV> &out) const;
+        /// Checks the values' symmetry (once per values) and builds the TV values when the
+        /// symmetric form applies; outside graph capture only (it allocates and synchronizes).
+        template <typename TV>
+        bool prepare_sym();
+        void build_sym(const std::vector<int> &slice_ptr, const std::vector<int> &cols,
+                       const std::vector<unsigned short> &lcols);
+        void reset_sym_values();
 
         /// Domains of system S: Morton chunks (coordinates), else the graph aggregation on the
         /// GPU when dcsc (A in device memory) is given, else on the host.
@@ -339,6 +363,10 @@ namespace psmas
         /// Algorithmic bytes of one SpMV: values (val_bytes each), columns, the halo lists when in
         /// use, and vec_passes row vectors of x_bytes per scalar.
         double spmv_bytes(int val_bytes, int x_bytes, int vec_passes) const;
+        /// Algorithmic bytes of one y = A p launch of the PCG loop (the symmetric form's when it
+        /// is used).
+        template <typename TV>
+        double spmv_ap_bytes() const;
         void build_mas();
         /// Several right-hand sides of the single-system factorisation just built: per-copy node
         /// maps (coarse map, restriction lists and masks, slot maps, systems), level-major

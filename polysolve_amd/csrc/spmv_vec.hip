@@ -237,6 +237,198 @@ namespace psmas
             store_partial<SPMV_BLOCK>(dot, red.pap, blk);
         }
 
+        /// Byte offset of the incoming arrays of k_spmv_sym in the dynamic LDS (after the halo).
+        template <int BD, typename TX>
+        __host__ __device__ constexpr size_t sym_incoming_offset(int halo_max)
+        {
+            return (static_cast<size_t>(halo_max) * BD * sizeof(TX) + 15) / 16 * 16;
+        }
+
+        /// k_spmv_ap over the symmetric form S: each row works through its stored slots U at a time
+        /// as SellRow does (x from the workgroup's halo copy, the next batch's metadata issued
+        /// ahead of the FMAs, its values after them).  A slot of an owned block that carries an
+        /// incoming slot also multiplies the block's transpose with the row's own x (slot 0 is the
+        /// diagonal block, so its column is the row's own node) and stores the product in the
+        /// incoming arrays (component i at inc[i * incoming_max + slot]); a foreign slot stores its
+        /// product A_rc x_c there instead of adding it to its own row.  After the workgroup
+        /// barrier each row adds its incoming products in slot order.  Fixed order throughout:
+        /// reproducible.
+        template <int BD, int U, typename TV, typename TX>
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_sym(SellView<TV> A, SymView<TV> S,
+                                                                 const TX *__restrict__ x, TX *__restrict__ y,
+                                                                 Reductions red, SysMap sm)
+        {
+            using TA = typename Acc<TV>::type;
+            constexpr uint32_t NONE = 0x7FFFu, FOREIGN = 0x8000u; // (fields of meta >> 16)
+            const int blk = xcd_block();
+            const int sys = system_of_block(sm, blk);
+            if (system_stopped(sm, sys))
+                return;
+            const int lane = threadIdx.x % WAVE;
+            const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
+            const HaloStage hs = halo_indices(A, blk);
+            const int beg = __builtin_amdgcn_readfirstlane(S.slice_ptr[slice]);
+            const int end = __builtin_amdgcn_readfirstlane(S.slice_ptr[slice + 1]);
+            const int last = end - 1;
+            const uint32_t *__restrict__ meta = S.meta + lane;
+            const TV *__restrict__ vals = S.vals + lane;
+            const uint32_t info = S.row_info[static_cast<size_t>(slice) * WAVE + lane];
+            uint32_t m[U];
+            TV v[U][BD * BD];
+            auto load_meta = [&](int k) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    m[u] = ldg_stream(meta + static_cast<size_t>(min(k + u, last)) * WAVE);
+            };
+            auto load_values = [&](int k) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    const size_t slot = static_cast<size_t>(min(k + u, last));
+#pragma unroll
+                    for (int e = 0; e < BD * BD; ++e)
+                        v[u][e] = ldg_stream(vals + (slot * (BD * BD) + e) * WAVE);
+                }
+            };
+            load_meta(beg); // every slice has at least one slot
+            load_values(beg);
+            const TX *xs = halo_fill<BD>(hs, A, x);
+            TA *inc = reinterpret_cast<TA *>(spmv_halo_lds + sym_incoming_offset<BD, TX>(A.halo_max));
+            const int im = S.incoming_max;
+            TA xself[BD];
+            {
+                const int ls = static_cast<int>(m[0] & 0xFFFFu); // slot 0: the diagonal block
+#pragma unroll
+                for (int j = 0; j < BD; ++j)
+                    xself[j] = static_cast<TA>(xs[ls * BD + j]);
+            }
+            TA acc[BD];
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+                acc[i] = TA(0);
+            for (int k = beg; k < end; k += U)
+            {
+                TA xv[U][BD];
+                uint32_t ts[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    const int c = static_cast<int>(m[u] & 0xFFFFu);
+                    ts[u] = m[u] >> 16;
+#pragma unroll
+                    for (int j = 0; j < BD; ++j)
+                        xv[u][j] = static_cast<TA>(xs[c * BD + j]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                load_meta(k + U);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    const bool ok = k + u < end; // wave-uniform
+                    const bool fo = (ts[u] & FOREIGN) != 0;
+                    const uint32_t t = ts[u] & NONE;
+                    TA r[BD];
+#pragma unroll
+                    for (int i = 0; i < BD; ++i)
+                    {
+                        r[i] = fo ? TA(0) : acc[i];
+#pragma unroll
+                        for (int j = 0; j < BD; ++j)
+                            r[i] = fma(ok ? static_cast<TA>(v[u][i * BD + j]) : TA(0), xv[u][j], r[i]);
+                    }
+                    if (fo)
+                    {
+                        if (ok)
+#pragma unroll
+                            for (int i = 0; i < BD; ++i)
+                                inc[i * im + static_cast<int>(t)] = r[i];
+                    }
+                    else
+                    {
+#pragma unroll
+                        for (int i = 0; i < BD; ++i)
+                            acc[i] = r[i];
+                    }
+                    if (ok && !fo && t != NONE)
+                    {
+#pragma unroll
+                        for (int j = 0; j < BD; ++j)
+                        {
+                            TA tr = TA(0);
+#pragma unroll
+                            for (int i = 0; i < BD; ++i)
+                                tr = fma(static_cast<TA>(v[u][i * BD + j]), xself[i], tr);
+                            inc[j * im + static_cast<int>(t)] = tr;
+                        }
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                if (k + U < end) // wave-uniform
+                    load_values(k + U);
+            }
+            __syncthreads();
+            const int lr = static_cast<int>(info & 0xFFu), cnt = static_cast<int>((info >> 8) & 0xFFu);
+            const int ib = static_cast<int>(info >> 16);
+            for (int q = 0; q < cnt; ++q)
+#pragma unroll
+                for (int i = 0; i < BD; ++i)
+                    acc[i] += inc[i * im + ib + q];
+            const size_t row = static_cast<size_t>(blk) * NODE_BLOCK + lr;
+            double dot = 0.0;
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+            {
+                const TX yi = static_cast<TX>(acc[i]);
+                y[row * BD + i] = yi;
+                dot += static_cast<double>(xself[i]) * static_cast<double>(yi);
+            }
+            store_partial<SPMV_BLOCK>(dot, red.pap, blk);
+        }
+
+        /// Values of the symmetric form: one thread per sym entry (slot * 64 + lane).
+        template <typename TV>
+        __global__ __launch_bounds__(256) void k_sym_gather(int bs, long long entries, const int *__restrict__ src,
+                                                            const double *__restrict__ v64, TV *__restrict__ out)
+        {
+            const long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+            if (e >= entries)
+                return;
+            const long long slot = e / WAVE, lane = e % WAVE;
+            const int s = src[e];
+            const long long ss = s / WAVE, sl = s % WAVE;
+            for (int el = 0; el < bs; ++el)
+                out[(slot * bs + el) * WAVE + lane] = s < 0 ? TV(0) : static_cast<TV>(v64[(ss * bs + el) * WAVE + sl]);
+        }
+
+        /// Symmetry of the blocks the symmetric form stores once: block src[e] against the
+        /// transpose of block mirror[e], relative to the pair's largest entry.
+        __global__ __launch_bounds__(256) void k_sym_check(int bd, long long entries, const int *__restrict__ src,
+                                                           const int *__restrict__ mirror,
+                                                           const double *__restrict__ v64, double rel_tol,
+                                                           int *__restrict__ flag)
+        {
+            const long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+            if (e >= entries)
+                return;
+            const int s = src[e], t = mirror[e];
+            if (s < 0 || t < 0)
+                return;
+            const int bs = bd * bd;
+            const long long ss = s / WAVE, sl = s % WAVE, ts = t / WAVE, tl = t % WAVE;
+            double big = 0.0, diff = 0.0;
+            for (int i = 0; i < bd; ++i)
+                for (int j = 0; j < bd; ++j)
+                {
+                    const double a = v64[(ss * bs + i * bd + j) * WAVE + sl];
+                    const double b = v64[(ts * bs + j * bd + i) * WAVE + tl];
+                    big = fmax(big, fmax(fabs(a), fabs(b)));
+                    diff = fmax(diff, fabs(a - b));
+                }
+            if (!(diff <= rel_tol * big))
+                flag[0] = 1;
+        }
+
         /// Streaming-read probe: every 16-byte chunk once (nontemporal); the sum is stored only
         /// under an impossible condition, which keeps the loads.
         typedef float probe_f4 __attribute__((ext_vector_type(4)));
@@ -1137,6 +1329,64 @@ namespace psmas
         PSMAS_LAUNCH_CHECK();
     }
 
+    template <int BD, typename TV, typename TX>
+    size_t spmv_sym_lds(int halo_max, int incoming_max)
+    {
+        return sym_incoming_offset<BD, TX>(halo_max) +
+               static_cast<size_t>(incoming_max) * BD * sizeof(typename Acc<TV>::type);
+    }
+
+    template <int BD, typename TV, typename TX>
+    void launch_spmv_sym(const SellView<TV> &A, const SymView<TV> &S, const TX *x, TX *y, const Reductions &red,
+                         const SysMap &sm, hipStream_t s)
+    {
+        require_device_ptrs("spmv_sym", {A.halo_ptr, A.halo, S.slice_ptr, S.meta, S.vals, S.row_info, x, y});
+        if (A.halo_max <= 0 || S.nslices != A.nslices || S.incoming_max < 0)
+            throw std::logic_error("[MAS] internal error: the symmetric SpMV needs the halo lists of its matrix.");
+        const size_t lds = spmv_sym_lds<BD, TV, TX>(A.halo_max, S.incoming_max);
+        if (lds > SPMV_SYM_LDS_CAP)
+            throw std::logic_error("[MAS] internal error: the symmetric SpMV's LDS exceeds its cap.");
+        const int grid = A.nslices / (SPMV_BLOCK / WAVE);
+        // Above 64 KB of dynamic LDS a kernel must be allowed it explicitly (once per instance).
+        auto allow = [](const void *fn) {
+            PSMAS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, SPMV_SYM_LDS_CAP));
+        };
+#define PSMAS_SYM_LAUNCH                                                                                               \
+    {                                                                                                                  \
+        if (lds > 65536)                                                                                               \
+        {                                                                                                              \
+            static const bool allowed = (allow(reinterpret_cast<const void *>(&k_spmv_sym<BD, U, TV, TX>)), true);     \
+            (void)allowed;                                                                                             \
+        }                                                                                                              \
+        hipLaunchKernelGGL((k_spmv_sym<BD, U, TV, TX>), dim3(grid), dim3(SPMV_BLOCK), lds, s, A, S, x, y, red, sm);    \
+    }
+        PSMAS_UNROLL_DISPATCH(S.unroll, PSMAS_SYM_LAUNCH);
+#undef PSMAS_SYM_LAUNCH
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    template <typename TV>
+    void launch_sym_gather(int bd, long long entries, const int *src, const double *vals64, TV *out, hipStream_t s)
+    {
+        if (entries <= 0)
+            return;
+        require_device_ptrs("sym_gather", {src, vals64, out});
+        hipLaunchKernelGGL((k_sym_gather<TV>), dim3(static_cast<unsigned>((entries + 255) / 256)), dim3(256), 0, s,
+                           bd * bd, entries, src, vals64, out);
+        PSMAS_LAUNCH_CHECK();
+    }
+
+    void launch_sym_check(int bd, long long entries, const int *src, const int *mirror, const double *vals64,
+                          double rel_tol, int *flag, hipStream_t s)
+    {
+        if (entries <= 0)
+            return;
+        require_device_ptrs("sym_check", {src, mirror, vals64, flag});
+        hipLaunchKernelGGL(k_sym_check, dim3(static_cast<unsigned>((entries + 255) / 256)), dim3(256), 0, s, bd,
+                           entries, src, mirror, vals64, rel_tol, flag);
+        PSMAS_LAUNCH_CHECK();
+    }
+
     template <int BD>
     void launch_spmv_cg(const SellView<float> &A, const CgProlong &pr, float *u, float *w, const float *r,
                         const float *s_prev, const float *u_prev, const CgReductions &red, const SysMap &sm,
@@ -1437,6 +1687,13 @@ namespace psmas
     }
 
 #define PSMAS_INST_BD(BD)                                                                                          \
+    template void launch_spmv_sym<BD, double, double>(const SellView<double> &, const SymView<double> &,          \
+                                                      const double *, double *, const Reductions &, const SysMap &, \
+                                                      hipStream_t);                                               \
+    template void launch_spmv_sym<BD, float, float>(const SellView<float> &, const SymView<float> &, const float *, \
+                                                    float *, const Reductions &, const SysMap &, hipStream_t);    \
+    template size_t spmv_sym_lds<BD, double, double>(int, int);                                                  \
+    template size_t spmv_sym_lds<BD, float, float>(int, int);                                                    \
     template void launch_spmv_ap<BD, double, double>(const SellView<double> &, const double *, double *,          \
                                                      const Reductions &, const SysMap &, hipStream_t);            \
     template void launch_spmv_ap<BD, float, float>(const SellView<float> &, const float *, float *,               \
@@ -1486,6 +1743,8 @@ namespace psmas
     PSMAS_INST_BD(3)
 #undef PSMAS_INST_BD
 
+    template void launch_sym_gather<double>(int, long long, const int *, const double *, double *, hipStream_t);
+    template void launch_sym_gather<float>(int, long long, const int *, const double *, float *, hipStream_t);
     template void launch_copy_convert<double, float>(long long, const double *, float *, hipStream_t);
     template void launch_copy_convert<float, double>(long long, const float *, double *, hipStream_t);
     template void launch_copy_convert<double, double>(long long, const double *, double *, hipStream_t);

@@ -59,7 +59,7 @@ def test_inner_object_extension():
 
 
 @pytest.mark.parametrize("key,default", [("spmv_fuse_direction", False), ("single_reduction", False),
-                                         ("apply_wave", True)])
+                                         ("apply_wave", True), ("spmv_symmetric", True)])
 def test_kernel_variant_switches(key, default):
     """Boolean switches between measured variants of the iteration's kernels: the default, the
     other value, and a non-boolean refused with the key named."""

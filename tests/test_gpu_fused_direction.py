@@ -11,9 +11,10 @@ from polysolve_amd import MASSolver, problems
 
 pytestmark = pytest.mark.gpu
 
-# The fused update is a form of the classic iteration (two reductions per iteration).
+# The fused update is a form of the classic iteration (two reductions per iteration) over the full
+# matrix: the separate update is compared on the same matrix form.
 BASE = {"block_dim": 3, "coarse_solver": "cholesky", "relative_tolerance": 1e-8, "absolute_tolerance": 0.0,
-        "max_iter": 5000, "single_reduction": False}
+        "max_iter": 5000, "single_reduction": False, "spmv_symmetric": False}
 
 
 def _solver(p, **extra):

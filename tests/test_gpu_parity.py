@@ -506,9 +506,10 @@ def test_spmv_halo_matches_global_gathers(gpu, het12, precision):
     """x staged per workgroup in LDS (16-bit local columns) gives the same products, bit for bit,
     as gathering x through the 32-bit columns: a whole solve (its A p and b - A x kernels) ends
     with the same iterate."""
-    # (The single-reduction iteration needs the halo form: both solves use the classic one.)
+    # (The single-reduction iteration and the symmetric form need the halo lists: both solves use
+    # the classic iteration over the full matrix.)
     s = _factorized(het12, {"precision": precision, "relative_tolerance": 1e-8, "absolute_tolerance": 0.0,
-                            "single_reduction": False})
+                            "single_reduction": False, "spmv_symmetric": False})
     info = s.get_info()["spmv_halo"]
     assert info["active"] and 0 < info["max_nodes"] <= 2048 and info["total_nodes"] >= het12.n // 3
     xs_halo = np.zeros(het12.n)

@@ -1,0 +1,110 @@
+"""The symmetric form of the matrix (spmv_symmetric): a block coupling two rows of one 256-node
+block is stored once and applied to both rows, and rows longer than their node block's mean hand
+cross-block blocks to short rows.  Checked against scipy's A @ x with the bounds of the full SpMV,
+against the full form inside whole solves (and the spsolve solution), for reproducibility, and for
+the case where it must switch itself off (values that are not symmetric)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from polysolve_amd import MASSolver, problems
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"relative_tolerance": 1e-8, "absolute_tolerance": 0.0, "max_iter": 5000}
+
+
+@pytest.fixture(scope="module")
+def het12():
+    return problems.hex_elasticity(12, E=problems.random_moduli(12, 3), name="het12")
+
+
+def _solver(p, **prm):
+    s = MASSolver()
+    s.set_parameters({"MAS": dict({"block_dim": 3}, **prm)})
+    s.set_coordinates(p.coords)
+    s.factorize(p.A)
+    return s
+
+
+@pytest.mark.parametrize("bd", [1, 2, 3])
+def test_symmetric_spmv_matches_scipy(gpu, het12, bd):
+    p = het12
+    s = _solver(p, block_dim=bd, precision="mixed")
+    info = s.get_info()["spmv_symmetric"]
+    assert info["fp32"], info
+    assert 0 < info["slots"] < info["full_slots"] and info["incoming_max"] > 0
+    x = np.random.default_rng(bd).standard_normal(p.n)
+    ref = p.A @ x
+    y32 = s.spmv(x, fp32=True)
+    # FP32 values, vectors and accumulation: the full SpMV's bound.
+    assert np.linalg.norm(y32 - ref) <= 2e-6 * np.linalg.norm(ref)
+    y64 = s.spmv(x)
+    assert np.linalg.norm(y64 - ref) <= 1e-13 * np.linalg.norm(ref)
+    # Reproducible (fixed summation order), and against the full form of the same FP32 values.
+    assert np.array_equal(s.spmv(x, fp32=True), y32)
+    s.set_parameters({"MAS": {"spmv_symmetric": False}})
+    assert not s.get_info()["spmv_symmetric"]["fp32"]
+    y32f = s.spmv(x, fp32=True)
+    assert np.linalg.norm(y32 - y32f) <= 2e-6 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_symmetric_solve_matches_full_and_spsolve(gpu, het12, precision):
+    p = het12
+    coarse = "cholesky" if precision == "mixed" else "bank"
+    s = _solver(p, precision=precision, coarse_solver=coarse, **TOL)
+    xd = spla.spsolve(p.A.tocsc(), p.b)
+    out = {}
+    for sym in (True, False):
+        s.set_parameters({"MAS": {"spmv_symmetric": sym}})
+        x = np.zeros(p.n)
+        s.solve(p.b, x)
+        info = s.get_info()
+        assert info["solver_status"] == "Reach relative tolerance"
+        assert np.linalg.norm(p.b - p.A @ x) <= 1e-8 * np.linalg.norm(p.b) * 1.01
+        assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+        out[sym] = (x, info["solver_iter"])
+    # Another summation order in A p: the iteration count moves by at most a few check intervals.
+    assert abs(out[True][1] - out[False][1]) <= 30
+
+
+def test_symmetric_solves_reproducible_with_and_without_graphs(gpu, het12):
+    p = het12
+    s = _solver(p, precision="mixed", coarse_solver="cholesky", **TOL)
+    assert s.get_info()["spmv_symmetric"]["fp32"]
+    xs = []
+    for graphs in (True, True, False):
+        s.set_parameters({"MAS": {"use_graphs": graphs}})
+        x = np.zeros(p.n)
+        s.solve(p.b, x)
+        xs.append(x)
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
+
+
+def test_asymmetric_values_switch_the_symmetric_form_off(gpu, het12):
+    """Values that differ from their mirror by more than 1e-12 of the pair's largest entry make
+    the SpMV read every block in its own row; a values-only refactorisation with symmetric values
+    switches the symmetric form back on."""
+    p = het12
+    A2 = (p.A + 1e-7 * sp.triu(p.A, k=1)).tocsc()  # same pattern, every coupling off its mirror
+    A2.sort_indices()
+    s = MASSolver()
+    s.set_parameters({"MAS": dict({"block_dim": 3, "precision": "mixed", "lazy_partitioning": True}, **TOL)})
+    s.set_coordinates(p.coords)
+    s.factorize(A2)
+    info = s.get_info()["spmv_symmetric"]
+    assert info["slots"] > 0 and not info["fp32"] and not info["fp64"]
+    assert "not symmetric" in info["note"]
+    x = np.zeros(p.n)
+    s.solve(p.b, x)
+    assert s.get_info()["solver_status"] == "Reach relative tolerance"
+    assert np.linalg.norm(p.b - A2 @ x) <= 1e-8 * np.linalg.norm(p.b) * 1.01
+    A = p.A.tocsc()
+    A.sort_indices()
+    s.factorize(A)  # same pattern: values only
+    assert s.get_info()["factorize_mode"] == "values"
+    assert s.get_info()["spmv_symmetric"]["fp32"]
