@@ -94,7 +94,7 @@ def run_path(solver, b_dev, x_dev, n, steps, warmup, sync):
 # (FETCH_SIZE reads 0.500x the bytes such a copy reads, WRITE_SIZE 1.000x).  Keyed by
 # (config, kernel); re-measure when a kernel's memory access changes.
 MEASURED_TRAFFIC = {
-    (2, "spmv_fp32"): 289.8e6 + 3.3e6,
+    (2, "spmv_fp32"): 215.0e6 + 3.3e6,  # the symmetric form (k_spmv_sym<3, 4, float, float>)
 }
 
 
@@ -127,7 +127,7 @@ def bench_batch(args) -> int:
     ntot = sum(p.n for p in probs)
     log(f"assembled 64 systems, sum n = {ntot} ({time.perf_counter() - t:.1f}s)")
     base = {"block_dim": 3, "relative_tolerance": 1e-8, "absolute_tolerance": 0.0, "max_iter": args.max_iter,
-            "precision": "mixed", "coarse_solver": args.coarse}
+            "precision": "mixed", "coarse_solver": args.coarse, "spmv_symmetric": args.spmv == "symmetric"}
     bdev = [DeviceArray(p.n, p.b) for p in probs]
     xdev = [DeviceArray(p.n, np.zeros(p.n)) for p in probs]
     bp, xp = [b.ptr for b in bdev], [x.ptr for x in xdev]
@@ -414,6 +414,8 @@ def main() -> int:
     ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (no HIP graph replay)")
     ap.add_argument("--pcg", default="classic", choices=["single", "classic"],
                     help="mixed-path PCG iteration: the classic two-reduction one (default) or the single-reduction one")
+    ap.add_argument("--spmv", default="symmetric", choices=["symmetric", "full"],
+                    help="form of the matrix the PCG loop's y = A p reads (spmv_symmetric)")
     ap.add_argument("--max-iter", type=int, default=20000, help="iteration cap (short counter-collection runs)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction path")
     ap.add_argument("--profile-learned", action="store_true",
@@ -452,7 +454,7 @@ def main() -> int:
     tol = 1e-8
     base = {"block_dim": 3, "relative_tolerance": tol, "absolute_tolerance": 0.0, "max_iter": args.max_iter,
             "aggregation": args.aggregation, "use_graphs": not args.no_graphs,
-            "single_reduction": args.pcg == "single"}
+            "single_reduction": args.pcg == "single", "spmv_symmetric": args.spmv == "symmetric"}
     solver = MASSolver()
     solver.set_parameters({"MAS": dict(base, precision="fp64")})
     solver.set_coordinates(prob.coords)
