@@ -37,6 +37,7 @@
 #include <mutex>
 #include <sstream>
 #include <atomic>
+#include <future>
 #include <thread>
 
 namespace psmas
@@ -792,7 +793,7 @@ namespace psmas
         std::vector<int> cols(static_cast<size_t>(nslots_) * WAVE);
         PSMAS_HIP(hipMemcpyAsync(cols.data(), d_cols_.data(), sizeof(int) * cols.size(), hipMemcpyDeviceToHost, stream_));
         PSMAS_HIP(hipStreamSynchronize(stream_));
-        build_halo(h_slice_ptr_, cols);
+        build_halo(h_slice_ptr_, std::move(cols));
         phase("halo");
         if (params_.lazy_partitioning && sd.map_ok)
         {
@@ -1313,14 +1314,17 @@ namespace psmas
         PSMAS_HIP(hipMemcpy(d_row_len_.data(), row_len.data(), d_row_len_.bytes(), hipMemcpyHostToDevice));
         PSMAS_HIP(hipMemcpy(d_block_of_slot_.data(), block_of_slot_.data(), d_block_of_slot_.bytes(),
                             hipMemcpyHostToDevice));
-        build_halo(slice_ptr, cols);
+        build_halo(slice_ptr, std::move(cols));
     }
 
-    void Solver::build_halo(const std::vector<int> &slice_ptr, const std::vector<int> &cols)
+    void Solver::build_halo(const std::vector<int> &slice_ptr, std::vector<int> cols)
     {
         halo_max_ = 0;
         halo_total_ = 0;
         halo_own_ok_ = false;
+        if (sym_future_.valid())
+            sym_future_.wait(); // (a previous pattern's build; its result is dropped)
+        sym_future_ = std::future<SymSell>();
         sym_built_ = false;
         sym_slots_ = 0;
         sym_stored_ = 0;
@@ -1428,16 +1432,31 @@ namespace psmas
         halo_own_ok_ = own_ok;
         halo_max_ = mx;
         halo_total_ = static_cast<long long>(all.size());
-        build_sym(slice_ptr, cols, lcols);
+        build_sym(std::vector<int>(slice_ptr), std::move(cols), std::move(lcols));
     }
 
-    void Solver::build_sym(const std::vector<int> &slice_ptr, const std::vector<int> &cols,
-                           const std::vector<unsigned short> &lcols)
+    void Solver::build_sym(std::vector<int> slice_ptr, std::vector<int> cols, std::vector<unsigned short> lcols)
     {
+        // The host build runs beside the rest of the factorisation (GPU work); join_sym() takes
+        // its result before the first use.
         std::vector<int> row_len(static_cast<size_t>(npad_));
         PSMAS_HIP(hipMemcpy(row_len.data(), d_row_len_.data(), sizeof(int) * row_len.size(), hipMemcpyDeviceToHost));
-        SymSell S;
-        if (!build_symmetric_sell(npad_, slice_ptr, cols, lcols, row_len, S))
+        sym_note_ = "being built";
+        sym_future_ = std::async(std::launch::async,
+                                 [npad = npad_, sp = std::move(slice_ptr), c = std::move(cols), lc = std::move(lcols),
+                                  rl = std::move(row_len)]() {
+                                     SymSell S;
+                                     build_symmetric_sell(npad, sp, c, lc, rl, S);
+                                     return S;
+                                 });
+    }
+
+    void Solver::join_sym()
+    {
+        if (!sym_future_.valid())
+            return;
+        SymSell S = sym_future_.get();
+        if (S.slots == 0)
         {
             sym_note_ = S.reason;
             return;
@@ -1469,6 +1488,8 @@ namespace psmas
     template <typename TV>
     bool Solver::prepare_sym()
     {
+        if (!use_scratch_)
+            join_sym();
         if (!params_.spmv_symmetric || !sym_built_ || copies_ != 1 || !params_.spmv_halo || halo_max_ <= 0)
             return false;
         const size_t lds = (static_cast<size_t>(halo_max_) * bd_ * sizeof(TV) + 15) / 16 * 16 +

@@ -19,6 +19,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <future>
 #include <string>
 #include <chrono>
 #include <vector>
@@ -208,6 +209,7 @@ namespace psmas
         bool sym_built_ = false;
         int sym_values_ok_ = -1;
         std::string sym_note_;
+        std::future<SymSell> sym_future_; ///< the host build of the structure, while it runs
 
         // systems
         DeviceBuffer<int> d_block_sys_, d_blk_begin_, d_blk_count_, d_active_, d_select_, d_bank_sys_;
@@ -350,8 +352,8 @@ namespace psmas
         /// symmetric form applies; outside graph capture only (it allocates and synchronizes).
         template <typename TV>
         bool prepare_sym();
-        void build_sym(const std::vector<int> &slice_ptr, const std::vector<int> &cols,
-                       const std::vector<unsigned short> &lcols);
+        void build_sym(std::vector<int> slice_ptr, std::vector<int> cols, std::vector<unsigned short> lcols);
+        void join_sym();
         void reset_sym_values();
 
         /// Domains of system S: Morton chunks (coordinates), else the graph aggregation on the
@@ -359,7 +361,7 @@ namespace psmas
         void build_partition(System &S, const CscView &A, const DeviceCsc *dcsc = nullptr);
         void upload_matrix(const PaddedBsr &bsr);
         /// Per node block, the distinct columns of its SELL slots and the 16-bit local columns.
-        void build_halo(const std::vector<int> &slice_ptr, const std::vector<int> &cols);
+        void build_halo(const std::vector<int> &slice_ptr, std::vector<int> cols);
         /// Algorithmic bytes of one SpMV: values (val_bytes each), columns, the halo lists when in
         /// use, and vec_passes row vectors of x_bytes per scalar.
         double spmv_bytes(int val_bytes, int x_bytes, int vec_passes) const;
