@@ -306,8 +306,8 @@ namespace psmas
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
-            for (int k = beg; k < end; k += U)
-            {
+            // One batch of U slots; only the slice's last batch checks for slots past its end.
+            auto batch = [&](const int k, auto check) {
                 TA xv[U][BD];
                 uint32_t ts[U];
 #pragma unroll
@@ -325,7 +325,7 @@ namespace psmas
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                 {
-                    const bool ok = k + u < end; // wave-uniform
+                    const bool ok = !decltype(check)::value || k + u < end; // wave-uniform
                     const bool fo = (ts[u] & FOREIGN) != 0;
                     const uint32_t t = ts[u] & NONE;
                     TA r[BD];
@@ -366,7 +366,12 @@ namespace psmas
                 __builtin_amdgcn_sched_barrier(0);
                 if (k + U < end) // wave-uniform
                     load_values(k + U);
-            }
+            };
+            int k = beg;
+            for (; k + U <= end; k += U)
+                batch(k, std::false_type());
+            if (k < end)
+                batch(k, std::true_type());
             __syncthreads();
             const int lr = static_cast<int>(info & 0xFFu), cnt = static_cast<int>((info >> 8) & 0xFFu);
             const int ib = static_cast<int>(info >> 16);

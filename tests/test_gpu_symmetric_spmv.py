@@ -108,3 +108,18 @@ def test_asymmetric_values_switch_the_symmetric_form_off(gpu, het12):
     s.factorize(A)  # same pattern: values only
     assert s.get_info()["factorize_mode"] == "values"
     assert s.get_info()["spmv_symmetric"]["fp32"]
+
+
+def test_symmetric_spmv_at_full_size(gpu):
+    """The 64^3 heterogeneous elasticity system (823,875 unknowns): the symmetric form is in use
+    for both precisions, stores about 70 % of the full form's slots, and its products match
+    scipy's A @ x with the full SpMV's bounds."""
+    p = problems.config2(seed=0)
+    s = _solver(p, precision="mixed")
+    info = s.get_info()["spmv_symmetric"]
+    assert info["fp32"] and info["fp64"], info
+    assert info["slots"] < 0.75 * info["full_slots"]
+    x = np.random.default_rng(64).standard_normal(p.n)
+    ref = p.A @ x
+    assert np.linalg.norm(s.spmv(x, fp32=True) - ref) <= 2e-6 * np.linalg.norm(ref)
+    assert np.linalg.norm(s.spmv(x) - ref) <= 1e-13 * np.linalg.norm(ref)
