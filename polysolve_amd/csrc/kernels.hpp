@@ -180,6 +180,11 @@ namespace psmas
                          const SysMap &sm, hipStream_t s);
     /// Largest dynamic LDS launch_spmv_sym takes (bytes; of the 160 KB of a gfx950 CU).
     constexpr size_t SPMV_SYM_LDS_CAP = 120 * 1024;
+    /// launch_spmv_res from the symmetric form S of A (the halo lists are A's).
+    template <int BD, typename TV, typename TX, typename TY>
+    void launch_spmv_res_sym(const SellView<TV> &A, const SymView<TV> &S, const TX *x, const TX *b, TY *y,
+                             FineRestrict fr, TY *yc0, double *rr_partials, const SysMap &sm, bool selected_only,
+                             hipStream_t s);
     /// Dynamic LDS of launch_spmv_sym: the halo of x, then BD incoming arrays.
     template <int BD, typename TV, typename TX>
     size_t spmv_sym_lds(int halo_max, int incoming_max);

@@ -1490,7 +1490,7 @@ namespace psmas
     {
         if (!use_scratch_)
             join_sym();
-        if (!params_.spmv_symmetric || !sym_built_ || copies_ != 1 || !params_.spmv_halo || halo_max_ <= 0)
+        if (!params_.spmv_symmetric || !sym_built_ || copies_ != 1 || !params_.spmv_halo || halo_max_ < NODE_BLOCK)
             return false;
         const size_t lds = (static_cast<size_t>(halo_max_) * bd_ * sizeof(TV) + 15) / 16 * 16 +
                            static_cast<size_t>(sym_incoming_max_) * bd_ * sizeof(TV);
@@ -1540,7 +1540,7 @@ namespace psmas
                 return d_symv64_;
         }();
         if (!params_.spmv_symmetric || !sym_built_ || sym_values_ok_ != 1 || copies_ != 1 || !params_.spmv_halo ||
-            halo_max_ <= 0 || !buf)
+            halo_max_ < NODE_BLOCK || !buf)
             return false;
         out.nslices = nslices_;
         out.slice_ptr = d_sym_slice_ptr_.data();
@@ -2138,6 +2138,13 @@ namespace psmas
     void Solver::spmv_res_all(const SellView<double> &A, const TX *x, const TX *b, TY *y, double *rr, bool selected)
     {
         TY *const level1 = y + static_cast<size_t>(npad_) * BD; // every copy's (copy-shifted ids)
+        SymView<double> S;
+        if (copies_ == 1 && sym_view(S))
+        {
+            launch_spmv_res_sym<BD, double, TX, TY>(A, S, x, b, y, fine_restrict(), level1, rr, sysmap(), selected,
+                                                    stream_);
+            return;
+        }
         for (int c = 0; c < copies_; ++c)
         {
             const size_t o = static_cast<size_t>(c) * npad1_;
@@ -2214,7 +2221,7 @@ namespace psmas
         if (k % params_.true_residual_period == 0)
         {
             launch_axpy_alpha<BD, double>(nblocks_, p, d_x_.data(), reductions(), sysmap(), stream_);
-            tic("spmv_res_fp64", spmv_bytes(8, 8, 2) + n_total_ * 8.0);
+            tic("spmv_res_fp64", spmv_ap_bytes<double>() + n_total_ * 8.0);
             spmv_res_all<BD, double, double>(A, d_x_.data(), d_b_.data(), d_mr_.data(), reductions().rr, false);
             toc();
         }
@@ -2504,7 +2511,7 @@ namespace psmas
     void Solver::mixed_reliable_residual(bool learned)
     {
         const SellView<double> A = sell(d_vals64_.data());
-        tic("spmv_res_fp64", spmv_bytes(8, 8, 3));
+        tic("spmv_res_fp64", spmv_ap_bytes<double>() + n_total_ * 8.0);
         spmv_res_all<BD, double, float>(A, d_x_.data(), d_b_.data(), d_mr32_.data(), reductions().rr_true, true);
         toc();
         // Block path of the learned correction: the preconditioner expects the lattice block

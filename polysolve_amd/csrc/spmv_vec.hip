@@ -244,26 +244,21 @@ namespace psmas
             return (static_cast<size_t>(halo_max) * BD * sizeof(TX) + 15) / 16 * 16;
         }
 
-        /// k_spmv_ap over the symmetric form S: each row works through its stored slots U at a time
-        /// as SellRow does (x from the workgroup's halo copy, the next batch's metadata issued
-        /// ahead of the FMAs, its values after them).  A slot of an owned block that carries an
-        /// incoming slot also multiplies the block's transpose with the row's own x (slot 0 is the
-        /// diagonal block, so its column is the row's own node) and stores the product in the
-        /// incoming arrays (component i at inc[i * incoming_max + slot]); a foreign slot stores its
-        /// product A_rc x_c there instead of adding it to its own row.  After the workgroup
-        /// barrier each row adds its incoming products in slot order.  Fixed order throughout:
-        /// reproducible.
-        template <int BD, int U, typename TV, typename TX>
-        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_sym(SellView<TV> A, SymView<TV> S,
-                                                                 const TX *__restrict__ x, TX *__restrict__ y,
-                                                                 Reductions red, SysMap sm)
+        /// The rows of node block blk over the symmetric form S: halo stage, then every row works
+        /// through its stored slots U at a time as SellRow does (x from the workgroup's halo copy,
+        /// the next batch's metadata issued ahead of the FMAs, its values after them).  A slot of
+        /// an owned block that carries an incoming slot also multiplies the block's transpose with
+        /// the row's own x (slot 0 is the diagonal block, so its column is the row's own node) and
+        /// stores the product in the incoming arrays (component i at inc[i * incoming_max +
+        /// slot]); a foreign slot stores its product A_rc x_c there instead of adding it to its own
+        /// row.  After the workgroup barrier each row adds its incoming products in slot order.
+        /// Fixed order throughout: reproducible.  Returns the row's local index; acc = (A x)_row,
+        /// xself = x_row.  Every thread of the workgroup must call it (it holds barriers).
+        template <int BD, int U, typename TV, typename TX, typename TA>
+        __device__ __forceinline__ int sym_rows(const SellView<TV> &A, const SymView<TV> &S, const TX *__restrict__ x,
+                                                const int blk, TA (&acc)[BD], TA (&xself)[BD])
         {
-            using TA = typename Acc<TV>::type;
             constexpr uint32_t NONE = 0x7FFFu, FOREIGN = 0x8000u; // (fields of meta >> 16)
-            const int blk = xcd_block();
-            const int sys = system_of_block(sm, blk);
-            if (system_stopped(sm, sys))
-                return;
             const int lane = threadIdx.x % WAVE;
             const int slice = blk * (SPMV_BLOCK / WAVE) + threadIdx.x / WAVE;
             const HaloStage hs = halo_indices(A, blk);
@@ -295,14 +290,12 @@ namespace psmas
             const TX *xs = halo_fill<BD>(hs, A, x);
             TA *inc = reinterpret_cast<TA *>(spmv_halo_lds + sym_incoming_offset<BD, TX>(A.halo_max));
             const int im = S.incoming_max;
-            TA xself[BD];
             {
                 const int ls = static_cast<int>(m[0] & 0xFFFFu); // slot 0: the diagonal block
 #pragma unroll
                 for (int j = 0; j < BD; ++j)
                     xself[j] = static_cast<TA>(xs[ls * BD + j]);
             }
-            TA acc[BD];
 #pragma unroll
             for (int i = 0; i < BD; ++i)
                 acc[i] = TA(0);
@@ -379,6 +372,22 @@ namespace psmas
 #pragma unroll
                 for (int i = 0; i < BD; ++i)
                     acc[i] += inc[i * im + ib + q];
+            return lr;
+        }
+
+        /// k_spmv_ap over the symmetric form S.
+        template <int BD, int U, typename TV, typename TX>
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_sym(SellView<TV> A, SymView<TV> S,
+                                                                 const TX *__restrict__ x, TX *__restrict__ y,
+                                                                 Reductions red, SysMap sm)
+        {
+            using TA = typename Acc<TV>::type;
+            const int blk = xcd_block();
+            const int sys = system_of_block(sm, blk);
+            if (system_stopped(sm, sys))
+                return;
+            TA acc[BD], xself[BD];
+            const int lr = sym_rows<BD, U>(A, S, x, blk, acc, xself);
             const size_t row = static_cast<size_t>(blk) * NODE_BLOCK + lr;
             double dot = 0.0;
 #pragma unroll
@@ -912,6 +921,45 @@ namespace psmas
             store_partial<SPMV_BLOCK>(dot, rr_partials, blk);
         }
 
+        /// k_spmv_res over the symmetric form S: the rows come in the form's order, so each
+        /// residual is staged in LDS at its local row for the bank restriction.  The stage reuses
+        /// the halo's LDS (free once every row has passed the barrier in sym_rows; the halo holds
+        /// at least the block's own 256 nodes), which keeps the kernel's LDS that of k_spmv_sym.
+        template <int BD, int U, typename TV, typename TX, typename TY>
+        __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_res_sym(SellView<TV> A, SymView<TV> S,
+                                                                     const TX *__restrict__ x, const TX *__restrict__ b,
+                                                                     TY *__restrict__ y, FineRestrict fr,
+                                                                     TY *__restrict__ yc0, double *__restrict__ rr_partials,
+                                                                     SysMap sm, int selected_only)
+        {
+            static_assert(sizeof(TY) <= sizeof(TX), "the residual stage fits the halo's LDS");
+            using TA = typename Acc<TV>::type;
+            TY *const sv = reinterpret_cast<TY *>(spmv_halo_lds);
+            const int blk = xcd_block();
+            const int sys = system_of_block(sm, blk);
+            if (system_stopped(sm, sys) || (selected_only && sm.select[sys] == 0))
+                return;
+            TA acc[BD], xself[BD];
+            const int lr = sym_rows<BD, U>(A, S, x, blk, acc, xself);
+            const size_t row = static_cast<size_t>(blk) * NODE_BLOCK + lr;
+            double dot = 0.0;
+#pragma unroll
+            for (int i = 0; i < BD; ++i)
+            {
+                const double ri = static_cast<double>(b[row * BD + i]) - static_cast<double>(acc[i]);
+                const TY yi = static_cast<TY>(ri);
+                y[row * BD + i] = yi;
+                sv[lr * BD + i] = yi;
+                dot += ri * ri;
+            }
+            const size_t own = static_cast<size_t>(blk) * NODE_BLOCK + threadIdx.x; // (bank lists by row)
+            const uint32_t mask = fr.lead[own];
+            const int gid = fr.gid[own];
+            __syncthreads();
+            restrict_bank<BD, TY>(sv, (threadIdx.x / 32) * 32, mask, gid, yc0);
+            store_partial<SPMV_BLOCK>(dot, rr_partials, blk);
+        }
+
         template <int BD, int U, typename TV, typename TX>
         __global__ __launch_bounds__(SPMV_BLOCK) void k_spmv_plain(SellView<TV> A, const TX *__restrict__ x,
                                                                    TX *__restrict__ y)
@@ -1370,6 +1418,39 @@ namespace psmas
         PSMAS_LAUNCH_CHECK();
     }
 
+    template <int BD, typename TV, typename TX, typename TY>
+    void launch_spmv_res_sym(const SellView<TV> &A, const SymView<TV> &S, const TX *x, const TX *b, TY *y,
+                             FineRestrict fr, TY *yc0, double *rr_partials, const SysMap &sm, bool selected_only,
+                             hipStream_t s)
+    {
+        require_device_ptrs("spmv_res_sym", {A.halo_ptr, A.halo, S.slice_ptr, S.meta, S.vals, S.row_info, x, b, y,
+                                             fr.lead, fr.gid, yc0, rr_partials});
+        if (A.halo_max < NODE_BLOCK || S.nslices != A.nslices || S.incoming_max < 0)
+            throw std::logic_error("[MAS] internal error: the symmetric SpMV needs the halo lists of its matrix.");
+        const size_t lds = spmv_sym_lds<BD, TV, TX>(A.halo_max, S.incoming_max);
+        if (lds > SPMV_SYM_LDS_CAP)
+            throw std::logic_error("[MAS] internal error: the symmetric SpMV's LDS exceeds its cap.");
+        const int grid = A.nslices / (SPMV_BLOCK / WAVE);
+        const int sel = selected_only ? 1 : 0;
+        auto allow = [](const void *fn) {
+            PSMAS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, SPMV_SYM_LDS_CAP));
+        };
+#define PSMAS_SYM_RES_LAUNCH                                                                                           \
+    {                                                                                                                  \
+        if (lds > 65536)                                                                                               \
+        {                                                                                                              \
+            static const bool allowed =                                                                                \
+                (allow(reinterpret_cast<const void *>(&k_spmv_res_sym<BD, U, TV, TX, TY>)), true);                     \
+            (void)allowed;                                                                                             \
+        }                                                                                                              \
+        hipLaunchKernelGGL((k_spmv_res_sym<BD, U, TV, TX, TY>), dim3(grid), dim3(SPMV_BLOCK), lds, s, A, S, x, b, y,  \
+                           fr, yc0, rr_partials, sm, sel);                                                             \
+    }
+        PSMAS_UNROLL_DISPATCH(S.unroll, PSMAS_SYM_RES_LAUNCH);
+#undef PSMAS_SYM_RES_LAUNCH
+        PSMAS_LAUNCH_CHECK();
+    }
+
     template <typename TV>
     void launch_sym_gather(int bd, long long entries, const int *src, const double *vals64, TV *out, hipStream_t s)
     {
@@ -1698,6 +1779,14 @@ namespace psmas
     template void launch_spmv_sym<BD, float, float>(const SellView<float> &, const SymView<float> &, const float *, \
                                                     float *, const Reductions &, const SysMap &, hipStream_t);    \
     template size_t spmv_sym_lds<BD, double, double>(int, int);                                                  \
+    template void launch_spmv_res_sym<BD, double, double, double>(const SellView<double> &, const SymView<double> &, \
+                                                                  const double *, const double *, double *,       \
+                                                                  FineRestrict, double *, double *, const SysMap &, \
+                                                                  bool, hipStream_t);                             \
+    template void launch_spmv_res_sym<BD, double, double, float>(const SellView<double> &, const SymView<double> &, \
+                                                                 const double *, const double *, float *,         \
+                                                                 FineRestrict, float *, double *, const SysMap &, \
+                                                                 bool, hipStream_t);                              \
     template size_t spmv_sym_lds<BD, float, float>(int, int);                                                    \
     template void launch_spmv_ap<BD, double, double>(const SellView<double> &, const double *, double *,          \
                                                      const Reductions &, const SysMap &, hipStream_t);            \
