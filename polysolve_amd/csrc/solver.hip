@@ -1335,7 +1335,7 @@ namespace psmas
         d_sym_mirror_.reset();
         d_sym_meta_.reset();
         d_sym_info_.reset();
-        reset_sym_values();
+        reset_sym_values(true);
         d_lcols_.reset();
         d_halo_ptr_.reset();
         d_halo_.reset();
@@ -1478,10 +1478,16 @@ namespace psmas
         sym_note_.clear();
     }
 
-    void Solver::reset_sym_values()
+    void Solver::reset_sym_values(bool free_buffers)
     {
-        d_symv32_.reset();
-        d_symv64_.reset();
+        // New values: the check and the gathers run again at the next prepare_sym, into the same
+        // buffers (freeing and re-allocating 0.2-0.4 GB per refresh would cost more than both).
+        if (free_buffers)
+        {
+            d_symv32_.reset();
+            d_symv64_.reset();
+        }
+        symv_fresh_[0] = symv_fresh_[1] = false;
         sym_values_ok_ = -1;
     }
 
@@ -1504,12 +1510,13 @@ namespace psmas
         }
         if (sym_values_ok_ < 0)
         {
-            DeviceBuffer<int> flag(1, "MAS symmetry check");
-            launch_fill<int>(1, flag.data(), 0, stream_);
+            if (!d_sym_flag_)
+                d_sym_flag_.allocate(1, "MAS symmetry check");
+            launch_fill<int>(1, d_sym_flag_.data(), 0, stream_);
             launch_sym_check(bd_, entries, d_sym_src_.data(), d_sym_mirror_.data(), d_vals64_.data(), 1e-12,
-                             flag.data(), stream_);
+                             d_sym_flag_.data(), stream_);
             int h = 0;
-            PSMAS_HIP(hipMemcpyAsync(&h, flag.data(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+            PSMAS_HIP(hipMemcpyAsync(&h, d_sym_flag_.data(), sizeof(int), hipMemcpyDeviceToHost, stream_));
             PSMAS_HIP(hipStreamSynchronize(stream_));
             sym_values_ok_ = h == 0 ? 1 : 0;
             sym_note_ = h == 0 ? "" : "the matrix is not symmetric to 1e-12 of its largest entries";
@@ -1522,10 +1529,13 @@ namespace psmas
             else
                 return d_symv64_;
         }();
-        if (!buf)
+        bool &fresh = symv_fresh_[std::is_same_v<TV, float> ? 0 : 1];
+        if (!buf || !fresh)
         {
-            buf.allocate(static_cast<size_t>(entries) * bd_ * bd_, "MAS symmetric SELL values");
+            if (!buf)
+                buf.allocate(static_cast<size_t>(entries) * bd_ * bd_, "MAS symmetric SELL values");
             launch_sym_gather<TV>(bd_, entries, d_sym_src_.data(), d_vals64_.data(), buf.data(), stream_);
+            fresh = true;
         }
         return true;
     }
@@ -1540,7 +1550,7 @@ namespace psmas
                 return d_symv64_;
         }();
         if (!params_.spmv_symmetric || !sym_built_ || sym_values_ok_ != 1 || copies_ != 1 || !params_.spmv_halo ||
-            halo_max_ < NODE_BLOCK || !buf)
+            halo_max_ < NODE_BLOCK || !buf || !symv_fresh_[std::is_same_v<TV, float> ? 0 : 1])
             return false;
         out.nslices = nslices_;
         out.slice_ptr = d_sym_slice_ptr_.data();

@@ -210,6 +210,8 @@ namespace psmas
         int sym_values_ok_ = -1;
         std::string sym_note_;
         std::future<SymSell> sym_future_; ///< the host build of the structure, while it runs
+        bool symv_fresh_[2] = {false, false}; ///< d_symv32_ / d_symv64_ hold the current values
+        DeviceBuffer<int> d_sym_flag_;
 
         // systems
         DeviceBuffer<int> d_block_sys_, d_blk_begin_, d_blk_count_, d_active_, d_select_, d_bank_sys_;
@@ -354,7 +356,7 @@ namespace psmas
         bool prepare_sym();
         void build_sym(std::vector<int> slice_ptr, std::vector<int> cols, std::vector<unsigned short> lcols);
         void join_sym();
-        void reset_sym_values();
+        void reset_sym_values(bool free_buffers = false);
 
         /// Domains of system S: Morton chunks (coordinates), else the graph aggregation on the
         /// GPU when dcsc (A in device memory) is given, else on the host.
